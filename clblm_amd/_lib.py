@@ -1,0 +1,70 @@
+"""ctypes binding of libclblm_hip.so (the C-ABI declared in include/clblm_hip.h).
+
+The HIP library is the only implementation: if it is missing or no GPU is
+visible, every call raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libclblm_hip.so")
+MAXMOL = 47
+FP64, FP32C = 0, 1
+
+EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
+           "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch",
+           "clblm_rt_up", "clblm_timing"]
+
+
+class Layer(ctypes.Structure):
+    """clblm_layer"""
+    _fields_ = [("pave", ctypes.c_double), ("tave", ctypes.c_double), ("wtot", ctypes.c_double),
+                ("zbot", ctypes.c_double), ("ztop", ctypes.c_double),
+                ("w", ctypes.c_double * (MAXMOL + 1)), ("sel", ctypes.c_int32 * (MAXMOL + 1))]
+
+
+class ODParams(ctypes.Structure):
+    """clblm_od_params"""
+    _fields_ = [("v1", ctypes.c_double), ("v2", ctypes.c_double), ("dvnormal", ctypes.c_double),
+                ("dvnarrow", ctypes.c_double), ("sample", ctypes.c_double), ("dvout", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("contnm", ctypes.c_int32),
+                ("continuum", ctypes.c_void_p), ("nptabs", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+_lib = None
+
+
+class ClblmError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ClblmError("libclblm_hip.so not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.clblm_last_error.restype = ctypes.c_char_p
+    lib.clblm_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.clblm_ctx_destroy.argtypes = [ctypes.c_void_p]
+    P = ctypes.c_void_p
+    lib.clblm_linelist_upload.argtypes = [P, ctypes.c_int64] + [P] * 9 + [ctypes.POINTER(ctypes.c_int)]
+    lib.clblm_linelist_free.argtypes = [P, ctypes.c_int]
+    lib.clblm_odlay.argtypes = [P, ctypes.c_int, ctypes.POINTER(Layer), ctypes.POINTER(ODParams),
+                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_double), P]
+    lib.clblm_od_batch.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int32),
+                                   P, P, P]
+    lib.clblm_rt_up.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, P, P, P, P]
+    lib.clblm_timing.argtypes = [P, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 4 + \
+        [ctypes.POINTER(ctypes.c_int64)]
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        raise ClblmError("clblm error %d: %s" % (rc, load().clblm_last_error().decode()))

@@ -1,0 +1,27 @@
+"""Builds libclblm_hip.so in-tree with hipcc for gfx950."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc", "clblm_od.hip")
+OUT = os.path.join(HERE, "libclblm_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+         # FP64 parity: every expression rounds as written, no FMA contraction
+         "-ffp-contract=off",
+         # no packed FP32 instructions (v_pk_*_f32)
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build_hip(verbose=False):
+    cmd = [HIPCC] + FLAGS + ["-o", OUT, SRC]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    if verbose:
+        print(" ".join(cmd))
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build_hip(verbose=True))

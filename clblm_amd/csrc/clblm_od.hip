@@ -1,0 +1,1755 @@
+// MI355X layer optical depth: the reference's ODLAY nested-grid Voigt line sum
+// (HIRACQ/LNCORQ/CNVFNQ/panelOut and LINF4Q/SHRINQ/LBLF4Q/CNVF4Q), restated as a
+// set of batched HIP kernels for gfx950 behind the C-ABI of include/clblm_hip.h.
+//
+// Pipeline for a batch of layers (every stage is one launch for all layers):
+//   host     plan_layer: panel and F4-block schedules (pure functions of V1, V2, DV)
+//   k_prep   LNCORQ per line and the first panel each line fits (CNVFNQ JMAX1 test)
+//   k_home   prefix max of that panel over the line order = the panel that convolves it
+//   k_ranges line range homed in each panel
+//   k_conv   pass 1: each panel's own R1/R2/R3 contributions (FP64 scatter or FP32 gather)
+//   k_f4*    LINF4Q per line, SHRINQ chunk simulation, CNVF4Q gather on R4 blocks
+//   k_pass2  pass 2: panel assembly with the carry of the previous panel, XINT R4->R3,
+//            R3->R2->R1 Lagrange interpolation, RADFNI radiation term, store
+//   k_regrid interpSpectrum to DVout
+// Device FP64 code is compiled with -ffp-contract=off so every expression rounds as
+// the reference's does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/clblm_hip.h"
+#include "clblm_common.h"
+
+using namespace clblm;
+
+#include "clblm_data.inc"
+
+// ------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+static int set_err(int code, const std::string &m) {
+    g_err = m;
+    return code;
+}
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return set_err(CLBLM_ERR_HIP, std::string(#x ": ") + hipGetErrorString(e_));   \
+    } while (0)
+
+// ------------------------------------------------------------------------------------
+// device constants
+// ------------------------------------------------------------------------------------
+__constant__ double c_avrat[102], c_cgauss[102], c_cf1[102], c_cf2[102], c_cf3[102], c_cer[102];
+__constant__ int c_maxiso[NMOL];
+
+struct Tables {
+    const double *f1, *f2, *f3, *fg, *xver;   // 1-based, NFMX+1 slots
+    const float *f1s, *f2s, *f3s, *fgs, *xvers;
+};
+
+__device__ __forceinline__ int ftrunc(double x) { return (int)x; }
+
+// RADFN, clblm_ODLAY_CommonSub.f90:293-328
+__host__ __device__ inline double radfn(double vi, double xkt) {
+    double xvi = vi;
+    if (xkt > 0.0) {
+        double xviokt = xvi / xkt;
+        if (xviokt <= 0.01) return 0.5 * xviokt * xvi;
+        if (xviokt <= 10.0) {
+            double e = exp(-xviokt);
+            return xvi * (1. - e) / (1. + e);
+        }
+        return xvi;
+    }
+    return xvi;
+}
+
+// RADFNI, clblm_ODLAY_CommonSub.f90:379-525 (one step of the piecewise-linear
+// radiation term; *vinew stays >= 0 on this path)
+__host__ __device__ inline double radfni(double vi, double dvi, double xkt, double *vinew,
+                                         double *rdel, double *rdlast) {
+    const double FACT1 = 3.0E-03;
+    double xvi = vi;
+    if (*rdlast < 0.) *rdlast = radfn(vi, xkt);
+    double r = *rdlast;
+    int intvls = 1;
+    double rdnext;
+    if (xkt > 0.0) {
+        double xviokt = xvi / xkt;
+        if (xviokt <= 0.01) {
+            if (*vinew >= 0.0) {
+                *vinew = vi + FACT1 * 0.5 * xvi;
+                intvls = (int)((*vinew - vi) / dvi);
+                if (intvls < 1) intvls = 1;
+                *vinew = vi + dvi * (double)intvls;
+            } else {
+                *vinew = fabs(*vinew);
+                intvls = (int)((*vinew - vi) / dvi);
+                if (intvls < 1) intvls = 1;
+            }
+            double xn = *vinew;
+            double xnk = xn / xkt;
+            rdnext = 0.5 * xnk * xn;
+        } else if (xviokt <= 10.0) {
+            double expvkt = exp(-xviokt);
+            double xminus = 1. - expvkt;
+            double xplus = 1. + expvkt;
+            if (*vinew >= 0.0) {
+                double cvikt = xviokt * expvkt;
+                *vinew = vi + FACT1 * xvi / (1. + (cvikt / xminus + cvikt / xplus));
+                intvls = (int)((*vinew - vi) / dvi);
+                if (intvls < 1) intvls = 1;
+                *vinew = vi + dvi * (double)intvls;
+            } else {
+                *vinew = fabs(*vinew);
+                intvls = (int)((*vinew - vi) / dvi);
+                if (intvls < 1) intvls = 1;
+            }
+            double xn = *vinew;
+            double xnk = xn / xkt;
+            double e = exp(-xnk);
+            rdnext = xn * (1. - e) / (1. + e);
+        } else {
+            if (*vinew >= 0.0) {
+                *vinew = vi + (FACT1 * xvi);
+                intvls = (int)((*vinew - vi) / dvi);
+                if (intvls < 1) intvls = 1;
+                *vinew = vi + dvi * (double)intvls;
+            } else {
+                *vinew = fabs(*vinew);
+                intvls = (int)((*vinew - vi) / dvi);
+                if (intvls < 1) intvls = 1;
+            }
+            rdnext = *vinew;
+        }
+    } else {
+        if (*vinew >= 0.0) {
+            *vinew = vi + (FACT1 * xvi);
+            intvls = (int)((*vinew - vi) / dvi);
+            if (intvls < 1) intvls = 1;
+            *vinew = vi + dvi * (double)intvls;
+        } else {
+            *vinew = fabs(*vinew);
+            intvls = (int)((*vinew - vi) / dvi);
+            if (intvls < 1) intvls = 1;
+        }
+        rdnext = *vinew;
+    }
+    *rdel = (rdnext - r) / (double)intvls;
+    *rdlast = rdnext;
+    return r;
+}
+
+__device__ __forceinline__ double avrat_lerp(const double *tab, int iz, double zetdif) {
+    return tab[iz - 1] + zetdif * (tab[iz] - tab[iz - 1]);
+}
+
+// First panel (>= 0) in which a line at vnu with 1/alpha_V = recalf fits below MAX1:
+// CNVFNQ clblm_ODLAY.f90:2150-2159.
+__device__ inline int first_fit_panel(const Panel *P, int np, double vnu, double recalf,
+                                      double dv, double wavdxf, double hwdxf, int max1) {
+    double zslope = recalf * wavdxf;
+    double bhwdxf = hwdxf / zslope;
+    double vft0 = P[0].vft;
+    double est = ((vnu - vft0) / dv + bhwdxf + 1.5 - (double)max1) / 2400.0;
+    int p = est > 0 ? (int)est - 1 : 0;
+    if (p < 0) p = 0;
+    if (p > np) p = np;
+    auto fits = [&](int q) { return ftrunc((vnu - P[q].vft) / dv + bhwdxf + 1.5) <= max1; };
+    while (p > 0 && p - 1 < np && fits(p - 1)) p--;
+    while (p < np && !fits(p)) p++;
+    return p;
+}
+
+// ------------------------------------------------------------------------------------
+// k_prep: LNCORQ (clblm_ODLAY.f90:1566-2041) for each line of the RDLIN range.
+// ------------------------------------------------------------------------------------
+struct LinesDev {
+    int64_t n;
+    const double *nu;
+    const float *s, *a_air, *a_self, *e_lo, *tmpalf, *shift;
+    const int32_t *mol, *iflg;
+};
+
+struct LinePrep {
+    double *vnu, *sp, *recalf, *zeta;
+    int *iz, *pmin, *home;
+};
+
+__global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePrep lp, int *err) {
+    const Job &J = jobs[blockIdx.y];
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= J.line_n) return;
+    int64_t i = J.line_lo + k, o = J.lp_off + k;
+    double sp = 0., vnu = L.nu[i], recalf = 0., zeta = 0.;
+    int iz = 1;
+    int pm = 0x3fffffff;
+    int fl = L.iflg[i];
+    if (fl != 0 && fl >= 0) atomicOr(err, 1);
+    int mol = L.mol[i];
+    int m = mol % 100, iso = (mol % 1000) / 100;
+    if (iso == 0) iso = 10;
+    bool ok = fl == 0 && m >= 1 && m <= NMOL && J.sel[m] && iso <= c_maxiso[m - 1];
+    if (ok) {
+        double sui = (double)L.s[i] * J.w[m];
+        if (J.jrad == 1) sui = sui * radfn(vnu, J.xkt);
+        if (sui != 0.) {
+            vnu = vnu + J.rhorat * (double)L.shift[i];
+            double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
+            double a_air = (double)L.a_air[i];
+            if (m == 7) {
+                double rvmr = 0.21;
+                a_air = (a_air - rvmr * (double)L.a_self[i]) / (1.0 - rvmr);
+            }
+            double alfa0i = a_air * tmpcor;
+            double hwhmsi = (double)L.a_self[i] * tmpcor;
+            double alfl = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+            double alfad = vnu * J.alfd1[m][iso];
+            zeta = alfl / (alfl + alfad);
+            double fzeta = 100. * zeta;
+            iz = ftrunc(fzeta + ONEPL);
+            double zetdif = fzeta - (double)(iz - 1);
+            double alfv = avrat_lerp(c_avrat, iz, zetdif) * (alfl + alfad);
+            if (alfv < J.dv) alfv = J.dv;
+            if (alfv > J.alfmax) alfv = J.alfmax;
+            if (!(64. * alfv + vnu < J.vft0)) {
+                recalf = 1. / alfv;
+                double srad = (double)L.e_lo[i];
+                if (srad <= -1.001) srad = fabs(srad);
+                if (srad > -0.999)
+                    sui = sui * J.scor[m][iso] * exp(-(srad * J.betacr)) * (1. + exp(-(vnu / J.xkt)));
+                if (sui * recalf > 0.) {
+                    sp = sui;
+                    const double DXF1 = 0.002, HWF1 = 4.;
+                    pm = first_fit_panel(panels + J.panel_off, J.npanel, vnu, recalf, J.dv, J.dv / DXF1,
+                                         HWF1 / DXF1, J.max1);
+                }
+            }
+        }
+    }
+    lp.vnu[o] = vnu;
+    lp.sp[o] = sp;
+    lp.recalf[o] = recalf;
+    lp.zeta[o] = zeta;
+    lp.iz[o] = iz;
+    lp.pmin[o] = sp != 0. ? pm : -1;
+}
+
+// Inclusive prefix max of pmin over the line order (one workgroup per layer).
+__global__ void __launch_bounds__(1024) k_home(const Job *jobs, LinePrep lp) {
+    const Job &J = jobs[blockIdx.x];
+    __shared__ int wmax[16];
+    __shared__ int carry_s;
+    int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < J.line_n; base += 1024) {
+        int64_t k = base + tid;
+        int v = k < J.line_n ? lp.pmin[J.lp_off + k] : -1;
+        for (int d = 1; d < 64; d <<= 1) {
+            int u = __shfl_up(v, d, 64);
+            if (lane >= d) v = max(v, u);
+        }
+        if (lane == 63) wmax[wid] = v;
+        __syncthreads();
+        int pre = carry_s;
+        for (int w = 0; w < wid; w++) pre = max(pre, wmax[w]);
+        v = max(v, pre);
+        if (k < J.line_n) lp.home[J.lp_off + k] = v;
+        __syncthreads();
+        if (tid == 1023) carry_s = v;
+        __syncthreads();
+    }
+}
+
+// Line range homed in each panel (home is non-decreasing).
+__global__ void k_ranges(const Job *jobs, Panel *panels, int64_t npanel_tot, LinePrep lp) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npanel_tot) return;
+    Panel &P = panels[q];
+    const Job &J = jobs[P.layer];
+    int p = (int)(q - J.panel_off);
+    const int *h = lp.home + J.lp_off;
+    auto lb = [&](int key) {
+        int64_t lo = 0, hi = J.line_n;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (h[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    P.lstart = J.lp_off + lb(p);
+    P.lend = J.lp_off + lb(p + 1);
+}
+
+// ------------------------------------------------------------------------------------
+// Pass 1, FP64: CNVFNQ (clblm_ODLAY.f90:2067-2258) for the lines homed in one panel.
+// One lane per line walks its support in the reference's order (the table argument
+// ZF is accumulated exactly as the reference does); contributions land in LDS with
+// FP64 atomics and the panel's arrays go to HBM.  Local index 1..MAX is kept; lower
+// indices are never read by the reference's panel logic.
+// ------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *panels, LinePrep lp,
+                                                Tables T, double *A) {
+    extern __shared__ double sm[];
+    const Panel &P = panels[blockIdx.x];
+    const Job &J = jobs[P.layer];
+    const int max1 = J.max1, max2 = J.max2, max3 = J.max3, stride = J.stride;
+    double *a1 = sm - 1, *a2 = sm + max1 - 1, *a3 = sm + max1 + max2 - 1;  // 1-based
+    for (int t = threadIdx.x; t < stride; t += blockDim.x) sm[t] = 0.;
+    __syncthreads();
+    const double DXF1 = 0.002, HWF1 = 4.;
+    const double DV = J.dv;
+    const double wavdxf = DV / DXF1, hwdxf = HWF1 / DXF1;
+    const double conf2 = DV / J.dvr2, conf3 = DV / J.dvr3;
+    const double vft = P.vft;
+    for (int64_t o = P.lstart + threadIdx.x; o < P.lend; o += blockDim.x) {
+        double sp = lp.sp[o];
+        if (sp == 0.) continue;
+        double recalf = lp.recalf[o];
+        double depth = sp * recalf;
+        int izm = lp.iz[o];
+        double zetdif = 100. * lp.zeta[o] - (double)(izm - 1);
+        double s1 = depth * avrat_lerp(c_cf1, izm, zetdif);
+        double s2 = depth * avrat_lerp(c_cf2, izm, zetdif);
+        double s3 = depth * avrat_lerp(c_cf3, izm, zetdif);
+        double sd = depth * avrat_lerp(c_cgauss, izm, zetdif);
+        double sv = depth * avrat_lerp(c_cer, izm, zetdif);
+        double zslope = recalf * wavdxf;
+        double zint = (lp.vnu[o] - vft) / DV;
+        double bhwdxf = hwdxf / zslope;
+        int jmax1 = ftrunc(zint + bhwdxf + 1.5);
+        int jmin1 = ftrunc(zint - bhwdxf + 1.5);
+        double rshft = zint < 0.0 ? -0.5 : 0.5;
+        int j2shft = ftrunc(zint * (1. - conf2) + rshft);
+        int j3shft = ftrunc(zint * (1. - conf3) + rshft);
+        int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
+        double zf1 = ((double)(jmin1 - 2) - zint) * zslope;
+        double zf2 = ((double)(jmin2 - 2) - zint * conf2) * zslope;
+        double zf3 = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+        for (int j1 = jmin1; j1 <= jmax1; j1++) {
+            int j2 = j1 - j2shft, j3 = j1 - j3shft;
+            zf3 = zf3 + zslope;
+            zf2 = zf2 + zslope;
+            zf1 = zf1 + zslope;
+            int iz3 = ftrunc(fabs(zf3) + 1.5);
+            int iz2 = ftrunc(fabs(zf2) + 1.5);
+            int iz1 = ftrunc(fabs(zf1) + 1.5);
+            if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], s3 * T.f3[iz3]);
+            if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], s2 * T.f2[iz2]);
+            if (j1 >= 1) atomicAdd(&a1[j1], s1 * T.f1[iz1] + sd * T.fg[iz1] + sv * T.xver[iz1]);
+        }
+    }
+    __syncthreads();
+    double *dst = A + P.a_off;
+    for (int t = threadIdx.x; t < stride; t += blockDim.x) dst[t] = sm[t];
+}
+
+// ------------------------------------------------------------------------------------
+// F4 far wings.  LINF4Q per-line physics (clblm_ODLAY_LineF4.f90:951-1170).
+// ------------------------------------------------------------------------------------
+struct F4Prep {
+    int8_t *acc, *co2;
+    double *vnu, *sp, *alfa, *epp;
+    int *stop;          // per layer: first accepted line with vnu > V2L4 (relative), else f4_n
+    int *rec_acc;       // per layer record: accepted lines up to the stop
+    int *G;             // per layer record: emitted count of a chunk starting there, by length
+    int *nshr;          // per layer: shrunk lines
+    int *nchunk, *nblk9;
+    int4 *chunks;       // (start record, length, output offset, layer)
+    int64_t *blk9_end;  // TAPE9 block ends (relative shrunk index), per layer capacity = nrec
+};
+
+struct Shrunk {
+    double *vnu, *sp, *alfa, *epp, *spp, *pmax, *smin, *bvmax;
+    int8_t *co2;
+};
+
+__global__ void k_f4prep(const Job *jobs, LinesDev L, F4Prep f, int *err) {
+    const Job &J = jobs[blockIdx.y];
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= J.f4_n) return;
+    int64_t i = J.f4_lo + k, o = J.f4_off + k;
+    int fl = L.iflg[i];
+    if (fl > 0) atomicOr(err, 1);
+    int mol = L.mol[i];
+    int m = mol % 100, iso = (mol % 1000) / 100;
+    if (iso == 0) iso = 10;
+    int8_t acc = 0;
+    double vnu = 0., sp = 0., alfa = 0., epp = 0.;
+    if (fl == 0 && m >= 1 && m <= NMOL && J.sel[m] && iso <= c_maxiso[m - 1]) {
+        double sui = (double)L.s[i] * J.w[m];
+        if (sui != 0. && !(L.nu[i] < J.v1l4)) {
+            acc = 1;
+            vnu = L.nu[i] + J.rhorat * (double)L.shift[i];
+            alfa = (double)L.a_air[i];
+            if (m == 7) {
+                double rvmr = 0.21;
+                alfa = (alfa - rvmr * (double)L.a_self[i]) / (1.0 - rvmr);
+            }
+            epp = (double)L.e_lo[i];
+            if (vnu == 0.) sui = 2. * sui;
+            if (epp <= -1.001) epp = fabs(epp);
+            if (epp > -0.999)
+                sui = sui * J.scor[m][iso] * exp(-(epp * J.betacr4)) * (1. + exp(-(vnu * J.beta4)));
+            double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
+            double alfa0i = alfa * tmpcor;
+            double hwhmsi = (double)L.a_self[i] * tmpcor;
+            alfa = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+            epp = vnu * J.alfd1[m][iso];
+            sp = sui;
+            if (vnu > J.v2l4) atomicMin(&f.stop[blockIdx.y], (int)k);
+        }
+    }
+    f.acc[o] = acc;
+    f.co2[o] = (m == 2);
+    f.vnu[o] = vnu;
+    f.sp[o] = sp;
+    f.alfa[o] = alfa;
+    f.epp[o] = epp;
+}
+
+// Accepted lines per TAPE3 record, truncated after the stop line.
+__global__ void k_reccount(const Job *jobs, F4Prep f) {
+    const Job &J = jobs[blockIdx.y];
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= J.nrec) return;
+    int64_t a = (J.rec0 + r) * REC - J.f4_lo, b = a + REC;
+    int stop = f.stop[blockIdx.y];
+    if (a < 0) a = 0;
+    if (b > J.f4_n) b = J.f4_n;
+    if (b > (int64_t)stop + 1) b = (int64_t)stop + 1;
+    int c = 0;
+    for (int64_t k = a; k < b; k++) c += f.acc[J.f4_off + k];
+    f.rec_acc[J.rec_off + r] = c;
+}
+
+// SHRINQ (clblm_ODLAY_LineF4.f90:1516-1647) over the accepted lines with relative
+// index in [k0, kend).  A group closes after line i when the next accepted line lies
+// beyond VLMT (or there is none); VLMT restarts from the closing line.  Strengths
+// below 1e-35 are zeroed but still counted in the group.  Non-CO2 and CO2 lines of a
+// group give separate effective lines.  In counting mode (EMIT false) the number of
+// lines a call of SHRINQ would emit if the chunk ended at each record boundary is
+// written to gout[0..lcap) (record ends relative to the record of k0).
+struct ShrinkSums {
+    double v, s, al, ad, c;
+};
+
+template <bool EMIT>
+__device__ int shrinq_walk(const Job &J, const F4Prep &f, int64_t k0, int64_t kend, int64_t rec_end0,
+                           int lcap, int *gout, Shrunk sh, int64_t out0) {
+    const double dvh = J.dvr4 / 2.;
+    ShrinkSums a{0, 0, 0, 0, 0}, b{0, 0, 0, 0, 0};
+    int emitted = 0, lrec = 0;
+    double vlmt = 0.;
+    int64_t rec_end = rec_end0;  // relative index one past the current record
+    int64_t cur = k0;
+    while (cur < kend && !f.acc[J.f4_off + cur]) cur++;
+    if (!EMIT) {
+        while (lrec < lcap && rec_end <= cur) { gout[lrec++] = 0; rec_end += REC; }
+    }
+    bool first = true;
+    while (cur < kend) {
+        int64_t o = J.f4_off + cur;
+        double v = f.vnu[o], s = f.sp[o], al = f.alfa[o], ad = f.epp[o], c = 0.;
+        if (first) { vlmt = v + dvh; first = false; }
+        if (s < 1.0e-35) { s = 0.0; c = 0.0; }
+        if (c != 0.0) {
+            if (EMIT) {
+                int64_t q = J.f4_off + out0 + emitted;
+                sh.vnu[q] = v; sh.sp[q] = s; sh.alfa[q] = al; sh.epp[q] = ad; sh.spp[q] = c; sh.co2[q] = f.co2[o];
+            }
+            emitted++;
+        } else if (!f.co2[o]) {
+            a.v = a.v + v * s; a.s = a.s + s; a.al = a.al + s * al; a.ad = a.ad + s * ad; a.c = a.c + c;
+        } else {
+            b.v = b.v + v * s; b.s = b.s + s; b.al = b.al + s * al; b.ad = b.ad + s * ad; b.c = b.c + c;
+        }
+        int64_t nxt = cur + 1;
+        while (nxt < kend && !f.acc[J.f4_off + nxt]) nxt++;
+        bool cont = nxt < kend && f.vnu[J.f4_off + nxt] <= vlmt;
+        if (!cont) {
+            vlmt = v + dvh;
+            if (a.s > 0.) {
+                if (EMIT) {
+                    int64_t q = J.f4_off + out0 + emitted;
+                    sh.vnu[q] = a.v / a.s; sh.sp[q] = a.s; sh.alfa[q] = a.al / a.s; sh.epp[q] = a.ad / a.s;
+                    sh.spp[q] = a.c; sh.co2[q] = 0;
+                }
+                emitted++;
+            }
+            if (b.s > 0.) {
+                if (EMIT) {
+                    int64_t q = J.f4_off + out0 + emitted;
+                    sh.vnu[q] = b.v / b.s; sh.sp[q] = b.s; sh.alfa[q] = b.al / b.s; sh.epp[q] = b.ad / b.s;
+                    sh.spp[q] = b.c; sh.co2[q] = 1;
+                }
+                emitted++;
+            }
+            a = ShrinkSums{0, 0, 0, 0, 0};
+            b = ShrinkSums{0, 0, 0, 0, 0};
+        }
+        if (!EMIT) {
+            int pend = emitted + (a.s > 0. ? 1 : 0) + (b.s > 0. ? 1 : 0);
+            // record ends between this line and the next accepted one
+            int64_t lim = nxt < kend ? nxt : kend;
+            while (lrec < lcap && rec_end <= lim) { gout[lrec++] = pend; rec_end += REC; }
+            if (nxt >= kend) {
+                while (lrec < lcap) { gout[lrec++] = pend; rec_end += REC; }
+            }
+        }
+        cur = nxt;
+    }
+    if (!EMIT) {
+        while (lrec < lcap) gout[lrec++] = emitted;
+    }
+    return emitted;
+}
+
+constexpr int LCAP = 64;  // longest shrink chunk, in records
+
+// G(r, L): lines a SHRINQ call emits for a chunk of L records starting at record r.
+__global__ void k_chunkG(const Job *jobs, F4Prep f, Shrunk sh) {
+    const Job &J = jobs[blockIdx.y];
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= J.nrec) return;
+    int64_t stop = f.stop[blockIdx.y];
+    int64_t k0 = (J.rec0 + r) * REC - J.f4_lo;
+    if (k0 < 0) k0 = 0;
+    int64_t kend = (J.rec0 + r + LCAP) * REC - J.f4_lo;
+    if (kend > J.f4_n) kend = J.f4_n;
+    if (kend > stop + 1) kend = stop + 1;
+    int64_t rec_end0 = (J.rec0 + r + 1) * REC - J.f4_lo;
+    shrinq_walk<false>(J, f, k0, kend, rec_end0, LCAP, f.G + (J.rec_off + r) * LCAP, sh, 0);
+}
+
+// LINF4Q buffer logic (clblm_ODLAY_LineF4.f90:1205-1243): records are appended until
+// the buffer holds LIMIN lines, the new lines are shrunk, and the buffer is written
+// as one TAPE9 block once it holds LIMIN shrunk lines.  One thread per layer walks
+// the records with the precomputed chunk counts.
+__global__ void k_sim(const Job *jobs, int nlay, F4Prep f, int *err) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlay) return;
+    const Job &J = jobs[l];
+    int64_t stop = f.stop[l];
+    int64_t stop_rec = stop < J.f4_n ? (J.f4_lo + stop) / REC - J.rec0 : J.nrec - 1;
+    int S = 0, raw = 0, nch = 0, nb = 0;
+    int64_t start = 0, r = 0, out = 0;
+    int4 *ch = f.chunks + J.rec_off;
+    int64_t *be = f.blk9_end + J.rec_off;
+    for (;;) {
+        if (r < J.nrec) raw += f.rec_acc[J.rec_off + r];
+        r++;
+        bool eof = r > stop_rec;
+        if (!eof && S + raw < LIMIN4) continue;
+        int64_t L = r - start;
+        if (raw > 0) {
+            if (L > LCAP) { atomicOr(err, 4); return; }
+            int g = f.G[(J.rec_off + start) * LCAP + L - 1];
+            ch[nch++] = make_int4((int)start, (int)L, (int)(out + S), l);
+            S += g;
+        }
+        raw = 0;
+        start = r;
+        if (S < LIMIN4 && !eof) continue;
+        if (S > 0) { out += S; be[nb++] = out; }
+        if (eof) break;
+        S = 0;
+    }
+    f.nshr[l] = (int)out;
+    f.nchunk[l] = nch;
+    f.nblk9[l] = nb;
+}
+
+__global__ void k_emit(const Job *jobs, F4Prep f, Shrunk sh, int64_t rec_tot) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= rec_tot) return;
+    int4 c = f.chunks[q];
+    // chunk slots are per layer at rec_off; only the first nchunk of each layer are valid
+    if (c.w < 0) return;
+    const Job &J = jobs[c.w];
+    int64_t slot = q - J.rec_off;
+    if (slot >= f.nchunk[c.w]) return;
+    int64_t stop = f.stop[c.w];
+    int64_t k0 = (J.rec0 + c.x) * REC - J.f4_lo;
+    if (k0 < 0) k0 = 0;
+    int64_t kend = (J.rec0 + c.x + c.y) * REC - J.f4_lo;
+    if (kend > J.f4_n) kend = J.f4_n;
+    if (kend > stop + 1) kend = stop + 1;
+    shrinq_walk<true>(J, f, k0, kend, 0, 0, nullptr, sh, c.z);
+}
+
+// Prefix max / suffix min of the shrunk centres and prefix max of the TAPE9 block
+// maxima, for the range searches of the R4 gather (one workgroup per layer).
+__global__ void __launch_bounds__(1024) k_shraux(const Job *jobs, F4Prep f, Shrunk sh) {
+    const Job &J = jobs[blockIdx.x];
+    const int64_t n = f.nshr[blockIdx.x];
+    __shared__ double wv[16];
+    __shared__ double carry;
+    int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const double *v = sh.vnu + J.f4_off;
+    for (int pass = 0; pass < 2; pass++) {
+        if (tid == 0) carry = pass == 0 ? -1e300 : 1e300;
+        __syncthreads();
+        for (int64_t base = 0; base < n; base += 1024) {
+            int64_t k = base + tid;
+            int64_t kk = pass == 0 ? k : n - 1 - k;
+            double x = k < n ? v[kk] : (pass == 0 ? -1e300 : 1e300);
+            for (int d = 1; d < 64; d <<= 1) {
+                double u = __shfl_up(x, d, 64);
+                if (lane >= d) x = pass == 0 ? fmax(x, u) : fmin(x, u);
+            }
+            if (lane == 63) wv[wid] = x;
+            __syncthreads();
+            double pre = carry;
+            for (int w = 0; w < wid; w++) pre = pass == 0 ? fmax(pre, wv[w]) : fmin(pre, wv[w]);
+            x = pass == 0 ? fmax(x, pre) : fmin(x, pre);
+            if (k < n) (pass == 0 ? sh.pmax : sh.smin)[J.f4_off + kk] = x;
+            __syncthreads();
+            if (tid == 1023) carry = x;
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        double m = -1e300;
+        for (int b = 0; b < f.nblk9[blockIdx.x]; b++) {
+            int64_t e = f.blk9_end[J.rec_off + b];
+            m = fmax(m, v[e - 1]);
+            sh.bvmax[J.f4_off + b] = m;
+        }
+    }
+}
+
+// Line range RDLN4Q/CNVF4Q read for each R4 block (clblm_ODLAY_LineF4.f90:554-638,
+// 187-240): from the first TAPE9 block whose last centre reaches V1R4-BOUND4, up to
+// the first line at or beyond V2R4+BOUND4.
+__global__ void k_r4range(const Job *jobs, R4Blk *blks, int64_t nblk_tot, F4Prep f, Shrunk sh) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nblk_tot) return;
+    R4Blk &B = blks[q];
+    const Job &J = jobs[B.layer];
+    int64_t n = f.nshr[B.layer];
+    int nb = f.nblk9[B.layer];
+    const double *bv = sh.bvmax + J.f4_off;
+    double vlo = B.v1r4 - J.bound4;
+    int lo_b = 0, hi_b = nb;
+    while (lo_b < hi_b) {
+        int mid = (lo_b + hi_b) >> 1;
+        if (bv[mid] < vlo) lo_b = mid + 1; else hi_b = mid;
+    }
+    int64_t lo = lo_b == 0 ? 0 : (lo_b >= nb ? n : f.blk9_end[J.rec_off + lo_b - 1]);
+    double vnulst = B.v2r4 + J.bound4;
+    const double *pm = sh.pmax + J.f4_off;
+    int64_t a = 0, b = n;
+    while (a < b) {
+        int64_t mid = (a + b) >> 1;
+        if (pm[mid] < vnulst) a = mid + 1; else b = mid;
+    }
+    B.lo = J.f4_off + lo;
+    B.hi = J.f4_off + (a > lo ? a : lo);
+}
+
+// ------------------------------------------------------------------------------------
+// CNVF4Q (clblm_ODLAY_LineF4.f90:187-494) as a gather: one lane per R4 point, the
+// shrunk lines that can reach a 256-point tile staged in LDS in batches of 256.
+// Lines are accumulated in the reference's line order.
+// ------------------------------------------------------------------------------------
+struct F4Line {
+    double xnui, xnum, sil, siv_a3, siv_b3, f4bnd, alfli2, rec_alfvi2, dptrat;
+    int jmin, jmax, jminm, jmaxm, flags;  // flags: 1 co2, 2 spp==0
+};
+
+template <typename T>
+__device__ __forceinline__ T f4_value(const F4Line &L, T xm) {
+    const T ZSQBND = (T)4096.;
+    const T r_bndsq = (T)1. / (T)625.;
+    T xmsq = xm * xm;
+    T zvsq = xmsq * (T)L.rec_alfvi2;
+    T f4fn;
+    if (L.flags & 1) {
+        T fcnt = ((T)2. - (xmsq * r_bndsq)) * (T)L.f4bnd;
+        if (zvsq <= ZSQBND) f4fn = ((T)L.siv_a3 + zvsq * (T)L.siv_b3) - fcnt;
+        else f4fn = (T)L.sil / ((T)L.alfli2 + xmsq) - fcnt;
+    } else {
+        if (zvsq <= ZSQBND) f4fn = ((T)L.siv_a3 + zvsq * (T)L.siv_b3) - (T)L.f4bnd;
+        else f4fn = (T)L.sil / ((T)L.alfli2 + xmsq) - (T)L.f4bnd;
+    }
+    return f4fn;
+}
+
+__device__ inline void f4_line_setup(F4Line &o, const Job &J, const R4Blk &B, double vnu, double sp,
+                                     double spp, double alfa, double epp, int co2) {
+    const double RECPI = 1. / (2. * asin(1.));
+    const double ZSQBND = 64. * 64.;
+    const double A3 = (1. + 2. * ZSQBND) / ((1. + ZSQBND) * (1. + ZSQBND));
+    const double B3 = -1. / ((1. + ZSQBND) * (1. + ZSQBND));
+    const double BNDSQ = J.bound4 * J.bound4;
+    o.jmin = 1; o.jmax = 0; o.jminm = 1; o.jmaxm = 0; o.flags = 0;
+    if (sp == 0. && spp == 0.) return;
+    double alfadi = epp, alfali = alfa;
+    double zetai = alfali / (alfali + alfadi);
+    int iz = (int)(100. * zetai + ONEPL);
+    double zetdif = 100. * zetai - (double)(iz - 1);
+    double alfavi = avrat_lerp(c_avrat, iz, zetdif) * (alfali + alfadi);
+    double ralfvi = 1. / alfavi;
+    double sil = sp * RECPI * alfali;
+    double siv = (alfali * ralfvi) * sp * RECPI * ralfvi;
+    double speak = A3 * fabs(siv);
+    if (speak <= 0. && spp == 0.) return;
+    double xnui = vnu - B.v1r4;
+    int jmin = (int)((xnui - J.bound4) / J.dvr4 + 2.);
+    if (jmin > B.nptr4) return;  // GO TO 60: no mirror either
+    if (jmin < 1) jmin = 1;
+    int jmax = (int)((xnui + J.bound4) / J.dvr4 + 1.);
+    if (jmax > B.nptr4) jmax = B.nptr4;
+    o.xnui = xnui; o.jmin = jmin; o.jmax = jmax;
+    if (vnu > 0. && vnu <= 25.) {
+        double xm = -vnu - B.v1r4;
+        int a = (int)((xm - J.bound4) / J.dvr4 + 2.);
+        if (a <= B.nptr4) {
+            if (a < 1) a = 1;
+            int b = (int)((xm + J.bound4) / J.dvr4 + 1.);
+            if (b > B.nptr4) b = B.nptr4;
+            o.xnum = xm; o.jminm = a; o.jmaxm = b;
+        }
+    }
+    o.alfli2 = alfali * alfali;
+    o.sil = sil;
+    o.f4bnd = sil / (o.alfli2 + BNDSQ);
+    o.dptrat = spp / (sp * alfavi);
+    o.rec_alfvi2 = 1. / (alfavi * alfavi);
+    o.siv_a3 = siv * A3;
+    o.siv_b3 = siv * B3;
+    o.flags = (co2 ? 1 : 0) | (spp == 0. ? 2 : 0);
+}
+
+__device__ __forceinline__ int64_t lower_bound_d(const double *a, int64_t lo, int64_t hi, double key) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ int64_t upper_bound_d(const double *a, int64_t lo, int64_t hi, double key) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// candidate shrunk-line range for R4 points [j0, j1] of block B (absolute indices)
+__device__ inline void f4_candidates(const Job &J, const R4Blk &B, const Shrunk &sh, int j0, int j1,
+                                     int64_t *c0, int64_t *c1) {
+    double vlo = B.v1r4 + (double)(j0 - 1) * J.dvr4 - J.bound4 - 2. * J.dvr4;
+    double vhi = B.v1r4 + (double)(j1 - 1) * J.dvr4 + J.bound4 + 2. * J.dvr4;
+    int64_t lo = B.lo, hi = B.hi;
+    int64_t a = lower_bound_d(sh.pmax, lo, hi, vlo);
+    int64_t b = upper_bound_d(sh.smin, lo, hi, vhi);
+    if (vlo < J.bound4 + 2. * J.dvr4) {  // Van Vleck-Weisskopf mirror images can reach
+        int64_t m0 = upper_bound_d(sh.pmax, lo, hi, 0.);
+        if (m0 < a) a = m0;
+        int64_t m1 = upper_bound_d(sh.smin, lo, hi, 25.);
+        if (m1 > b) b = m1;
+    }
+    *c0 = a;
+    *c1 = b > a ? b : a;
+}
+
+__global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *blks, const int2 *tiles,
+                                                  Shrunk sh, double *R4) {
+    __shared__ F4Line L[256];
+    int2 t = tiles[blockIdx.x];
+    const R4Blk &B = blks[t.x];
+    const Job &J = jobs[B.layer];
+    int j0 = t.y * 256 + 1, j = j0 + threadIdx.x;
+    int jl = min(j0 + 255, B.nptr4);
+    double acc = 0.;
+    if (j0 <= B.nptr4) {
+        int64_t c0, c1;
+        f4_candidates(J, B, sh, j0, jl, &c0, &c1);
+        const double xj = (double)(j - 1) * J.dvr4;
+        for (int64_t base = c0; base < c1; base += 256) {
+            int64_t i = base + threadIdx.x;
+            if (i < c1) f4_line_setup(L[threadIdx.x], J, B, sh.vnu[i], sh.sp[i], sh.spp[i], sh.alfa[i], sh.epp[i], sh.co2[i]);
+            __syncthreads();
+            int nb = (int)min((int64_t)256, c1 - base);
+            for (int k = 0; k < nb; k++) {
+                const F4Line &l = L[k];
+                if (j >= l.jmin && j <= l.jmax) {
+                    double xm = xj - l.xnui;
+                    double f = f4_value<double>(l, xm);
+                    if (!(l.flags & 2)) f = f + xm * l.dptrat * f;
+                    acc = acc + f;
+                }
+                if (j >= l.jminm && j <= l.jmaxm) {
+                    double xm = xj - l.xnum;
+                    double f = f4_value<double>(l, xm);
+                    if (!(l.flags & 2)) f = f + xm * (-l.dptrat) * f;
+                    acc = acc + f;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? acc : 0.;
+}
+
+// LBLF4Q radiation term on R4 when JRAD = 1 (clblm_ODLAY_LineF4.f90:142-177).
+__global__ void k_r4rad(const Job *jobs, const R4Blk *blks, int64_t nblk, double *R4) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nblk) return;
+    const R4Blk &B = blks[q];
+    const Job &J = jobs[B.layer];
+    if (J.jrad != 1) return;
+    double xkt = 1. / (RADCN2 / J.tave);
+    double vitst = B.v1r4 - J.dvr4, rdlast = -1., rdel;
+    int n2 = 0;
+    double *r = R4 + B.r4_off - 1;
+    do {
+        int n1 = n2 + 1;
+        double vi = B.v1r4 + J.dvr4 * (double)(n1 - 1);
+        double radvi = radfni(vi, J.dvr4, xkt, &vitst, &rdel, &rdlast);
+        n2 = (int)((vitst - B.v1r4) / J.dvr4 + 0.001);
+        if (n2 > B.nptr4) n2 = B.nptr4;
+        double tt = radvi;
+        for (int i = n1; i <= n2; i++) { r[i] = r[i] * tt; tt = tt + rdel; }
+    } while (n2 < B.nptr4);
+}
+
+// ------------------------------------------------------------------------------------
+// Pass 2: one workgroup per panel rebuilds the reference's R1/R2/R3 panel buffers at
+// the moment panelOut runs (clblm_ODLAY.f90:2289-2545): own contributions, the carry
+// of the previous panel (shifted by 2400/600/150 points), XINT of the active R4 block
+// and of the continuum into R3 (clblm_Spectrum.f90:874-914), then the R3->R2->R1
+// Lagrange steps, the RADFNI radiation term and the store.  R3 points 1..5 and R2
+// points 8..12 hold values the previous panel produced; they are rebuilt from that
+// panel's lines and F4 block only.
+// ------------------------------------------------------------------------------------
+__device__ inline double xint_point(double v1a, double v2a, double dva, const double *A /*1-based*/,
+                                    double vft, double dvr3, int I, int n1r3, int n2r3) {
+    double recdva = 1. / dva;
+    int ilo = (int)((v1a + dva - vft) / dvr3 + 1. + ONEMI);
+    if (ilo < n1r3) ilo = n1r3;
+    int ihi = (int)((v2a - dva - vft) / dvr3 + ONEMI);
+    if (ihi > n2r3) ihi = n2r3;
+    if (I < ilo || I > ihi) return 0.;
+    double vi = vft + dvr3 * (double)(I - 1);
+    int j = (int)((vi - v1a) * recdva + ONEPL);
+    double vj = v1a + dva * (double)(j - 1);
+    double p = recdva * (vi - vj);
+    double c = (3. - 2. * p) * p * p;
+    double b = 0.5 * p * (1. - p);
+    double b1 = b * (1. - p);
+    double b2 = b * p;
+    double conti = -A[j - 1] * b1 + A[j] * (1. - c + b2) + A[j + 1] * (c + b1) - A[j + 2] * b2;
+    return conti * 1.0;
+}
+
+template <typename TA>
+__global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *panels, const R4Blk *blks,
+                                               const TA *A, const double *R4, const double *absrb,
+                                               double *out) {
+    __shared__ double r3[168], r3e[8], r2[640], r1[2448], fac[2448];
+    const int64_t q = blockIdx.x;
+    const Panel &P = panels[q];
+    const Job &J = jobs[P.layer];
+    const int p = (int)(q - J.panel_off);
+    const bool prev = p > 0;
+    const Panel &PP = panels[prev ? q - 1 : q];
+    const int max1 = J.max1, max2 = J.max2, max3 = J.max3;
+    const TA *a1 = A + P.a_off - 1, *a2 = a1 + max1, *a3 = a2 + max2;
+    const TA *b1 = A + PP.a_off - 1, *b2 = b1 + max1, *b3 = b2 + max2;
+    const R4Blk &B = blks[P.blk];
+    const R4Blk &BP = blks[PP.blk];
+    const int N1R3 = prev ? 6 : 1, N1R3P = (prev && p > 1) ? 6 : 1, N2R3 = 155;
+    const double DVR3 = J.dvr3;
+    const double *r4 = R4 + B.r4_off - 1, *r4p = R4 + BP.r4_off - 1;
+    const double *ab = J.contnm ? absrb + J.absrb_off - 1 : nullptr;
+    const double v2abs = J.v1abs + (double)(J.nptabs - 1);
+    const int tid = threadIdx.x;
+    const int NLO = NSHIFT + 1, NHI = P.nhi;
+    // R3
+    for (int j = 1 + tid; j <= 160; j += blockDim.x) {
+        double v = (j <= max3 ? (double)a3[j] : 0.) + (prev && j + 150 <= max3 ? (double)b3[j + 150] : 0.);
+        if (j >= N1R3 && j <= N2R3) {
+            v = v + xint_point(B.v1r4, B.v2r4, J.dvr4, r4, P.vft, DVR3, j, N1R3, N2R3);
+            if (ab) v = v + xint_point(J.v1abs, v2abs, 1., ab, P.vft, DVR3, j, N1R3, N2R3);
+        }
+        if (prev && j <= 5) {
+            double xp = xint_point(BP.v1r4, BP.v2r4, J.dvr4, r4p, PP.vft, DVR3, j + 150, N1R3P, N2R3);
+            if (ab) xp = xp + xint_point(J.v1abs, v2abs, 1., ab, PP.vft, DVR3, j + 150, N1R3P, N2R3);
+            r3e[j] = (j + 150 <= max3 ? (double)b3[j + 150] : 0.) + xp;
+            v = v + xp;
+        }
+        r3[j] = v;
+    }
+    for (int j = 1 + tid; j < 640; j += blockDim.x)
+        r2[j] = (j <= max2 ? (double)a2[j] : 0.) + (prev && j + 600 <= max2 ? (double)b2[j + 600] : 0.);
+    __syncthreads();
+    const double X00 = -7. / 128., X01 = 105. / 128., X02 = 35. / 128., X03 = -5. / 128.;
+    const double X10 = -1. / 16., X11 = 9. / 16.;
+    // R3 -> R2
+    const int LIMLO = prev ? 13 : 5, LIMHI = NHI / 4 + 1;
+    const int ngr = LIMLO <= LIMHI ? (LIMHI - LIMLO) / 4 + 1 : 0;
+    if (prev && tid < 2) {  // groups J = 605, 609 of the previous panel (local 5..12)
+        int Jl = 5 + 4 * tid, J3 = 152 + tid;
+        const double *e = r3e - 150;  // e[151..155]
+        r2[Jl] = r2[Jl] + e[J3];
+        r2[Jl + 1] = r2[Jl + 1] + X00 * e[J3 - 1] + X01 * e[J3] + X02 * e[J3 + 1] + X03 * e[J3 + 2];
+        r2[Jl + 2] = r2[Jl + 2] + X10 * (e[J3 - 1] + e[J3 + 2]) + X11 * (e[J3] + e[J3 + 1]);
+        r2[Jl + 3] = r2[Jl + 3] + X03 * e[J3 - 1] + X02 * e[J3] + X01 * e[J3 + 1] + X00 * e[J3 + 2];
+    }
+    for (int g = tid; g < ngr; g += blockDim.x) {
+        int Jg = LIMLO + 4 * g, J3 = (Jg - 1) / 4 + 1;
+        r2[Jg] = r2[Jg] + r3[J3];
+        r2[Jg + 1] = r2[Jg + 1] + X00 * r3[J3 - 1] + X01 * r3[J3] + X02 * r3[J3 + 1] + X03 * r3[J3 + 2];
+        r2[Jg + 2] = r2[Jg + 2] + X10 * (r3[J3 - 1] + r3[J3 + 2]) + X11 * (r3[J3] + r3[J3 + 1]);
+        r2[Jg + 3] = r2[Jg + 3] + X03 * r3[J3 - 1] + X02 * r3[J3] + X01 * r3[J3 + 1] + X00 * r3[J3 + 2];
+    }
+    __syncthreads();
+    if (P.istop && ngr > 0 && tid == 0) {
+        int Jg = LIMLO + 4 * ngr, J3 = (Jg - 1) / 4 + 1;
+        r2[Jg] = r2[Jg] + r3[J3];
+        r2[Jg + 1] = r2[Jg + 1] + X00 * r3[J3 - 1] + X01 * r3[J3] + X02 * r3[J3 + 1] + X03 * r3[J3 + 2];
+    }
+    // R1
+    for (int j = NLO + tid; j <= NHI + 3; j += blockDim.x)
+        r1[j] = (j <= max1 ? (double)a1[j] : 0.) + (prev && j + 2400 <= max1 ? (double)b1[j + 2400] : 0.);
+    __syncthreads();
+    for (int g = tid; NLO + 4 * g <= NHI; g += blockDim.x) {
+        int Jg = NLO + 4 * g, J2 = (Jg - 1) / 4 + 1;
+        r1[Jg] = r1[Jg] + r2[J2];
+        r1[Jg + 1] = r1[Jg + 1] + X00 * r2[J2 - 1] + X01 * r2[J2] + X02 * r2[J2 + 1] + X03 * r2[J2 + 2];
+        r1[Jg + 2] = r1[Jg + 2] + X10 * (r2[J2 - 1] + r2[J2 + 2]) + X11 * (r2[J2] + r2[J2 + 1]);
+        r1[Jg + 3] = r1[Jg + 3] + X03 * r2[J2 - 1] + X02 * r2[J2] + X01 * r2[J2 + 1] + X00 * r2[J2 + 2];
+    }
+    __syncthreads();
+    if (J.jrad == 0) {
+        if (tid == 0) {
+            double xkt = J.tave / RADCN2;
+            double DV = J.dv;
+            double v1p = P.vft + (double)(NLO - 1) * DV;
+            double vitst = v1p - DV, rdlast = -1., rdel;
+            int n2 = NLO - 1;
+            do {
+                int n1 = n2 + 1;
+                double vi = P.vft + (double)(n1 - 1) * DV;
+                double radvi = radfni(vi, DV, xkt, &vitst, &rdel, &rdlast);
+                n2 = (int)((vitst - P.vft) / DV + 0.001);
+                if (n2 > NHI) n2 = NHI;
+                double tt = radvi;
+                for (int i = n1; i <= n2; i++) { fac[i] = tt; tt = tt + rdel; }
+            } while (n2 < NHI);
+        }
+        __syncthreads();
+        for (int i = NLO + tid; i <= NHI; i += blockDim.x) r1[i] = r1[i] * fac[i];
+        __syncthreads();
+    }
+    double *o = out + P.out_off - NLO;
+    for (int i = NLO + tid; i <= NHI; i += blockDim.x) o[i] = r1[i];
+}
+
+// interpSpectrum_array2array (clblm_Spectrum.f90:326-590), one lane per output point.
+__global__ void k_regrid(const Job *jobs, const int *lay_of_block, const int64_t *blk_first,
+                         const double *nat, double *out) {
+    int l = lay_of_block[blockIdx.x];
+    const Job &J = jobs[l];
+    int64_t jo = (int64_t)(blockIdx.x - blk_first[l]) * blockDim.x + threadIdx.x;  // 0-based output
+    if (jo >= J.npts) return;
+    const double *X = nat + J.nat_off - 1;  // X[1..nnat]
+    const int64_t NIN = J.nnat;
+    auto ext = [&](int64_t i) -> double {  // spInExt
+        if (i >= 1 && i <= NIN) return X[i];
+        if (i == 0 || i == -1) return 2.0 * X[1] - X[2];
+        if (i == NIN + 1 || i == NIN + 2) return 2.0 * X[NIN] - X[NIN - 1];
+        return 0.;
+    };
+    const double DVin = J.dv, DVout = J.dvout, V1 = J.v1;
+    const double TOL = 1.e-8;
+    double dvRatio = DVout / DVin;
+    if (fabs(DVout - DVin) < TOL) dvRatio = 1.;
+    const double XNUMCF = 201.;
+    int64_t I = jo + 1;
+    // general case (no aligned grids: the output and native grids share V1 only when the
+    // ratio is one of the n/(n+1) forms, handled below)
+    const double hi2lo[5] = {2. / 1., 3. / 2., 4. / 3., 5. / 4., 6. / 5.};
+    const double lo2hi[5] = {1. / 2., 2. / 3., 3. / 4., 4. / 5., 5. / 6.};
+    bool anyhi = false, anylo = false;
+    for (int k = 0; k < 5; k++) {
+        if (fabs(dvRatio - hi2lo[k]) < TOL) anyhi = true;
+        if (fabs(dvRatio - lo2hi[k]) < TOL) anylo = true;
+    }
+    int lenIn = 0, lenOut = 0, ITYPE = -1;
+    if (fabs(dvRatio - 1.) < 2.220446049250313e-16) { lenIn = 1; lenOut = 1; ITYPE = 0; }
+    else if (anyhi) { lenIn = (int)(DVout / (DVout - DVin) + 0.5); lenOut = lenIn - 1; ITYPE = lenOut - 1; }
+    else if (anylo) { lenIn = (int)(DVout / (DVin - DVout) + 0.5); lenOut = lenIn + 1; ITYPE = lenOut - 1; }
+    double res;
+    if (ITYPE == 0) {
+        int64_t step = (int64_t)llround(dvRatio);
+        res = ext(1 - step + I * step);
+    } else if (ITYPE > 0) {
+        int64_t nl = (I - 1) % lenOut + 1, m = (I - 1) / lenOut;  // NL-th interleaved sequence, m-th term
+        int64_t iIn = nl - lenIn;
+        if (nl > 1 && DVin > DVout) iIn = iIn - 1;
+        iIn += (m + 1) * lenIn;
+        if (nl == 1) res = ext(iIn);
+        else {
+            double AP = 1.0 / ((double)ITYPE + 1.0);
+            double pp = AP * (double)(nl - 1);
+            if (DVin > DVout) pp = 1.0 - pp;
+            double B1 = -pp * (pp - 1.0) * (pp - 2.0) / 6.0;
+            double B2 = (pp * pp - 1.0) * (pp - 2.0) * 0.5;
+            double B3 = -pp * (pp + 1.0) * (pp - 2.0) * 0.5;
+            double B4 = pp * (pp * pp - 1.0) / 6.0;
+            res = B1 * ext(iIn - 1) + B2 * ext(iIn) + B3 * ext(iIn + 1) + B4 * ext(iIn + 2);
+        }
+    } else {
+        int64_t J1 = ((I - 1) / 2400) * 2400 + 1;
+        double Vo = J1 == 1 ? V1 : V1 + (double)(J1 - 1) * DVout;
+        int64_t iIn = (int64_t)((Vo - V1) / DVin + 1.00001);
+        double Vi = V1 + (double)(iIn - 1) * DVin;
+        double P = (Vo - Vi) / DVin;
+        double Pk = P + (double)(I - J1) * dvRatio;  // P after I-J1 steps of the block walk
+        double fl = floor(Pk);
+        if (fl >= 1.0) { iIn += (int64_t)fl; Pk = Pk - fl; }
+        int ip = (int)(Pk * XNUMCF + 1.00001);
+        double p = ((double)ip - 1.0) / (XNUMCF - 1.0);
+        double pp = p * p;
+        double A1 = -p / 2.0 * ((1 - p) * (1 - p));
+        double A2 = 1.0 - pp * (3.0 - 2.0 * p) + pp / 2.0 * (1.0 - p);
+        double A3 = pp * (3.0 - 2.0 * p) + p / 2.0 * ((1.0 - p) * (1.0 - p));
+        double A4 = -pp / 2 * (1.0 - p);
+        res = A1 * ext(iIn - 1) + A2 * ext(iIn) + A3 * ext(iIn + 1) + A4 * ext(iIn + 2);
+    }
+    out[J.out_off + jo] = res;
+}
+
+// ====================================================================================
+// Host side
+// ====================================================================================
+namespace {
+
+// Line-shape tables (SHAPEL clblm_ODLAY.f90:2804-2909, SHAPEG :2914-2953, VERFN
+// :2959-3016), built on the host in the reference's operation order.
+void build_tables(std::vector<double> &F1, std::vector<double> &F2, std::vector<double> &F3,
+                  std::vector<double> &FG, std::vector<double> &XV) {
+    const double HWF1 = 4., HWF2 = 16., HWF3 = 64., DXF1 = 0.002, DXF2 = 0.008, DXF3 = 0.032;
+    F1.assign(NFMX + 1, 0.); F2.assign(NFMX + 1, 0.); F3.assign(NFMX + 1, 0.);
+    FG.assign(NFMX + 1, 0.); XV.assign(NFMX + 1, 0.);
+    auto Af = [](double z) { return (1. + 2. * z * z) / ((1. + z * z) * (1. + z * z)); };
+    auto Bf = [](double z) { return -1. / ((1. + z * z) * (1. + z * z)); };
+    const double recpi = 1. / (2. * std::asin(1.));
+    const double a1 = Af(HWF1), b1 = Bf(HWF1), a2 = Af(HWF2), b2 = Bf(HWF2), a3 = Af(HWF3), b3 = Bf(HWF3);
+    auto lor = [](double x) { return 1. / (1. + x); };
+    F1[1] = recpi * (lor(0.) - (a1 + b1 * 0.));
+    for (int j = 2; j <= NFPTS; j++) {
+        double x = (double)(j - 1) * DXF1, xs = x * x;
+        F1[j] = recpi * (lor(xs) - (a1 + b1 * xs));
+    }
+    F1[NFPTS] = 0.;
+    F2[1] = recpi * ((a1 + b1 * 0.) - (a2 + b2 * 0.));
+    int lim = (int)(HWF1 / DXF2 + 1.001);
+    for (int j = 2; j <= lim; j++) {
+        double x = (double)(j - 1) * DXF2, xs = x * x;
+        F2[j] = recpi * ((a1 + b1 * xs) - (a2 + b2 * xs));
+    }
+    for (int j = lim + 1; j <= NFPTS; j++) {
+        double x = (double)(j - 1) * DXF2, xs = x * x;
+        F2[j] = recpi * (lor(xs) - (a2 + b2 * xs));
+    }
+    F2[NFPTS] = 0.;
+    F3[1] = recpi * ((a2 + b2 * 0.) - (a3 + b3 * 0.));
+    lim = (int)(HWF2 / DXF3 + 1.001);
+    for (int j = 2; j <= lim; j++) {
+        double x = (double)(j - 1) * DXF3, xs = x * x;
+        F3[j] = recpi * ((a2 + b2 * xs) - (a3 + b3 * xs));
+    }
+    for (int j = lim + 1; j <= NFPTS; j++) {
+        double x = (double)(j - 1) * DXF3, xs = x * x;
+        F3[j] = recpi * (lor(xs) - (a3 + b3 * xs));
+    }
+    const double fln2 = std::log(2.);
+    const double fgnorm = std::sqrt(fln2 * recpi);
+    FG[1] = fgnorm * std::exp(-fln2 * 0.);
+    for (int j = 2; j <= NFPTS; j++) {
+        double x = (double)(j - 1) * DXF1, xs = x * x;
+        FG[j] = fgnorm * std::exp(-(fln2 * xs));
+    }
+    FG[NFPTS] = 0.;
+    const double cexp = 0.45, ce0 = 1., ce2 = -.20737285249, ce4 = -.00872684335747;
+    const double se0 = std::sqrt(cexp / PI_REF), ae2 = 2. * cexp, ae4 = ae2 * ae2 / 3.;
+    for (int i = 1; i <= NFMX; i++) {
+        double z = DXF1 * (double)(i - 1), z2 = z * z;
+        double xe0 = se0 * std::exp(-(cexp * z2));
+        XV[i] = (1. / (ce0 + ce2 + ce4)) * (ce0 + ce2 * ae2 * z2 + ce4 * ae4 * z2 * z2) * xe0;
+    }
+}
+
+// TIPS_2003 4-point Lagrange recall (clblm_ODLAY_TIPS.f90:5451-5547).
+double tips_interp(double t, const double *q) {
+    const double *A = CLBLM_TIPS_TDAT;
+    const int n = CLBLM_NT;
+    auto a = [&](int i) { return A[i - 1]; };
+    auto b = [&](int i) { return q[i - 1]; };
+    auto nz = [](double d) { return d == 0. ? 0.0001 : d; };
+    for (int i = 2; i <= n; i++) {
+        if (a(i) >= t) {
+            int j = i;
+            if (i < 3 || i == n) {
+                if (i < 3) j = 3;
+                double a0 = (t - a(j - 1)) * (t - a(j)) / (nz(a(j - 2) - a(j - 1)) * nz(a(j - 2) - a(j)));
+                double a1 = (t - a(j - 2)) * (t - a(j)) / (nz(a(j - 1) - a(j - 2)) * nz(a(j - 1) - a(j)));
+                double a2 = (t - a(j - 2)) * (t - a(j - 1)) / (nz(a(j) - a(j - 2)) * nz(a(j) - a(j - 1)));
+                return a0 * b(j - 2) + a1 * b(j - 1) + a2 * b(j);
+            }
+            double c0 = (t - a(j - 1)) * (t - a(j)) * (t - a(j + 1));
+            c0 = c0 / (nz(a(j - 2) - a(j - 1)) * nz(a(j - 2) - a(j)) * nz(a(j - 2) - a(j + 1)));
+            double c1 = (t - a(j - 2)) * (t - a(j)) * (t - a(j + 1));
+            c1 = c1 / (nz(a(j - 1) - a(j - 2)) * nz(a(j - 1) - a(j)) * nz(a(j - 1) - a(j + 1)));
+            double c2 = (t - a(j - 2)) * (t - a(j - 1)) * (t - a(j + 1));
+            c2 = c2 / (nz(a(j) - a(j - 2)) * nz(a(j) - a(j - 1)) * nz(a(j) - a(j + 1)));
+            double c3 = (t - a(j - 2)) * (t - a(j - 1)) * (t - a(j));
+            c3 = c3 / (nz(a(j + 1) - a(j - 2)) * nz(a(j + 1) - a(j - 1)) * nz(a(j + 1) - a(j)));
+            return c0 * b(j - 2) + c1 * b(j - 1) + c2 * b(j) + c3 * b(j + 1);
+        }
+    }
+    return 0.;
+}
+
+struct DevArena {
+    char *p = nullptr;
+    size_t cap = 0, used = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = bytes + bytes / 8;
+        if (hipMalloc(&p, want) != hipSuccess) { p = nullptr; return -1; }
+        cap = want;
+        return 0;
+    }
+};
+
+struct Carve {
+    size_t off = 0;
+    template <typename T>
+    size_t take(size_t n) {
+        size_t o = (off + 255) & ~(size_t)255;
+        off = o + n * sizeof(T);
+        return o;
+    }
+};
+
+}  // namespace
+
+struct LineListH {
+    int64_t n = 0;
+    std::vector<double> nu;
+    void *mem = nullptr;
+    LinesDev dev{};
+};
+
+struct clblm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *tab = nullptr;   // F1,F2,F3,FG,XVER consecutive, NFMX+1 each
+    float *tabs = nullptr;
+    std::vector<LineListH *> lists;
+    DevArena arena;
+    int *d_err = nullptr;
+    hipEvent_t ev[6]{};
+    double ms[4] = {0, 0, 0, 0};
+    int64_t launches = 0;
+    std::mutex mu;
+};
+
+static thread_local int g_tls_dummy = 0;
+
+static int ctx_tables(clblm_ctx *c) {
+    std::vector<double> f1, f2, f3, fg, xv;
+    build_tables(f1, f2, f3, fg, xv);
+    std::vector<double> all;
+    for (auto *v : {&f1, &f2, &f3, &fg, &xv}) all.insert(all.end(), v->begin(), v->end());
+    std::vector<float> alls(all.begin(), all.end());
+    HIPCHK(hipMalloc(&c->tab, all.size() * sizeof(double)));
+    HIPCHK(hipMalloc(&c->tabs, alls.size() * sizeof(float)));
+    HIPCHK(hipMemcpy(c->tab, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->tabs, alls.data(), alls.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_avrat), CLBLM_AVRAT, sizeof(CLBLM_AVRAT)));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cgauss), CLBLM_CGAUSS, sizeof(CLBLM_CGAUSS)));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cf1), CLBLM_CF1, sizeof(CLBLM_CF1)));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cf2), CLBLM_CF2, sizeof(CLBLM_CF2)));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cf3), CLBLM_CF3, sizeof(CLBLM_CF3)));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cer), CLBLM_CER, sizeof(CLBLM_CER)));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_maxiso), CLBLM_MAXISO, sizeof(CLBLM_MAXISO)));
+    return 0;
+}
+
+static Tables ctx_tabview(clblm_ctx *c) {
+    Tables t;
+    const size_t s = NFMX + 1;
+    t.f1 = c->tab; t.f2 = c->tab + s; t.f3 = c->tab + 2 * s; t.fg = c->tab + 3 * s; t.xver = c->tab + 4 * s;
+    t.f1s = c->tabs; t.f2s = c->tabs + s; t.f3s = c->tabs + 2 * s; t.fgs = c->tabs + 3 * s; t.xvers = c->tabs + 4 * s;
+    return t;
+}
+
+// ------------------------------------------------------------------------------------
+// Per-layer plan: ODLAY_nonStruct/OPDPTH/HIRACQ scalars, the panel schedule and the
+// R4 block schedule (clblm_ODLAY.f90:129-611,757-967,1002-1562).  These depend on
+// V1, V2 and DV only.
+// ------------------------------------------------------------------------------------
+struct Plan {
+    Job job;
+    std::vector<Panel> panels;
+    std::vector<R4Blk> blks;
+    std::vector<double> absrb;
+};
+
+static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od_params &par,
+                      int32_t *jrad_state, Plan &pl) {
+    Job &J = pl.job;
+    std::memset(&J, 0, sizeof J);
+    if (!(par.v2 > par.v1)) return set_err(CLBLM_ERR_ARG, "v2 must exceed v1");
+    if (!(par.dvnormal > 0.) || !(par.dvnarrow > 0.) || !(par.sample > 0.))
+        return set_err(CLBLM_ERR_ARG, "dvnormal, dvnarrow and sample must be positive");
+    if (!(lay.tave >= 70. && lay.tave <= 3000.)) return set_err(CLBLM_ERR_ARG, "TIPS: temperature out of range 70-3000 K");
+    if (!(lay.pave > 0.) || !(lay.wtot > 0.)) return set_err(CLBLM_ERR_ARG, "pave and wtot must be positive");
+    for (int m = NMOL + 1; m <= CLBLM_MAXMOL; m++)
+        if (lay.sel[m]) return set_err(CLBLM_ERR_UNSUPPORTED, "only molecules 1-7 carry TIPS/mass data in this build");
+    const double V1 = par.v1, V2 = par.v2, T = lay.tave, Pv = lay.pave;
+    double DV = par.dvnormal;
+    double ALFAV = par.sample * std::max(par.dvnormal, par.dvnarrow);
+    if (!(std::fabs(par.dvnormal - par.dvnarrow) / par.dvnormal > 0.2)) DV = par.dvnarrow;
+    double XKT = T / RADCN2;
+    if ((V1 / XKT) < 5. && *jrad_state != -1) *jrad_state = 0;
+    J.jrad = *jrad_state;
+    J.v1 = V1; J.v2 = V2; J.dv = DV; J.tave = T;
+    J.dvr2 = (0.008 / 0.002) * DV;
+    J.dvr3 = (0.032 / 0.002) * DV;
+    double ALFAV4 = 64. * par.sample * DV;
+    J.dvr4 = ALFAV4 / par.sample;
+    J.bound4 = 25.;
+    double V1R4 = V1 - 2. * J.dvr4, V2R4 = V2 + 2. * J.dvr4;
+    J.v1l4 = V1R4 - J.bound4 - J.dvr4;
+    J.v2l4 = V2R4 + J.bound4 + 2 * J.dvr4;
+    J.alfmax = 4. * ALFAV;
+    J.nalfmax = (int)(J.alfmax / DV);
+    int NBOUND = (int)((2. * 64.) * (double)J.nalfmax + 0.01);
+    int MAX1 = NSHIFT + NLIM1 + NSHIFT + NBOUND / 2;
+    int MAX2 = MAX1 / 4, MAX3 = MAX1 / 16;
+    MAX1 = MAX1 + 4 * J.nalfmax;
+    MAX2 = MAX2 + 16 * J.nalfmax / 4 + 1;
+    MAX3 = MAX3 + 64 * J.nalfmax / 16 + 1;
+    if (MAX1 > 2432 + 3600 || MAX2 < 616 || MAX3 < 160)
+        return set_err(CLBLM_ERR_UNSUPPORTED, "DVnormal/DVnarrow ratio gives an unsupported panel size");
+    J.max1 = MAX1; J.max2 = MAX2; J.max3 = MAX3; J.stride = MAX1 + MAX2 + MAX3;
+    double BOUND = (double)NBOUND * DV / 2.;
+    J.vft0 = V1 - (double)NSHIFT * DV;
+    if (!(J.vft0 > 0.)) return set_err(CLBLM_ERR_UNSUPPORTED, "panel start <= 0 cm-1 (RSYM) is not supported");
+    // LNCORQ constants (clblm_ODLAY.f90:1649-1667)
+    double XKT0 = TEMP0 / RADCN2;
+    J.xkt = XKT;
+    J.betacr = (1. / XKT) - (1. / XKT0);
+    J.tratio = T / TEMP0;
+    J.rhorat = (Pv / P0) * (TEMP0 / T);
+    J.pavp0 = Pv / P0;
+    // LINF4Q constants (clblm_ODLAY_LineF4.f90:945-951)
+    J.beta4 = RADCN2 / T;
+    J.betacr4 = J.beta4 - RADCN2 / TEMP0;
+    // MOLEC (clblm_ODLAY_CommonSub.f90:201-264)
+    double FAD = std::log(2.) * 2. * AVOGAD * BOLTZ / (CLIGHT * CLIGHT);
+    for (int m = 1; m <= NMOL; m++) {
+        J.sel[m] = lay.sel[m] ? 1 : 0;
+        J.w[m] = lay.w[m];
+        if (!J.sel[m]) continue;
+        for (int iso = 1; iso <= CLBLM_MAXISO[m - 1]; iso++) {
+            const double *row = CLBLM_TIPS_Q + (size_t)(CLBLM_TIPS_ROW[m - 1] + iso - 1) * CLBLM_NT;
+            double q296 = tips_interp(296., row), qt = tips_interp(T, row);
+            if (!(qt > 0.) || !(q296 > 0.)) return set_err(CLBLM_ERR_ARG, "TIPS: partition sum less than 0");
+            J.scor[m][iso] = q296 / qt;
+            J.rhoslf[m] = J.rhorat * lay.w[m] / lay.wtot;
+            J.alfd1[m][iso] = std::sqrt(FAD * T / CLBLM_ISO_MASS[CLBLM_ISO_OFFSET[m - 1] + iso - 1]);
+        }
+    }
+    // line ranges (RDLIN: first line >= VBOT to the first line > VTOP inclusive)
+    double VBOT = V1 - BOUND, VTOP = V2 + BOUND;
+    auto lb = [&](double x) { return (int64_t)(std::lower_bound(L.nu.begin(), L.nu.end(), x) - L.nu.begin()); };
+    auto ub = [&](double x) { return (int64_t)(std::upper_bound(L.nu.begin(), L.nu.end(), x) - L.nu.begin()); };
+    J.line_lo = lb(VBOT);
+    int64_t hi = ub(VTOP);
+    if (hi < L.n) hi++;
+    J.line_n = std::max<int64_t>(0, hi - J.line_lo);
+    J.f4_lo = lb(J.v1l4);
+    int64_t fhi = std::min<int64_t>(L.n, ub(J.v2l4) + 4 * REC);
+    J.f4_n = std::max<int64_t>(0, fhi - J.f4_lo);
+    J.rec0 = J.f4_lo / REC;
+    J.nrec = J.f4_n > 0 ? (J.f4_lo + J.f4_n - 1) / REC - J.rec0 + 1 : 0;
+    // panels and R4 blocks (HIRACQ panel loop and LBLF4Q refresh)
+    auto r4block = [&](double v1r4, double v2r4in) {
+        R4Blk b{};
+        b.v1r4 = v1r4;
+        int np = (int)((v2r4in - v1r4) / J.dvr4 + ONEPL);
+        if (np > LIMOUT4) np = LIMOUT4;
+        b.nptr4 = np;
+        b.v2r4 = v1r4 + J.dvr4 * (double)(np - 1);
+        return b;
+    };
+    const double V2R4ST = V2R4;
+    pl.blks.clear();
+    pl.panels.clear();
+    pl.blks.push_back(r4block(V1R4, V2R4));
+    double VFT = J.vft0;
+    int NHI = NLIM1 + NSHIFT - 1, NLO = NSHIFT + 1;
+    int64_t out = 0;
+    for (;;) {
+        Panel P{};
+        P.vft = VFT;
+        P.istop = 0;
+        if ((VFT + (double)(NHI - 1) * DV) > V2) {
+            NHI = (int)((V2 - VFT) / DV + 1.);
+            double V2P = VFT + (double)(NHI - 1) * DV;
+            if (V2P < V2) NHI = NHI + 1;
+            P.istop = 1;
+        }
+        P.nhi = NHI;
+        P.blk = (int)pl.blks.size() - 1;
+        P.out_off = out;
+        out += NHI - NLO + 1;
+        pl.panels.push_back(P);
+        if (P.istop) break;
+        VFT = VFT + (double)(NLIM1 - 1) * DV;
+        double VF1 = VFT - 2. * J.dvr4;
+        VF1 = V1 + (double)(int)std::floor((VF1 - V1) / J.dvr4) * J.dvr4;
+        double VF2 = VFT + 2. * J.dvr4 + (double)(155 + 4) * J.dvr3;
+        const R4Blk &cur = pl.blks.back();
+        if (VF2 > cur.v2r4 && cur.v2r4 != V2R4ST) pl.blks.push_back(r4block(VF1, V2R4ST));
+        if (pl.panels.size() > (size_t)1 << 26) return set_err(CLBLM_ERR_ARG, "spectral range too long");
+    }
+    J.npanel = (int)pl.panels.size();
+    J.nblk = (int)pl.blks.size();
+    J.npts = out;
+    J.nnat = out;
+    J.dvout = 0.;
+    if (par.dvout > 0. && !(std::fabs(par.dvout - DV) < 2.220446049250313e-16)) {
+        J.dvout = par.dvout;
+        J.npts = (int64_t)std::ceil((V2 - V1) / par.dvout + 1.);
+        if (V1 + (double)(J.npts - 1) * par.dvout > V1 + (double)(J.nnat - 1) * DV + DV)
+            return set_err(CLBLM_ERR_ARG, "DVout grid extends beyond the native grid");
+    }
+    // continuum on the OPDPTH ABSRB grid (clblm_ODLAY.f90:881-896)
+    J.contnm = par.contnm ? 1 : 0;
+    if (J.contnm) {
+        double v1abs = (double)(int)V1;
+        if (V1 < 0.) v1abs = v1abs - 1.;
+        v1abs = v1abs - 3.;
+        double v2abs = (double)(int)(V2 + 3. + 0.5);
+        int npt = (int)((v2abs - v1abs) / 1. + 1.5);
+        if (par.nptabs != npt || !par.continuum)
+            return set_err(CLBLM_ERR_ARG, "continuum needs " + std::to_string(npt) + " points from " +
+                                              std::to_string(v1abs) + " cm-1 at 1 cm-1");
+        J.v1abs = v1abs;
+        J.nptabs = npt;
+        pl.absrb.assign(par.continuum, par.continuum + npt);
+    }
+    return 0;
+}
+
+struct Batch {
+    std::vector<Job> jobs;
+    std::vector<Panel> panels;
+    std::vector<R4Blk> blks;
+    std::vector<int2> tiles;
+    std::vector<double> absrb;
+    std::vector<int> stop0;
+    std::vector<int> regrid_lay;
+    std::vector<int64_t> regrid_first;
+    int64_t max_line = 0, max_f4 = 0, max_rec = 0, tot_line = 0, tot_f4 = 0, tot_rec = 0;
+    int64_t tot_a = 0, tot_r4 = 0, tot_nat = 0, tot_out = 0;
+    int max_stride = 0;
+    bool any_jrad1 = false, any_regrid = false;
+};
+
+static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, const clblm_od_params *pars,
+                       int32_t *jrad_state, Batch &B, int precision) {
+    B = Batch{};
+    B.jobs.reserve(nlay);
+    for (int l = 0; l < nlay; l++) {
+        if (pars[l].precision != precision)
+            return set_err(CLBLM_ERR_ARG, "all layers of a batch must use the same precision");
+        Plan pl;
+        int rc = plan_layer(L, layers[l], pars[l], jrad_state, pl);
+        if (rc) return rc;
+        Job J = pl.job;
+        J.panel_off = (int64_t)B.panels.size();
+        J.blk_off = (int64_t)B.blks.size();
+        J.out_off = B.tot_out;
+        B.tot_out += J.npts;
+        J.lp_off = B.tot_line;
+        B.tot_line += J.line_n;
+        J.f4_off = B.tot_f4;
+        B.tot_f4 += J.f4_n;
+        J.rec_off = B.tot_rec;
+        B.tot_rec += J.nrec;
+        J.absrb_off = (int64_t)B.absrb.size();
+        B.absrb.insert(B.absrb.end(), pl.absrb.begin(), pl.absrb.end());
+        J.nat_off = B.tot_nat;
+        if (J.dvout > 0.) {
+            B.tot_nat += J.nnat;
+            B.any_regrid = true;
+        }
+        for (auto &P : pl.panels) {
+            P.layer = l;
+            P.blk += (int)J.blk_off;
+            P.out_off += (J.dvout > 0. ? J.nat_off : J.out_off);
+            P.a_off = B.tot_a;
+            B.tot_a += J.stride;
+            B.panels.push_back(P);
+        }
+        J.r4_off = B.tot_r4;
+        for (auto &b : pl.blks) {
+            b.layer = l;
+            b.r4_off = B.tot_r4;
+            B.tot_r4 += LIMOUT4 + 2;
+            int bi = (int)B.blks.size();
+            B.blks.push_back(b);
+            for (int t = 0; t < (LIMOUT4 + 2 + 255) / 256; t++) B.tiles.push_back(make_int2(bi, t));
+        }
+        B.max_line = std::max(B.max_line, J.line_n);
+        B.max_f4 = std::max(B.max_f4, J.f4_n);
+        B.max_rec = std::max(B.max_rec, J.nrec);
+        B.max_stride = std::max(B.max_stride, J.stride);
+        B.any_jrad1 |= J.jrad == 1;
+        B.stop0.push_back((int)J.f4_n);
+        B.jobs.push_back(J);
+    }
+    return 0;
+}
+
+template <typename T>
+static T *at(char *base, size_t off) { return reinterpret_cast<T *>(base + off); }
+
+static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, hipStream_t st, int precision);
+
+static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, hipStream_t st, int precision) {
+    const int nlay = (int)B.jobs.size();
+    if (nlay == 0) return 0;
+    // scratch layout
+    Carve cv;
+    size_t o_jobs = cv.take<Job>(nlay);
+    size_t o_pan = cv.take<Panel>(B.panels.size());
+    size_t o_blk = cv.take<R4Blk>(B.blks.size());
+    size_t o_tiles = cv.take<int2>(B.tiles.size());
+    size_t o_abs = cv.take<double>(std::max<size_t>(1, B.absrb.size()));
+    size_t o_lp_d = cv.take<double>(4 * (size_t)B.tot_line);
+    size_t o_lp_i = cv.take<int>(3 * (size_t)B.tot_line);
+    size_t o_A = cv.take<double>((size_t)B.tot_a);
+    size_t o_f4_b = cv.take<int8_t>(2 * (size_t)B.tot_f4);
+    size_t o_f4_d = cv.take<double>(4 * (size_t)B.tot_f4);
+    size_t o_sh_d = cv.take<double>(8 * (size_t)B.tot_f4);
+    size_t o_sh_b = cv.take<int8_t>((size_t)B.tot_f4);
+    size_t o_lay = cv.take<int>(4 * (size_t)nlay);
+    size_t o_racc = cv.take<int>((size_t)B.tot_rec);
+    size_t o_G = cv.take<int>((size_t)B.tot_rec * LCAP);
+    size_t o_ch = cv.take<int4>((size_t)B.tot_rec);
+    size_t o_be = cv.take<int64_t>((size_t)B.tot_rec);
+    size_t o_R4 = cv.take<double>((size_t)B.tot_r4);
+    size_t o_nat = cv.take<double>(std::max<int64_t>(1, B.tot_nat));
+    size_t o_rg = cv.take<int>(1);
+    size_t o_rgf = cv.take<int64_t>(nlay);
+    // regrid block map
+    std::vector<int> rg_lay;
+    std::vector<int64_t> rg_first(nlay, 0);
+    if (B.any_regrid) {
+        for (int l = 0; l < nlay; l++) {
+            rg_first[l] = (int64_t)rg_lay.size();
+            if (B.jobs[l].dvout > 0.)
+                for (unsigned b = 0; b < cdiv(B.jobs[l].npts, 256); b++) rg_lay.push_back(l);
+        }
+        o_rg = cv.take<int>(std::max<size_t>(1, rg_lay.size()));
+        o_rgf = cv.take<int64_t>(nlay);
+    }
+    if (C->arena.ensure(cv.off)) return set_err(CLBLM_ERR_HIP, "device scratch allocation failed");
+    char *base = C->arena.p;
+    Job *d_jobs = at<Job>(base, o_jobs);
+    Panel *d_pan = at<Panel>(base, o_pan);
+    R4Blk *d_blk = at<R4Blk>(base, o_blk);
+    int2 *d_tiles = at<int2>(base, o_tiles);
+    double *d_abs = at<double>(base, o_abs);
+    HIPCHK(hipMemcpyAsync(d_jobs, B.jobs.data(), sizeof(Job) * nlay, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_pan, B.panels.data(), sizeof(Panel) * B.panels.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_blk, B.blks.data(), sizeof(R4Blk) * B.blks.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_tiles, B.tiles.data(), sizeof(int2) * B.tiles.size(), hipMemcpyHostToDevice, st));
+    if (!B.absrb.empty())
+        HIPCHK(hipMemcpyAsync(d_abs, B.absrb.data(), sizeof(double) * B.absrb.size(), hipMemcpyHostToDevice, st));
+    LinePrep lp;
+    double *lpd = at<double>(base, o_lp_d);
+    int *lpi = at<int>(base, o_lp_i);
+    lp.vnu = lpd; lp.sp = lpd + B.tot_line; lp.recalf = lpd + 2 * B.tot_line; lp.zeta = lpd + 3 * B.tot_line;
+    lp.iz = lpi; lp.pmin = lpi + B.tot_line; lp.home = lpi + 2 * B.tot_line;
+    double *d_A = at<double>(base, o_A);
+    F4Prep f;
+    int8_t *f4b = at<int8_t>(base, o_f4_b);
+    double *f4d = at<double>(base, o_f4_d);
+    f.acc = f4b; f.co2 = f4b + B.tot_f4;
+    f.vnu = f4d; f.sp = f4d + B.tot_f4; f.alfa = f4d + 2 * B.tot_f4; f.epp = f4d + 3 * B.tot_f4;
+    int *lay = at<int>(base, o_lay);
+    f.stop = lay; f.nshr = lay + nlay; f.nchunk = lay + 2 * nlay; f.nblk9 = lay + 3 * nlay;
+    f.rec_acc = at<int>(base, o_racc);
+    f.G = at<int>(base, o_G);
+    f.chunks = at<int4>(base, o_ch);
+    f.blk9_end = at<int64_t>(base, o_be);
+    Shrunk sh;
+    double *shd = at<double>(base, o_sh_d);
+    sh.vnu = shd; sh.sp = shd + B.tot_f4; sh.alfa = shd + 2 * B.tot_f4; sh.epp = shd + 3 * B.tot_f4;
+    sh.spp = shd + 4 * B.tot_f4; sh.pmax = shd + 5 * B.tot_f4; sh.smin = shd + 6 * B.tot_f4;
+    sh.bvmax = shd + 7 * B.tot_f4;
+    sh.co2 = at<int8_t>(base, o_sh_b);
+    double *d_R4 = at<double>(base, o_R4);
+    double *d_nat = at<double>(base, o_nat);
+    HIPCHK(hipMemcpyAsync(f.stop, B.stop0.data(), sizeof(int) * nlay, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(f.chunks, 0xff, sizeof(int4) * std::max<int64_t>(1, B.tot_rec), st));
+    HIPCHK(hipMemsetAsync(C->d_err, 0, sizeof(int), st));
+    Tables T = ctx_tabview(C);
+    const int64_t npan = (int64_t)B.panels.size();
+    const int64_t nblk = (int64_t)B.blks.size();
+
+    HIPCHK(hipEventRecord(C->ev[0], st));
+    if (B.max_line > 0) {
+        hipLaunchKernelGGL(k_prep, dim3(cdiv(B.max_line, 256), nlay), dim3(256), 0, st, d_jobs, d_pan, L.dev, lp, C->d_err);
+        hipLaunchKernelGGL(k_home, dim3(nlay), dim3(1024), 0, st, d_jobs, lp);
+    }
+    hipLaunchKernelGGL(k_ranges, dim3(cdiv(npan, 256)), dim3(256), 0, st, d_jobs, d_pan, npan, lp);
+    HIPCHK(hipEventRecord(C->ev[1], st));
+    hipLaunchKernelGGL(k_conv64, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
+                       d_jobs, d_pan, lp, T, d_A);
+    HIPCHK(hipEventRecord(C->ev[2], st));
+    if (B.max_f4 > 0) {
+        hipLaunchKernelGGL(k_f4prep, dim3(cdiv(B.max_f4, 256), nlay), dim3(256), 0, st, d_jobs, L.dev, f, C->d_err);
+        hipLaunchKernelGGL(k_reccount, dim3(cdiv(B.max_rec, 256), nlay), dim3(256), 0, st, d_jobs, f);
+        hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, st, d_jobs, f, sh);
+        hipLaunchKernelGGL(k_sim, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, f, C->d_err);
+        hipLaunchKernelGGL(k_emit, dim3(cdiv(std::max<int64_t>(1, B.tot_rec), 64)), dim3(64), 0, st, d_jobs, f, sh,
+                           B.tot_rec);
+        hipLaunchKernelGGL(k_shraux, dim3(nlay), dim3(1024), 0, st, d_jobs, f, sh);
+    } else {
+        HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, st));
+    }
+    hipLaunchKernelGGL(k_r4range, dim3(cdiv(nblk, 256)), dim3(256), 0, st, d_jobs, d_blk, nblk, f, sh);
+    hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh, d_R4);
+    if (B.any_jrad1) hipLaunchKernelGGL(k_r4rad, dim3(cdiv(nblk, 64)), dim3(64), 0, st, d_jobs, d_blk, nblk, d_R4);
+    HIPCHK(hipEventRecord(C->ev[3], st));
+    double *pass2_out = d_out;
+    if (B.any_regrid) pass2_out = nullptr;
+    // layers without re-grid write straight to d_out; re-gridded ones go through d_nat
+    // (panel out_off already points into the right buffer per layer)
+    if (!B.any_regrid) {
+        hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
+                           d_abs, d_out);
+    } else {
+        // split: every panel writes through its out_off, which is relative to d_out for
+        // native layers and to d_nat for re-gridded ones; run twice with masks by layer
+        hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
+                           d_abs, d_nat);
+        int *d_rg = at<int>(base, o_rg);
+        int64_t *d_rgf = at<int64_t>(base, o_rgf);
+        HIPCHK(hipMemcpyAsync(d_rg, rg_lay.data(), sizeof(int) * rg_lay.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_rgf, rg_first.data(), sizeof(int64_t) * nlay, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_regrid, dim3((unsigned)rg_lay.size()), dim3(256), 0, st, d_jobs, d_rg, d_rgf, d_nat,
+                           d_out);
+    }
+    (void)pass2_out;
+    HIPCHK(hipEventRecord(C->ev[4], st));
+    HIPCHK(hipGetLastError());
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, C->d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (herr & 1) return set_err(CLBLM_ERR_UNSUPPORTED, "line coupling records (IFLG 1..3) are not supported");
+    if (herr & 4) return set_err(CLBLM_ERR_UNSUPPORTED, "shrink chunk longer than 64 TAPE3 records (too few lines of the selected molecules per record)");
+    float t01, t12, t23, t34;
+    HIPCHK(hipEventElapsedTime(&t01, C->ev[0], C->ev[1]));
+    HIPCHK(hipEventElapsedTime(&t12, C->ev[1], C->ev[2]));
+    HIPCHK(hipEventElapsedTime(&t23, C->ev[2], C->ev[3]));
+    HIPCHK(hipEventElapsedTime(&t34, C->ev[3], C->ev[4]));
+    C->ms[0] += t01; C->ms[1] += t12; C->ms[2] += t23; C->ms[3] += t34;
+    C->launches++;
+    (void)precision;
+    return 0;
+}
+
+// ====================================================================================
+// C-ABI
+// ====================================================================================
+extern "C" {
+
+int clblm_version(void) { return 1; }
+const char *clblm_last_error(void) { return g_err.c_str(); }
+
+int clblm_ctx_create(int device, clblm_ctx **out) {
+    if (!out) return set_err(CLBLM_ERR_ARG, "out is NULL");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_err(CLBLM_ERR_ARG, "no such device");
+    HIPCHK(hipSetDevice(device));
+    clblm_ctx *c = new clblm_ctx;
+    c->device = device;
+    HIPCHK(hipStreamCreate(&c->stream));
+    HIPCHK(hipMalloc(&c->d_err, sizeof(int)));
+    for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    int rc = ctx_tables(c);
+    if (rc) { delete c; return rc; }
+    *out = c;
+    return 0;
+}
+
+int clblm_ctx_destroy(clblm_ctx *c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    for (auto *l : c->lists)
+        if (l) { (void)hipFree(l->mem); delete l; }
+    if (c->arena.p) (void)hipFree(c->arena.p);
+    (void)hipFree(c->tab);
+    (void)hipFree(c->tabs);
+    (void)hipFree(c->d_err);
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int clblm_linelist_upload(clblm_ctx *c, int64_t n, const double *nu, const float *s, const float *a_air,
+                          const float *a_self, const float *e_lo, const float *tmpalf, const float *shift,
+                          const int32_t *mol, const int32_t *iflg, int *handle) {
+    if (!c || !handle) return set_err(CLBLM_ERR_ARG, "ctx or handle is NULL");
+    if (n <= 1 || !nu || !s || !a_air || !a_self || !e_lo || !tmpalf || !shift || !mol || !iflg)
+        return set_err(CLBLM_ERR_ARG, "line list needs at least two lines and every field");
+    for (int64_t i = 1; i < n; i++)
+        if (!(nu[i] >= nu[i - 1])) return set_err(CLBLM_ERR_ARG, "line centres must be sorted ascending");
+    for (int64_t i = 0; i < n; i++)
+        if (iflg[i] > 0) return set_err(CLBLM_ERR_UNSUPPORTED, "line coupling records (IFLG 1..3) are not supported");
+    HIPCHK(hipSetDevice(c->device));
+    LineListH *L = new LineListH;
+    L->n = n;
+    L->nu.assign(nu, nu + n);
+    size_t bytes = (size_t)n * (8 + 6 * 4 + 2 * 4) + 8 * 256;
+    HIPCHK(hipMalloc(&L->mem, bytes));
+    Carve cv;
+    char *b = (char *)L->mem;
+    size_t onu = cv.take<double>(n), os = cv.take<float>(n), oa = cv.take<float>(n), osf = cv.take<float>(n);
+    size_t oe = cv.take<float>(n), ot = cv.take<float>(n), osh = cv.take<float>(n), om = cv.take<int32_t>(n);
+    size_t oi = cv.take<int32_t>(n);
+    HIPCHK(hipMemcpy(b + onu, nu, 8 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + os, s, 4 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + oa, a_air, 4 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + osf, a_self, 4 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + oe, e_lo, 4 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + ot, tmpalf, 4 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + osh, shift, 4 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + om, mol, 4 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + oi, iflg, 4 * n, hipMemcpyHostToDevice));
+    L->dev.n = n;
+    L->dev.nu = at<double>(b, onu);
+    L->dev.s = at<float>(b, os);
+    L->dev.a_air = at<float>(b, oa);
+    L->dev.a_self = at<float>(b, osf);
+    L->dev.e_lo = at<float>(b, oe);
+    L->dev.tmpalf = at<float>(b, ot);
+    L->dev.shift = at<float>(b, osh);
+    L->dev.mol = at<int32_t>(b, om);
+    L->dev.iflg = at<int32_t>(b, oi);
+    std::lock_guard<std::mutex> g(c->mu);
+    c->lists.push_back(L);
+    *handle = (int)c->lists.size() - 1;
+    return 0;
+}
+
+int clblm_linelist_free(clblm_ctx *c, int handle) {
+    if (!c || handle < 0 || handle >= (int)c->lists.size() || !c->lists[handle])
+        return set_err(CLBLM_ERR_ARG, "bad line list handle");
+    (void)hipFree(c->lists[handle]->mem);
+    delete c->lists[handle];
+    c->lists[handle] = nullptr;
+    return 0;
+}
+
+int clblm_od_batch(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers, const clblm_od_params *pars,
+                   int32_t *jrad_state, int64_t *offsets, double *d_od, void *stream) {
+    if (!c || !layers || !pars || !jrad_state || !offsets || nlay <= 0)
+        return set_err(CLBLM_ERR_ARG, "null argument or nlay <= 0");
+    if (handle < 0 || handle >= (int)c->lists.size() || !c->lists[handle])
+        return set_err(CLBLM_ERR_ARG, "bad line list handle");
+    int prec = pars[0].precision;
+    if (prec != CLBLM_FP64 && prec != CLBLM_FP32C) return set_err(CLBLM_ERR_ARG, "unknown precision");
+    HIPCHK(hipSetDevice(c->device));
+    int32_t jr = *jrad_state;
+    Batch B;
+    int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec);
+    if (rc) return rc;
+    for (int l = 0; l < nlay; l++) offsets[l] = B.jobs[l].out_off;
+    offsets[nlay] = B.tot_out;
+    if (!d_od) return 0;
+    std::lock_guard<std::mutex> g(c->mu);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    rc = run_batch(c, *c->lists[handle], B, d_od, st, prec);
+    if (rc) return rc;
+    *jrad_state = jr;
+    return 0;
+}
+
+int clblm_odlay(clblm_ctx *c, int handle, const clblm_layer *layer, const clblm_od_params *par,
+                int32_t *jrad_state, int64_t *nsamp, double *dv_used, double *od_out) {
+    if (!c || !layer || !par || !jrad_state || !nsamp) return set_err(CLBLM_ERR_ARG, "null argument");
+    int64_t off[2];
+    int32_t jr = *jrad_state;
+    int rc = clblm_od_batch(c, handle, 1, layer, par, &jr, off, nullptr, nullptr);
+    if (rc) return rc;
+    *nsamp = off[1];
+    if (dv_used) {
+        double DV = par->dvnormal;
+        if (!(std::fabs(par->dvnormal - par->dvnarrow) / par->dvnormal > 0.2)) DV = par->dvnarrow;
+        *dv_used = (par->dvout > 0. && !(std::fabs(par->dvout - DV) < 2.220446049250313e-16)) ? par->dvout : DV;
+    }
+    if (!od_out) return 0;
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * off[1]));
+    rc = clblm_od_batch(c, handle, 1, layer, par, jrad_state, off, d, nullptr);
+    if (rc == 0) {
+        hipError_t e = hipMemcpy(od_out, d, sizeof(double) * off[1], hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = set_err(CLBLM_ERR_HIP, hipGetErrorString(e));
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+int clblm_timing(clblm_ctx *c, int reset, double *ms_prep, double *ms_conv, double *ms_f4, double *ms_panel,
+                 int64_t *launches) {
+    if (!c) return set_err(CLBLM_ERR_ARG, "ctx is NULL");
+    int64_t n = c->launches > 0 ? c->launches : 1;
+    if (ms_prep) *ms_prep = c->ms[0] / n;
+    if (ms_conv) *ms_conv = c->ms[1] / n;
+    if (ms_f4) *ms_f4 = c->ms[2] / n;
+    if (ms_panel) *ms_panel = c->ms[3] / n;
+    if (launches) *launches = c->launches;
+    if (reset) { c->ms[0] = c->ms[1] = c->ms[2] = c->ms[3] = 0; c->launches = 0; }
+    return 0;
+}
+
+int clblm_rt_up(clblm_ctx *, int, int64_t, double, double, const double *, const double *, const double *, double,
+                double, int, double *, double *, double *, void *) {
+    return set_err(CLBLM_ERR_UNSUPPORTED, "not built yet");
+}
+
+}  // extern "C"

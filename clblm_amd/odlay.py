@@ -1,0 +1,123 @@
+"""Python driver of the layer optical-depth path.
+
+``Context.odlay`` mirrors the reference entry point ODLAY_nonStruct
+(src/clblm_src/clblm_ODLAY.f90:129-176): one layer, its molecules and columns,
+V1/V2, DVnormal/DVnarrow, SAMPLE and an optional DVout, returning the OD
+spectrum and the DV actually used.  JRAD is carried across calls exactly like
+the reference's SAVE variable (clblm_ODLAY.f90:182-183, 349-352).
+``Context.od_batch`` runs many layers in one set of launches and returns a
+ROCm tensor.  Everything goes through libclblm_hip.so; nothing is computed on
+the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FP32C, FP64, ODParams, check
+
+
+def _layer_struct(layer) -> _lib.Layer:
+    s = _lib.Layer()
+    s.pave, s.tave, s.wtot = float(layer.pave), float(layer.tave), float(layer.wtot)
+    s.zbot, s.ztop = float(getattr(layer, "zbot", 0.0)), float(getattr(layer, "ztop", 0.0))
+    for m, v in layer.w.items():
+        s.w[int(m)] = float(v)
+        s.sel[int(m)] = 1
+    return s
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        self.ptr = ctypes.c_void_p()
+        check(self.lib.clblm_ctx_create(int(device), ctypes.byref(self.ptr)))
+        self.jrad = 1
+        self._keep = []
+
+    def close(self):
+        if self.ptr:
+            self.lib.clblm_ctx_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- line list ------------------------------------------------------------
+    def upload_lines(self, lines) -> int:
+        arrs = [np.ascontiguousarray(a) for a in (lines.nu, lines.s, lines.a_air, lines.a_self, lines.e_lo,
+                                                  lines.tmpalf, lines.shift, lines.mol, lines.iflg)]
+        assert arrs[0].dtype == np.float64 and arrs[7].dtype == np.int32
+        h = ctypes.c_int()
+        check(self.lib.clblm_linelist_upload(self.ptr, int(lines.n), *[a.ctypes.data for a in arrs],
+                                             ctypes.byref(h)))
+        return h.value
+
+    # -- one layer (host buffers) ----------------------------------------------
+    @staticmethod
+    def params(v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, precision=FP64, continuum=None):
+        p = ODParams()
+        p.v1, p.v2, p.dvnormal = float(v1), float(v2), float(dvnormal)
+        p.dvnarrow = float(dvnormal if dvnarrow is None else dvnarrow)
+        p.sample, p.dvout, p.precision = float(sample), float(dvout), int(precision)
+        if continuum is not None:
+            c = np.ascontiguousarray(continuum, np.float64)
+            p.contnm, p.continuum, p.nptabs = 1, c.ctypes.data, int(c.size)
+            p._keep = c
+        return p
+
+    def odlay(self, handle, layer, v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0,
+              precision=FP64, continuum=None):
+        lay = _layer_struct(layer)
+        par = self.params(v1, v2, dvnormal, dvnarrow, sample, dvout, precision, continuum)
+        jr = ctypes.c_int32(self.jrad)
+        n = ctypes.c_int64()
+        dvu = ctypes.c_double()
+        check(self.lib.clblm_odlay(self.ptr, handle, ctypes.byref(lay), ctypes.byref(par), ctypes.byref(jr),
+                                   ctypes.byref(n), ctypes.byref(dvu), None))
+        out = np.zeros(n.value)
+        jr = ctypes.c_int32(self.jrad)
+        check(self.lib.clblm_odlay(self.ptr, handle, ctypes.byref(lay), ctypes.byref(par), ctypes.byref(jr),
+                                   ctypes.byref(n), ctypes.byref(dvu), out.ctypes.data_as(ctypes.c_void_p)))
+        self.jrad = jr.value
+        return out, dvu.value
+
+    # -- many layers (device buffers) -------------------------------------------
+    def od_batch_plan(self, handle, layers, params):
+        nl = len(layers)
+        lay = (_lib.Layer * nl)(*[_layer_struct(l) for l in layers])
+        par = (ODParams * nl)(*params)
+        off = np.zeros(nl + 1, np.int64)
+        jr = ctypes.c_int32(self.jrad)
+        check(self.lib.clblm_od_batch(self.ptr, handle, nl, ctypes.cast(lay, ctypes.c_void_p),
+                                      ctypes.cast(par, ctypes.c_void_p), ctypes.byref(jr),
+                                      off.ctypes.data_as(ctypes.c_void_p), None, None))
+        return lay, par, off
+
+    def od_batch(self, handle, layers, params, out=None, stream=None):
+        """Returns (od tensor on the GPU, offsets).  ``out`` may be a preallocated
+        float64 cuda tensor of offsets[-1] elements."""
+        import torch
+        lay, par, off = self.od_batch_plan(handle, layers, params)
+        if out is None:
+            out = torch.empty(int(off[-1]), dtype=torch.float64, device="cuda")
+        assert out.dtype == torch.float64 and out.is_cuda and out.numel() >= off[-1]
+        jr = ctypes.c_int32(self.jrad)
+        st = None if stream is None else ctypes.c_void_p(stream)
+        check(self.lib.clblm_od_batch(self.ptr, handle, len(layers), ctypes.cast(lay, ctypes.c_void_p),
+                                      ctypes.cast(par, ctypes.c_void_p), ctypes.byref(jr),
+                                      off.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(out.data_ptr()), st))
+        self.jrad = jr.value
+        return out, off
+
+    def timing(self, reset=False):
+        vals = [ctypes.c_double() for _ in range(4)]
+        n = ctypes.c_int64()
+        check(self.lib.clblm_timing(self.ptr, int(reset), *[ctypes.byref(v) for v in vals], ctypes.byref(n)))
+        return dict(prep=vals[0].value, conv=vals[1].value, f4=vals[2].value, panel=vals[3].value,
+                    launches=n.value)

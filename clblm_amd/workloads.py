@@ -1,0 +1,186 @@
+"""Synthetic line lists and atmospheres for the benchmark workloads.
+
+The reference ships no line file (LNFL is external) and no scene, so every
+workload is generated from a fixed seed (``numpy.random.default_rng(1000 + k)``).
+Line fields are rounded to TAPE3 precision: centre FP64; intensity, widths,
+lower-state energy, temperature exponent and shift FP32
+(record layout: src/clblm_src/clblm_LineData.f90:60-70).
+
+TAPE3 conventions (LNFL is not in the reference; these follow the reference's
+algebra, clblm_ODLAY.f90:1805-1864,1928-1935):
+  * stored intensity SP = S_HITRAN(296) / (nu * (1 - exp(-c2 nu / 296)))  so that
+    SP * RADFN * (1 + exp(-c2 nu/T)) reproduces S(T) at 296 K;
+  * stored TMPALF = 1 - n for a HITRAN-style width exponent n, because the
+    reference scales widths by (p/p0)(296/T)(T/296)**TMPALF;
+  * molecule code MOL = m + 100*iso.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List
+
+import numpy as np
+
+RADCN2 = 1.4387752          # clblm_ConstParam.f90:71
+P0 = 1013.25                # clblm_ConstParam.f90:110
+T0 = 296.0                  # clblm_ConstParam.f90:112
+AIR_COLUMN_PER_HPA = 100.0 / (9.80665 * 0.0289644) * 6.02214199e23 * 1e-4  # molec cm-2 hPa-1
+
+SPECIES = [  # (HITRAN molecule number, name, mass for the synthetic list)
+    (1, "H2O", 18), (2, "CO2", 44), (3, "O3", 48), (4, "N2O", 44),
+    (5, "CO", 28), (6, "CH4", 16), (7, "O2", 32),
+]
+
+
+@dataclasses.dataclass
+class LineList:
+    """A TAPE3-equivalent line list, sorted by centre.  Records hold 250 lines."""
+    nu: np.ndarray          # float64
+    s: np.ndarray           # float32, stored intensity SP
+    a_air: np.ndarray       # float32
+    a_self: np.ndarray      # float32
+    e_lo: np.ndarray        # float32
+    tmpalf: np.ndarray      # float32
+    shift: np.ndarray       # float32
+    mol: np.ndarray         # int32, m + 100*iso
+    iflg: np.ndarray        # int32
+
+    @property
+    def n(self) -> int:
+        return int(self.nu.shape[0])
+
+
+@dataclasses.dataclass
+class Layer:
+    pave: float             # hPa
+    tave: float             # K
+    wtot: float             # molec cm-2 (broadening column)
+    w: dict                 # molecule number -> column, molec cm-2
+    zbot: float = 0.0
+    ztop: float = 0.0
+
+
+def make_lines(n: int, v1: float, v2: float, seed: int, mols=(1,)) -> LineList:
+    """Lines from the distributions the workloads name: centres uniform in [v1, v2],
+    S_HITRAN log-uniform in [1e-26, 1e-19], air widths uniform in [0.02, 0.10],
+    self widths 5x air, width exponent n uniform in [0.5, 0.8], E'' uniform in
+    [0, 3000].  Molecules are assigned round-robin over ``mols`` after sorting,
+    each line's isotopologue is 1."""
+    rng = np.random.default_rng(seed)
+    nu = np.sort(rng.uniform(v1, v2, n))
+    s296 = 10.0 ** rng.uniform(-26.0, -19.0, n)
+    a_air = rng.uniform(0.02, 0.10, n)
+    a_self = 5.0 * a_air
+    nexp = rng.uniform(0.5, 0.8, n)
+    e_lo = rng.uniform(0.0, 3000.0, n)
+    mol_pick = rng.integers(0, len(mols), n)
+    mol = np.asarray(mols, dtype=np.int32)[mol_pick] + 100
+    sp = s296 / (nu * (1.0 - np.exp(-RADCN2 * nu / T0)))
+    return LineList(
+        nu=nu.astype(np.float64),
+        s=sp.astype(np.float32),
+        a_air=a_air.astype(np.float32),
+        a_self=a_self.astype(np.float32),
+        e_lo=e_lo.astype(np.float32),
+        tmpalf=(1.0 - nexp).astype(np.float32),
+        shift=np.zeros(n, np.float32),
+        mol=mol.astype(np.int32),
+        iflg=np.zeros(n, np.int32),
+    )
+
+
+def calculate_dv(vbar, pave, tave, frh2o, alfal0=0.04, sample=4.0):
+    """calculateDV, clblm_setDV.f90:221-271."""
+    alfcor = (pave / P0) * math.sqrt(T0 / tave)
+    h2oslf = (1.0 - frh2o + 5.0 * frh2o)
+    albar = alfal0 * alfcor * h2oslf
+    adbar = 3.58115e-07 * vbar * math.sqrt(tave / 36.0)
+    avbar = 0.5 * (albar + math.sqrt(albar * albar + 4.0 * adbar * adbar))
+    return avbar / sample
+
+
+# --------------------------------------------------------------------------
+# Single-layer case: one layer at 1013.25 hPa, 296 K, 20 000 lines of one species.
+# --------------------------------------------------------------------------
+def single_layer(n_lines=20000, v1=600.0, v2=1100.0, seed=1000):
+    lines = make_lines(n_lines, v1, v2, seed, mols=(1,))
+    layer = Layer(pave=1013.25, tave=296.0, wtot=1e24, w={1: 1e22})
+    alfal0 = float(np.mean(lines.a_air.astype(np.float64)))
+    dv = calculate_dv(0.5 * (v1 + v2), layer.pave, layer.tave, 1e22 / 1e24, alfal0, 4.0)
+    return lines, [layer], dict(v1=v1, v2=v2, dv=dv, alfal0=alfal0)
+
+
+# --------------------------------------------------------------------------
+# Full-profile atmosphere.
+# --------------------------------------------------------------------------
+VMR_SURFACE = {1: 1.0e-2, 2: 400e-6, 3: 3e-8, 4: 320e-9, 5: 100e-9, 6: 1.8e-6, 7: 0.209}
+
+
+def vmr_profile(m: int, p: float) -> float:
+    """Recorded volume-mixing-ratio profiles (p in hPa)."""
+    if m == 1:   # H2O: 2 km scale height in pressure terms, floor 4 ppmv
+        return max(1.0e-2 * (p / 1013.0) ** 3.5, 4e-6)
+    if m == 3:   # O3: peak of 8 ppmv near 10 hPa
+        x = math.log(p / 10.0)
+        return 3e-8 + 8e-6 * math.exp(-0.5 * (x / 1.2) ** 2)
+    return VMR_SURFACE[m]
+
+
+def temperature_profile(z_km: float) -> float:
+    return max(288.0 - 6.5 * z_km, 217.0)
+
+
+def atmosphere(n_layers=60, p_top=0.1, mols=(1, 2, 3, 4, 5, 6, 7), t_noise=None,
+               h2o_scale=1.0) -> List[Layer]:
+    """Layers between log-spaced pressure levels from 1013 hPa to ``p_top``.
+    Temperature: 288 K at the surface, -6.5 K/km to 217 K, isothermal above;
+    heights from the hypsometric relation.  Layer pressure is the mass-weighted
+    mean (dp / dln p), temperature the level mean, columns from dp."""
+    levels = np.exp(np.linspace(math.log(1013.0), math.log(p_top), n_layers + 1))
+    z = [0.0]
+    t_lev = [temperature_profile(0.0)]
+    for k in range(1, n_layers + 1):
+        # iterate the hypsometric step with the layer-mean temperature
+        t_guess = t_lev[-1]
+        for _ in range(3):
+            dz = 287.05 * 0.5 * (t_lev[-1] + t_guess) / 9.80665 * math.log(levels[k - 1] / levels[k]) / 1000.0
+            t_guess = temperature_profile(z[-1] + dz)
+        z.append(z[-1] + dz)
+        t_lev.append(t_guess)
+    layers = []
+    for k in range(n_layers):
+        p1, p2 = levels[k], levels[k + 1]
+        pave = (p1 - p2) / math.log(p1 / p2)
+        tave = 0.5 * (t_lev[k] + t_lev[k + 1])
+        if t_noise is not None:
+            tave += float(t_noise[k])
+        wtot = (p1 - p2) * AIR_COLUMN_PER_HPA
+        w = {m: vmr_profile(m, pave) * wtot for m in mols}
+        if 1 in w:
+            w[1] *= h2o_scale
+        layers.append(Layer(pave=float(pave), tave=float(tave), wtot=float(wtot), w=w,
+                            zbot=float(z[k]), ztop=float(z[k + 1])))
+    return layers
+
+
+def profile_lines(n_lines=200000, v1=500.0, v2=2500.0, seed=1001):
+    """200 000 lines of molecules 1-7, split round-robin by a seeded draw."""
+    return make_lines(n_lines, v1, v2, seed, mols=tuple(m for m, _, _ in SPECIES))
+
+
+def continuum_absrb(layer: Layer, v1: float, v2: float, seed=1001, nknots=50):
+    """Smooth continuum on the 1 cm-1 grid OPDPTH uses for ABSRB
+    (clblm_ODLAY.f90:881-896): V1ABS = INT(V1)-3, V2ABS = INT(V2+3.5).
+    log10 of the coefficient is a natural cubic spline through ``nknots`` seeded
+    knots in [-27, -25] cm2/molec; the absorber column is the layer's H2O."""
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(seed + 7919)
+    v1abs = float(int(v1)) - (1.0 if v1 < 0 else 0.0) - 3.0
+    v2abs = float(int(v2 + 3.0 + 0.5))
+    npt = int((v2abs - v1abs) / 1.0 + 1.5)
+    knots = np.linspace(v1abs, v2abs, nknots)
+    vals = rng.uniform(-27.0, -25.0, nknots)
+    spl = CubicSpline(knots, vals, bc_type="natural")
+    grid = v1abs + np.arange(npt, dtype=np.float64)
+    return (10.0 ** spl(grid)) * layer.w.get(1, 0.0), v1abs, npt

@@ -1,0 +1,121 @@
+/*
+ * clblm_hip.h -- C-ABI of the MI355X layer optical-depth library.
+ *
+ * The library replaces the reference's per-layer OD entry point and the line
+ * file it reads.  Every entry point returns 0 on success and a nonzero status on
+ * error; clblm_last_error() returns the message of the last failure on the
+ * calling thread.  Plain pointers and sizes only; no torch types.
+ *
+ * Molecules 1-7 (H2O, CO2, O3, N2O, CO, CH4, O2) and all their isotopologues
+ * are supported (TIPS data and masses are built in).
+ */
+#ifndef CLBLM_HIP_H
+#define CLBLM_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLBLM_OK 0
+#define CLBLM_ERR_ARG 1
+#define CLBLM_ERR_UNSUPPORTED 2
+#define CLBLM_ERR_HIP 3
+#define CLBLM_ERR_CAPACITY 4
+
+#define CLBLM_FP64 0         /* plain path: FP64 everywhere, 1e-10 of the oracle */
+#define CLBLM_FP32C 1        /* FP64 offsets/intensities, FP32 Kahan line-shape sums */
+
+#define CLBLM_MAXMOL 47
+
+typedef struct clblm_ctx clblm_ctx;
+
+/* One atmospheric layer: the arguments Pav, Tav, W(:), Wtot, Zbot, Ztop and the
+ * molecule list of ODLAY_nonStruct (src/clblm_src/clblm_ODLAY.f90:129-176). */
+typedef struct {
+    double pave;                 /* hPa */
+    double tave;                 /* K */
+    double wtot;                 /* broadening column, molec/cm2 */
+    double zbot, ztop;           /* km (informational) */
+    double w[CLBLM_MAXMOL + 1];  /* w[m] = column of HITRAN molecule m, molec/cm2 */
+    int32_t sel[CLBLM_MAXMOL + 1]; /* sel[m] != 0: molecule m is a line molecule of the layer */
+} clblm_layer;
+
+/* Spectral grid and control record: V1, V2, DVnormal, DVnarrow, SAMPLE, DVout
+ * of ODLAY_nonStruct plus the CLBLM_OD_Ctrl fields this path honours
+ * (src/clblm_src/clblm_Config.f90:84-118). */
+typedef struct {
+    double v1, v2;               /* cm-1 */
+    double dvnormal, dvnarrow;   /* per-layer DV (setDV) */
+    double sample;               /* SAMPLE, 4 by default */
+    double dvout;                /* > 0: re-grid the OD to this DV (interpSpectrum); <= 0: native */
+    int32_t precision;           /* CLBLM_FP64 or CLBLM_FP32C */
+    int32_t contnm;              /* nonzero: continuum[] supplied on the 1 cm-1 ABSRB grid */
+    const double *continuum;     /* host pointer, nptabs values from V1ABS = INT(V1)-3 */
+    int32_t nptabs;
+    int32_t reserved;
+} clblm_od_params;
+
+/* ---- context ---------------------------------------------------------------- */
+int clblm_ctx_create(int device, clblm_ctx **out);
+int clblm_ctx_destroy(clblm_ctx *ctx);
+const char *clblm_last_error(void);
+int clblm_version(void);
+
+/* ---- line list --------------------------------------------------------------
+ * Replaces the TAPE3 reads of RDLIN (clblm_ODLAY.f90:2571-2798) and RDLNFL
+ * (clblm_ODLAY_LineF4.f90:1355-1498).  Host arrays of n lines sorted by nu;
+ * consecutive groups of 250 lines form the TAPE3 records.  mol = m + 100*iso.
+ * iflg: 0 for a plain line, < 0 for a record the readers skip (line coupling,
+ * IFLG 1..3, is not supported and is rejected). */
+int clblm_linelist_upload(clblm_ctx *ctx, int64_t n, const double *nu, const float *s,
+                          const float *a_air, const float *a_self, const float *e_lo,
+                          const float *tmpalf, const float *shift, const int32_t *mol,
+                          const int32_t *iflg, int *handle);
+int clblm_linelist_free(clblm_ctx *ctx, int handle);
+
+/* ---- one layer, host buffers -------------------------------------------------
+ * Drop-in for ODLAY_nonStruct (clblm_ODLAY.f90:129-611).  Two-phase sizing:
+ * call with od_out == NULL to get *nsamp, then with a buffer of >= *nsamp
+ * doubles.  *jrad_state carries the reference's SAVEd JRAD across calls
+ * (clblm_ODLAY.f90:182-183,349-352; OPDPTH :877-878): set it to the control
+ * record's JRAD (1) before the first call of a run and pass it back unchanged.
+ * *dv_used returns the DV of the returned grid (the DVout argument). */
+int clblm_odlay(clblm_ctx *ctx, int handle, const clblm_layer *layer, const clblm_od_params *par,
+                int32_t *jrad_state, int64_t *nsamp, double *dv_used, double *od_out);
+
+/* ---- many layers, device buffers, asynchronous --------------------------------
+ * nlay layers (layers[], pars[] host arrays) in one set of launches on `stream`
+ * (a hipStream_t; NULL = the context's stream).  Query with d_od == NULL:
+ * offsets[k] receives the first index of layer k in d_od and offsets[nlay] the
+ * total.  Then d_od (device memory, offsets[nlay] doubles) is filled.  jrad is
+ * applied to all layers with the reference's first-call rule (layers are treated
+ * as successive calls, in order). */
+int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *layers,
+                   const clblm_od_params *pars, int32_t *jrad_state, int64_t *offsets,
+                   double *d_od, void *stream);
+
+/* ---- radiance and Jacobians ----------------------------------------------------
+ * Upwelling nadir radiance at the top of nlay layers (layer 0 at the surface) on a
+ * common grid v1 + i*dv, i < npts, from layer ODs d_od[l*npts + i] and layer
+ * temperatures; the reference's mode2_lookDn_mergeUp/layerMerge_mode2
+ * (clblm_noScattRT.f90:1134-1870) with layerEmis (clblm_EMLAY.f90:104-178).
+ * lin_in_tau: 0 (isothermal layer) or 1 (LBLRTM linear-in-tau).
+ * t_lev[nlay+1] level temperatures (used when lin_in_tau != 0).
+ * Outputs (device): d_rad[npts]; if d_dr_dod != NULL also dR/dOD_l
+ * (nlay*npts) and d_dr_dtlay (nlay*npts) = dR/dT_l through the Planck source. */
+int clblm_rt_up(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv,
+                const double *d_od, const double *t_lay, const double *t_lev,
+                double surf_emis, double t_skin, int lin_in_tau, double *d_rad,
+                double *d_dr_dod, double *d_dr_dtlay, void *stream);
+
+/* ---- timing -------------------------------------------------------------------
+ * Average device time (ms) of each pipeline stage over the launches since the
+ * last reset, measured with events on the stream the stages run on. */
+int clblm_timing(clblm_ctx *ctx, int reset, double *ms_prep, double *ms_conv, double *ms_f4,
+                 double *ms_panel, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

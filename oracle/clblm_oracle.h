@@ -1,0 +1,84 @@
+/*
+ * CLBLM oracle: a sequential CPU restatement of the reference's layer optical-depth
+ * path (ODLAY -> OPDPTH -> LINF4Q/SHRINQ/LBLF4Q -> HIRACQ/LNCORQ/CNVFNQ/panelOut).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker.
+ * The product (clblm_amd/) never links, loads or calls it.
+ *
+ * All arithmetic is IEEE double with no FMA contraction (build with
+ * -ffp-contract=off), matching the reference's -fdefault-real-8 build, whose
+ * default REALs are 8 bytes.
+ */
+#ifndef CLBLM_ORACLE_H
+#define CLBLM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* TAPE3-equivalent line list (one record = 250 consecutive lines). */
+typedef struct {
+    int64_t n;
+    const double *nu;      /* line centre, cm-1, sorted ascending */
+    const float *s;        /* intensity as stored in TAPE3 */
+    const float *a_air;    /* air-broadened HWHM at 296 K, 1 atm */
+    const float *e_lo;     /* lower-state energy, cm-1 */
+    const int32_t *mol;    /* m + 100*iso */
+    const float *a_self;   /* self-broadened HWHM */
+    const float *tmpalf;   /* TAPE3 temperature exponent of the width */
+    const float *shift;    /* pressure shift, cm-1/atm */
+    const int32_t *iflg;   /* 0 = plain line, <0 = skipped record */
+} orc_lines;
+
+typedef struct {
+    double pave, tave, wtot; /* hPa, K, molec/cm2 */
+    double w[48];            /* column of molecule m at w[m], m = 1..47 */
+    int sel[48];             /* nonzero if molecule m is a line molecule of this layer */
+} orc_layer;
+
+typedef struct {
+    double v1, v2;           /* spectral range, cm-1 */
+    double dvnormal, dvnarrow, sample;
+    double dvout;            /* <= 0: native grid */
+    int jrad;                /* state in/out (reference SAVE variable) */
+    int contnm;              /* nonzero: absrb[] holds the continuum on the OPDPTH 1 cm-1 grid */
+    const double *absrb;
+    int nptabs;
+} orc_params;
+
+/* Builds the line-shape tables (SHAPEL/SHAPEG/VERFN).  Arrays are 1-based
+ * (element 0 unused) with 3002 slots. */
+void orc_tables(double *f1, double *f2, double *f3, double *fg, double *xver);
+
+/* Layer optical depth.  out may be NULL to query the number of points.
+ * Returns 0 on success, nonzero on error (message via orc_error()). */
+int orc_odlay(const orc_lines *lines, const orc_layer *layer, orc_params *par,
+              double *out, int64_t cap, int64_t *nsamp, double *dv_used);
+
+const char *orc_error(void);
+
+/* Helpers exposed for unit tests. */
+double orc_radfn(double vi, double xkt);
+double orc_radfni(double vi, double dvi, double xkt, double *vinew, double *rdel, double *rdlast);
+int orc_tips_scor(int mol, int iso, double t, double *scor);
+void orc_xint(double v1a, double v2a, double dva, const double *a1, double afact,
+              double vft, double dvr3, double *r3_1, int n1r3, int n2r3);
+void orc_interp(const double *spin, double dvin, double v1in, int nin,
+                double *spout, double dvout, double v1out, int nout);
+double orc_calculate_dv(double vbar, double pave, double tave, double frh2o,
+                        double alfal0, double sample);
+double orc_avrat(int iz);
+int orc_molec(const orc_layer *layer, double *scor /*[48*11]*/, double *rhoslf /*[48]*/,
+              double *alfd1 /*[48*11]*/);
+
+/* Radiative transfer and Jacobians (upwelling nadir, merge-up). */
+double orc_planck(double v, double t);
+void orc_planck_drv(double v, double t, double *b, double *dbdt);
+void orc_rlin(double tau, double *w, double *dw);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

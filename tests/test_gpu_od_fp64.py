@@ -1,0 +1,90 @@
+"""FP64 GPU path vs the oracle (relative 1e-10), through the C-ABI."""
+import numpy as np
+import pytest
+
+from clblm_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from clblm_amd.odlay import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _cmp(od, ref, rtol=RTOL):
+    assert od.shape == ref.shape, (od.shape, ref.shape)
+    err = np.abs(od - ref) / np.maximum(np.abs(ref), 1e-300)
+    i = int(np.argmax(err))
+    assert err[i] <= rtol, "max rel err %.3e at %d (gpu %r ref %r)" % (err[i], i, od[i], ref[i])
+    return float(err.max())
+
+
+def test_single_layer_case(ctx):
+    """One layer, 1013.25 hPa, 296 K, 20 000 lines of one species over 600-1100 cm-1."""
+    from oracle.py import Oracle
+    lines, layers, cfg = W.single_layer()
+    h = ctx.upload_lines(lines)
+    ctx.jrad = 1
+    od, dv = ctx.odlay(h, layers[0], cfg["v1"], cfg["v2"], cfg["dv"])
+    orc = Oracle(lines)
+    ref, dvr = orc.odlay(layers[0], cfg["v1"], cfg["v2"], cfg["dv"])
+    assert dv == dvr and ctx.jrad == orc.jrad == 0
+    print("single layer: %d points, max rel err %.3e" % (od.size, _cmp(od, ref)))
+
+
+@pytest.mark.parametrize("k", [0, 9, 25, 41, 59])
+def test_profile_layer_seven_species(ctx, k):
+    """Layers of the 60-layer profile (7 species) on a 100 cm-1 window."""
+    from oracle.py import Oracle
+    lines = W.profile_lines(40000, 500.0, 800.0, seed=1001)
+    layers = W.atmosphere()
+    lay = layers[k]
+    dv = W.calculate_dv(650.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+    h = ctx.upload_lines(lines)
+    ctx.jrad = 1
+    od, _ = ctx.odlay(h, lay, 600.0, 700.0, dv)
+    ref, _ = Oracle(lines).odlay(lay, 600.0, 700.0, dv)
+    print("layer %d p=%.3g T=%.1f dv=%.3g: %d points, max rel err %.3e" % (k, lay.pave, lay.tave, dv, od.size,
+                                                                          _cmp(od, ref)))
+
+
+def test_continuum_and_regrid(ctx):
+    from oracle.py import Oracle
+    lines = W.profile_lines(20000, 500.0, 800.0, seed=1001)
+    lay = W.atmosphere()[5]
+    dv = W.calculate_dv(650.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+    cont, v1abs, npt = W.continuum_absrb(lay, 600.0, 700.0)
+    h = ctx.upload_lines(lines)
+    for dvout in (0.0, dv * 0.37, dv * 0.5):
+        ctx.jrad = 1
+        od, dvu = ctx.odlay(h, lay, 600.0, 700.0, dv, dvout=dvout, continuum=cont)
+        orc = Oracle(lines)
+        ref, dvr = orc.odlay(lay, 600.0, 700.0, dv, dvout=dvout, continuum=cont)
+        assert dvu == dvr
+        print("dvout %.4g: %d points, max rel err %.3e" % (dvout, od.size, _cmp(od, ref)))
+
+
+def test_batch_matches_single_calls(ctx):
+    import torch
+    from clblm_amd.odlay import Context
+    lines = W.profile_lines(30000, 500.0, 800.0, seed=1001)
+    layers = W.atmosphere()[::12]
+    h = ctx.upload_lines(lines)
+    pars, singles = [], []
+    ctx.jrad = 1
+    for lay in layers:
+        dv = W.calculate_dv(650.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+        pars.append(Context.params(600.0, 700.0, dv))
+        od, _ = ctx.odlay(h, lay, 600.0, 700.0, dv)
+        singles.append(od)
+    ctx.jrad = 1
+    out, off = ctx.od_batch(h, layers, pars)
+    torch.cuda.synchronize()
+    host = out.cpu().numpy()
+    for k, od in enumerate(singles):
+        np.testing.assert_array_equal(host[off[k]:off[k + 1]], od)
