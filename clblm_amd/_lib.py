@@ -13,7 +13,7 @@ FP64, FP32C = 0, 1
 
 EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
            "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch",
-           "clblm_rt_up", "clblm_timing"]
+           "clblm_rt_up", "clblm_timing", "clblm_work"]
 
 
 class Layer(ctypes.Structure):
@@ -61,6 +61,8 @@ def load():
                                 P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, P, P, P, P]
     lib.clblm_timing.argtypes = [P, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 4 + \
         [ctypes.POINTER(ctypes.c_int64)]
+    lib.clblm_work.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     _lib = lib
     return lib
 

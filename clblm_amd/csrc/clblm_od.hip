@@ -300,7 +300,7 @@ __global__ void k_ranges(const Job *jobs, Panel *panels, int64_t npanel_tot, Lin
 // indices are never read by the reference's panel logic.
 // ------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *panels, LinePrep lp,
-                                                Tables T, double *A) {
+                                                Tables T, double *A, unsigned long long *work) {
     extern __shared__ double sm[];
     const Panel &P = panels[blockIdx.x];
     const Job &J = jobs[P.layer];
@@ -313,6 +313,7 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
     const double wavdxf = DV / DXF1, hwdxf = HWF1 / DXF1;
     const double conf2 = DV / J.dvr2, conf3 = DV / J.dvr3;
     const double vft = P.vft;
+    unsigned long long nev = 0;
     for (int64_t o = P.lstart + threadIdx.x; o < P.lend; o += blockDim.x) {
         double sp = lp.sp[o];
         if (sp == 0.) continue;
@@ -337,6 +338,7 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
         double zf1 = ((double)(jmin1 - 2) - zint) * zslope;
         double zf2 = ((double)(jmin2 - 2) - zint * conf2) * zslope;
         double zf3 = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+        nev += (unsigned long long)(jmax1 - jmin1 + 1);
         for (int j1 = jmin1; j1 <= jmax1; j1++) {
             int j2 = j1 - j2shft, j3 = j1 - j3shft;
             zf3 = zf3 + zslope;
@@ -350,6 +352,8 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
             if (j1 >= 1) atomicAdd(&a1[j1], s1 * T.f1[iz1] + sd * T.fg[iz1] + sv * T.xver[iz1]);
         }
     }
+    for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
+    if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work, nev);
     __syncthreads();
     double *dst = A + P.a_off;
     for (int t = threadIdx.x; t < stride; t += blockDim.x) dst[t] = sm[t];
@@ -767,7 +771,7 @@ __device__ inline void f4_candidates(const Job &J, const R4Blk &B, const Shrunk 
 }
 
 __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *blks, const int2 *tiles,
-                                                  Shrunk sh, double *R4) {
+                                                  Shrunk sh, double *R4, unsigned long long *work) {
     __shared__ F4Line L[256];
     int2 t = tiles[blockIdx.x];
     const R4Blk &B = blks[t.x];
@@ -775,6 +779,7 @@ __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *
     int j0 = t.y * 256 + 1, j = j0 + threadIdx.x;
     int jl = min(j0 + 255, B.nptr4);
     double acc = 0.;
+    unsigned long long nev = 0;
     if (j0 <= B.nptr4) {
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, jl, &c0, &c1);
@@ -791,6 +796,7 @@ __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *
                     double f = f4_value<double>(l, xm);
                     if (!(l.flags & 2)) f = f + xm * l.dptrat * f;
                     acc = acc + f;
+                    nev++;
                 }
                 if (j >= l.jminm && j <= l.jmaxm) {
                     double xm = xj - l.xnum;
@@ -803,6 +809,8 @@ __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *
         }
     }
     if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? acc : 0.;
+    for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
+    if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work + 1, nev);
 }
 
 // LBLF4Q radiation term on R4 when JRAD = 1 (clblm_ODLAY_LineF4.f90:142-177).
@@ -859,7 +867,7 @@ __device__ inline double xint_point(double v1a, double v2a, double dva, const do
 template <typename TA>
 __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *panels, const R4Blk *blks,
                                                const TA *A, const double *R4, const double *absrb,
-                                               double *out) {
+                                               double *out, double *nat) {
     __shared__ double r3[168], r3e[8], r2[640], r1[2448], fac[2448];
     const int64_t q = blockIdx.x;
     const Panel &P = panels[q];
@@ -956,17 +964,41 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
         for (int i = NLO + tid; i <= NHI; i += blockDim.x) r1[i] = r1[i] * fac[i];
         __syncthreads();
     }
-    double *o = out + P.out_off - NLO;
+    double *o = (J.dvout > 0. ? nat : out) + P.out_off - NLO;
     for (int i = NLO + tid; i <= NHI; i += blockDim.x) o[i] = r1[i];
 }
 
-// interpSpectrum_array2array (clblm_Spectrum.f90:326-590), one lane per output point.
-__global__ void k_regrid(const Job *jobs, const int *lay_of_block, const int64_t *blk_first,
-                         const double *nat, double *out) {
-    int l = lay_of_block[blockIdx.x];
+// interpSpectrum_array2array (clblm_Spectrum.f90:326-590).  For the n/(n+1) ratios
+// every output point is independent (one lane per point).  The general case walks
+// blocks of 2400 output points with a sequentially accumulated fraction P, whose
+// rounding the reference's quantised coefficient index depends on; one lane walks
+// one block.
+__device__ inline int regrid_type(double DVin, double DVout, double *ratio, int *lenIn, int *lenOut) {
+    const double TOL = 1.e-8;
+    double dvRatio = DVout / DVin;
+    if (fabs(DVout - DVin) < TOL) dvRatio = 1.;
+    const double hi2lo[5] = {2. / 1., 3. / 2., 4. / 3., 5. / 4., 6. / 5.};
+    const double lo2hi[5] = {1. / 2., 2. / 3., 3. / 4., 4. / 5., 5. / 6.};
+    bool anyhi = false, anylo = false;
+    for (int k = 0; k < 5; k++) {
+        if (fabs(dvRatio - hi2lo[k]) < TOL) anyhi = true;
+        if (fabs(dvRatio - lo2hi[k]) < TOL) anylo = true;
+    }
+    *ratio = dvRatio;
+    if (fabs(dvRatio - 1.) < 2.220446049250313e-16) { *lenIn = 1; *lenOut = 1; return 0; }
+    if (anyhi) { *lenIn = (int)(DVout / (DVout - DVin) + 0.5); *lenOut = *lenIn - 1; return *lenOut - 1; }
+    if (anylo) { *lenIn = (int)(DVout / (DVin - DVout) + 0.5); *lenOut = *lenIn + 1; return *lenOut - 1; }
+    *lenIn = 0; *lenOut = 0;
+    return -1;
+}
+
+__global__ void k_regrid(const Job *jobs, const int *lay_of_unit, const int64_t *unit_first,
+                         const double *nat, double *out, int64_t nunits) {
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    int l = lay_of_unit[u];
     const Job &J = jobs[l];
-    int64_t jo = (int64_t)(blockIdx.x - blk_first[l]) * blockDim.x + threadIdx.x;  // 0-based output
-    if (jo >= J.npts) return;
+    int64_t ul = u - unit_first[l];
     const double *X = nat + J.nat_off - 1;  // X[1..nnat]
     const int64_t NIN = J.nnat;
     auto ext = [&](int64_t i) -> double {  // spInExt
@@ -976,35 +1008,23 @@ __global__ void k_regrid(const Job *jobs, const int *lay_of_block, const int64_t
         return 0.;
     };
     const double DVin = J.dv, DVout = J.dvout, V1 = J.v1;
-    const double TOL = 1.e-8;
-    double dvRatio = DVout / DVin;
-    if (fabs(DVout - DVin) < TOL) dvRatio = 1.;
-    const double XNUMCF = 201.;
-    int64_t I = jo + 1;
-    // general case (no aligned grids: the output and native grids share V1 only when the
-    // ratio is one of the n/(n+1) forms, handled below)
-    const double hi2lo[5] = {2. / 1., 3. / 2., 4. / 3., 5. / 4., 6. / 5.};
-    const double lo2hi[5] = {1. / 2., 2. / 3., 3. / 4., 4. / 5., 5. / 6.};
-    bool anyhi = false, anylo = false;
-    for (int k = 0; k < 5; k++) {
-        if (fabs(dvRatio - hi2lo[k]) < TOL) anyhi = true;
-        if (fabs(dvRatio - lo2hi[k]) < TOL) anylo = true;
-    }
-    int lenIn = 0, lenOut = 0, ITYPE = -1;
-    if (fabs(dvRatio - 1.) < 2.220446049250313e-16) { lenIn = 1; lenOut = 1; ITYPE = 0; }
-    else if (anyhi) { lenIn = (int)(DVout / (DVout - DVin) + 0.5); lenOut = lenIn - 1; ITYPE = lenOut - 1; }
-    else if (anylo) { lenIn = (int)(DVout / (DVin - DVout) + 0.5); lenOut = lenIn + 1; ITYPE = lenOut - 1; }
-    double res;
+    double dvRatio;
+    int lenIn, lenOut;
+    int ITYPE = regrid_type(DVin, DVout, &dvRatio, &lenIn, &lenOut);
+    double *o = out + J.out_off - 1;  // o[1..npts]
     if (ITYPE == 0) {
+        int64_t I = ul + 1;
         int64_t step = (int64_t)llround(dvRatio);
-        res = ext(1 - step + I * step);
+        o[I] = ext(1 - step + I * step);
     } else if (ITYPE > 0) {
-        int64_t nl = (I - 1) % lenOut + 1, m = (I - 1) / lenOut;  // NL-th interleaved sequence, m-th term
+        int64_t I = ul + 1;
+        int64_t nl = (I - 1) % lenOut + 1, m = (I - 1) / lenOut;
         int64_t iIn = nl - lenIn;
         if (nl > 1 && DVin > DVout) iIn = iIn - 1;
         iIn += (m + 1) * lenIn;
-        if (nl == 1) res = ext(iIn);
-        else {
+        if (nl == 1) {
+            o[I] = ext(iIn);
+        } else {
             double AP = 1.0 / ((double)ITYPE + 1.0);
             double pp = AP * (double)(nl - 1);
             if (DVin > DVout) pp = 1.0 - pp;
@@ -1012,27 +1032,32 @@ __global__ void k_regrid(const Job *jobs, const int *lay_of_block, const int64_t
             double B2 = (pp * pp - 1.0) * (pp - 2.0) * 0.5;
             double B3 = -pp * (pp + 1.0) * (pp - 2.0) * 0.5;
             double B4 = pp * (pp * pp - 1.0) / 6.0;
-            res = B1 * ext(iIn - 1) + B2 * ext(iIn) + B3 * ext(iIn + 1) + B4 * ext(iIn + 2);
+            o[I] = B1 * ext(iIn - 1) + B2 * ext(iIn) + B3 * ext(iIn + 1) + B4 * ext(iIn + 2);
         }
     } else {
-        int64_t J1 = ((I - 1) / 2400) * 2400 + 1;
-        double Vo = J1 == 1 ? V1 : V1 + (double)(J1 - 1) * DVout;
+        const double XNUMCF = 201.;
+        int64_t J1 = ul * 2400 + 1;
+        double Vo = ul == 0 ? V1 : V1 + (double)(J1 - 1) * DVout;
         int64_t iIn = (int64_t)((Vo - V1) / DVin + 1.00001);
         double Vi = V1 + (double)(iIn - 1) * DVin;
         double P = (Vo - Vi) / DVin;
-        double Pk = P + (double)(I - J1) * dvRatio;  // P after I-J1 steps of the block walk
-        double fl = floor(Pk);
-        if (fl >= 1.0) { iIn += (int64_t)fl; Pk = Pk - fl; }
-        int ip = (int)(Pk * XNUMCF + 1.00001);
-        double p = ((double)ip - 1.0) / (XNUMCF - 1.0);
-        double pp = p * p;
-        double A1 = -p / 2.0 * ((1 - p) * (1 - p));
-        double A2 = 1.0 - pp * (3.0 - 2.0 * p) + pp / 2.0 * (1.0 - p);
-        double A3 = pp * (3.0 - 2.0 * p) + p / 2.0 * ((1.0 - p) * (1.0 - p));
-        double A4 = -pp / 2 * (1.0 - p);
-        res = A1 * ext(iIn - 1) + A2 * ext(iIn) + A3 * ext(iIn + 1) + A4 * ext(iIn + 2);
+        int64_t J2 = J1 + 2400 - 1 < J.npts ? J1 + 2400 - 1 : J.npts;
+        for (int64_t j = J1; j <= J2; j++) {
+            int ip = (int)(P * XNUMCF + 1.00001);
+            double p = ((double)ip - 1.0) / (XNUMCF - 1.0);
+            double pp = p * p;
+            double A1 = -p / 2.0 * ((1 - p) * (1 - p));
+            double A2 = 1.0 - pp * (3.0 - 2.0 * p) + pp / 2.0 * (1.0 - p);
+            double A3 = pp * (3.0 - 2.0 * p) + p / 2.0 * ((1.0 - p) * (1.0 - p));
+            double A4 = -pp / 2 * (1.0 - p);
+            o[j] = A1 * ext(iIn - 1) + A2 * ext(iIn) + A3 * ext(iIn + 1) + A4 * ext(iIn + 2);
+            P = P + dvRatio;
+            if (P >= 1.0) {
+                iIn = iIn + (int64_t)P;
+                P = P - (double)(int64_t)P;
+            }
+        }
     }
-    out[J.out_off + jo] = res;
 }
 
 // ====================================================================================
@@ -1169,7 +1194,9 @@ struct clblm_ctx {
     std::vector<LineListH *> lists;
     DevArena arena;
     int *d_err = nullptr;
-    hipEvent_t ev[6]{};
+    unsigned long long *d_work = nullptr;   // [conv evaluations, F4 evaluations]
+    hipEvent_t ev[8]{};
+    double ms_conv_k = 0, ms_f4_k = 0;
     double ms[4] = {0, 0, 0, 0};
     int64_t launches = 0;
     std::mutex mu;
@@ -1470,14 +1497,23 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_nat = cv.take<double>(std::max<int64_t>(1, B.tot_nat));
     size_t o_rg = cv.take<int>(1);
     size_t o_rgf = cv.take<int64_t>(nlay);
-    // regrid block map
+    // re-grid work units: points for the n/(n+1) ratios, 2400-point blocks otherwise
     std::vector<int> rg_lay;
     std::vector<int64_t> rg_first(nlay, 0);
     if (B.any_regrid) {
         for (int l = 0; l < nlay; l++) {
             rg_first[l] = (int64_t)rg_lay.size();
-            if (B.jobs[l].dvout > 0.)
-                for (unsigned b = 0; b < cdiv(B.jobs[l].npts, 256); b++) rg_lay.push_back(l);
+            const Job &Jl = B.jobs[l];
+            if (!(Jl.dvout > 0.)) continue;
+            double ratio = Jl.dvout / Jl.dv;
+            if (std::fabs(Jl.dvout - Jl.dv) < 1e-8) ratio = 1.;
+            bool aligned = std::fabs(ratio - 1.) < 2.220446049250313e-16;
+            for (int k = 0; k < 5; k++) {
+                double a = (k + 2.) / (k + 1.), b = (k + 1.) / (k + 2.);
+                if (std::fabs(ratio - a) < 1e-8 || std::fabs(ratio - b) < 1e-8) aligned = true;
+            }
+            int64_t units = aligned ? Jl.npts : (Jl.npts + 2399) / 2400;
+            for (int64_t u = 0; u < units; u++) rg_lay.push_back(l);
         }
         o_rg = cv.take<int>(std::max<size_t>(1, rg_lay.size()));
         o_rgf = cv.take<int64_t>(nlay);
@@ -1535,7 +1571,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     hipLaunchKernelGGL(k_ranges, dim3(cdiv(npan, 256)), dim3(256), 0, st, d_jobs, d_pan, npan, lp);
     HIPCHK(hipEventRecord(C->ev[1], st));
     hipLaunchKernelGGL(k_conv64, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
-                       d_jobs, d_pan, lp, T, d_A);
+                       d_jobs, d_pan, lp, T, d_A, C->d_work);
     HIPCHK(hipEventRecord(C->ev[2], st));
     if (B.max_f4 > 0) {
         hipLaunchKernelGGL(k_f4prep, dim3(cdiv(B.max_f4, 256), nlay), dim3(256), 0, st, d_jobs, L.dev, f, C->d_err);
@@ -1549,29 +1585,22 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, st));
     }
     hipLaunchKernelGGL(k_r4range, dim3(cdiv(nblk, 256)), dim3(256), 0, st, d_jobs, d_blk, nblk, f, sh);
-    hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh, d_R4);
+    HIPCHK(hipEventRecord(C->ev[5], st));
+    hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh, d_R4,
+                       C->d_work);
+    HIPCHK(hipEventRecord(C->ev[6], st));
     if (B.any_jrad1) hipLaunchKernelGGL(k_r4rad, dim3(cdiv(nblk, 64)), dim3(64), 0, st, d_jobs, d_blk, nblk, d_R4);
     HIPCHK(hipEventRecord(C->ev[3], st));
-    double *pass2_out = d_out;
-    if (B.any_regrid) pass2_out = nullptr;
-    // layers without re-grid write straight to d_out; re-gridded ones go through d_nat
-    // (panel out_off already points into the right buffer per layer)
-    if (!B.any_regrid) {
-        hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
-                           d_abs, d_out);
-    } else {
-        // split: every panel writes through its out_off, which is relative to d_out for
-        // native layers and to d_nat for re-gridded ones; run twice with masks by layer
-        hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
-                           d_abs, d_nat);
+    hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
+                       d_abs, d_out, d_nat);
+    if (B.any_regrid) {
         int *d_rg = at<int>(base, o_rg);
         int64_t *d_rgf = at<int64_t>(base, o_rgf);
         HIPCHK(hipMemcpyAsync(d_rg, rg_lay.data(), sizeof(int) * rg_lay.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_rgf, rg_first.data(), sizeof(int64_t) * nlay, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_regrid, dim3((unsigned)rg_lay.size()), dim3(256), 0, st, d_jobs, d_rg, d_rgf, d_nat,
-                           d_out);
+        hipLaunchKernelGGL(k_regrid, dim3(cdiv((int64_t)rg_lay.size(), 64)), dim3(64), 0, st, d_jobs, d_rg, d_rgf,
+                           d_nat, d_out, (int64_t)rg_lay.size());
     }
-    (void)pass2_out;
     HIPCHK(hipEventRecord(C->ev[4], st));
     HIPCHK(hipGetLastError());
     int herr = 0;
@@ -1584,7 +1613,11 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     HIPCHK(hipEventElapsedTime(&t12, C->ev[1], C->ev[2]));
     HIPCHK(hipEventElapsedTime(&t23, C->ev[2], C->ev[3]));
     HIPCHK(hipEventElapsedTime(&t34, C->ev[3], C->ev[4]));
+    float t56;
+    HIPCHK(hipEventElapsedTime(&t56, C->ev[5], C->ev[6]));
     C->ms[0] += t01; C->ms[1] += t12; C->ms[2] += t23; C->ms[3] += t34;
+    C->ms_conv_k += t12;
+    C->ms_f4_k += t56;
     C->launches++;
     (void)precision;
     return 0;
@@ -1608,6 +1641,8 @@ int clblm_ctx_create(int device, clblm_ctx **out) {
     c->device = device;
     HIPCHK(hipStreamCreate(&c->stream));
     HIPCHK(hipMalloc(&c->d_err, sizeof(int)));
+    HIPCHK(hipMalloc(&c->d_work, 2 * sizeof(unsigned long long)));
+    HIPCHK(hipMemset(c->d_work, 0, 2 * sizeof(unsigned long long)));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
     int rc = ctx_tables(c);
     if (rc) { delete c; return rc; }
@@ -1624,6 +1659,7 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     (void)hipFree(c->tab);
     (void)hipFree(c->tabs);
     (void)hipFree(c->d_err);
+    (void)hipFree(c->d_work);
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1744,6 +1780,25 @@ int clblm_timing(clblm_ctx *c, int reset, double *ms_prep, double *ms_conv, doub
     if (ms_panel) *ms_panel = c->ms[3] / n;
     if (launches) *launches = c->launches;
     if (reset) { c->ms[0] = c->ms[1] = c->ms[2] = c->ms[3] = 0; c->launches = 0; }
+    return 0;
+}
+
+int clblm_work(clblm_ctx *c, int reset, int64_t *conv_evals, int64_t *f4_evals, double *ms_conv_kernel,
+               double *ms_f4_kernel) {
+    if (!c) return set_err(CLBLM_ERR_ARG, "ctx is NULL");
+    unsigned long long w[2];
+    HIPCHK(hipMemcpy(w, c->d_work, sizeof w, hipMemcpyDeviceToHost));
+    int64_t n = c->launches > 0 ? c->launches : 1;
+    if (conv_evals) *conv_evals = (int64_t)(w[0] / (unsigned long long)n);
+    if (f4_evals) *f4_evals = (int64_t)(w[1] / (unsigned long long)n);
+    if (ms_conv_kernel) *ms_conv_kernel = c->ms_conv_k / n;
+    if (ms_f4_kernel) *ms_f4_kernel = c->ms_f4_k / n;
+    if (reset) {
+        HIPCHK(hipMemset(c->d_work, 0, sizeof w));
+        c->ms_conv_k = c->ms_f4_k = 0;
+        c->ms[0] = c->ms[1] = c->ms[2] = c->ms[3] = 0;
+        c->launches = 0;
+    }
     return 0;
 }
 

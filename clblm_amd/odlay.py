@@ -115,6 +115,13 @@ class Context:
         self.jrad = jr.value
         return out, off
 
+    def work(self, reset=False):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        c, d = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.clblm_work(self.ptr, int(reset), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                  ctypes.byref(d)))
+        return dict(conv_evals=a.value, f4_evals=b.value, ms_conv=c.value, ms_f4=d.value)
+
     def timing(self, reset=False):
         vals = [ctypes.c_double() for _ in range(4)]
         n = ctypes.c_int64()

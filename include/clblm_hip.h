@@ -115,6 +115,13 @@ int clblm_rt_up(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv,
 int clblm_timing(clblm_ctx *ctx, int reset, double *ms_prep, double *ms_conv, double *ms_f4,
                  double *ms_panel, int64_t *launches);
 
+/* Work done per launch, averaged since the last reset: line-shape evaluations of
+ * the R1/R2/R3 convolution (points of each line's R1 support) and of the F4 wing
+ * sum (line, R4 point) pairs, with the average device time of those two kernels.
+ * reset != 0 also clears clblm_timing's accumulators. */
+int clblm_work(clblm_ctx *ctx, int reset, int64_t *conv_evals, int64_t *f4_evals, double *ms_conv_kernel,
+               double *ms_f4_kernel);
+
 #ifdef __cplusplus
 }
 #endif
