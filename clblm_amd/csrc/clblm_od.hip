@@ -360,6 +360,89 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
 }
 
 // ------------------------------------------------------------------------------------
+// Pass 1, FP32-compensated: same walk and the same FP64 table argument ZF (so the
+// table indices are the reference's), FP32 tables in LDS, FP32 line-shape products,
+// and a compensated LDS sum: every atomic add returns the old value, the lane
+// recovers the rounding error of that add exactly (TwoSum) and adds it to a second
+// accumulator.  The panel arrays leave as (hi, lo) FP32 pairs.
+// ------------------------------------------------------------------------------------
+constexpr int TABN = 2512;  // table entries staged in LDS (R2/R3 indices reach 2501)
+
+__device__ __forceinline__ void csum_add(float *hi, float *lo, float v) {
+    float old = atomicAdd(hi, v);
+    float s = old + v;
+    float bb = s - old;
+    float err = (old - (s - bb)) + (v - bb);
+    if (err != 0.f) atomicAdd(lo, err);
+}
+
+__global__ void __launch_bounds__(512) k_conv32(const Job *jobs, const Panel *panels, LinePrep lp, Tables T,
+                                                float *A, unsigned long long *work) {
+    extern __shared__ float smf[];
+    const Panel &P = panels[blockIdx.x];
+    const Job &J = jobs[P.layer];
+    const int max1 = J.max1, max2 = J.max2, max3 = J.max3, stride = J.stride;
+    float *hi = smf, *lo = smf + stride;
+    float *t1 = smf + 2 * stride, *tg = t1 + TABN, *tv = tg + TABN, *t2 = tv + TABN, *t3 = t2 + TABN;
+    for (int t = threadIdx.x; t < 2 * stride; t += blockDim.x) smf[t] = 0.f;
+    for (int t = threadIdx.x; t < TABN; t += blockDim.x) {
+        t1[t] = T.f1s[t]; tg[t] = T.fgs[t]; tv[t] = T.xvers[t]; t2[t] = T.f2s[t]; t3[t] = T.f3s[t];
+    }
+    __syncthreads();
+    float *h1 = hi - 1, *h2 = hi + max1 - 1, *h3 = hi + max1 + max2 - 1;
+    float *l1 = lo - 1, *l2 = lo + max1 - 1, *l3 = lo + max1 + max2 - 1;
+    const double DXF1 = 0.002, HWF1 = 4.;
+    const double DV = J.dv;
+    const double wavdxf = DV / DXF1, hwdxf = HWF1 / DXF1;
+    const double conf2 = DV / J.dvr2, conf3 = DV / J.dvr3;
+    const double vft = P.vft;
+    unsigned long long nev = 0;
+    for (int64_t o = P.lstart + threadIdx.x; o < P.lend; o += blockDim.x) {
+        double sp = lp.sp[o];
+        if (sp == 0.) continue;
+        double recalf = lp.recalf[o];
+        double depth = sp * recalf;
+        int izm = lp.iz[o];
+        double zetdif = 100. * lp.zeta[o] - (double)(izm - 1);
+        float s1 = (float)(depth * avrat_lerp(c_cf1, izm, zetdif));
+        float s2 = (float)(depth * avrat_lerp(c_cf2, izm, zetdif));
+        float s3 = (float)(depth * avrat_lerp(c_cf3, izm, zetdif));
+        float sd = (float)(depth * avrat_lerp(c_cgauss, izm, zetdif));
+        float sv = (float)(depth * avrat_lerp(c_cer, izm, zetdif));
+        double zslope = recalf * wavdxf;
+        double zint = (lp.vnu[o] - vft) / DV;
+        double bhwdxf = hwdxf / zslope;
+        int jmax1 = ftrunc(zint + bhwdxf + 1.5);
+        int jmin1 = ftrunc(zint - bhwdxf + 1.5);
+        double rshft = zint < 0.0 ? -0.5 : 0.5;
+        int j2shft = ftrunc(zint * (1. - conf2) + rshft);
+        int j3shft = ftrunc(zint * (1. - conf3) + rshft);
+        int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
+        double zf1 = ((double)(jmin1 - 2) - zint) * zslope;
+        double zf2 = ((double)(jmin2 - 2) - zint * conf2) * zslope;
+        double zf3 = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+        nev += (unsigned long long)(jmax1 - jmin1 + 1);
+        for (int j1 = jmin1; j1 <= jmax1; j1++) {
+            int j2 = j1 - j2shft, j3 = j1 - j3shft;
+            zf3 = zf3 + zslope;
+            zf2 = zf2 + zslope;
+            zf1 = zf1 + zslope;
+            int iz3 = ftrunc(fabs(zf3) + 1.5);
+            int iz2 = ftrunc(fabs(zf2) + 1.5);
+            int iz1 = ftrunc(fabs(zf1) + 1.5);
+            if (j3 >= 1 && j3 <= max3) csum_add(&h3[j3], &l3[j3], s3 * t3[iz3]);
+            if (j2 >= 1 && j2 <= max2) csum_add(&h2[j2], &l2[j2], s2 * t2[iz2]);
+            if (j1 >= 1) csum_add(&h1[j1], &l1[j1], __fmaf_rn(sv, tv[iz1], __fmaf_rn(sd, tg[iz1], s1 * t1[iz1])));
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
+    if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work, nev);
+    __syncthreads();
+    float *dst = A + 2 * P.a_off;
+    for (int t = threadIdx.x; t < 2 * stride; t += blockDim.x) dst[t] = smf[t];
+}
+
+// ------------------------------------------------------------------------------------
 // F4 far wings.  LINF4Q per-line physics (clblm_ODLAY_LineF4.f90:951-1170).
 // ------------------------------------------------------------------------------------
 struct F4Prep {
@@ -813,6 +896,92 @@ __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *
     if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work + 1, nev);
 }
 
+// CNVF4Q in FP32 with FP64 offsets: per (line, block) the grid point nearest the line
+// centre and the FP64 offset to it; per lane the offset is rebuilt in FP32 from the
+// integer distance, so it keeps ~1e-7 relative accuracy at every distance.  Wing
+// values in FP32 (reciprocal for the Lorentz branch), Kahan sum per R4 point.
+struct F4LineF {
+    float d, dm, sil, siv_a3, siv_b3, f4bnd, alfli2, rec_alfvi2, dptrat;
+    int jc, jcm, jmin, jmax, jminm, jmaxm, flags;
+};
+
+__device__ __forceinline__ float f4_value_f(const F4LineF &L, float xm) {
+    const float ZSQBND = 4096.f;
+    const float r_bndsq = 1.f / 625.f;
+    float xmsq = xm * xm;
+    float zvsq = xmsq * L.rec_alfvi2;
+    float base = (L.flags & 1) ? (2.f - xmsq * r_bndsq) * L.f4bnd : L.f4bnd;
+    float f = (zvsq <= ZSQBND) ? __fmaf_rn(zvsq, L.siv_b3, L.siv_a3)
+                                : L.sil * __builtin_amdgcn_rcpf(L.alfli2 + xmsq);
+    return f - base;
+}
+
+__global__ void __launch_bounds__(256) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles,
+                                                  Shrunk sh, double *R4, unsigned long long *work) {
+    __shared__ F4LineF L[256];
+    int2 t = tiles[blockIdx.x];
+    const R4Blk &B = blks[t.x];
+    const Job &J = jobs[B.layer];
+    int j0 = t.y * 256 + 1, j = j0 + threadIdx.x;
+    int jl = min(j0 + 255, B.nptr4);
+    float hi = 0.f, lo = 0.f;
+    unsigned long long nev = 0;
+    const float dvr4f = (float)J.dvr4;
+    if (j0 <= B.nptr4) {
+        int64_t c0, c1;
+        f4_candidates(J, B, sh, j0, jl, &c0, &c1);
+        for (int64_t base = c0; base < c1; base += 256) {
+            int64_t i = base + threadIdx.x;
+            if (i < c1) {
+                F4Line d;
+                f4_line_setup(d, J, B, sh.vnu[i], sh.sp[i], sh.spp[i], sh.alfa[i], sh.epp[i], sh.co2[i]);
+                F4LineF &o = L[threadIdx.x];
+                o.jmin = d.jmin; o.jmax = d.jmax; o.jminm = d.jminm; o.jmaxm = d.jmaxm; o.flags = d.flags;
+                o.sil = (float)d.sil; o.siv_a3 = (float)d.siv_a3; o.siv_b3 = (float)d.siv_b3;
+                o.f4bnd = (float)d.f4bnd; o.alfli2 = (float)d.alfli2; o.rec_alfvi2 = (float)d.rec_alfvi2;
+                o.dptrat = (float)d.dptrat;
+                // nearest grid point and FP64 offset: x(j) = (j-1)*dvr4 - xnui = (j-jc)*dvr4 + d
+                double q = d.xnui / J.dvr4;
+                int jc = (int)floor(q + 0.5) + 1;
+                o.jc = jc;
+                o.d = (float)((double)(jc - 1) * J.dvr4 - d.xnui);
+                double qm = d.xnum / J.dvr4;
+                int jcm = d.jmaxm >= d.jminm ? (int)floor(qm + 0.5) + 1 : 0;
+                o.jcm = jcm;
+                o.dm = d.jmaxm >= d.jminm ? (float)((double)(jcm - 1) * J.dvr4 - d.xnum) : 0.f;
+            }
+            __syncthreads();
+            int nb = (int)min((int64_t)256, c1 - base);
+            for (int k = 0; k < nb; k++) {
+                const F4LineF &l = L[k];
+                if (j >= l.jmin && j <= l.jmax) {
+                    float xm = __fmaf_rn((float)(j - l.jc), dvr4f, l.d);
+                    float f = f4_value_f(l, xm);
+                    if (!(l.flags & 2)) f = __fmaf_rn(xm * l.dptrat, f, f);
+                    float y = f - lo;
+                    float tt = hi + y;
+                    lo = (tt - hi) - y;
+                    hi = tt;
+                    nev++;
+                }
+                if (j >= l.jminm && j <= l.jmaxm) {
+                    float xm = __fmaf_rn((float)(j - l.jcm), dvr4f, l.dm);
+                    float f = f4_value_f(l, xm);
+                    if (!(l.flags & 2)) f = __fmaf_rn(xm * (-l.dptrat), f, f);
+                    float y = f - lo;
+                    float tt = hi + y;
+                    lo = (tt - hi) - y;
+                    hi = tt;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? (double)hi - (double)lo : 0.;
+    for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
+    if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work + 1, nev);
+}
+
 // LBLF4Q radiation term on R4 when JRAD = 1 (clblm_ODLAY_LineF4.f90:142-177).
 __global__ void k_r4rad(const Job *jobs, const R4Blk *blks, int64_t nblk, double *R4) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -844,6 +1013,28 @@ __global__ void k_r4rad(const Job *jobs, const R4Blk *blks, int64_t nblk, double
 // points 8..12 hold values the previous panel produced; they are rebuilt from that
 // panel's lines and F4 block only.
 // ------------------------------------------------------------------------------------
+template <typename TA>
+struct PanelArr;
+template <>
+struct PanelArr<double> {  // FP64: one array of stride doubles per panel
+    const double *p;
+    int m1, m2;
+    __device__ PanelArr(const double *A, int64_t off, int max1, int max2, int) : p(A + off - 1), m1(max1), m2(max2) {}
+    __device__ double r1(int j) const { return p[j]; }
+    __device__ double r2(int j) const { return p[m1 + j]; }
+    __device__ double r3(int j) const { return p[m1 + m2 + j]; }
+};
+template <>
+struct PanelArr<float> {  // FP32-compensated: (hi, lo) FP32 arrays, summed in FP64
+    const float *h, *l;
+    int m1, m2;
+    __device__ PanelArr(const float *A, int64_t off, int max1, int max2, int stride)
+        : h(A + 2 * off - 1), l(A + 2 * off + stride - 1), m1(max1), m2(max2) {}
+    __device__ double r1(int j) const { return (double)h[j] + (double)l[j]; }
+    __device__ double r2(int j) const { return (double)h[m1 + j] + (double)l[m1 + j]; }
+    __device__ double r3(int j) const { return (double)h[m1 + m2 + j] + (double)l[m1 + m2 + j]; }
+};
+
 __device__ inline double xint_point(double v1a, double v2a, double dva, const double *A /*1-based*/,
                                     double vft, double dvr3, int I, int n1r3, int n2r3) {
     double recdva = 1. / dva;
@@ -876,8 +1067,13 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
     const bool prev = p > 0;
     const Panel &PP = panels[prev ? q - 1 : q];
     const int max1 = J.max1, max2 = J.max2, max3 = J.max3;
-    const TA *a1 = A + P.a_off - 1, *a2 = a1 + max1, *a3 = a2 + max2;
-    const TA *b1 = A + PP.a_off - 1, *b2 = b1 + max1, *b3 = b2 + max2;
+    const PanelArr<TA> pa(A, P.a_off, max1, max2, J.stride), pb(A, PP.a_off, max1, max2, J.stride);
+    auto a1 = [&](int j) { return pa.r1(j); };
+    auto a2 = [&](int j) { return pa.r2(j); };
+    auto a3 = [&](int j) { return pa.r3(j); };
+    auto b1 = [&](int j) { return pb.r1(j); };
+    auto b2 = [&](int j) { return pb.r2(j); };
+    auto b3 = [&](int j) { return pb.r3(j); };
     const R4Blk &B = blks[P.blk];
     const R4Blk &BP = blks[PP.blk];
     const int N1R3 = prev ? 6 : 1, N1R3P = (prev && p > 1) ? 6 : 1, N2R3 = 155;
@@ -889,7 +1085,7 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
     const int NLO = NSHIFT + 1, NHI = P.nhi;
     // R3
     for (int j = 1 + tid; j <= 160; j += blockDim.x) {
-        double v = (j <= max3 ? (double)a3[j] : 0.) + (prev && j + 150 <= max3 ? (double)b3[j + 150] : 0.);
+        double v = (j <= max3 ? a3(j) : 0.) + (prev && j + 150 <= max3 ? b3(j + 150) : 0.);
         if (j >= N1R3 && j <= N2R3) {
             v = v + xint_point(B.v1r4, B.v2r4, J.dvr4, r4, P.vft, DVR3, j, N1R3, N2R3);
             if (ab) v = v + xint_point(J.v1abs, v2abs, 1., ab, P.vft, DVR3, j, N1R3, N2R3);
@@ -897,13 +1093,13 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
         if (prev && j <= 5) {
             double xp = xint_point(BP.v1r4, BP.v2r4, J.dvr4, r4p, PP.vft, DVR3, j + 150, N1R3P, N2R3);
             if (ab) xp = xp + xint_point(J.v1abs, v2abs, 1., ab, PP.vft, DVR3, j + 150, N1R3P, N2R3);
-            r3e[j] = (j + 150 <= max3 ? (double)b3[j + 150] : 0.) + xp;
+            r3e[j] = (j + 150 <= max3 ? b3(j + 150) : 0.) + xp;
             v = v + xp;
         }
         r3[j] = v;
     }
     for (int j = 1 + tid; j < 640; j += blockDim.x)
-        r2[j] = (j <= max2 ? (double)a2[j] : 0.) + (prev && j + 600 <= max2 ? (double)b2[j + 600] : 0.);
+        r2[j] = (j <= max2 ? a2(j) : 0.) + (prev && j + 600 <= max2 ? b2(j + 600) : 0.);
     __syncthreads();
     const double X00 = -7. / 128., X01 = 105. / 128., X02 = 35. / 128., X03 = -5. / 128.;
     const double X10 = -1. / 16., X11 = 9. / 16.;
@@ -933,7 +1129,7 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
     }
     // R1
     for (int j = NLO + tid; j <= NHI + 3; j += blockDim.x)
-        r1[j] = (j <= max1 ? (double)a1[j] : 0.) + (prev && j + 2400 <= max1 ? (double)b1[j + 2400] : 0.);
+        r1[j] = (j <= max1 ? a1(j) : 0.) + (prev && j + 2400 <= max1 ? b1(j + 2400) : 0.);
     __syncthreads();
     for (int g = tid; NLO + 4 * g <= NHI; g += blockDim.x) {
         int Jg = NLO + 4 * g, J2 = (Jg - 1) / 4 + 1;
@@ -1570,8 +1766,15 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     }
     hipLaunchKernelGGL(k_ranges, dim3(cdiv(npan, 256)), dim3(256), 0, st, d_jobs, d_pan, npan, lp);
     HIPCHK(hipEventRecord(C->ev[1], st));
-    hipLaunchKernelGGL(k_conv64, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
-                       d_jobs, d_pan, lp, T, d_A, C->d_work);
+    if (precision == CLBLM_FP64) {
+        hipLaunchKernelGGL(k_conv64, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
+                           d_jobs, d_pan, lp, T, d_A, C->d_work);
+    } else {
+        size_t lds = (size_t)(2 * B.max_stride + 5 * TABN) * sizeof(float);
+        HIPCHK(hipFuncSetAttribute((const void *)k_conv32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_conv32, dim3((unsigned)npan), dim3(512), lds, st, d_jobs, d_pan, lp, T,
+                           reinterpret_cast<float *>(d_A), C->d_work);
+    }
     HIPCHK(hipEventRecord(C->ev[2], st));
     if (B.max_f4 > 0) {
         hipLaunchKernelGGL(k_f4prep, dim3(cdiv(B.max_f4, 256), nlay), dim3(256), 0, st, d_jobs, L.dev, f, C->d_err);
@@ -1586,13 +1789,21 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     }
     hipLaunchKernelGGL(k_r4range, dim3(cdiv(nblk, 256)), dim3(256), 0, st, d_jobs, d_blk, nblk, f, sh);
     HIPCHK(hipEventRecord(C->ev[5], st));
-    hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh, d_R4,
-                       C->d_work);
+    if (precision == CLBLM_FP64)
+        hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh,
+                           d_R4, C->d_work);
+    else
+        hipLaunchKernelGGL(k_f4conv32, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh,
+                           d_R4, C->d_work);
     HIPCHK(hipEventRecord(C->ev[6], st));
     if (B.any_jrad1) hipLaunchKernelGGL(k_r4rad, dim3(cdiv(nblk, 64)), dim3(64), 0, st, d_jobs, d_blk, nblk, d_R4);
     HIPCHK(hipEventRecord(C->ev[3], st));
-    hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
-                       d_abs, d_out, d_nat);
+    if (precision == CLBLM_FP64)
+        hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
+                           d_abs, d_out, d_nat);
+    else
+        hipLaunchKernelGGL(k_pass2<float>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk,
+                           reinterpret_cast<const float *>(d_A), d_R4, d_abs, d_out, d_nat);
     if (B.any_regrid) {
         int *d_rg = at<int>(base, o_rg);
         int64_t *d_rgf = at<int64_t>(base, o_rgf);
@@ -1619,7 +1830,6 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     C->ms_conv_k += t12;
     C->ms_f4_k += t56;
     C->launches++;
-    (void)precision;
     return 0;
 }
 
