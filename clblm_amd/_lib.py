@@ -45,6 +45,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ClblmError("libclblm_hip.so not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    # PyTorch-ROCm ships its own libamdhip64.so.7; loading it first makes the
+    # library resolve to that one runtime, so torch tensors and this library
+    # share devices, streams and memory.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     lib.clblm_last_error.restype = ctypes.c_char_p
     lib.clblm_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]

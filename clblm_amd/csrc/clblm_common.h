@@ -47,6 +47,7 @@ struct Job {
     double v1abs;
     int64_t absrb_off;
     int64_t r4_off;                              // start of this layer's R4 blocks
+    int64_t f4rec_off, f4rec_cap;                // per-(line, block) F4 records of this layer
     // grid of the returned spectrum (DVout re-grid)
     double dvout;
     int64_t nat_off;                             // native OD offset when re-gridding
@@ -66,6 +67,7 @@ struct R4Blk {
     int nptr4, layer;
     int64_t r4_off;         // offset of R4(1) in the R4 buffer (2502 slots per block)
     int64_t lo, hi;         // shrunk-line range read by RDLN4Q/CNVF4Q (absolute)
+    int64_t rec;            // first per-(line, block) F4 record of this block
 };
 
 }  // namespace clblm

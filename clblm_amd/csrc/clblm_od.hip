@@ -366,7 +366,6 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
 // recovers the rounding error of that add exactly (TwoSum) and adds it to a second
 // accumulator.  The panel arrays leave as (hi, lo) FP32 pairs.
 // ------------------------------------------------------------------------------------
-constexpr int TABN = 2512;  // table entries staged in LDS (R2/R3 indices reach 2501)
 
 __device__ __forceinline__ void csum_add(float *hi, float *lo, float v) {
     float old = atomicAdd(hi, v);
@@ -376,18 +375,16 @@ __device__ __forceinline__ void csum_add(float *hi, float *lo, float v) {
     if (err != 0.f) atomicAdd(lo, err);
 }
 
-__global__ void __launch_bounds__(512) k_conv32(const Job *jobs, const Panel *panels, LinePrep lp, Tables T,
+__global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *panels, LinePrep lp, Tables T,
                                                 float *A, unsigned long long *work) {
     extern __shared__ float smf[];
     const Panel &P = panels[blockIdx.x];
     const Job &J = jobs[P.layer];
     const int max1 = J.max1, max2 = J.max2, max3 = J.max3, stride = J.stride;
     float *hi = smf, *lo = smf + stride;
-    float *t1 = smf + 2 * stride, *tg = t1 + TABN, *tv = tg + TABN, *t2 = tv + TABN, *t3 = t2 + TABN;
+    const float *__restrict__ t1 = T.f1s, *__restrict__ tg = T.fgs, *__restrict__ tv = T.xvers;
+    const float *__restrict__ t2 = T.f2s, *__restrict__ t3 = T.f3s;
     for (int t = threadIdx.x; t < 2 * stride; t += blockDim.x) smf[t] = 0.f;
-    for (int t = threadIdx.x; t < TABN; t += blockDim.x) {
-        t1[t] = T.f1s[t]; tg[t] = T.fgs[t]; tv[t] = T.xvers[t]; t2[t] = T.f2s[t]; t3[t] = T.f3s[t];
-    }
     __syncthreads();
     float *h1 = hi - 1, *h2 = hi + max1 - 1, *h3 = hi + max1 + max2 - 1;
     float *l1 = lo - 1, *l2 = lo + max1 - 1, *l3 = lo + max1 + max2 - 1;
@@ -537,7 +534,7 @@ __device__ int shrinq_walk(const Job &J, const F4Prep &f, int64_t k0, int64_t ke
                            int lcap, int *gout, Shrunk sh, int64_t out0) {
     const double dvh = J.dvr4 / 2.;
     ShrinkSums a{0, 0, 0, 0, 0}, b{0, 0, 0, 0, 0};
-    int emitted = 0, lrec = 0;
+    int emitted = 0, lrec = 0, walked = 0;
     double vlmt = 0.;
     int64_t rec_end = rec_end0;  // relative index one past the current record
     int64_t cur = k0;
@@ -590,10 +587,14 @@ __device__ int shrinq_walk(const Job &J, const F4Prep &f, int64_t k0, int64_t ke
             int pend = emitted + (a.s > 0. ? 1 : 0) + (b.s > 0. ? 1 : 0);
             // record ends between this line and the next accepted one
             int64_t lim = nxt < kend ? nxt : kend;
-            while (lrec < lcap && rec_end <= lim) { gout[lrec++] = pend; rec_end += REC; }
+            bool crossed = false;
+            while (lrec < lcap && rec_end <= lim) { gout[lrec++] = pend; rec_end += REC; crossed = true; }
             if (nxt >= kend) {
                 while (lrec < lcap) { gout[lrec++] = pend; rec_end += REC; }
             }
+            // a chunk never needs more than LIMIN4 new lines: stop at the first record
+            // end after that many (the remaining entries are never read)
+            if (++walked >= LIMIN4 && crossed) return emitted;
         }
         cur = nxt;
     }
@@ -853,9 +854,89 @@ __device__ inline void f4_candidates(const Job &J, const R4Blk &B, const Shrunk 
     *c1 = b > a ? b : a;
 }
 
+// Per-(line, block) F4 records: the CNVF4Q per-line quantities and the block's
+// point range for the line, computed once; the gather kernels read them with
+// wave-uniform (scalar) loads.
+struct alignas(16) F4Rec32 {
+    float d, dm, sil, siv_a3, siv_b3, f4bnd, alfli2, rec_alfvi2, dptrat;
+    int jc, jcm, jmin, jmax, jminm, jmaxm, flags;
+};
+struct alignas(16) F4Rec64 {
+    double xnui, xnum, sil, siv_a3, siv_b3, f4bnd, alfli2, rec_alfvi2, dptrat;
+    int jmin, jmax, jminm, jmaxm, flags, pad0, pad1, pad2;
+};
+
+// Record offsets of each block (one lane per layer walks its blocks in order).
+__global__ void k_f4recoff(const Job *jobs, int nlay, R4Blk *blks, int *err) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlay) return;
+    const Job &J = jobs[l];
+    int64_t o = J.f4rec_off;
+    for (int b = 0; b < J.nblk; b++) {
+        R4Blk &B = blks[J.blk_off + b];
+        B.rec = o;
+        o += B.hi - B.lo;
+    }
+    if (o - J.f4rec_off > J.f4rec_cap) atomicOr(err, 8);
+}
+
+template <typename R>
+__device__ inline void f4_fill(R &r, const F4Line &d, const Job &J);
+template <>
+__device__ inline void f4_fill<F4Rec64>(F4Rec64 &r, const F4Line &d, const Job &) {
+    r.xnui = d.xnui; r.xnum = d.xnum; r.sil = d.sil; r.siv_a3 = d.siv_a3; r.siv_b3 = d.siv_b3;
+    r.f4bnd = d.f4bnd; r.alfli2 = d.alfli2; r.rec_alfvi2 = d.rec_alfvi2; r.dptrat = d.dptrat;
+    r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm; r.flags = d.flags;
+    r.pad0 = r.pad1 = r.pad2 = 0;
+}
+template <>
+__device__ inline void f4_fill<F4Rec32>(F4Rec32 &r, const F4Line &d, const Job &J) {
+    r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm; r.flags = d.flags;
+    r.sil = (float)d.sil; r.siv_a3 = (float)d.siv_a3; r.siv_b3 = (float)d.siv_b3; r.f4bnd = (float)d.f4bnd;
+    r.alfli2 = (float)d.alfli2; r.rec_alfvi2 = (float)d.rec_alfvi2; r.dptrat = (float)d.dptrat;
+    // nearest grid point and the FP64 offset to it: x(j) = (j-1)*dvr4 - xnui = (j-jc)*dvr4 + d
+    if (d.jmax >= d.jmin) {
+        int jc = (int)floor(d.xnui / J.dvr4 + 0.5) + 1;
+        r.jc = jc;
+        r.d = (float)((double)(jc - 1) * J.dvr4 - d.xnui);
+    } else { r.jc = 0; r.d = 0.f; }
+    if (d.jmaxm >= d.jminm) {
+        int jc = (int)floor(d.xnum / J.dvr4 + 0.5) + 1;
+        r.jcm = jc;
+        r.dm = (float)((double)(jc - 1) * J.dvr4 - d.xnum);
+    } else { r.jcm = 0; r.dm = 0.f; }
+}
+
+template <typename R>
+__global__ void k_f4setup(const Job *jobs, const R4Blk *blks, int64_t nblk, Shrunk sh, R *recs) {
+    int64_t b = blockIdx.y;
+    if (b >= nblk) return;
+    const R4Blk &B = blks[b];
+    const Job &J = jobs[B.layer];
+    for (int64_t i = B.lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < B.hi;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        F4Line d;
+        f4_line_setup(d, J, B, sh.vnu[i], sh.sp[i], sh.spp[i], sh.alfa[i], sh.epp[i], sh.co2[i]);
+        f4_fill<R>(recs[B.rec + (i - B.lo)], d, J);
+    }
+}
+
+__device__ __forceinline__ double f4_eval64(const F4Rec64 &l, double xm) {
+    const double ZSQBND = 4096.;
+    const double r_bndsq = 1. / 625.;
+    double xmsq = xm * xm;
+    double zvsq = xmsq * l.rec_alfvi2;
+    double base = (l.flags & 1) ? (2. - (xmsq * r_bndsq)) * l.f4bnd : l.f4bnd;
+    double f = (zvsq <= ZSQBND) ? (l.siv_a3 + zvsq * l.siv_b3) - base : l.sil / (l.alfli2 + xmsq) - base;
+    if (!(l.flags & 2)) f = f + xm * l.dptrat * f;
+    return f;
+}
+
+// FP64 gather: one lane per R4 point, lines in the reference's order (bit-for-bit
+// sums apart from the direct grid offset), every wave walks the tile's records.
 __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *blks, const int2 *tiles,
-                                                  Shrunk sh, double *R4, unsigned long long *work) {
-    __shared__ F4Line L[256];
+                                                  Shrunk sh, const F4Rec64 *__restrict__ recs, double *R4,
+                                                  unsigned long long *work) {
     int2 t = tiles[blockIdx.x];
     const R4Blk &B = blks[t.x];
     const Job &J = jobs[B.layer];
@@ -867,28 +948,18 @@ __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, jl, &c0, &c1);
         const double xj = (double)(j - 1) * J.dvr4;
-        for (int64_t base = c0; base < c1; base += 256) {
-            int64_t i = base + threadIdx.x;
-            if (i < c1) f4_line_setup(L[threadIdx.x], J, B, sh.vnu[i], sh.sp[i], sh.spp[i], sh.alfa[i], sh.epp[i], sh.co2[i]);
-            __syncthreads();
-            int nb = (int)min((int64_t)256, c1 - base);
-            for (int k = 0; k < nb; k++) {
-                const F4Line &l = L[k];
-                if (j >= l.jmin && j <= l.jmax) {
-                    double xm = xj - l.xnui;
-                    double f = f4_value<double>(l, xm);
-                    if (!(l.flags & 2)) f = f + xm * l.dptrat * f;
-                    acc = acc + f;
-                    nev++;
-                }
-                if (j >= l.jminm && j <= l.jmaxm) {
-                    double xm = xj - l.xnum;
-                    double f = f4_value<double>(l, xm);
-                    if (!(l.flags & 2)) f = f + xm * (-l.dptrat) * f;
-                    acc = acc + f;
-                }
+        const F4Rec64 *r = recs + B.rec - B.lo;
+        for (int64_t i = c0; i < c1; i++) {
+            const F4Rec64 &l = r[i];
+            if (j >= l.jmin && j <= l.jmax) {
+                acc = acc + f4_eval64(l, xj - l.xnui);
+                nev++;
             }
-            __syncthreads();
+            if (l.jmaxm >= l.jminm && j >= l.jminm && j <= l.jmaxm) {
+                F4Rec64 m = l;
+                m.dptrat = -l.dptrat;
+                acc = acc + f4_eval64(m, xj - l.xnum);
+            }
         }
     }
     if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? acc : 0.;
@@ -896,29 +967,23 @@ __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *
     if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work + 1, nev);
 }
 
-// CNVF4Q in FP32 with FP64 offsets: per (line, block) the grid point nearest the line
-// centre and the FP64 offset to it; per lane the offset is rebuilt in FP32 from the
-// integer distance, so it keeps ~1e-7 relative accuracy at every distance.  Wing
-// values in FP32 (reciprocal for the Lorentz branch), Kahan sum per R4 point.
-struct F4LineF {
-    float d, dm, sil, siv_a3, siv_b3, f4bnd, alfli2, rec_alfvi2, dptrat;
-    int jc, jcm, jmin, jmax, jminm, jmaxm, flags;
-};
-
-__device__ __forceinline__ float f4_value_f(const F4LineF &L, float xm) {
+__device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dptrat) {
     const float ZSQBND = 4096.f;
     const float r_bndsq = 1.f / 625.f;
     float xmsq = xm * xm;
-    float zvsq = xmsq * L.rec_alfvi2;
-    float base = (L.flags & 1) ? (2.f - xmsq * r_bndsq) * L.f4bnd : L.f4bnd;
-    float f = (zvsq <= ZSQBND) ? __fmaf_rn(zvsq, L.siv_b3, L.siv_a3)
-                                : L.sil * __builtin_amdgcn_rcpf(L.alfli2 + xmsq);
-    return f - base;
+    float zvsq = xmsq * l.rec_alfvi2;
+    float base = (l.flags & 1) ? (2.f - xmsq * r_bndsq) * l.f4bnd : l.f4bnd;
+    float q = __fmaf_rn(zvsq, l.siv_b3, l.siv_a3);
+    float lz = l.sil * __builtin_amdgcn_rcpf(l.alfli2 + xmsq);
+    float f = ((zvsq <= ZSQBND) ? q : lz) - base;
+    if (!(l.flags & 2)) f = __fmaf_rn(xm * dptrat, f, f);
+    return f;
 }
 
+// FP32-compensated gather: offsets from the FP64 per-record anchor, Kahan sum per point.
 __global__ void __launch_bounds__(256) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles,
-                                                  Shrunk sh, double *R4, unsigned long long *work) {
-    __shared__ F4LineF L[256];
+                                                  Shrunk sh, const F4Rec32 *__restrict__ recs, double *R4,
+                                                  unsigned long long *work) {
     int2 t = tiles[blockIdx.x];
     const R4Blk &B = blks[t.x];
     const Job &J = jobs[B.layer];
@@ -930,51 +995,24 @@ __global__ void __launch_bounds__(256) k_f4conv32(const Job *jobs, const R4Blk *
     if (j0 <= B.nptr4) {
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, jl, &c0, &c1);
-        for (int64_t base = c0; base < c1; base += 256) {
-            int64_t i = base + threadIdx.x;
-            if (i < c1) {
-                F4Line d;
-                f4_line_setup(d, J, B, sh.vnu[i], sh.sp[i], sh.spp[i], sh.alfa[i], sh.epp[i], sh.co2[i]);
-                F4LineF &o = L[threadIdx.x];
-                o.jmin = d.jmin; o.jmax = d.jmax; o.jminm = d.jminm; o.jmaxm = d.jmaxm; o.flags = d.flags;
-                o.sil = (float)d.sil; o.siv_a3 = (float)d.siv_a3; o.siv_b3 = (float)d.siv_b3;
-                o.f4bnd = (float)d.f4bnd; o.alfli2 = (float)d.alfli2; o.rec_alfvi2 = (float)d.rec_alfvi2;
-                o.dptrat = (float)d.dptrat;
-                // nearest grid point and FP64 offset: x(j) = (j-1)*dvr4 - xnui = (j-jc)*dvr4 + d
-                double q = d.xnui / J.dvr4;
-                int jc = (int)floor(q + 0.5) + 1;
-                o.jc = jc;
-                o.d = (float)((double)(jc - 1) * J.dvr4 - d.xnui);
-                double qm = d.xnum / J.dvr4;
-                int jcm = d.jmaxm >= d.jminm ? (int)floor(qm + 0.5) + 1 : 0;
-                o.jcm = jcm;
-                o.dm = d.jmaxm >= d.jminm ? (float)((double)(jcm - 1) * J.dvr4 - d.xnum) : 0.f;
+        const F4Rec32 *r = recs + B.rec - B.lo;
+        for (int64_t i = c0; i < c1; i++) {
+            const F4Rec32 &l = r[i];
+            if ((unsigned)(j - l.jmin) <= (unsigned)(l.jmax - l.jmin)) {
+                float xm = __fmaf_rn((float)(j - l.jc), dvr4f, l.d);
+                float y = f4_eval32(l, xm, l.dptrat) - lo;
+                float tt = hi + y;
+                lo = (tt - hi) - y;
+                hi = tt;
+                nev++;
             }
-            __syncthreads();
-            int nb = (int)min((int64_t)256, c1 - base);
-            for (int k = 0; k < nb; k++) {
-                const F4LineF &l = L[k];
-                if (j >= l.jmin && j <= l.jmax) {
-                    float xm = __fmaf_rn((float)(j - l.jc), dvr4f, l.d);
-                    float f = f4_value_f(l, xm);
-                    if (!(l.flags & 2)) f = __fmaf_rn(xm * l.dptrat, f, f);
-                    float y = f - lo;
-                    float tt = hi + y;
-                    lo = (tt - hi) - y;
-                    hi = tt;
-                    nev++;
-                }
-                if (j >= l.jminm && j <= l.jmaxm) {
-                    float xm = __fmaf_rn((float)(j - l.jcm), dvr4f, l.dm);
-                    float f = f4_value_f(l, xm);
-                    if (!(l.flags & 2)) f = __fmaf_rn(xm * (-l.dptrat), f, f);
-                    float y = f - lo;
-                    float tt = hi + y;
-                    lo = (tt - hi) - y;
-                    hi = tt;
-                }
+            if (l.jmaxm >= l.jminm && (unsigned)(j - l.jminm) <= (unsigned)(l.jmaxm - l.jminm)) {
+                float xm = __fmaf_rn((float)(j - l.jcm), dvr4f, l.dm);
+                float y = f4_eval32(l, xm, -l.dptrat) - lo;
+                float tt = hi + y;
+                lo = (tt - hi) - y;
+                hi = tt;
             }
-            __syncthreads();
         }
     }
     if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? (double)hi - (double)lo : 0.;
@@ -1599,7 +1637,7 @@ struct Batch {
     std::vector<int> regrid_lay;
     std::vector<int64_t> regrid_first;
     int64_t max_line = 0, max_f4 = 0, max_rec = 0, tot_line = 0, tot_f4 = 0, tot_rec = 0;
-    int64_t tot_a = 0, tot_r4 = 0, tot_nat = 0, tot_out = 0;
+    int64_t tot_a = 0, tot_r4 = 0, tot_nat = 0, tot_out = 0, tot_f4rec = 0;
     int max_stride = 0;
     bool any_jrad1 = false, any_regrid = false;
 };
@@ -1641,6 +1679,10 @@ static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, 
             B.panels.push_back(P);
         }
         J.r4_off = B.tot_r4;
+        // each shrunk line lies in the read range of at most 3 + 50/(2500 DVR4) blocks
+        J.f4rec_cap = J.f4_n * (int64_t)(3 + std::ceil(50.0 / (LIMOUT4 * J.dvr4))) + 64;
+        J.f4rec_off = B.tot_f4rec;
+        B.tot_f4rec += J.f4rec_cap;
         for (auto &b : pl.blks) {
             b.layer = l;
             b.r4_off = B.tot_r4;
@@ -1690,6 +1732,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_ch = cv.take<int4>((size_t)B.tot_rec);
     size_t o_be = cv.take<int64_t>((size_t)B.tot_rec);
     size_t o_R4 = cv.take<double>((size_t)B.tot_r4);
+    size_t o_rec = precision == CLBLM_FP64 ? cv.take<F4Rec64>((size_t)B.tot_f4rec) : cv.take<F4Rec32>((size_t)B.tot_f4rec);
     size_t o_nat = cv.take<double>(std::max<int64_t>(1, B.tot_nat));
     size_t o_rg = cv.take<int>(1);
     size_t o_rgf = cv.take<int64_t>(nlay);
@@ -1770,9 +1813,8 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         hipLaunchKernelGGL(k_conv64, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
                            d_jobs, d_pan, lp, T, d_A, C->d_work);
     } else {
-        size_t lds = (size_t)(2 * B.max_stride + 5 * TABN) * sizeof(float);
-        HIPCHK(hipFuncSetAttribute((const void *)k_conv32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_conv32, dim3((unsigned)npan), dim3(512), lds, st, d_jobs, d_pan, lp, T,
+        size_t lds = (size_t)(2 * B.max_stride) * sizeof(float);
+        hipLaunchKernelGGL(k_conv32, dim3((unsigned)npan), dim3(256), lds, st, d_jobs, d_pan, lp, T,
                            reinterpret_cast<float *>(d_A), C->d_work);
     }
     HIPCHK(hipEventRecord(C->ev[2], st));
@@ -1788,13 +1830,23 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, st));
     }
     hipLaunchKernelGGL(k_r4range, dim3(cdiv(nblk, 256)), dim3(256), 0, st, d_jobs, d_blk, nblk, f, sh);
+    hipLaunchKernelGGL(k_f4recoff, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, d_blk, C->d_err);
+    {
+        unsigned gx = std::max<unsigned>(1, cdiv(B.max_f4 * 2, 256 * 8));
+        if (precision == CLBLM_FP64)
+            hipLaunchKernelGGL(k_f4setup<F4Rec64>, dim3(gx, (unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, nblk,
+                               sh, at<F4Rec64>(base, o_rec));
+        else
+            hipLaunchKernelGGL(k_f4setup<F4Rec32>, dim3(gx, (unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, nblk,
+                               sh, at<F4Rec32>(base, o_rec));
+    }
     HIPCHK(hipEventRecord(C->ev[5], st));
     if (precision == CLBLM_FP64)
         hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh,
-                           d_R4, C->d_work);
+                           at<F4Rec64>(base, o_rec), d_R4, C->d_work);
     else
         hipLaunchKernelGGL(k_f4conv32, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh,
-                           d_R4, C->d_work);
+                           at<F4Rec32>(base, o_rec), d_R4, C->d_work);
     HIPCHK(hipEventRecord(C->ev[6], st));
     if (B.any_jrad1) hipLaunchKernelGGL(k_r4rad, dim3(cdiv(nblk, 64)), dim3(64), 0, st, d_jobs, d_blk, nblk, d_R4);
     HIPCHK(hipEventRecord(C->ev[3], st));
@@ -1819,6 +1871,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     HIPCHK(hipStreamSynchronize(st));
     if (herr & 1) return set_err(CLBLM_ERR_UNSUPPORTED, "line coupling records (IFLG 1..3) are not supported");
     if (herr & 4) return set_err(CLBLM_ERR_UNSUPPORTED, "shrink chunk longer than 64 TAPE3 records (too few lines of the selected molecules per record)");
+    if (herr & 8) return set_err(CLBLM_ERR_CAPACITY, "F4 record capacity exceeded");
     float t01, t12, t23, t34;
     HIPCHK(hipEventElapsedTime(&t01, C->ev[0], C->ev[1]));
     HIPCHK(hipEventElapsedTime(&t12, C->ev[1], C->ev[2]));
