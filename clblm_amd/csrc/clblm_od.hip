@@ -949,13 +949,14 @@ __global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *
         f4_candidates(J, B, sh, j0, jl, &c0, &c1);
         const double xj = (double)(j - 1) * J.dvr4;
         const F4Rec64 *r = recs + B.rec - B.lo;
+#pragma unroll 2
         for (int64_t i = c0; i < c1; i++) {
             const F4Rec64 &l = r[i];
             if (j >= l.jmin && j <= l.jmax) {
                 acc = acc + f4_eval64(l, xj - l.xnui);
                 nev++;
             }
-            if (l.jmaxm >= l.jminm && j >= l.jminm && j <= l.jmaxm) {
+            if (j >= l.jminm && j <= l.jmaxm) {
                 F4Rec64 m = l;
                 m.dptrat = -l.dptrat;
                 acc = acc + f4_eval64(m, xj - l.xnum);
@@ -996,9 +997,10 @@ __global__ void __launch_bounds__(256) k_f4conv32(const Job *jobs, const R4Blk *
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, jl, &c0, &c1);
         const F4Rec32 *r = recs + B.rec - B.lo;
+#pragma unroll 2
         for (int64_t i = c0; i < c1; i++) {
             const F4Rec32 &l = r[i];
-            if ((unsigned)(j - l.jmin) <= (unsigned)(l.jmax - l.jmin)) {
+            if (j >= l.jmin && j <= l.jmax) {
                 float xm = __fmaf_rn((float)(j - l.jc), dvr4f, l.d);
                 float y = f4_eval32(l, xm, l.dptrat) - lo;
                 float tt = hi + y;
@@ -1006,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_f4conv32(const Job *jobs, const R4Blk *
                 hi = tt;
                 nev++;
             }
-            if (l.jmaxm >= l.jminm && (unsigned)(j - l.jminm) <= (unsigned)(l.jmaxm - l.jminm)) {
+            if (j >= l.jminm && j <= l.jmaxm) {
                 float xm = __fmaf_rn((float)(j - l.jcm), dvr4f, l.dm);
                 float y = f4_eval32(l, xm, -l.dptrat) - lo;
                 float tt = hi + y;

@@ -86,5 +86,7 @@ def test_batch_matches_single_calls(ctx):
     out, off = ctx.od_batch(h, layers, pars)
     torch.cuda.synchronize()
     host = out.cpu().numpy()
+    # pass 1 sums with LDS atomics, so the order of the FP64 additions (and the
+    # last bits) may differ between launches
     for k, od in enumerate(singles):
-        np.testing.assert_array_equal(host[off[k]:off[k + 1]], od)
+        np.testing.assert_allclose(host[off[k]:off[k + 1]], od, rtol=1e-13, atol=0)
