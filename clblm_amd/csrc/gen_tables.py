@@ -61,7 +61,7 @@ def parse_tdat(src):
 
 
 def parse_qt(src, name):
-    start = src.index("Subroutine " + name)
+    start = re.search(r"Subroutine\s+" + name + r"\b", src).start()
     end = src.index("END SUBROUTINE", start)
     body = src[start:end]
     niso = int(re.search(r"QofT\(\s*(\d+)\s*,\s*119\)", body).group(1))

@@ -1204,11 +1204,73 @@ done:
 }
 
 /* ---------------------------------------------------------------------------
- * Source functions, clblm_EMLAY.f90:18-72,187-223.
+ * Source functions, clblm_EMLAY.f90:18-72,104-223.
  * ------------------------------------------------------------------------- */
-double orc_planck(double v, double t) {
-    double xkt = t / RADCN2;
-    return RADCN1 * v * v * v / (exp(v / xkt) - 1.);
+static double planck_x(double v, double xkt) {
+    if (xkt <= 1.e-6) return 0.;
+    double cv3 = RADCN1 * v * v * v;
+    double evt = exp(v / xkt);
+    return cv3 / (evt - 1.);
 }
-void orc_planck_drv(double v, double t, double *b, double *dbdt) { (void)v; (void)t; *b = 0; *dbdt = 0; }
-void orc_rlin(double tau, double *w, double *dw) { (void)tau; *w = 0; *dw = 0; }
+double orc_planck(double v, double t) { return planck_x(v, t / RADCN2); }
+
+void orc_planck_drv(double v, double t, double *b, double *dbdt) {
+    double xkt = t / RADCN2;
+    if (xkt <= 1.e-6) { *b = 0.; *dbdt = 0.; return; }
+    double cv3 = RADCN1 * v * v * v;
+    double evt = exp(v / xkt);
+    *b = cv3 / (evt - 1.);
+    *dbdt = *b * *b * (evt * v) / (RADCN2 * cv3 * xkt * xkt);
+}
+
+void orc_rlin(double od, double *hlf, double *dhlf) {
+    const double dbAvg2dtau = -1. / 12;
+    double tex = 0.;
+    if (od < 1.e-03) *hlf = .5 + od * dbAvg2dtau;
+    else if (od > 20.) *hlf = 1. / od;
+    else { tex = exp(od); *hlf = 1. / (od) - 1. / (tex - 1.); }
+    if (dhlf) {
+        if (od < 1.e-03) *dhlf = dbAvg2dtau;
+        else if (od > 20.) *dhlf = -1. / (od * od);
+        else *dhlf = -1. / (od * od) + tex / ((tex - 1) * (tex - 1));
+    }
+}
+
+/* layerEmis (clblm_EMLAY.f90:104-178) at one point: ta is the boundary the layer's
+ * linear-in-tau source starts from, tave the layer temperature. */
+static double layer_emis(double v, double ta, double tb, double tave, double tx, double od, int lin) {
+    double abso = 1 - tx, hlf = 0.;
+    if (lin != 0) orc_rlin(od, &hlf, NULL);
+    double xkta = ta / RADCN2, xktb = tb / RADCN2, xkt = tave / RADCN2;
+    if (lin == 0) return abso * planck_x(v, xkt);
+    if (lin == 1) {
+        double bba = planck_x(v, xkta), bb = planck_x(v, xkt);
+        return abso * (bba + 2 * (bb - bba) * hlf);
+    }
+    double bba = planck_x(v, xkta), bbb = planck_x(v, xktb);
+    return abso * (bba + (bbb - bba) * hlf);
+}
+
+/* Upwelling radiance at the top of nlay layers (layer 0 at the surface), every
+ * layer below the observer: mode2_lookDn_mergeUp / layerMerge_mode2
+ * (clblm_noScattRT.f90:1287-1296, 1704-1794) on a common grid.  od[l*npts+i] is the
+ * layer OD along the view path; the down path uses od * sec_dn. */
+void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
+               const double *t_lev, double emis, double tskin, int lin, double sec_dn, double *rad) {
+    for (int64_t i = 0; i < npts; i++) {
+        double v = v1 + (double)i * dv;
+        double r = emis * orc_planck(v, tskin);
+        double rho = 1. - emis;
+        for (int l = 0; l < nlay; l++) {
+            double oda = od[(int64_t)l * npts + i];
+            double odb = oda * sec_dn;
+            double txa = exp(-oda), txb = exp(-odb);
+            double tbot = t_lev[l], ttop = t_lev[l + 1], tave = t_lay[l];
+            double emdn = layer_emis(v, tbot, ttop, tave, txb, odb, lin);
+            double emup = lin != 0 ? layer_emis(v, ttop, tbot, tave, txa, oda, lin) : emdn;
+            r = emup + txa * (r + emdn * rho);
+            rho = txa * txb * rho;
+        }
+        rad[i] = r;
+    }
+}
