@@ -65,7 +65,8 @@ def load():
     lib.clblm_od_batch.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int32),
                                    P, P, P]
     lib.clblm_rt_up.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
-                                P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, P, P, P, P]
+                                P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                P, P, P, P, P, P]
     lib.clblm_timing.argtypes = [P, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 4 + \
         [ctypes.POINTER(ctypes.c_int64)]
     lib.clblm_work.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),

@@ -3,7 +3,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "clblm_od.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("clblm_od.hip", "clblm_rt.hip")]
 OUT = os.path.join(HERE, "libclblm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -14,7 +14,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
 
 
 def build_hip(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-o", OUT, SRC]
+    cmd = [HIPCC] + FLAGS + ["-o", OUT] + SRCS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)

@@ -1431,6 +1431,8 @@ struct clblm_ctx {
     DevArena arena;
     int *d_err = nullptr;
     unsigned long long *d_work = nullptr;   // [conv evaluations, F4 evaluations]
+    double *rt_state = nullptr;             // per-layer recursion states for the reverse sweep
+    size_t rt_state_cap = 0;
     hipEvent_t ev[8]{};
     double ms_conv_k = 0, ms_f4_k = 0;
     double ms[4] = {0, 0, 0, 0};
@@ -1891,6 +1893,11 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
 // ====================================================================================
 // C-ABI
 // ====================================================================================
+extern "C" int clblm_rt_up_impl(int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                                const double *d_tlay, const double *d_tlev, double emis, double tskin, int lin,
+                                double sec_dn, double *d_rad, double *d_dr_dod, double *d_dr_dt,
+                                double *d_dr_dtskin, double *d_dr_demis, double *d_state, void *stream);
+
 extern "C" {
 
 int clblm_version(void) { return 1; }
@@ -1925,6 +1932,7 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     (void)hipFree(c->tabs);
     (void)hipFree(c->d_err);
     (void)hipFree(c->d_work);
+    if (c->rt_state) (void)hipFree(c->rt_state);
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2067,9 +2075,34 @@ int clblm_work(clblm_ctx *c, int reset, int64_t *conv_evals, int64_t *f4_evals, 
     return 0;
 }
 
-int clblm_rt_up(clblm_ctx *, int, int64_t, double, double, const double *, const double *, const double *, double,
-                double, int, double *, double *, double *, void *) {
-    return set_err(CLBLM_ERR_UNSUPPORTED, "not built yet");
+int clblm_rt_up(clblm_ctx *c, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                const double *d_tlay, const double *d_tlev, double surf_emis, double t_skin, int lin_in_tau,
+                double sec_dn, double *d_rad, double *d_dr_dod, double *d_dr_dtlay, double *d_dr_dtskin,
+                double *d_dr_demis, void *stream) {
+    if (!c || nlay <= 0 || npts <= 0 || !d_od || !d_tlay || !d_tlev || !d_rad)
+        return set_err(CLBLM_ERR_ARG, "null argument or empty grid");
+    if (lin_in_tau < 0 || lin_in_tau > 2) return set_err(CLBLM_ERR_ARG, "lin_in_tau must be 0, 1 or 2");
+    if (d_dr_dod && !d_dr_dtlay) return set_err(CLBLM_ERR_ARG, "dR/dOD needs the dR/dT output too");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    double *state = nullptr;
+    if (d_dr_dod) {
+        size_t bytes = sizeof(double) * 2 * (size_t)nlay * (size_t)npts;
+        if (c->rt_state_cap < bytes) {
+            if (c->rt_state) (void)hipFree(c->rt_state);
+            c->rt_state = nullptr;
+            c->rt_state_cap = 0;
+            HIPCHK(hipMalloc(&c->rt_state, bytes));
+            c->rt_state_cap = bytes;
+        }
+        state = c->rt_state;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = clblm_rt_up_impl(nlay, npts, v1, dv, d_od, d_tlay, d_tlev, surf_emis, t_skin, lin_in_tau, sec_dn,
+                              d_rad, d_dr_dod, d_dr_dtlay, d_dr_dtskin, d_dr_demis, state, st);
+    if (rc) return set_err(rc, "radiance kernel launch failed");
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
 }
 
 }  // extern "C"

@@ -95,19 +95,22 @@ int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
                    const clblm_od_params *pars, int32_t *jrad_state, int64_t *offsets,
                    double *d_od, void *stream);
 
-/* ---- radiance and Jacobians ----------------------------------------------------
- * Upwelling nadir radiance at the top of nlay layers (layer 0 at the surface) on a
- * common grid v1 + i*dv, i < npts, from layer ODs d_od[l*npts + i] and layer
- * temperatures; the reference's mode2_lookDn_mergeUp/layerMerge_mode2
- * (clblm_noScattRT.f90:1134-1870) with layerEmis (clblm_EMLAY.f90:104-178).
- * lin_in_tau: 0 (isothermal layer) or 1 (LBLRTM linear-in-tau).
- * t_lev[nlay+1] level temperatures (used when lin_in_tau != 0).
- * Outputs (device): d_rad[npts]; if d_dr_dod != NULL also dR/dOD_l
- * (nlay*npts) and d_dr_dtlay (nlay*npts) = dR/dT_l through the Planck source. */
-int clblm_rt_up(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv,
-                const double *d_od, const double *t_lay, const double *t_lev,
-                double surf_emis, double t_skin, int lin_in_tau, double *d_rad,
-                double *d_dr_dod, double *d_dr_dtlay, void *stream);
+/* ---- radiance and its derivatives ---------------------------------------------
+ * Upwelling radiance at the top of nlay layers (layer 0 at the surface) on a
+ * common grid v1 + i*dv, i < npts: the merge-up recursion of mode2_lookDn_mergeUp /
+ * layerMerge_mode2 (clblm_noScattRT.f90:1287-1296, 1704-1794) with layerEmis
+ * (clblm_EMLAY.f90:104-178).  d_od[l*npts + i] is the layer OD along the view path;
+ * the downwelling path uses sec_dn * OD (1 for a specular surface, 1.67 for the
+ * diffusivity approximation).  lin_in_tau: 0, 1 (LBLRTM) or 2.  Temperatures are
+ * device arrays: d_tlay[nlay], d_tlev[nlay+1].  All outputs are device arrays.
+ * d_rad[npts]; when d_dr_dod is non-NULL the reverse sweep also writes dR/dOD_l and
+ * dR/dT_l through the layer source (nlay*npts each), and, if non-NULL,
+ * dR/dTskin and dR/deps (npts each).  This is the per-point analytic derivative
+ * that noScattJacob's Jacobians routine forms (clblm_noScattJac.f90:346-716). */
+int clblm_rt_up(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                const double *d_tlay, const double *d_tlev, double surf_emis, double t_skin, int lin_in_tau,
+                double sec_dn, double *d_rad, double *d_dr_dod, double *d_dr_dtlay, double *d_dr_dtskin,
+                double *d_dr_demis, void *stream);
 
 /* ---- timing -------------------------------------------------------------------
  * Average device time (ms) of each pipeline stage over the launches since the
