@@ -135,7 +135,9 @@ def test_jacobian_t_vs_pipeline_difference(ctx):
     jb = _jac_setup(ctx, FP64)
     base = jb.block(640.0, 650.0)
     for l in (0, 5, 11):
-        plus, minus = copy.deepcopy(jb), copy.deepcopy(jb)
+        plus, minus = copy.copy(jb), copy.copy(jb)
+        plus.layers = [copy.copy(x) for x in jb.layers]
+        minus.layers = [copy.copy(x) for x in jb.layers]
         plus.layers[l].tave += 1.0
         minus.layers[l].tave -= 1.0
         rp, rm = plus.block(640.0, 650.0)["rad"], minus.block(640.0, 650.0)["rad"]
