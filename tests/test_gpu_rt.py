@@ -66,26 +66,27 @@ def test_reverse_sweep_vs_central_differences(ctx, lin, sec):
     od, t_lay, t_lev = _case(n=500)
     v1, dv = 700.0, 1.1
     rad, g_od, g_t, g_ts, g_e = _run(ctx, od, t_lay, t_lev, v1, dv, lin, sec)
+    scale = np.abs(rad).max()
     for l in (0, 3, 7, 11):
-        h = 1e-6 * np.maximum(od[l], 1e-3)
+        h = np.maximum(1e-5 * od[l], 1e-4)
         a, b = od.copy(), od.copy()
         a[l] += h
         b[l] -= h
         fd = (_run(ctx, a, t_lay, t_lev, v1, dv, lin, sec, False)[0] -
               _run(ctx, b, t_lay, t_lev, v1, dv, lin, sec, False)[0]) / (2 * h)
-        np.testing.assert_allclose(g_od[l], fd, rtol=2e-5, atol=1e-12 * np.abs(rad).max())
+        np.testing.assert_allclose(g_od[l], fd, rtol=1e-5, atol=1e-9 * scale)
         ta, tb = t_lay.copy(), t_lay.copy()
         ta[l] += 1e-3
         tb[l] -= 1e-3
         fd = (_run(ctx, od, ta, t_lev, v1, dv, lin, sec, False)[0] -
               _run(ctx, od, tb, t_lev, v1, dv, lin, sec, False)[0]) / 2e-3
-        np.testing.assert_allclose(g_t[l], fd, rtol=2e-5, atol=1e-12 * np.abs(rad).max())
+        np.testing.assert_allclose(g_t[l], fd, rtol=1e-5, atol=1e-9 * scale)
     fd = (_run(ctx, od, t_lay, t_lev, v1, dv, lin, sec, False, tskin=295.001)[0] -
           _run(ctx, od, t_lay, t_lev, v1, dv, lin, sec, False, tskin=294.999)[0]) / 2e-3
-    np.testing.assert_allclose(g_ts, fd, rtol=1e-5, atol=1e-14)
+    np.testing.assert_allclose(g_ts, fd, rtol=1e-5, atol=1e-9 * scale)
     fd = (_run(ctx, od, t_lay, t_lev, v1, dv, lin, sec, False, emis=0.981)[0] -
           _run(ctx, od, t_lay, t_lev, v1, dv, lin, sec, False, emis=0.979)[0]) / 2e-3
-    np.testing.assert_allclose(g_e, fd, rtol=1e-5, atol=1e-14)
+    np.testing.assert_allclose(g_e, fd, rtol=1e-5, atol=1e-9 * scale)
 
 
 def test_autograd_radiance(ctx):
@@ -104,11 +105,15 @@ def test_autograd_radiance(ctx):
     np.testing.assert_allclose(tl.grad.cpu().numpy(), (g_t * wn[None, :]).sum(1), rtol=1e-12)
 
 
-def _jac_setup(ctx, prec, n_lines=20000, nlay=12):
+def _jac_setup(ctx, prec, n_lines=3000, nlay=12):
+    # a thin line list and a non-isothermal profile, so that every layer shows through
     from clblm_amd.jacobian import UpwellingJacobian
     lines = W.profile_lines(n_lines, 500.0, 800.0, seed=1001)
-    layers = W.atmosphere(nlay, 1.0)
-    t_lev = np.array([l.tave for l in layers] + [layers[-1].tave])
+    lines.s[:] = lines.s * np.float32(1e-3)
+    noise = np.random.default_rng(5).normal(0.0, 3.0, nlay)
+    layers = W.atmosphere(nlay, 1.0, t_noise=noise)
+    t_lev = np.array([layers[0].tave + 2.0] + [0.5 * (a.tave + b.tave) for a, b in zip(layers[:-1], layers[1:])] +
+                     [layers[-1].tave - 1.0])
     dvs = [W.calculate_dv(650.0, l.pave, l.tave, l.w[1] / l.wtot) for l in layers]
     h = ctx.upload_lines(lines)
     return UpwellingJacobian(ctx, h, layers, t_lev, dvs, precision=prec, dt=1.0,
@@ -133,14 +138,15 @@ def test_jacobian_t_vs_pipeline_difference(ctx):
     import copy
     from clblm_amd._lib import FP64
     jb = _jac_setup(ctx, FP64)
+    jb.dt = 0.1
     base = jb.block(640.0, 650.0)
     for l in (0, 5, 11):
         plus, minus = copy.copy(jb), copy.copy(jb)
         plus.layers = [copy.copy(x) for x in jb.layers]
         minus.layers = [copy.copy(x) for x in jb.layers]
-        plus.layers[l].tave += 1.0
-        minus.layers[l].tave -= 1.0
+        plus.layers[l].tave += 0.1
+        minus.layers[l].tave -= 0.1
         rp, rm = plus.block(640.0, 650.0)["rad"], minus.block(640.0, 650.0)["rad"]
-        fd = ((rp - rm) / 2.0).cpu().numpy()
+        fd = ((rp - rm) / 0.2).cpu().numpy()
         j = base["jac_t"][l].cpu().numpy()
         assert np.abs(j - fd).max() <= 1e-3 * np.abs(fd).max() + 1e-15, l
