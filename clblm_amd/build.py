@@ -23,5 +23,27 @@ def build_hip(verbose=False):
     return OUT
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FDIR = os.path.join(HERE, "fortran")
+FBIN = os.path.join(FDIR, "_bin")
+
+
+def build_fortran():
+    """The ISO_C_BINDING module and the Fortran test caller (skipped without flang)."""
+    if not os.path.exists(FLANG):
+        return None
+    os.makedirs(FBIN, exist_ok=True)
+    steps = [
+        [FLANG, "-O2", "-J", FBIN, "-c", os.path.join(FDIR, "clblm_hip.f90"), "-o", os.path.join(FBIN, "clblm_hip.o")],
+        [FLANG, "-O2", "-J", FBIN, "-o", os.path.join(FBIN, "test_odlay_f"), os.path.join(FDIR, "test_odlay_f.f90"),
+         os.path.join(FBIN, "clblm_hip.o"), "-L" + HERE, "-lclblm_hip", "-Wl,-rpath," + HERE],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
+    return os.path.join(FBIN, "test_odlay_f")
+
+
 if __name__ == "__main__":
     print(build_hip(verbose=True))

@@ -1,0 +1,68 @@
+! Drop-in Module_ODLAY: the reference's per-layer OD entry point ODLAY_nonStruct
+! (src/clblm_src/clblm_ODLAY.f90:129-176) with the same dummy arguments, computed
+! by the HIP library through clblm_hip.  Compile it in place of clblm_ODLAY.f90,
+! against the reference's Module_ConstParam and Module_Config; the callers
+! (layerMerge_*, pathOD_up_dn_sun, drv_LevelTemp, drv_LevelConc) are unchanged.
+! The line list is uploaded once with clblm_hip_lines (the TAPE3 reader is not
+! part of this library).  Supported: molecules 1-7, no NLTE, no cross-sections,
+! no MT_CKD continuum (contnmOff must be set), no line coupling.
+module Module_ODLAY
+   use iso_c_binding
+   use clblm_hip
+   implicit none
+   private
+   public :: ODLAY
+   interface ODLAY
+      module procedure ODLAY_nonStruct
+   end interface
+   logical, save :: firstPassJRAD = .true.
+contains
+   subroutine ODLAY_nonStruct(nMol, molID, Pav, Tav, W, Wtot, Zbot, Ztop, odCtrl, &
+                              V1, V2, DVnormal, DVnarrow, &
+                              SAMPLE, ALFAL0, IPATH, LayNo, &
+                              nSamp, monoOD, NLTE, nlteEmisFac, DVout, ajMolName)
+      use Module_ConstParam, only: r8 => kind_r8, molNum
+      use Module_Config, only: CLBLM_OD_Ctrl
+      integer, intent(in) :: nMol
+      character(*), intent(in) :: molID(:)
+      real, intent(in) :: Pav, Tav, W(:), Wtot, Zbot, Ztop
+      type(CLBLM_OD_Ctrl), intent(in) :: odCtrl
+      real(r8), intent(in) :: V1, V2
+      real, intent(in) :: DVnormal, DVnarrow, SAMPLE, ALFAL0
+      integer, intent(in) :: IPATH, layNo
+      integer, intent(out) :: nSamp
+      real, allocatable, intent(out) :: monoOD(:)
+      logical, optional, intent(in) :: NLTE
+      real, allocatable, optional, intent(out) :: nlteEmisFac(:)
+      real, optional, intent(inout) :: DVout
+      character(*), optional, intent(in) :: ajMolName
+      integer :: mols(nMol), k
+      real(c_double) :: dvo, dvu
+      real(c_double), allocatable :: od(:)
+      if (present(NLTE)) then
+         if (NLTE) stop '--- ODLAY(): NLTE is not supported by the HIP path'
+      end if
+      if (.not. odCtrl%contnmOff) stop '--- ODLAY(): the MT_CKD continuum is not part of the HIP path'
+      if (odCtrl%lineOff) stop '--- ODLAY(): lineOff is not supported by the HIP path'
+      do k = 1, nMol
+         mols(k) = molNum(molID(k))
+      end do
+      ! the first call takes JRAD from the control record, later calls keep the
+      ! SAVEd value (clblm_ODLAY.f90:349-352)
+      if (firstPassJRAD) then
+         clblm_hip_jrad = odCtrl%JRAD
+         firstPassJRAD = .false.
+      end if
+      dvo = 0.
+      if (present(DVout)) then
+         if (DVout > 0.) dvo = DVout
+      end if
+      call clblm_hip_odlay_plain(mols, real(W(1:nMol), c_double), real(Pav, c_double), real(Tav, c_double), &
+                                 real(Wtot, c_double), real(V1, c_double), real(V2, c_double), &
+                                 real(DVnormal, c_double), real(DVnarrow, c_double), real(SAMPLE, c_double), &
+                                 dvo, CLBLM_FP64, nSamp, od, dvu)
+      allocate (monoOD(nSamp))
+      monoOD = od
+      if (present(DVout)) DVout = dvu
+   end subroutine
+end module

@@ -1,0 +1,69 @@
+"""The Fortran side: the ISO_C_BINDING module and the drop-in Module_ODLAY compile, the
+drop-in against the reference's own Module_ConstParam/Module_Config; the Fortran
+caller reproduces the Python/ctypes result on the GPU."""
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from clblm_amd import build as B
+from clblm_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REFMOD = os.path.join(ROOT, "oracle", "_ref")
+
+
+@pytest.mark.skipif(not os.path.exists(B.FLANG), reason="flang not installed")
+def test_shim_compiles():
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([B.FLANG, "-J", d, "-c", os.path.join(B.FDIR, "clblm_hip.f90"), "-o",
+                            os.path.join(d, "a.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFMOD, "module_config.mod")),
+                    reason="reference modules not built (oracle/_ref)")
+def test_dropin_module_compiles_against_reference_modules():
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([B.FLANG, "-J", d, "-c", os.path.join(B.FDIR, "clblm_hip.f90"), "-o",
+                            os.path.join(d, "a.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        r = subprocess.run([B.FLANG, "-fdefault-real-8", "-cpp", "-I", d, "-I", REFMOD, "-J", d, "-c",
+                            os.path.join(B.FDIR, "module_odlay_hip.f90"), "-o", os.path.join(d, "b.o")],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_fortran_caller_matches_ctypes():
+    exe = os.path.join(B.FBIN, "test_odlay_f")
+    assert os.path.exists(exe), "build() compiles the Fortran caller"
+    from clblm_amd.odlay import Context
+    lines = W.profile_lines(8000, 500.0, 800.0, seed=1001)
+    lay = W.atmosphere()[7]
+    dv = W.calculate_dv(650.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+    mols = sorted(lay.w)
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<ii", lines.n, len(mols)))
+            for a in (lines.nu, lines.s, lines.a_air, lines.a_self, lines.e_lo, lines.tmpalf, lines.shift,
+                      lines.mol, lines.iflg):
+                f.write(np.ascontiguousarray(a).tobytes())
+            f.write(np.array(mols, np.int32).tobytes())
+            f.write(np.array([lay.w[m] for m in mols], np.float64).tobytes())
+            f.write(np.array([lay.pave, lay.tave, lay.wtot, 600.0, 700.0, dv, dv, 4.0, 0.0]).tobytes())
+        r = subprocess.run([exe, fin, fout], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        raw = open(fout, "rb").read()
+        n, dvu, jrad = struct.unpack("<idi", raw[:16])
+        od_f = np.frombuffer(raw[16:16 + 8 * n], np.float64)
+    ctx = Context(0)
+    h = ctx.upload_lines(lines)
+    od, dvp = ctx.odlay(h, lay, 600.0, 700.0, dv)
+    ctx.close()
+    assert dvu == dvp and jrad == ctx.jrad == 0
+    np.testing.assert_allclose(od_f, od, rtol=1e-13, atol=0)
