@@ -360,34 +360,23 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
 }
 
 // ------------------------------------------------------------------------------------
-// Pass 1, FP32-compensated: same walk and the same FP64 table argument ZF (so the
-// table indices are the reference's), FP32 tables in LDS, FP32 line-shape products,
-// and a compensated LDS sum: every atomic add returns the old value, the lane
-// recovers the rounding error of that add exactly (TwoSum) and adds it to a second
-// accumulator.  The panel arrays leave as (hi, lo) FP32 pairs.
+// Pass 1, FP32 line shapes: same walk and the same FP64 table argument ZF (so the
+// table indices are the reference's), FP32 tables and FP32 line-shape products, and
+// the per-point sums accumulated in FP64 in LDS.  On gfx950 a no-return ds_add_f64 is
+// cheaper than a compensated FP32 sum, which needs the returned old value of every
+// add; an FP64 accumulator of FP32 terms is at least as accurate as a Kahan sum.
 // ------------------------------------------------------------------------------------
-
-__device__ __forceinline__ void csum_add(float *hi, float *lo, float v) {
-    float old = atomicAdd(hi, v);
-    float s = old + v;
-    float bb = s - old;
-    float err = (old - (s - bb)) + (v - bb);
-    if (err != 0.f) atomicAdd(lo, err);
-}
-
 __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *panels, LinePrep lp, Tables T,
-                                                float *A, unsigned long long *work) {
-    extern __shared__ float smf[];
+                                                double *A, unsigned long long *work) {
+    extern __shared__ double sm[];
     const Panel &P = panels[blockIdx.x];
     const Job &J = jobs[P.layer];
     const int max1 = J.max1, max2 = J.max2, max3 = J.max3, stride = J.stride;
-    float *hi = smf, *lo = smf + stride;
+    double *a1 = sm - 1, *a2 = sm + max1 - 1, *a3 = sm + max1 + max2 - 1;
     const float *__restrict__ t1 = T.f1s, *__restrict__ tg = T.fgs, *__restrict__ tv = T.xvers;
     const float *__restrict__ t2 = T.f2s, *__restrict__ t3 = T.f3s;
-    for (int t = threadIdx.x; t < 2 * stride; t += blockDim.x) smf[t] = 0.f;
+    for (int t = threadIdx.x; t < stride; t += blockDim.x) sm[t] = 0.;
     __syncthreads();
-    float *h1 = hi - 1, *h2 = hi + max1 - 1, *h3 = hi + max1 + max2 - 1;
-    float *l1 = lo - 1, *l2 = lo + max1 - 1, *l3 = lo + max1 + max2 - 1;
     const double DXF1 = 0.002, HWF1 = 4.;
     const double DV = J.dv;
     const double wavdxf = DV / DXF1, hwdxf = HWF1 / DXF1;
@@ -427,16 +416,17 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
             int iz3 = ftrunc(fabs(zf3) + 1.5);
             int iz2 = ftrunc(fabs(zf2) + 1.5);
             int iz1 = ftrunc(fabs(zf1) + 1.5);
-            if (j3 >= 1 && j3 <= max3) csum_add(&h3[j3], &l3[j3], s3 * t3[iz3]);
-            if (j2 >= 1 && j2 <= max2) csum_add(&h2[j2], &l2[j2], s2 * t2[iz2]);
-            if (j1 >= 1) csum_add(&h1[j1], &l1[j1], __fmaf_rn(sv, tv[iz1], __fmaf_rn(sd, tg[iz1], s1 * t1[iz1])));
+            if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(s3 * t3[iz3]));
+            if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(s2 * t2[iz2]));
+            if (j1 >= 1)
+                atomicAdd(&a1[j1], (double)__fmaf_rn(sv, tv[iz1], __fmaf_rn(sd, tg[iz1], s1 * t1[iz1])));
         }
     }
     for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
     if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work, nev);
     __syncthreads();
-    float *dst = A + 2 * P.a_off;
-    for (int t = threadIdx.x; t < 2 * stride; t += blockDim.x) dst[t] = smf[t];
+    double *dst = A + P.a_off;
+    for (int t = threadIdx.x; t < stride; t += blockDim.x) dst[t] = sm[t];
 }
 
 // ------------------------------------------------------------------------------------
@@ -1064,17 +1054,6 @@ struct PanelArr<double> {  // FP64: one array of stride doubles per panel
     __device__ double r2(int j) const { return p[m1 + j]; }
     __device__ double r3(int j) const { return p[m1 + m2 + j]; }
 };
-template <>
-struct PanelArr<float> {  // FP32-compensated: (hi, lo) FP32 arrays, summed in FP64
-    const float *h, *l;
-    int m1, m2;
-    __device__ PanelArr(const float *A, int64_t off, int max1, int max2, int stride)
-        : h(A + 2 * off - 1), l(A + 2 * off + stride - 1), m1(max1), m2(max2) {}
-    __device__ double r1(int j) const { return (double)h[j] + (double)l[j]; }
-    __device__ double r2(int j) const { return (double)h[m1 + j] + (double)l[m1 + j]; }
-    __device__ double r3(int j) const { return (double)h[m1 + m2 + j] + (double)l[m1 + m2 + j]; }
-};
-
 __device__ inline double xint_point(double v1a, double v2a, double dva, const double *A /*1-based*/,
                                     double vft, double dvr3, int I, int n1r3, int n2r3) {
     double recdva = 1. / dva;
@@ -1817,9 +1796,8 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         hipLaunchKernelGGL(k_conv64, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
                            d_jobs, d_pan, lp, T, d_A, C->d_work);
     } else {
-        size_t lds = (size_t)(2 * B.max_stride) * sizeof(float);
-        hipLaunchKernelGGL(k_conv32, dim3((unsigned)npan), dim3(256), lds, st, d_jobs, d_pan, lp, T,
-                           reinterpret_cast<float *>(d_A), C->d_work);
+        hipLaunchKernelGGL(k_conv32, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
+                           d_jobs, d_pan, lp, T, d_A, C->d_work);
     }
     HIPCHK(hipEventRecord(C->ev[2], st));
     if (B.max_f4 > 0) {
@@ -1854,12 +1832,8 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     HIPCHK(hipEventRecord(C->ev[6], st));
     if (B.any_jrad1) hipLaunchKernelGGL(k_r4rad, dim3(cdiv(nblk, 64)), dim3(64), 0, st, d_jobs, d_blk, nblk, d_R4);
     HIPCHK(hipEventRecord(C->ev[3], st));
-    if (precision == CLBLM_FP64)
-        hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
-                           d_abs, d_out, d_nat);
-    else
-        hipLaunchKernelGGL(k_pass2<float>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk,
-                           reinterpret_cast<const float *>(d_A), d_R4, d_abs, d_out, d_nat);
+    hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
+                       d_abs, d_out, d_nat);
     if (B.any_regrid) {
         int *d_rg = at<int>(base, o_rg);
         int64_t *d_rgf = at<int64_t>(base, o_rgf);
