@@ -738,32 +738,14 @@ __global__ void k_r4range(const Job *jobs, R4Blk *blks, int64_t nblk_tot, F4Prep
 }
 
 // ------------------------------------------------------------------------------------
-// CNVF4Q (clblm_ODLAY_LineF4.f90:187-494) as a gather: one lane per R4 point, the
-// shrunk lines that can reach a 256-point tile staged in LDS in batches of 256.
-// Lines are accumulated in the reference's line order.
+// CNVF4Q (clblm_ODLAY_LineF4.f90:187-494) as a gather: the per-line constants are
+// computed once per (shrunk line, R4 block) into records, then each R4 point sums the
+// records that reach it, in the reference's line order.
 // ------------------------------------------------------------------------------------
 struct F4Line {
     double xnui, xnum, sil, siv_a3, siv_b3, f4bnd, alfli2, rec_alfvi2, dptrat;
     int jmin, jmax, jminm, jmaxm, flags;  // flags: 1 co2, 2 spp==0
 };
-
-template <typename T>
-__device__ __forceinline__ T f4_value(const F4Line &L, T xm) {
-    const T ZSQBND = (T)4096.;
-    const T r_bndsq = (T)1. / (T)625.;
-    T xmsq = xm * xm;
-    T zvsq = xmsq * (T)L.rec_alfvi2;
-    T f4fn;
-    if (L.flags & 1) {
-        T fcnt = ((T)2. - (xmsq * r_bndsq)) * (T)L.f4bnd;
-        if (zvsq <= ZSQBND) f4fn = ((T)L.siv_a3 + zvsq * (T)L.siv_b3) - fcnt;
-        else f4fn = (T)L.sil / ((T)L.alfli2 + xmsq) - fcnt;
-    } else {
-        if (zvsq <= ZSQBND) f4fn = ((T)L.siv_a3 + zvsq * (T)L.siv_b3) - (T)L.f4bnd;
-        else f4fn = (T)L.sil / ((T)L.alfli2 + xmsq) - (T)L.f4bnd;
-    }
-    return f4fn;
-}
 
 __device__ inline void f4_line_setup(F4Line &o, const Job &J, const R4Blk &B, double vnu, double sp,
                                      double spp, double alfa, double epp, int co2) {
@@ -845,16 +827,23 @@ __device__ inline void f4_candidates(const Job &J, const R4Blk &B, const Shrunk 
 }
 
 // Per-(line, block) F4 records: the CNVF4Q per-line quantities and the block's
-// point range for the line, computed once; the gather kernels read them with
-// wave-uniform (scalar) loads.
+// point range for the line, computed once.  The line shape is written branch-free:
+//   base(x)  = (c2 - x^2 rb) f4bnd        (c2 = 2, rb = 1/BNDSQ for CO2 lines; 1, 0 otherwise)
+//   shift(f) = f + x dptrat f             (dptrat = 0 when the line has no shift)
+// which reproduces CNVF4Q's CO2 and other-molecule loops (clblm_ODLAY_LineF4.f90:388-462).
 struct alignas(16) F4Rec32 {
-    float d, dm, sil, siv_a3, siv_b3, f4bnd, alfli2, rec_alfvi2, dptrat;
-    int jc, jcm, jmin, jmax, jminm, jmaxm, flags;
+    float d, sil, alfli2, rec_alfvi2;  // x(j) = (j - jc) dvr4 + d, FP64-anchored
+    float siv_a3, siv_b3, b0, b1;      // base(x) = b0 + b1 x^2
+    float dptrat, dm;
+    int jc, jcm;
+    int jmin, jmax, jminm, jmaxm;
 };
 struct alignas(16) F4Rec64 {
-    double xnui, xnum, sil, siv_a3, siv_b3, f4bnd, alfli2, rec_alfvi2, dptrat;
-    int jmin, jmax, jminm, jmaxm, flags, pad0, pad1, pad2;
+    double xnui, xnum, sil, alfli2, rec_alfvi2, siv_a3, siv_b3, f4bnd, c2, rb, dptrat;
+    int jmin, jmax, jminm, jmaxm;
+    double pad[3];
 };
+static_assert(sizeof(F4Rec32) == 64 && sizeof(F4Rec64) == 128, "F4 record layout");
 
 // Record offsets of each block (one lane per layer walks its blocks in order).
 __global__ void k_f4recoff(const Job *jobs, int nlay, R4Blk *blks, int *err) {
@@ -874,16 +863,24 @@ template <typename R>
 __device__ inline void f4_fill(R &r, const F4Line &d, const Job &J);
 template <>
 __device__ inline void f4_fill<F4Rec64>(F4Rec64 &r, const F4Line &d, const Job &) {
-    r.xnui = d.xnui; r.xnum = d.xnum; r.sil = d.sil; r.siv_a3 = d.siv_a3; r.siv_b3 = d.siv_b3;
-    r.f4bnd = d.f4bnd; r.alfli2 = d.alfli2; r.rec_alfvi2 = d.rec_alfvi2; r.dptrat = d.dptrat;
-    r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm; r.flags = d.flags;
-    r.pad0 = r.pad1 = r.pad2 = 0;
+    const bool co2 = (d.flags & 1) != 0;
+    r.xnui = d.xnui; r.xnum = d.xnum; r.sil = d.sil; r.alfli2 = d.alfli2; r.rec_alfvi2 = d.rec_alfvi2;
+    r.siv_a3 = d.siv_a3; r.siv_b3 = d.siv_b3; r.f4bnd = d.f4bnd;
+    r.c2 = co2 ? 2. : 1.;
+    r.rb = co2 ? 1. / 625. : 0.;
+    r.dptrat = (d.flags & 2) ? 0. : d.dptrat;
+    r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0.;
 }
 template <>
 __device__ inline void f4_fill<F4Rec32>(F4Rec32 &r, const F4Line &d, const Job &J) {
-    r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm; r.flags = d.flags;
-    r.sil = (float)d.sil; r.siv_a3 = (float)d.siv_a3; r.siv_b3 = (float)d.siv_b3; r.f4bnd = (float)d.f4bnd;
-    r.alfli2 = (float)d.alfli2; r.rec_alfvi2 = (float)d.rec_alfvi2; r.dptrat = (float)d.dptrat;
+    const bool co2 = (d.flags & 1) != 0;
+    r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm;
+    r.sil = (float)d.sil; r.alfli2 = (float)d.alfli2; r.rec_alfvi2 = (float)d.rec_alfvi2;
+    r.siv_a3 = (float)d.siv_a3; r.siv_b3 = (float)d.siv_b3;
+    r.b0 = (float)(co2 ? 2. * d.f4bnd : d.f4bnd);
+    r.b1 = (float)(co2 ? -d.f4bnd / 625. : 0.);
+    r.dptrat = (d.flags & 2) ? 0.f : (float)d.dptrat;
     // nearest grid point and the FP64 offset to it: x(j) = (j-1)*dvr4 - xnui = (j-jc)*dvr4 + d
     if (d.jmax >= d.jmin) {
         int jc = (int)floor(d.xnui / J.dvr4 + 0.5) + 1;
@@ -911,105 +908,154 @@ __global__ void k_f4setup(const Job *jobs, const R4Blk *blks, int64_t nblk, Shru
     }
 }
 
-__device__ __forceinline__ double f4_eval64(const F4Rec64 &l, double xm) {
-    const double ZSQBND = 4096.;
-    const double r_bndsq = 1. / 625.;
-    double xmsq = xm * xm;
-    double zvsq = xmsq * l.rec_alfvi2;
-    double base = (l.flags & 1) ? (2. - (xmsq * r_bndsq)) * l.f4bnd : l.f4bnd;
-    double f = (zvsq <= ZSQBND) ? (l.siv_a3 + zvsq * l.siv_b3) - base : l.sil / (l.alfli2 + xmsq) - base;
-    if (!(l.flags & 2)) f = f + xm * l.dptrat * f;
-    return f;
+// Quotient through the hardware reciprocal refined by two Newton-Raphson steps
+// (r <- r + r (1 - b r)); within 2 ulp of a / b for the normal, positive b used here.
+__device__ __forceinline__ double div_nr(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.);
+    r = fma(e, r, r);
+    e = fma(-b, r, 1.);
+    r = fma(e, r, r);
+    return a * r;
 }
 
-// FP64 gather: one lane per R4 point, lines in the reference's order (bit-for-bit
-// sums apart from the direct grid offset), every wave walks the tile's records.
-__global__ void __launch_bounds__(256) k_f4conv64(const Job *jobs, const R4Blk *blks, const int2 *tiles,
-                                                  Shrunk sh, const F4Rec64 *__restrict__ recs, double *R4,
-                                                  unsigned long long *work) {
-    int2 t = tiles[blockIdx.x];
-    const R4Blk &B = blks[t.x];
-    const Job &J = jobs[B.layer];
-    int j0 = t.y * 256 + 1, j = j0 + threadIdx.x;
-    int jl = min(j0 + 255, B.nptr4);
-    double acc = 0.;
-    unsigned long long nev = 0;
-    if (j0 <= B.nptr4) {
-        int64_t c0, c1;
-        f4_candidates(J, B, sh, j0, jl, &c0, &c1);
-        const double xj = (double)(j - 1) * J.dvr4;
-        const F4Rec64 *r = recs + B.rec - B.lo;
-#pragma unroll 2
-        for (int64_t i = c0; i < c1; i++) {
-            const F4Rec64 &l = r[i];
-            if (j >= l.jmin && j <= l.jmax) {
-                acc = acc + f4_eval64(l, xj - l.xnui);
-                nev++;
-            }
-            if (j >= l.jminm && j <= l.jmaxm) {
-                F4Rec64 m = l;
-                m.dptrat = -l.dptrat;
-                acc = acc + f4_eval64(m, xj - l.xnum);
-            }
-        }
-    }
-    if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? acc : 0.;
-    for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
-    if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work + 1, nev);
+__device__ __forceinline__ double f4_eval64(const F4Rec64 &l, double xm, double dptrat) {
+    double xmsq = xm * xm;
+    double zvsq = xmsq * l.rec_alfvi2;
+    double base = (l.c2 - xmsq * l.rb) * l.f4bnd;
+    double q = l.siv_a3 + zvsq * l.siv_b3;
+    double lz = div_nr(l.sil, l.alfli2 + xmsq);
+    double f = ((zvsq <= 4096.) ? q : lz) - base;
+    return f + xm * dptrat * f;
 }
 
 __device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dptrat) {
-    const float ZSQBND = 4096.f;
-    const float r_bndsq = 1.f / 625.f;
     float xmsq = xm * xm;
     float zvsq = xmsq * l.rec_alfvi2;
-    float base = (l.flags & 1) ? (2.f - xmsq * r_bndsq) * l.f4bnd : l.f4bnd;
     float q = __fmaf_rn(zvsq, l.siv_b3, l.siv_a3);
     float lz = l.sil * __builtin_amdgcn_rcpf(l.alfli2 + xmsq);
-    float f = ((zvsq <= ZSQBND) ? q : lz) - base;
-    if (!(l.flags & 2)) f = __fmaf_rn(xm * dptrat, f, f);
-    return f;
+    float f = ((zvsq <= 4096.f) ? q : lz) - __fmaf_rn(xmsq, l.b1, l.b0);
+    return __fmaf_rn(xm * dptrat, f, f);
 }
 
-// FP32-compensated gather: offsets from the FP64 per-record anchor, Kahan sum per point.
-__global__ void __launch_bounds__(256) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles,
-                                                  Shrunk sh, const F4Rec32 *__restrict__ recs, double *R4,
-                                                  unsigned long long *work) {
-    int2 t = tiles[blockIdx.x];
+// F4 gather: one wave per tile of F4PTS consecutive R4 points, F4K points per lane
+// (j = j0 + lane + 64 k), lines walked in the reference's order so each point's sum
+// has the reference's summation order.  A record is read once (wave-uniform loads)
+// and used for F4K points; records that cover the whole tile skip the per-point
+// range test.
+constexpr int F4K = 4, F4PTS = 64 * F4K;
+
+__device__ __forceinline__ unsigned long long f4_count(int jmin, int jmax, int j0, int j1) {
+    int a = jmin > j0 ? jmin : j0, b = jmax < j1 ? jmax : j1;
+    return b >= a ? (unsigned long long)(b - a + 1) : 0ull;
+}
+
+__global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
+                                                 const F4Rec64 *__restrict__ recs, double *R4,
+                                                 unsigned long long *work) {
+    const int2 t = tiles[blockIdx.x];
     const R4Blk &B = blks[t.x];
     const Job &J = jobs[B.layer];
-    int j0 = t.y * 256 + 1, j = j0 + threadIdx.x;
-    int jl = min(j0 + 255, B.nptr4);
-    float hi = 0.f, lo = 0.f;
+    const int lane = threadIdx.x;
+    const int j0 = t.y * F4PTS + 1, j1 = j0 + F4PTS - 1;
+    double acc[F4K], xj[F4K];
+#pragma unroll
+    for (int k = 0; k < F4K; k++) {
+        acc[k] = 0.;
+        xj[k] = (double)(j0 + lane + 64 * k - 1) * J.dvr4;
+    }
     unsigned long long nev = 0;
-    const float dvr4f = (float)J.dvr4;
     if (j0 <= B.nptr4) {
         int64_t c0, c1;
-        f4_candidates(J, B, sh, j0, jl, &c0, &c1);
-        const F4Rec32 *r = recs + B.rec - B.lo;
-#pragma unroll 2
+        f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
+        const F4Rec64 *r = recs + B.rec - B.lo;
         for (int64_t i = c0; i < c1; i++) {
-            const F4Rec32 &l = r[i];
-            if (j >= l.jmin && j <= l.jmax) {
-                float xm = __fmaf_rn((float)(j - l.jc), dvr4f, l.d);
-                float y = f4_eval32(l, xm, l.dptrat) - lo;
-                float tt = hi + y;
-                lo = (tt - hi) - y;
-                hi = tt;
-                nev++;
+            const F4Rec64 l = r[i];
+            if (l.jmin <= l.jmax) {
+                nev += f4_count(l.jmin, l.jmax, j0, j1);
+                if (l.jmin <= j0 && l.jmax >= j1) {
+#pragma unroll
+                    for (int k = 0; k < F4K; k++) acc[k] += f4_eval64(l, xj[k] - l.xnui, l.dptrat);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < F4K; k++) {
+                        const int j = j0 + lane + 64 * k;
+                        double f = f4_eval64(l, xj[k] - l.xnui, l.dptrat);
+                        acc[k] += (j >= l.jmin && j <= l.jmax) ? f : 0.;
+                    }
+                }
             }
-            if (j >= l.jminm && j <= l.jmaxm) {
-                float xm = __fmaf_rn((float)(j - l.jcm), dvr4f, l.dm);
-                float y = f4_eval32(l, xm, -l.dptrat) - lo;
-                float tt = hi + y;
-                lo = (tt - hi) - y;
-                hi = tt;
+            if (l.jminm <= l.jmaxm) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
+#pragma unroll
+                for (int k = 0; k < F4K; k++) {
+                    const int j = j0 + lane + 64 * k;
+                    double f = f4_eval64(l, xj[k] - l.xnum, -l.dptrat);
+                    acc[k] += (j >= l.jminm && j <= l.jmaxm) ? f : 0.;
+                }
             }
         }
     }
-    if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? (double)hi - (double)lo : 0.;
-    for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
-    if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work + 1, nev);
+#pragma unroll
+    for (int k = 0; k < F4K; k++) {
+        const int j = j0 + lane + 64 * k;
+        if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? acc[k] : 0.;
+    }
+    if (lane == 0 && nev) atomicAdd(work + 1, nev);
+}
+
+// FP32 line shapes (offsets from the FP64 per-record anchor), FP64 sums per point.
+__global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
+                                                 const F4Rec32 *__restrict__ recs, double *R4,
+                                                 unsigned long long *work) {
+    const int2 t = tiles[blockIdx.x];
+    const R4Blk &B = blks[t.x];
+    const Job &J = jobs[B.layer];
+    const int lane = threadIdx.x;
+    const int j0 = t.y * F4PTS + 1, j1 = j0 + F4PTS - 1;
+    const float dvr4f = (float)J.dvr4;
+    double acc[F4K];
+#pragma unroll
+    for (int k = 0; k < F4K; k++) acc[k] = 0.;
+    unsigned long long nev = 0;
+    if (j0 <= B.nptr4) {
+        int64_t c0, c1;
+        f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
+        const F4Rec32 *r = recs + B.rec - B.lo;
+        for (int64_t i = c0; i < c1; i++) {
+            const F4Rec32 l = r[i];
+            if (l.jmin <= l.jmax) {
+                nev += f4_count(l.jmin, l.jmax, j0, j1);
+                const float fl = (float)(j0 + lane - l.jc);
+                if (l.jmin <= j0 && l.jmax >= j1) {
+#pragma unroll
+                    for (int k = 0; k < F4K; k++)
+                        acc[k] += (double)f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < F4K; k++) {
+                        const int j = j0 + lane + 64 * k;
+                        float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
+                        acc[k] += (double)((j >= l.jmin && j <= l.jmax) ? f : 0.f);
+                    }
+                }
+            }
+            if (l.jminm <= l.jmaxm) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
+                const float fl = (float)(j0 + lane - l.jcm);
+#pragma unroll
+                for (int k = 0; k < F4K; k++) {
+                    const int j = j0 + lane + 64 * k;
+                    float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.dm), -l.dptrat);
+                    acc[k] += (double)((j >= l.jminm && j <= l.jmaxm) ? f : 0.f);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < F4K; k++) {
+        const int j = j0 + lane + 64 * k;
+        if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? acc[k] : 0.;
+    }
+    if (lane == 0 && nev) atomicAdd(work + 1, nev);
 }
 
 // LBLF4Q radiation term on R4 when JRAD = 1 (clblm_ODLAY_LineF4.f90:142-177).
@@ -1672,7 +1718,7 @@ static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, 
             B.tot_r4 += LIMOUT4 + 2;
             int bi = (int)B.blks.size();
             B.blks.push_back(b);
-            for (int t = 0; t < (LIMOUT4 + 2 + 255) / 256; t++) B.tiles.push_back(make_int2(bi, t));
+            for (int t = 0; t < (LIMOUT4 + 2 + F4PTS - 1) / F4PTS; t++) B.tiles.push_back(make_int2(bi, t));
         }
         B.max_line = std::max(B.max_line, J.line_n);
         B.max_f4 = std::max(B.max_f4, J.f4_n);
@@ -1824,10 +1870,10 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     }
     HIPCHK(hipEventRecord(C->ev[5], st));
     if (precision == CLBLM_FP64)
-        hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh,
+        hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec64>(base, o_rec), d_R4, C->d_work);
     else
-        hipLaunchKernelGGL(k_f4conv32, dim3((unsigned)B.tiles.size()), dim3(256), 0, st, d_jobs, d_blk, d_tiles, sh,
+        hipLaunchKernelGGL(k_f4conv32, dim3((unsigned)B.tiles.size()), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec32>(base, o_rec), d_R4, C->d_work);
     HIPCHK(hipEventRecord(C->ev[6], st));
     if (B.any_jrad1) hipLaunchKernelGGL(k_r4rad, dim3(cdiv(nblk, 64)), dim3(64), 0, st, d_jobs, d_blk, nblk, d_R4);
