@@ -26,6 +26,7 @@
 
 #include "../../include/clblm_hip.h"
 #include "clblm_common.h"
+#include "seq_ramp.h"
 
 using namespace clblm;
 
@@ -944,6 +945,7 @@ __device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dpt
 // and used for F4K points; records that cover the whole tile skip the per-point
 // range test.
 constexpr int F4K = 4, F4PTS = 64 * F4K;
+constexpr int RAMP_CAP = 256;  // ramp pieces per panel held in LDS
 
 __device__ __forceinline__ unsigned long long f4_count(int jmin, int jmax, int j0, int j1) {
     int a = jmin > j0 ? jmin : j0, b = jmax < j1 ? jmax : j1;
@@ -1058,26 +1060,57 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
     if (lane == 0 && nev) atomicAdd(work + 1, nev);
 }
 
-// LBLF4Q radiation term on R4 when JRAD = 1 (clblm_ODLAY_LineF4.f90:142-177).
-__global__ void k_r4rad(const Job *jobs, const R4Blk *blks, int64_t nblk, double *R4) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nblk) return;
-    const R4Blk &B = blks[q];
+// LBLF4Q radiation term on R4 when JRAD = 1 (clblm_ODLAY_LineF4.f90:142-177): one
+// workgroup per R4 block; one lane turns the RADFNI segments into exact ramp pieces,
+// all lanes apply them.
+__global__ void __launch_bounds__(256) k_r4rad(const Job *jobs, const R4Blk *blks, double *R4) {
+    __shared__ double rbase[RAMP_CAP], rstep[RAMP_CAP];
+    __shared__ int rk[RAMP_CAP + 1], nramp;
+    const R4Blk &B = blks[blockIdx.x];
     const Job &J = jobs[B.layer];
     if (J.jrad != 1) return;
-    double xkt = 1. / (RADCN2 / J.tave);
-    double vitst = B.v1r4 - J.dvr4, rdlast = -1., rdel;
-    int n2 = 0;
     double *r = R4 + B.r4_off - 1;
-    do {
-        int n1 = n2 + 1;
-        double vi = B.v1r4 + J.dvr4 * (double)(n1 - 1);
-        double radvi = radfni(vi, J.dvr4, xkt, &vitst, &rdel, &rdlast);
-        n2 = (int)((vitst - B.v1r4) / J.dvr4 + 0.001);
-        if (n2 > B.nptr4) n2 = B.nptr4;
-        double tt = radvi;
-        for (int i = n1; i <= n2; i++) { r[i] = r[i] * tt; tt = tt + rdel; }
-    } while (n2 < B.nptr4);
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        const double xkt = 1. / (RADCN2 / J.tave);
+        double vitst = B.v1r4 - J.dvr4, rdlast = -1., rdel;
+        int n2 = 0, np = 0;
+        do {
+            int n1 = n2 + 1;
+            double vi = B.v1r4 + J.dvr4 * (double)(n1 - 1);
+            double radvi = radfni(vi, J.dvr4, xkt, &vitst, &rdel, &rdlast);
+            n2 = (int)((vitst - B.v1r4) / J.dvr4 + 0.001);
+            if (n2 > B.nptr4) n2 = B.nptr4;
+            int c = seqramp::ramp_pieces(radvi, rdel, n2 - n1 + 1, RAMP_CAP - np, [&](int k, double b, double st) {
+                rk[np] = n1 + k; rbase[np] = b; rstep[np] = st; np++;
+            });
+            if (c < 0) { np = -1; break; }
+        } while (n2 < B.nptr4);
+        if (np < 0) {  // more pieces than the table holds: the reference's loop, in place
+            vitst = B.v1r4 - J.dvr4;
+            rdlast = -1.;
+            n2 = 0;
+            do {
+                int n1 = n2 + 1;
+                double vi = B.v1r4 + J.dvr4 * (double)(n1 - 1);
+                double tt = radfni(vi, J.dvr4, xkt, &vitst, &rdel, &rdlast);
+                n2 = (int)((vitst - B.v1r4) / J.dvr4 + 0.001);
+                if (n2 > B.nptr4) n2 = B.nptr4;
+                for (int i = n1; i <= n2; i++) { r[i] = r[i] * tt; tt = tt + rdel; }
+            } while (n2 < B.nptr4);
+            np = 0;
+        }
+        rk[np] = B.nptr4 + 1;
+        nramp = np;
+    }
+    __syncthreads();
+    const int np = nramp;
+    if (np == 0) return;
+    int pc = 0;
+    for (int i = 1 + tid; i <= B.nptr4; i += blockDim.x) {
+        while (rk[pc + 1] <= i) pc++;
+        r[i] = r[i] * (rbase[pc] + (double)(i - rk[pc]) * rstep[pc]);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -1124,7 +1157,9 @@ template <typename TA>
 __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *panels, const R4Blk *blks,
                                                const TA *A, const double *R4, const double *absrb,
                                                double *out, double *nat) {
-    __shared__ double r3[168], r3e[8], r2[640], r1[2448], fac[2448];
+    __shared__ double r3[168], r3e[8], r2[640], r1[2448];
+    __shared__ double rbase[RAMP_CAP], rstep[RAMP_CAP];
+    __shared__ int rk[RAMP_CAP + 1], nramp;
     const int64_t q = blockIdx.x;
     const Panel &P = panels[q];
     const Job &J = jobs[P.layer];
@@ -1204,29 +1239,56 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
         r1[Jg + 3] = r1[Jg + 3] + X03 * r2[J2 - 1] + X02 * r2[J2] + X01 * r2[J2 + 1] + X00 * r2[J2 + 2];
     }
     __syncthreads();
+    // Radiation term (panelOut, clblm_ODLAY.f90:2447-2483): RADFNI segments whose
+    // sequential ramps are turned into exact linear pieces by one lane, then applied
+    // by all lanes at the store.
     if (J.jrad == 0) {
         if (tid == 0) {
-            double xkt = J.tave / RADCN2;
-            double DV = J.dv;
-            double v1p = P.vft + (double)(NLO - 1) * DV;
-            double vitst = v1p - DV, rdlast = -1., rdel;
-            int n2 = NLO - 1;
+            const double xkt = J.tave / RADCN2, DV = J.dv;
+            double vitst = P.vft + (double)(NLO - 1) * DV - DV, rdlast = -1., rdel;
+            int n2 = NLO - 1, np = 0;
             do {
                 int n1 = n2 + 1;
                 double vi = P.vft + (double)(n1 - 1) * DV;
                 double radvi = radfni(vi, DV, xkt, &vitst, &rdel, &rdlast);
                 n2 = (int)((vitst - P.vft) / DV + 0.001);
                 if (n2 > NHI) n2 = NHI;
-                double tt = radvi;
-                for (int i = n1; i <= n2; i++) { fac[i] = tt; tt = tt + rdel; }
+                int c = seqramp::ramp_pieces(radvi, rdel, n2 - n1 + 1, RAMP_CAP - np,
+                                             [&](int k, double b, double st) {
+                                                 rk[np] = n1 + k; rbase[np] = b; rstep[np] = st; np++;
+                                             });
+                if (c < 0) { np = -1; break; }
             } while (n2 < NHI);
+            if (np < 0) {  // more pieces than the table holds: the reference's loop, in place
+                vitst = P.vft + (double)(NLO - 1) * DV - DV;
+                rdlast = -1.;
+                n2 = NLO - 1;
+                do {
+                    int n1 = n2 + 1;
+                    double vi = P.vft + (double)(n1 - 1) * DV;
+                    double tt = radfni(vi, DV, xkt, &vitst, &rdel, &rdlast);
+                    n2 = (int)((vitst - P.vft) / DV + 0.001);
+                    if (n2 > NHI) n2 = NHI;
+                    for (int i = n1; i <= n2; i++) { r1[i] = r1[i] * tt; tt = tt + rdel; }
+                } while (n2 < NHI);
+                np = 0;
+            }
+            rk[np] = NHI + 1;
+            nramp = np;
         }
         __syncthreads();
-        for (int i = NLO + tid; i <= NHI; i += blockDim.x) r1[i] = r1[i] * fac[i];
-        __syncthreads();
     }
+    const int np = J.jrad == 0 ? nramp : 0;
     double *o = (J.dvout > 0. ? nat : out) + P.out_off - NLO;
-    for (int i = NLO + tid; i <= NHI; i += blockDim.x) o[i] = r1[i];
+    int pc = 0;
+    for (int i = NLO + tid; i <= NHI; i += blockDim.x) {
+        double v = r1[i];
+        if (np > 0) {
+            while (rk[pc + 1] <= i) pc++;
+            v = v * (rbase[pc] + (double)(i - rk[pc]) * rstep[pc]);
+        }
+        o[i] = v;
+    }
 }
 
 // interpSpectrum_array2array (clblm_Spectrum.f90:326-590).  For the n/(n+1) ratios
@@ -1876,7 +1938,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         hipLaunchKernelGGL(k_f4conv32, dim3((unsigned)B.tiles.size()), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec32>(base, o_rec), d_R4, C->d_work);
     HIPCHK(hipEventRecord(C->ev[6], st));
-    if (B.any_jrad1) hipLaunchKernelGGL(k_r4rad, dim3(cdiv(nblk, 64)), dim3(64), 0, st, d_jobs, d_blk, nblk, d_R4);
+    if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, d_R4);
     HIPCHK(hipEventRecord(C->ev[3], st));
     hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
                        d_abs, d_out, d_nat);

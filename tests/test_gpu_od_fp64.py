@@ -53,6 +53,24 @@ def test_profile_layer_seven_species(ctx, k):
                                                                           _cmp(od, ref)))
 
 
+@pytest.mark.parametrize("k,jrad", [(3, 1), (30, 1), (50, 1), (12, -1)])
+def test_radiation_modes(ctx, k, jrad):
+    """JRAD = 1 (radiation term in the line strengths and on R4, V1/kT >= 5 at
+    2000 cm-1) and JRAD = -1 (no radiation term)."""
+    from oracle.py import Oracle
+    lines = W.profile_lines(30000, 1900.0, 2200.0, seed=1001)
+    lay = W.atmosphere()[k]
+    dv = W.calculate_dv(2040.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+    h = ctx.upload_lines(lines)
+    ctx.jrad = jrad
+    od, _ = ctx.odlay(h, lay, 2000.0, 2080.0, dv)
+    orc = Oracle(lines)
+    orc.jrad = jrad
+    ref, _ = orc.odlay(lay, 2000.0, 2080.0, dv)
+    assert ctx.jrad == orc.jrad == jrad
+    print("jrad %d layer %d: %d points, max rel err %.3e" % (jrad, k, od.size, _cmp(od, ref)))
+
+
 def test_continuum_and_regrid(ctx):
     from oracle.py import Oracle
     lines = W.profile_lines(20000, 500.0, 800.0, seed=1001)
