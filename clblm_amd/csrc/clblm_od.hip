@@ -7,7 +7,7 @@
 //   k_prep   LNCORQ per line and the first panel each line fits (CNVFNQ JMAX1 test)
 //   k_home   prefix max of that panel over the line order = the panel that convolves it
 //   k_ranges line range homed in each panel
-//   k_conv   pass 1: each panel's own R1/R2/R3 contributions (FP64 scatter or FP32 gather)
+//   k_conv   pass 1: each panel's own R1/R2/R3 contributions (lane per line, FP64 LDS sums)
 //   k_f4*    LINF4Q per line, SHRINQ chunk simulation, CNVF4Q gather on R4 blocks
 //   k_pass2  pass 2: panel assembly with the carry of the previous panel, XINT R4->R3,
 //            R3->R2->R1 Lagrange interpolation, RADFNI radiation term, store

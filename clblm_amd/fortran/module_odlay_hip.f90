@@ -1,8 +1,11 @@
-! Drop-in Module_ODLAY: the reference's per-layer OD entry point ODLAY_nonStruct
-! (src/clblm_src/clblm_ODLAY.f90:129-176) with the same dummy arguments, computed
-! by the HIP library through clblm_hip.  Compile it in place of clblm_ODLAY.f90,
-! against the reference's Module_ConstParam and Module_Config; the callers
-! (layerMerge_*, pathOD_up_dn_sun, drv_LevelTemp, drv_LevelConc) are unchanged.
+! Drop-in Module_ODLAY: the reference's generic ODLAY (src/clblm_src/clblm_ODLAY.f90:12-18)
+! with both of its forms, ODLAY_struct (:26-124, the form every caller uses) and
+! ODLAY_nonStruct (:129-176), with the same dummy arguments, computed by the HIP
+! library through clblm_hip.  Compile it with -cpp in place of clblm_ODLAY.f90,
+! against the reference's modules; the callers (layerMerge_*, pathOD_up_dn_sun,
+! drv_LevelTemp, drv_LevelConc) are unchanged.  ODLAY_struct needs the reference's
+! Module_AtmPath and Module_DV, which come with its netCDF-dependent scene modules;
+! -DCLBLM_HIP_NO_STRUCT leaves it out for builds without them.
 ! The line list is uploaded once with clblm_hip_lines (the TAPE3 reader is not
 ! part of this library).  Supported: molecules 1-7, no NLTE, no cross-sections,
 ! no MT_CKD continuum (contnmOff must be set), no line coupling.
@@ -13,10 +16,48 @@ module Module_ODLAY
    private
    public :: ODLAY
    interface ODLAY
+#ifndef CLBLM_HIP_NO_STRUCT
+      module procedure ODLAY_struct
+#endif
       module procedure ODLAY_nonStruct
    end interface
    logical, save :: firstPassJRAD = .true.
 contains
+#ifndef CLBLM_HIP_NO_STRUCT
+   ! Unpacks the layer, grid and DV records as the reference's wrapper does and
+   ! returns the OD as a CLBLM_Spectrum on the grid actually used (V1, DVout).
+   subroutine ODLAY_struct(aLayer, odCtrl, spectGrid, dvCtrl, layOD, NLTE, layNLTEEmisFac, JacMolName)
+      use Module_Config, only: CLBLM_OD_Ctrl, CLBLM_DV_Ctrl
+      use Module_DV, only: CLBLM_SpectGrid
+      use Module_Spectrum, only: CLBLM_Spectrum, CLBLM_Spectrum_init
+      use Module_AtmPath, only: CLBLM_Layer
+      type(CLBLM_Layer), intent(in) :: aLayer
+      type(CLBLM_OD_Ctrl), intent(in) :: odCtrl
+      type(CLBLM_SpectGrid), intent(in) :: spectGrid
+      type(CLBLM_DV_Ctrl), intent(in) :: dvCtrl
+      type(CLBLM_Spectrum), intent(out) :: layOD
+      logical, optional, intent(in) :: NLTE
+      type(CLBLM_Spectrum), optional, intent(out) :: layNLTEEmisFac
+      character(*), optional, intent(in) :: JacMolName
+      integer :: nSamp, L
+      real, allocatable :: monoOD(:)
+      real :: dvo
+      if (present(NLTE)) then
+         if (NLTE) stop '--- ODLAY(): NLTE is not supported by the HIP path'
+      end if
+      ! JacMolName only selects the continuum derivative (clblm_ODLAY.f90:438), and the
+      ! continuum is not part of this path
+      L = aLayer%layNo
+      dvo = spectGrid%DVout
+      call ODLAY_nonStruct(aLayer%nMol, aLayer%molID, aLayer%Pave, aLayer%Tave, aLayer%W, aLayer%Wtot, &
+                           aLayer%Zbot, aLayer%Ztop, odCtrl, spectGrid%V1, spectGrid%V2, &
+                           spectGrid%DVnormal(L), spectGrid%DVnarrow(L), dvCtrl%SAMPLE, dvCtrl%ALFAL0, &
+                           0, L, nSamp, monoOD, DVout=dvo)
+      call CLBLM_Spectrum_init(layOD, spectGrid%V1, dvo, nSamp)
+      call move_alloc(monoOD, layOD%spect)
+   end subroutine
+#endif
+
    subroutine ODLAY_nonStruct(nMol, molID, Pav, Tav, W, Wtot, Zbot, Ztop, odCtrl, &
                               V1, V2, DVnormal, DVnarrow, &
                               SAMPLE, ALFAL0, IPATH, LayNo, &
