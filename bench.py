@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Benchmark of the layer optical-depth line sum on one MI355X.
+"""Benchmark of the layer optical-depth line sum and its Jacobian pass on one MI355X.
 
 Workload (the full-profile OD case): 60 layers with log-spaced pressure from 1013
 to 0.1 hPa, US-standard-like temperatures, 7 species (HITRAN molecules 1-7),
@@ -10,7 +10,12 @@ in HBM.
 
 Metric: Voigt line-evals/s, E / step time, where
 E = sum over layers and lines of the number of layer-grid points within
-+-25 cm-1 of the shifted line centre (a closed form of the line list and grids).
++-25 cm-1 of the line centre (a closed form of the line list and grids).
+`value` is the FP32-compensated path; the plain FP64 GPU path is timed beside it
+("fp64_gpu").  "jacobian" times the upwelling-radiance Jacobian pass of the same
+atmosphere (dR/dT for 60 layers and dR/dln q for 60 layers x 7 species over
+500-2500 cm-1 on the finest layer grid, in 100 cm-1 blocks with layer ODs
+recomputed per block).
 
 Prints one JSON line.
 """
@@ -30,8 +35,8 @@ METRIC = "Voigt line-evals/s & Jacobian spectra/s vs FP64 GPU path at ≤1e-5 OD
 V1, V2 = 500.0, 2500.0
 # MI355X datasheet peaks (vector FP64 and FP32 without packed math are the same issue rate)
 PEAK = {"fp64": 78.6, "fp32": 157.3, "hbm": 8000.0}
-FLOP_CONV = 13   # per R1/R2/R3 evaluation: 3 table-argument adds, 5 products, 5 sums
-FLOP_F4 = 7      # per (line, R4 point): offset, square, scaled square, 3 for the shape, 1 sum
+FLOP_CONV = 17   # per R1 point of a line (R1/R2/R3 table terms), the per-unit figure of the path's cost model
+FLOP_F4 = 10     # per (shrunk line, R4 point) of the F4 wing sum, divide counted as 1
 
 
 def workload():
@@ -78,18 +83,78 @@ def cpu_baseline(lines, layers, dvs, conts, per_layer_evals, budget_s=20.0):
     return e_tot / t_tot, used, t_tot
 
 
+def run_od(ctx, h, layers, pars, steps, warmup):
+    """Time `steps` batched layer-OD passes after `warmup` untimed ones."""
+    import torch
+    ctx.jrad = 1
+    out, off = ctx.od_batch(h, layers, pars)
+    for _ in range(max(0, warmup - 1)):
+        ctx.jrad = 1
+        ctx.od_batch(h, layers, pars, out=out)
+    torch.cuda.synchronize()
+    ctx.work(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.jrad = 1
+        ctx.od_batch(h, layers, pars, out=out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.sync()   # raises any error found on the device during the timed batches
+    w = ctx.work(reset=False)
+    tm = ctx.timing(reset=True)
+    return dt, w, tm, off
+
+
+def run_jacobian(ctx, h, layers, dvs, precision, block=100.0):
+    """Radiance + dR/dT_l + dR/dln q_l,m over V1..V2 in spectral blocks; returns
+    (seconds, spectra, points per spectrum, blocks)."""
+    import torch
+    from clblm_amd import workloads as W
+    from clblm_amd.jacobian import UpwellingJacobian
+    t_lev = W.level_temperatures(len(layers), 0.1)
+    jac = UpwellingJacobian(ctx, h, layers, t_lev, dvs, precision=precision,
+                            continuum=lambda l, a, b: W.continuum_absrb(l, a, b))
+    edges = np.arange(V1, V2 + 0.5 * block, block)
+    jac.block(edges[0], edges[1])          # warm-up
+    torch.cuda.synchronize()
+    npts = 0
+    t0 = time.perf_counter()
+    for a, b in zip(edges[:-1], edges[1:]):
+        r = jac.block(a, b)
+        npts += r["n"]
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ctx.sync()
+    spectra = len(layers) * (1 + len(jac.mols))
+    return t, spectra, npts, len(edges) - 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32c", choices=["fp64", "fp32c"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison run")
+    ap.add_argument("--no-jac", action="store_true", help="skip the Jacobian pass")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device copy of this many bytes 5 times (calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
-    import torch
+    if args.copy_bytes > 0:
+        import torch
+        n = args.copy_bytes // 8
+        x = torch.arange(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8, "launches": 5}))
+        return
     from clblm_amd.odlay import Context
     from clblm_amd._lib import FP32C, FP64
 
@@ -98,22 +163,8 @@ def main():
     prec = FP64 if args.precision == "fp64" else FP32C
     ctx = Context(0)
     h = ctx.upload_lines(lines)
-    pars = [Context.params(V1, V2, dv, precision=prec, continuum=c) for dv, c in zip(dvs, conts)]
-    ctx.jrad = 1
-    out, off = ctx.od_batch(h, layers, pars)
-    for _ in range(max(0, args.warmup - 1)):
-        ctx.jrad = 1
-        ctx.od_batch(h, layers, pars, out=out)
-    torch.cuda.synchronize()
-    ctx.work(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.jrad = 1
-        ctx.od_batch(h, layers, pars, out=out)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    w = ctx.work(reset=False)
-    tm = ctx.timing(reset=True)
+    mk = lambda p: [Context.params(V1, V2, dv, precision=p, continuum=c) for dv, c in zip(dvs, conts)]
+    dt, w, tm, off = run_od(ctx, h, layers, mk(prec), args.steps, args.warmup)
     value = E / dt
     conv_tf = w["conv_evals"] * FLOP_CONV / (w["ms_conv"] * 1e-3) / 1e12 if w["ms_conv"] > 0 else 0.0
     f4_tf = w["f4_evals"] * FLOP_F4 / (w["ms_f4"] * 1e-3) / 1e12 if w["ms_f4"] > 0 else 0.0
@@ -131,10 +182,20 @@ def main():
                    "precision": args.precision, "line_evals_per_step": E},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
                      "frac": ach / peak, "traffic": None,
-                     "evals": {"conv": w["conv_evals"], "f4": w["f4_evals"]},
+                     "flop_per_unit": {"k_conv": FLOP_CONV, "k_f4conv": FLOP_F4},
+                     "units_per_launch": {"k_conv": w["conv_evals"], "k_f4conv": w["f4_evals"]},
                      "ms": {"conv_kernel": w["ms_conv"], "f4_kernel": w["ms_f4"], "prep": tm["prep"],
                             "conv_stage": tm["conv"], "f4_stage": tm["f4"], "panel_stage": tm["panel"]}},
     }
+    if prec != FP64 and not args.no_fp64:
+        dt64, _, _, _ = run_od(ctx, h, layers, mk(FP64), args.steps, args.warmup)
+        res["fp64_gpu"] = {"value": E / dt64, "unit": "line-evals/s", "ms_per_step": dt64 * 1e3,
+                           "fp32c_over_fp64": dt64 / dt}
+    if not args.no_jac:
+        t, spectra, npts, nblk = run_jacobian(ctx, h, layers, dvs, prec)
+        res["jacobian"] = {"spectra_per_s": spectra / t, "points_per_s": spectra * npts / t, "seconds": t,
+                           "spectra": spectra, "points_per_spectrum": npts, "blocks": nblk,
+                           "precision": args.precision}
     if not args.no_cpu:
         v, used, t = cpu_baseline(lines, layers, dvs, conts, per)
         res["cpu_baseline"] = {"value": v, "unit": "line-evals/s", "cores": 1, "kind": "port",

@@ -12,7 +12,7 @@ MAXMOL = 47
 FP64, FP32C = 0, 1
 
 EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
-           "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch",
+           "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
            "clblm_rt_up", "clblm_timing", "clblm_work"]
 
 
@@ -64,6 +64,7 @@ def load():
                                 ctypes.POINTER(ctypes.c_double), P]
     lib.clblm_od_batch.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int32),
                                    P, P, P]
+    lib.clblm_sync.argtypes = [P]
     lib.clblm_rt_up.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                 P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                 P, P, P, P, P, P]

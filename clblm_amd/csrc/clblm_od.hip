@@ -616,7 +616,7 @@ __global__ void k_chunkG(const Job *jobs, F4Prep f, Shrunk sh) {
 // the buffer holds LIMIN lines, the new lines are shrunk, and the buffer is written
 // as one TAPE9 block once it holds LIMIN shrunk lines.  One thread per layer walks
 // the records with the precomputed chunk counts.
-__global__ void k_sim(const Job *jobs, int nlay, F4Prep f, int *err) {
+__global__ void k_sim(const Job *jobs, int nlay, F4Prep f) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlay) return;
     const Job &J = jobs[l];
@@ -633,8 +633,18 @@ __global__ void k_sim(const Job *jobs, int nlay, F4Prep f, int *err) {
         if (!eof && S + raw < LIMIN4) continue;
         int64_t L = r - start;
         if (raw > 0) {
-            if (L > LCAP) { atomicOr(err, 4); return; }
-            int g = f.G[(J.rec_off + start) * LCAP + L - 1];
+            int g;
+            if (L <= LCAP) {
+                g = f.G[(J.rec_off + start) * LCAP + L - 1];
+            } else {  // a chunk of few accepted lines over many records: count it here
+                int64_t k0 = (J.rec0 + start) * REC - J.f4_lo;
+                if (k0 < 0) k0 = 0;
+                int64_t kend = (J.rec0 + start + L) * REC - J.f4_lo;
+                if (kend > J.f4_n) kend = J.f4_n;
+                if (kend > stop + 1) kend = stop + 1;
+                g = 0;
+                shrinq_walk<false>(J, f, k0, kend, kend, 1, &g, Shrunk{}, 0);
+            }
             ch[nch++] = make_int4((int)start, (int)L, (int)(out + S), l);
             S += g;
         }
@@ -1509,6 +1519,21 @@ struct LineListH {
     LinesDev dev{};
 };
 
+struct Batch {
+    std::vector<Job> jobs;
+    std::vector<Panel> panels;
+    std::vector<R4Blk> blks;
+    std::vector<int2> tiles;
+    std::vector<double> absrb;
+    std::vector<int> stop0;
+    std::vector<int> regrid_lay;
+    std::vector<int64_t> regrid_first;
+    int64_t max_line = 0, max_f4 = 0, max_rec = 0, tot_line = 0, tot_f4 = 0, tot_rec = 0;
+    int64_t tot_a = 0, tot_r4 = 0, tot_nat = 0, tot_out = 0, tot_f4rec = 0;
+    int max_stride = 0;
+    bool any_jrad1 = false, any_regrid = false;
+};
+
 struct clblm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1520,12 +1545,84 @@ struct clblm_ctx {
     unsigned long long *d_work = nullptr;   // [conv evaluations, F4 evaluations]
     double *rt_state = nullptr;             // per-layer recursion states for the reverse sweep
     size_t rt_state_cap = 0;
-    hipEvent_t ev[8]{};
+    // Batches are asynchronous: each call takes the next slot of a ring, which holds
+    // its stage events and its device error word (copied to pinned host memory at
+    // the end of the call).  A slot is read back when it is reused, by clblm_sync,
+    // or by the timing queries; a device-detected error is returned by the next
+    // call on the context.
+    struct Slot {
+        hipEvent_t ev[7]{};
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    static constexpr int NSLOT = 4;
+    Slot slot[NSLOT];
+    int *h_err = nullptr;                   // pinned, NSLOT words
+    int64_t ncalls = 0;
+    hipStream_t last_stream = nullptr;
+    // plan uploads go through two alternating pinned buffers
+    char *pin[2] = {nullptr, nullptr};
+    size_t pin_cap[2] = {0, 0};
+    hipEvent_t pin_ev[2]{};
+    int pin_next = 0;
+    int deferred_rc = 0;
+    std::string deferred_msg;
+    // the sizing call and the run call of one batch share one plan
+    std::vector<char> plan_key;
+    Batch plan;
+    int32_t plan_jr = 0;
+    bool plan_valid = false;
     double ms_conv_k = 0, ms_f4_k = 0;
     double ms[4] = {0, 0, 0, 0};
     int64_t launches = 0;
     std::mutex mu;
 };
+
+// Reads back a finished slot: stage times and the device error word.
+static void finish_slot(clblm_ctx *C, int k, bool wait) {
+    clblm_ctx::Slot &S = C->slot[k];
+    if (!S.pending) return;
+    if (wait) (void)hipEventSynchronize(S.done);
+    else if (hipEventQuery(S.done) != hipSuccess) return;
+    S.pending = false;
+    float t01 = 0, t12 = 0, t23 = 0, t34 = 0, t56 = 0;
+    (void)hipEventElapsedTime(&t01, S.ev[0], S.ev[1]);
+    (void)hipEventElapsedTime(&t12, S.ev[1], S.ev[2]);
+    (void)hipEventElapsedTime(&t23, S.ev[2], S.ev[3]);
+    (void)hipEventElapsedTime(&t34, S.ev[3], S.ev[4]);
+    (void)hipEventElapsedTime(&t56, S.ev[5], S.ev[6]);
+    C->ms[0] += t01; C->ms[1] += t12; C->ms[2] += t23; C->ms[3] += t34;
+    C->ms_conv_k += t12;
+    C->ms_f4_k += t56;
+    C->launches++;
+    int herr = C->h_err[k];
+    if (herr && !C->deferred_rc) {
+        if (herr & 1) {
+            C->deferred_rc = CLBLM_ERR_UNSUPPORTED;
+            C->deferred_msg = "line coupling records (IFLG 1..3) are not supported";
+        } else {
+            C->deferred_rc = CLBLM_ERR_CAPACITY;
+            C->deferred_msg = "F4 record capacity exceeded";
+        }
+    }
+}
+
+// Reads back every finished slot in call order (all of them when wait is set).
+static void drain(clblm_ctx *C, bool wait) {
+    for (int64_t j = std::max<int64_t>(0, C->ncalls - clblm_ctx::NSLOT); j < C->ncalls; j++) {
+        int k = (int)(j % clblm_ctx::NSLOT);
+        finish_slot(C, k, wait);
+        if (C->slot[k].pending) break;
+    }
+}
+
+static int take_deferred(clblm_ctx *C) {
+    if (!C->deferred_rc) return 0;
+    int rc = set_err(C->deferred_rc, C->deferred_msg + " (reported by a batch enqueued earlier)");
+    C->deferred_rc = 0;
+    C->deferred_msg.clear();
+    return rc;
+}
 
 static thread_local int g_tls_dummy = 0;
 
@@ -1718,21 +1815,6 @@ static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od
     return 0;
 }
 
-struct Batch {
-    std::vector<Job> jobs;
-    std::vector<Panel> panels;
-    std::vector<R4Blk> blks;
-    std::vector<int2> tiles;
-    std::vector<double> absrb;
-    std::vector<int> stop0;
-    std::vector<int> regrid_lay;
-    std::vector<int64_t> regrid_first;
-    int64_t max_line = 0, max_f4 = 0, max_rec = 0, tot_line = 0, tot_f4 = 0, tot_rec = 0;
-    int64_t tot_a = 0, tot_r4 = 0, tot_nat = 0, tot_out = 0, tot_f4rec = 0;
-    int max_stride = 0;
-    bool any_jrad1 = false, any_regrid = false;
-};
-
 static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, const clblm_od_params *pars,
                        int32_t *jrad_state, Batch &B, int precision) {
     B = Batch{};
@@ -1848,19 +1930,52 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         o_rg = cv.take<int>(std::max<size_t>(1, rg_lay.size()));
         o_rgf = cv.take<int64_t>(nlay);
     }
+    // stream order with the previous batch (the arena is shared), then the slot
+    const int k = (int)(C->ncalls % clblm_ctx::NSLOT);
+    finish_slot(C, k, true);
+    if (C->last_stream && C->last_stream != st) {
+        const int kp = (int)((C->ncalls + clblm_ctx::NSLOT - 1) % clblm_ctx::NSLOT);
+        HIPCHK(hipStreamWaitEvent(st, C->slot[kp].done, 0));
+    }
+    clblm_ctx::Slot &S = C->slot[k];
+    if (cv.off > C->arena.cap) {  // growing frees the old arena: let every batch finish first
+        drain(C, true);
+        HIPCHK(hipDeviceSynchronize());
+    }
     if (C->arena.ensure(cv.off)) return set_err(CLBLM_ERR_HIP, "device scratch allocation failed");
     char *base = C->arena.p;
+    // host-built tables, staged in pinned memory: the header region [0, o_lp_d) as it
+    // lies in the arena, then the per-layer stop indices and the re-grid units
+    const size_t h_stop = (o_lp_d + 255) & ~(size_t)255;
+    const size_t h_rg = (h_stop + sizeof(int) * nlay + 255) & ~(size_t)255;
+    const size_t h_rgf = (h_rg + sizeof(int) * rg_lay.size() + 255) & ~(size_t)255;
+    const size_t h_size = h_rgf + sizeof(int64_t) * nlay;
+    const int ps = C->pin_next;
+    C->pin_next ^= 1;
+    HIPCHK(hipEventSynchronize(C->pin_ev[ps]));
+    if (C->pin_cap[ps] < h_size) {
+        if (C->pin[ps]) HIPCHK(hipHostFree(C->pin[ps]));
+        C->pin[ps] = nullptr;
+        C->pin_cap[ps] = 0;
+        HIPCHK(hipHostMalloc((void **)&C->pin[ps], h_size + h_size / 4, hipHostMallocDefault));
+        C->pin_cap[ps] = h_size + h_size / 4;
+    }
+    char *hp = C->pin[ps];
+    std::memcpy(hp + o_jobs, B.jobs.data(), sizeof(Job) * nlay);
+    std::memcpy(hp + o_pan, B.panels.data(), sizeof(Panel) * B.panels.size());
+    std::memcpy(hp + o_blk, B.blks.data(), sizeof(R4Blk) * B.blks.size());
+    std::memcpy(hp + o_tiles, B.tiles.data(), sizeof(int2) * B.tiles.size());
+    if (!B.absrb.empty()) std::memcpy(hp + o_abs, B.absrb.data(), sizeof(double) * B.absrb.size());
+    std::memcpy(hp + h_stop, B.stop0.data(), sizeof(int) * nlay);
+    if (!rg_lay.empty()) std::memcpy(hp + h_rg, rg_lay.data(), sizeof(int) * rg_lay.size());
+    std::memcpy(hp + h_rgf, rg_first.data(), sizeof(int64_t) * nlay);
     Job *d_jobs = at<Job>(base, o_jobs);
     Panel *d_pan = at<Panel>(base, o_pan);
     R4Blk *d_blk = at<R4Blk>(base, o_blk);
     int2 *d_tiles = at<int2>(base, o_tiles);
     double *d_abs = at<double>(base, o_abs);
-    HIPCHK(hipMemcpyAsync(d_jobs, B.jobs.data(), sizeof(Job) * nlay, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_pan, B.panels.data(), sizeof(Panel) * B.panels.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_blk, B.blks.data(), sizeof(R4Blk) * B.blks.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_tiles, B.tiles.data(), sizeof(int2) * B.tiles.size(), hipMemcpyHostToDevice, st));
-    if (!B.absrb.empty())
-        HIPCHK(hipMemcpyAsync(d_abs, B.absrb.data(), sizeof(double) * B.absrb.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base, hp, o_lp_d, hipMemcpyHostToDevice, st));
+    (void)d_jobs; (void)d_pan; (void)d_blk; (void)d_tiles; (void)d_abs;
     LinePrep lp;
     double *lpd = at<double>(base, o_lp_d);
     int *lpi = at<int>(base, o_lp_i);
@@ -1886,20 +2001,21 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     sh.co2 = at<int8_t>(base, o_sh_b);
     double *d_R4 = at<double>(base, o_R4);
     double *d_nat = at<double>(base, o_nat);
-    HIPCHK(hipMemcpyAsync(f.stop, B.stop0.data(), sizeof(int) * nlay, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(f.stop, hp + h_stop, sizeof(int) * nlay, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(f.chunks, 0xff, sizeof(int4) * std::max<int64_t>(1, B.tot_rec), st));
-    HIPCHK(hipMemsetAsync(C->d_err, 0, sizeof(int), st));
+    int *d_err = C->d_err + k;
+    HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int), st));
     Tables T = ctx_tabview(C);
     const int64_t npan = (int64_t)B.panels.size();
     const int64_t nblk = (int64_t)B.blks.size();
 
-    HIPCHK(hipEventRecord(C->ev[0], st));
+    HIPCHK(hipEventRecord(S.ev[0], st));
     if (B.max_line > 0) {
-        hipLaunchKernelGGL(k_prep, dim3(cdiv(B.max_line, 256), nlay), dim3(256), 0, st, d_jobs, d_pan, L.dev, lp, C->d_err);
+        hipLaunchKernelGGL(k_prep, dim3(cdiv(B.max_line, 256), nlay), dim3(256), 0, st, d_jobs, d_pan, L.dev, lp, d_err);
         hipLaunchKernelGGL(k_home, dim3(nlay), dim3(1024), 0, st, d_jobs, lp);
     }
     hipLaunchKernelGGL(k_ranges, dim3(cdiv(npan, 256)), dim3(256), 0, st, d_jobs, d_pan, npan, lp);
-    HIPCHK(hipEventRecord(C->ev[1], st));
+    HIPCHK(hipEventRecord(S.ev[1], st));
     if (precision == CLBLM_FP64) {
         hipLaunchKernelGGL(k_conv64, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
                            d_jobs, d_pan, lp, T, d_A, C->d_work);
@@ -1907,12 +2023,12 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         hipLaunchKernelGGL(k_conv32, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
                            d_jobs, d_pan, lp, T, d_A, C->d_work);
     }
-    HIPCHK(hipEventRecord(C->ev[2], st));
+    HIPCHK(hipEventRecord(S.ev[2], st));
     if (B.max_f4 > 0) {
-        hipLaunchKernelGGL(k_f4prep, dim3(cdiv(B.max_f4, 256), nlay), dim3(256), 0, st, d_jobs, L.dev, f, C->d_err);
+        hipLaunchKernelGGL(k_f4prep, dim3(cdiv(B.max_f4, 256), nlay), dim3(256), 0, st, d_jobs, L.dev, f, d_err);
         hipLaunchKernelGGL(k_reccount, dim3(cdiv(B.max_rec, 256), nlay), dim3(256), 0, st, d_jobs, f);
         hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, st, d_jobs, f, sh);
-        hipLaunchKernelGGL(k_sim, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, f, C->d_err);
+        hipLaunchKernelGGL(k_sim, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, f);
         hipLaunchKernelGGL(k_emit, dim3(cdiv(std::max<int64_t>(1, B.tot_rec), 64)), dim3(64), 0, st, d_jobs, f, sh,
                            B.tot_rec);
         hipLaunchKernelGGL(k_shraux, dim3(nlay), dim3(1024), 0, st, d_jobs, f, sh);
@@ -1920,7 +2036,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, st));
     }
     hipLaunchKernelGGL(k_r4range, dim3(cdiv(nblk, 256)), dim3(256), 0, st, d_jobs, d_blk, nblk, f, sh);
-    hipLaunchKernelGGL(k_f4recoff, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, d_blk, C->d_err);
+    hipLaunchKernelGGL(k_f4recoff, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, d_blk, d_err);
     {
         unsigned gx = std::max<unsigned>(1, cdiv(B.max_f4 * 2, 256 * 8));
         if (precision == CLBLM_FP64)
@@ -1930,45 +2046,34 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
             hipLaunchKernelGGL(k_f4setup<F4Rec32>, dim3(gx, (unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, nblk,
                                sh, at<F4Rec32>(base, o_rec));
     }
-    HIPCHK(hipEventRecord(C->ev[5], st));
+    HIPCHK(hipEventRecord(S.ev[5], st));
     if (precision == CLBLM_FP64)
         hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec64>(base, o_rec), d_R4, C->d_work);
     else
         hipLaunchKernelGGL(k_f4conv32, dim3((unsigned)B.tiles.size()), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec32>(base, o_rec), d_R4, C->d_work);
-    HIPCHK(hipEventRecord(C->ev[6], st));
+    HIPCHK(hipEventRecord(S.ev[6], st));
     if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, d_R4);
-    HIPCHK(hipEventRecord(C->ev[3], st));
+    HIPCHK(hipEventRecord(S.ev[3], st));
     hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
                        d_abs, d_out, d_nat);
     if (B.any_regrid) {
         int *d_rg = at<int>(base, o_rg);
         int64_t *d_rgf = at<int64_t>(base, o_rgf);
-        HIPCHK(hipMemcpyAsync(d_rg, rg_lay.data(), sizeof(int) * rg_lay.size(), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(d_rgf, rg_first.data(), sizeof(int64_t) * nlay, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_rg, hp + h_rg, sizeof(int) * rg_lay.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_rgf, hp + h_rgf, sizeof(int64_t) * nlay, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_regrid, dim3(cdiv((int64_t)rg_lay.size(), 64)), dim3(64), 0, st, d_jobs, d_rg, d_rgf,
                            d_nat, d_out, (int64_t)rg_lay.size());
     }
-    HIPCHK(hipEventRecord(C->ev[4], st));
+    HIPCHK(hipEventRecord(S.ev[4], st));
+    HIPCHK(hipEventRecord(C->pin_ev[ps], st));
     HIPCHK(hipGetLastError());
-    int herr = 0;
-    HIPCHK(hipMemcpyAsync(&herr, C->d_err, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (herr & 1) return set_err(CLBLM_ERR_UNSUPPORTED, "line coupling records (IFLG 1..3) are not supported");
-    if (herr & 4) return set_err(CLBLM_ERR_UNSUPPORTED, "shrink chunk longer than 64 TAPE3 records (too few lines of the selected molecules per record)");
-    if (herr & 8) return set_err(CLBLM_ERR_CAPACITY, "F4 record capacity exceeded");
-    float t01, t12, t23, t34;
-    HIPCHK(hipEventElapsedTime(&t01, C->ev[0], C->ev[1]));
-    HIPCHK(hipEventElapsedTime(&t12, C->ev[1], C->ev[2]));
-    HIPCHK(hipEventElapsedTime(&t23, C->ev[2], C->ev[3]));
-    HIPCHK(hipEventElapsedTime(&t34, C->ev[3], C->ev[4]));
-    float t56;
-    HIPCHK(hipEventElapsedTime(&t56, C->ev[5], C->ev[6]));
-    C->ms[0] += t01; C->ms[1] += t12; C->ms[2] += t23; C->ms[3] += t34;
-    C->ms_conv_k += t12;
-    C->ms_f4_k += t56;
-    C->launches++;
+    HIPCHK(hipMemcpyAsync(C->h_err + k, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(S.done, st));
+    S.pending = true;
+    C->ncalls++;
+    C->last_stream = st;
     return 0;
 }
 
@@ -1994,10 +2099,15 @@ int clblm_ctx_create(int device, clblm_ctx **out) {
     clblm_ctx *c = new clblm_ctx;
     c->device = device;
     HIPCHK(hipStreamCreate(&c->stream));
-    HIPCHK(hipMalloc(&c->d_err, sizeof(int)));
+    HIPCHK(hipMalloc(&c->d_err, clblm_ctx::NSLOT * sizeof(int)));
+    HIPCHK(hipHostMalloc((void **)&c->h_err, clblm_ctx::NSLOT * sizeof(int), hipHostMallocDefault));
     HIPCHK(hipMalloc(&c->d_work, 2 * sizeof(unsigned long long)));
     HIPCHK(hipMemset(c->d_work, 0, 2 * sizeof(unsigned long long)));
-    for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    for (auto &S : c->slot) {
+        for (auto &e : S.ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventCreate(&S.done));
+    }
+    for (auto &e : c->pin_ev) HIPCHK(hipEventCreate(&e));
     int rc = ctx_tables(c);
     if (rc) { delete c; return rc; }
     *out = c;
@@ -2007,6 +2117,7 @@ int clblm_ctx_create(int device, clblm_ctx **out) {
 int clblm_ctx_destroy(clblm_ctx *c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
     for (auto *l : c->lists)
         if (l) { (void)hipFree(l->mem); delete l; }
     if (c->arena.p) (void)hipFree(c->arena.p);
@@ -2015,7 +2126,15 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     (void)hipFree(c->d_err);
     (void)hipFree(c->d_work);
     if (c->rt_state) (void)hipFree(c->rt_state);
-    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    for (auto &S : c->slot) {
+        for (auto &e : S.ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(S.done);
+    }
+    for (int i = 0; i < 2; i++) {
+        (void)hipEventDestroy(c->pin_ev[i]);
+        if (c->pin[i]) (void)hipHostFree(c->pin[i]);
+    }
+    (void)hipHostFree(c->h_err);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -2085,19 +2204,58 @@ int clblm_od_batch(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
     int prec = pars[0].precision;
     if (prec != CLBLM_FP64 && prec != CLBLM_FP32C) return set_err(CLBLM_ERR_ARG, "unknown precision");
     HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    // the inputs that define the plan: a sizing call followed by the run call with
+    // the same inputs plans once
+    std::vector<char> key;
+    auto put = [&key](const void *p, size_t n) {
+        const char *b = (const char *)p;
+        key.insert(key.end(), b, b + n);
+    };
+    put(&handle, sizeof handle);
+    put(&nlay, sizeof nlay);
+    put(jrad_state, sizeof *jrad_state);
+    put(layers, sizeof(clblm_layer) * nlay);
+    put(pars, sizeof(clblm_od_params) * nlay);
+    for (int l = 0; l < nlay; l++)
+        if (pars[l].contnm && pars[l].continuum && pars[l].nptabs > 0)
+            put(pars[l].continuum, sizeof(double) * (size_t)pars[l].nptabs);
     int32_t jr = *jrad_state;
     Batch B;
-    int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec);
-    if (rc) return rc;
+    if (c->plan_valid && c->plan_key == key) {
+        B = std::move(c->plan);
+        jr = c->plan_jr;
+        c->plan_valid = false;
+    } else {
+        c->plan_valid = false;
+        int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec);
+        if (rc) return rc;
+    }
     for (int l = 0; l < nlay; l++) offsets[l] = B.jobs[l].out_off;
     offsets[nlay] = B.tot_out;
-    if (!d_od) return 0;
-    std::lock_guard<std::mutex> g(c->mu);
+    if (!d_od) {
+        c->plan = std::move(B);
+        c->plan_key = std::move(key);
+        c->plan_jr = jr;
+        c->plan_valid = true;
+        return 0;
+    }
+    drain(c, false);
+    int rc = take_deferred(c);
+    if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     rc = run_batch(c, *c->lists[handle], B, d_od, st, prec);
     if (rc) return rc;
     *jrad_state = jr;
     return 0;
+}
+
+int clblm_sync(clblm_ctx *c) {
+    if (!c) return set_err(CLBLM_ERR_ARG, "ctx is NULL");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    drain(c, true);
+    return take_deferred(c);
 }
 
 int clblm_odlay(clblm_ctx *c, int handle, const clblm_layer *layer, const clblm_od_params *par,
@@ -2117,6 +2275,7 @@ int clblm_odlay(clblm_ctx *c, int handle, const clblm_layer *layer, const clblm_
     double *d = nullptr;
     HIPCHK(hipMalloc(&d, sizeof(double) * off[1]));
     rc = clblm_od_batch(c, handle, 1, layer, par, jrad_state, off, d, nullptr);
+    if (rc == 0) rc = clblm_sync(c);
     if (rc == 0) {
         hipError_t e = hipMemcpy(od_out, d, sizeof(double) * off[1], hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = set_err(CLBLM_ERR_HIP, hipGetErrorString(e));
@@ -2128,6 +2287,10 @@ int clblm_odlay(clblm_ctx *c, int handle, const clblm_layer *layer, const clblm_
 int clblm_timing(clblm_ctx *c, int reset, double *ms_prep, double *ms_conv, double *ms_f4, double *ms_panel,
                  int64_t *launches) {
     if (!c) return set_err(CLBLM_ERR_ARG, "ctx is NULL");
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        drain(c, true);
+    }
     int64_t n = c->launches > 0 ? c->launches : 1;
     if (ms_prep) *ms_prep = c->ms[0] / n;
     if (ms_conv) *ms_conv = c->ms[1] / n;
@@ -2141,6 +2304,10 @@ int clblm_timing(clblm_ctx *c, int reset, double *ms_prep, double *ms_conv, doub
 int clblm_work(clblm_ctx *c, int reset, int64_t *conv_evals, int64_t *f4_evals, double *ms_conv_kernel,
                double *ms_f4_kernel) {
     if (!c) return set_err(CLBLM_ERR_ARG, "ctx is NULL");
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        drain(c, true);
+    }
     unsigned long long w[2];
     HIPCHK(hipMemcpy(w, c->d_work, sizeof w, hipMemcpyDeviceToHost));
     int64_t n = c->launches > 0 ? c->launches : 1;

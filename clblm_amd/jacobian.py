@@ -88,7 +88,8 @@ class UpwellingJacobian:
         check(self.ctx.lib.clblm_rt_up(self.ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(self.dvout),
                                        ptr(base), ptr(tlay), ptr(tlev), ctypes.c_double(self.emis),
                                        ctypes.c_double(self.tskin), self.lin, ctypes.c_double(self.sec_dn),
-                                       ptr(rad), ptr(dr_dod), ptr(dr_dt), ptr(dr_dts), ptr(dr_de), None))
+                                       ptr(rad), ptr(dr_dod), ptr(dr_dt), ptr(dr_dts), ptr(dr_de),
+                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
         jac_t = dr_dod * dtau_dt + dr_dt
         jac_q = dr_dod[:, None, :] * tau_m
         return dict(rad=rad, jac_t=jac_t, jac_lnq=jac_q, dr_dtskin=dr_dts, dr_demis=dr_de, od=base,
@@ -109,7 +110,8 @@ def _rt(ctx, od, t_lay, t_lev, v1, dv, emis, tskin, lin, sec_dn, grads):
     p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
     check(ctx.lib.clblm_rt_up(ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(dv), p(od), p(t_lay), p(t_lev),
                               ctypes.c_double(emis), ctypes.c_double(tskin), lin_in_tau_int(lin),
-                              ctypes.c_double(sec_dn), p(rad), p(g1), p(g2), None, None, None))
+                              ctypes.c_double(sec_dn), p(rad), p(g1), p(g2), None, None,
+                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
     return rad, g1, g2
 
 

@@ -101,19 +101,28 @@ class Context:
 
     def od_batch(self, handle, layers, params, out=None, stream=None):
         """Returns (od tensor on the GPU, offsets).  ``out`` may be a preallocated
-        float64 cuda tensor of offsets[-1] elements."""
+        float64 cuda tensor of offsets[-1] elements.  The work is enqueued on
+        ``stream`` (default: torch's current stream), so torch operations on the
+        result are ordered after it; errors found on the device are raised by the
+        next call or by ``sync()``."""
         import torch
         lay, par, off = self.od_batch_plan(handle, layers, params)
         if out is None:
             out = torch.empty(int(off[-1]), dtype=torch.float64, device="cuda")
         assert out.dtype == torch.float64 and out.is_cuda and out.numel() >= off[-1]
         jr = ctypes.c_int32(self.jrad)
-        st = None if stream is None else ctypes.c_void_p(stream)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        st = ctypes.c_void_p(stream)
         check(self.lib.clblm_od_batch(self.ptr, handle, len(layers), ctypes.cast(lay, ctypes.c_void_p),
                                       ctypes.cast(par, ctypes.c_void_p), ctypes.byref(jr),
                                       off.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(out.data_ptr()), st))
         self.jrad = jr.value
         return out, off
+
+    def sync(self):
+        """Wait for the enqueued batches; raises the first error found on the device."""
+        check(self.lib.clblm_sync(self.ptr))
 
     def work(self, reset=False):
         a, b = ctypes.c_int64(), ctypes.c_int64()

@@ -131,12 +131,8 @@ def temperature_profile(z_km: float) -> float:
     return max(288.0 - 6.5 * z_km, 217.0)
 
 
-def atmosphere(n_layers=60, p_top=0.1, mols=(1, 2, 3, 4, 5, 6, 7), t_noise=None,
-               h2o_scale=1.0) -> List[Layer]:
-    """Layers between log-spaced pressure levels from 1013 hPa to ``p_top``.
-    Temperature: 288 K at the surface, -6.5 K/km to 217 K, isothermal above;
-    heights from the hypsometric relation.  Layer pressure is the mass-weighted
-    mean (dp / dln p), temperature the level mean, columns from dp."""
+def _levels(n_layers, p_top):
+    """Log-spaced pressure levels, their heights (hypsometric) and temperatures."""
     levels = np.exp(np.linspace(math.log(1013.0), math.log(p_top), n_layers + 1))
     z = [0.0]
     t_lev = [temperature_profile(0.0)]
@@ -148,6 +144,21 @@ def atmosphere(n_layers=60, p_top=0.1, mols=(1, 2, 3, 4, 5, 6, 7), t_noise=None,
             t_guess = temperature_profile(z[-1] + dz)
         z.append(z[-1] + dz)
         t_lev.append(t_guess)
+    return levels, z, t_lev
+
+
+def level_temperatures(n_layers=60, p_top=0.1) -> np.ndarray:
+    """Level temperatures (n_layers + 1, surface first) of ``atmosphere(n_layers, p_top)``."""
+    return np.asarray(_levels(n_layers, p_top)[2], dtype=np.float64)
+
+
+def atmosphere(n_layers=60, p_top=0.1, mols=(1, 2, 3, 4, 5, 6, 7), t_noise=None,
+               h2o_scale=1.0) -> List[Layer]:
+    """Layers between log-spaced pressure levels from 1013 hPa to ``p_top``.
+    Temperature: 288 K at the surface, -6.5 K/km to 217 K, isothermal above;
+    heights from the hypsometric relation.  Layer pressure is the mass-weighted
+    mean (dp / dln p), temperature the level mean, columns from dp."""
+    levels, z, t_lev = _levels(n_layers, p_top)
     layers = []
     for k in range(n_layers):
         p1, p2 = levels[k], levels[k + 1]

@@ -90,10 +90,20 @@ int clblm_odlay(clblm_ctx *ctx, int handle, const clblm_layer *layer, const clbl
  * offsets[k] receives the first index of layer k in d_od and offsets[nlay] the
  * total.  Then d_od (device memory, offsets[nlay] doubles) is filled.  jrad is
  * applied to all layers with the reference's first-call rule (layers are treated
- * as successive calls, in order). */
+ * as successive calls, in order).  A query followed by the fill call with the same
+ * inputs plans the batch once.
+ * The fill call returns once the work is enqueued: d_od is ready in stream order,
+ * and the host arrays may be reused at once.  Argument errors are returned at
+ * once; errors found on the device (line coupling records, capacity) are returned
+ * by the next clblm_od_batch, clblm_odlay or clblm_sync on the context.  Batches
+ * of one context run in call order, also across streams. */
 int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *layers,
                    const clblm_od_params *pars, int32_t *jrad_state, int64_t *offsets,
                    double *d_od, void *stream);
+
+/* Waits for every batch enqueued on the context and returns the first device-found
+ * error among them (0 if none). */
+int clblm_sync(clblm_ctx *ctx);
 
 /* ---- radiance and its derivatives ---------------------------------------------
  * Upwelling radiance at the top of nlay layers (layer 0 at the surface) on a
@@ -114,7 +124,8 @@ int clblm_rt_up(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, co
 
 /* ---- timing -------------------------------------------------------------------
  * Average device time (ms) of each pipeline stage over the launches since the
- * last reset, measured with events on the stream the stages run on. */
+ * last reset, measured with events on the stream the stages run on (waits for the
+ * enqueued batches). */
 int clblm_timing(clblm_ctx *ctx, int reset, double *ms_prep, double *ms_conv, double *ms_f4,
                  double *ms_panel, int64_t *launches);
 

@@ -108,3 +108,43 @@ def test_batch_matches_single_calls(ctx):
     # last bits) may differ between launches
     for k, od in enumerate(singles):
         np.testing.assert_allclose(host[off[k]:off[k + 1]], od, rtol=1e-13, atol=0)
+
+
+def test_rare_molecule_long_shrink_chunks(ctx):
+    """A selected molecule with about one line in 150: LINF4Q's 1000-line buffer then
+    spans about 80 TAPE3 records in one SHRINQ chunk (clblm_ODLAY_LineF4.f90:1172-1210)."""
+    from oracle.py import Oracle
+    lines = W.profile_lines(40000, 500.0, 800.0, seed=1001)
+    lines.mol[:] = 101
+    lines.mol[::150] = 102
+    lay = W.atmosphere()[20]
+    lay.w = {2: lay.w[2]}
+    dv = W.calculate_dv(650.0, lay.pave, lay.tave, 0.0)
+    h = ctx.upload_lines(lines)
+    ctx.jrad = 1
+    od, _ = ctx.odlay(h, lay, 600.0, 700.0, dv)
+    ref, _ = Oracle(lines).odlay(lay, 600.0, 700.0, dv)
+    print("rare molecule: %d points, max rel err %.3e" % (od.size, _cmp(od, ref)))
+
+
+def test_batches_queue_without_host_waits(ctx):
+    """Batches return once enqueued; several in flight on one stream give the result
+    of a batch run alone."""
+    import torch
+    from clblm_amd.odlay import Context
+    lines = W.profile_lines(20000, 500.0, 800.0, seed=1001)
+    layers = W.atmosphere()[::15]
+    h = ctx.upload_lines(lines)
+    pars = [Context.params(600.0, 700.0, W.calculate_dv(650.0, l.pave, l.tave, l.w[1] / l.wtot)) for l in layers]
+    ctx.jrad = 1
+    ref, off = ctx.od_batch(h, layers, pars)
+    ctx.sync()
+    ref = ref.cpu().numpy()
+    outs = []
+    for _ in range(6):
+        ctx.jrad = 1
+        outs.append(ctx.od_batch(h, layers, pars)[0])
+    torch.cuda.synchronize()
+    ctx.sync()
+    for o in outs:
+        np.testing.assert_allclose(o.cpu().numpy(), ref, rtol=1e-13, atol=0)
