@@ -5,7 +5,7 @@
 // Pipeline for a batch of layers (every stage is one launch for all layers):
 //   host     plan_layer: panel and F4-block schedules (pure functions of V1, V2, DV)
 //   k_prep   LNCORQ per line and the first panel each line fits (CNVFNQ JMAX1 test)
-//   k_home   prefix max of that panel over the line order = the panel that convolves it
+//   k_home_* running max of that panel over the line order = the panel that convolves it
 //   k_ranges line range homed in each panel
 //   k_conv   pass 1: each panel's own R1/R2/R3 contributions (lane per line, FP64 LDS sums)
 //   k_f4*    LINF4Q per line, SHRINQ chunk simulation, CNVF4Q gather on R4 blocks
@@ -247,30 +247,80 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
 }
 
 // Inclusive prefix max of pmin over the line order (one workgroup per layer).
-__global__ void __launch_bounds__(1024) k_home(const Job *jobs, LinePrep lp) {
-    const Job &J = jobs[blockIdx.x];
-    __shared__ int wmax[16];
-    __shared__ int carry_s;
-    int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) carry_s = 0;
-    __syncthreads();
-    for (int64_t base = 0; base < J.line_n; base += 1024) {
-        int64_t k = base + tid;
-        int v = k < J.line_n ? lp.pmin[J.lp_off + k] : -1;
-        for (int d = 1; d < 64; d <<= 1) {
-            int u = __shfl_up(v, d, 64);
-            if (lane >= d) v = max(v, u);
-        }
-        if (lane == 63) wmax[wid] = v;
-        __syncthreads();
-        int pre = carry_s;
-        for (int w = 0; w < wid; w++) pre = max(pre, wmax[w]);
-        v = max(v, pre);
-        if (k < J.line_n) lp.home[J.lp_off + k] = v;
-        __syncthreads();
-        if (tid == 1023) carry_s = v;
-        __syncthreads();
+// Per-layer running max (or min, or a min scan from the end) of one array, in two
+// launches over chunks of SCAN_C elements: each workgroup scans its chunk and keeps
+// the chunk's total, then each workgroup combines the totals of the chunks before
+// it.  Max and min are exact, so the result does not depend on the split.
+constexpr int SCAN_T = 256, SCAN_V = 16, SCAN_C = SCAN_T * SCAN_V;
+
+template <typename T, bool MAX>
+__device__ __forceinline__ T scan_op(T a, T b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); }
+
+// element p of the scan order of an n-element run at off (reversed when REV)
+template <bool REV>
+__device__ __forceinline__ int64_t scan_at(int64_t off, int64_t n, int64_t p) { return off + (REV ? n - 1 - p : p); }
+
+template <typename T, bool MAX, bool REV>
+__device__ void scan_local(const T *src, T *dst, int64_t off, int64_t n, T ident, T *tot) {
+    __shared__ T wtot[SCAN_T / 64];
+    const int64_t c = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t p0 = c * SCAN_C + (int64_t)tid * SCAN_V;
+    T v[SCAN_V];
+    T run = ident;
+#pragma unroll
+    for (int u = 0; u < SCAN_V; u++) {
+        const int64_t p = p0 + u;
+        T x = p < n ? src[scan_at<REV>(off, n, p)] : ident;
+        run = scan_op<T, MAX>(run, x);
+        v[u] = run;
     }
+    T incl = run;  // inclusive scan of the thread totals across the wave, then the waves
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl = scan_op<T, MAX>(incl, o);
+    }
+    if (lane == 63) wtot[wid] = incl;
+    __syncthreads();
+    T pre = __shfl_up(incl, 1, 64);
+    if (lane == 0) pre = ident;
+    for (int w = 0; w < wid; w++) pre = scan_op<T, MAX>(wtot[w], pre);
+#pragma unroll
+    for (int u = 0; u < SCAN_V; u++) {
+        const int64_t p = p0 + u;
+        if (p < n) dst[scan_at<REV>(off, n, p)] = scan_op<T, MAX>(pre, v[u]);
+    }
+    if (tid == SCAN_T - 1) tot[c] = scan_op<T, MAX>(pre, incl);
+}
+
+template <typename T, bool MAX, bool REV>
+__device__ void scan_fix(T *dst, int64_t off, int64_t n, T ident, const T *tot) {
+    __shared__ T pre_s;
+    const int64_t c = blockIdx.x;
+    if (c == 0) return;
+    if (threadIdx.x == 0) {
+        T pre = ident;
+        for (int64_t q = 0; q < c; q++) pre = scan_op<T, MAX>(pre, tot[q]);
+        pre_s = pre;
+    }
+    __syncthreads();
+    const T pre = pre_s;
+    for (int64_t p = c * SCAN_C + threadIdx.x; p < (c + 1) * SCAN_C && p < n; p += SCAN_T) {
+        const int64_t i = scan_at<REV>(off, n, p);
+        dst[i] = scan_op<T, MAX>(pre, dst[i]);
+    }
+}
+
+// home = running max of pmin over the line order (the panel that convolves each line)
+__global__ void __launch_bounds__(SCAN_T) k_home_local(const Job *jobs, LinePrep lp, int *tot, int64_t cap) {
+    const Job &J = jobs[blockIdx.y];
+    if ((int64_t)blockIdx.x * SCAN_C >= J.line_n) return;
+    scan_local<int, true, false>(lp.pmin, lp.home, J.lp_off, J.line_n, -1, tot + blockIdx.y * cap);
+}
+__global__ void __launch_bounds__(SCAN_T) k_home_fix(const Job *jobs, LinePrep lp, const int *tot, int64_t cap) {
+    const Job &J = jobs[blockIdx.y];
+    if ((int64_t)blockIdx.x * SCAN_C >= J.line_n) return;
+    scan_fix<int, true, false>(lp.home, J.lp_off, J.line_n, -1, tot + blockIdx.y * cap);
 }
 
 // Line range homed in each panel (home is non-decreasing).
@@ -523,71 +573,90 @@ struct ShrinkSums {
 template <bool EMIT>
 __device__ int shrinq_walk(const Job &J, const F4Prep &f, int64_t k0, int64_t kend, int64_t rec_end0,
                            int lcap, int *gout, Shrunk sh, int64_t out0) {
+    // Lines are read WB at a time with independent loads (each lane walks its own
+    // stretch of the list, so this is what hides the memory latency); every accepted
+    // line is processed once the next accepted one is known.
+    constexpr int WB = 8;
     const double dvh = J.dvr4 / 2.;
     ShrinkSums a{0, 0, 0, 0, 0}, b{0, 0, 0, 0, 0};
     int emitted = 0, lrec = 0, walked = 0;
     double vlmt = 0.;
     int64_t rec_end = rec_end0;  // relative index one past the current record
-    int64_t cur = k0;
-    while (cur < kend && !f.acc[J.f4_off + cur]) cur++;
-    if (!EMIT) {
-        while (lrec < lcap && rec_end <= cur) { gout[lrec++] = 0; rec_end += REC; }
-    }
-    bool first = true;
-    while (cur < kend) {
-        int64_t o = J.f4_off + cur;
-        double v = f.vnu[o], s = f.sp[o], al = f.alfa[o], ad = f.epp[o], c = 0.;
+    bool first = true, have = false;
+    double pv = 0., ps = 0., pal = 0., pad = 0.;
+    int8_t pco2 = 0;
+    auto flush = [&](const ShrinkSums &g, int8_t co2) {
+        if (EMIT) {
+            int64_t q = J.f4_off + out0 + emitted;
+            sh.vnu[q] = g.v / g.s; sh.sp[q] = g.s; sh.alfa[q] = g.al / g.s; sh.epp[q] = g.ad / g.s;
+            sh.spp[q] = g.c; sh.co2[q] = co2;
+        }
+        emitted++;
+    };
+    // the pending line, with nxt the relative index of the next accepted line (kend if
+    // none) and nv its centre; returns true where a counting walk may stop
+    auto step = [&](int64_t nxt, double nv) -> bool {
+        double v = pv, s = ps, al = pal, ad = pad, c = 0.;
         if (first) { vlmt = v + dvh; first = false; }
         if (s < 1.0e-35) { s = 0.0; c = 0.0; }
-        if (c != 0.0) {
-            if (EMIT) {
-                int64_t q = J.f4_off + out0 + emitted;
-                sh.vnu[q] = v; sh.sp[q] = s; sh.alfa[q] = al; sh.epp[q] = ad; sh.spp[q] = c; sh.co2[q] = f.co2[o];
-            }
-            emitted++;
-        } else if (!f.co2[o]) {
+        if (!pco2) {
             a.v = a.v + v * s; a.s = a.s + s; a.al = a.al + s * al; a.ad = a.ad + s * ad; a.c = a.c + c;
         } else {
             b.v = b.v + v * s; b.s = b.s + s; b.al = b.al + s * al; b.ad = b.ad + s * ad; b.c = b.c + c;
         }
-        int64_t nxt = cur + 1;
-        while (nxt < kend && !f.acc[J.f4_off + nxt]) nxt++;
-        bool cont = nxt < kend && f.vnu[J.f4_off + nxt] <= vlmt;
+        bool cont = nxt < kend && nv <= vlmt;
         if (!cont) {
             vlmt = v + dvh;
-            if (a.s > 0.) {
-                if (EMIT) {
-                    int64_t q = J.f4_off + out0 + emitted;
-                    sh.vnu[q] = a.v / a.s; sh.sp[q] = a.s; sh.alfa[q] = a.al / a.s; sh.epp[q] = a.ad / a.s;
-                    sh.spp[q] = a.c; sh.co2[q] = 0;
-                }
-                emitted++;
-            }
-            if (b.s > 0.) {
-                if (EMIT) {
-                    int64_t q = J.f4_off + out0 + emitted;
-                    sh.vnu[q] = b.v / b.s; sh.sp[q] = b.s; sh.alfa[q] = b.al / b.s; sh.epp[q] = b.ad / b.s;
-                    sh.spp[q] = b.c; sh.co2[q] = 1;
-                }
-                emitted++;
-            }
+            if (a.s > 0.) flush(a, 0);
+            if (b.s > 0.) flush(b, 1);
             a = ShrinkSums{0, 0, 0, 0, 0};
             b = ShrinkSums{0, 0, 0, 0, 0};
         }
         if (!EMIT) {
             int pend = emitted + (a.s > 0. ? 1 : 0) + (b.s > 0. ? 1 : 0);
             // record ends between this line and the next accepted one
-            int64_t lim = nxt < kend ? nxt : kend;
             bool crossed = false;
-            while (lrec < lcap && rec_end <= lim) { gout[lrec++] = pend; rec_end += REC; crossed = true; }
+            while (lrec < lcap && rec_end <= nxt) { gout[lrec++] = pend; rec_end += REC; crossed = true; }
             if (nxt >= kend) {
                 while (lrec < lcap) { gout[lrec++] = pend; rec_end += REC; }
             }
             // a chunk never needs more than LIMIN4 new lines: stop at the first record
             // end after that many (the remaining entries are never read)
-            if (++walked >= LIMIN4 && crossed) return emitted;
+            if (++walked >= LIMIN4 && crossed) return true;
         }
-        cur = nxt;
+        return false;
+    };
+    auto start = [&](int64_t cur) {  // record ends before the first accepted line
+        if (!EMIT) {
+            while (lrec < lcap && rec_end <= cur) { gout[lrec++] = 0; rec_end += REC; }
+        }
+    };
+    for (int64_t kb = k0; kb < kend; kb += WB) {
+        int8_t ac[WB], cc[WB];
+        double vv[WB], ss[WB], aa[WB], ee[WB];
+#pragma unroll
+        for (int u = 0; u < WB; u++) {
+            const int64_t k = kb + u;
+            const int64_t o = J.f4_off + (k < kend ? k : kend - 1);
+            ac[u] = k < kend ? f.acc[o] : (int8_t)0;
+            vv[u] = f.vnu[o]; ss[u] = f.sp[o]; aa[u] = f.alfa[o]; ee[u] = f.epp[o]; cc[u] = f.co2[o];
+        }
+#pragma unroll
+        for (int u = 0; u < WB; u++) {
+            if (!ac[u]) continue;
+            if (have) {
+                if (step(kb + u, vv[u])) return emitted;
+            } else {
+                start(kb + u);
+            }
+            have = true;
+            pv = vv[u]; ps = ss[u]; pal = aa[u]; pad = ee[u]; pco2 = cc[u];
+        }
+    }
+    if (have) {
+        if (step(kend, 0.)) return emitted;
+    } else {
+        start(kend);
     }
     if (!EMIT) {
         while (lrec < lcap) gout[lrec++] = emitted;
@@ -678,44 +747,38 @@ __global__ void k_emit(const Job *jobs, F4Prep f, Shrunk sh, int64_t rec_tot) {
     shrinq_walk<true>(J, f, k0, kend, 0, 0, nullptr, sh, c.z);
 }
 
-// Prefix max / suffix min of the shrunk centres and prefix max of the TAPE9 block
-// maxima, for the range searches of the R4 gather (one workgroup per layer).
-__global__ void __launch_bounds__(1024) k_shraux(const Job *jobs, F4Prep f, Shrunk sh) {
-    const Job &J = jobs[blockIdx.x];
-    const int64_t n = f.nshr[blockIdx.x];
-    __shared__ double wv[16];
-    __shared__ double carry;
-    int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+// Prefix max / suffix min of the shrunk centres (the range searches of the R4
+// gather), as per-layer scans, and the prefix max of the TAPE9 block maxima.
+__global__ void __launch_bounds__(SCAN_T) k_shr_local(const Job *jobs, F4Prep f, Shrunk sh, double *tot,
+                                                      int64_t cap) {
+    const Job &J = jobs[blockIdx.y];
+    const int64_t n = f.nshr[blockIdx.y];
+    if ((int64_t)blockIdx.x * SCAN_C >= n) return;
+    if (blockIdx.z == 0)
+        scan_local<double, true, false>(sh.vnu, sh.pmax, J.f4_off, n, -1e300, tot + (2 * blockIdx.y) * cap);
+    else
+        scan_local<double, false, true>(sh.vnu, sh.smin, J.f4_off, n, 1e300, tot + (2 * blockIdx.y + 1) * cap);
+}
+__global__ void __launch_bounds__(SCAN_T) k_shr_fix(const Job *jobs, F4Prep f, Shrunk sh, const double *tot,
+                                                    int64_t cap) {
+    const Job &J = jobs[blockIdx.y];
+    const int64_t n = f.nshr[blockIdx.y];
+    if ((int64_t)blockIdx.x * SCAN_C >= n) return;
+    if (blockIdx.z == 0)
+        scan_fix<double, true, false>(sh.pmax, J.f4_off, n, -1e300, tot + (2 * blockIdx.y) * cap);
+    else
+        scan_fix<double, false, true>(sh.smin, J.f4_off, n, 1e300, tot + (2 * blockIdx.y + 1) * cap);
+}
+__global__ void k_shr_blk9(const Job *jobs, int nlay, F4Prep f, Shrunk sh) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlay) return;
+    const Job &J = jobs[l];
     const double *v = sh.vnu + J.f4_off;
-    for (int pass = 0; pass < 2; pass++) {
-        if (tid == 0) carry = pass == 0 ? -1e300 : 1e300;
-        __syncthreads();
-        for (int64_t base = 0; base < n; base += 1024) {
-            int64_t k = base + tid;
-            int64_t kk = pass == 0 ? k : n - 1 - k;
-            double x = k < n ? v[kk] : (pass == 0 ? -1e300 : 1e300);
-            for (int d = 1; d < 64; d <<= 1) {
-                double u = __shfl_up(x, d, 64);
-                if (lane >= d) x = pass == 0 ? fmax(x, u) : fmin(x, u);
-            }
-            if (lane == 63) wv[wid] = x;
-            __syncthreads();
-            double pre = carry;
-            for (int w = 0; w < wid; w++) pre = pass == 0 ? fmax(pre, wv[w]) : fmin(pre, wv[w]);
-            x = pass == 0 ? fmax(x, pre) : fmin(x, pre);
-            if (k < n) (pass == 0 ? sh.pmax : sh.smin)[J.f4_off + kk] = x;
-            __syncthreads();
-            if (tid == 1023) carry = x;
-            __syncthreads();
-        }
-    }
-    if (tid == 0) {
-        double m = -1e300;
-        for (int b = 0; b < f.nblk9[blockIdx.x]; b++) {
-            int64_t e = f.blk9_end[J.rec_off + b];
-            m = fmax(m, v[e - 1]);
-            sh.bvmax[J.f4_off + b] = m;
-        }
+    double m = -1e300;
+    for (int b = 0; b < f.nblk9[l]; b++) {
+        int64_t e = f.blk9_end[J.rec_off + b];
+        m = fmax(m, v[e - 1]);
+        sh.bvmax[J.f4_off + b] = m;
     }
 }
 
@@ -951,8 +1014,8 @@ __device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dpt
 
 // F4 gather: one wave per tile of F4PTS consecutive R4 points, F4K points per lane
 // (j = j0 + lane + 64 k), lines walked in the reference's order so each point's sum
-// has the reference's summation order.  A record is read once (wave-uniform loads)
-// and used for F4K points; records that cover the whole tile skip the per-point
+// has the reference's summation order.  Records pass through LDS and each is used
+// for F4K points per lane; records that cover the whole tile skip the per-point
 // range test.
 constexpr int F4K = 4, F4PTS = 64 * F4K;
 constexpr int RAMP_CAP = 256;  // ramp pieces per panel held in LDS
@@ -980,29 +1043,41 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
     if (j0 <= B.nptr4) {
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
+        // records are staged in LDS 64 at a time (one coalesced load per lane, the
+        // next batch in flight while this one is summed) and read back as broadcasts
         const F4Rec64 *r = recs + B.rec - B.lo;
-        for (int64_t i = c0; i < c1; i++) {
-            const F4Rec64 l = r[i];
-            if (l.jmin <= l.jmax) {
-                nev += f4_count(l.jmin, l.jmax, j0, j1);
-                if (l.jmin <= j0 && l.jmax >= j1) {
+        __shared__ F4Rec64 srec[64];
+        F4Rec64 pre;
+        if (lane < c1 - c0) pre = r[c0 + lane];
+        for (int64_t base = c0; base < c1; base += 64) {
+            const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
+            __syncthreads();
+            if (lane < nb) srec[lane] = pre;
+            __syncthreads();
+            if (lane < c1 - base - 64) pre = r[base + 64 + lane];
+            for (int q = 0; q < nb; q++) {
+                const F4Rec64 l = srec[q];
+                if (l.jmin <= l.jmax) {
+                    nev += f4_count(l.jmin, l.jmax, j0, j1);
+                    if (l.jmin <= j0 && l.jmax >= j1) {
 #pragma unroll
-                    for (int k = 0; k < F4K; k++) acc[k] += f4_eval64(l, xj[k] - l.xnui, l.dptrat);
-                } else {
+                        for (int k = 0; k < F4K; k++) acc[k] += f4_eval64(l, xj[k] - l.xnui, l.dptrat);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < F4K; k++) {
+                            const int j = j0 + lane + 64 * k;
+                            double f = f4_eval64(l, xj[k] - l.xnui, l.dptrat);
+                            acc[k] += (j >= l.jmin && j <= l.jmax) ? f : 0.;
+                        }
+                    }
+                }
+                if (l.jminm <= l.jmaxm) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
 #pragma unroll
                     for (int k = 0; k < F4K; k++) {
                         const int j = j0 + lane + 64 * k;
-                        double f = f4_eval64(l, xj[k] - l.xnui, l.dptrat);
-                        acc[k] += (j >= l.jmin && j <= l.jmax) ? f : 0.;
+                        double f = f4_eval64(l, xj[k] - l.xnum, -l.dptrat);
+                        acc[k] += (j >= l.jminm && j <= l.jmaxm) ? f : 0.;
                     }
-                }
-            }
-            if (l.jminm <= l.jmaxm) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
-#pragma unroll
-                for (int k = 0; k < F4K; k++) {
-                    const int j = j0 + lane + 64 * k;
-                    double f = f4_eval64(l, xj[k] - l.xnum, -l.dptrat);
-                    acc[k] += (j >= l.jminm && j <= l.jmaxm) ? f : 0.;
                 }
             }
         }
@@ -1032,32 +1107,44 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
     if (j0 <= B.nptr4) {
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
+        // records are staged in LDS 64 at a time (one coalesced load per lane, the
+        // next batch in flight while this one is summed) and read back as broadcasts
         const F4Rec32 *r = recs + B.rec - B.lo;
-        for (int64_t i = c0; i < c1; i++) {
-            const F4Rec32 l = r[i];
-            if (l.jmin <= l.jmax) {
-                nev += f4_count(l.jmin, l.jmax, j0, j1);
-                const float fl = (float)(j0 + lane - l.jc);
-                if (l.jmin <= j0 && l.jmax >= j1) {
+        __shared__ F4Rec32 srec[64];
+        F4Rec32 pre;
+        if (lane < c1 - c0) pre = r[c0 + lane];
+        for (int64_t base = c0; base < c1; base += 64) {
+            const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
+            __syncthreads();
+            if (lane < nb) srec[lane] = pre;
+            __syncthreads();
+            if (lane < c1 - base - 64) pre = r[base + 64 + lane];
+            for (int q = 0; q < nb; q++) {
+                const F4Rec32 l = srec[q];
+                if (l.jmin <= l.jmax) {
+                    nev += f4_count(l.jmin, l.jmax, j0, j1);
+                    const float fl = (float)(j0 + lane - l.jc);
+                    if (l.jmin <= j0 && l.jmax >= j1) {
 #pragma unroll
-                    for (int k = 0; k < F4K; k++)
-                        acc[k] += (double)f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
-                } else {
+                        for (int k = 0; k < F4K; k++)
+                            acc[k] += (double)f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < F4K; k++) {
+                            const int j = j0 + lane + 64 * k;
+                            float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
+                            acc[k] += (double)((j >= l.jmin && j <= l.jmax) ? f : 0.f);
+                        }
+                    }
+                }
+                if (l.jminm <= l.jmaxm) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
+                    const float fl = (float)(j0 + lane - l.jcm);
 #pragma unroll
                     for (int k = 0; k < F4K; k++) {
                         const int j = j0 + lane + 64 * k;
-                        float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
-                        acc[k] += (double)((j >= l.jmin && j <= l.jmax) ? f : 0.f);
+                        float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.dm), -l.dptrat);
+                        acc[k] += (double)((j >= l.jminm && j <= l.jmaxm) ? f : 0.f);
                     }
-                }
-            }
-            if (l.jminm <= l.jmaxm) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
-                const float fl = (float)(j0 + lane - l.jcm);
-#pragma unroll
-                for (int k = 0; k < F4K; k++) {
-                    const int j = j0 + lane + 64 * k;
-                    float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.dm), -l.dptrat);
-                    acc[k] += (double)((j >= l.jminm && j <= l.jmaxm) ? f : 0.f);
                 }
             }
         }
@@ -1901,6 +1988,9 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_sh_b = cv.take<int8_t>((size_t)B.tot_f4);
     size_t o_lay = cv.take<int>(4 * (size_t)nlay);
     size_t o_racc = cv.take<int>((size_t)B.tot_rec);
+    const int64_t hcap = (B.max_line + SCAN_C - 1) / SCAN_C, fcap = (B.max_f4 + SCAN_C - 1) / SCAN_C;
+    size_t o_htot = cv.take<int>((size_t)std::max<int64_t>(1, hcap * nlay));
+    size_t o_ftot = cv.take<double>((size_t)std::max<int64_t>(1, 2 * fcap * nlay));
     size_t o_G = cv.take<int>((size_t)B.tot_rec * LCAP);
     size_t o_ch = cv.take<int4>((size_t)B.tot_rec);
     size_t o_be = cv.take<int64_t>((size_t)B.tot_rec);
@@ -2012,7 +2102,9 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     HIPCHK(hipEventRecord(S.ev[0], st));
     if (B.max_line > 0) {
         hipLaunchKernelGGL(k_prep, dim3(cdiv(B.max_line, 256), nlay), dim3(256), 0, st, d_jobs, d_pan, L.dev, lp, d_err);
-        hipLaunchKernelGGL(k_home, dim3(nlay), dim3(1024), 0, st, d_jobs, lp);
+        int *htot = at<int>(base, o_htot);
+        hipLaunchKernelGGL(k_home_local, dim3((unsigned)hcap, nlay), dim3(SCAN_T), 0, st, d_jobs, lp, htot, hcap);
+        hipLaunchKernelGGL(k_home_fix, dim3((unsigned)hcap, nlay), dim3(SCAN_T), 0, st, d_jobs, lp, htot, hcap);
     }
     hipLaunchKernelGGL(k_ranges, dim3(cdiv(npan, 256)), dim3(256), 0, st, d_jobs, d_pan, npan, lp);
     HIPCHK(hipEventRecord(S.ev[1], st));
@@ -2031,7 +2123,14 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         hipLaunchKernelGGL(k_sim, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, f);
         hipLaunchKernelGGL(k_emit, dim3(cdiv(std::max<int64_t>(1, B.tot_rec), 64)), dim3(64), 0, st, d_jobs, f, sh,
                            B.tot_rec);
-        hipLaunchKernelGGL(k_shraux, dim3(nlay), dim3(1024), 0, st, d_jobs, f, sh);
+        double *ftot = at<double>(base, o_ftot);
+        if (fcap > 0) {
+            hipLaunchKernelGGL(k_shr_local, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, st, d_jobs, f, sh, ftot,
+                               fcap);
+            hipLaunchKernelGGL(k_shr_fix, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, st, d_jobs, f, sh, ftot,
+                               fcap);
+        }
+        hipLaunchKernelGGL(k_shr_blk9, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, f, sh);
     } else {
         HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, st));
     }
@@ -2047,11 +2146,12 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
                                sh, at<F4Rec32>(base, o_rec));
     }
     HIPCHK(hipEventRecord(S.ev[5], st));
+    const unsigned f4grid = (unsigned)B.tiles.size();
     if (precision == CLBLM_FP64)
-        hipLaunchKernelGGL(k_f4conv64, dim3((unsigned)B.tiles.size()), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
+        hipLaunchKernelGGL(k_f4conv64, dim3(f4grid), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec64>(base, o_rec), d_R4, C->d_work);
     else
-        hipLaunchKernelGGL(k_f4conv32, dim3((unsigned)B.tiles.size()), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
+        hipLaunchKernelGGL(k_f4conv32, dim3(f4grid), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec32>(base, o_rec), d_R4, C->d_work);
     HIPCHK(hipEventRecord(S.ev[6], st));
     if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, d_R4);
