@@ -1274,3 +1274,300 @@ void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, c
         rad[i] = r;
     }
 }
+
+/* ==== Instrument scan ===========================================================
+ * clblm_FFTSCN_struct (clblm_FFT.f90:352-844) for the scanning functions with an
+ * analytic apodization (IVX = 1), with LoadSpect (:1114-1334), Scntrn/Scnfnt
+ * (:2070-2244, 1411-1958), Multrn (:1334-1411), wrtSpect (:2260-2382) and the
+ * 4-point Intpdr/clblm_INTERP/panelInput (clblm_FFT.f90:2392-2443,
+ * clblm_ScanFilter.f90:1975-2375).  The FFT itself is left to the caller. */
+
+#define SCAN_LPTSMX 65536
+static const double SCAN_C[14] = {0.0, 2.0, 0.849322, 2.257609, 3.314800, 2.100669, 2.195676,
+                                  2.0, 2.570274, 2.367714, 2.071759, -3.3, -2.5, 3.314800};
+static const double SCAN_CRATIO[14] = {0.0, 40., 10., 40., 160., 20., 20., 20., 40., 40., 20., 100., 10., 1e300};
+static const double SCAN_CLIMIT[14] = {1., 2., 3., 40., 160., 20., 20., 20., 40., 40., 20., 100., 10., 160.};
+
+typedef struct {
+    double v1;      /* spectrum start */
+    double dv;
+    int64_t n;
+} scan_grid;
+
+/* panelInput (clblm_ScanFilter.f90:2302-2370): the next 2400-point panel of the
+ * input that reaches VBOT, into T(5..NNI) (1-based T), panel frequencies accumulated
+ * as LBLRTM does.  Returns 0 at the end of the input (outputs untouched). */
+static int scan_panel(const double *x, const scan_grid *g, double *T, double vbot, double *vmin_o,
+                      double *vmax_o, double *dvi_o, int *nni_o, int64_t *done, double *done_v2) {
+    int nlim;
+    double vmin, vmax, dvi;
+    for (;;) {
+        nlim = 2400;
+        if (*done + nlim > g->n) nlim = (int)(g->n - *done);
+        if (nlim == 0) return 0;
+        dvi = g->dv;
+        vmin = *done_v2 + dvi;
+        vmax = vmin + (double)(nlim - 1) * dvi;
+        *done += nlim;
+        *done_v2 = vmax;
+        if (vmax >= vbot) break;
+    }
+    const int nlow = 4 + 1;
+    vmin = vmin - (double)(nlow - 1) * dvi;
+    int nni = nlim + nlow - 1;
+    for (int i = 0; i < nlim; i++) T[nlow + i] = x[*done - nlim + i];
+    *vmin_o = vmin; *vmax_o = vmax; *dvi_o = dvi; *nni_o = nni;
+    return 1;
+}
+
+/* Intpdr + clblm_INTERP with 4-point interpolation.  x[0..n) on v1 + i dv onto the
+ * grid vv1 + j dvo up to vv2.  Writes min(count, cap) points; returns the count and
+ * the output start in *v1res. */
+int64_t orc_scan_interp(const double *x, int64_t n, double v1, double dv, double vv1, double vv2, double dvo,
+                        double *out, int64_t cap, double *v1res) {
+    scan_grid g = {v1, dv, n};
+    const double V1C = v1, V2C = v1 + dv * (double)(n - 1);
+    double VV1 = vv1, VV2 = vv2;
+    if (VV1 < V1C) {
+        double AN = (V1C - VV1) / dvo;
+        if (AN == (double)(int)AN) VV1 = VV1 + dvo * (double)(int)AN;
+        else VV1 = VV1 + dvo * (double)((int)AN + 1);
+    }
+    if (VV2 > V2C) {
+        double AN = (VV2 - V2C) / dvo;
+        if (AN == (double)(int)AN) VV2 = VV2 - dvo * (double)(int)AN;
+        else VV2 = VV2 - dvo * (double)((int)AN + 1);
+    }
+    const double V1 = VV1, V2 = VV2, DVO = dvo;
+    static double C1[203], C2[203], C3[203], C4[203];
+    const double XNUMCF = 201.;
+    for (int ip = 0; ip <= 202; ip++) {
+        double P = ((double)ip - 1.0) / (XNUMCF - 1.0);
+        double PP = P * P;
+        C1[ip] = -(P / 2.0 * ((1 - P) * (1 - P)));
+        C2[ip] = 1.0 - PP * (3.0 - 2.0 * P) + PP / 2.0 * (1.0 - P);
+        C3[ip] = PP * (3.0 - 2.0 * P) + P / 2.0 * ((1.0 - P) * (1.0 - P));
+        C4[ip] = -(PP / 2. * (1.0 - P));
+    }
+    const double BOUND = 4.0 * DVO;
+    const double VBOT = V1 - BOUND;
+    double T[2412];
+    double R[2402];
+    int64_t nout = 0, done = 0;
+    double done_v2 = v1 - dv;
+    for (int i = 1; i <= 4; i++) T[i] = 0.0;
+    double V1I = 0, V2I = 0, DVI = 0;
+    int NNI = 0;
+    int eof = !scan_panel(x, &g, T, VBOT, &V1I, &V2I, &DVI, &NNI, &done, &done_v2);
+    const double DVJ = DVO;
+    double V1J = V1, VJ = V1J, V2J;
+    const double VRATIO = DVJ / DVI;
+    T[4] = 2.0 * T[5] - T[6];
+#define EMIT_PANEL(nnj)                                                   \
+    do {                                                                  \
+        for (int q = 1; q <= (nnj); q++) {                                \
+            if (nout < cap) out[nout] = R[q];                             \
+            nout++;                                                       \
+        }                                                                 \
+    } while (0)
+    while (V1J < V1I) {
+        int J2 = (int)((V1I - V1J) / DVJ) + 1;
+        if (J2 > 2400) J2 = 2400;
+        for (int j = 1; j <= J2; j++) R[j] = 0.0;
+        V2J = V1J + DVJ * (double)(J2 - 1);
+        EMIT_PANEL(J2);
+        V1J = V2J + DVJ;
+        VJ = V1J;
+    }
+    int J2 = 0;
+    for (;;) {
+        int I = (int)((VJ - V1I) / DVI + 1.00001);
+        if (I < 1) break;
+        double VI = V1I + DVI * (double)(I - 1);
+        double P = (VJ - VI) / DVI;
+        int J1 = (int)((VJ - V1J) / DVJ + 1.001);
+        int a = (int)((V2 - V1J) / DVJ + 1.001), b = (int)((V2I - DVI - V1J) / DVJ + 1.);
+        J2 = a < b ? a : b;
+        if (J2 > 2400) J2 = 2400;
+        for (int J = J1; J <= J2; J++) {
+            int IP = (int)(P * XNUMCF + 1.00001);
+            R[J] = C1[IP] * T[I - 1] + C2[IP] * T[I] + C3[IP] * T[I + 1] + C4[IP] * T[I + 2];
+            P = P + VRATIO;
+            if (P >= 1.0) {
+                I = (int)((double)I + P);
+                P = P - (double)(int)P;
+            }
+        }
+        VJ = V1J + DVJ * (double)J2;
+        if (J2 >= 2400 || VJ >= V2) {
+            V2J = V1J + DVJ * (double)(J2 - 1);
+            EMIT_PANEL(J2);
+            V1J = V2J + DVJ;
+            J2 = 0;
+        }
+        if (VJ >= V2) goto done_interp;
+        if (eof) break;
+        if (I >= NNI - 2) {
+            for (int q = 1; q <= 4; q++) T[q] = T[q + NNI - 4];
+            double a1 = V1I, a2 = V2I, a3 = DVI;
+            int a4 = NNI;
+            if (scan_panel(x, &g, T, VBOT, &a1, &a2, &a3, &a4, &done, &done_v2)) {
+                V1I = a1; V2I = a2; DVI = a3; NNI = a4;
+            } else {
+                eof = 1;
+                T[NNI + 1] = 2.0 * T[NNI] - T[NNI - 1];
+                T[NNI + 2] = 0.0;
+                V2I = V2I + DVI;
+            }
+        }
+    }
+    /* label 80: zero fill to V2 */
+    for (;;) {
+        int J1 = J2 + 1;
+        J2 = (int)((V2 - V1J) / DVJ + 1.0001);
+        if (J2 > 2400) J2 = 2400;
+        for (int J = J1; J <= J2; J++) R[J] = 0.0;
+        VJ = V1J + DVJ * (double)J2;
+        if (J2 >= 2400 || VJ >= V2) {
+            V2J = V1J + DVJ * (double)(J2 - 1);
+            EMIT_PANEL(J2);
+            V1J = V2J + DVJ;
+            J2 = 0;
+        }
+        if (!(VJ < V2)) break;
+    }
+done_interp:
+#undef EMIT_PANEL
+    if (v1res) *v1res = V1;
+    return nout;
+}
+
+/* The bookkeeping of clblm_FFTSCN_struct for a spectrum on v1c + i dv (i < n),
+ * scan function functID (2..8: triangle, Gaussian, sinc**2, sinc, Beer, Hamming,
+ * Hanning) of half width hwhm, limits v1out..v2out.  The re-gridded spectrum (when
+ * pre_interp) starts at the grid point returned in v1z with dv = dvn.  Returns 0,
+ * or -1 with a message. */
+int orc_fftscn_plan(double v1c, double dv, int64_t n, double v1out, double v2out, int funct_id, double hwhm,
+                    orc_scan_plan *p) {
+    memset(p, 0, sizeof *p);
+    int JFN = funct_id - 1;
+    if (JFN < 1 || JFN > 7) return fail("scan function not supported (2..8 only)");
+    p->jfn = JFN;
+    p->hwhm = hwhm;
+    p->a = SCAN_C[JFN] * hwhm;
+    double V2C = v1c + dv * (double)(n - 1);
+    if (v1out >= v2out) return fail("v1out >= v2out");
+    if (v1out >= V2C || v2out <= v1c) return fail("v1out..v2out outside the data");
+    double V1Z = v1out - hwhm * SCAN_CLIMIT[JFN], V2Z = v2out + hwhm * SCAN_CLIMIT[JFN];
+    V1Z = v1c + (double)(int)((V1Z - v1c) / dv) * dv;
+    V2Z = v1c + (double)(int)((V2Z - v1c) / dv + 1.01) * dv;
+    if (V1Z < v1c) V1Z = v1c;
+    if (V2Z > V2C) V2Z = V2C;
+    if ((V2Z - V1Z) / hwhm < SCAN_CLIMIT[JFN]) return fail("insufficient data for the scan");
+    double DVN = 2. * hwhm / 12;
+    double DV = dv, V1P = v1c;
+    p->dvn = DVN;
+    if (DV > DVN) {
+        p->pre_interp = 1;
+        /* Intpdr(spect, V1Z, V2Z, DVN): the re-gridded spectrum starts at the clamped
+         * V1Z with DVN; its end follows from the point count (computed by the caller) */
+        double a = V1Z;
+        if (a < v1c) {
+            double AN = (v1c - a) / DVN;
+            if (AN == (double)(int)AN) a = a + DVN * (double)(int)AN;
+            else a = a + DVN * (double)((int)AN + 1);
+        }
+        V1Z = a;
+        DV = DVN;
+        V1P = a;
+        p->v2z = V2Z; /* the caller replaces it with the re-gridded spectrum's end */
+    }
+    p->v1z = V1Z;
+    p->dv = DV;
+    if (!p->pre_interp) p->v2z = V2Z;
+    (void)V1P;
+    return 0;
+}
+
+/* Second half of the bookkeeping once the scan-grid spectrum is known (v1p, its
+ * n points): LoadSpect's count and record rounding, the transform length, and
+ * wrtSpect's window. */
+int orc_fftscn_finish(orc_scan_plan *p, double v1p, int64_t np, double v1out, double v2out) {
+    const double DV = p->dv;
+    double V1Z = p->v1z, V2Z = p->v2z;
+    if (p->pre_interp) {
+        V1Z = v1p;
+        V2Z = v1p + DV * (double)(np - 1);
+    }
+    if ((V2Z - V1Z) / p->hwhm > SCAN_CRATIO[p->jfn]) {
+        /* IVX = 1: analytic apodization */
+    } else {
+        return fail("scan range too short for the analytic apodization (IVX = -1 not supported)");
+    }
+    if (v1p > V1Z) return fail("LoadSpect: first data frequency above V1Z");
+    int64_t LTOTAL = (int64_t)(int)((V2Z - V1Z) / DV + 1.1);
+    V2Z = V1Z + DV * (double)(LTOTAL - 1);
+    double ARDATA = (double)LTOTAL / (double)SCAN_LPTSMX;
+    int64_t LRDATA = (ARDATA - (double)(int)ARDATA == 0.0) ? (int64_t)(int)ARDATA : (int64_t)(int)(ARDATA + 1);
+    double POWER = log((double)LRDATA) / log(2.0);
+    int64_t LREC = (POWER - (double)(int)POWER == 0) ? ((int64_t)1 << (int)(POWER + 2))
+                                                      : ((int64_t)1 << ((int)POWER + 3));
+    int64_t N1L = (int64_t)(int)((V1Z - v1p) / DV + 1.00001);
+    if (N1L + LTOTAL - 1 > np) return fail("LoadSpect: ran out of data");
+    p->v1z = V1Z; p->v2z = V2Z;
+    p->ltotal = LTOTAL;
+    p->n1load = N1L;
+    p->lrec = LREC;
+    p->nfft = LREC * SCAN_LPTSMX;
+    int64_t N1 = (int64_t)(int)((v1out - V1Z) / DV);
+    double V1S = V1Z + (double)(N1 - 1) * DV;
+    if (V1S < V1Z) { N1 = 1; V1S = V1Z; }
+    int64_t N2 = (int64_t)(int)((v2out - V1Z) / DV + 3.01);
+    double V2S = V1Z + (double)(N2 - 1) * DV;
+    if (V2S > V2Z) {
+        N2 = (int64_t)(int)((V2Z - V1Z) / DV + 1.01);
+        V2S = V1Z + (double)(N2 - 1) * DV;
+    }
+    (void)V2S;
+    p->n1 = N1;
+    p->ntotal = N2 - N1 + 1;
+    p->v1s = V1S;
+    return 0;
+}
+
+/* Scntrn with IVX = 1 (clblm_FFT.f90:2217-2229) and Scnfnt's apodization branch:
+ * h[m] for the m-th complex coefficient of the packed real transform, m < nfft/2.
+ * The Nyquist coefficient is multiplied by the imaginary slot of h's first element,
+ * which Scnfnt sets to 0. */
+void orc_fft_apod(const orc_scan_plan *p, double *h) {
+    const int LPTS = SCAN_LPTSMX;
+    const double PI = 2.0 * asin(1.0);
+    const double A = p->a;
+    const double DX = 1.0 / ((double)(LPTS * p->lrec) * p->dv);
+    double X = 0.0;
+    for (int64_t J = 0; J < p->lrec; J++) {
+        const double X0 = X, XSTART = X;
+        double *f = h + J * (LPTS / 2);
+        for (int L = 1; L <= LPTS - 1; L += 2) {
+            double v;
+            switch (p->jfn) {
+            case 1: v = (X == 0) ? 1.0 : (sin(PI * X * A) / (PI * X * A)) * (sin(PI * X * A) / (PI * X * A)); break;
+            case 2: { double t = 2.0 * ((PI * X * A) * (PI * X * A)); v = exp(-(t < 34. ? t : 34.)); } break;
+            case 3: v = (X < 1 / A) ? 1.0 - X * A : 0.0; break;
+            case 4: v = (X < 1.0 / A) ? 1.0 : 0.0; break;
+            case 5: v = (X < 1 / A) ? (1.0 - (X * A) * (X * A)) * (1.0 - (X * A) * (X * A)) : 0.0; break;
+            case 6: v = (X < 1 / A) ? 0.53835685 * (1.0 + 0.857504 * cos(PI * X * A)) : 0.0; break;
+            default: v = (X < 1. / A) ? 0.5 * (1.0 + cos(PI * X * A)) : 0.0; break;
+            }
+            f[(L - 1) / 2] = v;
+            X = X0 + (double)(L + 1) * DX / 2.;
+        }
+        if (p->jfn == 4) {  /* the rectangle's last point, interpolated (Scnfnt :1642-1652) */
+            const double ALEN = 1.0 / A;
+            int N = (int)((ALEN - XSTART) / DX) + 1;
+            double R = (ALEN - XSTART) / DX + 1 - N;
+            if (R <= 0.5) { R = R + 1.0; N = N - 1; }
+            if (N >= 1 && N <= LPTS / 2 - 1) f[N] = R - 0.5;
+        }
+    }
+}

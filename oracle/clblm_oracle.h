@@ -80,6 +80,25 @@ void orc_rlin(double tau, double *w, double *dw);
 void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
                const double *t_lev, double emis, double tskin, int lin, double sec_dn, double *rad);
 
+/* Instrument scan (FFT), see clblm_oracle.c. */
+typedef struct {
+    int jfn;
+    double hwhm, a;            /* A = C(JFN)*HWHM */
+    int pre_interp;            /* input re-gridded to dvn before the transform */
+    double dvn;
+    double v1z, v2z, dv;       /* scan grid (after the optional re-gridding) */
+    int64_t n1load, ltotal;    /* first point (1-based) of the scan-grid spectrum loaded, count */
+    int64_t lrec, nfft;        /* records of LPTSMX and transform length */
+    int64_t n1, ntotal;        /* wrtSpect: first point (1-based) and count on the scan grid */
+    double v1s;
+} orc_scan_plan;
+int64_t orc_scan_interp(const double *x, int64_t n, double v1, double dv, double vv1, double vv2, double dvo,
+                        double *out, int64_t cap, double *v1res);
+int orc_fftscn_plan(double v1c, double dv, int64_t n, double v1out, double v2out, int funct_id, double hwhm,
+                    orc_scan_plan *p);
+int orc_fftscn_finish(orc_scan_plan *p, double v1p, int64_t np, double v1out, double v2out);
+void orc_fft_apod(const orc_scan_plan *p, double *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,3 +120,58 @@ class Oracle:
             raise RuntimeError(L.orc_error().decode())
         self.jrad = par.jrad
         return out[: ns.value], dvu.value
+
+
+# ---- instrument scan (clblm_FFT.f90) ------------------------------------------------
+class ScanPlan(ctypes.Structure):
+    _fields_ = [("jfn", ctypes.c_int), ("hwhm", ctypes.c_double), ("a", ctypes.c_double),
+                ("pre_interp", ctypes.c_int), ("dvn", ctypes.c_double), ("v1z", ctypes.c_double),
+                ("v2z", ctypes.c_double), ("dv", ctypes.c_double), ("n1load", ctypes.c_int64),
+                ("ltotal", ctypes.c_int64), ("lrec", ctypes.c_int64), ("nfft", ctypes.c_int64),
+                ("n1", ctypes.c_int64), ("ntotal", ctypes.c_int64), ("v1s", ctypes.c_double)]
+
+
+def scan_interp(x, v1, dv, vv1, vv2, dvo):
+    """Intpdr/clblm_INTERP (4-point): returns (values, first frequency)."""
+    L = lib()
+    L.orc_scan_interp.restype = ctypes.c_int64
+    L.orc_scan_interp.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_double] * 5 + \
+        [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
+    x = np.ascontiguousarray(x, np.float64)
+    v1r = ctypes.c_double()
+    n = L.orc_scan_interp(x.ctypes.data, x.size, v1, dv, vv1, vv2, dvo, None, 0, ctypes.byref(v1r))
+    out = np.zeros(max(n, 1))
+    L.orc_scan_interp(x.ctypes.data, x.size, v1, dv, vv1, vv2, dvo, out.ctypes.data, n, ctypes.byref(v1r))
+    return out[:n], v1r.value
+
+
+def fft_scan(x, v1, dv, v1out, v2out, dvout, funct_id=5, hwhm=0.6034, fill=0.0):
+    """clblm_FFTSCN_struct with an analytic apodization: returns (values, v1, dv).
+    The transforms are numpy's; everything around them is the C restatement."""
+    L = lib()
+    L.orc_fftscn_plan.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ScanPlan)]
+    L.orc_fftscn_finish.argtypes = [ctypes.POINTER(ScanPlan), ctypes.c_double, ctypes.c_int64, ctypes.c_double,
+                                    ctypes.c_double]
+    L.orc_fft_apod.argtypes = [ctypes.POINTER(ScanPlan), ctypes.c_void_p]
+    x = np.ascontiguousarray(x, np.float64)
+    p = ScanPlan()
+    if L.orc_fftscn_plan(v1, dv, x.size, v1out, v2out, int(funct_id), hwhm, ctypes.byref(p)):
+        raise RuntimeError(L.orc_error().decode())
+    if p.pre_interp:
+        xs, v1p = scan_interp(x, v1, dv, p.v1z, p.v2z, p.dvn)
+    else:
+        xs, v1p = x, v1
+    if L.orc_fftscn_finish(ctypes.byref(p), v1p, xs.size, v1out, v2out):
+        raise RuntimeError(L.orc_error().decode())
+    buf = np.full(p.nfft, fill, np.float64)
+    buf[:p.ltotal] = xs[p.n1load - 1:p.n1load - 1 + p.ltotal]
+    h = np.zeros(p.nfft // 2 + 1)
+    L.orc_fft_apod(ctypes.byref(p), h.ctypes.data)
+    h[-1] = 0.0                                      # Nyquist coefficient
+    y = np.fft.irfft(np.fft.rfft(buf) * h, n=p.nfft)
+    seg = y[p.n1 - 1:p.n1 - 1 + p.ntotal]
+    if dvout > 0:
+        out, v1o = scan_interp(seg, p.v1s, p.dv, v1out, v2out, dvout)
+        return out, v1o, dvout
+    return seg, p.v1s, p.dv

@@ -7,10 +7,58 @@ module ref_bind
    use Module_ConstParam,      only: r8 => kind_r8, clblmMolNames, MXMOL, MXISOTPL
    use Module_ODLAY_CommonSub, only: MOLEC, RADFN, RADFNI, AVRAT, CGAUSS, CF1, CF2, CF3, CER
    use Module_TIPS,            only: TIPS_2003
-   use Module_Spectrum,        only: XINT, interpSpectrum
+   use Module_Spectrum,        only: XINT, interpSpectrum, CLBLM_Spectrum, CLBLM_Spectrum_init
    use Module_EMLAY,           only: planck, planckFnDrv, rlin
+   use Module_FFT,             only: clblm_FFTSCN, Intpdr
    implicit none
 contains
+
+   ! Intpdr (clblm_FFT.f90:2392-2443): sp(1:np) on v1 + (i-1) dv onto vv1..vv2 at dvv
+   subroutine ref_intpdr(np, sp, v1, dv, vv1, vv2, dvv, cap, out, nout, v1o) bind(C, name="ref_intpdr")
+      integer(c_int), value :: np, cap
+      real(c_double), value :: v1, dv, vv1, vv2, dvv
+      real(c_double), intent(in) :: sp(np)
+      real(c_double), intent(out) :: out(cap)
+      integer(c_int), intent(out) :: nout
+      real(c_double), intent(out) :: v1o
+      type(CLBLM_Spectrum) :: s
+      real(r8) :: a, b, c
+      real :: d, e
+      integer :: ierr, m
+      c = v1; d = dv
+      call CLBLM_Spectrum_init(s, c, d, np)
+      s%spect(s%indV1:s%indV2) = sp
+      a = vv1; b = vv2; e = dvv
+      call Intpdr(s, a, b, e, ierr)
+      nout = s%NLIM
+      v1o = s%V1
+      m = min(nout, cap)
+      out(1:m) = s%spect(s%indV1:s%indV1 + m - 1)
+   end subroutine
+
+   ! clblm_FFTSCN_array (clblm_FFT.f90:308-349): scan of the spectrum sp(1:np) on
+   ! v1 + (i-1) dv with scanning function functid of half width hwhm, limits
+   ! v1out..v2out, output grid dvout (<= 0: the scan grid), no pre-boxcar.
+   subroutine ref_fftscn(np, sp, v1, v2, dv, v1out, v2out, dvout, functid, hwhm, fill, cap, out, nout, &
+                         v1o, v2o, dvo) bind(C, name="ref_fftscn")
+      integer(c_int), value :: np, functid, cap
+      real(c_double), value :: v1, v2, dv, v1out, v2out, dvout, hwhm, fill
+      real(c_double), intent(in) :: sp(np)
+      real(c_double), intent(out) :: out(cap)
+      integer(c_int), intent(out) :: nout
+      real(c_double), intent(out) :: v1o, v2o, dvo
+      real, allocatable :: d(:)
+      real(r8) :: a, b, c1, c2
+      real :: dvx, dvs, hw, fl, parms(3)
+      integer :: n
+      a = v1; b = v2; c1 = v1out; c2 = v2out; dvx = dvout; dvs = dv; hw = hwhm; fl = fill
+      parms = 0.
+      n = 0
+      call clblm_FFTSCN(sp, a, b, dvs, np, d, c1, c2, dvx, n, functid, hw, fl, -1.0, .false., parms)
+      nout = n
+      v1o = c1; v2o = c2; dvo = dvx
+      out(1:min(n, cap)) = d(1:min(n, cap))
+   end subroutine
 
    subroutine ref_zeta_tables(out) bind(C, name="ref_zeta_tables")
       real(c_double), intent(out) :: out(102, 6)

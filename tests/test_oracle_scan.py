@@ -1,0 +1,127 @@
+"""Pins the oracle's instrument scan (FFTSCN) against the reference's own compiled
+clblm_FFT.f90 / clblm_ScanFilter.f90 (oracle/_ref), and against closed forms.
+
+* Intpdr/clblm_INTERP (4-point interpolation with the sequentially accumulated
+  fraction and 2400-point panels) is compared bit for bit.
+* clblm_FFTSCN_array is compared for every analytic-apodization scanning function
+  (triangle, Gaussian, sinc**2, sinc, Beer, Hamming, Hanning): identical output
+  grid (count, V1, DV) and values within 1e-10 of the largest value; the
+  remaining difference is the reference's transform (twiddles by recurrence)
+  against numpy's.
+* Closed forms: a constant spectrum scanned with the same constant as fill is
+  multiplied by the apodization's value at zero path difference, and the sinc scan
+  of a narrow Gaussian line is the sinc of the stated half width broadened by the
+  line (the transform of a rectangle times a Gaussian)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref", "libclblm_ref.so")
+needs_ref = pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (reference sources absent)")
+
+D = ctypes.c_double
+I = ctypes.c_int
+P = ctypes.c_void_p
+
+
+@pytest.fixture(scope="module")
+def ref():
+    lib = ctypes.CDLL(REF, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    lib.ref_intpdr.argtypes = [I, P, D, D, D, D, D, I, P, ctypes.POINTER(I), ctypes.POINTER(D)]
+    lib.ref_fftscn.argtypes = [I, P, D, D, D, D, D, D, I, D, D, I, P, ctypes.POINTER(I), ctypes.POINTER(D),
+                               ctypes.POINTER(D), ctypes.POINTER(D)]
+    return lib
+
+
+def _ref_intpdr(lib, x, v1, dv, vv1, vv2, dvv):
+    out = np.zeros(200000)
+    n, a = I(), D()
+    lib.ref_intpdr(x.size, x.ctypes.data, v1, dv, vv1, vv2, dvv, out.size, out.ctypes.data, ctypes.byref(n),
+                   ctypes.byref(a))
+    return out[:n.value], a.value
+
+
+def _ref_scan(lib, x, v1, dv, v1o, v2o, dvo, fid, hw, fill):
+    out = np.zeros(200000)
+    n, a, b, c = I(), D(), D(), D()
+    lib.ref_fftscn(x.size, x.ctypes.data, v1, v1 + dv * (x.size - 1), dv, v1o, v2o, dvo, fid, hw, fill, out.size,
+                   out.ctypes.data, ctypes.byref(n), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return out[:n.value], a.value, c.value
+
+
+def _spectrum(n=20001, seed=3):
+    rng = np.random.default_rng(seed)
+    v = 600.0 + 0.01 * np.arange(n)
+    return v, 1.0 + 0.5 * np.sin(v / 3.0) + 0.3 * np.exp(-((v - 700.0) / 0.05) ** 2) + 0.1 * rng.normal(size=n)
+
+
+@needs_ref
+@pytest.mark.parametrize("vv1,vv2,dvv", [(620.0, 780.0, 0.05), (590.0, 810.0, 0.037), (600.5, 799.9, 0.0031),
+                                         (650.0, 650.5, 0.01), (601.234, 700.111, 0.017)])
+def test_scan_interp_bitwise(ref, vv1, vv2, dvv):
+    from oracle.py import scan_interp
+    x = np.cumsum(np.random.default_rng(3).normal(size=20001)) * 0.01 + 2.0
+    a, va = scan_interp(x, 600.0, 0.01, vv1, vv2, dvv)
+    b, vb = _ref_intpdr(ref, x, 600.0, 0.01, vv1, vv2, dvv)
+    assert va == vb and a.size == b.size
+    assert np.array_equal(a, b)
+
+
+@needs_ref
+@pytest.mark.parametrize("fid", range(2, 9))
+@pytest.mark.parametrize("v1o,v2o,dvo,hw,fill", [(620.0, 780.0, 0.05, 0.6034, 0.0), (650.0, 700.0, 0.0, 0.25, 1.0)])
+def test_fft_scan_matches_reference(ref, fid, v1o, v2o, dvo, hw, fill):
+    from oracle.py import fft_scan
+    _, sp = _spectrum()
+    a, va, da = fft_scan(sp, 600.0, 0.01, v1o, v2o, dvo, fid, hw, fill)
+    b, vb, db = _ref_scan(ref, sp, 600.0, 0.01, v1o, v2o, dvo, fid, hw, fill)
+    assert a.size == b.size and va == vb and da == db
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+
+
+@needs_ref
+def test_fft_scan_coarse_input_is_regridded_first(ref):
+    """DV > 2 HWHM/12: the reference interpolates onto DVN before the transform."""
+    from oracle.py import fft_scan
+    v = 600.0 + 0.2 * np.arange(1001)
+    sp = 1.0 + 0.5 * np.sin(v / 3.0)
+    a, va, _ = fft_scan(sp, 600.0, 0.2, 650.0, 750.0, 0.1, 5, 0.6034, 0.0)
+    b, vb, _ = _ref_scan(ref, sp, 600.0, 0.2, 650.0, 750.0, 0.1, 5, 0.6034, 0.0)
+    assert a.size == b.size and va == vb
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("fid", range(2, 9))
+def test_constant_spectrum_is_scaled_by_the_zero_path_gain(fid):
+    """A constant spectrum (filled with the same constant) is multiplied by the
+    apodization at zero path difference: 1, except Hamming's 0.53835685 (1 + 0.857504)
+    (clblm_FFT.f90:1690)."""
+    from oracle.py import fft_scan
+    sp = np.full(20001, 3.25)
+    y, _, _ = fft_scan(sp, 600.0, 0.01, 650.0, 750.0, 0.0, fid, 0.3, 3.25)
+    gain = 0.53835685 * (1.0 + 0.857504) if fid == 7 else 1.0
+    np.testing.assert_allclose(y, 3.25 * gain, rtol=1e-12)
+
+
+def test_sinc_scan_of_a_narrow_line_is_the_instrument_line_shape():
+    """A Gaussian line of area S and width sigma << HWHM scans to
+    S int_{-L}^{L} exp(-2 pi^2 sigma^2 x^2) cos(2 pi (v - v0) x) dx, the sinc of maximum
+    path difference L = 1/(C4 HWHM) broadened by the line; its half maximum lies at
+    v0 +- HWHM."""
+    from oracle.py import fft_scan
+    dv, hw, S, v0, sig = 0.01, 0.6034, 2.0, 700.003, 0.02
+    v = 600.0 + dv * np.arange(20001)
+    sp = S * np.exp(-0.5 * ((v - v0) / sig) ** 2) / (sig * np.sqrt(2 * np.pi))
+    y, v1, dvy = fft_scan(sp, 600.0, dv, 650.0, 750.0, 0.0, 5, hw, 0.0)
+    L = 1.0 / (3.3148 * hw)
+    vy = v1 + dvy * np.arange(y.size)
+    near = np.abs(vy - v0) < 8.0
+    x = np.linspace(-L, L, 20001)
+    wgt = np.exp(-2 * np.pi ** 2 * sig ** 2 * x ** 2)
+    ils = np.array([S * np.trapezoid(wgt * np.cos(2 * np.pi * (vv - v0) * x), x) for vv in vy[near]])
+    assert np.abs(y[near] - ils).max() <= 2e-3 * ils.max()
+    half = vy[near][y[near] >= 0.5 * y.max()]
+    assert abs(0.5 * (half.max() - half.min()) - hw) < 0.03 * hw
