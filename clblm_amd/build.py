@@ -3,7 +3,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("clblm_od.hip", "clblm_rt.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("clblm_od.hip", "clblm_rt.hip", "clblm_scan.hip")]
 OUT = os.path.join(HERE, "libclblm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -13,8 +13,13 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+# hipFFT for the instrument scan; PyTorch-ROCm loads a copy with the same soname
+# (libhipfft.so.0) at import, so both resolve to one library and one HIP runtime
+LIBS = ["-L/opt/rocm/lib", "-lhipfft"]
+
+
 def build_hip(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-o", OUT] + SRCS
+    cmd = [HIPCC] + FLAGS + ["-o", OUT] + SRCS + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)

@@ -1662,6 +1662,7 @@ struct clblm_ctx {
     double ms_conv_k = 0, ms_f4_k = 0;
     double ms[4] = {0, 0, 0, 0};
     int64_t launches = 0;
+    void *scan = nullptr;                   // instrument-scan state (FFT plans, buffers)
     std::mutex mu;
 };
 
@@ -2184,6 +2185,12 @@ extern "C" int clblm_rt_up_impl(int nlay, int64_t npts, double v1, double dv, co
                                 const double *d_tlay, const double *d_tlev, double emis, double tskin, int lin,
                                 double sec_dn, double *d_rad, double *d_dr_dod, double *d_dr_dt,
                                 double *d_dr_dtskin, double *d_dr_demis, double *d_state, void *stream);
+extern "C" void *clblm_scan_state_new();
+extern "C" void clblm_scan_state_free(void *s);
+extern "C" const char *clblm_scan_error();
+extern "C" int clblm_fft_scan_impl(void *state, const double *d_in, int64_t n, double v1, double dv, double v1out,
+                                   double v2out, double dvout, int funct_id, double hwhm, double fill, int precision,
+                                   double *d_out, int64_t *nout, double *v1res, double *dvres, void *stream);
 
 extern "C" {
 
@@ -2235,6 +2242,7 @@ int clblm_ctx_destroy(clblm_ctx *c) {
         if (c->pin[i]) (void)hipHostFree(c->pin[i]);
     }
     (void)hipHostFree(c->h_err);
+    if (c->scan) clblm_scan_state_free(c->scan);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -2382,6 +2390,24 @@ int clblm_odlay(clblm_ctx *c, int handle, const clblm_layer *layer, const clblm_
     }
     (void)hipFree(d);
     return rc;
+}
+
+int clblm_fft_scan(clblm_ctx *c, const double *d_in, int64_t n, double v1, double dv, double v1out, double v2out,
+                   double dvout, int funct_id, double hwhm, double fill, int precision, double *d_out,
+                   int64_t *nout, double *v1res, double *dvres, void *stream) {
+    if (!c || !nout || !v1res || !dvres) return set_err(CLBLM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->scan) c->scan = clblm_scan_state_new();
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (d_out && c->last_stream && c->last_stream != st && c->ncalls > 0) {  // after the batches that made d_in
+        const int kp = (int)((c->ncalls + clblm_ctx::NSLOT - 1) % clblm_ctx::NSLOT);
+        HIPCHK(hipStreamWaitEvent(st, c->slot[kp].done, 0));
+    }
+    int rc = clblm_fft_scan_impl(c->scan, d_in, n, v1, dv, v1out, v2out, dvout, funct_id, hwhm, fill, precision,
+                                 d_out, nout, v1res, dvres, st);
+    if (rc) return set_err(rc, clblm_scan_error());
+    return 0;
 }
 
 int clblm_timing(clblm_ctx *c, int reset, double *ms_prep, double *ms_conv, double *ms_f4, double *ms_panel,

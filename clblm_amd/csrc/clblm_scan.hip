@@ -1,0 +1,597 @@
+// Instrument scan on the GPU: clblm_FFTSCN_struct (clblm_FFT.f90:352-844) for the
+// scanning functions whose apodization is evaluated analytically (IVX = 1):
+// triangle, Gaussian, sinc**2, sinc, Beer, Hamming, Hanning.
+//
+//   host    the scan bookkeeping (V1Z/V2Z, LoadSpect's LTOTAL and record rounding,
+//           wrtSpect's window) and the control flow of the 4-point interpolation
+//           (Intpdr/clblm_INTERP/panelInput), which depends on the grids only and is
+//           cut into segments that restart the accumulated fraction
+//   device  load + fill, real-to-complex FFT (hipFFT), apodization (Scntrn/Scnfnt),
+//           complex-to-real FFT, window, and one lane per interpolation segment
+//
+// The reference transforms LREC*65536 points (LREC = 4 x the next power of two of
+// the data records, clblm_FFT.f90:1183-1197); the same length is used here, so the
+// wrap-around of the periodic convolution is the reference's.
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/clblm_hip.h"
+
+namespace {
+
+constexpr int LPTSMX = 65536;
+// Block Data Apod_Init (clblm_FFT.f90:120-146)
+constexpr double SC_C[14] = {0.0, 2.0, 0.849322, 2.257609, 3.314800, 2.100669, 2.195676,
+                             2.0, 2.570274, 2.367714, 2.071759, -3.3, -2.5, 3.314800};
+constexpr double SC_CRATIO[14] = {0.0, 40., 10., 40., 160., 20., 20., 20., 40., 40., 20., 100., 10., 1e300};
+constexpr double SC_CLIMIT[14] = {1., 2., 3., 40., 160., 20., 20., 20., 40., 40., 20., 100., 10., 160.};
+
+thread_local std::string g_scan_err;
+int fail(int code, const std::string &m) {
+    g_scan_err = m;
+    return code;
+}
+
+// ---- interpolation plan ---------------------------------------------------------
+// One segment: output points [out0, out0 + count) computed from input panel s (T(5)
+// is input point s) with the fraction restarted at (I0, p0).  kind 1 writes zeros.
+struct ISeg {
+    int64_t out0;
+    int64_t s;
+    double p0;
+    int32_t count, kind, I0, nni, flags;  // flags: 1 first panel, 2 input exhausted
+    int32_t pad;
+};
+
+struct IPlan {
+    double v1 = 0., dvo = 0., vratio = 0.;
+    int64_t nout = 0;
+    std::vector<ISeg> segs;
+};
+
+// Intpdr (clblm_FFT.f90:2392-2443) + clblm_INTERP (clblm_ScanFilter.f90:1975-2300) +
+// panelInput (:2302-2370): the input x[0..n) on v1 + i dv, output from vv1 at dvo up to vv2.
+int plan_interp(int64_t n, double v1, double dv, double vv1, double vv2, double dvo, IPlan &ip) {
+    if (n < 8) return fail(CLBLM_ERR_ARG, "interpolation needs at least 8 input points");
+    if (!(dvo > 0.) || !(dv > 0.)) return fail(CLBLM_ERR_ARG, "grid steps must be positive");
+    const double V1C = v1, V2C = v1 + dv * (double)(n - 1);
+    double VV1 = vv1, VV2 = vv2;
+    if (VV1 < V1C) {
+        double AN = (V1C - VV1) / dvo;
+        VV1 = (AN == (double)(int)AN) ? VV1 + dvo * (double)(int)AN : VV1 + dvo * (double)((int)AN + 1);
+    }
+    if (VV2 > V2C) {
+        double AN = (VV2 - V2C) / dvo;
+        VV2 = (AN == (double)(int)AN) ? VV2 - dvo * (double)(int)AN : VV2 - dvo * (double)((int)AN + 1);
+    }
+    const double V1 = VV1, V2 = VV2;
+    const double VBOT = V1 - 4.0 * dvo;
+    int64_t done = 0;
+    double done_v2 = v1 - dv;
+    // next 2400-point panel reaching VBOT: T(5) = input point *s, T(1) at *vmin
+    auto panel = [&](int64_t *s, int *nni, double *vmin, double *vmax, double *dvi) {
+        for (;;) {
+            int nlim = 2400;
+            if (done + nlim > n) nlim = (int)(n - done);
+            if (nlim == 0) return false;
+            const double d = dv;
+            const double a = done_v2 + d;
+            const double b = a + (double)(nlim - 1) * d;
+            done += nlim;
+            done_v2 = b;
+            if (b >= VBOT) {
+                *s = done - nlim;
+                *nni = nlim + 4;
+                *vmin = a - (double)(4) * d;
+                *vmax = b;
+                *dvi = d;
+                return true;
+            }
+        }
+    };
+    int64_t s = 0;
+    int NNI = 0;
+    double V1I = 0., V2I = 0., DVI = 0.;
+    if (!panel(&s, &NNI, &V1I, &V2I, &DVI)) return fail(CLBLM_ERR_ARG, "interpolation range lies beyond the data");
+    if (NNI < 6) return fail(CLBLM_ERR_ARG, "interpolation range starts on the last input point");
+    int flags = 1;
+    const double DVJ = dvo;
+    double V1J = V1, VJ = V1J, V2J;
+    const double VRATIO = DVJ / DVI;
+    int64_t nout = 0;
+    auto seg = [&](int64_t out0, int count, int kind, int I0, double p0) {
+        if (count <= 0) return;
+        ISeg g{};
+        g.out0 = out0; g.count = count; g.kind = kind; g.I0 = I0; g.p0 = p0; g.s = s; g.nni = NNI; g.flags = flags;
+        ip.segs.push_back(g);
+    };
+    while (V1J < V1I) {  // zero fill below the data (label 20)
+        int J2 = (int)((V1I - V1J) / DVJ) + 1;
+        if (J2 > 2400) J2 = 2400;
+        seg(nout, J2, 1, 0, 0.);
+        nout += J2;
+        V2J = V1J + DVJ * (double)(J2 - 1);
+        V1J = V2J + DVJ;
+        VJ = V1J;
+    }
+    int J2 = 0;
+    bool eof = false, fin = false;
+    for (;;) {  // label 40
+        int I = (int)((VJ - V1I) / DVI + 1.00001);
+        if (I < 1) return fail(CLBLM_ERR_ARG, "interpolation index below the panel");
+        const double VI = V1I + DVI * (double)(I - 1);
+        double P = (VJ - VI) / DVI;
+        const int J1 = (int)((VJ - V1J) / DVJ + 1.001);
+        const int ja = (int)((V2 - V1J) / DVJ + 1.001), jb = (int)((V2I - DVI - V1J) / DVJ + 1.);
+        J2 = ja < jb ? ja : jb;
+        if (J2 > 2400) J2 = 2400;
+        if (J2 >= J1) {
+            seg(nout + J1 - 1, J2 - J1 + 1, 0, I, P);
+            for (int J = J1; J <= J2; J++) {
+                P = P + VRATIO;
+                if (P >= 1.0) {
+                    I = (int)((double)I + P);
+                    P = P - (double)(int)P;
+                }
+            }
+        }
+        VJ = V1J + DVJ * (double)J2;
+        if (J2 >= 2400 || VJ >= V2) {
+            nout += J2;
+            V2J = V1J + DVJ * (double)(J2 - 1);
+            V1J = V2J + DVJ;
+            J2 = 0;
+        }
+        if (VJ >= V2) { fin = true; break; }
+        if (eof) break;
+        if (I >= NNI - 2) {
+            int64_t s2;
+            int n2;
+            double a1, a2, a3;
+            if (panel(&s2, &n2, &a1, &a2, &a3)) {
+                s = s2; NNI = n2; V1I = a1; V2I = a2; DVI = a3;
+                flags = 0;
+            } else {  // the reference keeps the panel, shifts T(1..4) and extends it by one point
+                eof = true;
+                flags |= 2;
+                V2I = V2I + DVI;
+            }
+        }
+    }
+    if (!fin) {
+        for (;;) {  // zero fill above the data (label 80)
+            const int J1 = J2 + 1;
+            J2 = (int)((V2 - V1J) / DVJ + 1.0001);
+            if (J2 > 2400) J2 = 2400;
+            seg(nout + J1 - 1, J2 - J1 + 1, 1, 0, 0.);
+            VJ = V1J + DVJ * (double)J2;
+            if (J2 >= 2400 || VJ >= V2) {
+                nout += J2;
+                V2J = V1J + DVJ * (double)(J2 - 1);
+                V1J = V2J + DVJ;
+                J2 = 0;
+            }
+            if (!(VJ < V2)) break;
+        }
+    }
+    ip.v1 = V1;
+    ip.dvo = dvo;
+    ip.vratio = VRATIO;
+    ip.nout = nout;
+    return 0;
+}
+
+struct ICoef {
+    double c1[203], c2[203], c3[203], c4[203];
+};
+
+// four-point coefficients for 201 fractions plus one at each end (clblm_ScanFilter.f90:2054-2063)
+ICoef make_coef() {
+    ICoef c;
+    const double XNUMCF = 201.;
+    for (int ip = 0; ip <= 202; ip++) {
+        double P = ((double)ip - 1.0) / (XNUMCF - 1.0);
+        double PP = P * P;
+        c.c1[ip] = -(P / 2.0 * ((1 - P) * (1 - P)));
+        c.c2[ip] = 1.0 - PP * (3.0 - 2.0 * P) + PP / 2.0 * (1.0 - P);
+        c.c3[ip] = PP * (3.0 - 2.0 * P) + P / 2.0 * ((1.0 - P) * (1.0 - P));
+        c.c4[ip] = -(PP / 2. * (1.0 - P));
+    }
+    return c;
+}
+
+// T(k) of the segment's panel.  T(5..NNI) are input points s.., T(4) of the first
+// panel is extrapolated, and once the input is exhausted T(1..4) hold T(NNI-3..NNI),
+// T(NNI+1) = 2 T(NNI) - T(NNI-1) and T(NNI+2) = 0.
+template <typename TX>
+__device__ __forceinline__ double tval(const TX *x, const ISeg &g, int k) {
+    const bool eof = (g.flags & 2) != 0;
+    if (eof && k <= 4) return (double)x[g.s + g.nni - 9 + k];
+    if ((g.flags & 1) && k <= 4) return k == 4 ? 2.0 * (double)x[g.s] - (double)x[g.s + 1] : 0.0;
+    if (eof && k == g.nni + 2) return 0.0;
+    if (eof && k == g.nni + 1) {
+        const double tn = (double)x[g.s + g.nni - 5];
+        const int km = g.nni - 1;
+        const double tm = km <= 4 ? (double)x[g.s + g.nni - 9 + km] : (double)x[g.s + km - 5];
+        return 2.0 * tn - tm;
+    }
+    // beyond the panel the reference holds stale values, which it reads only with a
+    // zero coefficient (the fraction is 0 at the panel's last usable point)
+    if (k > g.nni) return 0.0;
+    return (double)x[g.s + k - 5];
+}
+
+template <typename TX>
+__global__ void k_scan_interp(const ISeg *segs, int64_t nseg, const TX *x, const ICoef *cf, double vratio,
+                              double *out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nseg) return;
+    const ISeg g = segs[q];
+    double *o = out + g.out0;
+    if (g.kind == 1) {
+        for (int j = 0; j < g.count; j++) o[j] = 0.0;
+        return;
+    }
+    int I = g.I0;
+    double P = g.p0;
+    for (int j = 0; j < g.count; j++) {
+        const int IP = (int)(P * 201. + 1.00001);
+        o[j] = cf->c1[IP] * tval(x, g, I - 1) + cf->c2[IP] * tval(x, g, I) + cf->c3[IP] * tval(x, g, I + 1) +
+               cf->c4[IP] * tval(x, g, I + 2);
+        P = P + vratio;
+        if (P >= 1.0) {
+            I = (int)((double)I + P);
+            P = P - (double)(int)P;
+        }
+    }
+}
+
+// ---- transform, apodization --------------------------------------------------------
+template <typename TB>
+__global__ void k_scan_load(const double *xs, int64_t first, int64_t ltotal, double fill, TB *buf, int64_t nfft) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nfft) return;
+    buf[i] = (TB)(i < ltotal ? xs[first + i] : fill);
+}
+
+// Scnfnt's apodization branch (clblm_FFT.f90:1411-1958) for coefficient m of block J:
+// X = X0(J) + m' DX with X0 accumulated block by block as the reference does.
+__device__ inline double apod(int jfn, double X, double A) {
+    const double PI = 2.0 * asin(1.0);
+    switch (jfn) {
+    case 1: {
+        if (X == 0) return 1.0;
+        const double s = sin(PI * X * A) / (PI * X * A);
+        return s * s;
+    }
+    case 2: {
+        const double t = 2.0 * ((PI * X * A) * (PI * X * A));
+        return exp(-(t < 34. ? t : 34.));
+    }
+    case 3: return (X < 1 / A) ? 1.0 - X * A : 0.0;
+    case 4: return (X < 1.0 / A) ? 1.0 : 0.0;
+    case 5: return (X < 1 / A) ? (1.0 - (X * A) * (X * A)) * (1.0 - (X * A) * (X * A)) : 0.0;
+    case 6: return (X < 1 / A) ? 0.53835685 * (1.0 + 0.857504 * cos(PI * X * A)) : 0.0;
+    default: return (X < 1. / A) ? 0.5 * (1.0 + cos(PI * X * A)) : 0.0;
+    }
+}
+
+struct ApodBlock {
+    double x0;     // X at the block's first coefficient
+    int32_t nend;  // sinc: coefficient whose value is the interpolated rectangle end (-1: none)
+    int32_t pad;
+    double vend;
+};
+
+template <typename TC, typename TS>
+__global__ void k_scan_apod(TC *c, int64_t nfft, const ApodBlock *blk, int jfn, double A, double DX,
+                            double scale) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t half = nfft / 2;
+    if (m > half) return;
+    double h;
+    if (m == half) {
+        h = 0.0;  // Nyquist: multiplied by the imaginary slot of the first coefficient (0)
+    } else {
+        const int64_t J = m / (LPTSMX / 2);
+        const int kb = (int)(m % (LPTSMX / 2));
+        const ApodBlock b = blk[J];
+        h = (kb == b.nend) ? b.vend : apod(jfn, b.x0 + (double)kb * DX, A);
+    }
+    const double f = h * scale;
+    c[m].x = (TS)((double)c[m].x * f);
+    c[m].y = (TS)((double)c[m].y * f);
+}
+
+template <typename TB>
+__global__ void k_scan_window(const TB *y, int64_t first, int64_t n, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)y[first + i];
+}
+
+// ---- per-context state -------------------------------------------------------------
+struct ScanState {
+    int64_t plan_n = 0;
+    int plan_prec = -1;
+    hipfftHandle fwd = 0, inv = 0;
+    void *buf = nullptr;           // nfft + 2 reals (in-place transforms)
+    size_t buf_cap = 0;
+    double *xs = nullptr;          // re-gridded input
+    size_t xs_cap = 0;
+    double *win = nullptr;         // scan window before the output interpolation
+    size_t win_cap = 0;
+    void *meta = nullptr;          // segments, apodization blocks, coefficients
+    size_t meta_cap = 0;
+    ~ScanState() {
+        if (fwd) hipfftDestroy(fwd);
+        if (inv) hipfftDestroy(inv);
+        (void)hipFree(buf);
+        (void)hipFree(xs);
+        (void)hipFree(win);
+        (void)hipFree(meta);
+    }
+};
+
+int grow(void **p, size_t *cap, size_t bytes) {
+    if (bytes <= *cap) return 0;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, bytes) != hipSuccess) return fail(CLBLM_ERR_HIP, "scan buffer allocation failed");
+    *cap = bytes;
+    return 0;
+}
+
+// The scan's host-side plan.
+struct ScanPlan {
+    int jfn = 0;
+    double a = 0.;
+    bool pre = false;
+    double dvn = 0.;
+    IPlan pre_ip;          // re-gridding of the input (DV > 2 HWHM / 12)
+    double v1p = 0.;       // scan-grid spectrum: start, step, points
+    double dv = 0.;
+    int64_t np = 0;
+    double v1z = 0., v2z = 0.;
+    int64_t ltotal = 0, n1load = 0, lrec = 0, nfft = 0;
+    int64_t n1 = 0, ntotal = 0;
+    double v1s = 0.;
+    bool out_interp = false;
+    IPlan out_ip;
+    std::vector<ApodBlock> blocks;
+};
+
+int plan_scan(int64_t n, double v1, double dv, double v1out, double v2out, double dvout, int funct_id, double hwhm,
+              ScanPlan &p) {
+    const int JFN = funct_id - 1;
+    if (JFN < 1 || JFN > 7)
+        return fail(CLBLM_ERR_UNSUPPORTED, "scan functions 2..8 (triangle .. Hanning) are supported");
+    if (!(hwhm > 0.) || !(dv > 0.) || n < 8) return fail(CLBLM_ERR_ARG, "hwhm, dv must be positive and n >= 8");
+    p.jfn = JFN;
+    p.a = SC_C[JFN] * hwhm;
+    const double V1C = v1, V2C = v1 + dv * (double)(n - 1);
+    if (v1out >= v2out) return fail(CLBLM_ERR_ARG, "v1out >= v2out");
+    if (v1out >= V2C || v2out <= V1C) return fail(CLBLM_ERR_ARG, "v1out..v2out lies outside the data");
+    double V1Z = v1out - hwhm * SC_CLIMIT[JFN], V2Z = v2out + hwhm * SC_CLIMIT[JFN];
+    V1Z = V1C + (double)(int)((V1Z - V1C) / dv) * dv;
+    V2Z = V1C + (double)(int)((V2Z - V1C) / dv + 1.01) * dv;
+    if (V1Z < V1C) V1Z = V1C;
+    if (V2Z > V2C) V2Z = V2C;
+    if ((V2Z - V1Z) / hwhm < SC_CLIMIT[JFN]) return fail(CLBLM_ERR_ARG, "insufficient data for an accurate scan");
+    const double DVN = 2. * hwhm / 12;
+    p.dvn = DVN;
+    double DV = dv;
+    if (DV > DVN) {  // Intpdr onto DVN first (clblm_FFT.f90:594-614)
+        p.pre = true;
+        int rc = plan_interp(n, v1, dv, V1Z, V2Z, DVN, p.pre_ip);
+        if (rc) return rc;
+        DV = DVN;
+        p.v1p = p.pre_ip.v1;
+        p.np = p.pre_ip.nout;
+        V1Z = p.v1p;
+        V2Z = p.v1p + DV * (double)(p.np - 1);
+    } else {
+        p.v1p = v1;
+        p.np = n;
+    }
+    p.dv = DV;
+    if (!((V2Z - V1Z) / hwhm > SC_CRATIO[JFN]))
+        return fail(CLBLM_ERR_UNSUPPORTED,
+                    "scan range under CRATIO half widths: the apodization would be the transform of the scanning "
+                    "function (IVX = -1), which is not supported");
+    if (p.v1p > V1Z) return fail(CLBLM_ERR_ARG, "first data frequency above the scan start");
+    // LoadSpect (clblm_FFT.f90:1166-1206)
+    const int64_t LTOTAL = (int64_t)(int)((V2Z - V1Z) / DV + 1.1);
+    V2Z = V1Z + DV * (double)(LTOTAL - 1);
+    const double ARDATA = (double)LTOTAL / (double)LPTSMX;
+    const int64_t LRDATA = (ARDATA - (double)(int)ARDATA == 0.0) ? (int64_t)(int)ARDATA : (int64_t)(int)(ARDATA + 1);
+    const double POWER = std::log((double)LRDATA) / std::log(2.0);
+    const int64_t LREC = (POWER - (double)(int)POWER == 0) ? ((int64_t)1 << (int)(POWER + 2))
+                                                            : ((int64_t)1 << ((int)POWER + 3));
+    const int64_t N1L = (int64_t)(int)((V1Z - p.v1p) / DV + 1.00001);
+    if (N1L < 1 || N1L + LTOTAL - 1 > p.np) return fail(CLBLM_ERR_ARG, "scan window runs out of data");
+    p.v1z = V1Z;
+    p.v2z = V2Z;
+    p.ltotal = LTOTAL;
+    p.n1load = N1L;
+    p.lrec = LREC;
+    p.nfft = LREC * LPTSMX;
+    // wrtSpect window (clblm_FFT.f90:776-796)
+    int64_t N1 = (int64_t)(int)((v1out - V1Z) / DV);
+    double V1S = V1Z + (double)(N1 - 1) * DV;
+    if (V1S < V1Z) { N1 = 1; V1S = V1Z; }
+    int64_t N2 = (int64_t)(int)((v2out - V1Z) / DV + 3.01);
+    const double V2S = V1Z + (double)(N2 - 1) * DV;
+    if (V2S > V2Z) N2 = (int64_t)(int)((V2Z - V1Z) / DV + 1.01);
+    p.n1 = N1;
+    p.ntotal = N2 - N1 + 1;
+    p.v1s = V1S;
+    if (p.n1 - 1 + p.ntotal > p.nfft || p.ntotal < 1) return fail(CLBLM_ERR_ARG, "scan window outside the transform");
+    if (dvout > 0.) {
+        p.out_interp = true;
+        int rc = plan_interp(p.ntotal, V1S, DV, v1out, v2out, dvout, p.out_ip);
+        if (rc) return rc;
+    }
+    // Scntrn, IVX = 1 (clblm_FFT.f90:2217-2229): X per block, sinc end point per block
+    const double DX = 1.0 / ((double)(LPTSMX * LREC) * DV);
+    p.blocks.resize((size_t)LREC);
+    double X = 0.0;
+    for (int64_t J = 0; J < LREC; J++) {
+        ApodBlock &b = p.blocks[(size_t)J];
+        b.x0 = X;
+        b.nend = -1;
+        b.vend = 0.;
+        if (JFN == 4) {
+            const double ALEN = 1.0 / p.a, XSTART = X;
+            int N = (int)((ALEN - XSTART) / DX) + 1;
+            double R = (ALEN - XSTART) / DX + 1 - N;
+            if (R <= 0.5) { R = R + 1.0; N = N - 1; }
+            if (N >= 1 && N <= LPTSMX / 2 - 1) { b.nend = N; b.vend = R - 0.5; }
+        }
+        X = X + (double)(LPTSMX) * DX / 2.;
+    }
+    return 0;
+}
+
+unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+int ensure_plans(ScanState &S, int64_t nfft, int prec, hipStream_t st) {
+    if (S.plan_n == nfft && S.plan_prec == prec) {
+        if (hipfftSetStream(S.fwd, st) != HIPFFT_SUCCESS || hipfftSetStream(S.inv, st) != HIPFFT_SUCCESS)
+            return fail(CLBLM_ERR_HIP, "hipfftSetStream failed");
+        return 0;
+    }
+    if (S.fwd) hipfftDestroy(S.fwd);
+    if (S.inv) hipfftDestroy(S.inv);
+    S.fwd = S.inv = 0;
+    S.plan_n = 0;
+    const bool d = prec == CLBLM_FP64;
+    if (hipfftPlan1d(&S.fwd, (int)nfft, d ? HIPFFT_D2Z : HIPFFT_R2C, 1) != HIPFFT_SUCCESS ||
+        hipfftPlan1d(&S.inv, (int)nfft, d ? HIPFFT_Z2D : HIPFFT_C2R, 1) != HIPFFT_SUCCESS)
+        return fail(CLBLM_ERR_HIP, "hipFFT plan creation failed");
+    if (hipfftSetStream(S.fwd, st) != HIPFFT_SUCCESS || hipfftSetStream(S.inv, st) != HIPFFT_SUCCESS)
+        return fail(CLBLM_ERR_HIP, "hipfftSetStream failed");
+    S.plan_n = nfft;
+    S.plan_prec = prec;
+    return 0;
+}
+
+#define SCHK(x)                                                                        \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) return fail(CLBLM_ERR_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename TB, typename TC>
+int run_transform(ScanState &S, const ScanPlan &p, const double *xs, double fill, const ApodBlock *d_blk,
+                  hipStream_t st) {
+    TB *buf = (TB *)S.buf;
+    hipLaunchKernelGGL(k_scan_load<TB>, dim3(cdiv(p.nfft, 256)), dim3(256), 0, st, xs, p.n1load - 1, p.ltotal, fill,
+                       buf, p.nfft);
+    hipfftResult r;
+    if constexpr (sizeof(TB) == 8)
+        r = hipfftExecD2Z(S.fwd, (hipfftDoubleReal *)buf, (hipfftDoubleComplex *)buf);
+    else
+        r = hipfftExecR2C(S.fwd, (hipfftReal *)buf, (hipfftComplex *)buf);
+    if (r != HIPFFT_SUCCESS) return fail(CLBLM_ERR_HIP, "forward FFT failed");
+    const double DX = 1.0 / ((double)(LPTSMX * p.lrec) * p.dv);
+    hipLaunchKernelGGL((k_scan_apod<TC, TB>), dim3(cdiv(p.nfft / 2 + 1, 256)), dim3(256), 0, st, (TC *)buf, p.nfft, d_blk,
+                       p.jfn, p.a, DX, 1.0 / (double)p.nfft);
+    if constexpr (sizeof(TB) == 8)
+        r = hipfftExecZ2D(S.inv, (hipfftDoubleComplex *)buf, (hipfftDoubleReal *)buf);
+    else
+        r = hipfftExecC2R(S.inv, (hipfftComplex *)buf, (hipfftReal *)buf);
+    if (r != HIPFFT_SUCCESS) return fail(CLBLM_ERR_HIP, "inverse FFT failed");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" __attribute__((visibility("hidden"))) void *clblm_scan_state_new() { return new ScanState; }
+extern "C" __attribute__((visibility("hidden"))) void clblm_scan_state_free(void *s) { delete (ScanState *)s; }
+extern "C" __attribute__((visibility("hidden"))) const char *clblm_scan_error() { return g_scan_err.c_str(); }
+
+extern "C" __attribute__((visibility("hidden"))) int clblm_fft_scan_impl(
+    void *state, const double *d_in, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
+    int funct_id, double hwhm, double fill, int precision, double *d_out, int64_t *nout, double *v1res,
+    double *dvres, void *stream) {
+    if (precision != CLBLM_FP64 && precision != CLBLM_FP32C) return fail(CLBLM_ERR_ARG, "unknown precision");
+    ScanPlan p;
+    int rc = plan_scan(n, v1, dv, v1out, v2out, dvout, funct_id, hwhm, p);
+    if (rc) return rc;
+    if (p.out_interp) {
+        *nout = p.out_ip.nout;
+        *v1res = p.out_ip.v1;
+        *dvres = dvout;
+    } else {
+        *nout = p.ntotal;
+        *v1res = p.v1s;
+        *dvres = p.dv;
+    }
+    if (!d_out) return 0;
+    if (!d_in) return fail(CLBLM_ERR_ARG, "input spectrum is NULL");
+    ScanState &S = *(ScanState *)state;
+    hipStream_t st = (hipStream_t)stream;
+    // metadata: coefficients, apodization blocks, both segment lists
+    const size_t o_cf = 0;
+    const size_t o_blk = (sizeof(ICoef) + 255) & ~(size_t)255;
+    const size_t o_s1 = (o_blk + sizeof(ApodBlock) * p.blocks.size() + 255) & ~(size_t)255;
+    const size_t o_s2 = (o_s1 + sizeof(ISeg) * p.pre_ip.segs.size() + 255) & ~(size_t)255;
+    const size_t meta = o_s2 + sizeof(ISeg) * p.out_ip.segs.size() + 256;
+    // the metadata arrays are read by the kernels below: finish the previous scan first
+    SCHK(hipStreamSynchronize(st));
+    rc = grow(&S.meta, &S.meta_cap, meta);
+    if (rc) return rc;
+    std::vector<char> host(meta, 0);
+    const ICoef cf = make_coef();
+    std::memcpy(host.data() + o_cf, &cf, sizeof cf);
+    std::memcpy(host.data() + o_blk, p.blocks.data(), sizeof(ApodBlock) * p.blocks.size());
+    if (!p.pre_ip.segs.empty()) std::memcpy(host.data() + o_s1, p.pre_ip.segs.data(), sizeof(ISeg) * p.pre_ip.segs.size());
+    if (!p.out_ip.segs.empty()) std::memcpy(host.data() + o_s2, p.out_ip.segs.data(), sizeof(ISeg) * p.out_ip.segs.size());
+    SCHK(hipMemcpyAsync(S.meta, host.data(), meta, hipMemcpyHostToDevice, st));
+    SCHK(hipStreamSynchronize(st));  // the host copy of the metadata goes out of scope
+    char *mb = (char *)S.meta;
+    const ICoef *d_cf = (const ICoef *)(mb + o_cf);
+    const double *xs = d_in;
+    if (p.pre) {
+        rc = grow((void **)&S.xs, &S.xs_cap, sizeof(double) * (size_t)p.np);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_scan_interp<double>, dim3(cdiv((int64_t)p.pre_ip.segs.size(), 64)), dim3(64), 0, st,
+                           (const ISeg *)(mb + o_s1), (int64_t)p.pre_ip.segs.size(), d_in, d_cf, p.pre_ip.vratio,
+                           S.xs);
+        xs = S.xs;
+    }
+    const bool d64 = precision == CLBLM_FP64;
+    rc = grow(&S.buf, &S.buf_cap, (d64 ? 8 : 4) * (size_t)(p.nfft + 2));
+    if (rc) return rc;
+    rc = ensure_plans(S, p.nfft, precision, st);
+    if (rc) return rc;
+    const ApodBlock *d_blk = (const ApodBlock *)(mb + o_blk);
+    rc = d64 ? run_transform<double, hipfftDoubleComplex>(S, p, xs, fill, d_blk, st)
+             : run_transform<float, hipfftComplex>(S, p, xs, fill, d_blk, st);
+    if (rc) return rc;
+    if (!p.out_interp) {
+        if (d64)
+            hipLaunchKernelGGL(k_scan_window<double>, dim3(cdiv(p.ntotal, 256)), dim3(256), 0, st,
+                               (const double *)S.buf, p.n1 - 1, p.ntotal, d_out);
+        else
+            hipLaunchKernelGGL(k_scan_window<float>, dim3(cdiv(p.ntotal, 256)), dim3(256), 0, st,
+                               (const float *)S.buf, p.n1 - 1, p.ntotal, d_out);
+    } else {
+        const ISeg *segs = (const ISeg *)(mb + o_s2);
+        const int64_t ns = (int64_t)p.out_ip.segs.size();
+        if (d64)
+            hipLaunchKernelGGL(k_scan_interp<double>, dim3(cdiv(ns, 64)), dim3(64), 0, st, segs, ns,
+                               (const double *)S.buf + (p.n1 - 1), d_cf, p.out_ip.vratio, d_out);
+        else
+            hipLaunchKernelGGL(k_scan_interp<float>, dim3(cdiv(ns, 64)), dim3(64), 0, st, segs, ns,
+                               (const float *)S.buf + (p.n1 - 1), d_cf, p.out_ip.vratio, d_out);
+    }
+    SCHK(hipGetLastError());
+    return 0;
+}

@@ -120,6 +120,26 @@ class Context:
         self.jrad = jr.value
         return out, off
 
+    # -- instrument scan ---------------------------------------------------------------
+    def fft_scan(self, x, v1, dv, v1out, v2out, dvout=0.0, funct_id=5, hwhm=0.6034, fill=0.0, precision=FP64,
+                 stream=None):
+        """clblm_FFTSCN_array (src/clblm_src/clblm_FFT.f90:308-349) on a float64 cuda
+        tensor x on v1 + i*dv: returns (scanned tensor, its v1, its dv)."""
+        import torch
+        assert x.dtype == torch.float64 and x.is_cuda
+        x = x.contiguous()
+        n, a, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        args = (int(x.numel()), float(v1), float(dv), float(v1out), float(v2out), float(dvout), int(funct_id),
+                float(hwhm), float(fill), int(precision))
+        check(self.lib.clblm_fft_scan(self.ptr, None, *args, None, ctypes.byref(n), ctypes.byref(a),
+                                      ctypes.byref(b), None))
+        out = torch.empty(n.value, dtype=torch.float64, device=x.device)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        check(self.lib.clblm_fft_scan(self.ptr, ctypes.c_void_p(x.data_ptr()), *args, ctypes.c_void_p(out.data_ptr()),
+                                      ctypes.byref(n), ctypes.byref(a), ctypes.byref(b), ctypes.c_void_p(stream)))
+        return out, a.value, b.value
+
     def sync(self):
         """Wait for the enqueued batches; raises the first error found on the device."""
         check(self.lib.clblm_sync(self.ptr))

@@ -122,6 +122,20 @@ int clblm_rt_up(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, co
                 double sec_dn, double *d_rad, double *d_dr_dod, double *d_dr_dtlay, double *d_dr_dtskin,
                 double *d_dr_demis, void *stream);
 
+/* ---- instrument scan ------------------------------------------------------------
+ * Drop-in for clblm_FFTSCN_array (clblm_FFT.f90:308-349) with no boxcar pre-scan:
+ * the spectrum d_in[0..n) (device) on v1 + i*dv is convolved with scanning function
+ * funct_id of half width hwhm (2 triangle, 3 Gaussian, 4 sinc**2, 5 sinc, 6 Beer,
+ * 7 Hamming, 8 Hanning; the apodization is evaluated analytically, which the
+ * reference does when the scanned range exceeds CRATIO half widths) by FFT over the
+ * reference's transform length, the transform padded with `fill`, between v1out
+ * and v2out, then interpolated (4-point) onto dvout when dvout > 0.  Query with
+ * d_out == NULL: *nout, *v1res and *dvres give the output grid; then d_out (device,
+ * *nout doubles) is filled on `stream`.  precision CLBLM_FP32C transforms in FP32. */
+int clblm_fft_scan(clblm_ctx *ctx, const double *d_in, int64_t n, double v1, double dv, double v1out,
+                   double v2out, double dvout, int funct_id, double hwhm, double fill, int precision,
+                   double *d_out, int64_t *nout, double *v1res, double *dvres, void *stream);
+
 /* ---- timing -------------------------------------------------------------------
  * Average device time (ms) of each pipeline stage over the launches since the
  * last reset, measured with events on the stream the stages run on (waits for the
