@@ -129,6 +129,18 @@ def run_jacobian(ctx, h, layers, dvs, precision, block=100.0):
     return t, spectra, npts, len(edges) - 1
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` as measured by the FETCH_SIZE/WRITE_SIZE counter
+    passes summarised under profiles/ (a line `traffic_bytes <kernel>: <bytes>`), or None."""
+    for name in sorted(os.listdir(os.path.join(ROOT, "profiles"))) if os.path.isdir(os.path.join(ROOT, "profiles")) else []:
+        if not name.endswith("_hbm_bytes.md"):
+            continue
+        for ln in open(os.path.join(ROOT, "profiles", name)):
+            if ln.startswith("traffic_bytes %s:" % kernel):
+                return int(ln.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,7 +193,8 @@ def main():
                    "layers": len(layers), "lines": lines.n, "points": int(off[-1]),
                    "precision": args.precision, "line_evals_per_step": E},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                     "frac": ach / peak, "traffic": None,
+                     "frac": ach / peak,
+                     "traffic": measured_traffic(dom + ("64" if prec == FP64 else "32")),
                      "flop_per_unit": {"k_conv": FLOP_CONV, "k_f4conv": FLOP_F4},
                      "units_per_launch": {"k_conv": w["conv_evals"], "k_f4conv": w["f4_evals"]},
                      "ms": {"conv_kernel": w["ms_conv"], "f4_kernel": w["ms_f4"], "prep": tm["prep"],

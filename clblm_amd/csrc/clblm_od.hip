@@ -585,7 +585,7 @@ __device__ int shrinq_walk(const Job &J, const F4Prep &f, int64_t k0, int64_t ke
     bool first = true, have = false;
     double pv = 0., ps = 0., pal = 0., pad = 0.;
     int8_t pco2 = 0;
-    auto flush = [&](const ShrinkSums &g, int8_t co2) {
+    auto flush = [&](const ShrinkSums g, int8_t co2) {
         if (EMIT) {
             int64_t q = J.f4_off + out0 + emitted;
             sh.vnu[q] = g.v / g.s; sh.sp[q] = g.s; sh.alfa[q] = g.al / g.s; sh.epp[q] = g.ad / g.s;
@@ -667,7 +667,7 @@ __device__ int shrinq_walk(const Job &J, const F4Prep &f, int64_t k0, int64_t ke
 constexpr int LCAP = 64;  // longest shrink chunk, in records
 
 // G(r, L): lines a SHRINQ call emits for a chunk of L records starting at record r.
-__global__ void k_chunkG(const Job *jobs, F4Prep f, Shrunk sh) {
+__global__ void __launch_bounds__(64) k_chunkG(const Job *jobs, F4Prep f, Shrunk sh) {
     const Job &J = jobs[blockIdx.y];
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= J.nrec) return;
@@ -685,7 +685,7 @@ __global__ void k_chunkG(const Job *jobs, F4Prep f, Shrunk sh) {
 // the buffer holds LIMIN lines, the new lines are shrunk, and the buffer is written
 // as one TAPE9 block once it holds LIMIN shrunk lines.  One thread per layer walks
 // the records with the precomputed chunk counts.
-__global__ void k_sim(const Job *jobs, int nlay, F4Prep f) {
+__global__ void __launch_bounds__(64) k_sim(const Job *jobs, int nlay, F4Prep f) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlay) return;
     const Job &J = jobs[l];
@@ -729,7 +729,7 @@ __global__ void k_sim(const Job *jobs, int nlay, F4Prep f) {
     f.nblk9[l] = nb;
 }
 
-__global__ void k_emit(const Job *jobs, F4Prep f, Shrunk sh, int64_t rec_tot) {
+__global__ void __launch_bounds__(64) k_emit(const Job *jobs, F4Prep f, Shrunk sh, int64_t rec_tot) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= rec_tot) return;
     int4 c = f.chunks[q];
@@ -1018,6 +1018,7 @@ __device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dpt
 // for F4K points per lane; records that cover the whole tile skip the per-point
 // range test.
 constexpr int F4K = 4, F4PTS = 64 * F4K;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int RAMP_CAP = 256;  // ramp pieces per panel held in LDS
 
 __device__ __forceinline__ unsigned long long f4_count(int jmin, int jmax, int j0, int j1) {
@@ -1045,18 +1046,34 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
         f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
         // records are staged in LDS 64 at a time (one coalesced load per lane, the
         // next batch in flight while this one is summed) and read back as broadcasts
-        const F4Rec64 *r = recs + B.rec - B.lo;
-        __shared__ F4Rec64 srec[64];
-        F4Rec64 pre;
-        if (lane < c1 - c0) pre = r[c0 + lane];
+        // (plain 16-byte vectors, so that no copy of a record is made in private memory)
+        constexpr int W = (int)(sizeof(F4Rec64) / 16);
+        const u32x4 *r = reinterpret_cast<const u32x4 *>(recs + B.rec - B.lo);
+        __shared__ u32x4 srec[64 * W];
+        // the prefetch is unconditional (index clamped to the last record) so that it
+        // stays in registers
+        u32x4 pre[W];
+        const int64_t last = c1 - 1;
+        if (c1 > c0) {
+            const int64_t k = c0 + lane < last ? c0 + lane : last;
+#pragma unroll
+            for (int w = 0; w < W; w++) pre[w] = r[k * W + w];
+        }
         for (int64_t base = c0; base < c1; base += 64) {
             const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
             __syncthreads();
-            if (lane < nb) srec[lane] = pre;
+            if (lane < nb) {
+#pragma unroll
+                for (int w = 0; w < W; w++) srec[lane * W + w] = pre[w];
+            }
             __syncthreads();
-            if (lane < c1 - base - 64) pre = r[base + 64 + lane];
+            {
+                const int64_t k = base + 64 + lane < last ? base + 64 + lane : last;
+#pragma unroll
+                for (int w = 0; w < W; w++) pre[w] = r[k * W + w];
+            }
             for (int q = 0; q < nb; q++) {
-                const F4Rec64 l = srec[q];
+                const F4Rec64 &l = *reinterpret_cast<const F4Rec64 *>(&srec[q * W]);
                 if (l.jmin <= l.jmax) {
                     nev += f4_count(l.jmin, l.jmax, j0, j1);
                     if (l.jmin <= j0 && l.jmax >= j1) {
@@ -1109,18 +1126,34 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
         f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
         // records are staged in LDS 64 at a time (one coalesced load per lane, the
         // next batch in flight while this one is summed) and read back as broadcasts
-        const F4Rec32 *r = recs + B.rec - B.lo;
-        __shared__ F4Rec32 srec[64];
-        F4Rec32 pre;
-        if (lane < c1 - c0) pre = r[c0 + lane];
+        // (plain 16-byte vectors, so that no copy of a record is made in private memory)
+        constexpr int W = (int)(sizeof(F4Rec32) / 16);
+        const u32x4 *r = reinterpret_cast<const u32x4 *>(recs + B.rec - B.lo);
+        __shared__ u32x4 srec[64 * W];
+        // the prefetch is unconditional (index clamped to the last record) so that it
+        // stays in registers
+        u32x4 pre[W];
+        const int64_t last = c1 - 1;
+        if (c1 > c0) {
+            const int64_t k = c0 + lane < last ? c0 + lane : last;
+#pragma unroll
+            for (int w = 0; w < W; w++) pre[w] = r[k * W + w];
+        }
         for (int64_t base = c0; base < c1; base += 64) {
             const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
             __syncthreads();
-            if (lane < nb) srec[lane] = pre;
+            if (lane < nb) {
+#pragma unroll
+                for (int w = 0; w < W; w++) srec[lane * W + w] = pre[w];
+            }
             __syncthreads();
-            if (lane < c1 - base - 64) pre = r[base + 64 + lane];
+            {
+                const int64_t k = base + 64 + lane < last ? base + 64 + lane : last;
+#pragma unroll
+                for (int w = 0; w < W; w++) pre[w] = r[k * W + w];
+            }
             for (int q = 0; q < nb; q++) {
-                const F4Rec32 l = srec[q];
+                const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(&srec[q * W]);
                 if (l.jmin <= l.jmax) {
                     nev += f4_count(l.jmin, l.jmax, j0, j1);
                     const float fl = (float)(j0 + lane - l.jc);
