@@ -13,9 +13,10 @@ E = sum over layers and lines of the number of layer-grid points within
 +-25 cm-1 of the line centre (a closed form of the line list and grids).
 `value` is the FP32-compensated path; the plain FP64 GPU path is timed beside it
 ("fp64_gpu").  "jacobian" times the upwelling-radiance Jacobian pass of the same
-atmosphere (dR/dT for 60 layers and dR/dln q for 60 layers x 7 species over
-500-2500 cm-1 on the finest layer grid, in 100 cm-1 blocks with layer ODs
-recomputed per block).
+atmosphere (noScattJacob: radiance, dR/dT at 61 levels and dR/dln q at 61 levels x 7
+species over 500-2500 cm-1 on the finest layer grid, in 100 cm-1 blocks with the
+layer ODs and their T + 1 K, Wtot + 1 % and single-molecule variants recomputed per
+block).
 
 Prints one JSON line.
 """
@@ -106,14 +107,15 @@ def run_od(ctx, h, layers, pars, steps, warmup):
 
 
 def run_jacobian(ctx, h, layers, dvs, precision, block=100.0):
-    """Radiance + dR/dT_l + dR/dln q_l,m over V1..V2 in spectral blocks; returns
+    """Radiance + dR/dT_lev + dR/dln q_lev,m over V1..V2 in spectral blocks; returns
     (seconds, spectra, points per spectrum, blocks)."""
     import torch
     from clblm_amd import workloads as W
     from clblm_amd.jacobian import UpwellingJacobian
-    t_lev = W.level_temperatures(len(layers), 0.1)
-    jac = UpwellingJacobian(ctx, h, layers, t_lev, dvs, precision=precision,
-                            continuum=lambda l, a, b: W.continuum_absrb(l, a, b))
+    nlay = len(layers)
+    jac = UpwellingJacobian(ctx, h, layers, W.level_temperatures(nlay, 0.1), dvs,
+                            p_lev=W.level_pressures(nlay, 0.1), q_lev=W.level_densities(nlay, 0.1),
+                            precision=precision, continuum=lambda l, a, b: W.continuum_absrb(l, a, b))
     edges = np.arange(V1, V2 + 0.5 * block, block)
     jac.block(edges[0], edges[1])          # warm-up
     torch.cuda.synchronize()
@@ -125,7 +127,7 @@ def run_jacobian(ctx, h, layers, dvs, precision, block=100.0):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     ctx.sync()
-    spectra = len(layers) * (1 + len(jac.mols))
+    spectra = (nlay + 1) * (1 + len(jac.mols))    # dR/dT_lev and dR/dln q_lev of every level
     return t, spectra, npts, len(edges) - 1
 
 

@@ -13,7 +13,8 @@ FP64, FP32C = 0, 1
 
 EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
            "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
-           "clblm_rt_up", "clblm_fft_scan", "clblm_timing", "clblm_work"]
+           "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
+           "clblm_timing", "clblm_work"]
 
 
 class Layer(ctypes.Structure):
@@ -68,6 +69,11 @@ def load():
     lib.clblm_rt_up.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                 P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                 P, P, P, P, P, P]
+    lib.clblm_level2layer.argtypes = [ctypes.c_int, ctypes.c_int] + [P] * 8
+    lib.clblm_rt_jac.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double, P, P, P, P, P,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, P, P, P, P, P, P]
+    lib.clblm_level_jac.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P, P, P, ctypes.c_double,
+                                    ctypes.c_double, P, P, P, P, P, P, P]
     lib.clblm_fft_scan.argtypes = [P, P, ctypes.c_int64] + [ctypes.c_double] * 5 + [ctypes.c_int] + \
         [ctypes.c_double] * 2 + [ctypes.c_int, P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_double), P]

@@ -2218,6 +2218,19 @@ extern "C" int clblm_rt_up_impl(int nlay, int64_t npts, double v1, double dv, co
                                 const double *d_tlay, const double *d_tlev, double emis, double tskin, int lin,
                                 double sec_dn, double *d_rad, double *d_dr_dod, double *d_dr_dt,
                                 double *d_dr_dtskin, double *d_dr_demis, double *d_state, void *stream);
+extern "C" void clblm_level2layer_impl(int nlay, int nmol, const double *p_lev, const double *t_lev,
+                                       const double *t_lay, const double *q_lev, double *dt_dtup, double *dt_dtlo,
+                                       double *dw_dqup, double *dw_dqlo);
+extern "C" int clblm_rt_jac_impl(int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                                 const double *d_tlay, const double *d_tlev, const double *d_dt_dtup,
+                                 const double *d_dt_dtlo, double emis, double tskin, int lin, double sec_dn,
+                                 double *d_rad, double *d_dr_dtau, double *d_dr_dtlev, double *d_dr_dtskin,
+                                 double *d_dr_demis, double *d_state, void *stream);
+extern "C" int clblm_level_jac_impl(int nlay, int nmol, int64_t npts, const double *d_od_var,
+                                    const double *d_dr_dtau, const double *d_dr_dtlev, const double *d_wtot,
+                                    const double *d_tlay, double dt, double dw_frac, const double *d_dt_dtup,
+                                    const double *d_dt_dtlo, const double *d_dw_dqup, const double *d_dw_dqlo,
+                                    double *d_jac_t, double *d_jac_q, void *stream);
 extern "C" void *clblm_scan_state_new();
 extern "C" void clblm_scan_state_free(void *s);
 extern "C" const char *clblm_scan_error();
@@ -2480,6 +2493,70 @@ int clblm_work(clblm_ctx *c, int reset, int64_t *conv_evals, int64_t *f4_evals, 
         c->ms[0] = c->ms[1] = c->ms[2] = c->ms[3] = 0;
         c->launches = 0;
     }
+    return 0;
+}
+
+int clblm_level2layer(int nlay, int nmol, const double *p_lev, const double *t_lev, const double *t_lay,
+                      const double *q_lev, double *dt_dtup, double *dt_dtlo, double *dlnw_dlnq_up,
+                      double *dlnw_dlnq_lo) {
+    if (nlay <= 0 || nmol < 0 || !p_lev || !t_lev || !t_lay || !dt_dtup || !dt_dtlo)
+        return set_err(CLBLM_ERR_ARG, "null argument or no layers");
+    if (nmol > 0 && (!q_lev || !dlnw_dlnq_up || !dlnw_dlnq_lo))
+        return set_err(CLBLM_ERR_ARG, "molecular densities and outputs are needed when nmol > 0");
+    for (int k = 0; k <= nlay; k++)
+        if (!(p_lev[k] > 0.) || !(t_lev[k] > 0.)) return set_err(CLBLM_ERR_ARG, "level pressures and temperatures must be positive");
+    for (int k = 0; k < nlay; k++)
+        if ((p_lev[k + 1] / t_lev[k + 1]) / (p_lev[k] / t_lev[k]) == 1.0)
+            return set_err(CLBLM_ERR_ARG, "a layer has equal air densities at both levels");
+    clblm_level2layer_impl(nlay, nmol, p_lev, t_lev, t_lay, q_lev, dt_dtup, dt_dtlo, dlnw_dlnq_up, dlnw_dlnq_lo);
+    return 0;
+}
+
+int clblm_rt_jac(clblm_ctx *c, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                 const double *d_tlay, const double *d_tlev, const double *d_dt_dtup, const double *d_dt_dtlo,
+                 double surf_emis, double t_skin, int lin_in_tau, double sec_dn, double *d_rad, double *d_dr_dtau,
+                 double *d_dr_dtlev, double *d_dr_dtskin, double *d_dr_demis, void *stream) {
+    if (!c || nlay <= 0 || npts <= 0 || !d_od || !d_tlay || !d_tlev || !d_rad || !d_dr_dtau || !d_dr_dtlev)
+        return set_err(CLBLM_ERR_ARG, "null argument or empty grid");
+    if (lin_in_tau < 0 || lin_in_tau > 2) return set_err(CLBLM_ERR_ARG, "lin_in_tau must be 0, 1 or 2");
+    if (lin_in_tau != 2 && (!d_dt_dtup || !d_dt_dtlo))
+        return set_err(CLBLM_ERR_ARG, "lin_in_tau 0 and 1 need dT/dT_lev of the layer means");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    const size_t bytes = sizeof(double) * (size_t)(3 * nlay + 2) * (size_t)npts;
+    if (c->rt_state_cap < bytes) {
+        if (c->rt_state) (void)hipFree(c->rt_state);
+        c->rt_state = nullptr;
+        c->rt_state_cap = 0;
+        HIPCHK(hipMalloc(&c->rt_state, bytes));
+        c->rt_state_cap = bytes;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = clblm_rt_jac_impl(nlay, npts, v1, dv, d_od, d_tlay, d_tlev, d_dt_dtup, d_dt_dtlo, surf_emis, t_skin,
+                               lin_in_tau, sec_dn, d_rad, d_dr_dtau, d_dr_dtlev, d_dr_dtskin, d_dr_demis,
+                               c->rt_state, st);
+    if (rc) return set_err(rc, "Jacobian kernel launch failed");
+    HIPCHK(hipStreamSynchronize(st));   // the scratch states are reused by the next call
+    return 0;
+}
+
+int clblm_level_jac(clblm_ctx *c, int nlay, int nmol, int64_t npts, const double *d_od_var,
+                    const double *d_dr_dtau, const double *d_dr_dtlev, const double *d_wtot, const double *d_tlay,
+                    double dt, double dw_frac, const double *d_dt_dtup, const double *d_dt_dtlo,
+                    const double *d_dw_dqup, const double *d_dw_dqlo, double *d_jac_t, double *d_jac_q,
+                    void *stream) {
+    if (!c || nlay <= 0 || nmol < 0 || npts <= 0 || !d_od_var || !d_dr_dtau || !d_dr_dtlev || !d_wtot || !d_tlay ||
+        !d_dt_dtup || !d_dt_dtlo || !d_jac_t)
+        return set_err(CLBLM_ERR_ARG, "null argument or empty grid");
+    if (nmol > 0 && d_jac_q && (!d_dw_dqup || !d_dw_dqlo))
+        return set_err(CLBLM_ERR_ARG, "molecular Jacobians need dlnW/dlnq");
+    if (!(dt != 0.) || !(dw_frac != 0.)) return set_err(CLBLM_ERR_ARG, "dt and dw_frac must be nonzero");
+    if (nmol + 1 > 65535) return set_err(CLBLM_ERR_ARG, "too many molecules");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = clblm_level_jac_impl(nlay, nmol, npts, d_od_var, d_dr_dtau, d_dr_dtlev, d_wtot, d_tlay, dt, dw_frac,
+                                  d_dt_dtup, d_dt_dtlo, d_dw_dqup, d_dw_dqlo, d_jac_t, d_jac_q, st);
+    if (rc) return set_err(rc, "level Jacobian kernel launch failed");
     return 0;
 }
 

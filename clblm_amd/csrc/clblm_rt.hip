@@ -171,6 +171,205 @@ __global__ void k_rt_up(int nlay, int64_t npts, double v1, double dv, const doub
     if (dr_demis) dr_demis[i] = rb * bs - rhob;
 }
 
+// ---------------------------------------------------------------------------------
+// Jacobians (src/clblm_src/clblm_noScattJac.f90:346-716) per spectral point, for a
+// down-looking sensor above the top layer (obsLev = nlay + 1), thermal only: the view
+// OD of layer l is od[l*n+i] (amfView = 1) and the down-path OD is sec_dn times it
+// (amfDown = sec_dn, the air-mass-scaled path of pathOD_up_dn_sun :826-838).  The
+// reference's cumulative-transmittance form with its summation order, including the
+// rule that the reflected downwelling term is dropped when the total view
+// transmittance is 1e-6 or less (:480).  Layer 0 is at the surface; level k is the
+// bottom of layer k.  Per-layer states the loops pass to each other live in `sa`
+// (view transmittance to the top, nlay+1 levels), `sb` (two-path transmittance,
+// nlay+1) and `sbar` (layer-mean source derivative of the downwelling pass, nlay),
+// all [k][n]; dr_dtlev holds the downwelling level terms until the upwelling pass
+// folds them in.
+__global__ void __launch_bounds__(256) k_rt_jac(int L, int64_t n, double v1, double dv, const double *__restrict__ od,
+                                                const double *__restrict__ t_lay, const double *__restrict__ t_lev,
+                                                double emis, double tskin, int lin, double sec,
+                                                const double *__restrict__ dt_dtup, const double *__restrict__ dt_dtlo,
+                                                double *__restrict__ rad_o, double *__restrict__ dtau,
+                                                double *__restrict__ dtlev, double *__restrict__ dts,
+                                                double *__restrict__ de, double *__restrict__ sa,
+                                                double *__restrict__ sb, double *__restrict__ sbar) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double vn = v1 + (double)i * dv;
+    auto at = [n, i](int k) { return (int64_t)k * n + i; };
+    // 0- view-path transmittance from the top down to the surface
+    double cumA = 0.;
+    sa[at(L)] = 1.;
+    for (int l = L - 1; l >= 0; l--) {
+        cumA = cumA + od[at(l)];
+        sa[at(l)] = exp(-cumA);
+    }
+    const double totODUp = cumA, totTxUp = sa[at(0)];
+    double rad = 0.;
+    // 1- downwelling thermal radiance, reflected at the surface
+    const bool dn = totTxUp > 1.e-06;
+    if (dn) {
+        sb[at(0)] = totTxUp;
+        double cumB = 0.;
+        for (int l = 0; l < L; l++) {
+            cumB = cumB + od[at(l)] * sec;
+            sb[at(l + 1)] = exp(-(totODUp + cumB));
+        }
+        double bhi = 0., dbhi = 0., tdn_hi = 0.;   // level l+1 values carried down
+        if (lin != 0) planck_drv(vn, t_lev[L], &bhi, &dbhi);
+        double bhi_tx = sb[at(L)];
+        for (int l = L - 1; l >= 0; l--) {
+            const double blo_tx = sb[at(l)];
+            const double dTran = blo_tx - bhi_tx;
+            const double odd = od[at(l)] * sec;
+            double bbar, dbbar;
+            planck_drv(vn, t_lay[l], &bbar, &dbbar);
+            double Blt;
+            if (lin != 0) {
+                double blo, dblo, h, dh;
+                planck_drv(vn, t_lev[l], &blo, &dblo);
+                rlin(odd, &h, &dh);
+                if (lin == 2) {
+                    Blt = blo + (bhi - blo) * h;
+                    dtau[at(l)] = (bhi_tx * Blt - rad + dTran * dh * (bhi - blo)) * sec;
+                    tdn_hi = tdn_hi + dTran * dbhi * h;
+                    dtlev[at(l + 1)] = tdn_hi;                 // level l+1 complete
+                    tdn_hi = dTran * dblo * (1. - h);
+                } else {
+                    Blt = 2.0 * bbar * h + blo * (1.0 - 2.0 * h);
+                    dtau[at(l)] = (bhi_tx * Blt - rad + 2.0 * dTran * dh * (bbar - blo)) * sec;
+                    dtlev[at(l)] = dTran * dblo * (1.0 - 2.0 * h);
+                    sbar[at(l)] = dTran * 2.0 * dbbar * h;
+                }
+                bhi = blo;
+                dbhi = dblo;
+            } else {
+                Blt = bbar;
+                dtau[at(l)] = (bhi_tx * bbar - rad) * sec;
+                sbar[at(l)] = dTran * dbbar;
+            }
+            rad = rad + dTran * Blt;
+            bhi_tx = blo_tx;
+        }
+        if (lin == 2) dtlev[at(0)] = tdn_hi;
+    }
+    // 2-3- surface reflectance and emission
+    const double rsfc = 1. - emis;
+    double bs, dbs;
+    planck_drv(vn, tskin, &bs, &dbs);
+    if (de) de[i] = totTxUp * bs - rad;
+    if (dts) dts[i] = emis * totTxUp * dbs;
+    rad = rad * rsfc + emis * totTxUp * bs;
+    // 5- upwelling thermal radiance; level derivatives are completed on the way up
+    double blo = 0., dblo = 0.;
+    if (lin != 0) planck_drv(vn, t_lev[0], &blo, &dblo);
+    double tup_lo = 0.;          // dRad_dT_up of the current layer's bottom level
+    double tbar_prev = 0.;       // layer-mean derivative of the layer below
+    double alo = sa[at(0)];
+    for (int l = 0; l < L; l++) {
+        const double ahi = sa[at(l + 1)];
+        const double dTran = ahi - alo;
+        const double oda = od[at(l)];
+        double bbar, dbbar;
+        planck_drv(vn, t_lay[l], &bbar, &dbbar);
+        double t_l = dn ? dtau[at(l)] * rsfc : 0. * rsfc;
+        double tb_l = (dn && lin != 2) ? sbar[at(l)] * rsfc : 0.;
+        double Blt, tup_hi = 0., bhi = 0., dbhi = 0.;
+        if (lin != 0) {
+            double h, dh;
+            planck_drv(vn, t_lev[l + 1], &bhi, &dbhi);
+            rlin(oda, &h, &dh);
+            if (lin == 2) {
+                Blt = bhi * (1 - h) + blo * h;
+                t_l = t_l + (alo * Blt - rad + dTran * dh * (blo - bhi));
+                tup_hi = dTran * dbhi * (1 - h);
+                tup_lo = tup_lo + dTran * dblo * h;
+            } else {
+                Blt = 2.0 * bbar * h + bhi * (1.0 - 2.0 * h);
+                t_l = t_l + (alo * Blt - rad + dTran * dh * 2.0 * (bbar - bhi));
+                tup_hi = dTran * dbhi * (1.0 - 2.0 * h);
+                tb_l = tb_l + dTran * 2.0 * dbbar * h;
+            }
+        } else {
+            Blt = bbar;
+            t_l = t_l + (alo * bbar - rad);
+            tb_l = tb_l + dTran * dbbar;
+        }
+        rad = rad + dTran * Blt;
+        dtau[at(l)] = t_l;
+        // level l: emission terms, then the layer-mean terms of layers l and l-1
+        double lev = 0.;
+        if (lin != 0) {
+            const bool has_dn = dn && (lin == 2 || l < L);
+            lev = tup_lo + (has_dn ? dtlev[at(l)] : 0.) * rsfc;
+        }
+        if (lin != 2) {
+            lev = lev + tb_l * dt_dtlo[l];
+            if (l > 0) lev = lev + tbar_prev * dt_dtup[l - 1];
+        }
+        dtlev[at(l)] = lev;
+        tbar_prev = tb_l;
+        tup_lo = tup_hi;
+        blo = bhi;
+        dblo = dbhi;
+        alo = ahi;
+    }
+    // top level L
+    double lev = 0.;
+    if (lin != 0) lev = tup_lo + ((dn && lin == 2) ? dtlev[at(L)] : 0.) * rsfc;
+    if (lin != 2) lev = lev + tbar_prev * dt_dtup[L - 1];
+    dtlev[at(L)] = lev;
+    rad_o[i] = rad;
+}
+
+// drv_LevelTemp and drv_LevelConc (src/clblm_src/clblm_noScattJac.f90:899-1203): one
+// thread per (point, quantity); quantity 0 is temperature, q = 1..nmol the molecules.
+// od_var holds 3 + nmol blocks of [L][n]: the layer OD, the OD at Tave + dt, the OD at
+// Wtot + dw_frac*Wtot, then each molecule alone.
+__global__ void __launch_bounds__(256) k_level_jac(int L, int nmol, int64_t n, const double *__restrict__ od_var,
+                                                   const double *__restrict__ dtau,
+                                                   const double *__restrict__ dtlev,
+                                                   const double *__restrict__ wtot, const double *__restrict__ tave,
+                                                   double dt, double dw_frac, const double *__restrict__ dt_dtup,
+                                                   const double *__restrict__ dt_dtlo,
+                                                   const double *__restrict__ dw_dqup,
+                                                   const double *__restrict__ dw_dqlo, double *__restrict__ jac_t,
+                                                   double *__restrict__ jac_q) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (i >= n) return;
+    const int64_t LN = (int64_t)L * n;
+    double lo = 0., up = 0.;
+    if (q == 0) {
+        const double *od = od_var, *odp = od_var + LN, *odw = od_var + 2 * LN;
+        for (int l = 0; l < L; l++) {
+            const int64_t j = (int64_t)l * n + i;
+            const double base = od[j];
+            double dTau_dT = (odp[j] - base) / dt;
+            const double dTau_dW = (odw[j] - base) / (dw_frac * wtot[l]);
+            dTau_dT = dTau_dT - dTau_dW * (wtot[l] / tave[l]);
+            const double dRaddT = dtau[j] * dTau_dT;
+            up = dRaddT * dt_dtup[l];
+            lo = lo + dRaddT * dt_dtlo[l] + dtlev[j];
+            jac_t[j] = lo;
+            lo = up;
+        }
+        jac_t[LN + i] = lo + dtlev[LN + i];
+    } else {
+        const int m = q - 1;
+        const double *odm = od_var + (3 + (int64_t)m) * LN;
+        double *jq = jac_q + (int64_t)m * (L + 1) * n;
+        for (int l = 0; l < L; l++) {
+            const int64_t j = (int64_t)l * n + i;
+            const double dRad_dW = dtau[j] * odm[j];
+            up = dRad_dW * dw_dqup[(int64_t)m * L + l];
+            lo = lo + dRad_dW * dw_dqlo[(int64_t)m * L + l];
+            jq[j] = lo;
+            lo = up;
+        }
+        jq[LN + i] = lo;
+    }
+}
+
 thread_local std::string g_rt_err;
 }  // namespace
 
@@ -185,5 +384,62 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_rt_up_impl(int nlay, 
     hipLaunchKernelGGL(k_rt_up, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, st, nlay, npts, v1, dv, d_od,
                        d_tlay, d_tlev, emis, tskin, lin, sec_dn, d_rad, d_dr_dod, d_dr_dt, d_dr_dtskin, d_dr_demis,
                        d_state);
+    return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}
+
+// level2layer (src/clblm_src/clblm_noScattJac.f90:1208-1279) on the host: layer-mean
+// temperature and ln(layer amount) derivatives with respect to the bounding levels,
+// for a profile with exponential density and linear temperature between levels.
+// q_lev[m*(nlay+1)+k] is the number density of molecule m at level k.
+extern "C" __attribute__((visibility("hidden"))) void clblm_level2layer_impl(
+    int nlay, int nmol, const double *p_lev, const double *t_lev, const double *t_lay, const double *q_lev,
+    double *dt_dtup, double *dt_dtlo, double *dw_dqup, double *dw_dqlo) {
+    const double BOLTZ_HPA = 1.3806503E-19;   // hPa cm3 / K (clblm_noScattJac.f90:1240)
+    for (int l = 0; l < nlay; l++) {
+        const double pup = p_lev[l + 1], plo = p_lev[l], tup = t_lev[l + 1], tlo = t_lev[l], tbar = t_lay[l];
+        const double rhoU = pup / (tup * BOLTZ_HPA), rhoL = plo / (tlo * BOLTZ_HPA);
+        const double alpha = rhoU / rhoL;
+        const double alphaT = -(tup - tlo) / std::log(alpha);
+        dt_dtlo[l] = ((tbar - alphaT) / tlo) * (rhoL / (rhoL - rhoU)) + (1.0 - alphaT / tlo) / std::log(alpha);
+        dt_dtup[l] = ((tbar - alphaT) / tup) * (-rhoU / (rhoL - rhoU)) - (1.0 - alphaT / tup) / std::log(alpha);
+        for (int m = 0; m < nmol; m++) {
+            const double dup = q_lev[(int64_t)m * (nlay + 1) + l + 1], dlo = q_lev[(int64_t)m * (nlay + 1) + l];
+            double &wu = dw_dqup[(int64_t)m * nlay + l], &wl = dw_dqlo[(int64_t)m * nlay + l];
+            if (dlo != 0.0) {
+                const double ratU = dup / rhoU, ratL = dlo / rhoL;
+                wl = (ratL / (ratL - alpha * ratU)) + 1.0 / std::log(alpha * ratU / ratL);
+                wu = ((-alpha * ratU) / (ratL - alpha * ratU)) - 1.0 / std::log(alpha * ratU / ratL);
+            } else {
+                wl = 0.0;
+                wu = 0.0;
+            }
+        }
+    }
+}
+
+extern "C" __attribute__((visibility("hidden"))) int clblm_rt_jac_impl(
+    int nlay, int64_t npts, double v1, double dv, const double *d_od, const double *d_tlay, const double *d_tlev,
+    const double *d_dt_dtup, const double *d_dt_dtlo, double emis, double tskin, int lin, double sec_dn,
+    double *d_rad, double *d_dr_dtau, double *d_dr_dtlev, double *d_dr_dtskin, double *d_dr_demis, double *d_state,
+    void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = npts;
+    double *sa = d_state, *sb = d_state + (int64_t)(nlay + 1) * n, *sbar = d_state + 2 * (int64_t)(nlay + 1) * n;
+    hipLaunchKernelGGL(k_rt_jac, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nlay, n, v1, dv, d_od, d_tlay,
+                       d_tlev, emis, tskin, lin, sec_dn, d_dt_dtup, d_dt_dtlo, d_rad, d_dr_dtau, d_dr_dtlev,
+                       d_dr_dtskin, d_dr_demis, sa, sb, sbar);
+    return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}
+
+extern "C" __attribute__((visibility("hidden"))) int clblm_level_jac_impl(
+    int nlay, int nmol, int64_t npts, const double *d_od_var, const double *d_dr_dtau, const double *d_dr_dtlev,
+    const double *d_wtot, const double *d_tlay, double dt, double dw_frac, const double *d_dt_dtup,
+    const double *d_dt_dtlo, const double *d_dw_dqup, const double *d_dw_dqlo, double *d_jac_t, double *d_jac_q,
+    void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const int nq = 1 + (d_jac_q ? nmol : 0);   // quantity 0 (temperature) always, then the molecules
+    hipLaunchKernelGGL(k_level_jac, dim3((unsigned)((npts + 255) / 256), (unsigned)nq), dim3(256), 0, st, nlay, nmol,
+                       npts, d_od_var, d_dr_dtau, d_dr_dtlev, d_wtot, d_tlay, dt, dw_frac, d_dt_dtup, d_dt_dtlo,
+                       d_dw_dqup, d_dw_dqlo, d_jac_t, d_jac_q);
     return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
 }

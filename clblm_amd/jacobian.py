@@ -1,69 +1,104 @@
-"""Upwelling radiance and its Jacobians through the HIP library.
+"""noScattJacob (src/clblm_src/clblm_noScattJac.f90:20-336) through the HIP library,
+for a nadir, down-looking, thermal sensor above the top layer.
 
-Mirrors noScattJacob (src/clblm_src/clblm_noScattJac.f90:20-336) for a nadir,
-down-looking, thermal case: layer ODs on one uniform grid (DVout, the RT_Jac
-grid rule of clblm_setDV.f90:203-208), the merge-up radiance, and per spectral
-point the analytic derivative of the radiance with respect to each layer's OD
-and source temperature.  Layer Jacobians are assembled as the reference does:
+Per spectral block [v1, v2] on the uniform grid DVout (the RT_Jac grid rule,
+clblm_setDV.f90:203-208):
 
-    dR/dT_l      = dR/dtau_l * dtau_l/dT_l + dR/dB_l * dB_l/dT_l
-    dR/dln q_l,m = dR/dtau_l * tau_l,m      (tau_l,m: OD of molecule m alone,
-                                             clblm_noScattJac.f90:1159-1171)
+1. one clblm_od_batch of (3 + M) x L layer-OD jobs, in this order: the layer ODs
+   (pathOD_up_dn_sun, :796-838), the ODs at Tave + dt (drv_LevelTemp, :1005-1009,
+   dt = 1 K), the ODs at Wtot + 0.01 Wtot (:1018-1024), and the OD of each Jacobian
+   molecule alone (drv_LevelConc, :1156-1166);
+2. clblm_rt_jac: the radiance and its per-point derivatives (Jacobians, :346-716);
+3. clblm_level_jac: the level Jacobians dR/dT_lev and dR/dln q_lev (:899-1203) with
+   level2layer's factors (:1208-1279, clblm_level2layer, computed once).
 
-dtau_l/dT_l is the central difference of two full layer-OD passes at T_l +- dT
-(the reference uses a one-sided +1 K difference, clblm_noScattJac.f90:1005-1014).
-Layer ODs are recomputed per spectral block rather than stored for the whole range.
+The down path is the view path scaled by sec_dn (the air-mass-scaled form of
+pathOD_up_dn_sun, :826-838).  The OD variants are recomputed per block rather than
+stored for the whole range, as the reference does.
 """
 from __future__ import annotations
 
 import ctypes
-import math
+import dataclasses
 
 import numpy as np
 
-from . import workloads as W
-from ._lib import FP32C, FP64, check
+from ._lib import FP32C, check
 from .odlay import Context
 
 
 class UpwellingJacobian:
-    def __init__(self, ctx: Context, handle: int, layers, t_lev, dvs, *, emis=0.98, tskin=295.0, lin_in_tau=1,
-                 sec_dn=1.0, precision=FP32C, dt=0.1, continuum=None, dvout=None):
+    """Radiance and level Jacobians of one scene.
+
+    layers: L layers from the surface up; t_lev, p_lev: the L + 1 level temperatures
+    and pressures; dvs: each layer's DV (the OD grids before the re-grid to DVout);
+    q_lev: {molecule: L + 1 level number densities} of the Jacobian molecules (the
+    reference's path%Q); mols: the Jacobian molecules (default: those of q_lev)."""
+
+    def __init__(self, ctx: Context, handle: int, layers, t_lev, dvs, *, p_lev, q_lev=None, mols=None, emis=0.98,
+                 tskin=295.0, lin_in_tau=1, sec_dn=1.0, precision=FP32C, dt=1.0, dw_frac=0.01, continuum=None,
+                 dvout=None):
+        import torch
         self.ctx, self.h, self.layers = ctx, handle, list(layers)
-        self.t_lev = np.asarray(t_lev, np.float64)
+        L = len(self.layers)
+        self.t_lev = np.ascontiguousarray(t_lev, np.float64)
+        self.p_lev = np.ascontiguousarray(p_lev, np.float64)
+        if self.t_lev.size != L + 1 or self.p_lev.size != L + 1:
+            raise ValueError("t_lev and p_lev need nlay + 1 levels")
         self.dvs = list(dvs)
-        self.emis, self.tskin, self.lin, self.sec_dn = emis, tskin, lin_in_tau, sec_dn
-        self.prec, self.dt = precision, dt
+        self.emis, self.tskin, self.lin, self.sec_dn = emis, tskin, int(lin_in_tau), sec_dn
+        self.prec, self.dt, self.dw_frac = precision, float(dt), float(dw_frac)
         self.continuum = continuum   # callable(layer, v1, v2) -> (absrb, v1abs, npt) or None
         self.dvout = min(self.dvs) if dvout is None else dvout
-        self.mols = sorted({m for l in self.layers for m in l.w})
+        q_lev = q_lev or {}
+        self.mols = sorted(q_lev) if mols is None else list(mols)
+        missing = [m for m in self.mols if m not in q_lev]
+        if missing:
+            raise ValueError("no level densities for molecules %s" % missing)
+        M = len(self.mols)
+        t_lay = np.array([l.tave for l in self.layers], np.float64)
+        q = np.ascontiguousarray(np.array([q_lev[m] for m in self.mols], np.float64).reshape(M, L + 1))
+        up, lo = np.zeros(L), np.zeros(L)
+        wu, wl = np.zeros((M, L)), np.zeros((M, L))
+        check(ctx.lib.clblm_level2layer(L, M, self.p_lev.ctypes.data, self.t_lev.ctypes.data, t_lay.ctypes.data,
+                                        q.ctypes.data if M else None, up.ctypes.data, lo.ctypes.data,
+                                        wu.ctypes.data if M else None, wl.ctypes.data if M else None))
+        self.dt_dtup, self.dt_dtlo, self.dw_dqup, self.dw_dqlo = up, lo, wu, wl
+        d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+        self._d = dict(tlay=d(t_lay), tlev=d(self.t_lev), wtot=d([l.wtot for l in self.layers]), up=d(up), lo=d(lo),
+                       wu=d(wu.reshape(-1)) if M else None, wl=d(wl.reshape(-1)) if M else None)
+
+    def _variants(self):
+        """The (3 + M) x L layer-OD jobs of one block, variant-major."""
+        out = []
+        for lay in self.layers:
+            out.append((lay, True))
+        for lay in self.layers:
+            out.append((dataclasses.replace(lay, tave=lay.tave + self.dt), True))
+        for lay in self.layers:
+            out.append((dataclasses.replace(lay, wtot=lay.wtot + self.dw_frac * lay.wtot), True))
+        for m in self.mols:
+            for lay in self.layers:
+                w = {mm: (v if mm == m else 0.0) for mm, v in lay.w.items()}
+                out.append((dataclasses.replace(lay, w=w), m == 1))
+        return out
 
     def _jobs(self, v1, v2):
-        """Layer-OD jobs of one spectral block: base, T+dT, T-dT, then each molecule alone."""
-        import copy
         jobs, pars = [], []
-        for k, lay in enumerate(self.layers):
-            variants = [(lay, True)]
-            for sgn in (+1, -1):
-                l2 = copy.copy(lay)
-                l2.tave = lay.tave + sgn * self.dt
-                variants.append((l2, True))
-            for m in self.mols:
-                l2 = copy.copy(lay)
-                l2.w = {mm: (v if mm == m else 0.0) for mm, v in lay.w.items()}
-                variants.append((l2, m == 1))
-            for lv, cont in variants:
-                c = None
-                if cont and self.continuum is not None:
-                    c = self.continuum(lv, v1, v2)[0]
-                jobs.append(lv)
-                pars.append(Context.params(v1, v2, self.dvs[k], dvout=self.dvout, precision=self.prec,
-                                           continuum=c))
+        L = len(self.layers)
+        for k, (lv, cont) in enumerate(self._variants()):
+            c = None
+            if cont and self.continuum is not None:
+                c = self.continuum(lv, v1, v2)[0]
+            jobs.append(lv)
+            pars.append(Context.params(v1, v2, self.dvs[k % L], dvout=self.dvout, precision=self.prec, continuum=c))
         return jobs, pars
 
     def block(self, v1, v2):
-        """Radiance and Jacobians on [v1, v2] (uniform grid DVout).  Returns torch
-        tensors: rad [N], jac_t [L, N], jac_lnq [L, M, N], plus tskin/emis derivatives."""
+        """Radiance and Jacobians on [v1, v2] (uniform grid DVout).  Returns torch tensors:
+        rad [N]; jac_t [L+1, N] = dR/dT_lev; jac_lnq [M, L+1, N] = dR/dln q_lev;
+        dr_dtskin, dr_demis [N]; and the per-point derivatives dr_dtau [L, N],
+        dr_dtlev [L+1, N] and the layer ODs od [L, N]."""
         import torch
         jobs, pars = self._jobs(v1, v2)
         self.ctx.jrad = 1
@@ -71,29 +106,24 @@ class UpwellingJacobian:
         n = int(off[1] - off[0])
         assert np.all(np.diff(off) == n), "all jobs of a block share the DVout grid"
         L, M = len(self.layers), len(self.mols)
-        nv = 3 + M
-        od = od.view(L, nv, n)
-        base = od[:, 0, :].contiguous()
-        dtau_dt = (od[:, 1, :] - od[:, 2, :]) / (2.0 * self.dt)
-        tau_m = od[:, 3:, :]
-        dev = base.device
-        tlay = torch.tensor([l.tave for l in self.layers], dtype=torch.float64, device=dev)
-        tlev = torch.tensor(self.t_lev, dtype=torch.float64, device=dev)
-        rad = torch.empty(n, dtype=torch.float64, device=dev)
-        dr_dod = torch.empty(L, n, dtype=torch.float64, device=dev)
-        dr_dt = torch.empty(L, n, dtype=torch.float64, device=dev)
-        dr_dts = torch.empty(n, dtype=torch.float64, device=dev)
-        dr_de = torch.empty(n, dtype=torch.float64, device=dev)
-        ptr = lambda t: ctypes.c_void_p(t.data_ptr())
-        check(self.ctx.lib.clblm_rt_up(self.ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(self.dvout),
-                                       ptr(base), ptr(tlay), ptr(tlev), ctypes.c_double(self.emis),
-                                       ctypes.c_double(self.tskin), self.lin, ctypes.c_double(self.sec_dn),
-                                       ptr(rad), ptr(dr_dod), ptr(dr_dt), ptr(dr_dts), ptr(dr_de),
-                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
-        jac_t = dr_dod * dtau_dt + dr_dt
-        jac_q = dr_dod[:, None, :] * tau_m
-        return dict(rad=rad, jac_t=jac_t, jac_lnq=jac_q, dr_dtskin=dr_dts, dr_demis=dr_de, od=base,
-                    dr_dod=dr_dod, n=n)
+        od = od.view(3 + M, L, n)
+        dev = od.device
+        e = lambda *shape: torch.empty(*shape, dtype=torch.float64, device=dev)
+        rad, dr_dtau, dr_dtlev, dts, de = e(n), e(L, n), e(L + 1, n), e(n), e(n)
+        jac_t, jac_q = e(L + 1, n), (e(M, L + 1, n) if M else None)
+        p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        D = self._d
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        lib = self.ctx.lib
+        check(lib.clblm_rt_jac(self.ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(self.dvout), p(od[0]),
+                               p(D["tlay"]), p(D["tlev"]), p(D["up"]), p(D["lo"]), ctypes.c_double(self.emis),
+                               ctypes.c_double(self.tskin), self.lin, ctypes.c_double(self.sec_dn), p(rad),
+                               p(dr_dtau), p(dr_dtlev), p(dts), p(de), st))
+        check(lib.clblm_level_jac(self.ctx.ptr, L, M, n, p(od), p(dr_dtau), p(dr_dtlev), p(D["wtot"]), p(D["tlay"]),
+                                  ctypes.c_double(self.dt), ctypes.c_double(self.dw_frac), p(D["up"]), p(D["lo"]),
+                                  p(D["wu"]), p(D["wl"]), p(jac_t), p(jac_q), st))
+        return dict(rad=rad, jac_t=jac_t, jac_lnq=jac_q, dr_dtskin=dts, dr_demis=de, dr_dtau=dr_dtau,
+                    dr_dtlev=dr_dtlev, od=od[0], od_var=od, n=n)
 
 
 def upwelling_radiance(ctx, od, t_lay, t_lev, v1, dv, emis=0.98, tskin=295.0, lin_in_tau=1, sec_dn=1.0):

@@ -152,6 +152,26 @@ def level_temperatures(n_layers=60, p_top=0.1) -> np.ndarray:
     return np.asarray(_levels(n_layers, p_top)[2], dtype=np.float64)
 
 
+def level_pressures(n_layers=60, p_top=0.1) -> np.ndarray:
+    """Level pressures (hPa, n_layers + 1, surface first) of ``atmosphere(n_layers, p_top)``."""
+    return np.asarray(_levels(n_layers, p_top)[0], dtype=np.float64)
+
+
+BOLTZ_HPA = 1.3806503e-19   # hPa cm3 / K, the constant of clblm_noScattJac.f90:1240
+
+
+def level_densities(n_layers=60, p_top=0.1, mols=(1, 2, 3, 4, 5, 6, 7), h2o_scale=1.0) -> dict:
+    """Number densities (molec cm-3) of each molecule at the levels of
+    ``atmosphere(n_layers, p_top)``: the recorded mixing ratio at the level pressure
+    times p / (k T)."""
+    levels, _, t_lev = _levels(n_layers, p_top)
+    out = {}
+    for m in mols:
+        q = np.array([vmr_profile(m, p) * p / (BOLTZ_HPA * t) for p, t in zip(levels, t_lev)])
+        out[m] = q * (h2o_scale if m == 1 else 1.0)
+    return out
+
+
 def atmosphere(n_layers=60, p_top=0.1, mols=(1, 2, 3, 4, 5, 6, 7), t_noise=None,
                h2o_scale=1.0) -> List[Layer]:
     """Layers between log-spaced pressure levels from 1013 hPa to ``p_top``.

@@ -115,12 +115,54 @@ int clblm_sync(clblm_ctx *ctx);
  * device arrays: d_tlay[nlay], d_tlev[nlay+1].  All outputs are device arrays.
  * d_rad[npts]; when d_dr_dod is non-NULL the reverse sweep also writes dR/dOD_l and
  * dR/dT_l through the layer source (nlay*npts each), and, if non-NULL,
- * dR/dTskin and dR/deps (npts each).  This is the per-point analytic derivative
- * that noScattJacob's Jacobians routine forms (clblm_noScattJac.f90:346-716). */
+ * dR/dTskin and dR/deps (npts each): the adjoint of this recursion, for callers that
+ * differentiate the radiance-only path.  Synchronous. */
 int clblm_rt_up(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, const double *d_od,
                 const double *d_tlay, const double *d_tlev, double surf_emis, double t_skin, int lin_in_tau,
                 double sec_dn, double *d_rad, double *d_dr_dod, double *d_dr_dtlay, double *d_dr_dtskin,
                 double *d_dr_demis, void *stream);
+
+/* ---- level Jacobians (noScattJacob, clblm_noScattJac.f90:20-336) ---------------
+ * level2layer (clblm_noScattJac.f90:1208-1279), host arrays, no device work:
+ * derivatives of each layer's mean temperature with respect to its upper and lower
+ * level temperatures, and of ln(layer amount) with respect to ln(level number
+ * density) of each molecule.  p_lev, t_lev [nlay+1] (level 0 at the surface, hPa, K),
+ * t_lay [nlay], q_lev[m*(nlay+1) + k] the density of molecule m at level k (the
+ * reference's path%Q); outputs dt_dtup, dt_dtlo [nlay], dlnw_dlnq_up/lo[m*nlay + l]. */
+int clblm_level2layer(int nlay, int nmol, const double *p_lev, const double *t_lev, const double *t_lay,
+                      const double *q_lev, double *dt_dtup, double *dt_dtlo, double *dlnw_dlnq_up,
+                      double *dlnw_dlnq_lo);
+
+/* Jacobians (clblm_noScattJac.f90:346-716) at every point of v1 + i*dv: radiance and
+ * its analytic derivatives for a down-looking sensor above the top layer, thermal
+ * only.  d_od[l*npts + i] is the layer OD along the view path (nadir: the vertical
+ * OD, amfView 1); the down path uses sec_dn times it (the air-mass-scaled path,
+ * amfDown = sec_dn).  Device inputs d_tlay[nlay], d_tlev[nlay+1], and for lin_in_tau
+ * 0 and 1 the level2layer factors d_dt_dtup, d_dt_dtlo[nlay].  Device outputs:
+ * d_rad[npts]; d_dr_dtau[l*npts + i] = dR/d(layer OD); d_dr_dtlev[k*npts + i]
+ * (nlay+1 levels) = dR/dT_lev through the level Planck terms and the layer-mean
+ * terms, the reference's dRad_dTlev; d_dr_dtskin, d_dr_demis (npts each, optional).
+ * As in the reference, the reflected downwelling term is dropped where the total
+ * view transmittance is 1e-6 or less.  Synchronous. */
+int clblm_rt_jac(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                 const double *d_tlay, const double *d_tlev, const double *d_dt_dtup, const double *d_dt_dtlo,
+                 double surf_emis, double t_skin, int lin_in_tau, double sec_dn, double *d_rad, double *d_dr_dtau,
+                 double *d_dr_dtlev, double *d_dr_dtskin, double *d_dr_demis, void *stream);
+
+/* drv_LevelTemp and drv_LevelConc (clblm_noScattJac.f90:899-1203): level Jacobians
+ * from clblm_rt_jac's derivatives and the layer-OD variants the reference computes.
+ * d_od_var holds 3 + nmol device blocks of nlay*npts: the layer OD, the OD at
+ * Tave + dt, the OD at Wtot + dw_frac*Wtot, then the OD of each molecule alone (all
+ * other amounts zero).  dtau/dT = (OD(T+dt) - OD)/dt - dtau/dWtot * Wtot/Tave (the
+ * reference: dt = 1 K, dw_frac = 0.01).  Device arrays d_wtot, d_tlay, d_dt_dtup,
+ * d_dt_dtlo [nlay], d_dw_dqup/lo[m*nlay + l].  Outputs (device):
+ * d_jac_t[k*npts + i] = dR/dT_lev (nlay+1 levels) and, if non-NULL,
+ * d_jac_q[(m*(nlay+1) + k)*npts + i] = dR/dln q_lev.  Enqueued on `stream`. */
+int clblm_level_jac(clblm_ctx *ctx, int nlay, int nmol, int64_t npts, const double *d_od_var,
+                    const double *d_dr_dtau, const double *d_dr_dtlev, const double *d_wtot, const double *d_tlay,
+                    double dt, double dw_frac, const double *d_dt_dtup, const double *d_dt_dtlo,
+                    const double *d_dw_dqup, const double *d_dw_dqlo, double *d_jac_t, double *d_jac_q,
+                    void *stream);
 
 /* ---- instrument scan ------------------------------------------------------------
  * Drop-in for clblm_FFTSCN_array (clblm_FFT.f90:308-349) with no boxcar pre-scan:

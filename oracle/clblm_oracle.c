@@ -1275,6 +1275,217 @@ void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, c
     }
 }
 
+/* ==== Jacobians ================================================================
+ * level2layer (clblm_noScattJac.f90:1208-1279): derivatives of the layer mean
+ * temperature and of ln(layer amount) with respect to the bounding levels' T and
+ * ln(number density).  qlev[m*(nlay+1)+k] is the level density of molecule m. */
+void orc_level2layer(int nlay, int nmol, const double *plev, const double *tlev, const double *tave,
+                     const double *qlev, double *dt_dtup, double *dt_dtlo, double *dw_dqup, double *dw_dqlo) {
+    const double BOLTZ_HPA = 1.3806503E-19;   /* hPa cm3 / K, clblm_noScattJac.f90:1240 */
+    for (int l = 0; l < nlay; l++) {
+        double pup = plev[l + 1], plo = plev[l], tup = tlev[l + 1], tlo = tlev[l], tbar = tave[l];
+        double rhoU = pup / (tup * BOLTZ_HPA);
+        double rhoL = plo / (tlo * BOLTZ_HPA);
+        double alpha = rhoU / rhoL;
+        double alphaT = -(tup - tlo) / log(alpha);
+        dt_dtlo[l] = ((tbar - alphaT) / tlo) * (rhoL / (rhoL - rhoU)) + (1.0 - alphaT / tlo) / log(alpha);
+        dt_dtup[l] = ((tbar - alphaT) / tup) * (-rhoU / (rhoL - rhoU)) - (1.0 - alphaT / tup) / log(alpha);
+        for (int k = 0; k < nmol; k++) {
+            double denUp = qlev[(int64_t)k * (nlay + 1) + l + 1], denLo = qlev[(int64_t)k * (nlay + 1) + l];
+            double *up = &dw_dqup[(int64_t)k * nlay + l], *lo = &dw_dqlo[(int64_t)k * nlay + l];
+            if (denLo != 0.0) {
+                double ratU = denUp / rhoU, ratL = denLo / rhoL;
+                *lo = (ratL / (ratL - alpha * ratU)) + 1.0 / log(alpha * ratU / ratL);
+                *up = ((-alpha * ratU) / (ratL - alpha * ratU)) - 1.0 / log(alpha * ratU / ratL);
+            } else {
+                *lo = 0.0;
+                *up = 0.0;
+            }
+        }
+    }
+}
+
+/* Jacobians (clblm_noScattJac.f90:346-716) at every point of v1 + i*dv for a
+ * down-looking sensor above the top layer (obsLev = nlay + 1), thermal only: the
+ * view OD of layer l is od[l*npts+i] (amfView 1), the down-path OD is sec_dn times it
+ * (the air-mass-scaled path, amfDown = sec_dn, pathOD_up_dn_sun :826-838).  Layer 0 is
+ * at the surface.  Outputs: rad, dr_dtau[l*npts+i] (derivative with respect to the
+ * layer's vertical OD), dr_dtlev[k*npts+i] for the nlay+1 levels (level Planck terms
+ * plus, for lin_in_tau 0 and 1, the layer-mean terms carried by dt_dtup/dt_dtlo),
+ * dr_dtskin and dr_demis.  The summation order is the reference's. */
+void orc_jacobians(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
+                   const double *t_lev, double emis, double tskin, int lin, double sec_dn, const double *dt_dtup,
+                   const double *dt_dtlo, double *rad_o, double *dr_dtau, double *dr_dtlev, double *dr_dtskin,
+                   double *dr_demis) {
+    const int L = nlay;
+    double *Blev = malloc(sizeof(double) * (L + 2)), *dBlev = malloc(sizeof(double) * (L + 2));
+    double *Bbar = malloc(sizeof(double) * (L + 1)), *dBbar = malloc(sizeof(double) * (L + 1));
+    double *cumTxA = malloc(sizeof(double) * (L + 2)), *cumTxB2 = malloc(sizeof(double) * (L + 2));
+    double *dTau = malloc(sizeof(double) * (L + 1)), *dTbar = malloc(sizeof(double) * (L + 1));
+    double *dTup = malloc(sizeof(double) * (L + 2)), *dTdn = malloc(sizeof(double) * (L + 2));
+    double *dTlev = malloc(sizeof(double) * (L + 2));
+    double *vOD = malloc(sizeof(double) * (L + 1)), *dOD = malloc(sizeof(double) * (L + 1));
+    /* 1-based arrays as in the reference: layers 1..L, levels 1..L+1 */
+    for (int64_t i = 0; i < npts; i++) {
+        double vn = v1 + (double)i * dv;
+        for (int l = 1; l <= L; l++) {
+            vOD[l] = od[(int64_t)(l - 1) * npts + i];
+            dOD[l] = vOD[l] * sec_dn;
+        }
+        if (lin != 0)
+            for (int k = 1; k <= L + 1; k++) orc_planck_drv(vn, t_lev[k - 1], &Blev[k], &dBlev[k]);
+        for (int l = 1; l <= L; l++) orc_planck_drv(vn, t_lay[l - 1], &Bbar[l], &dBbar[l]);
+        double Bsfc, dBsfc;
+        orc_planck_drv(vn, tskin, &Bsfc, &dBsfc);
+        double rad = 0.;
+        for (int l = 1; l <= L; l++) { dTau[l] = 0.; dTbar[l] = 0.; }
+        for (int k = 1; k <= L + 1; k++) { dTup[k] = 0.; dTdn[k] = 0.; dTlev[k] = 0.; }
+        const int obsLay = L + 1;
+        /* 0- transmittance along the view path from the top down to the surface */
+        double cumODA = 0.;
+        cumTxA[obsLay] = 1.;
+        for (int l = obsLay - 1; l >= 1; l--) {
+            cumODA = cumODA + vOD[l];
+            cumTxA[l] = exp(-cumODA);
+        }
+        double totODUp = cumODA, totTxUp = cumTxA[1];
+        /* 1- downwelling thermal radiance, reflected at the surface */
+        if (totTxUp > 1.e-06) {
+            cumTxB2[1] = totTxUp;
+            double cumODB = 0.;
+            for (int l = 1; l <= L; l++) {
+                cumODB = cumODB + dOD[l];
+                cumTxB2[l + 1] = exp(-(totODUp + cumODB));
+            }
+            for (int l = L; l >= 1; l--) {
+                double dTran = cumTxB2[l] - cumTxB2[l + 1], Blt;
+                if (lin != 0) {
+                    double h, dh;
+                    orc_rlin(dOD[l], &h, &dh);
+                    if (lin == 2) {
+                        Blt = Blev[l] + (Blev[l + 1] - Blev[l]) * h;
+                        dTau[l] = (cumTxB2[l + 1] * Blt - rad + dTran * dh * (Blev[l + 1] - Blev[l])) * sec_dn;
+                        dTdn[l + 1] = dTdn[l + 1] + dTran * dBlev[l + 1] * h;
+                        dTdn[l] = dTran * dBlev[l] * (1. - h);
+                        dTbar[l] = 0.0;
+                    } else {
+                        Blt = 2.0 * Bbar[l] * h + Blev[l] * (1.0 - 2.0 * h);
+                        dTau[l] = (cumTxB2[l + 1] * Blt - rad + 2.0 * dTran * dh * (Bbar[l] - Blev[l])) * sec_dn;
+                        dTdn[l] = dTran * dBlev[l] * (1.0 - 2.0 * h);
+                        dTbar[l] = dTran * 2.0 * dBbar[l] * h;
+                    }
+                    rad = rad + dTran * Blt;
+                } else {
+                    dTau[l] = (cumTxB2[l + 1] * Bbar[l] - rad) * sec_dn;
+                    rad = rad + dTran * Bbar[l];
+                    dTbar[l] = dTran * dBbar[l];
+                }
+            }
+        }
+        /* 2- scale by the surface reflectance */
+        double rsfc = (1. - emis);
+        for (int l = 1; l <= L; l++) {
+            dTau[l] = dTau[l] * rsfc;
+            dTbar[l] = dTbar[l] * rsfc;
+        }
+        /* 3- surface emission */
+        double dEmis = totTxUp * Bsfc - rad;
+        double dTsfc = emis * totTxUp * dBsfc;
+        double sfcEmit = emis * totTxUp * Bsfc;
+        rad = rad * rsfc + sfcEmit;
+        /* 5- upwelling thermal radiance */
+        for (int l = 1; l <= obsLay - 1; l++) {
+            double dTran = cumTxA[l + 1] - cumTxA[l], Blt;
+            if (lin != 0) {
+                double h, dh;
+                orc_rlin(vOD[l], &h, &dh);
+                if (lin == 2) {
+                    Blt = Blev[l + 1] * (1 - h) + Blev[l] * h;
+                    dTau[l] = dTau[l] + (cumTxA[l] * Blt - rad + dTran * dh * (Blev[l] - Blev[l + 1]));
+                    dTup[l + 1] = dTran * dBlev[l + 1] * (1 - h);
+                    dTup[l] = dTup[l] + dTran * dBlev[l] * h;
+                } else {
+                    Blt = 2.0 * Bbar[l] * h + Blev[l + 1] * (1.0 - 2.0 * h);
+                    dTau[l] = dTau[l] + (cumTxA[l] * Blt - rad + dTran * dh * 2.0 * (Bbar[l] - Blev[l + 1]));
+                    dTup[l + 1] = dTran * dBlev[l + 1] * (1.0 - 2.0 * h);
+                    dTbar[l] = dTbar[l] + dTran * 2.0 * dBbar[l] * h;
+                }
+                rad = rad + dTran * Blt;
+            } else {
+                dTau[l] = dTau[l] + (cumTxA[l] * Bbar[l] - rad);
+                rad = rad + dTran * Bbar[l];
+                dTbar[l] = dTbar[l] + dTran * dBbar[l];
+            }
+        }
+        /* level temperature derivatives */
+        if (lin != 0)
+            for (int k = 1; k <= L + 1; k++) dTlev[k] = dTup[k] + dTdn[k] * rsfc;
+        if (lin == 0 || lin == 1)
+            for (int l = L; l >= 1; l--) {
+                dTlev[l + 1] = dTlev[l + 1] + dTbar[l] * dt_dtup[l - 1];
+                dTlev[l] = dTlev[l] + dTbar[l] * dt_dtlo[l - 1];
+            }
+        rad_o[i] = rad;
+        if (dr_dtau)
+            for (int l = 1; l <= L; l++) dr_dtau[(int64_t)(l - 1) * npts + i] = dTau[l];
+        if (dr_dtlev)
+            for (int k = 1; k <= L + 1; k++) dr_dtlev[(int64_t)(k - 1) * npts + i] = dTlev[k];
+        if (dr_dtskin) dr_dtskin[i] = dTsfc;
+        if (dr_demis) dr_demis[i] = dEmis;
+    }
+    free(Blev); free(dBlev); free(Bbar); free(dBbar); free(cumTxA); free(cumTxB2);
+    free(dTau); free(dTbar); free(dTup); free(dTdn); free(dTlev); free(vOD); free(dOD);
+}
+
+/* drv_LevelTemp and drv_LevelConc (clblm_noScattJac.f90:899-1203): level Jacobians
+ * from the per-point derivatives and the layer-OD variants.  od_var holds 3 + nmol
+ * blocks of [nlay][npts]: the layer OD, the OD at Tave + dt, the OD at
+ * Wtot + dw_frac*Wtot, then the OD of each molecule alone.  dtau/dT is
+ * (od(T+dt) - od)/dt - dtau/dWtot * Wtot/Tave (the reference: dt = 1 K, dw_frac = 0.01).
+ * Outputs jac_t[k*npts+i] (nlay+1 levels) and jac_q[(m*(nlay+1)+k)*npts+i]
+ * (derivatives with respect to ln of the level density). */
+void orc_level_jac(int nlay, int nmol, int64_t npts, const double *od_var, const double *dr_dtau,
+                   const double *dr_dtlev, const double *wtot, const double *tave, double dt, double dw_frac,
+                   const double *dt_dtup, const double *dt_dtlo, const double *dw_dqup, const double *dw_dqlo,
+                   double *jac_t, double *jac_q) {
+    const int64_t LN = (int64_t)nlay * npts;
+    const double *od = od_var, *odp = od_var + LN, *odw = od_var + 2 * LN;
+    for (int64_t i = 0; i < npts; i++) {
+        if (jac_t) {
+            double lo = 0., up = 0.;
+            for (int l = 0; l < nlay; l++) {
+                int64_t j = (int64_t)l * npts + i;
+                double dTau_dT = (odp[j] - od[j]) / dt;
+                double deltaW = dw_frac * wtot[l];
+                double dTau_dW = (odw[j] - od[j]) / deltaW;
+                double fac = wtot[l] / tave[l];
+                dTau_dT = dTau_dT - dTau_dW * fac;
+                double dRaddT = dr_dtau[j] * dTau_dT;
+                up = dRaddT * dt_dtup[l];
+                lo = lo + dRaddT * dt_dtlo[l] + dr_dtlev[j];
+                jac_t[j] = lo;
+                lo = up;
+            }
+            jac_t[LN + i] = lo + dr_dtlev[LN + i];
+        }
+        if (jac_q)
+            for (int m = 0; m < nmol; m++) {
+                const double *odm = od_var + (3 + (int64_t)m) * LN;
+                double *jq = jac_q + (int64_t)m * (nlay + 1) * npts;
+                double lo = 0., up = 0.;
+                for (int l = 0; l < nlay; l++) {
+                    int64_t j = (int64_t)l * npts + i;
+                    double dRad_dW = dr_dtau[j] * odm[j];
+                    up = dRad_dW * dw_dqup[(int64_t)m * nlay + l];
+                    lo = lo + dRad_dW * dw_dqlo[(int64_t)m * nlay + l];
+                    jq[j] = lo;
+                    lo = up;
+                }
+                jq[LN + i] = lo;
+            }
+    }
+}
+
 /* ==== Instrument scan ===========================================================
  * clblm_FFTSCN_struct (clblm_FFT.f90:352-844) for the scanning functions with an
  * analytic apodization (IVX = 1), with LoadSpect (:1114-1334), Scntrn/Scnfnt

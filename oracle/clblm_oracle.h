@@ -80,6 +80,18 @@ void orc_rlin(double tau, double *w, double *dw);
 void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
                const double *t_lev, double emis, double tskin, int lin, double sec_dn, double *rad);
 
+/* Level Jacobians: level2layer, the per-point Jacobians and drv_LevelTemp/drv_LevelConc. */
+void orc_level2layer(int nlay, int nmol, const double *plev, const double *tlev, const double *tave,
+                     const double *qlev, double *dt_dtup, double *dt_dtlo, double *dw_dqup, double *dw_dqlo);
+void orc_jacobians(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
+                   const double *t_lev, double emis, double tskin, int lin, double sec_dn, const double *dt_dtup,
+                   const double *dt_dtlo, double *rad, double *dr_dtau, double *dr_dtlev, double *dr_dtskin,
+                   double *dr_demis);
+void orc_level_jac(int nlay, int nmol, int64_t npts, const double *od_var, const double *dr_dtau,
+                   const double *dr_dtlev, const double *wtot, const double *tave, double dt, double dw_frac,
+                   const double *dt_dtup, const double *dt_dtlo, const double *dw_dqup, const double *dw_dqlo,
+                   double *jac_t, double *jac_q);
+
 /* Instrument scan (FFT), see clblm_oracle.c. */
 typedef struct {
     int jfn;

@@ -175,3 +175,63 @@ def fft_scan(x, v1, dv, v1out, v2out, dvout, funct_id=5, hwhm=0.6034, fill=0.0):
         out, v1o = scan_interp(seg, p.v1s, p.dv, v1out, v2out, dvout)
         return out, v1o, dvout
     return seg, p.v1s, p.dv
+
+
+# ---- level Jacobians (clblm_noScattJac.f90) ---------------------------------------
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+def level2layer(p_lev, t_lev, t_lay, q_lev):
+    """level2layer: q_lev [nmol, nlay+1] level densities.  Returns dT_dTup, dT_dTlo [nlay],
+    dlnW_dlnQup, dlnW_dlnQlo [nmol, nlay]."""
+    L = lib()
+    t_lay = np.ascontiguousarray(t_lay, np.float64)
+    nlay = t_lay.size
+    q = np.ascontiguousarray(np.atleast_2d(q_lev), np.float64) if q_lev is not None else np.zeros((0, nlay + 1))
+    nmol = q.shape[0]
+    outs = [np.zeros(nlay), np.zeros(nlay), np.zeros((nmol, nlay)), np.zeros((nmol, nlay))]
+    p_lev = np.ascontiguousarray(p_lev, np.float64)
+    t_lev = np.ascontiguousarray(t_lev, np.float64)
+    L.orc_level2layer.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 8
+    L.orc_level2layer(nlay, nmol, _p(p_lev), _p(t_lev), _p(t_lay), _p(q), *[_p(o) for o in outs])
+    return tuple(outs)
+
+
+def jacobians(od, t_lay, t_lev, v1, dv, emis=0.98, tskin=295.0, lin=1, sec_dn=1.0, dt_dtup=None, dt_dtlo=None):
+    """Per-point Jacobians (down-looking, thermal): returns rad [n], dR/dtau [nlay, n],
+    dR/dT_lev [nlay+1, n], dR/dTskin [n], dR/demis [n]."""
+    L = lib()
+    od = np.ascontiguousarray(od, np.float64)
+    nlay, n = od.shape
+    up = np.zeros(nlay) if dt_dtup is None else np.ascontiguousarray(dt_dtup, np.float64)
+    lo = np.zeros(nlay) if dt_dtlo is None else np.ascontiguousarray(dt_dtlo, np.float64)
+    rad, dtau, dtlev, dts, de = np.zeros(n), np.zeros((nlay, n)), np.zeros((nlay + 1, n)), np.zeros(n), np.zeros(n)
+    D = ctypes.c_double
+    L.orc_jacobians.argtypes = [ctypes.c_int, ctypes.c_int64, D, D, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                D, D, ctypes.c_int, D] + [ctypes.c_void_p] * 7
+    t_lay = np.ascontiguousarray(t_lay, np.float64)
+    t_lev = np.ascontiguousarray(t_lev, np.float64)
+    L.orc_jacobians(nlay, n, v1, dv, _p(od), _p(t_lay), _p(t_lev), emis, tskin, lin, sec_dn, _p(up), _p(lo), _p(rad),
+                    _p(dtau), _p(dtlev), _p(dts), _p(de))
+    return rad, dtau, dtlev, dts, de
+
+
+def level_jac(od_var, dr_dtau, dr_dtlev, wtot, tave, dt, dw_frac, dt_dtup, dt_dtlo, dw_dqup, dw_dqlo):
+    """drv_LevelTemp + drv_LevelConc: od_var [3+nmol, nlay, n].  Returns jac_t [nlay+1, n]
+    and jac_q [nmol, nlay+1, n]."""
+    L = lib()
+    od_var = np.ascontiguousarray(od_var, np.float64)
+    nv, nlay, n = od_var.shape
+    nmol = nv - 3
+    jt, jq = np.zeros((nlay + 1, n)), np.zeros((nmol, nlay + 1, n))
+    D = ctypes.c_double
+    L.orc_level_jac.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64] + [ctypes.c_void_p] * 5 + [D, D] + \
+        [ctypes.c_void_p] * 6
+    keep = [np.ascontiguousarray(a, np.float64) for a in
+            (dr_dtau, dr_dtlev, wtot, tave, dt_dtup, dt_dtlo, np.reshape(dw_dqup, (nmol, nlay)),
+             np.reshape(dw_dqlo, (nmol, nlay)))]
+    a = [_p(k) for k in keep]
+    L.orc_level_jac(nlay, nmol, n, _p(od_var), a[0], a[1], a[2], a[3], dt, dw_frac, a[4], a[5], a[6], a[7], _p(jt),
+                    _p(jq))
+    return jt, jq

@@ -105,18 +105,89 @@ def test_autograd_radiance(ctx):
     np.testing.assert_allclose(tl.grad.cpu().numpy(), (g_t * wn[None, :]).sum(1), rtol=1e-12)
 
 
-def _jac_setup(ctx, prec, n_lines=3000, nlay=12):
+def _d(a):
+    import torch
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def _rt_jac(ctx, od, t_lay, t_lev, up, lo, v1, dv, lin, sec, emis=0.98, tskin=295.0):
+    import torch
+    from clblm_amd._lib import check
+    L, n = od.shape
+    e = lambda *sh: torch.empty(*sh, dtype=torch.float64, device="cuda")
+    outs = [e(n), e(L, n), e(L + 1, n), e(n), e(n)]
+    ins = [_d(od), _d(t_lay), _d(t_lev), _d(up), _d(lo)]
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    check(ctx.lib.clblm_rt_jac(ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(dv), *[p(t) for t in ins],
+                               ctypes.c_double(emis), ctypes.c_double(tskin), lin, ctypes.c_double(sec),
+                               *[p(t) for t in outs], None))
+    return [t.cpu().numpy() for t in outs]
+
+
+@pytest.mark.parametrize("lin,sec,opaque", [(0, 1.0, False), (0, 1.66, False), (1, 1.0, False), (1, 1.66, False),
+                                            (2, 1.5, False), (1, 1.66, True), (2, 1.0, True)])
+def test_rt_jac_matches_oracle(ctx, lin, sec, opaque):
+    """clblm_rt_jac against the oracle's Jacobians restatement, the reference's own
+    formulation (radiance, dR/dtau, dR/dT_lev, surface terms); the opaque case takes the
+    branch without the reflected downwelling."""
+    from oracle.py import jacobians
+    od, t_lay, t_lev = _case(seed=31, nlay=15, n=3000)
+    if opaque:
+        od = od + 2.0
+    rng = np.random.default_rng(32)
+    up, lo = rng.uniform(0.3, 0.7, 15), rng.uniform(0.3, 0.7, 15)
+    got = _rt_jac(ctx, od, t_lay, t_lev, up, lo, 620.0, 0.013, lin, sec)
+    ref = jacobians(od, t_lay, t_lev, 620.0, 0.013, emis=0.98, tskin=295.0, lin=lin, sec_dn=sec, dt_dtup=up,
+                    dt_dtlo=lo)
+    for name, a, b in zip(("rad", "dtau", "dtlev", "dtskin", "demis"), got, ref):
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print("%s max err / max = %.2e" % (name, err))
+        assert err < 1e-12, name
+
+
+def test_level_jac_matches_oracle(ctx):
+    import torch
+    from clblm_amd._lib import check
+    from oracle.py import level_jac
+    rng = np.random.default_rng(41)
+    L, M, n = 11, 3, 2500
+    od_var = 10 ** rng.uniform(-4, 0.5, (3 + M, L, n))
+    dtau, dtlev = rng.normal(0, 1e-6, (L, n)), rng.normal(0, 1e-8, (L + 1, n))
+    wtot, tave = rng.uniform(1e21, 1e24, L), rng.uniform(200, 300, L)
+    up, lo = rng.uniform(0.3, 0.7, L), rng.uniform(0.3, 0.7, L)
+    wu, wl = rng.uniform(0.2, 0.8, (M, L)), rng.uniform(0.2, 0.8, (M, L))
+    jt, jq = (torch.empty(L + 1, n, dtype=torch.float64, device="cuda"),
+              torch.empty(M, L + 1, n, dtype=torch.float64, device="cuda"))
+    ins = [_d(a) for a in (od_var, dtau, dtlev, wtot, tave)]
+    fac = [_d(a) for a in (up, lo, wu.reshape(-1), wl.reshape(-1))]
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    check(ctx.lib.clblm_level_jac(ctx.ptr, L, M, n, *[p(t) for t in ins], ctypes.c_double(1.0),
+                                  ctypes.c_double(0.01), *[p(t) for t in fac], p(jt), p(jq), None))
+    torch.cuda.synchronize()
+    rt, rq = level_jac(od_var, dtau, dtlev, wtot, tave, 1.0, 0.01, up, lo, wu, wl)
+    np.testing.assert_allclose(jt.cpu().numpy(), rt, rtol=1e-13, atol=1e-13 * np.abs(rt).max())
+    np.testing.assert_allclose(jq.cpu().numpy(), rq, rtol=1e-13, atol=1e-13 * np.abs(rq).max())
+
+
+def _scene(nlay=12, n_lines=3000):
     # a thin line list and a non-isothermal profile, so that every layer shows through
-    from clblm_amd.jacobian import UpwellingJacobian
     lines = W.profile_lines(n_lines, 500.0, 800.0, seed=1001)
     lines.s[:] = lines.s * np.float32(1e-3)
     noise = np.random.default_rng(5).normal(0.0, 3.0, nlay)
     layers = W.atmosphere(nlay, 1.0, t_noise=noise)
     t_lev = np.array([layers[0].tave + 2.0] + [0.5 * (a.tave + b.tave) for a, b in zip(layers[:-1], layers[1:])] +
                      [layers[-1].tave - 1.0])
+    p_lev = W.level_pressures(nlay, 1.0)
+    q_lev = W.level_densities(nlay, 1.0)
     dvs = [W.calculate_dv(650.0, l.pave, l.tave, l.w[1] / l.wtot) for l in layers]
+    return lines, layers, t_lev, p_lev, q_lev, dvs
+
+
+def _jac_setup(ctx, prec, nlay=12, mols=None):
+    from clblm_amd.jacobian import UpwellingJacobian
+    lines, layers, t_lev, p_lev, q_lev, dvs = _scene(nlay)
     h = ctx.upload_lines(lines)
-    return UpwellingJacobian(ctx, h, layers, t_lev, dvs, precision=prec, dt=1.0,
+    return UpwellingJacobian(ctx, h, layers, t_lev, dvs, p_lev=p_lev, q_lev=q_lev, mols=mols, precision=prec,
                              continuum=lambda lay, a, b: W.continuum_absrb(lay, a, b))
 
 
@@ -133,20 +204,38 @@ def test_jacobian_fp32c_vs_fp64(ctx):
         assert err < 1e-4, (k, err)
 
 
-def test_jacobian_t_vs_pipeline_difference(ctx):
-    """dR/dT_l from the block equals the central difference of the whole OD+radiance pipeline."""
-    import copy
+def test_level_jacobians_match_the_oracle_pipeline(ctx):
+    """The whole noScattJacob block in FP64 against the oracle: every layer-OD variant
+    from the oracle's ODLAY, then its Jacobians and drv_LevelTemp/drv_LevelConc.  The
+    layer ODs agree to 1e-10, so differences of ODs (T + 1 K, Wtot + 1 %) agree to
+    about 1e-10 / (relative size of the difference)."""
     from clblm_amd._lib import FP64
-    jb = _jac_setup(ctx, FP64)
-    jb.dt = 0.1
-    base = jb.block(640.0, 650.0)
-    for l in (0, 5, 11):
-        plus, minus = copy.copy(jb), copy.copy(jb)
-        plus.layers = [copy.copy(x) for x in jb.layers]
-        minus.layers = [copy.copy(x) for x in jb.layers]
-        plus.layers[l].tave += 0.1
-        minus.layers[l].tave -= 0.1
-        rp, rm = plus.block(640.0, 650.0)["rad"], minus.block(640.0, 650.0)["rad"]
-        fd = ((rp - rm) / 0.2).cpu().numpy()
-        j = base["jac_t"][l].cpu().numpy()
-        assert np.abs(j - fd).max() <= 1e-3 * np.abs(fd).max() + 1e-15, l
+    from oracle.py import Oracle, jacobians, level_jac
+    jb = _jac_setup(ctx, FP64, nlay=8, mols=[1, 2, 3])
+    v1, v2 = 640.0, 646.0
+    got = jb.block(v1, v2)
+    lines, layers, t_lev, p_lev, q_lev, dvs = _scene(8)
+    orc = Oracle(lines)
+    orc.jrad = 1
+    jobs, pars = jb._jobs(v1, v2)
+    L, n = len(layers), got["n"]
+    od_var = []
+    for k, (lay, par) in enumerate(zip(jobs, pars)):
+        c = W.continuum_absrb(lay, v1, v2)[0] if (k < 3 * L or jb.mols[(k - 3 * L) // L] == 1) else None
+        od, _ = orc.odlay(lay, v1, v2, dvs[k % L], dvout=jb.dvout, continuum=c)
+        assert od.size == n
+        od_var.append(od)
+    od_var = np.array(od_var).reshape(3 + len(jb.mols), L, n)
+    np.testing.assert_allclose(got["od_var"].cpu().numpy(), od_var, rtol=1e-10, atol=1e-10 * np.abs(od_var).max())
+    t_lay = np.array([l.tave for l in layers])
+    rad, dtau, dtlev, dts, de = jacobians(od_var[0], t_lay, t_lev, v1, jb.dvout, emis=jb.emis, tskin=jb.tskin,
+                                          lin=jb.lin, sec_dn=jb.sec_dn, dt_dtup=jb.dt_dtup, dt_dtlo=jb.dt_dtlo)
+    jt, jq = level_jac(od_var, dtau, dtlev, [l.wtot for l in layers], t_lay, jb.dt, jb.dw_frac, jb.dt_dtup,
+                       jb.dt_dtlo, jb.dw_dqup, jb.dw_dqlo)
+    for name, a, b, tol in (("rad", got["rad"], rad, 1e-11), ("dr_dtau", got["dr_dtau"], dtau, 1e-9),
+                            ("jac_t", got["jac_t"], jt, 1e-7), ("jac_lnq", got["jac_lnq"], jq, 1e-9),
+                            ("dr_dtskin", got["dr_dtskin"], dts, 1e-11), ("dr_demis", got["dr_demis"], de, 1e-11)):
+        a = a.cpu().numpy()
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print("%s max err / max = %.2e" % (name, err))
+        assert err < tol, (name, err)
