@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -905,19 +906,31 @@ __device__ inline void f4_candidates(const Job &J, const R4Blk &B, const Shrunk 
 //   base(x)  = (c2 - x^2 rb) f4bnd        (c2 = 2, rb = 1/BNDSQ for CO2 lines; 1, 0 otherwise)
 //   shift(f) = f + x dptrat f             (dptrat = 0 when the line has no shift)
 // which reproduces CNVF4Q's CO2 and other-molecule loops (clblm_ODLAY_LineF4.f90:388-462).
+// [jqlo, jqhi] contains every point where the shape takes its quadratic form
+// (zeta^2 <= 64^2, :397-401) with two points of margin; a tile outside it, inside
+// [jmin, jmax] and with no shift needs only the Lorentz form.
+enum { F4_CO2 = 1, F4_SHIFT = 2, F4_MIRROR = 4 };
 struct alignas(16) F4Rec32 {
-    float d, sil, alfli2, rec_alfvi2;  // x(j) = (j - jc) dvr4 + d, FP64-anchored
-    float siv_a3, siv_b3, b0, b1;      // base(x) = b0 + b1 x^2
-    float dptrat, dm;
-    int jc, jcm;
-    int jmin, jmax, jminm, jmaxm;
+    float d, sil, alfli2, b0;          // x(j) = (j - jc) dvr4 + d, FP64-anchored; base(x) = b0 + b1 x^2
+    int jc, jmin, jmax, kind;
+    int jqlo, jqhi;
+    float b1, dptrat;
+    float rec_alfvi2, siv_a3, siv_b3, dm;
+    int jcm, jminm, jmaxm, pad;
 };
 struct alignas(16) F4Rec64 {
     double xnui, xnum, sil, alfli2, rec_alfvi2, siv_a3, siv_b3, f4bnd, c2, rb, dptrat;
     int jmin, jmax, jminm, jmaxm;
-    double pad[3];
+    int jqlo, jqhi, kind, pad;
+    double pad2;
 };
-static_assert(sizeof(F4Rec32) == 64 && sizeof(F4Rec64) == 128, "F4 record layout");
+static_assert(sizeof(F4Rec32) == 80 && sizeof(F4Rec64) == 128, "F4 record layout");
+// the gathers read these words as 16-byte vectors: F4Rec32 words 4-9, F4Rec64 words 22-28
+static_assert(offsetof(F4Rec32, jc) == 16 && offsetof(F4Rec32, kind) == 28 && offsetof(F4Rec32, jqlo) == 32 &&
+                  offsetof(F4Rec32, b1) == 40,
+              "F4Rec32 control words");
+static_assert(offsetof(F4Rec64, jmin) == 88 && offsetof(F4Rec64, jqlo) == 104 && offsetof(F4Rec64, kind) == 112,
+              "F4Rec64 control words");
 
 // Record offsets of each block (one lane per layer walks its blocks in order).
 __global__ void k_f4recoff(const Job *jobs, int nlay, R4Blk *blks, int *err) {
@@ -933,10 +946,20 @@ __global__ void k_f4recoff(const Job *jobs, int nlay, R4Blk *blks, int *err) {
     if (o - J.f4rec_off > J.f4rec_cap) atomicOr(err, 8);
 }
 
+// the quadratic zone of a line at xnui, widened by two points each side
+__device__ inline void f4_qzone(const F4Line &d, const Job &J, int *lo, int *hi) {
+    const double R = 64. * sqrt(1. / d.rec_alfvi2);
+    *lo = (int)floor((d.xnui - R) / J.dvr4) + 1 - 2;
+    *hi = (int)ceil((d.xnui + R) / J.dvr4) + 1 + 2;
+}
+__device__ inline int f4_kind(const F4Line &d) {
+    return ((d.flags & 1) ? F4_CO2 : 0) | ((d.flags & 2) ? 0 : F4_SHIFT) | (d.jmaxm >= d.jminm ? F4_MIRROR : 0);
+}
+
 template <typename R>
 __device__ inline void f4_fill(R &r, const F4Line &d, const Job &J);
 template <>
-__device__ inline void f4_fill<F4Rec64>(F4Rec64 &r, const F4Line &d, const Job &) {
+__device__ inline void f4_fill<F4Rec64>(F4Rec64 &r, const F4Line &d, const Job &J) {
     const bool co2 = (d.flags & 1) != 0;
     r.xnui = d.xnui; r.xnum = d.xnum; r.sil = d.sil; r.alfli2 = d.alfli2; r.rec_alfvi2 = d.rec_alfvi2;
     r.siv_a3 = d.siv_a3; r.siv_b3 = d.siv_b3; r.f4bnd = d.f4bnd;
@@ -944,12 +967,18 @@ __device__ inline void f4_fill<F4Rec64>(F4Rec64 &r, const F4Line &d, const Job &
     r.rb = co2 ? 1. / 625. : 0.;
     r.dptrat = (d.flags & 2) ? 0. : d.dptrat;
     r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm;
-    r.pad[0] = r.pad[1] = r.pad[2] = 0.;
+    r.kind = f4_kind(d);
+    if (d.jmax >= d.jmin) f4_qzone(d, J, &r.jqlo, &r.jqhi);
+    else { r.jqlo = 1; r.jqhi = 0; }
+    r.pad = 0;
+    r.pad2 = 0.;
 }
 template <>
 __device__ inline void f4_fill<F4Rec32>(F4Rec32 &r, const F4Line &d, const Job &J) {
     const bool co2 = (d.flags & 1) != 0;
     r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm;
+    r.kind = f4_kind(d);
+    r.pad = 0;
     r.sil = (float)d.sil; r.alfli2 = (float)d.alfli2; r.rec_alfvi2 = (float)d.rec_alfvi2;
     r.siv_a3 = (float)d.siv_a3; r.siv_b3 = (float)d.siv_b3;
     r.b0 = (float)(co2 ? 2. * d.f4bnd : d.f4bnd);
@@ -960,7 +989,8 @@ __device__ inline void f4_fill<F4Rec32>(F4Rec32 &r, const F4Line &d, const Job &
         int jc = (int)floor(d.xnui / J.dvr4 + 0.5) + 1;
         r.jc = jc;
         r.d = (float)((double)(jc - 1) * J.dvr4 - d.xnui);
-    } else { r.jc = 0; r.d = 0.f; }
+        f4_qzone(d, J, &r.jqlo, &r.jqhi);
+    } else { r.jc = 0; r.d = 0.f; r.jqlo = 1; r.jqhi = 0; }
     if (d.jmaxm >= d.jminm) {
         int jc = (int)floor(d.xnum / J.dvr4 + 0.5) + 1;
         r.jcm = jc;
@@ -1074,9 +1104,27 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
             }
             for (int q = 0; q < nb; q++) {
                 const F4Rec64 &l = *reinterpret_cast<const F4Rec64 *>(&srec[q * W]);
-                if (l.jmin <= l.jmax) {
-                    nev += f4_count(l.jmin, l.jmax, j0, j1);
-                    if (l.jmin <= j0 && l.jmax >= j1) {
+                // control words (bytes 88-119) as scalars, so that the per-record tests are
+                // scalar branches
+                const u32x4 h5 = srec[q * W + 5], h6 = srec[q * W + 6], h7 = srec[q * W + 7];
+                const int jmin = __builtin_amdgcn_readfirstlane((int)h5.z);
+                const int jmax = __builtin_amdgcn_readfirstlane((int)h5.w);
+                const int jqlo = __builtin_amdgcn_readfirstlane((int)h6.z);
+                const int jqhi = __builtin_amdgcn_readfirstlane((int)h6.w);
+                const int kind = __builtin_amdgcn_readfirstlane((int)h7.x);
+                if (jmin <= jmax) {
+                    nev += f4_count(jmin, jmax, j0, j1);
+                    const bool cover = (jmin <= j0) & (jmax >= j1);
+                    if (cover & ((jqhi < j0) | (jqlo > j1)) & ((kind & F4_SHIFT) == 0)) {
+                        // the whole tile in the Lorentz zone: f4_eval64 without the branches
+                        const double xnui = l.xnui, sil = l.sil, alfli2 = l.alfli2, c2 = l.c2, rb = l.rb,
+                                     f4bnd = l.f4bnd;
+#pragma unroll
+                        for (int k = 0; k < F4K; k++) {
+                            const double xm = xj[k] - xnui, xmsq = xm * xm;
+                            acc[k] += div_nr(sil, alfli2 + xmsq) - (c2 - xmsq * rb) * f4bnd;
+                        }
+                    } else if (cover) {
 #pragma unroll
                         for (int k = 0; k < F4K; k++) acc[k] += f4_eval64(l, xj[k] - l.xnui, l.dptrat);
                     } else {
@@ -1084,11 +1132,11 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
                         for (int k = 0; k < F4K; k++) {
                             const int j = j0 + lane + 64 * k;
                             double f = f4_eval64(l, xj[k] - l.xnui, l.dptrat);
-                            acc[k] += (j >= l.jmin && j <= l.jmax) ? f : 0.;
+                            acc[k] += (j >= jmin && j <= jmax) ? f : 0.;
                         }
                     }
                 }
-                if (l.jminm <= l.jmaxm) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
+                if (kind & F4_MIRROR) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
 #pragma unroll
                     for (int k = 0; k < F4K; k++) {
                         const int j = j0 + lane + 64 * k;
@@ -1154,10 +1202,38 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
             }
             for (int q = 0; q < nb; q++) {
                 const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(&srec[q * W]);
-                if (l.jmin <= l.jmax) {
-                    nev += f4_count(l.jmin, l.jmax, j0, j1);
-                    const float fl = (float)(j0 + lane - l.jc);
-                    if (l.jmin <= j0 && l.jmax >= j1) {
+                // the record's first 48 bytes as three broadcast reads; its control words
+                // as scalars, so that the per-record tests are scalar branches
+                const u32x4 h0 = srec[q * W], h1 = srec[q * W + 1], h2 = srec[q * W + 2];
+                const int jmin = __builtin_amdgcn_readfirstlane((int)h1.y);
+                const int jmax = __builtin_amdgcn_readfirstlane((int)h1.z);
+                const int kind = __builtin_amdgcn_readfirstlane((int)h1.w);
+                const int jqlo = __builtin_amdgcn_readfirstlane((int)h2.x);
+                const int jqhi = __builtin_amdgcn_readfirstlane((int)h2.y);
+                if (jmin <= jmax) {
+                    nev += f4_count(jmin, jmax, j0, j1);
+                    const float fl = (float)(j0 + lane - (int)h1.x);
+                    const bool cover = (jmin <= j0) & (jmax >= j1);
+                    if (cover & ((jqhi < j0) | (jqlo > j1)) & ((kind & F4_SHIFT) == 0)) {
+                        // the whole tile in the Lorentz zone: sil/(alfli2 + x^2) - base(x)
+                        const float d = __uint_as_float(h0.x), sil = __uint_as_float(h0.y);
+                        const float alfli2 = __uint_as_float(h0.z), b0 = __uint_as_float(h0.w);
+                        if (kind & F4_CO2) {
+                            const float b1 = __uint_as_float(h2.z);
+#pragma unroll
+                            for (int k = 0; k < F4K; k++) {
+                                const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d), xmsq = xm * xm;
+                                const float f = __fmaf_rn(sil, __builtin_amdgcn_rcpf(alfli2 + xmsq), -b0);
+                                acc[k] += (double)__fmaf_rn(-xmsq, b1, f);
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < F4K; k++) {
+                                const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d);
+                                acc[k] += (double)__fmaf_rn(sil, __builtin_amdgcn_rcpf(__fmaf_rn(xm, xm, alfli2)), -b0);
+                            }
+                        }
+                    } else if (cover) {
 #pragma unroll
                         for (int k = 0; k < F4K; k++)
                             acc[k] += (double)f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
@@ -1166,11 +1242,11 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
                         for (int k = 0; k < F4K; k++) {
                             const int j = j0 + lane + 64 * k;
                             float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
-                            acc[k] += (double)((j >= l.jmin && j <= l.jmax) ? f : 0.f);
+                            acc[k] += (double)((j >= jmin && j <= jmax) ? f : 0.f);
                         }
                     }
                 }
-                if (l.jminm <= l.jmaxm) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
+                if (kind & F4_MIRROR) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
                     const float fl = (float)(j0 + lane - l.jcm);
 #pragma unroll
                     for (int k = 0; k < F4K; k++) {
