@@ -1042,20 +1042,46 @@ __device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dpt
     return __fmaf_rn(xm * dptrat, f, f);
 }
 
-// F4 gather: one wave per tile of F4PTS consecutive R4 points, F4K points per lane
-// (j = j0 + lane + 64 k), lines walked in the reference's order so each point's sum
-// has the reference's summation order.  Records pass through LDS and each is used
-// for F4K points per lane; records that cover the whole tile skip the per-point
-// range test.
 constexpr int F4K = 4, F4PTS = 64 * F4K;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int RAMP_CAP = 256;  // ramp pieces per panel held in LDS
+
 
 __device__ __forceinline__ unsigned long long f4_count(int jmin, int jmax, int j0, int j1) {
     int a = jmin > j0 ? jmin : j0, b = jmax < j1 ? jmax : j1;
     return b >= a ? (unsigned long long)(b - a + 1) : 0ull;
 }
 
+// Plan of the records staged for a tile, made by the lane that stages each record and
+// shared as ballots, so that the walk over the records branches on scalar bits:
+//   touch  the record reaches the tile;
+//   cover  its range [jmin, jmax] contains the whole tile;
+//   lorp   the tile lies outside its quadratic zone and it has no shift, so the
+//          Lorentz form alone gives the shape there;
+//   all    cover and lorp: no per-point test at all.
+struct F4Plan {
+    unsigned long long all, cover, lorp, touch, co2, mirror;
+};
+__device__ __forceinline__ void f4_plan(F4Plan &P, bool live, int jmin, int jmax, int jqlo, int jqhi, int kind,
+                                        int j0, int j1) {
+    P.mirror = __ballot(live && (kind & F4_MIRROR) != 0);
+    live = live && jmin <= jmax;
+    const bool touch = live && jmin <= j1 && jmax >= j0;
+    const bool cover = live && jmin <= j0 && jmax >= j1;
+    const bool lorq = (jqhi < j0 || jqlo > j1) && (kind & F4_SHIFT) == 0;
+    P.touch = __ballot(touch);
+    P.cover = __ballot(cover);
+    P.lorp = __ballot(touch && lorq);
+    P.all = __ballot(cover && lorq);
+    P.co2 = __ballot(live && (kind & F4_CO2) != 0);
+}
+
+// F4 gather, FP64: one wave per tile of F4PTS consecutive R4 points, F4K points per
+// lane (j = j0 + lane + 64 k), lines walked in the reference's order so each point's
+// sum has the reference's summation order.  Records pass through LDS 64 at a time
+// (one coalesced load per lane, the next batch in flight while this one is summed)
+// and are read back as broadcasts of plain 16-byte vectors (so that no copy of a
+// record is made in private memory).
 __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
                                                  const F4Rec64 *__restrict__ recs, double *R4,
                                                  unsigned long long *work) {
@@ -1070,13 +1096,10 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
         acc[k] = 0.;
         xj[k] = (double)(j0 + lane + 64 * k - 1) * J.dvr4;
     }
-    unsigned long long nev = 0;
+    unsigned long long nev = 0;   // (line, point) pairs of this tile, one record per lane
     if (j0 <= B.nptr4) {
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
-        // records are staged in LDS 64 at a time (one coalesced load per lane, the
-        // next batch in flight while this one is summed) and read back as broadcasts
-        // (plain 16-byte vectors, so that no copy of a record is made in private memory)
         constexpr int W = (int)(sizeof(F4Rec64) / 16);
         const u32x4 *r = reinterpret_cast<const u32x4 *>(recs + B.rec - B.lo);
         __shared__ u32x4 srec[64 * W];
@@ -1091,6 +1114,9 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
         }
         for (int64_t base = c0; base < c1; base += 64) {
             const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
+            F4Plan P;
+            f4_plan(P, lane < nb, (int)pre[5].z, (int)pre[5].w, (int)pre[6].z, (int)pre[6].w, (int)pre[7].x, j0, j1);
+            if (lane < nb) nev += f4_count((int)pre[5].z, (int)pre[5].w, j0, j1);
             __syncthreads();
             if (lane < nb) {
 #pragma unroll
@@ -1103,40 +1129,40 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
                 for (int w = 0; w < W; w++) pre[w] = r[k * W + w];
             }
             for (int q = 0; q < nb; q++) {
+                const unsigned long long bit = 1ull << q;
                 const F4Rec64 &l = *reinterpret_cast<const F4Rec64 *>(&srec[q * W]);
-                // control words (bytes 88-119) as scalars, so that the per-record tests are
-                // scalar branches
-                const u32x4 h5 = srec[q * W + 5], h6 = srec[q * W + 6], h7 = srec[q * W + 7];
-                const int jmin = __builtin_amdgcn_readfirstlane((int)h5.z);
-                const int jmax = __builtin_amdgcn_readfirstlane((int)h5.w);
-                const int jqlo = __builtin_amdgcn_readfirstlane((int)h6.z);
-                const int jqhi = __builtin_amdgcn_readfirstlane((int)h6.w);
-                const int kind = __builtin_amdgcn_readfirstlane((int)h7.x);
-                if (jmin <= jmax) {
-                    nev += f4_count(jmin, jmax, j0, j1);
-                    const bool cover = (jmin <= j0) & (jmax >= j1);
-                    if (cover & ((jqhi < j0) | (jqlo > j1)) & ((kind & F4_SHIFT) == 0)) {
-                        // the whole tile in the Lorentz zone: f4_eval64 without the branches
-                        const double xnui = l.xnui, sil = l.sil, alfli2 = l.alfli2, c2 = l.c2, rb = l.rb,
-                                     f4bnd = l.f4bnd;
+                if (P.all & bit) {
+                    // the whole tile in the Lorentz zone: f4_eval64 without the branches
+                    const double xnui = l.xnui, sil = l.sil, alfli2 = l.alfli2, c2 = l.c2, rb = l.rb, f4bnd = l.f4bnd;
 #pragma unroll
-                        for (int k = 0; k < F4K; k++) {
-                            const double xm = xj[k] - xnui, xmsq = xm * xm;
-                            acc[k] += div_nr(sil, alfli2 + xmsq) - (c2 - xmsq * rb) * f4bnd;
-                        }
-                    } else if (cover) {
+                    for (int k = 0; k < F4K; k++) {
+                        const double xm = xj[k] - xnui, xmsq = xm * xm;
+                        acc[k] += div_nr(sil, alfli2 + xmsq) - (c2 - xmsq * rb) * f4bnd;
+                    }
+                } else if (P.cover & bit) {
 #pragma unroll
-                        for (int k = 0; k < F4K; k++) acc[k] += f4_eval64(l, xj[k] - l.xnui, l.dptrat);
-                    } else {
+                    for (int k = 0; k < F4K; k++) acc[k] += f4_eval64(l, xj[k] - l.xnui, l.dptrat);
+                } else if (P.lorp & bit) {
+                    // part of the tile, all of it in the Lorentz zone
+                    const double xnui = l.xnui, sil = l.sil, alfli2 = l.alfli2, c2 = l.c2, rb = l.rb, f4bnd = l.f4bnd;
+                    const int jmn = l.jmin, jmx = l.jmax;
 #pragma unroll
-                        for (int k = 0; k < F4K; k++) {
-                            const int j = j0 + lane + 64 * k;
-                            double f = f4_eval64(l, xj[k] - l.xnui, l.dptrat);
-                            acc[k] += (j >= jmin && j <= jmax) ? f : 0.;
-                        }
+                    for (int k = 0; k < F4K; k++) {
+                        const int j = j0 + lane + 64 * k;
+                        const double xm = xj[k] - xnui, xmsq = xm * xm;
+                        const double f = div_nr(sil, alfli2 + xmsq) - (c2 - xmsq * rb) * f4bnd;
+                        acc[k] += (j >= jmn && j <= jmx) ? f : 0.;
+                    }
+                } else if (P.touch & bit) {
+                    const int jmn = l.jmin, jmx = l.jmax;
+#pragma unroll
+                    for (int k = 0; k < F4K; k++) {
+                        const int j = j0 + lane + 64 * k;
+                        double f = f4_eval64(l, xj[k] - l.xnui, l.dptrat);
+                        acc[k] += (j >= jmn && j <= jmx) ? f : 0.;
                     }
                 }
-                if (kind & F4_MIRROR) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
+                if (P.mirror & bit) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
 #pragma unroll
                     for (int k = 0; k < F4K; k++) {
                         const int j = j0 + lane + 64 * k;
@@ -1152,10 +1178,13 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
         const int j = j0 + lane + 64 * k;
         if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? acc[k] : 0.;
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nev += __shfl_xor(nev, o);
     if (lane == 0 && nev) atomicAdd(work + 1, nev);
 }
 
-// FP32 line shapes (offsets from the FP64 per-record anchor), FP64 sums per point.
+// F4 gather, FP32 line shapes (offsets from the FP64 per-record anchor) with FP64 sums
+// per point; the same walk as k_f4conv64.
 __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
                                                  const F4Rec32 *__restrict__ recs, double *R4,
                                                  unsigned long long *work) {
@@ -1168,18 +1197,13 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
     double acc[F4K];
 #pragma unroll
     for (int k = 0; k < F4K; k++) acc[k] = 0.;
-    unsigned long long nev = 0;
+    unsigned long long nev = 0;   // (line, point) pairs of this tile, one record per lane
     if (j0 <= B.nptr4) {
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
-        // records are staged in LDS 64 at a time (one coalesced load per lane, the
-        // next batch in flight while this one is summed) and read back as broadcasts
-        // (plain 16-byte vectors, so that no copy of a record is made in private memory)
         constexpr int W = (int)(sizeof(F4Rec32) / 16);
         const u32x4 *r = reinterpret_cast<const u32x4 *>(recs + B.rec - B.lo);
         __shared__ u32x4 srec[64 * W];
-        // the prefetch is unconditional (index clamped to the last record) so that it
-        // stays in registers
         u32x4 pre[W];
         const int64_t last = c1 - 1;
         if (c1 > c0) {
@@ -1189,6 +1213,9 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
         }
         for (int64_t base = c0; base < c1; base += 64) {
             const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
+            F4Plan P;
+            f4_plan(P, lane < nb, (int)pre[1].y, (int)pre[1].z, (int)pre[2].x, (int)pre[2].y, (int)pre[1].w, j0, j1);
+            if (lane < nb) nev += f4_count((int)pre[1].y, (int)pre[1].z, j0, j1);
             __syncthreads();
             if (lane < nb) {
 #pragma unroll
@@ -1201,57 +1228,63 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
                 for (int w = 0; w < W; w++) pre[w] = r[k * W + w];
             }
             for (int q = 0; q < nb; q++) {
-                const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(&srec[q * W]);
-                // the record's first 48 bytes as three broadcast reads; its control words
-                // as scalars, so that the per-record tests are scalar branches
-                const u32x4 h0 = srec[q * W], h1 = srec[q * W + 1], h2 = srec[q * W + 2];
-                const int jmin = __builtin_amdgcn_readfirstlane((int)h1.y);
-                const int jmax = __builtin_amdgcn_readfirstlane((int)h1.z);
-                const int kind = __builtin_amdgcn_readfirstlane((int)h1.w);
-                const int jqlo = __builtin_amdgcn_readfirstlane((int)h2.x);
-                const int jqhi = __builtin_amdgcn_readfirstlane((int)h2.y);
-                if (jmin <= jmax) {
-                    nev += f4_count(jmin, jmax, j0, j1);
-                    const float fl = (float)(j0 + lane - (int)h1.x);
-                    const bool cover = (jmin <= j0) & (jmax >= j1);
-                    if (cover & ((jqhi < j0) | (jqlo > j1)) & ((kind & F4_SHIFT) == 0)) {
-                        // the whole tile in the Lorentz zone: sil/(alfli2 + x^2) - base(x)
-                        const float d = __uint_as_float(h0.x), sil = __uint_as_float(h0.y);
-                        const float alfli2 = __uint_as_float(h0.z), b0 = __uint_as_float(h0.w);
-                        if (kind & F4_CO2) {
-                            const float b1 = __uint_as_float(h2.z);
+                const unsigned long long bit = 1ull << q;
+                const u32x4 *rq = &srec[q * W];
+                const u32x4 h0 = rq[0];
+                const float d = __uint_as_float(h0.x), sil = __uint_as_float(h0.y);
+                const float alfli2 = __uint_as_float(h0.z), b0 = __uint_as_float(h0.w);
+                const float fl = (float)(j0 + lane - (int)rq[1].x);
+                if (P.all & bit) {
+                    // the whole tile in the Lorentz zone: sil/(alfli2 + x^2) - base(x)
+                    if (P.co2 & bit) {
+                        const float b1 = __uint_as_float(rq[2].z);
 #pragma unroll
-                            for (int k = 0; k < F4K; k++) {
-                                const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d), xmsq = xm * xm;
-                                const float f = __fmaf_rn(sil, __builtin_amdgcn_rcpf(alfli2 + xmsq), -b0);
-                                acc[k] += (double)__fmaf_rn(-xmsq, b1, f);
-                            }
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < F4K; k++) {
-                                const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d);
-                                acc[k] += (double)__fmaf_rn(sil, __builtin_amdgcn_rcpf(__fmaf_rn(xm, xm, alfli2)), -b0);
-                            }
+                        for (int k = 0; k < F4K; k++) {
+                            const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d), xmsq = xm * xm;
+                            const float f = __fmaf_rn(sil, __builtin_amdgcn_rcpf(alfli2 + xmsq), -b0);
+                            acc[k] += (double)__fmaf_rn(-xmsq, b1, f);
                         }
-                    } else if (cover) {
-#pragma unroll
-                        for (int k = 0; k < F4K; k++)
-                            acc[k] += (double)f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
                     } else {
 #pragma unroll
                         for (int k = 0; k < F4K; k++) {
-                            const int j = j0 + lane + 64 * k;
-                            float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.d), l.dptrat);
-                            acc[k] += (double)((j >= jmin && j <= jmax) ? f : 0.f);
+                            const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d);
+                            acc[k] += (double)__fmaf_rn(sil, __builtin_amdgcn_rcpf(__fmaf_rn(xm, xm, alfli2)), -b0);
                         }
                     }
-                }
-                if (kind & F4_MIRROR) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
-                    const float fl = (float)(j0 + lane - l.jcm);
+                } else if (P.cover & bit) {
+                    const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(rq);
+#pragma unroll
+                    for (int k = 0; k < F4K; k++)
+                        acc[k] += (double)f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, d), l.dptrat);
+                } else if (P.lorp & bit) {
+                    // part of the tile, all of it in the Lorentz zone
+                    const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(rq);
+                    const int jmn = l.jmin, jmx = l.jmax;
+                    const float b1 = l.b1;
 #pragma unroll
                     for (int k = 0; k < F4K; k++) {
                         const int j = j0 + lane + 64 * k;
-                        float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, l.dm), -l.dptrat);
+                        const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d), xmsq = xm * xm;
+                        const float f = __fmaf_rn(-xmsq, b1, __fmaf_rn(sil, __builtin_amdgcn_rcpf(alfli2 + xmsq), -b0));
+                        acc[k] += (double)((j >= jmn && j <= jmx) ? f : 0.f);
+                    }
+                } else if (P.touch & bit) {
+                    const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(rq);
+                    const int jmn = l.jmin, jmx = l.jmax;
+#pragma unroll
+                    for (int k = 0; k < F4K; k++) {
+                        const int j = j0 + lane + 64 * k;
+                        float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, d), l.dptrat);
+                        acc[k] += (double)((j >= jmn && j <= jmx) ? f : 0.f);
+                    }
+                }
+                if (P.mirror & bit) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
+                    const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(rq);
+                    const float flm = (float)(j0 + lane - l.jcm);
+#pragma unroll
+                    for (int k = 0; k < F4K; k++) {
+                        const int j = j0 + lane + 64 * k;
+                        float f = f4_eval32(l, __fmaf_rn(flm + (float)(64 * k), dvr4f, l.dm), -l.dptrat);
                         acc[k] += (double)((j >= l.jminm && j <= l.jmaxm) ? f : 0.f);
                     }
                 }
@@ -1263,6 +1296,8 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
         const int j = j0 + lane + 64 * k;
         if (j <= LIMOUT4 + 2) R4[B.r4_off + j - 1] = (j <= B.nptr4) ? acc[k] : 0.;
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nev += __shfl_xor(nev, o);
     if (lane == 0 && nev) atomicAdd(work + 1, nev);
 }
 
