@@ -1042,7 +1042,11 @@ __device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dpt
     return __fmaf_rn(xm * dptrat, f, f);
 }
 
-constexpr int F4K = 4, F4PTS = 64 * F4K;
+// points per lane of the F4 gathers (tiles of 64 F4K points), measured best on the
+// 60-layer workload: 3 for the FP32 shapes, 2 for FP64
+constexpr int F4K32 = 3, F4K64 = 2;
+// F4 records summed in FP32 before each FP64 accumulation (FP32 gather)
+constexpr int F4CHUNK = 16;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int RAMP_CAP = 256;  // ramp pieces per panel held in LDS
 
@@ -1076,12 +1080,13 @@ __device__ __forceinline__ void f4_plan(F4Plan &P, bool live, int jmin, int jmax
     P.co2 = __ballot(live && (kind & F4_CO2) != 0);
 }
 
-// F4 gather, FP64: one wave per tile of F4PTS consecutive R4 points, F4K points per
+// F4 gather, FP64: one wave per tile of 64 F4K consecutive R4 points, F4K points per
 // lane (j = j0 + lane + 64 k), lines walked in the reference's order so each point's
 // sum has the reference's summation order.  Records pass through LDS 64 at a time
 // (one coalesced load per lane, the next batch in flight while this one is summed)
 // and are read back as broadcasts of plain 16-byte vectors (so that no copy of a
 // record is made in private memory).
+template <int F4K>
 __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
                                                  const F4Rec64 *__restrict__ recs, double *R4,
                                                  unsigned long long *work) {
@@ -1089,7 +1094,7 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
     const R4Blk &B = blks[t.x];
     const Job &J = jobs[B.layer];
     const int lane = threadIdx.x;
-    const int j0 = t.y * F4PTS + 1, j1 = j0 + F4PTS - 1;
+    const int j0 = t.y * 64 * F4K + 1, j1 = j0 + 64 * F4K - 1;
     double acc[F4K], xj[F4K];
 #pragma unroll
     for (int k = 0; k < F4K; k++) {
@@ -1185,6 +1190,7 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
 
 // F4 gather, FP32 line shapes (offsets from the FP64 per-record anchor) with FP64 sums
 // per point; the same walk as k_f4conv64.
+template <int F4K>
 __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
                                                  const F4Rec32 *__restrict__ recs, double *R4,
                                                  unsigned long long *work) {
@@ -1192,7 +1198,7 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
     const R4Blk &B = blks[t.x];
     const Job &J = jobs[B.layer];
     const int lane = threadIdx.x;
-    const int j0 = t.y * F4PTS + 1, j1 = j0 + F4PTS - 1;
+    const int j0 = t.y * 64 * F4K + 1, j1 = j0 + 64 * F4K - 1;
     const float dvr4f = (float)J.dvr4;
     double acc[F4K];
 #pragma unroll
@@ -1227,7 +1233,13 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
 #pragma unroll
                 for (int w = 0; w < W; w++) pre[w] = r[k * W + w];
             }
-            for (int q = 0; q < nb; q++) {
+            for (int q0 = 0; q0 < nb; q0 += F4CHUNK) {
+            // FP32 partial sums over F4CHUNK records, then one FP64 accumulation
+            float a32[F4K];
+#pragma unroll
+            for (int k = 0; k < F4K; k++) a32[k] = 0.f;
+            const int qe = q0 + F4CHUNK < nb ? q0 + F4CHUNK : nb;
+            for (int q = q0; q < qe; q++) {
                 const unsigned long long bit = 1ull << q;
                 const u32x4 *rq = &srec[q * W];
                 const u32x4 h0 = rq[0];
@@ -1242,20 +1254,20 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
                         for (int k = 0; k < F4K; k++) {
                             const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d), xmsq = xm * xm;
                             const float f = __fmaf_rn(sil, __builtin_amdgcn_rcpf(alfli2 + xmsq), -b0);
-                            acc[k] += (double)__fmaf_rn(-xmsq, b1, f);
+                            a32[k] += __fmaf_rn(-xmsq, b1, f);
                         }
                     } else {
 #pragma unroll
                         for (int k = 0; k < F4K; k++) {
                             const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d);
-                            acc[k] += (double)__fmaf_rn(sil, __builtin_amdgcn_rcpf(__fmaf_rn(xm, xm, alfli2)), -b0);
+                            a32[k] += __fmaf_rn(sil, __builtin_amdgcn_rcpf(__fmaf_rn(xm, xm, alfli2)), -b0);
                         }
                     }
                 } else if (P.cover & bit) {
                     const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(rq);
 #pragma unroll
                     for (int k = 0; k < F4K; k++)
-                        acc[k] += (double)f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, d), l.dptrat);
+                        a32[k] += f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, d), l.dptrat);
                 } else if (P.lorp & bit) {
                     // part of the tile, all of it in the Lorentz zone
                     const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(rq);
@@ -1266,7 +1278,7 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
                         const int j = j0 + lane + 64 * k;
                         const float xm = __fmaf_rn(fl + (float)(64 * k), dvr4f, d), xmsq = xm * xm;
                         const float f = __fmaf_rn(-xmsq, b1, __fmaf_rn(sil, __builtin_amdgcn_rcpf(alfli2 + xmsq), -b0));
-                        acc[k] += (double)((j >= jmn && j <= jmx) ? f : 0.f);
+                        a32[k] += ((j >= jmn && j <= jmx) ? f : 0.f);
                     }
                 } else if (P.touch & bit) {
                     const F4Rec32 &l = *reinterpret_cast<const F4Rec32 *>(rq);
@@ -1275,7 +1287,7 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
                     for (int k = 0; k < F4K; k++) {
                         const int j = j0 + lane + 64 * k;
                         float f = f4_eval32(l, __fmaf_rn(fl + (float)(64 * k), dvr4f, d), l.dptrat);
-                        acc[k] += (double)((j >= jmn && j <= jmx) ? f : 0.f);
+                        a32[k] += ((j >= jmn && j <= jmx) ? f : 0.f);
                     }
                 }
                 if (P.mirror & bit) {  // Van Vleck-Weisskopf mirror (lines below 25 cm-1)
@@ -1285,9 +1297,12 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
                     for (int k = 0; k < F4K; k++) {
                         const int j = j0 + lane + 64 * k;
                         float f = f4_eval32(l, __fmaf_rn(flm + (float)(64 * k), dvr4f, l.dm), -l.dptrat);
-                        acc[k] += (double)((j >= l.jminm && j <= l.jmaxm) ? f : 0.f);
+                        a32[k] += ((j >= l.jminm && j <= l.jmaxm) ? f : 0.f);
                     }
                 }
+            }
+#pragma unroll
+            for (int k = 0; k < F4K; k++) acc[k] += (double)a32[k];
             }
         }
     }
@@ -2094,7 +2109,8 @@ static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, 
             B.tot_r4 += LIMOUT4 + 2;
             int bi = (int)B.blks.size();
             B.blks.push_back(b);
-            for (int t = 0; t < (LIMOUT4 + 2 + F4PTS - 1) / F4PTS; t++) B.tiles.push_back(make_int2(bi, t));
+            const int tile = 64 * (precision == CLBLM_FP64 ? F4K64 : F4K32);
+            for (int t = 0; t < (LIMOUT4 + 2 + tile - 1) / tile; t++) B.tiles.push_back(make_int2(bi, t));
         }
         B.max_line = std::max(B.max_line, J.line_n);
         B.max_f4 = std::max(B.max_f4, J.f4_n);
@@ -2293,10 +2309,10 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     HIPCHK(hipEventRecord(S.ev[5], st));
     const unsigned f4grid = (unsigned)B.tiles.size();
     if (precision == CLBLM_FP64)
-        hipLaunchKernelGGL(k_f4conv64, dim3(f4grid), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
+        hipLaunchKernelGGL(k_f4conv64<F4K64>, dim3(f4grid), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec64>(base, o_rec), d_R4, C->d_work);
     else
-        hipLaunchKernelGGL(k_f4conv32, dim3(f4grid), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
+        hipLaunchKernelGGL(k_f4conv32<F4K32>, dim3(f4grid), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
                            at<F4Rec32>(base, o_rec), d_R4, C->d_work);
     HIPCHK(hipEventRecord(S.ev[6], st));
     if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, d_R4);
