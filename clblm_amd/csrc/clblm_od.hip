@@ -57,6 +57,9 @@ __constant__ int c_maxiso[NMOL];
 struct Tables {
     const double *f1, *f2, *f3, *fg, *xver;   // 1-based, NFMX+1 slots
     const float *f1s, *f2s, *f3s, *fgs, *xvers;
+    // {F1, FG, XVER, 0} per slot: the three shapes of one R1 point in one load
+    const double4 *g1;
+    const float4 *g1s;
 };
 
 __device__ __forceinline__ int ftrunc(double x) { return (int)x; }
@@ -401,7 +404,10 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
             int iz1 = ftrunc(fabs(zf1) + 1.5);
             if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], s3 * T.f3[iz3]);
             if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], s2 * T.f2[iz2]);
-            if (j1 >= 1) atomicAdd(&a1[j1], s1 * T.f1[iz1] + sd * T.fg[iz1] + sv * T.xver[iz1]);
+            if (j1 >= 1) {
+                const double4 g = T.g1[iz1];
+                atomicAdd(&a1[j1], s1 * g.x + sd * g.y + sv * g.z);
+            }
         }
     }
     for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
@@ -425,7 +431,6 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
     const Job &J = jobs[P.layer];
     const int max1 = J.max1, max2 = J.max2, max3 = J.max3, stride = J.stride;
     double *a1 = sm - 1, *a2 = sm + max1 - 1, *a3 = sm + max1 + max2 - 1;
-    const float *__restrict__ t1 = T.f1s, *__restrict__ tg = T.fgs, *__restrict__ tv = T.xvers;
     const float *__restrict__ t2 = T.f2s, *__restrict__ t3 = T.f3s;
     for (int t = threadIdx.x; t < stride; t += blockDim.x) sm[t] = 0.;
     __syncthreads();
@@ -470,8 +475,10 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
             int iz1 = ftrunc(fabs(zf1) + 1.5);
             if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(s3 * t3[iz3]));
             if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(s2 * t2[iz2]));
-            if (j1 >= 1)
-                atomicAdd(&a1[j1], (double)__fmaf_rn(sv, tv[iz1], __fmaf_rn(sd, tg[iz1], s1 * t1[iz1])));
+            if (j1 >= 1) {
+                const float4 g = T.g1s[iz1];
+                atomicAdd(&a1[j1], (double)__fmaf_rn(sv, g.z, __fmaf_rn(sd, g.y, s1 * g.x)));
+            }
         }
     }
     for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
@@ -1785,6 +1792,8 @@ struct clblm_ctx {
     hipStream_t stream = nullptr;
     double *tab = nullptr;   // F1,F2,F3,FG,XVER consecutive, NFMX+1 each
     float *tabs = nullptr;
+    double4 *tabg = nullptr;
+    float4 *tabgs = nullptr;
     std::vector<LineListH *> lists;
     DevArena arena;
     int *d_err = nullptr;
@@ -1883,6 +1892,18 @@ static int ctx_tables(clblm_ctx *c) {
     HIPCHK(hipMalloc(&c->tabs, alls.size() * sizeof(float)));
     HIPCHK(hipMemcpy(c->tab, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->tabs, alls.data(), alls.size() * sizeof(float), hipMemcpyHostToDevice));
+    {
+        std::vector<double4> g(NFMX + 1);
+        std::vector<float4> gs(NFMX + 1);
+        for (int i = 0; i <= NFMX; i++) {
+            g[i] = make_double4(f1[i], fg[i], xv[i], 0.);
+            gs[i] = make_float4((float)f1[i], (float)fg[i], (float)xv[i], 0.f);
+        }
+        HIPCHK(hipMalloc(&c->tabg, g.size() * sizeof(double4)));
+        HIPCHK(hipMalloc(&c->tabgs, gs.size() * sizeof(float4)));
+        HIPCHK(hipMemcpy(c->tabg, g.data(), g.size() * sizeof(double4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->tabgs, gs.data(), gs.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_avrat), CLBLM_AVRAT, sizeof(CLBLM_AVRAT)));
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cgauss), CLBLM_CGAUSS, sizeof(CLBLM_CGAUSS)));
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cf1), CLBLM_CF1, sizeof(CLBLM_CF1)));
@@ -1898,6 +1919,8 @@ static Tables ctx_tabview(clblm_ctx *c) {
     const size_t s = NFMX + 1;
     t.f1 = c->tab; t.f2 = c->tab + s; t.f3 = c->tab + 2 * s; t.fg = c->tab + 3 * s; t.xver = c->tab + 4 * s;
     t.f1s = c->tabs; t.f2s = c->tabs + s; t.f3s = c->tabs + 2 * s; t.fgs = c->tabs + 3 * s; t.xvers = c->tabs + 4 * s;
+    t.g1 = c->tabg;
+    t.g1s = c->tabgs;
     return t;
 }
 
@@ -2403,6 +2426,8 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     if (c->arena.p) (void)hipFree(c->arena.p);
     (void)hipFree(c->tab);
     (void)hipFree(c->tabs);
+    (void)hipFree(c->tabg);
+    (void)hipFree(c->tabgs);
     (void)hipFree(c->d_err);
     (void)hipFree(c->d_work);
     if (c->rt_state) (void)hipFree(c->rt_state);
