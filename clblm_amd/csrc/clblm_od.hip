@@ -354,6 +354,8 @@ __global__ void k_ranges(const Job *jobs, Panel *panels, int64_t npanel_tot, Lin
 // FP64 atomics and the panel's arrays go to HBM.  Local index 1..MAX is kept; lower
 // indices are never read by the reference's panel logic.
 // ------------------------------------------------------------------------------------
+constexpr int CONV_U = 4;   // R1 points per step of the pass-1 walk
+
 __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *panels, LinePrep lp,
                                                 Tables T, double *A, unsigned long long *work) {
     extern __shared__ double sm[];
@@ -394,7 +396,29 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
         double zf2 = ((double)(jmin2 - 2) - zint * conf2) * zslope;
         double zf3 = ((double)(jmin3 - 2) - zint * conf3) * zslope;
         nev += (unsigned long long)(jmax1 - jmin1 + 1);
-        for (int j1 = jmin1; j1 <= jmax1; j1++) {
+        // CONV_U R1 points at a time (see k_conv32)
+        int j1 = jmin1;
+        for (; j1 + (CONV_U - 1) <= jmax1; j1 += CONV_U) {
+            double4 g[CONV_U];
+            double v2[CONV_U], v3[CONV_U];
+#pragma unroll
+            for (int u = 0; u < CONV_U; u++) {
+                zf3 = zf3 + zslope;
+                zf2 = zf2 + zslope;
+                zf1 = zf1 + zslope;
+                g[u] = T.g1[min(ftrunc(fabs(zf1) + 1.5), NFMX)];
+                v2[u] = T.f2[min(ftrunc(fabs(zf2) + 1.5), NFMX)];
+                v3[u] = T.f3[min(ftrunc(fabs(zf3) + 1.5), NFMX)];
+            }
+#pragma unroll
+            for (int u = 0; u < CONV_U; u++) {
+                const int jj = j1 + u, j2 = jj - j2shft, j3 = jj - j3shft;
+                if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], s3 * v3[u]);
+                if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], s2 * v2[u]);
+                if (jj >= 1) atomicAdd(&a1[jj], s1 * g[u].x + sd * g[u].y + sv * g[u].z);
+            }
+        }
+        for (; j1 <= jmax1; j1++) {
             int j2 = j1 - j2shft, j3 = j1 - j3shft;
             zf3 = zf3 + zslope;
             zf2 = zf2 + zslope;
@@ -465,7 +489,31 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
         double zf2 = ((double)(jmin2 - 2) - zint * conf2) * zslope;
         double zf3 = ((double)(jmin3 - 2) - zint * conf3) * zslope;
         nev += (unsigned long long)(jmax1 - jmin1 + 1);
-        for (int j1 = jmin1; j1 <= jmax1; j1++) {
+        // CONV_U R1 points at a time, so that their table loads are in flight together;
+        // the ZF arguments are still accumulated one step at a time.  Indices of points
+        // outside the R2/R3 buffers are clamped into the tables (their values unused).
+        int j1 = jmin1;
+        for (; j1 + (CONV_U - 1) <= jmax1; j1 += CONV_U) {
+            float4 g[CONV_U];
+            float v2[CONV_U], v3[CONV_U];
+#pragma unroll
+            for (int u = 0; u < CONV_U; u++) {
+                zf3 = zf3 + zslope;
+                zf2 = zf2 + zslope;
+                zf1 = zf1 + zslope;
+                g[u] = T.g1s[min(ftrunc(fabs(zf1) + 1.5), NFMX)];
+                v2[u] = t2[min(ftrunc(fabs(zf2) + 1.5), NFMX)];
+                v3[u] = t3[min(ftrunc(fabs(zf3) + 1.5), NFMX)];
+            }
+#pragma unroll
+            for (int u = 0; u < CONV_U; u++) {
+                const int jj = j1 + u, j2 = jj - j2shft, j3 = jj - j3shft;
+                if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(s3 * v3[u]));
+                if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(s2 * v2[u]));
+                if (jj >= 1) atomicAdd(&a1[jj], (double)__fmaf_rn(sv, g[u].z, __fmaf_rn(sd, g[u].y, s1 * g[u].x)));
+            }
+        }
+        for (; j1 <= jmax1; j1++) {
             int j2 = j1 - j2shft, j3 = j1 - j3shft;
             zf3 = zf3 + zslope;
             zf2 = zf2 + zslope;
