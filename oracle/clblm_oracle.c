@@ -1658,27 +1658,65 @@ done_interp:
  * Hanning) of half width hwhm, limits v1out..v2out.  The re-gridded spectrum (when
  * pre_interp) starts at the grid point returned in v1z with dv = dvn.  Returns 0,
  * or -1 with a message. */
-int orc_fftscn_plan(double v1c, double dv, int64_t n, double v1out, double v2out, int funct_id, double hwhm,
-                    orc_scan_plan *p) {
+/* clblm_FFTSCN_struct's set-up (clblm_FFT.f90:434-614): the scanning function (JFN =
+ * funct_id - 1; 101..113 and -2 are the externally pre-scanned and deconvolution-only
+ * modes), the boxcar pre-scan and deconvolution switches, the scan window V1Z..V2Z
+ * and the optional re-gridding onto DVN.  Brault and Kaiser-Bessel (JFN 11, 12) stop
+ * in the reference (they need an optical path difference, :480-489); Kiruna (JFN 13)
+ * is not restated (its multi-record scanning function sets the next block's start
+ * from a variable that is never assigned, Scntrn :2136). */
+int orc_fftscn_plan2(double v1c, double dv, int64_t n, double v1out, double v2out, int funct_id, double hwhm,
+                     double fill, double pre_box_hw, int deconv_pre_box, orc_scan_plan *p) {
     memset(p, 0, sizeof *p);
+    if (fill < 0 || fill > 1) return fail("FILL value is out of range");
+    if (!((funct_id >= 1 && funct_id <= 14) || (funct_id >= 102 && funct_id <= 114) || funct_id == -1))
+        return fail("functID must be in [1,14], [102,114] or -1");
     int JFN = funct_id - 1;
-    if (JFN < 1 || JFN > 7) return fail("scan function not supported (2..8 only)");
+    if (JFN == 0) {
+        p->prebox = 1;
+        p->deconv = 0;
+        p->deconv_a = hwhm;   /* boxcar half width until the boxcar factor is known */
+    } else if (JFN >= 1 && JFN <= 13) {
+        if (pre_box_hw > 0.) {
+            p->prebox = 1;
+            p->deconv = deconv_pre_box != 0;
+            p->deconv_a = pre_box_hw;
+        }
+    } else if (JFN >= 101 && JFN <= 113) {
+        JFN -= 100;
+        p->deconv = 1;
+        p->deconv_a = pre_box_hw;
+    } else if (JFN == -2) {
+        p->deconv_only = 1;
+        JFN = 0;
+        p->deconv = 1;
+        p->deconv_a = pre_box_hw;
+    }
+    if (JFN == 11 || JFN == 12) return fail("this scanning function needs the interferometer optical path difference");
+    if (JFN == 13) return fail("the Kiruna scanning function is not supported");
     p->jfn = JFN;
     p->hwhm = hwhm;
-    p->a = SCAN_C[JFN] * hwhm;
+    p->a = JFN > 0 ? SCAN_C[JFN] * hwhm : 0.;
     double V2C = v1c + dv * (double)(n - 1);
     if (v1out >= v2out) return fail("v1out >= v2out");
     if (v1out >= V2C || v2out <= v1c) return fail("v1out..v2out outside the data");
-    double V1Z = v1out - hwhm * SCAN_CLIMIT[JFN], V2Z = v2out + hwhm * SCAN_CLIMIT[JFN];
-    V1Z = v1c + (double)(int)((V1Z - v1c) / dv) * dv;
-    V2Z = v1c + (double)(int)((V2Z - v1c) / dv + 1.01) * dv;
+    double V1Z, V2Z;
+    if (p->deconv_only) {
+        V1Z = v1c;
+        V2Z = V2C;
+    } else {
+        V1Z = v1out - hwhm * SCAN_CLIMIT[JFN];
+        V2Z = v2out + hwhm * SCAN_CLIMIT[JFN];
+        V1Z = v1c + (double)(int)((V1Z - v1c) / dv) * dv;
+        V2Z = v1c + (double)(int)((V2Z - v1c) / dv + 1.01) * dv;
+    }
     if (V1Z < v1c) V1Z = v1c;
     if (V2Z > V2C) V2Z = V2C;
     if ((V2Z - V1Z) / hwhm < SCAN_CLIMIT[JFN]) return fail("insufficient data for the scan");
     double DVN = 2. * hwhm / 12;
-    double DV = dv, V1P = v1c;
+    double DV = dv;
     p->dvn = DVN;
-    if (DV > DVN) {
+    if (DV > DVN && JFN != 0) {
         p->pre_interp = 1;
         /* Intpdr(spect, V1Z, V2Z, DVN): the re-gridded spectrum starts at the clamped
          * V1Z with DVN; its end follows from the point count (computed by the caller) */
@@ -1690,44 +1728,73 @@ int orc_fftscn_plan(double v1c, double dv, int64_t n, double v1out, double v2out
         }
         V1Z = a;
         DV = DVN;
-        V1P = a;
-        p->v2z = V2Z; /* the caller replaces it with the re-gridded spectrum's end */
     }
     p->v1z = V1Z;
+    p->v2z = V2Z;
     p->dv = DV;
-    if (!p->pre_interp) p->v2z = V2Z;
-    (void)V1P;
     return 0;
 }
 
+int orc_fftscn_plan(double v1c, double dv, int64_t n, double v1out, double v2out, int funct_id, double hwhm,
+                    orc_scan_plan *p) {
+    if (funct_id < 2 || funct_id > 8) return fail("scan function not supported (2..8 only)");
+    return orc_fftscn_plan2(v1c, dv, n, v1out, v2out, funct_id, hwhm, 0., -1., 0, p);
+}
+
+static int64_t scan_records(int64_t ntot) {  /* LoadSpect/Boxcar record count (:1189-1197, :890-910) */
+    double ARDATA = (double)ntot / (double)SCAN_LPTSMX;
+    int64_t LRDATA = (ARDATA - (double)(int)ARDATA == 0.0) ? (int64_t)(int)ARDATA : (int64_t)(int)(ARDATA + 1);
+    double POWER = log((double)LRDATA) / log(2.0);
+    return (POWER - (double)(int)POWER == 0) ? ((int64_t)1 << (int)(POWER + 2)) : ((int64_t)1 << ((int)POWER + 3));
+}
+
 /* Second half of the bookkeeping once the scan-grid spectrum is known (v1p, its
- * n points): LoadSpect's count and record rounding, the transform length, and
- * wrtSpect's window. */
+ * n points): the choice of apodization (IVX, :616-634), LoadSpect's count and
+ * record rounding, the boxcar pre-scan's new grid (:689-706), the transform length,
+ * and wrtSpect's window (:776-796). */
 int orc_fftscn_finish(orc_scan_plan *p, double v1p, int64_t np, double v1out, double v2out) {
-    const double DV = p->dv;
+    double DV = p->dv;
     double V1Z = p->v1z, V2Z = p->v2z;
     if (p->pre_interp) {
         V1Z = v1p;
         V2Z = v1p + DV * (double)(np - 1);
     }
-    if ((V2Z - V1Z) / p->hwhm > SCAN_CRATIO[p->jfn]) {
-        /* IVX = 1: analytic apodization */
-    } else {
-        return fail("scan range too short for the analytic apodization (IVX = -1 not supported)");
-    }
+    if (p->jfn == 0) p->ivx = 0;
+    else p->ivx = ((V2Z - V1Z) / p->hwhm > SCAN_CRATIO[p->jfn]) ? 1 : -1;
+    if (p->ivx == -1 && p->jfn >= 8)
+        return fail("Norton-Beer scanning functions have no spectral-domain form (range too short)");
     if (v1p > V1Z) return fail("LoadSpect: first data frequency above V1Z");
     int64_t LTOTAL = (int64_t)(int)((V2Z - V1Z) / DV + 1.1);
     V2Z = V1Z + DV * (double)(LTOTAL - 1);
-    double ARDATA = (double)LTOTAL / (double)SCAN_LPTSMX;
-    int64_t LRDATA = (ARDATA - (double)(int)ARDATA == 0.0) ? (int64_t)(int)ARDATA : (int64_t)(int)(ARDATA + 1);
-    double POWER = log((double)LRDATA) / log(2.0);
-    int64_t LREC = (POWER - (double)(int)POWER == 0) ? ((int64_t)1 << (int)(POWER + 2))
-                                                      : ((int64_t)1 << ((int)POWER + 3));
+    int64_t LREC = scan_records(LTOTAL);
     int64_t N1L = (int64_t)(int)((V1Z - v1p) / DV + 1.00001);
     if (N1L + LTOTAL - 1 > np) return fail("LoadSpect: ran out of data");
-    p->v1z = V1Z; p->v2z = V2Z;
-    p->ltotal = LTOTAL;
     p->n1load = N1L;
+    p->ltotal0 = LTOTAL;
+    p->lrec0 = LREC;
+    p->v1z0 = V1Z;
+    p->dv0 = DV;
+    if (p->prebox) {
+        const double boxhw = p->deconv_a;
+        int M = (int)(2.0 * boxhw / DV);
+        if (M >= 2) {
+            V1Z = V1Z + DV * (double)(M - 1) / 2.0;
+            int64_t KTOTAL = LTOTAL / M;
+            if (KTOTAL < 1) return fail("boxcar wider than the scan range");
+            DV = (double)M * DV;
+            V2Z = V1Z + DV * (double)(KTOTAL - 1);
+            LREC = scan_records(KTOTAL);
+            LTOTAL = KTOTAL;
+            p->m = M;
+            p->deconv_a = DV;
+        } else {
+            p->prebox = 0;
+            p->deconv = 0;
+        }
+    }
+    p->fft = !(p->jfn == 0 && !p->deconv);
+    p->v1z = V1Z; p->v2z = V2Z; p->dv = DV;
+    p->ltotal = LTOTAL;
     p->lrec = LREC;
     p->nfft = LREC * SCAN_LPTSMX;
     int64_t N1 = (int64_t)(int)((v1out - V1Z) / DV);
@@ -1744,6 +1811,90 @@ int orc_fftscn_finish(orc_scan_plan *p, double v1p, int64_t np, double v1out, do
     p->ntotal = N2 - N1 + 1;
     p->v1s = V1S;
     return 0;
+}
+
+/* Boxcar (clblm_FFT.f90:867-990): means of m consecutive loaded points, sequential
+ * sums; the transform buffer (nfft points) is filled with `fill` beyond them. */
+void orc_scan_boxcar(const orc_scan_plan *p, const double *loaded, double fill, double *out) {
+    for (int64_t k = 0; k < p->nfft; k++) {
+        if (k < p->ltotal) {
+            double sum = 0.0;
+            for (int mm = 0; mm < p->m; mm++) sum = sum + loaded[k * p->m + mm];
+            out[k] = sum / (double)p->m;
+        } else {
+            out[k] = fill;
+        }
+    }
+}
+
+/* Scnfnt's scanning functions (IAPSC = -1) for JFN 1..7 at X >= 0. */
+static double scan_sfn(int jfn, double X, double A) {
+    const double Pi = 2.0 * asin(1.0);
+    switch (jfn) {
+    case 1: return (X > A) ? 0.0 : (1.0 - X / A);
+    case 2: { double t = 0.5 * ((X / A) * (X / A)); return exp(-(t < 34. ? t : 34.)); }
+    case 3: if (X != 0) { double s = sin(Pi * X / A) / (Pi * X / A); return s * s; } return 1.0;
+    case 4: return (X != 0) ? sin(2.0 * Pi * X / A) / (2.0 * Pi * X / A) : 1.0;
+    case 5: {
+        double U = 2.0 * Pi * X / A;
+        if (U >= 0.5) {
+            double U2 = U * U, U5 = U2 * U2 * U;
+            return 15. * ((3.0 - U2) * sin(U) - 3.0 * U * cos(U)) / U5;
+        }
+        return (1.0 - 15.0 / 210 * (U * U));
+    }
+    default: {
+        const double c = jfn == 6 ? 0.428752 : 0.5;
+        double U = 2.0 * Pi * X / A;
+        if (U != 0 && fabs(U) != Pi) return (sin(U) / U + c * (sin(U + Pi) / (U + Pi) + sin(U - Pi) / (U - Pi)));
+        if (U == 0) return 1.0;
+        return c;
+    }
+    }
+}
+
+/* Scntrn with IVX = -1 (clblm_FFT.f90:2113-2175): the scanning function on the
+ * transform grid, block by block, for an even number of records: block J holds
+ * positive frequencies from V(J) = V(J-1) + LPTS DV, and block LREC+1-J their mirror
+ * images (first element the value at V(J+1)).  The caller transforms it and divides
+ * by its first (DC) coefficient. */
+void orc_scan_sfn(const orc_scan_plan *p, double *f) {
+    const int LPTS = SCAN_LPTSMX;
+    const double DV = p->dv;
+    double *fv = malloc(sizeof(double) * (LPTS + 1));
+    double V = 0.0;
+    for (int64_t J = 1; J <= p->lrec / 2; J++) {
+        const double VSAVE = V;
+        for (int L = 1; L <= LPTS; L++) {
+            double X = (L == 1) ? VSAVE : VSAVE + (double)(L - 1) * DV;
+            fv[L] = scan_sfn(p->jfn, X, p->a);
+        }
+        V = VSAVE + (double)LPTS * DV;
+        double *pos = f + (J - 1) * LPTS, *neg = f + (p->lrec - J) * LPTS;
+        for (int L = 1; L <= LPTS; L++) pos[L - 1] = fv[L];
+        neg[0] = scan_sfn(p->jfn, V, p->a);
+        for (int q = 1; q < LPTS; q++) neg[q] = fv[LPTS + 1 - q];
+    }
+    free(fv);
+}
+
+/* The deconvolution factor (Scnfnt function -1, :1920-1946): Z/sin(Z), Z = 2 pi X A,
+ * at the positions of the IVX = 1 apodization, for the boxcar of width parameter A. */
+void orc_fft_deconv(const orc_scan_plan *p, double *h) {
+    const int LPTS = SCAN_LPTSMX;
+    const double PI = 2.0 * asin(1.0);
+    const double A = p->deconv_a;
+    const double DX = 1.0 / ((double)(LPTS * p->lrec) * p->dv);
+    double X = 0.0;
+    for (int64_t J = 0; J < p->lrec; J++) {
+        const double X0 = X;
+        double *f = h + J * (LPTS / 2);
+        for (int L = 1; L <= LPTS - 1; L += 2) {
+            double Z = 2.0 * PI * X * A;
+            f[(L - 1) / 2] = (Z != 0) ? Z / sin(Z) : 1.0;
+            X = X0 + (double)(L + 1) * DX / 2.;
+        }
+    }
 }
 
 /* Scntrn with IVX = 1 (clblm_FFT.f90:2217-2229) and Scnfnt's apodization branch:
@@ -1768,7 +1919,17 @@ void orc_fft_apod(const orc_scan_plan *p, double *h) {
             case 4: v = (X < 1.0 / A) ? 1.0 : 0.0; break;
             case 5: v = (X < 1 / A) ? (1.0 - (X * A) * (X * A)) * (1.0 - (X * A) * (X * A)) : 0.0; break;
             case 6: v = (X < 1 / A) ? 0.53835685 * (1.0 + 0.857504 * cos(PI * X * A)) : 0.0; break;
-            default: v = (X < 1. / A) ? 0.5 * (1.0 + cos(PI * X * A)) : 0.0; break;
+            case 7: v = (X < 1. / A) ? 0.5 * (1.0 + cos(PI * X * A)) : 0.0; break;
+            default: {  /* Norton-Beer weak, medium, strong (:1758-1798) */
+                double U = 1.0 - (X * A) * (X * A);
+                if (X < 1. / A) {
+                    if (p->jfn == 8) v = 0.384093 - 0.087577 * U + 0.703484 * (U * U);
+                    else if (p->jfn == 9) v = 0.152442 - 0.136176 * U + 0.983734 * (U * U);
+                    else v = 0.045355 + 0.554883 * (U * U) + 0.399782 * ((U * U) * (U * U));
+                } else {
+                    v = 0.0;
+                }
+            } break;
             }
             f[(L - 1) / 2] = v;
             X = X0 + (double)(L + 1) * DX / 2.;

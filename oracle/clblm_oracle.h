@@ -103,9 +103,22 @@ typedef struct {
     int64_t lrec, nfft;        /* records of LPTSMX and transform length */
     int64_t n1, ntotal;        /* wrtSpect: first point (1-based) and count on the scan grid */
     double v1s;
+    int ivx;                   /* 1 analytic apodization, -1 transform of the scanning function, 0 none */
+    int fft;                   /* 0: no transform (boxcar scan without deconvolution) */
+    int prebox, m;             /* boxcar pre-scan over m points (m >= 2) */
+    int64_t ltotal0, lrec0;    /* points and records as loaded, before the boxcar */
+    double v1z0, dv0;          /* scan grid as loaded, before the boxcar */
+    int deconv;                /* divide by the boxcar's transform, width parameter deconv_a */
+    double deconv_a;
+    int deconv_only;
 } orc_scan_plan;
 int64_t orc_scan_interp(const double *x, int64_t n, double v1, double dv, double vv1, double vv2, double dvo,
                         double *out, int64_t cap, double *v1res);
+int orc_fftscn_plan2(double v1c, double dv, int64_t n, double v1out, double v2out, int funct_id, double hwhm,
+                     double fill, double pre_box_hw, int deconv_pre_box, orc_scan_plan *p);
+void orc_scan_boxcar(const orc_scan_plan *p, const double *loaded, double fill, double *out);
+void orc_fft_deconv(const orc_scan_plan *p, double *h);
+void orc_scan_sfn(const orc_scan_plan *p, double *f);
 int orc_fftscn_plan(double v1c, double dv, int64_t n, double v1out, double v2out, int funct_id, double hwhm,
                     orc_scan_plan *p);
 int orc_fftscn_finish(orc_scan_plan *p, double v1p, int64_t np, double v1out, double v2out);

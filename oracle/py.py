@@ -128,7 +128,11 @@ class ScanPlan(ctypes.Structure):
                 ("pre_interp", ctypes.c_int), ("dvn", ctypes.c_double), ("v1z", ctypes.c_double),
                 ("v2z", ctypes.c_double), ("dv", ctypes.c_double), ("n1load", ctypes.c_int64),
                 ("ltotal", ctypes.c_int64), ("lrec", ctypes.c_int64), ("nfft", ctypes.c_int64),
-                ("n1", ctypes.c_int64), ("ntotal", ctypes.c_int64), ("v1s", ctypes.c_double)]
+                ("n1", ctypes.c_int64), ("ntotal", ctypes.c_int64), ("v1s", ctypes.c_double),
+                ("ivx", ctypes.c_int), ("fft", ctypes.c_int), ("prebox", ctypes.c_int), ("m", ctypes.c_int),
+                ("ltotal0", ctypes.c_int64), ("lrec0", ctypes.c_int64), ("v1z0", ctypes.c_double),
+                ("dv0", ctypes.c_double), ("deconv", ctypes.c_int), ("deconv_a", ctypes.c_double),
+                ("deconv_only", ctypes.c_int)]
 
 
 def scan_interp(x, v1, dv, vv1, vv2, dvo):
@@ -145,18 +149,25 @@ def scan_interp(x, v1, dv, vv1, vv2, dvo):
     return out[:n], v1r.value
 
 
-def fft_scan(x, v1, dv, v1out, v2out, dvout, funct_id=5, hwhm=0.6034, fill=0.0):
-    """clblm_FFTSCN_struct with an analytic apodization: returns (values, v1, dv).
-    The transforms are numpy's; everything around them is the C restatement."""
+def fft_scan(x, v1, dv, v1out, v2out, dvout, funct_id=5, hwhm=0.6034, fill=0.0, pre_box_hw=-1.0,
+             deconv_pre_box=False):
+    """clblm_FFTSCN_struct: returns (values, v1, dv).  The transforms are numpy's;
+    everything around them is the C restatement: the boxcar pre-scan, the analytic
+    apodization (IVX = 1) or the transform of the scanning function (IVX = -1), the
+    boxcar deconvolution, the window and the output re-gridding."""
     L = lib()
-    L.orc_fftscn_plan.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_double,
-                                  ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ScanPlan)]
+    L.orc_fftscn_plan2.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.POINTER(ScanPlan)]
     L.orc_fftscn_finish.argtypes = [ctypes.POINTER(ScanPlan), ctypes.c_double, ctypes.c_int64, ctypes.c_double,
                                     ctypes.c_double]
-    L.orc_fft_apod.argtypes = [ctypes.POINTER(ScanPlan), ctypes.c_void_p]
+    for fn in (L.orc_fft_apod, L.orc_fft_deconv, L.orc_scan_sfn):
+        fn.argtypes = [ctypes.POINTER(ScanPlan), ctypes.c_void_p]
+    L.orc_scan_boxcar.argtypes = [ctypes.POINTER(ScanPlan), ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
     x = np.ascontiguousarray(x, np.float64)
     p = ScanPlan()
-    if L.orc_fftscn_plan(v1, dv, x.size, v1out, v2out, int(funct_id), hwhm, ctypes.byref(p)):
+    if L.orc_fftscn_plan2(v1, dv, x.size, v1out, v2out, int(funct_id), hwhm, fill, pre_box_hw,
+                          int(bool(deconv_pre_box)), ctypes.byref(p)):
         raise RuntimeError(L.orc_error().decode())
     if p.pre_interp:
         xs, v1p = scan_interp(x, v1, dv, p.v1z, p.v2z, p.dvn)
@@ -164,12 +175,36 @@ def fft_scan(x, v1, dv, v1out, v2out, dvout, funct_id=5, hwhm=0.6034, fill=0.0):
         xs, v1p = x, v1
     if L.orc_fftscn_finish(ctypes.byref(p), v1p, xs.size, v1out, v2out):
         raise RuntimeError(L.orc_error().decode())
-    buf = np.full(p.nfft, fill, np.float64)
-    buf[:p.ltotal] = xs[p.n1load - 1:p.n1load - 1 + p.ltotal]
-    h = np.zeros(p.nfft // 2 + 1)
-    L.orc_fft_apod(ctypes.byref(p), h.ctypes.data)
-    h[-1] = 0.0                                      # Nyquist coefficient
-    y = np.fft.irfft(np.fft.rfft(buf) * h, n=p.nfft)
+    loaded = np.full(p.lrec0 * 65536, fill, np.float64)
+    loaded[:p.ltotal0] = xs[p.n1load - 1:p.n1load - 1 + p.ltotal0]
+    if p.prebox:
+        buf = np.zeros(p.nfft)
+        L.orc_scan_boxcar(ctypes.byref(p), loaded.ctypes.data, fill, buf.ctypes.data)
+    else:
+        buf = loaded
+    if p.fft:
+        X = np.fft.rfft(buf)
+        if p.ivx == 1:
+            h = np.zeros(p.nfft // 2 + 1)
+            L.orc_fft_apod(ctypes.byref(p), h.ctypes.data)
+            h[-1] = 0.0                              # Nyquist coefficient
+            X = X * h
+        elif p.ivx == -1:
+            f = np.zeros(p.nfft)
+            L.orc_scan_sfn(ctypes.byref(p), f.ctypes.data)
+            H = np.fft.rfft(f)
+            H = H / H[0].real                        # unit area
+            X0, XN = X[0].real * H[0].real, X[-1].real * H[-1].real
+            X = X * H
+            X[0], X[-1] = X0, XN
+        if p.deconv:
+            d = np.zeros(p.nfft // 2 + 1)
+            L.orc_fft_deconv(ctypes.byref(p), d.ctypes.data)
+            d[-1] = 0.0
+            X = X * d
+        y = np.fft.irfft(X, n=p.nfft)
+    else:
+        y = buf
     seg = y[p.n1 - 1:p.n1 - 1 + p.ntotal]
     if dvout > 0:
         out, v1o = scan_interp(seg, p.v1s, p.dv, v1out, v2out, dvout)

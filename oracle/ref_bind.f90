@@ -60,6 +60,27 @@ contains
       out(1:min(n, cap)) = d(1:min(n, cap))
    end subroutine
 
+   subroutine ref_fftscn2(np, sp, v1, v2, dv, v1out, v2out, dvout, functid, hwhm, fill, preboxhw, deconv, parm, &
+                          cap, out, nout, v1o, v2o, dvo) bind(C, name="ref_fftscn2")
+      integer(c_int), value :: np, functid, cap, deconv
+      real(c_double), value :: v1, v2, dv, v1out, v2out, dvout, hwhm, fill, preboxhw
+      real(c_double), intent(in) :: sp(np), parm(3)
+      real(c_double), intent(out) :: out(cap)
+      integer(c_int), intent(out) :: nout
+      real(c_double), intent(out) :: v1o, v2o, dvo
+      real, allocatable :: d(:)
+      real(r8) :: a, b, c1, c2
+      real :: dvx, dvs, hw, fl, pb, parms(3)
+      integer :: n
+      a = v1; b = v2; c1 = v1out; c2 = v2out; dvx = dvout; dvs = dv; hw = hwhm; fl = fill; pb = preboxhw
+      parms = parm
+      n = 0
+      call clblm_FFTSCN(sp, a, b, dvs, np, d, c1, c2, dvx, n, functid, hw, fl, pb, deconv /= 0, parms)
+      nout = n
+      v1o = c1; v2o = c2; dvo = dvx
+      out(1:min(n, cap)) = d(1:min(n, cap))
+   end subroutine
+
    subroutine ref_zeta_tables(out) bind(C, name="ref_zeta_tables")
       real(c_double), intent(out) :: out(102, 6)
       out(:, 1) = AVRAT; out(:, 2) = CGAUSS; out(:, 3) = CF1

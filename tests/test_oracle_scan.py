@@ -100,10 +100,10 @@ def test_constant_spectrum_is_scaled_by_the_zero_path_gain(fid):
     apodization at zero path difference: 1, except Hamming's 0.53835685 (1 + 0.857504)
     (clblm_FFT.f90:1690)."""
     from oracle.py import fft_scan
-    sp = np.full(20001, 3.25)
-    y, _, _ = fft_scan(sp, 600.0, 0.01, 650.0, 750.0, 0.0, fid, 0.3, 3.25)
+    sp = np.full(20001, 0.8)   # FILL must lie in [0, 1] (clblm_FFT.f90:428-431)
+    y, _, _ = fft_scan(sp, 600.0, 0.01, 650.0, 750.0, 0.0, fid, 0.3, 0.8)
     gain = 0.53835685 * (1.0 + 0.857504) if fid == 7 else 1.0
-    np.testing.assert_allclose(y, 3.25 * gain, rtol=1e-12)
+    np.testing.assert_allclose(y, 0.8 * gain, rtol=1e-12)
 
 
 def test_sinc_scan_of_a_narrow_line_is_the_instrument_line_shape():
@@ -125,3 +125,76 @@ def test_sinc_scan_of_a_narrow_line_is_the_instrument_line_shape():
     assert np.abs(y[near] - ils).max() <= 2e-3 * ils.max()
     half = vy[near][y[near] >= 0.5 * y.max()]
     assert abs(0.5 * (half.max() - half.min()) - hw) < 0.03 * hw
+
+
+def _ref_scan2(lib, x, v1, dv, v1o, v2o, dvo, fid, hw, fill, pb, dec):
+    lib.ref_fftscn2.argtypes = [I, P, D, D, D, D, D, D, I, D, D, D, I, P, I, P, ctypes.POINTER(I), ctypes.POINTER(D),
+                                ctypes.POINTER(D), ctypes.POINTER(D)]
+    out = np.zeros(400000)
+    n, a, b, c = I(), D(), D(), D()
+    parm = np.zeros(3)
+    lib.ref_fftscn2(x.size, x.ctypes.data, v1, v1 + dv * (x.size - 1), dv, v1o, v2o, dvo, fid, hw, fill, pb, dec,
+                    parm.ctypes.data, out.size, out.ctypes.data, ctypes.byref(n), ctypes.byref(a), ctypes.byref(b),
+                    ctypes.byref(c))
+    return out[:n.value], a.value, c.value
+
+
+# (functID, v1out, v2out, dvout, hwhm, preBoxHW, deconvPreBox)
+SCAN_MODES = [
+    (1, 620., 780., 0., 0.05, -1., 0),      # boxcar scan
+    (1, 620., 780., 0.03, 0.2, -1., 0),
+    (9, 620., 780., 0., 0.6, -1., 0),       # Norton-Beer weak, medium, strong
+    (10, 620., 780., 0.05, 0.6, -1., 0),
+    (11, 620., 780., 0., 0.6, -1., 0),
+    (2, 690., 710., 0., 1.0, -1., 0),       # triangle, Gaussian below CRATIO: transform of the scanning function
+    (3, 698., 702., 0., 1.0, -1., 0),
+    (5, 620., 780., 0., 0.6, 0.05, 1),      # boxcar pre-scan, with and without deconvolution
+    (5, 620., 780., 0., 0.6, 0.05, 0),
+    (3, 620., 780., 0.02, 0.3, 0.03, 1),
+    (105, 620., 780., 0., 0.6, 0.02, 0),    # pre-scanned elsewhere: deconvolution
+    (-1, 620., 780., 0., 0.6, 0.02, 0),     # deconvolution only
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("fid,v1o,v2o,dvo,hw,pb,dec", SCAN_MODES)
+def test_fft_scan_modes_match_reference(ref, fid, v1o, v2o, dvo, hw, pb, dec):
+    from oracle.py import fft_scan
+    _, sp = _spectrum()
+    a, va, da = fft_scan(sp, 600.0, 0.01, v1o, v2o, dvo, fid, hw, 0.0, pb, dec)
+    b, vb, db = _ref_scan2(ref, sp, 600.0, 0.01, v1o, v2o, dvo, fid, hw, 0.0, pb, dec)
+    assert a.size == b.size and va == vb and da == db
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+    if fid == 1:
+        assert np.array_equal(a, b)        # no transform: the boxcar means are bit-identical
+
+
+def test_boxcar_scan_is_the_mean_of_m_points():
+    """functID 1: M = int(2 HWHM / DV) consecutive points averaged, the grid shifted by
+    (M-1) DV / 2 (clblm_FFT.f90:689-706)."""
+    from oracle.py import fft_scan
+    _, sp = _spectrum()
+    y, v1, dvy = fft_scan(sp, 600.0, 0.01, 620.0, 780.0, 0.0, 1, 0.05, 0.0)
+    assert abs(dvy - 0.1) < 1e-12
+    k0 = int(round((v1 - 0.045 - 600.0) / 0.01))
+    means = sp[k0:k0 + 10 * y.size].reshape(y.size, 10).mean(axis=1)
+    np.testing.assert_allclose(y, means, rtol=1e-13)
+
+
+@pytest.mark.parametrize("fid,v1o,v2o,hw", [(2, 690., 710., 1.0), (3, 698., 702., 1.0)])
+def test_scanning_function_transform_has_unit_gain(fid, v1o, v2o, hw):
+    """IVX = -1: the transform of the scanning function is divided by its area, so a
+    constant spectrum padded with the same constant is unchanged."""
+    from oracle.py import fft_scan
+    y, _, _ = fft_scan(np.full(20001, 0.75), 600.0, 0.01, v1o, v2o, 0.0, fid, hw, 0.75)
+    np.testing.assert_allclose(y, 0.75, rtol=1e-12)
+
+
+@pytest.mark.parametrize("fid", [12, 13, 14])
+def test_functions_the_reference_cannot_run_are_rejected(fid):
+    """Brault and Kaiser-Bessel stop in the reference (they need an optical path
+    difference, clblm_FFT.f90:480-489); Kiruna's multi-record scanning function reads a
+    variable that is never set (Scntrn :2136)."""
+    from oracle.py import fft_scan
+    with pytest.raises(RuntimeError):
+        fft_scan(np.ones(20001), 600.0, 0.01, 620.0, 780.0, 0.0, fid, 0.6)
