@@ -75,8 +75,8 @@ def load():
     lib.clblm_level_jac.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P, P, P, ctypes.c_double,
                                     ctypes.c_double, P, P, P, P, P, P, P]
     lib.clblm_fft_scan.argtypes = [P, P, ctypes.c_int64] + [ctypes.c_double] * 5 + [ctypes.c_int] + \
-        [ctypes.c_double] * 2 + [ctypes.c_int, P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
-                                 ctypes.POINTER(ctypes.c_double), P]
+        [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), P]
     lib.clblm_timing.argtypes = [P, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 4 + \
         [ctypes.POINTER(ctypes.c_int64)]
     lib.clblm_work.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),

@@ -2433,8 +2433,9 @@ extern "C" void *clblm_scan_state_new();
 extern "C" void clblm_scan_state_free(void *s);
 extern "C" const char *clblm_scan_error();
 extern "C" int clblm_fft_scan_impl(void *state, const double *d_in, int64_t n, double v1, double dv, double v1out,
-                                   double v2out, double dvout, int funct_id, double hwhm, double fill, int precision,
-                                   double *d_out, int64_t *nout, double *v1res, double *dvres, void *stream);
+                                   double v2out, double dvout, int funct_id, double hwhm, double fill,
+                                   double pre_box_hw, int deconv_pre_box, int precision, double *d_out, int64_t *nout,
+                                   double *v1res, double *dvres, void *stream);
 
 extern "C" {
 
@@ -2639,8 +2640,8 @@ int clblm_odlay(clblm_ctx *c, int handle, const clblm_layer *layer, const clblm_
 }
 
 int clblm_fft_scan(clblm_ctx *c, const double *d_in, int64_t n, double v1, double dv, double v1out, double v2out,
-                   double dvout, int funct_id, double hwhm, double fill, int precision, double *d_out,
-                   int64_t *nout, double *v1res, double *dvres, void *stream) {
+                   double dvout, int funct_id, double hwhm, double fill, double pre_box_hw, int deconv_pre_box,
+                   int precision, double *d_out, int64_t *nout, double *v1res, double *dvres, void *stream) {
     if (!c || !nout || !v1res || !dvres) return set_err(CLBLM_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(c->device));
     std::lock_guard<std::mutex> g(c->mu);
@@ -2650,8 +2651,8 @@ int clblm_fft_scan(clblm_ctx *c, const double *d_in, int64_t n, double v1, doubl
         const int kp = (int)((c->ncalls + clblm_ctx::NSLOT - 1) % clblm_ctx::NSLOT);
         HIPCHK(hipStreamWaitEvent(st, c->slot[kp].done, 0));
     }
-    int rc = clblm_fft_scan_impl(c->scan, d_in, n, v1, dv, v1out, v2out, dvout, funct_id, hwhm, fill, precision,
-                                 d_out, nout, v1res, dvres, st);
+    int rc = clblm_fft_scan_impl(c->scan, d_in, n, v1, dv, v1out, v2out, dvout, funct_id, hwhm, fill, pre_box_hw,
+                                 deconv_pre_box, precision, d_out, nout, v1res, dvres, st);
     if (rc) return set_err(rc, clblm_scan_error());
     return 0;
 }

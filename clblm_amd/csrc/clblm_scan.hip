@@ -1,13 +1,18 @@
 // Instrument scan on the GPU: clblm_FFTSCN_struct (clblm_FFT.f90:352-844) for the
-// scanning functions whose apodization is evaluated analytically (IVX = 1):
-// triangle, Gaussian, sinc**2, sinc, Beer, Hamming, Hanning.
+// boxcar scan, the scanning functions 2..11 (triangle, Gaussian, sinc**2, sinc, Beer,
+// Hamming, Hanning, Norton-Beer weak/medium/strong), the optional boxcar pre-scan with
+// or without deconvolution, the externally pre-scanned and deconvolution-only modes.
 //
-//   host    the scan bookkeeping (V1Z/V2Z, LoadSpect's LTOTAL and record rounding,
-//           wrtSpect's window) and the control flow of the 4-point interpolation
-//           (Intpdr/clblm_INTERP/panelInput), which depends on the grids only and is
-//           cut into segments that restart the accumulated fraction
-//   device  load + fill, real-to-complex FFT (hipFFT), apodization (Scntrn/Scnfnt),
-//           complex-to-real FFT, window, and one lane per interpolation segment
+//   host    the scan bookkeeping (V1Z/V2Z, LoadSpect's LTOTAL and record rounding, the
+//           boxcar's new grid, wrtSpect's window), the apodization blocks and the
+//           control flow of the 4-point interpolation (Intpdr/clblm_INTERP/panelInput),
+//           which depends on the grids only and is cut into segments that restart the
+//           accumulated fraction
+//   device  load + fill (or the boxcar means), real-to-complex FFT (hipFFT), the
+//           apodization evaluated analytically (IVX = 1) or the normalised transform of
+//           the scanning function (IVX = -1, below CRATIO half widths), the boxcar
+//           deconvolution, complex-to-real FFT, window, and one lane per
+//           interpolation segment
 //
 // The reference transforms LREC*65536 points (LREC = 4 x the next power of two of
 // the data records, clblm_FFT.f90:1183-1197); the same length is used here, so the
@@ -259,6 +264,23 @@ __global__ void k_scan_load(const double *xs, int64_t first, int64_t ltotal, dou
     buf[i] = (TB)(i < ltotal ? xs[first + i] : fill);
 }
 
+// Boxcar (clblm_FFT.f90:867-990): the mean of m consecutive loaded points, summed in
+// order, then `fill` to the end of the transform buffer.
+template <typename TB>
+__global__ void k_scan_boxcar(const double *xs, int64_t first, int64_t ktotal, int m, double fill, TB *buf,
+                              int64_t nfft) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nfft) return;
+    if (i < ktotal) {
+        const double *x = xs + first + i * m;
+        double sum = 0.0;
+        for (int q = 0; q < m; q++) sum = sum + x[q];
+        buf[i] = (TB)(sum / (double)m);
+    } else {
+        buf[i] = (TB)fill;
+    }
+}
+
 // Scnfnt's apodization branch (clblm_FFT.f90:1411-1958) for coefficient m of block J:
 // X = X0(J) + m' DX with X0 accumulated block by block as the reference does.
 __device__ inline double apod(int jfn, double X, double A) {
@@ -277,9 +299,78 @@ __device__ inline double apod(int jfn, double X, double A) {
     case 4: return (X < 1.0 / A) ? 1.0 : 0.0;
     case 5: return (X < 1 / A) ? (1.0 - (X * A) * (X * A)) * (1.0 - (X * A) * (X * A)) : 0.0;
     case 6: return (X < 1 / A) ? 0.53835685 * (1.0 + 0.857504 * cos(PI * X * A)) : 0.0;
-    default: return (X < 1. / A) ? 0.5 * (1.0 + cos(PI * X * A)) : 0.0;
+    case 7: return (X < 1. / A) ? 0.5 * (1.0 + cos(PI * X * A)) : 0.0;
+    default: {  // Norton-Beer weak, medium, strong (clblm_FFT.f90:1758-1798)
+        if (!(X < 1. / A)) return 0.0;
+        const double U = 1.0 - (X * A) * (X * A);
+        if (jfn == 8) return 0.384093 - 0.087577 * U + 0.703484 * (U * U);
+        if (jfn == 9) return 0.152442 - 0.136176 * U + 0.983734 * (U * U);
+        return 0.045355 + 0.554883 * (U * U) + 0.399782 * ((U * U) * (U * U));
+    }
     }
 }
+
+// The boxcar deconvolution factor, Scnfnt's function -1 (clblm_FFT.f90:1920-1946).
+__device__ inline double deconv_fac(double X, double A) {
+    const double PI = 2.0 * asin(1.0);
+    const double Z = 2.0 * PI * X * A;
+    return Z != 0 ? Z / sin(Z) : 1.0;
+}
+
+// Scnfnt's scanning functions (IAPSC = -1) for JFN 1..7 at X >= 0 (clblm_FFT.f90:1519-1755).
+__device__ inline double sfn(int jfn, double X, double A) {
+    const double Pi = 2.0 * asin(1.0);
+    switch (jfn) {
+    case 1: return (X > A) ? 0.0 : (1.0 - X / A);
+    case 2: {
+        const double t = 0.5 * ((X / A) * (X / A));
+        return exp(-(t < 34. ? t : 34.));
+    }
+    case 3:
+        if (X != 0) {
+            const double s = sin(Pi * X / A) / (Pi * X / A);
+            return s * s;
+        }
+        return 1.0;
+    case 4: return (X != 0) ? sin(2.0 * Pi * X / A) / (2.0 * Pi * X / A) : 1.0;
+    case 5: {
+        const double U = 2.0 * Pi * X / A;
+        if (U >= 0.5) {
+            const double U2 = U * U, U5 = U2 * U2 * U;
+            return 15. * ((3.0 - U2) * sin(U) - 3.0 * U * cos(U)) / U5;
+        }
+        return (1.0 - 15.0 / 210 * (U * U));
+    }
+    default: {
+        const double c = jfn == 6 ? 0.428752 : 0.5;
+        const double U = 2.0 * Pi * X / A;
+        if (U != 0 && fabs(U) != Pi) return (sin(U) / U + c * (sin(U + Pi) / (U + Pi) + sin(U - Pi) / (U - Pi)));
+        if (U == 0) return 1.0;
+        return c;
+    }
+    }
+}
+
+// Scntrn with IVX = -1 (clblm_FFT.f90:2113-2175): the scanning function on the
+// transform grid for an even number of records.  Block J < lrec/2 holds f(V_J + q DV);
+// block lrec-1-J holds the mirror images, its first element f(V_{J+1}).
+// vblk[J] = V_J, accumulated block by block as the reference does (lrec/2 + 1 values).
+template <typename TB>
+__global__ void k_scan_sfn(TB *f, int64_t nfft, int64_t lrec, const double *vblk, int jfn, double A, double DV) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nfft) return;
+    const int64_t J = i / LPTSMX;
+    const int64_t q = i % LPTSMX;
+    double X;
+    if (J < lrec / 2) {
+        X = (q == 0) ? vblk[J] : vblk[J] + (double)q * DV;
+    } else {
+        const int64_t jp = lrec - J;   // the positive block (1-based) this one mirrors
+        X = (q == 0) ? vblk[jp] : vblk[jp - 1] + (double)(LPTSMX - q) * DV;
+    }
+    f[i] = (TB)sfn(jfn, X, A);
+}
+
 
 struct ApodBlock {
     double x0;     // X at the block's first coefficient
@@ -288,9 +379,12 @@ struct ApodBlock {
     double vend;
 };
 
+// Multrn (clblm_FFT.f90:1344-1405) with the analytic apodization of JFN (none for
+// JFN 0) and, for deconv_a > 0, the boxcar deconvolution factor; `scale` is the
+// inverse transform's 1/N.
 template <typename TC, typename TS>
 __global__ void k_scan_apod(TC *c, int64_t nfft, const ApodBlock *blk, int jfn, double A, double DX,
-                            double scale) {
+                            double deconv_a, double scale) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t half = nfft / 2;
     if (m > half) return;
@@ -301,11 +395,45 @@ __global__ void k_scan_apod(TC *c, int64_t nfft, const ApodBlock *blk, int jfn, 
         const int64_t J = m / (LPTSMX / 2);
         const int kb = (int)(m % (LPTSMX / 2));
         const ApodBlock b = blk[J];
-        h = (kb == b.nend) ? b.vend : apod(jfn, b.x0 + (double)kb * DX, A);
+        const double X = b.x0 + (double)kb * DX;
+        h = jfn == 0 ? 1.0 : ((kb == b.nend) ? b.vend : apod(jfn, X, A));
+        if (deconv_a > 0.) h = h * deconv_fac(X, deconv_a);
     }
     const double f = h * scale;
     c[m].x = (TS)((double)c[m].x * f);
     c[m].y = (TS)((double)c[m].y * f);
+}
+
+// Multrn with the transform of the scanning function (IVX = -1), divided by its first
+// coefficient (the area, Scntrn :2180-2197); DC and Nyquist are products of reals.
+template <typename TC, typename TS>
+__global__ void k_scan_cmul(TC *c, const TC *hc, int64_t nfft, const ApodBlock *blk, double DX, double deconv_a,
+                            double scale) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t half = nfft / 2;
+    if (m > half) return;
+    const double h0 = (double)hc[0].x;
+    const double hr = (double)hc[m].x / h0, hi = (double)hc[m].y / h0;
+    const double xr = (double)c[m].x, xi = (double)c[m].y;
+    double yr, yi;
+    if (m == 0 || m == half) {
+        yr = xr * hr;
+        yi = 0.0;
+    } else {
+        yr = xr * hr - xi * hi;
+        yi = xr * hi + xi * hr;
+    }
+    if (deconv_a > 0.) {
+        double d = 0.0;
+        if (m < half) {
+            const ApodBlock b = blk[m / (LPTSMX / 2)];
+            d = deconv_fac(b.x0 + (double)(m % (LPTSMX / 2)) * DX, deconv_a);
+        }
+        yr = yr * d;
+        yi = yi * d;
+    }
+    c[m].x = (TS)(yr * scale);
+    c[m].y = (TS)(yi * scale);
 }
 
 template <typename TB>
@@ -321,6 +449,8 @@ struct ScanState {
     hipfftHandle fwd = 0, inv = 0;
     void *buf = nullptr;           // nfft + 2 reals (in-place transforms)
     size_t buf_cap = 0;
+    void *hbuf = nullptr;          // the scanning function and its transform (IVX = -1)
+    size_t hbuf_cap = 0;
     double *xs = nullptr;          // re-gridded input
     size_t xs_cap = 0;
     double *win = nullptr;         // scan window before the output interpolation
@@ -331,6 +461,7 @@ struct ScanState {
         if (fwd) hipfftDestroy(fwd);
         if (inv) hipfftDestroy(inv);
         (void)hipFree(buf);
+        (void)hipFree(hbuf);
         (void)hipFree(xs);
         (void)hipFree(win);
         (void)hipFree(meta);
@@ -359,34 +490,83 @@ struct ScanPlan {
     int64_t np = 0;
     double v1z = 0., v2z = 0.;
     int64_t ltotal = 0, n1load = 0, lrec = 0, nfft = 0;
+    int ivx = 0;           // 1 analytic apodization, -1 transform of the scanning function, 0 none
+    bool fft = true;       // false: boxcar scan without deconvolution, no transform
+    int m = 0;             // boxcar pre-scan over m points (0: none)
+    int64_t ktotal = 0;    // boxcar means
+    double deconv_a = 0.;  // > 0: divide by the boxcar's transform of this width parameter
     int64_t n1 = 0, ntotal = 0;
     double v1s = 0.;
     bool out_interp = false;
     IPlan out_ip;
     std::vector<ApodBlock> blocks;
+    std::vector<double> vblk;   // IVX = -1: start of each positive block of the scanning function
 };
 
+int64_t scan_records(int64_t ntot) {  // LoadSpect/Boxcar record count (clblm_FFT.f90:1189-1197, 890-910)
+    const double ARDATA = (double)ntot / (double)LPTSMX;
+    const int64_t LRDATA = (ARDATA - (double)(int)ARDATA == 0.0) ? (int64_t)(int)ARDATA : (int64_t)(int)(ARDATA + 1);
+    const double POWER = std::log((double)LRDATA) / std::log(2.0);
+    return (POWER - (double)(int)POWER == 0) ? ((int64_t)1 << (int)(POWER + 2)) : ((int64_t)1 << ((int)POWER + 3));
+}
+
 int plan_scan(int64_t n, double v1, double dv, double v1out, double v2out, double dvout, int funct_id, double hwhm,
-              ScanPlan &p) {
-    const int JFN = funct_id - 1;
-    if (JFN < 1 || JFN > 7)
-        return fail(CLBLM_ERR_UNSUPPORTED, "scan functions 2..8 (triangle .. Hanning) are supported");
+              double fill, double pre_box_hw, int deconv_pre_box, ScanPlan &p) {
+    if (!(fill >= 0. && fill <= 1.)) return fail(CLBLM_ERR_ARG, "fill must lie in [0, 1]");
+    if (!((funct_id >= 1 && funct_id <= 14) || (funct_id >= 102 && funct_id <= 114) || funct_id == -1))
+        return fail(CLBLM_ERR_ARG, "funct_id must be in [1, 14], [102, 114] or -1");
     if (!(hwhm > 0.) || !(dv > 0.) || n < 8) return fail(CLBLM_ERR_ARG, "hwhm, dv must be positive and n >= 8");
+    // the function and the boxcar/deconvolution switches (clblm_FFT.f90:447-474)
+    int JFN = funct_id - 1;
+    bool prebox = false, deconv = false, deconv_only = false;
+    double boxhw = 0., dec_a = 0.;
+    if (JFN == 0) {
+        prebox = true;
+        boxhw = hwhm;
+    } else if (JFN >= 1 && JFN <= 13) {
+        if (pre_box_hw > 0.) {
+            prebox = true;
+            deconv = deconv_pre_box != 0;
+            boxhw = pre_box_hw;
+        }
+    } else if (JFN >= 101 && JFN <= 113) {
+        JFN -= 100;
+        deconv = true;
+        dec_a = pre_box_hw;
+    } else {  // -2: deconvolution only
+        deconv_only = true;
+        JFN = 0;
+        deconv = true;
+        dec_a = pre_box_hw;
+    }
+    if (JFN == 11 || JFN == 12)
+        return fail(CLBLM_ERR_ARG, "the Brault and Kaiser-Bessel functions need the interferometer optical path "
+                                   "difference, which clblm_FFTSCN does not take");
+    if (JFN == 13)
+        return fail(CLBLM_ERR_UNSUPPORTED, "the Kiruna scanning function is not supported");
+    if (deconv && !(dec_a > 0.) && !prebox) return fail(CLBLM_ERR_ARG, "deconvolution needs pre_box_hw > 0");
     p.jfn = JFN;
-    p.a = SC_C[JFN] * hwhm;
+    p.a = JFN > 0 ? SC_C[JFN] * hwhm : 0.;
     const double V1C = v1, V2C = v1 + dv * (double)(n - 1);
     if (v1out >= v2out) return fail(CLBLM_ERR_ARG, "v1out >= v2out");
     if (v1out >= V2C || v2out <= V1C) return fail(CLBLM_ERR_ARG, "v1out..v2out lies outside the data");
-    double V1Z = v1out - hwhm * SC_CLIMIT[JFN], V2Z = v2out + hwhm * SC_CLIMIT[JFN];
-    V1Z = V1C + (double)(int)((V1Z - V1C) / dv) * dv;
-    V2Z = V1C + (double)(int)((V2Z - V1C) / dv + 1.01) * dv;
+    double V1Z, V2Z;
+    if (deconv_only) {
+        V1Z = V1C;
+        V2Z = V2C;
+    } else {
+        V1Z = v1out - hwhm * SC_CLIMIT[JFN];
+        V2Z = v2out + hwhm * SC_CLIMIT[JFN];
+        V1Z = V1C + (double)(int)((V1Z - V1C) / dv) * dv;
+        V2Z = V1C + (double)(int)((V2Z - V1C) / dv + 1.01) * dv;
+    }
     if (V1Z < V1C) V1Z = V1C;
     if (V2Z > V2C) V2Z = V2C;
     if ((V2Z - V1Z) / hwhm < SC_CLIMIT[JFN]) return fail(CLBLM_ERR_ARG, "insufficient data for an accurate scan");
     const double DVN = 2. * hwhm / 12;
     p.dvn = DVN;
     double DV = dv;
-    if (DV > DVN) {  // Intpdr onto DVN first (clblm_FFT.f90:594-614)
+    if (DV > DVN && JFN != 0) {  // Intpdr onto DVN first (clblm_FFT.f90:594-614)
         p.pre = true;
         int rc = plan_interp(n, v1, dv, V1Z, V2Z, DVN, p.pre_ip);
         if (rc) return rc;
@@ -399,26 +579,43 @@ int plan_scan(int64_t n, double v1, double dv, double v1out, double v2out, doubl
         p.v1p = v1;
         p.np = n;
     }
-    p.dv = DV;
-    if (!((V2Z - V1Z) / hwhm > SC_CRATIO[JFN]))
-        return fail(CLBLM_ERR_UNSUPPORTED,
-                    "scan range under CRATIO half widths: the apodization would be the transform of the scanning "
-                    "function (IVX = -1), which is not supported");
+    // IVX (clblm_FFT.f90:616-634)
+    p.ivx = JFN == 0 ? 0 : (((V2Z - V1Z) / hwhm > SC_CRATIO[JFN]) ? 1 : -1);
+    if (p.ivx == -1 && JFN >= 8)
+        return fail(CLBLM_ERR_ARG, "Norton-Beer functions have no spectral-domain form: the scan range must exceed "
+                                   "CRATIO half widths");
     if (p.v1p > V1Z) return fail(CLBLM_ERR_ARG, "first data frequency above the scan start");
     // LoadSpect (clblm_FFT.f90:1166-1206)
-    const int64_t LTOTAL = (int64_t)(int)((V2Z - V1Z) / DV + 1.1);
+    int64_t LTOTAL = (int64_t)(int)((V2Z - V1Z) / DV + 1.1);
     V2Z = V1Z + DV * (double)(LTOTAL - 1);
-    const double ARDATA = (double)LTOTAL / (double)LPTSMX;
-    const int64_t LRDATA = (ARDATA - (double)(int)ARDATA == 0.0) ? (int64_t)(int)ARDATA : (int64_t)(int)(ARDATA + 1);
-    const double POWER = std::log((double)LRDATA) / std::log(2.0);
-    const int64_t LREC = (POWER - (double)(int)POWER == 0) ? ((int64_t)1 << (int)(POWER + 2))
-                                                            : ((int64_t)1 << ((int)POWER + 3));
+    int64_t LREC = scan_records(LTOTAL);
     const int64_t N1L = (int64_t)(int)((V1Z - p.v1p) / DV + 1.00001);
     if (N1L < 1 || N1L + LTOTAL - 1 > p.np) return fail(CLBLM_ERR_ARG, "scan window runs out of data");
+    p.n1load = N1L;
+    // boxcar pre-scan (clblm_FFT.f90:689-706)
+    if (prebox) {
+        const int M = (int)(2.0 * boxhw / DV);
+        if (M >= 2) {
+            V1Z = V1Z + DV * (double)(M - 1) / 2.0;
+            const int64_t KTOTAL = LTOTAL / M;
+            if (KTOTAL < 1) return fail(CLBLM_ERR_ARG, "boxcar wider than the scan range");
+            DV = (double)M * DV;
+            V2Z = V1Z + DV * (double)(KTOTAL - 1);
+            LREC = scan_records(KTOTAL);
+            LTOTAL = KTOTAL;
+            p.m = M;
+            p.ktotal = KTOTAL;
+            dec_a = DV;
+        } else {
+            deconv = false;
+        }
+    }
+    p.deconv_a = deconv ? dec_a : 0.;
+    p.fft = !(JFN == 0 && !deconv);
+    p.dv = DV;
     p.v1z = V1Z;
     p.v2z = V2Z;
     p.ltotal = LTOTAL;
-    p.n1load = N1L;
     p.lrec = LREC;
     p.nfft = LREC * LPTSMX;
     // wrtSpect window (clblm_FFT.f90:776-796)
@@ -437,7 +634,8 @@ int plan_scan(int64_t n, double v1, double dv, double v1out, double v2out, doubl
         int rc = plan_interp(p.ntotal, V1S, DV, v1out, v2out, dvout, p.out_ip);
         if (rc) return rc;
     }
-    // Scntrn, IVX = 1 (clblm_FFT.f90:2217-2229): X per block, sinc end point per block
+    // Scntrn, IVX = 1 (clblm_FFT.f90:2217-2229): X per block, sinc end point per block;
+    // the same positions serve the deconvolution factor
     const double DX = 1.0 / ((double)(LPTSMX * LREC) * DV);
     p.blocks.resize((size_t)LREC);
     double X = 0.0;
@@ -446,7 +644,7 @@ int plan_scan(int64_t n, double v1, double dv, double v1out, double v2out, doubl
         b.x0 = X;
         b.nend = -1;
         b.vend = 0.;
-        if (JFN == 4) {
+        if (JFN == 4 && p.ivx == 1) {
             const double ALEN = 1.0 / p.a, XSTART = X;
             int N = (int)((ALEN - XSTART) / DX) + 1;
             double R = (ALEN - XSTART) / DX + 1 - N;
@@ -454,6 +652,14 @@ int plan_scan(int64_t n, double v1, double dv, double v1out, double v2out, doubl
             if (N >= 1 && N <= LPTSMX / 2 - 1) { b.nend = N; b.vend = R - 0.5; }
         }
         X = X + (double)(LPTSMX) * DX / 2.;
+    }
+    if (p.ivx == -1) {  // Scntrn, IVX = -1: block starts V_J (clblm_FFT.f90:2113-2134)
+        p.vblk.resize((size_t)(LREC / 2 + 1));
+        double V = 0.0;
+        for (int64_t J = 0; J <= LREC / 2; J++) {
+            p.vblk[(size_t)J] = V;
+            V = V + (double)LPTSMX * DV;
+        }
     }
     return 0;
 }
@@ -487,21 +693,47 @@ int ensure_plans(ScanState &S, int64_t nfft, int prec, hipStream_t st) {
         if (e_ != hipSuccess) return fail(CLBLM_ERR_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
     } while (0)
 
+// The transform buffer: the loaded spectrum or its boxcar means, padded with `fill`.
+template <typename TB>
+void fill_buffer(const ScanPlan &p, const double *xs, double fill, TB *buf, hipStream_t st) {
+    if (p.m >= 2)
+        hipLaunchKernelGGL(k_scan_boxcar<TB>, dim3(cdiv(p.nfft, 256)), dim3(256), 0, st, xs, p.n1load - 1, p.ktotal,
+                           p.m, fill, buf, p.nfft);
+    else
+        hipLaunchKernelGGL(k_scan_load<TB>, dim3(cdiv(p.nfft, 256)), dim3(256), 0, st, xs, p.n1load - 1, p.ltotal,
+                           fill, buf, p.nfft);
+}
+
 template <typename TB, typename TC>
 int run_transform(ScanState &S, const ScanPlan &p, const double *xs, double fill, const ApodBlock *d_blk,
-                  hipStream_t st) {
+                  const double *d_vblk, hipStream_t st) {
     TB *buf = (TB *)S.buf;
-    hipLaunchKernelGGL(k_scan_load<TB>, dim3(cdiv(p.nfft, 256)), dim3(256), 0, st, xs, p.n1load - 1, p.ltotal, fill,
-                       buf, p.nfft);
-    hipfftResult r;
-    if constexpr (sizeof(TB) == 8)
-        r = hipfftExecD2Z(S.fwd, (hipfftDoubleReal *)buf, (hipfftDoubleComplex *)buf);
-    else
-        r = hipfftExecR2C(S.fwd, (hipfftReal *)buf, (hipfftComplex *)buf);
-    if (r != HIPFFT_SUCCESS) return fail(CLBLM_ERR_HIP, "forward FFT failed");
+    fill_buffer<TB>(p, xs, fill, buf, st);
+    auto fwd = [&](TB *x) {
+        hipfftResult r;
+        if constexpr (sizeof(TB) == 8)
+            r = hipfftExecD2Z(S.fwd, (hipfftDoubleReal *)x, (hipfftDoubleComplex *)x);
+        else
+            r = hipfftExecR2C(S.fwd, (hipfftReal *)x, (hipfftComplex *)x);
+        return r == HIPFFT_SUCCESS ? 0 : fail(CLBLM_ERR_HIP, "forward FFT failed");
+    };
+    int rc = fwd(buf);
+    if (rc) return rc;
     const double DX = 1.0 / ((double)(LPTSMX * p.lrec) * p.dv);
-    hipLaunchKernelGGL((k_scan_apod<TC, TB>), dim3(cdiv(p.nfft / 2 + 1, 256)), dim3(256), 0, st, (TC *)buf, p.nfft, d_blk,
-                       p.jfn, p.a, DX, 1.0 / (double)p.nfft);
+    const unsigned g = cdiv(p.nfft / 2 + 1, 256);
+    if (p.ivx == -1) {
+        TB *h = (TB *)S.hbuf;
+        hipLaunchKernelGGL(k_scan_sfn<TB>, dim3(cdiv(p.nfft, 256)), dim3(256), 0, st, h, p.nfft, p.lrec, d_vblk,
+                           p.jfn, p.a, p.dv);
+        rc = fwd(h);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_scan_cmul<TC, TB>), dim3(g), dim3(256), 0, st, (TC *)buf, (const TC *)h, p.nfft, d_blk,
+                           DX, p.deconv_a, 1.0 / (double)p.nfft);
+    } else {
+        hipLaunchKernelGGL((k_scan_apod<TC, TB>), dim3(g), dim3(256), 0, st, (TC *)buf, p.nfft, d_blk, p.jfn, p.a,
+                           DX, p.deconv_a, 1.0 / (double)p.nfft);
+    }
+    hipfftResult r;
     if constexpr (sizeof(TB) == 8)
         r = hipfftExecZ2D(S.inv, (hipfftDoubleComplex *)buf, (hipfftDoubleReal *)buf);
     else
@@ -518,11 +750,11 @@ extern "C" __attribute__((visibility("hidden"))) const char *clblm_scan_error() 
 
 extern "C" __attribute__((visibility("hidden"))) int clblm_fft_scan_impl(
     void *state, const double *d_in, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
-    int funct_id, double hwhm, double fill, int precision, double *d_out, int64_t *nout, double *v1res,
-    double *dvres, void *stream) {
+    int funct_id, double hwhm, double fill, double pre_box_hw, int deconv_pre_box, int precision, double *d_out,
+    int64_t *nout, double *v1res, double *dvres, void *stream) {
     if (precision != CLBLM_FP64 && precision != CLBLM_FP32C) return fail(CLBLM_ERR_ARG, "unknown precision");
     ScanPlan p;
-    int rc = plan_scan(n, v1, dv, v1out, v2out, dvout, funct_id, hwhm, p);
+    int rc = plan_scan(n, v1, dv, v1out, v2out, dvout, funct_id, hwhm, fill, pre_box_hw, deconv_pre_box, p);
     if (rc) return rc;
     if (p.out_interp) {
         *nout = p.out_ip.nout;
@@ -537,11 +769,13 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_fft_scan_impl(
     if (!d_in) return fail(CLBLM_ERR_ARG, "input spectrum is NULL");
     ScanState &S = *(ScanState *)state;
     hipStream_t st = (hipStream_t)stream;
-    // metadata: coefficients, apodization blocks, both segment lists
+    // metadata: coefficients, apodization blocks, block starts, both segment lists
+    auto up = [](size_t o) { return (o + 255) & ~(size_t)255; };
     const size_t o_cf = 0;
-    const size_t o_blk = (sizeof(ICoef) + 255) & ~(size_t)255;
-    const size_t o_s1 = (o_blk + sizeof(ApodBlock) * p.blocks.size() + 255) & ~(size_t)255;
-    const size_t o_s2 = (o_s1 + sizeof(ISeg) * p.pre_ip.segs.size() + 255) & ~(size_t)255;
+    const size_t o_blk = up(sizeof(ICoef));
+    const size_t o_vb = up(o_blk + sizeof(ApodBlock) * p.blocks.size());
+    const size_t o_s1 = up(o_vb + sizeof(double) * p.vblk.size());
+    const size_t o_s2 = up(o_s1 + sizeof(ISeg) * p.pre_ip.segs.size());
     const size_t meta = o_s2 + sizeof(ISeg) * p.out_ip.segs.size() + 256;
     // the metadata arrays are read by the kernels below: finish the previous scan first
     SCHK(hipStreamSynchronize(st));
@@ -551,6 +785,7 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_fft_scan_impl(
     const ICoef cf = make_coef();
     std::memcpy(host.data() + o_cf, &cf, sizeof cf);
     std::memcpy(host.data() + o_blk, p.blocks.data(), sizeof(ApodBlock) * p.blocks.size());
+    if (!p.vblk.empty()) std::memcpy(host.data() + o_vb, p.vblk.data(), sizeof(double) * p.vblk.size());
     if (!p.pre_ip.segs.empty()) std::memcpy(host.data() + o_s1, p.pre_ip.segs.data(), sizeof(ISeg) * p.pre_ip.segs.size());
     if (!p.out_ip.segs.empty()) std::memcpy(host.data() + o_s2, p.out_ip.segs.data(), sizeof(ISeg) * p.out_ip.segs.size());
     SCHK(hipMemcpyAsync(S.meta, host.data(), meta, hipMemcpyHostToDevice, st));
@@ -566,15 +801,25 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_fft_scan_impl(
                            S.xs);
         xs = S.xs;
     }
-    const bool d64 = precision == CLBLM_FP64;
+    // without a transform (boxcar scan) the buffer is FP64 in both precisions
+    const bool d64 = precision == CLBLM_FP64 || !p.fft;
     rc = grow(&S.buf, &S.buf_cap, (d64 ? 8 : 4) * (size_t)(p.nfft + 2));
     if (rc) return rc;
-    rc = ensure_plans(S, p.nfft, precision, st);
-    if (rc) return rc;
-    const ApodBlock *d_blk = (const ApodBlock *)(mb + o_blk);
-    rc = d64 ? run_transform<double, hipfftDoubleComplex>(S, p, xs, fill, d_blk, st)
-             : run_transform<float, hipfftComplex>(S, p, xs, fill, d_blk, st);
-    if (rc) return rc;
+    if (p.fft) {
+        if (p.ivx == -1) {
+            rc = grow(&S.hbuf, &S.hbuf_cap, (d64 ? 8 : 4) * (size_t)(p.nfft + 2));
+            if (rc) return rc;
+        }
+        rc = ensure_plans(S, p.nfft, d64 ? CLBLM_FP64 : CLBLM_FP32C, st);
+        if (rc) return rc;
+        const ApodBlock *d_blk = (const ApodBlock *)(mb + o_blk);
+        const double *d_vb = (const double *)(mb + o_vb);
+        rc = d64 ? run_transform<double, hipfftDoubleComplex>(S, p, xs, fill, d_blk, d_vb, st)
+                 : run_transform<float, hipfftComplex>(S, p, xs, fill, d_blk, d_vb, st);
+        if (rc) return rc;
+    } else {
+        fill_buffer<double>(p, xs, fill, (double *)S.buf, st);
+    }
     if (!p.out_interp) {
         if (d64)
             hipLaunchKernelGGL(k_scan_window<double>, dim3(cdiv(p.ntotal, 256)), dim3(256), 0, st,

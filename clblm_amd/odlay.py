@@ -122,15 +122,16 @@ class Context:
 
     # -- instrument scan ---------------------------------------------------------------
     def fft_scan(self, x, v1, dv, v1out, v2out, dvout=0.0, funct_id=5, hwhm=0.6034, fill=0.0, precision=FP64,
-                 stream=None):
+                 stream=None, pre_box_hw=-1.0, deconv_pre_box=False):
         """clblm_FFTSCN_array (src/clblm_src/clblm_FFT.f90:308-349) on a float64 cuda
-        tensor x on v1 + i*dv: returns (scanned tensor, its v1, its dv)."""
+        tensor x on v1 + i*dv: returns (scanned tensor, its v1, its dv).  pre_box_hw > 0
+        pre-scans with a boxcar of that half width (deconvolved if deconv_pre_box)."""
         import torch
         assert x.dtype == torch.float64 and x.is_cuda
         x = x.contiguous()
         n, a, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
         args = (int(x.numel()), float(v1), float(dv), float(v1out), float(v2out), float(dvout), int(funct_id),
-                float(hwhm), float(fill), int(precision))
+                float(hwhm), float(fill), float(pre_box_hw), int(bool(deconv_pre_box)), int(precision))
         check(self.lib.clblm_fft_scan(self.ptr, None, *args, None, ctypes.byref(n), ctypes.byref(a),
                                       ctypes.byref(b), None))
         out = torch.empty(n.value, dtype=torch.float64, device=x.device)

@@ -165,18 +165,29 @@ int clblm_level_jac(clblm_ctx *ctx, int nlay, int nmol, int64_t npts, const doub
                     void *stream);
 
 /* ---- instrument scan ------------------------------------------------------------
- * Drop-in for clblm_FFTSCN_array (clblm_FFT.f90:308-349) with no boxcar pre-scan:
- * the spectrum d_in[0..n) (device) on v1 + i*dv is convolved with scanning function
- * funct_id of half width hwhm (2 triangle, 3 Gaussian, 4 sinc**2, 5 sinc, 6 Beer,
- * 7 Hamming, 8 Hanning; the apodization is evaluated analytically, which the
- * reference does when the scanned range exceeds CRATIO half widths) by FFT over the
- * reference's transform length, the transform padded with `fill`, between v1out
- * and v2out, then interpolated (4-point) onto dvout when dvout > 0.  Query with
- * d_out == NULL: *nout, *v1res and *dvres give the output grid; then d_out (device,
- * *nout doubles) is filled on `stream`.  precision CLBLM_FP32C transforms in FP32. */
+ * Drop-in for clblm_FFTSCN_array (clblm_FFT.f90:308-349): the spectrum d_in[0..n)
+ * (device) on v1 + i*dv is scanned with function funct_id of half width hwhm between
+ * v1out and v2out, then interpolated (4-point) onto dvout when dvout > 0.
+ *   funct_id  1 boxcar; 2 triangle, 3 Gaussian, 4 sinc**2, 5 sinc, 6 Beer, 7 Hamming,
+ *             8 Hanning, 9/10/11 Norton-Beer weak/medium/strong, convolved by FFT over
+ *             the reference's transform length with the apodization evaluated
+ *             analytically above CRATIO half widths of scanned range and as the
+ *             normalised transform of the scanning function below (2..8 only);
+ *             102..111 the same for a spectrum pre-scanned elsewhere with a boxcar
+ *             of half width pre_box_hw, which is deconvolved; -1 deconvolution of
+ *             such a boxcar only.  12 (Brault) and 13 (Kaiser-Bessel) need an optical
+ *             path difference that clblm_FFTSCN does not take, and 14 (Kiruna) is not
+ *             supported: these return an error.
+ *   fill      the transform padding, in [0, 1].
+ *   pre_box_hw  > 0: boxcar pre-scan of that half width (functions 2..11), with the
+ *             boxcar deconvolved when deconv_pre_box != 0; <= 0: none.
+ * Query with d_out == NULL: *nout, *v1res and *dvres give the output grid; then d_out
+ * (device, *nout doubles) is filled on `stream`.  precision CLBLM_FP32C transforms in
+ * FP32. */
 int clblm_fft_scan(clblm_ctx *ctx, const double *d_in, int64_t n, double v1, double dv, double v1out,
-                   double v2out, double dvout, int funct_id, double hwhm, double fill, int precision,
-                   double *d_out, int64_t *nout, double *v1res, double *dvres, void *stream);
+                   double v2out, double dvout, int funct_id, double hwhm, double fill, double pre_box_hw,
+                   int deconv_pre_box, int precision, double *d_out, int64_t *nout, double *v1res, double *dvres,
+                   void *stream);
 
 /* ---- timing -------------------------------------------------------------------
  * Average device time (ms) of each pipeline stage over the launches since the

@@ -65,3 +65,58 @@ def test_scan_large_fp32_vs_fp64(ctx):
     err = np.abs(y32 - y64).max() / np.abs(y64).max()
     print("FP32 transform: %d points, max err / max %.2e" % (y64.size, err))
     assert err <= 1e-5
+
+
+# (functID, v1out, v2out, dvout, hwhm, preBoxHW, deconvPreBox), as pinned against the
+# reference in tests/test_oracle_scan.py
+SCAN_MODES = [
+    (1, 620., 780., 0., 0.05, -1., 0),      # boxcar scan
+    (1, 620., 780., 0.03, 0.2, -1., 0),
+    (9, 620., 780., 0., 0.6, -1., 0),       # Norton-Beer weak, medium, strong
+    (10, 620., 780., 0.05, 0.6, -1., 0),
+    (11, 620., 780., 0., 0.6, -1., 0),
+    (2, 690., 710., 0., 1.0, -1., 0),       # below CRATIO: transform of the scanning function
+    (3, 698., 702., 0., 1.0, -1., 0),
+    (3, 698., 702., 0.01, 1.0, 0.02, 1),
+    (5, 620., 780., 0., 0.6, 0.05, 1),      # boxcar pre-scan, with and without deconvolution
+    (5, 620., 780., 0., 0.6, 0.05, 0),
+    (3, 620., 780., 0.02, 0.3, 0.03, 1),
+    (105, 620., 780., 0., 0.6, 0.02, 0),    # pre-scanned elsewhere: deconvolution
+    # deconvolution only: without an apodization the factor Z/sin(Z) is bounded only
+    # for a boxcar narrower than DV (here Z <= pi/2), else FP32 noise is amplified
+    (-1, 620., 780., 0., 0.6, 0.005, 0),
+]
+
+
+@pytest.mark.parametrize("fid,v1o,v2o,dvo,hw,pb,dec", SCAN_MODES)
+def test_scan_modes_match_oracle(ctx, fid, v1o, v2o, dvo, hw, pb, dec):
+    """FP64 within 1e-10 of the largest value (the boxcar scan, which has no transform,
+    bit for bit); the FP32 transform within 1e-5 of FP64."""
+    import torch
+    from clblm_amd._lib import FP32C, FP64
+    from oracle.py import fft_scan
+    sp = _spectrum()
+    ref, va, da = fft_scan(sp, 600.0, 0.01, v1o, v2o, dvo, fid, hw, 0.0, pb, dec)
+    x = torch.tensor(sp, dtype=torch.float64, device="cuda")
+    out = {}
+    for prec in (FP64, FP32C):
+        y, vb, db = ctx.fft_scan(x, 600.0, 0.01, v1o, v2o, dvo, fid, hw, 0.0, prec, pre_box_hw=pb,
+                                 deconv_pre_box=bool(dec))
+        torch.cuda.synchronize()
+        out[prec] = y.cpu().numpy()
+        assert out[prec].size == ref.size and vb == va and db == da
+    err = np.abs(out[FP64] - ref).max() / np.abs(ref).max()
+    err32 = np.abs(out[FP32C] - out[FP64]).max() / np.abs(out[FP64]).max()
+    print("fid %d: %d points, FP64 err %.2e, FP32 vs FP64 %.2e" % (fid, ref.size, err, err32))
+    assert err <= 1e-10 and err32 <= 1e-5
+    if fid == 1 and dvo == 0.:
+        assert np.array_equal(out[FP64], ref)
+
+
+@pytest.mark.parametrize("fid,pb,fill", [(12, -1., 0.), (13, -1., 0.), (14, -1., 0.), (5, -1., 1.5)])
+def test_scan_rejects_what_the_reference_cannot_run(ctx, fid, pb, fill):
+    import torch
+    from clblm_amd._lib import ClblmError, FP64
+    x = torch.ones(20001, dtype=torch.float64, device="cuda")
+    with pytest.raises(ClblmError):
+        ctx.fft_scan(x, 600.0, 0.01, 620.0, 780.0, 0.0, fid, 0.6, fill, FP64, pre_box_hw=pb)
