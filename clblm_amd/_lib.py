@@ -14,7 +14,8 @@ FP64, FP32C = 0, 1
 EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
            "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
-           "clblm_timing", "clblm_work"]
+           "clblm_timing", "clblm_work", "clblm_tuning"]
+TUNE_F4_XCD_CHUNK = 1
 
 
 class Layer(ctypes.Structure):
@@ -81,6 +82,7 @@ def load():
         [ctypes.POINTER(ctypes.c_int64)]
     lib.clblm_work.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.clblm_tuning.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     _lib = lib
     return lib
 

@@ -987,6 +987,28 @@ static_assert(offsetof(F4Rec32, jc) == 16 && offsetof(F4Rec32, kind) == 28 && of
 static_assert(offsetof(F4Rec64, jmin) == 88 && offsetof(F4Rec64, jqlo) == 104 && offsetof(F4Rec64, kind) == 112,
               "F4Rec64 control words");
 
+// CNVF4Q's grid coordinate (clblm_ODLAY_LineF4.f90:356,426,464) is accumulated per
+// line: XJJ = (JMIN-1) DVR4, then XJJ = XJJ + DVR4 at every point.  The rounding of
+// that sum drifts from (j-1) DVR4 by hundreds of ulps along a 2500-point block.
+// Every line's sequence takes the same steps as the one that starts from 0 at
+// j = 1 (the step is fixed inside a binade), so XJJ(j) = S(j) + (X0 - S(JMIN))
+// to within one ulp, where S is that shared sequence.  S is tabulated per layer
+// (XJJ_N values, S(j) at index j-1) by the sequential sum itself, one lane per
+// layer; the records carry XNUI - (X0 - S(JMIN)), so that x = S(j) - XNUI' in the
+// gathers.
+constexpr int XJJ_N = LIMOUT4 + 64 * 4;   // covers every point of the last tile
+__global__ void __launch_bounds__(64) k_xjj(const Job *jobs, int nlay, double *xjj) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlay) return;
+    const double d = jobs[l].dvr4;
+    double *S = xjj + (int64_t)l * XJJ_N;
+    double t = 0.;
+    for (int j = 0; j < XJJ_N; j++) {
+        S[j] = t;
+        t = t + d;
+    }
+}
+
 // Record offsets of each block (one lane per layer walks its blocks in order).
 __global__ void k_f4recoff(const Job *jobs, int nlay, R4Blk *blks, int *err) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1011,12 +1033,20 @@ __device__ inline int f4_kind(const F4Line &d) {
     return ((d.flags & 1) ? F4_CO2 : 0) | ((d.flags & 2) ? 0 : F4_SHIFT) | (d.jmaxm >= d.jminm ? F4_MIRROR : 0);
 }
 
+// XNUI' = XNUI - (X0 - S(JMIN)) with X0 = (JMIN-1) DVR4 (see k_xjj)
+__device__ inline double xnui_seq(double xnui, int jmin, const double *S, const Job &J) {
+    const int i = jmin - 1 < XJJ_N - 1 ? jmin - 1 : XJJ_N - 1;
+    return xnui - ((double)(jmin - 1) * J.dvr4 - S[i]);
+}
+
 template <typename R>
-__device__ inline void f4_fill(R &r, const F4Line &d, const Job &J);
+__device__ inline void f4_fill(R &r, const F4Line &d, const Job &J, const double *S);
 template <>
-__device__ inline void f4_fill<F4Rec64>(F4Rec64 &r, const F4Line &d, const Job &J) {
+__device__ inline void f4_fill<F4Rec64>(F4Rec64 &r, const F4Line &d, const Job &J, const double *S) {
     const bool co2 = (d.flags & 1) != 0;
-    r.xnui = d.xnui; r.xnum = d.xnum; r.sil = d.sil; r.alfli2 = d.alfli2; r.rec_alfvi2 = d.rec_alfvi2;
+    r.xnui = d.jmax >= d.jmin ? xnui_seq(d.xnui, d.jmin, S, J) : d.xnui;
+    r.xnum = d.jmaxm >= d.jminm ? xnui_seq(d.xnum, d.jminm, S, J) : d.xnum;
+    r.sil = d.sil; r.alfli2 = d.alfli2; r.rec_alfvi2 = d.rec_alfvi2;
     r.siv_a3 = d.siv_a3; r.siv_b3 = d.siv_b3; r.f4bnd = d.f4bnd;
     r.c2 = co2 ? 2. : 1.;
     r.rb = co2 ? 1. / 625. : 0.;
@@ -1029,7 +1059,7 @@ __device__ inline void f4_fill<F4Rec64>(F4Rec64 &r, const F4Line &d, const Job &
     r.pad2 = 0.;
 }
 template <>
-__device__ inline void f4_fill<F4Rec32>(F4Rec32 &r, const F4Line &d, const Job &J) {
+__device__ inline void f4_fill<F4Rec32>(F4Rec32 &r, const F4Line &d, const Job &J, const double *S) {
     const bool co2 = (d.flags & 1) != 0;
     r.jmin = d.jmin; r.jmax = d.jmax; r.jminm = d.jminm; r.jmaxm = d.jmaxm;
     r.kind = f4_kind(d);
@@ -1039,31 +1069,36 @@ __device__ inline void f4_fill<F4Rec32>(F4Rec32 &r, const F4Line &d, const Job &
     r.b0 = (float)(co2 ? 2. * d.f4bnd : d.f4bnd);
     r.b1 = (float)(co2 ? -d.f4bnd / 625. : 0.);
     r.dptrat = (d.flags & 2) ? 0.f : (float)d.dptrat;
-    // nearest grid point and the FP64 offset to it: x(j) = (j-1)*dvr4 - xnui = (j-jc)*dvr4 + d
+    // nearest grid point and the FP64 offset to it (on the accumulated grid coordinate):
+    // x(j) = S(j) - xnui' = (j-jc)*dvr4 + d
     if (d.jmax >= d.jmin) {
         int jc = (int)floor(d.xnui / J.dvr4 + 0.5) + 1;
+        const int ic = min(max(jc - 1, 0), XJJ_N - 1);
         r.jc = jc;
-        r.d = (float)((double)(jc - 1) * J.dvr4 - d.xnui);
+        r.d = (float)((S[ic] + (double)(jc - 1 - ic) * J.dvr4) - xnui_seq(d.xnui, d.jmin, S, J));
         f4_qzone(d, J, &r.jqlo, &r.jqhi);
     } else { r.jc = 0; r.d = 0.f; r.jqlo = 1; r.jqhi = 0; }
     if (d.jmaxm >= d.jminm) {
         int jc = (int)floor(d.xnum / J.dvr4 + 0.5) + 1;
+        const int ic = min(max(jc - 1, 0), XJJ_N - 1);
         r.jcm = jc;
-        r.dm = (float)((double)(jc - 1) * J.dvr4 - d.xnum);
+        r.dm = (float)((S[ic] + (double)(jc - 1 - ic) * J.dvr4) - xnui_seq(d.xnum, d.jminm, S, J));
     } else { r.jcm = 0; r.dm = 0.f; }
 }
 
 template <typename R>
-__global__ void k_f4setup(const Job *jobs, const R4Blk *blks, int64_t nblk, Shrunk sh, R *recs) {
+__global__ void k_f4setup(const Job *jobs, const R4Blk *blks, int64_t nblk, Shrunk sh, R *recs,
+                          const double *xjj) {
     int64_t b = blockIdx.y;
     if (b >= nblk) return;
     const R4Blk &B = blks[b];
     const Job &J = jobs[B.layer];
+    const double *S = xjj + (int64_t)B.layer * XJJ_N;
     for (int64_t i = B.lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < B.hi;
          i += (int64_t)gridDim.x * blockDim.x) {
         F4Line d;
         f4_line_setup(d, J, B, sh.vnu[i], sh.sp[i], sh.spp[i], sh.alfa[i], sh.epp[i], sh.co2[i]);
-        f4_fill<R>(recs[B.rec + (i - B.lo)], d, J);
+        f4_fill<R>(recs[B.rec + (i - B.lo)], d, J, S);
     }
 }
 
@@ -1141,20 +1176,35 @@ __device__ __forceinline__ void f4_plan(F4Plan &P, bool live, int jmin, int jmax
 // (one coalesced load per lane, the next batch in flight while this one is summed)
 // and are read back as broadcasts of plain 16-byte vectors (so that no copy of a
 // record is made in private memory).
+// XCD-aware tile order.  Workgroups are dispatched to the 8 XCDs round-robin
+// (workgroup g runs on XCD g % 8), and each XCD has its own L2.  Neighbouring
+// tiles read nearly the same line records, so runs of `chunk` consecutive tiles go
+// to one XCD (chunks dealt round-robin, which keeps the XCDs' loads even).  The map
+// is a bijection of [0, 8 chunk ceil(n / (8 chunk))); indices >= n exit.
+__device__ __forceinline__ int64_t xcd_tile(int64_t g, int chunk) {
+    const int64_t x = g & 7, s = g >> 3;
+    return ((s / chunk) * 8 + x) * chunk + (s % chunk);
+}
+
 template <int F4K>
 __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
                                                  const F4Rec64 *__restrict__ recs, double *R4,
-                                                 unsigned long long *work) {
-    const int2 t = tiles[blockIdx.x];
+                                                 unsigned long long *work, int64_t ntile, int chunk,
+                                                 const double *__restrict__ xjj) {
+    const int64_t ti = xcd_tile(blockIdx.x, chunk);
+    if (ti >= ntile) return;
+    const int2 t = tiles[ti];
     const R4Blk &B = blks[t.x];
     const Job &J = jobs[B.layer];
     const int lane = threadIdx.x;
     const int j0 = t.y * 64 * F4K + 1, j1 = j0 + 64 * F4K - 1;
-    double acc[F4K], xj[F4K];
+    double acc[F4K], xj[F4K];   // xj: the accumulated grid coordinate S(j) (k_xjj)
+    const double *S = xjj + (int64_t)B.layer * XJJ_N;
 #pragma unroll
     for (int k = 0; k < F4K; k++) {
         acc[k] = 0.;
-        xj[k] = (double)(j0 + lane + 64 * k - 1) * J.dvr4;
+        const int i = j0 + lane + 64 * k - 1;
+        xj[k] = S[i < XJJ_N ? i : XJJ_N - 1];
     }
     unsigned long long nev = 0;   // (line, point) pairs of this tile, one record per lane
     if (j0 <= B.nptr4) {
@@ -1248,8 +1298,10 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
 template <int F4K>
 __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
                                                  const F4Rec32 *__restrict__ recs, double *R4,
-                                                 unsigned long long *work) {
-    const int2 t = tiles[blockIdx.x];
+                                                 unsigned long long *work, int64_t ntile, int chunk) {
+    const int64_t ti = xcd_tile(blockIdx.x, chunk);
+    if (ti >= ntile) return;
+    const int2 t = tiles[ti];
     const R4Blk &B = blks[t.x];
     const Job &J = jobs[B.layer];
     const int lane = threadIdx.x;
@@ -1838,6 +1890,12 @@ struct Batch {
 struct clblm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // The F4 wing chain (LINF4Q, SHRINQ, CNVF4Q) and pass 1 (CNVFNQ) read disjoint
+    // inputs and write disjoint buffers until pass 2 combines them, so a batch forks
+    // the F4 chain onto this stream (default priority) and joins it before pass 2: the
+    // latency-bound SHRINQ walks then run beside the pass-1 convolution.
+    hipStream_t f4stream = nullptr;
+    int f4_chunk = 16;       // consecutive F4 tiles per XCD run (xcd_tile)
     double *tab = nullptr;   // F1,F2,F3,FG,XVER consecutive, NFMX+1 each
     float *tabs = nullptr;
     double4 *tabg = nullptr;
@@ -1853,8 +1911,13 @@ struct clblm_ctx {
     // the end of the call).  A slot is read back when it is reused, by clblm_sync,
     // or by the timing queries; a device-detected error is returned by the next
     // call on the context.
+    // Stage events: 0 start, 1 after the line prep, 2 after pass 1 (main stream);
+    // 7 start of the F4 chain, 5/6 around the F4 gather, 3 end of the F4 chain (F4
+    // stream); 8 after the join, 4 end (main stream).  fork/join order the F4 stream
+    // inside the batch.
     struct Slot {
-        hipEvent_t ev[7]{};
+        hipEvent_t ev[9]{};
+        hipEvent_t fork = nullptr, join = nullptr;
         hipEvent_t done = nullptr;
         bool pending = false;
     };
@@ -1892,8 +1955,8 @@ static void finish_slot(clblm_ctx *C, int k, bool wait) {
     float t01 = 0, t12 = 0, t23 = 0, t34 = 0, t56 = 0;
     (void)hipEventElapsedTime(&t01, S.ev[0], S.ev[1]);
     (void)hipEventElapsedTime(&t12, S.ev[1], S.ev[2]);
-    (void)hipEventElapsedTime(&t23, S.ev[2], S.ev[3]);
-    (void)hipEventElapsedTime(&t34, S.ev[3], S.ev[4]);
+    (void)hipEventElapsedTime(&t23, S.ev[7], S.ev[3]);
+    (void)hipEventElapsedTime(&t34, S.ev[8], S.ev[4]);
     (void)hipEventElapsedTime(&t56, S.ev[5], S.ev[6]);
     C->ms[0] += t01; C->ms[1] += t12; C->ms[2] += t23; C->ms[3] += t34;
     C->ms_conv_k += t12;
@@ -2227,6 +2290,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_ch = cv.take<int4>((size_t)B.tot_rec);
     size_t o_be = cv.take<int64_t>((size_t)B.tot_rec);
     size_t o_R4 = cv.take<double>((size_t)B.tot_r4);
+    size_t o_xjj = cv.take<double>((size_t)nlay * XJJ_N);
     size_t o_rec = precision == CLBLM_FP64 ? cv.take<F4Rec64>((size_t)B.tot_f4rec) : cv.take<F4Rec32>((size_t)B.tot_f4rec);
     size_t o_nat = cv.take<double>(std::max<int64_t>(1, B.tot_nat));
     size_t o_rg = cv.take<int>(1);
@@ -2331,6 +2395,58 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     const int64_t npan = (int64_t)B.panels.size();
     const int64_t nblk = (int64_t)B.blks.size();
 
+    // fork: the F4 chain runs on the F4 stream from here to the join before pass 2
+    const hipStream_t s4 = C->f4stream;
+    HIPCHK(hipEventRecord(S.fork, st));
+    HIPCHK(hipStreamWaitEvent(s4, S.fork, 0));
+    HIPCHK(hipEventRecord(S.ev[7], s4));
+    if (B.max_f4 > 0) {
+        hipLaunchKernelGGL(k_f4prep, dim3(cdiv(B.max_f4, 256), nlay), dim3(256), 0, s4, d_jobs, L.dev, f, d_err);
+        hipLaunchKernelGGL(k_reccount, dim3(cdiv(B.max_rec, 256), nlay), dim3(256), 0, s4, d_jobs, f);
+        hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, s4, d_jobs, f, sh);
+        hipLaunchKernelGGL(k_sim, dim3(cdiv(nlay, 64)), dim3(64), 0, s4, d_jobs, nlay, f);
+        hipLaunchKernelGGL(k_emit, dim3(cdiv(std::max<int64_t>(1, B.tot_rec), 64)), dim3(64), 0, s4, d_jobs, f, sh,
+                           B.tot_rec);
+        double *ftot = at<double>(base, o_ftot);
+        if (fcap > 0) {
+            hipLaunchKernelGGL(k_shr_local, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, s4, d_jobs, f, sh, ftot,
+                               fcap);
+            hipLaunchKernelGGL(k_shr_fix, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, s4, d_jobs, f, sh, ftot,
+                               fcap);
+        }
+        hipLaunchKernelGGL(k_shr_blk9, dim3(cdiv(nlay, 64)), dim3(64), 0, s4, d_jobs, nlay, f, sh);
+    } else {
+        HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, s4));
+    }
+    double *d_xjj = at<double>(base, o_xjj);
+    hipLaunchKernelGGL(k_xjj, dim3(cdiv(nlay, 64)), dim3(64), 0, s4, d_jobs, nlay, d_xjj);
+    hipLaunchKernelGGL(k_r4range, dim3(cdiv(nblk, 256)), dim3(256), 0, s4, d_jobs, d_blk, nblk, f, sh);
+    hipLaunchKernelGGL(k_f4recoff, dim3(cdiv(nlay, 64)), dim3(64), 0, s4, d_jobs, nlay, d_blk, d_err);
+    {
+        unsigned gx = std::max<unsigned>(1, cdiv(B.max_f4 * 2, 256 * 8));
+        if (precision == CLBLM_FP64)
+            hipLaunchKernelGGL(k_f4setup<F4Rec64>, dim3(gx, (unsigned)nblk), dim3(256), 0, s4, d_jobs, d_blk, nblk,
+                               sh, at<F4Rec64>(base, o_rec), d_xjj);
+        else
+            hipLaunchKernelGGL(k_f4setup<F4Rec32>, dim3(gx, (unsigned)nblk), dim3(256), 0, s4, d_jobs, d_blk, nblk,
+                               sh, at<F4Rec32>(base, o_rec), d_xjj);
+    }
+    HIPCHK(hipEventRecord(S.ev[5], s4));
+    const int64_t ntile = (int64_t)B.tiles.size();
+    const int xchunk = C->f4_chunk > 0 ? C->f4_chunk : 1;
+    const unsigned f4grid = (unsigned)(8 * (int64_t)xchunk * ((ntile + 8 * xchunk - 1) / (8 * xchunk)));
+    if (ntile > 0) {
+        if (precision == CLBLM_FP64)
+            hipLaunchKernelGGL(k_f4conv64<F4K64>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,
+                               at<F4Rec64>(base, o_rec), d_R4, C->d_work, ntile, xchunk, d_xjj);
+        else
+            hipLaunchKernelGGL(k_f4conv32<F4K32>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,
+                               at<F4Rec32>(base, o_rec), d_R4, C->d_work, ntile, xchunk);
+    }
+    HIPCHK(hipEventRecord(S.ev[6], s4));
+    if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, s4, d_jobs, d_blk, d_R4);
+    HIPCHK(hipEventRecord(S.ev[3], s4));
+    HIPCHK(hipEventRecord(S.join, s4));
     HIPCHK(hipEventRecord(S.ev[0], st));
     if (B.max_line > 0) {
         hipLaunchKernelGGL(k_prep, dim3(cdiv(B.max_line, 256), nlay), dim3(256), 0, st, d_jobs, d_pan, L.dev, lp, d_err);
@@ -2348,46 +2464,8 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
                            d_jobs, d_pan, lp, T, d_A, C->d_work);
     }
     HIPCHK(hipEventRecord(S.ev[2], st));
-    if (B.max_f4 > 0) {
-        hipLaunchKernelGGL(k_f4prep, dim3(cdiv(B.max_f4, 256), nlay), dim3(256), 0, st, d_jobs, L.dev, f, d_err);
-        hipLaunchKernelGGL(k_reccount, dim3(cdiv(B.max_rec, 256), nlay), dim3(256), 0, st, d_jobs, f);
-        hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, st, d_jobs, f, sh);
-        hipLaunchKernelGGL(k_sim, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, f);
-        hipLaunchKernelGGL(k_emit, dim3(cdiv(std::max<int64_t>(1, B.tot_rec), 64)), dim3(64), 0, st, d_jobs, f, sh,
-                           B.tot_rec);
-        double *ftot = at<double>(base, o_ftot);
-        if (fcap > 0) {
-            hipLaunchKernelGGL(k_shr_local, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, st, d_jobs, f, sh, ftot,
-                               fcap);
-            hipLaunchKernelGGL(k_shr_fix, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, st, d_jobs, f, sh, ftot,
-                               fcap);
-        }
-        hipLaunchKernelGGL(k_shr_blk9, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, f, sh);
-    } else {
-        HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, st));
-    }
-    hipLaunchKernelGGL(k_r4range, dim3(cdiv(nblk, 256)), dim3(256), 0, st, d_jobs, d_blk, nblk, f, sh);
-    hipLaunchKernelGGL(k_f4recoff, dim3(cdiv(nlay, 64)), dim3(64), 0, st, d_jobs, nlay, d_blk, d_err);
-    {
-        unsigned gx = std::max<unsigned>(1, cdiv(B.max_f4 * 2, 256 * 8));
-        if (precision == CLBLM_FP64)
-            hipLaunchKernelGGL(k_f4setup<F4Rec64>, dim3(gx, (unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, nblk,
-                               sh, at<F4Rec64>(base, o_rec));
-        else
-            hipLaunchKernelGGL(k_f4setup<F4Rec32>, dim3(gx, (unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, nblk,
-                               sh, at<F4Rec32>(base, o_rec));
-    }
-    HIPCHK(hipEventRecord(S.ev[5], st));
-    const unsigned f4grid = (unsigned)B.tiles.size();
-    if (precision == CLBLM_FP64)
-        hipLaunchKernelGGL(k_f4conv64<F4K64>, dim3(f4grid), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
-                           at<F4Rec64>(base, o_rec), d_R4, C->d_work);
-    else
-        hipLaunchKernelGGL(k_f4conv32<F4K32>, dim3(f4grid), dim3(64), 0, st, d_jobs, d_blk, d_tiles, sh,
-                           at<F4Rec32>(base, o_rec), d_R4, C->d_work);
-    HIPCHK(hipEventRecord(S.ev[6], st));
-    if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, st, d_jobs, d_blk, d_R4);
-    HIPCHK(hipEventRecord(S.ev[3], st));
+    HIPCHK(hipStreamWaitEvent(st, S.join, 0));
+    HIPCHK(hipEventRecord(S.ev[8], st));
     hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
                        d_abs, d_out, d_nat);
     if (B.any_regrid) {
@@ -2455,8 +2533,11 @@ int clblm_ctx_create(int device, clblm_ctx **out) {
     HIPCHK(hipHostMalloc((void **)&c->h_err, clblm_ctx::NSLOT * sizeof(int), hipHostMallocDefault));
     HIPCHK(hipMalloc(&c->d_work, 2 * sizeof(unsigned long long)));
     HIPCHK(hipMemset(c->d_work, 0, 2 * sizeof(unsigned long long)));
+    HIPCHK(hipStreamCreateWithFlags(&c->f4stream, hipStreamNonBlocking));
     for (auto &S : c->slot) {
         for (auto &e : S.ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
         HIPCHK(hipEventCreate(&S.done));
     }
     for (auto &e : c->pin_ev) HIPCHK(hipEventCreate(&e));
@@ -2482,6 +2563,8 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     if (c->rt_state) (void)hipFree(c->rt_state);
     for (auto &S : c->slot) {
         for (auto &e : S.ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(S.fork);
+        (void)hipEventDestroy(S.join);
         (void)hipEventDestroy(S.done);
     }
     for (int i = 0; i < 2; i++) {
@@ -2490,6 +2573,7 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     }
     (void)hipHostFree(c->h_err);
     if (c->scan) clblm_scan_state_free(c->scan);
+    (void)hipStreamDestroy(c->f4stream);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -2602,6 +2686,23 @@ int clblm_od_batch(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
     rc = run_batch(c, *c->lists[handle], B, d_od, st, prec);
     if (rc) return rc;
     *jrad_state = jr;
+    return 0;
+}
+
+int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value) {
+    if (!c) return set_err(CLBLM_ERR_ARG, "ctx is NULL");
+    int *slot = nullptr;
+    int lo = 0, hi = 0;
+    switch (key) {
+    case CLBLM_TUNE_F4_XCD_CHUNK: slot = &c->f4_chunk; lo = 1; hi = 4096; break;
+    default: return set_err(CLBLM_ERR_ARG, "unknown tuning key");
+    }
+    if (old_value) *old_value = *slot;
+    if (value == 0) return 0;   // query only
+    if (value < lo || value > hi) return set_err(CLBLM_ERR_ARG, "tuning value out of range");
+    std::lock_guard<std::mutex> g(c->mu);
+    *slot = value;
+    c->plan_valid = false;
     return 0;
 }
 

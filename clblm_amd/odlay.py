@@ -141,6 +141,13 @@ class Context:
                                       ctypes.byref(n), ctypes.byref(a), ctypes.byref(b), ctypes.c_void_p(stream)))
         return out, a.value, b.value
 
+    def tuning(self, key, value=0):
+        """Sets launch-shape parameter ``key`` (``_lib.TUNE_*``) to ``value`` (0: query
+        only); returns the previous setting."""
+        old = ctypes.c_int()
+        check(self.lib.clblm_tuning(self.ptr, int(key), int(value), ctypes.byref(old)))
+        return old.value
+
     def sync(self):
         """Wait for the enqueued batches; raises the first error found on the device."""
         check(self.lib.clblm_sync(self.ptr))

@@ -215,3 +215,50 @@ def continuum_absrb(layer: Layer, v1: float, v2: float, seed=1001, nknots=50):
     spl = CubicSpline(knots, vals, bc_type="natural")
     grid = v1abs + np.arange(npt, dtype=np.float64)
     return (10.0 ** spl(grid)) * layer.w.get(1, 0.0), v1abs, npt
+
+
+# --------------------------------------------------------------------------
+# Batches of scenes: uneven layer counts, perturbed temperatures, scaled water vapour.
+# --------------------------------------------------------------------------
+@dataclasses.dataclass
+class Scene:
+    layers: List[Layer]
+    t_noise: np.ndarray     # K, added to each layer's mean temperature
+    h2o_scale: float        # factor on the water-vapour column
+
+
+def scene_batch(n_scenes=256, seed=1003, n_min=40, n_max=80, t_sigma=3.0, h2o_lo=0.1, h2o_hi=10.0,
+                p_top=0.1) -> List[Scene]:
+    """Scenes with N_lay uniform in {n_min..n_max}, layer temperatures perturbed by
+    seeded Gaussian noise of ``t_sigma`` K and water-vapour columns scaled
+    log-uniformly in [h2o_lo, h2o_hi]; each scene's layers span 1013 hPa to p_top."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_scenes):
+        nl = int(rng.integers(n_min, n_max + 1))
+        noise = rng.normal(0.0, t_sigma, nl)
+        scale = float(10.0 ** rng.uniform(math.log10(h2o_lo), math.log10(h2o_hi)))
+        out.append(Scene(layers=atmosphere(nl, p_top, t_noise=noise, h2o_scale=scale), t_noise=noise,
+                         h2o_scale=scale))
+    return out
+
+
+# --------------------------------------------------------------------------
+# Large single scene: lines over 10-3500 cm-1, 80 layers to 0.01 hPa.
+# --------------------------------------------------------------------------
+LARGE_V1, LARGE_V2 = 10.0, 3500.0
+
+
+def large_scene_lines(n_lines=2000000, v1=LARGE_V1, v2=LARGE_V2, seed=1004):
+    """Lines of molecules 1-7 over the large scene's range (2 000 000 by default)."""
+    return make_lines(n_lines, v1, v2, seed, mols=tuple(m for m, _, _ in SPECIES))
+
+
+def large_scene_atmosphere():
+    """The large scene's 80 layers from 1013 hPa to 0.01 hPa."""
+    return atmosphere(80, 0.01)
+
+
+def large_scene_dv(layer: Layer) -> float:
+    """Per-layer DV of the large scene: calculateDV at the range's mid point."""
+    return calculate_dv(0.5 * (LARGE_V1 + LARGE_V2), layer.pave, layer.tave, layer.w.get(1, 0.0) / layer.wtot)

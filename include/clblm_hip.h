@@ -101,6 +101,13 @@ int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
                    const clblm_od_params *pars, int32_t *jrad_state, int64_t *offsets,
                    double *d_od, void *stream);
 
+/* Launch-shape tuning of a context (no counterpart in the reference, whose
+ * loops are sequential).  key selects the parameter; value 0 only queries it;
+ * *old_value (if non-NULL) receives the setting before the call.
+ *   CLBLM_TUNE_F4_XCD_CHUNK  consecutive F4 tiles given to one XCD (1-4096). */
+#define CLBLM_TUNE_F4_XCD_CHUNK 1
+int clblm_tuning(clblm_ctx *ctx, int key, int value, int *old_value);
+
 /* Waits for every batch enqueued on the context and returns the first device-found
  * error among them (0 if none). */
 int clblm_sync(clblm_ctx *ctx);

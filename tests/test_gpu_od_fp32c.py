@@ -44,3 +44,43 @@ def test_profile_layers(ctx, k):
     e, n = _both(ctx, h, lay, 600.0, 700.0, dv, continuum=cont)
     print("layer %d fp32c: %d points, max rel err %.3e" % (k, n, e))
     assert e <= RTOL
+
+
+def test_scene_batch_fp32c_vs_fp64(ctx):
+    """The scene batch (four scenes, 207 layers, one batch each) in FP32C against FP64."""
+    import torch
+    from clblm_amd._lib import FP32C, FP64
+    from clblm_amd.odlay import Context
+    scenes = W.scene_batch(4, seed=1003)
+    lines = W.profile_lines(40000, 500.0, 800.0, seed=1001)
+    layers = [l for s in scenes for l in s.layers]
+    dvs = [W.calculate_dv(650.0, l.pave, l.tave, l.w[1] / l.wtot) for l in layers]
+    h = ctx.upload_lines(lines)
+    res = {}
+    for p in (FP64, FP32C):
+        ctx.jrad = 1
+        out, off = ctx.od_batch(h, layers, [Context.params(600.0, 700.0, dv, precision=p) for dv in dvs])
+        torch.cuda.synchronize()
+        res[p] = out.cpu().numpy()
+    e = float((np.abs(res[FP32C] - res[FP64]) / np.abs(res[FP64])).max())
+    print("scene batch fp32c: %d layers, %d points, max rel err %.3e" % (len(layers), off[-1], e))
+    assert e <= RTOL
+
+
+def test_low_wavenumber_fp32c(ctx):
+    """Large-scene layers over 10-60 cm-1 (Van Vleck-Weisskopf mirrors, RADFNI ramp) in
+    FP32C against FP64; the denominator is floored at 1e-4 of the maximum where the
+    spectrum crosses zero."""
+    from clblm_amd._lib import FP32C, FP64
+    lines = W.make_lines(50000, 10.0, 100.0, seed=1004, mols=(1, 2, 3, 4, 5, 6, 7))
+    h = ctx.upload_lines(lines)
+    for k in (0, 40, 79):
+        lay = W.large_scene_atmosphere()[k]
+        dv = W.large_scene_dv(lay)
+        ctx.jrad = 1
+        a, _ = ctx.odlay(h, lay, 10.0, 60.0, dv, precision=FP64)
+        ctx.jrad = 1
+        b, _ = ctx.odlay(h, lay, 10.0, 60.0, dv, precision=FP32C)
+        e = float((np.abs(b - a) / np.maximum(np.abs(a), 1e-4 * np.abs(a).max())).max())
+        print("low-wavenumber layer %d fp32c: %d points, max rel err %.3e" % (k, a.size, e))
+        assert e <= RTOL

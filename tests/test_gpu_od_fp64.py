@@ -148,3 +148,65 @@ def test_batches_queue_without_host_waits(ctx):
     ctx.sync()
     for o in outs:
         np.testing.assert_allclose(o.cpu().numpy(), ref, rtol=1e-13, atol=0)
+
+
+def _cmp_floor(od, ref, rtol=RTOL, floor=1e-4):
+    """Relative error with the denominator floored at ``floor`` times the spectrum's
+    maximum, for spectra that cross zero: at low wavenumber the approximate Voigt
+    minus F4BND dips slightly below zero between lines, and near the crossings the
+    OD is a difference of terms ~1e4 times larger, whose FP64 rounding (~1e-15 of
+    the maximum) is all a relative measure would see."""
+    assert od.shape == ref.shape, (od.shape, ref.shape)
+    den = np.maximum(np.abs(ref), floor * np.abs(ref).max())
+    err = np.abs(od - ref) / den
+    i = int(np.argmax(err))
+    assert err[i] <= rtol, "max rel err %.3e at %d (gpu %r ref %r)" % (err[i], i, od[i], ref[i])
+    return float(err.max())
+
+
+@pytest.mark.parametrize("k", [0, 25, 40, 79])
+def test_low_wavenumber_large_scene_layers(ctx, k):
+    """Layers of the large scene (80 layers to 0.01 hPa, DV from the 10-3500 cm-1 mid
+    point) over 10-60 cm-1 with its line density (50 000 lines over 10-100 cm-1).
+    Lines below 25 cm-1 add their Van Vleck-Weisskopf mirror images on R4
+    (clblm_ODLAY_LineF4.f90:468-476); V1/kT < 5 turns JRAD to 0, so the radiation
+    term is the panel-wise RADFNI ramp (clblm_ODLAY.f90:877-878, 2449-2483)."""
+    from oracle.py import Oracle
+    lines = W.make_lines(50000, 10.0, 100.0, seed=1004, mols=(1, 2, 3, 4, 5, 6, 7))
+    lay = W.large_scene_atmosphere()[k]
+    dv = W.large_scene_dv(lay)
+    h = ctx.upload_lines(lines)
+    ctx.jrad = 1
+    od, _ = ctx.odlay(h, lay, 10.0, 60.0, dv)
+    orc = Oracle(lines)
+    ref, _ = orc.odlay(lay, 10.0, 60.0, dv)
+    assert ctx.jrad == orc.jrad == 0
+    print("low-wavenumber layer %d p=%.3g dv=%.3g: %d points, max rel err %.3e" % (k, lay.pave, dv, od.size,
+                                                                                 _cmp_floor(od, ref)))
+
+
+def test_scene_batch_matches_oracle(ctx):
+    """Four scenes of the scene batch (40-80 layers each, temperatures perturbed by
+    3 K noise, water vapour scaled by 0.1-10) as one batch of 207 layers; every layer
+    against the oracle called layer after layer (JRAD carried as the reference's
+    SAVE variable)."""
+    from clblm_amd.odlay import Context
+    from oracle.py import Oracle
+    import torch
+    scenes = W.scene_batch(4, seed=1003)
+    lines = W.profile_lines(20000, 500.0, 800.0, seed=1001)
+    layers = [l for s in scenes for l in s.layers]
+    dvs = [W.calculate_dv(650.0, l.pave, l.tave, l.w[1] / l.wtot) for l in layers]
+    h = ctx.upload_lines(lines)
+    ctx.jrad = 1
+    out, off = ctx.od_batch(h, layers, [Context.params(640.0, 660.0, dv) for dv in dvs])
+    torch.cuda.synchronize()
+    host = out.cpu().numpy()
+    orc = Oracle(lines)
+    orc.jrad = 1
+    worst = 0.0
+    for k, (lay, dv) in enumerate(zip(layers, dvs)):
+        ref, _ = orc.odlay(lay, 640.0, 660.0, dv)
+        worst = max(worst, _cmp(host[off[k]:off[k + 1]], ref))
+    assert ctx.jrad == orc.jrad
+    print("scene batch: %d layers, %d points, max rel err %.3e" % (len(layers), off[-1], worst))
