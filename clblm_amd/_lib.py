@@ -16,6 +16,7 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
            "clblm_timing", "clblm_work", "clblm_tuning"]
 TUNE_F4_XCD_CHUNK = 1
+TUNE_F4_FAR_FIELD = 2
 
 
 class Layer(ctypes.Structure):

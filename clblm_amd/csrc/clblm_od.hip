@@ -1137,6 +1137,15 @@ __device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dpt
 constexpr int F4K32 = 3, F4K64 = 2;
 // F4 records summed in FP32 before each FP64 accumulation (FP32 gather)
 constexpr int F4CHUNK = 16;
+// Far field of the FP32 gather.  A record whose line centre lies at least F4DFAR
+// points outside the tile, and whose shape over the whole tile is the plain Lorentz
+// form (covered, no shift, no mirror), is a smooth function there: it is evaluated
+// at F4NF Chebyshev nodes of the tile instead of at every point, and the node sums
+// are interpolated to the points once per tile (Lagrange weights tabulated in the
+// context, F4Far).  For 1/(a^2 + x^2) with its centre F4DFAR = half a tile outside
+// the tile, 16 nodes interpolate to within 1.4e-8 of its maximum over the tile.
+constexpr int F4NF = 16;
+constexpr int F4DFAR = 32 * F4K32;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int RAMP_CAP = 256;  // ramp pieces per panel held in LDS
 
@@ -1154,19 +1163,24 @@ __device__ __forceinline__ unsigned long long f4_count(int jmin, int jmax, int j
 //          Lorentz form alone gives the shape there;
 //   all    cover and lorp: no per-point test at all.
 struct F4Plan {
-    unsigned long long all, cover, lorp, touch, co2, mirror;
+    unsigned long long all, cover, lorp, touch, co2, mirror, far;
 };
+// far (FP32 gather only, dfar > 0): an `all` record without a mirror whose centre
+// point jc lies at least dfar points outside the tile; it leaves every other set.
 __device__ __forceinline__ void f4_plan(F4Plan &P, bool live, int jmin, int jmax, int jqlo, int jqhi, int kind,
-                                        int j0, int j1) {
-    P.mirror = __ballot(live && (kind & F4_MIRROR) != 0);
+                                        int j0, int j1, int jc = 0, int dfar = 0) {
+    const bool mir = live && (kind & F4_MIRROR) != 0;
+    P.mirror = __ballot(mir);
     live = live && jmin <= jmax;
     const bool touch = live && jmin <= j1 && jmax >= j0;
     const bool cover = live && jmin <= j0 && jmax >= j1;
     const bool lorq = (jqhi < j0 || jqlo > j1) && (kind & F4_SHIFT) == 0;
-    P.touch = __ballot(touch);
-    P.cover = __ballot(cover);
-    P.lorp = __ballot(touch && lorq);
-    P.all = __ballot(cover && lorq);
+    const bool far = dfar > 0 && cover && lorq && !mir && (jc <= j0 - dfar || jc >= j1 + dfar);
+    P.touch = __ballot(touch && !far);
+    P.cover = __ballot(cover && !far);
+    P.lorp = __ballot(touch && lorq && !far);
+    P.far = __ballot(far);
+    P.all = __ballot(cover && lorq && !far);
     P.co2 = __ballot(live && (kind & F4_CO2) != 0);
 }
 
@@ -1298,7 +1312,8 @@ __global__ void __launch_bounds__(64) k_f4conv64(const Job *jobs, const R4Blk *b
 template <int F4K>
 __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
                                                  const F4Rec32 *__restrict__ recs, double *R4,
-                                                 unsigned long long *work, int64_t ntile, int chunk) {
+                                                 unsigned long long *work, int64_t ntile, int chunk,
+                                                 const double *__restrict__ far_w, const float *__restrict__ far_tau) {
     const int64_t ti = xcd_tile(blockIdx.x, chunk);
     if (ti >= ntile) return;
     const int2 t = tiles[ti];
@@ -1311,6 +1326,10 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
 #pragma unroll
     for (int k = 0; k < F4K; k++) acc[k] = 0.;
     unsigned long long nev = 0;   // (line, point) pairs of this tile, one record per lane
+    float node[F4NF];             // this lane's far-field sums at the tile's Chebyshev nodes
+#pragma unroll
+    for (int i = 0; i < F4NF; i++) node[i] = 0.f;
+    bool anyfar = false;
     if (j0 <= B.nptr4) {
         int64_t c0, c1;
         f4_candidates(J, B, sh, j0, min(j1, B.nptr4), &c0, &c1);
@@ -1327,8 +1346,24 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
         for (int64_t base = c0; base < c1; base += 64) {
             const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
             F4Plan P;
-            f4_plan(P, lane < nb, (int)pre[1].y, (int)pre[1].z, (int)pre[2].x, (int)pre[2].y, (int)pre[1].w, j0, j1);
+            f4_plan(P, lane < nb, (int)pre[1].y, (int)pre[1].z, (int)pre[2].x, (int)pre[2].y, (int)pre[1].w, j0, j1,
+                    (int)pre[1].x, far_tau ? F4DFAR : 0);
             if (lane < nb) nev += f4_count((int)pre[1].y, (int)pre[1].z, j0, j1);
+            if (P.far) {
+                anyfar = true;
+                if ((P.far >> lane) & 1ull) {  // this lane's own record, at the nodes
+                    const float d = __uint_as_float(pre[0].x), sil = __uint_as_float(pre[0].y);
+                    const float alfli2 = __uint_as_float(pre[0].z), b0 = __uint_as_float(pre[0].w);
+                    const float b1 = __uint_as_float(pre[2].z);
+                    const float off = (float)(j0 - (int)pre[1].x);   // exact
+#pragma unroll
+                    for (int i = 0; i < F4NF; i++) {
+                        const float xm = __fmaf_rn(off + far_tau[i], dvr4f, d), xmsq = xm * xm;
+                        const float f = __fmaf_rn(sil, __builtin_amdgcn_rcpf(alfli2 + xmsq), -b0);
+                        node[i] += __fmaf_rn(-xmsq, b1, f);
+                    }
+                }
+            }
             __syncthreads();
             if (lane < nb) {
 #pragma unroll
@@ -1340,13 +1375,18 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
 #pragma unroll
                 for (int w = 0; w < W; w++) pre[w] = r[k * W + w];
             }
-            for (int q0 = 0; q0 < nb; q0 += F4CHUNK) {
+            // only the records with work at the points: those that reach the tile and are
+            // not far, and the mirror images (a scalar bit scan, so far records and
+            // records that miss the tile cost nothing here)
+            unsigned long long need = P.touch | P.mirror;
+            while (need) {
             // FP32 partial sums over F4CHUNK records, then one FP64 accumulation
             float a32[F4K];
 #pragma unroll
             for (int k = 0; k < F4K; k++) a32[k] = 0.f;
-            const int qe = q0 + F4CHUNK < nb ? q0 + F4CHUNK : nb;
-            for (int q = q0; q < qe; q++) {
+            for (int c = 0; c < F4CHUNK && need; c++) {
+                const int q = __builtin_ctzll(need);
+                need &= need - 1;
                 const unsigned long long bit = 1ull << q;
                 const u32x4 *rq = &srec[q * W];
                 const u32x4 h0 = rq[0];
@@ -1411,6 +1451,24 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
 #pragma unroll
             for (int k = 0; k < F4K; k++) acc[k] += (double)a32[k];
             }
+        }
+    }
+    if (anyfar) {  // wave-uniform: sum the nodes over the lanes, interpolate to the points
+        double nd[F4NF];
+#pragma unroll
+        for (int i = 0; i < F4NF; i++) {
+            double v = (double)node[i];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            nd[i] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < F4K; k++) {
+            const double *w = far_w + (lane + 64 * k) * F4NF;
+            double s = 0.;
+#pragma unroll
+            for (int i = 0; i < F4NF; i++) s = fma(w[i], nd[i], s);
+            acc[k] += s;
         }
     }
 #pragma unroll
@@ -1896,6 +1954,9 @@ struct clblm_ctx {
     // latency-bound SHRINQ walks then run beside the pass-1 convolution.
     hipStream_t f4stream = nullptr;
     int f4_chunk = 16;       // consecutive F4 tiles per XCD run (xcd_tile)
+    int f4_far = 1;          // far-field evaluation in the FP32 gather: 1 on, 2 off
+    double *far_w = nullptr; // Lagrange weights of the far-field nodes, [64 F4K32][F4NF]
+    float *far_tau = nullptr;   // node positions, tile points from the tile's first point
     double *tab = nullptr;   // F1,F2,F3,FG,XVER consecutive, NFMX+1 each
     float *tabs = nullptr;
     double4 *tabg = nullptr;
@@ -2022,6 +2083,28 @@ static int ctx_tables(clblm_ctx *c) {
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cf3), CLBLM_CF3, sizeof(CLBLM_CF3)));
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cer), CLBLM_CER, sizeof(CLBLM_CER)));
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_maxiso), CLBLM_MAXISO, sizeof(CLBLM_MAXISO)));
+    {   // far-field nodes of the FP32 gather: Chebyshev points of [0, W-1] (rounded to
+        // FP32, as the kernel uses them) and the Lagrange weights of every tile point
+        const int W = 64 * F4K32;
+        const double PI = 3.14159265358979323846;
+        std::vector<float> tau(F4NF);
+        std::vector<double> td(F4NF), w((size_t)W * F4NF);
+        for (int i = 0; i < F4NF; i++) {
+            tau[i] = (float)(0.5 * (W - 1) * (1. + std::cos((2 * i + 1) * PI / (2 * F4NF))));
+            td[i] = (double)tau[i];
+        }
+        for (int t = 0; t < W; t++)
+            for (int i = 0; i < F4NF; i++) {
+                double v = 1.;
+                for (int k = 0; k < F4NF; k++)
+                    if (k != i) v *= ((double)t - td[k]) / (td[i] - td[k]);
+                w[(size_t)t * F4NF + i] = v;
+            }
+        HIPCHK(hipMalloc(&c->far_w, w.size() * sizeof(double)));
+        HIPCHK(hipMalloc(&c->far_tau, tau.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(c->far_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->far_tau, tau.data(), tau.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
@@ -2441,7 +2524,8 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
                                at<F4Rec64>(base, o_rec), d_R4, C->d_work, ntile, xchunk, d_xjj);
         else
             hipLaunchKernelGGL(k_f4conv32<F4K32>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,
-                               at<F4Rec32>(base, o_rec), d_R4, C->d_work, ntile, xchunk);
+                               at<F4Rec32>(base, o_rec), d_R4, C->d_work, ntile, xchunk, C->far_w,
+                               C->f4_far == 1 ? C->far_tau : nullptr);
     }
     HIPCHK(hipEventRecord(S.ev[6], s4));
     if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, s4, d_jobs, d_blk, d_R4);
@@ -2558,6 +2642,8 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     (void)hipFree(c->tabs);
     (void)hipFree(c->tabg);
     (void)hipFree(c->tabgs);
+    (void)hipFree(c->far_w);
+    (void)hipFree(c->far_tau);
     (void)hipFree(c->d_err);
     (void)hipFree(c->d_work);
     if (c->rt_state) (void)hipFree(c->rt_state);
@@ -2695,6 +2781,7 @@ int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value) {
     int lo = 0, hi = 0;
     switch (key) {
     case CLBLM_TUNE_F4_XCD_CHUNK: slot = &c->f4_chunk; lo = 1; hi = 4096; break;
+    case CLBLM_TUNE_F4_FAR_FIELD: slot = &c->f4_far; lo = 1; hi = 2; break;
     default: return set_err(CLBLM_ERR_ARG, "unknown tuning key");
     }
     if (old_value) *old_value = *slot;

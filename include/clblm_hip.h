@@ -104,8 +104,12 @@ int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
 /* Launch-shape tuning of a context (no counterpart in the reference, whose
  * loops are sequential).  key selects the parameter; value 0 only queries it;
  * *old_value (if non-NULL) receives the setting before the call.
- *   CLBLM_TUNE_F4_XCD_CHUNK  consecutive F4 tiles given to one XCD (1-4096). */
+ *   CLBLM_TUNE_F4_XCD_CHUNK  consecutive F4 tiles given to one XCD (1-4096).
+ *   CLBLM_TUNE_F4_FAR_FIELD  1: CLBLM_FP32C evaluates far-wing records at 16
+ *                            Chebyshev nodes per F4 tile (default); 2: at every
+ *                            point.  CLBLM_FP64 always sums every point. */
 #define CLBLM_TUNE_F4_XCD_CHUNK 1
+#define CLBLM_TUNE_F4_FAR_FIELD 2
 int clblm_tuning(clblm_ctx *ctx, int key, int value, int *old_value);
 
 /* Waits for every batch enqueued on the context and returns the first device-found
