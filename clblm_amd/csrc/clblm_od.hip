@@ -540,10 +540,9 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
 // F4 far wings.  LINF4Q per-line physics (clblm_ODLAY_LineF4.f90:951-1170).
 // ------------------------------------------------------------------------------------
 struct F4Prep {
-    int8_t *acc, *co2;
-    double *vnu, *sp, *alfa, *epp;
+    int8_t *acc;        // LINF4Q accepts the line
     int *stop;          // per layer: first accepted line with vnu > V2L4 (relative), else f4_n
-    int *rec_acc;       // per layer record: accepted lines up to the stop
+    int *rec_acc;       // per layer record: accepted lines (before the stop is applied)
     int *G;             // per layer record: emitted count of a chunk starting there, by length
     int *nshr;          // per layer: shrunk lines
     int *nchunk, *nblk9;
@@ -556,251 +555,446 @@ struct Shrunk {
     int8_t *co2;
 };
 
-__global__ void k_f4prep(const Job *jobs, LinesDev L, F4Prep f, int *err) {
-    const Job &J = jobs[blockIdx.y];
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= J.f4_n) return;
-    int64_t i = J.f4_lo + k, o = J.f4_off + k;
-    int fl = L.iflg[i];
-    if (fl > 0) atomicOr(err, 1);
-    int mol = L.mol[i];
+// Which lines LINF4Q accepts (RDLNFL's molecule and strength tests, the F4 band's
+// lower edge), the stop line (the first accepted line beyond V2L4, where reading
+// stops), and the accepted count of each TAPE3 record before the stop is applied.
+// One workgroup per record.
+__device__ __forceinline__ bool f4_accept(const Job &J, const LinesDev &L, int64_t i, int *m_out, int *iso_out) {
+    const int mol = L.mol[i];
     int m = mol % 100, iso = (mol % 1000) / 100;
     if (iso == 0) iso = 10;
-    int8_t acc = 0;
-    double vnu = 0., sp = 0., alfa = 0., epp = 0.;
-    if (fl == 0 && m >= 1 && m <= NMOL && J.sel[m] && iso <= c_maxiso[m - 1]) {
-        double sui = (double)L.s[i] * J.w[m];
-        if (sui != 0. && !(L.nu[i] < J.v1l4)) {
-            acc = 1;
-            vnu = L.nu[i] + J.rhorat * (double)L.shift[i];
-            alfa = (double)L.a_air[i];
-            if (m == 7) {
-                double rvmr = 0.21;
-                alfa = (alfa - rvmr * (double)L.a_self[i]) / (1.0 - rvmr);
-            }
-            epp = (double)L.e_lo[i];
-            if (vnu == 0.) sui = 2. * sui;
-            if (epp <= -1.001) epp = fabs(epp);
-            if (epp > -0.999)
-                sui = sui * J.scor[m][iso] * exp(-(epp * J.betacr4)) * (1. + exp(-(vnu * J.beta4)));
-            double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
-            double alfa0i = alfa * tmpcor;
-            double hwhmsi = (double)L.a_self[i] * tmpcor;
-            alfa = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
-            epp = vnu * J.alfd1[m][iso];
-            sp = sui;
-            if (vnu > J.v2l4) atomicMin(&f.stop[blockIdx.y], (int)k);
-        }
-    }
-    f.acc[o] = acc;
-    f.co2[o] = (m == 2);
-    f.vnu[o] = vnu;
-    f.sp[o] = sp;
-    f.alfa[o] = alfa;
-    f.epp[o] = epp;
+    *m_out = m;
+    *iso_out = iso;
+    if (!(L.iflg[i] == 0 && m >= 1 && m <= NMOL && J.sel[m] && iso <= c_maxiso[m - 1])) return false;
+    const double sui = (double)L.s[i] * J.w[m];
+    return sui != 0. && !(L.nu[i] < J.v1l4);
 }
-
-// Accepted lines per TAPE3 record, truncated after the stop line.
-__global__ void k_reccount(const Job *jobs, F4Prep f) {
+__global__ void __launch_bounds__(256) k_f4acc(const Job *jobs, LinesDev L, F4Prep f, int *err) {
     const Job &J = jobs[blockIdx.y];
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = blockIdx.x;
     if (r >= J.nrec) return;
+    __shared__ int wcnt[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     int64_t a = (J.rec0 + r) * REC - J.f4_lo, b = a + REC;
-    int stop = f.stop[blockIdx.y];
     if (a < 0) a = 0;
     if (b > J.f4_n) b = J.f4_n;
-    if (b > (int64_t)stop + 1) b = (int64_t)stop + 1;
-    int c = 0;
-    for (int64_t k = a; k < b; k++) c += f.acc[J.f4_off + k];
-    f.rec_acc[J.rec_off + r] = c;
+    const int64_t k = a + tid;
+    bool on = false;
+    if (k < b) {
+        const int64_t i = J.f4_lo + k;
+        if (L.iflg[i] > 0) atomicOr(err, 1);
+        int m, iso;
+        on = f4_accept(J, L, i, &m, &iso);
+        if (on && L.nu[i] + J.rhorat * (double)L.shift[i] > J.v2l4) atomicMin(&f.stop[blockIdx.y], (int)k);
+        f.acc[J.f4_off + k] = on ? 1 : 0;
+    }
+    const unsigned long long bal = __ballot(on);
+    if (lane == 0) wcnt[wid] = __popcll(bal);
+    __syncthreads();
+    if (tid == 0) f.rec_acc[J.rec_off + r] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
 }
 
-// SHRINQ (clblm_ODLAY_LineF4.f90:1516-1647) over the accepted lines with relative
-// index in [k0, kend).  A group closes after line i when the next accepted line lies
-// beyond VLMT (or there is none); VLMT restarts from the closing line.  Strengths
-// below 1e-35 are zeroed but still counted in the group.  Non-CO2 and CO2 lines of a
-// group give separate effective lines.  In counting mode (EMIT false) the number of
-// lines a call of SHRINQ would emit if the chunk ended at each record boundary is
-// written to gout[0..lcap) (record ends relative to the record of k0).
-struct ShrinkSums {
-    double v, s, al, ad, c;
+// SHRINQ (clblm_ODLAY_LineF4.f90:1516-1647) and LINF4Q's buffer logic (:1172-1243).
+//
+// The accepted lines of each layer up to the stop line are first compacted, in
+// order, into positions (k_recpos, k_f4prep).  SHRINQ over a chunk of positions
+// [p0, p1) is greedy: the first line opens a group with VLMT = v + DVR4/2; each
+// following line joins the open group while v <= VLMT; otherwise the group closes
+// (VLMT restarts from the closing line, VLMT = v_close + DVR4/2) and the line opens
+// the next one; the last line of the chunk closes its group.  A group yields one
+// effective line for its non-CO2 lines and one for its CO2 lines, each only if its
+// strength sum is positive: strengths below 1e-35 are zeroed (they still belong to
+// the group), so a part is emitted iff it holds a line of strength >= 1e-35.
+struct Comp {
+    double *v, *s, *al, *ad;   // per layer at f4_off + position
+    uint8_t *fl;               // bit 0: CO2; bit 1: strength >= 1e-35
+    int *recpos;               // per layer record: position of its first accepted line
+    int *nacc;                 // per layer: accepted lines up to the stop line
 };
+constexpr uint8_t CF_CO2 = 1, CF_LIVE = 2;
 
-template <bool EMIT>
-__device__ int shrinq_walk(const Job &J, const F4Prep &f, int64_t k0, int64_t kend, int64_t rec_end0,
-                           int lcap, int *gout, Shrunk sh, int64_t out0) {
-    // Lines are read WB at a time with independent loads (each lane walks its own
-    // stretch of the list, so this is what hides the memory latency); every accepted
-    // line is processed once the next accepted one is known.
-    constexpr int WB = 8;
-    const double dvh = J.dvr4 / 2.;
-    ShrinkSums a{0, 0, 0, 0, 0}, b{0, 0, 0, 0, 0};
-    int emitted = 0, lrec = 0, walked = 0;
-    double vlmt = 0.;
-    int64_t rec_end = rec_end0;  // relative index one past the current record
-    bool first = true, have = false;
-    double pv = 0., ps = 0., pal = 0., pad = 0.;
-    int8_t pco2 = 0;
-    auto flush = [&](const ShrinkSums g, int8_t co2) {
-        if (EMIT) {
-            int64_t q = J.f4_off + out0 + emitted;
-            sh.vnu[q] = g.v / g.s; sh.sp[q] = g.s; sh.alfa[q] = g.al / g.s; sh.epp[q] = g.ad / g.s;
-            sh.spp[q] = g.c; sh.co2[q] = co2;
-        }
-        emitted++;
-    };
-    // the pending line, with nxt the relative index of the next accepted line (kend if
-    // none) and nv its centre; returns true where a counting walk may stop
-    auto step = [&](int64_t nxt, double nv) -> bool {
-        double v = pv, s = ps, al = pal, ad = pad, c = 0.;
-        if (first) { vlmt = v + dvh; first = false; }
-        if (s < 1.0e-35) { s = 0.0; c = 0.0; }
-        if (!pco2) {
-            a.v = a.v + v * s; a.s = a.s + s; a.al = a.al + s * al; a.ad = a.ad + s * ad; a.c = a.c + c;
-        } else {
-            b.v = b.v + v * s; b.s = b.s + s; b.al = b.al + s * al; b.ad = b.ad + s * ad; b.c = b.c + c;
-        }
-        bool cont = nxt < kend && nv <= vlmt;
-        if (!cont) {
-            vlmt = v + dvh;
-            if (a.s > 0.) flush(a, 0);
-            if (b.s > 0.) flush(b, 1);
-            a = ShrinkSums{0, 0, 0, 0, 0};
-            b = ShrinkSums{0, 0, 0, 0, 0};
-        }
-        if (!EMIT) {
-            int pend = emitted + (a.s > 0. ? 1 : 0) + (b.s > 0. ? 1 : 0);
-            // record ends between this line and the next accepted one
-            bool crossed = false;
-            while (lrec < lcap && rec_end <= nxt) { gout[lrec++] = pend; rec_end += REC; crossed = true; }
-            if (nxt >= kend) {
-                while (lrec < lcap) { gout[lrec++] = pend; rec_end += REC; }
-            }
-            // a chunk never needs more than LIMIN4 new lines: stop at the first record
-            // end after that many (the remaining entries are never read)
-            if (++walked >= LIMIN4 && crossed) return true;
-        }
-        return false;
-    };
-    auto start = [&](int64_t cur) {  // record ends before the first accepted line
-        if (!EMIT) {
-            while (lrec < lcap && rec_end <= cur) { gout[lrec++] = 0; rec_end += REC; }
-        }
-    };
-    for (int64_t kb = k0; kb < kend; kb += WB) {
-        int8_t ac[WB], cc[WB];
-        double vv[WB], ss[WB], aa[WB], ee[WB];
-#pragma unroll
-        for (int u = 0; u < WB; u++) {
-            const int64_t k = kb + u;
-            const int64_t o = J.f4_off + (k < kend ? k : kend - 1);
-            ac[u] = k < kend ? f.acc[o] : (int8_t)0;
-            vv[u] = f.vnu[o]; ss[u] = f.sp[o]; aa[u] = f.alfa[o]; ee[u] = f.epp[o]; cc[u] = f.co2[o];
-        }
-#pragma unroll
-        for (int u = 0; u < WB; u++) {
-            if (!ac[u]) continue;
-            if (have) {
-                if (step(kb + u, vv[u])) return emitted;
-            } else {
-                start(kb + u);
-            }
-            have = true;
-            pv = vv[u]; ps = ss[u]; pal = aa[u]; pad = ee[u]; pco2 = cc[u];
-        }
+// Exclusive prefix of the per-record accepted counts truncated after the stop line
+// (the stop record counts its lines up to the stop line, later records none), one
+// workgroup per layer.
+__global__ void __launch_bounds__(256) k_recpos(const Job *jobs, F4Prep f, Comp c) {
+    const Job &J = jobs[blockIdx.x];
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t stop = f.stop[blockIdx.x];
+    int64_t stop_rec = J.nrec;   // no truncation
+    int stop_cnt = 0;
+    if (stop < J.f4_n) {
+        stop_rec = (J.f4_lo + stop) / REC - J.rec0;
+        int64_t a = (J.rec0 + stop_rec) * REC - J.f4_lo;
+        if (a < 0) a = 0;
+        const int64_t k = a + tid;
+        const bool on = tid < REC && k <= stop && f.acc[J.f4_off + k] != 0;
+        const unsigned long long bal = __ballot(on);
+        if (lane == 0) wsum[wid] = __popcll(bal);
+        __syncthreads();
+        stop_cnt = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
     }
-    if (have) {
-        if (step(kend, 0.)) return emitted;
-    } else {
-        start(kend);
+    int carry = 0;
+    for (int64_t b = 0; b < J.nrec; b += 256) {
+        const int64_t r = b + tid;
+        const int x = r < stop_rec ? f.rec_acc[J.rec_off + r] : (r == stop_rec ? stop_cnt : 0);
+        int incl = x;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int pre = carry;
+        for (int w = 0; w < wid; w++) pre += wsum[w];
+        if (r < J.nrec) c.recpos[J.rec_off + r] = pre + incl - x;
+        carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
     }
-    if (!EMIT) {
-        while (lrec < lcap) gout[lrec++] = emitted;
-    }
-    return emitted;
+    if (tid == 0) c.nacc[blockIdx.x] = carry;
 }
 
-constexpr int LCAP = 64;  // longest shrink chunk, in records
-
-// G(r, L): lines a SHRINQ call emits for a chunk of L records starting at record r.
-__global__ void __launch_bounds__(64) k_chunkG(const Job *jobs, F4Prep f, Shrunk sh) {
+// LINF4Q per-line physics (clblm_ODLAY_LineF4.f90:951-1170) for the accepted lines of
+// one TAPE3 record up to the stop line, written at their positions.
+__global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4Prep f, Comp c) {
     const Job &J = jobs[blockIdx.y];
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = blockIdx.x;
     if (r >= J.nrec) return;
-    int64_t stop = f.stop[blockIdx.y];
-    int64_t k0 = (J.rec0 + r) * REC - J.f4_lo;
-    if (k0 < 0) k0 = 0;
-    int64_t kend = (J.rec0 + r + LCAP) * REC - J.f4_lo;
-    if (kend > J.f4_n) kend = J.f4_n;
-    if (kend > stop + 1) kend = stop + 1;
-    int64_t rec_end0 = (J.rec0 + r + 1) * REC - J.f4_lo;
-    shrinq_walk<false>(J, f, k0, kend, rec_end0, LCAP, f.G + (J.rec_off + r) * LCAP, sh, 0);
+    __shared__ int wcnt[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int64_t a = (J.rec0 + r) * REC - J.f4_lo, b = a + REC;
+    if (a < 0) a = 0;
+    if (b > J.f4_n) b = J.f4_n;
+    const int64_t stop = f.stop[blockIdx.y];
+    if (b > stop + 1) b = stop + 1;
+    const int64_t k = a + tid;
+    const bool on = k < b && f.acc[J.f4_off + k] != 0;
+    const unsigned long long bal = __ballot(on);
+    if (lane == 0) wcnt[wid] = __popcll(bal);
+    __syncthreads();
+    if (!on) return;
+    int pos = c.recpos[J.rec_off + r] + __popcll(bal & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wid; w++) pos += wcnt[w];
+    const int64_t i = J.f4_lo + k, o = J.f4_off + pos;
+    int m, iso;
+    (void)f4_accept(J, L, i, &m, &iso);
+    double sui = (double)L.s[i] * J.w[m];
+    const double vnu = L.nu[i] + J.rhorat * (double)L.shift[i];
+    double alfa = (double)L.a_air[i];
+    if (m == 7) {
+        const double rvmr = 0.21;
+        alfa = (alfa - rvmr * (double)L.a_self[i]) / (1.0 - rvmr);
+    }
+    double epp = (double)L.e_lo[i];
+    if (vnu == 0.) sui = 2. * sui;
+    if (epp <= -1.001) epp = fabs(epp);
+    if (epp > -0.999) sui = sui * J.scor[m][iso] * exp(-(epp * J.betacr4)) * (1. + exp(-(vnu * J.beta4)));
+    const double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
+    const double alfa0i = alfa * tmpcor;
+    const double hwhmsi = (double)L.a_self[i] * tmpcor;
+    alfa = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+    c.v[o] = vnu;
+    c.s[o] = sui;
+    c.al[o] = alfa;
+    c.ad[o] = vnu * J.alfd1[m][iso];
+    c.fl[o] = (uint8_t)((m == 2 ? CF_CO2 : 0) | (sui < 1.0e-35 ? 0 : CF_LIVE));
+}
+
+// record end e (relative record index) of a layer as a position
+__device__ __forceinline__ int rec_end_pos(const Comp &c, const Job &J, int l, int64_t e) {
+    return e < J.nrec ? c.recpos[J.rec_off + e] : c.nacc[l];
+}
+
+constexpr int LCAP = 64;  // longest shrink chunk, in records, with a precomputed count
+
+// One step of the SHRINQ grouping at position q with centre x and flags fl: the open
+// group closes before q when x lies beyond VLMT (its emissions are counted), then
+// q joins the open group.  fa / fb: the open group holds a non-CO2 / CO2 line of
+// strength >= 1e-35.
+struct ShrCount {
+    double vlmt, vprev;
+    int emitted, fa, fb;
+};
+__device__ __forceinline__ void shr_count_step(ShrCount &w, bool first, double x, int fl, double dvh) {
+    if (first) {
+        w.vlmt = x + dvh;
+    } else if (!(x <= w.vlmt)) {
+        w.emitted += w.fa + w.fb;
+        w.fa = 0; w.fb = 0;
+        w.vlmt = w.vprev + dvh;
+    }
+    fl &= 3;
+    w.fa |= fl == CF_LIVE;
+    w.fb |= fl == (CF_LIVE | CF_CO2);
+    w.vprev = x;
+}
+
+// Lines SHRINQ emits for the chunk of positions [p0, p1), one lane (rare path).
+__device__ int shrinq_count_lane(const Comp &c, const Job &J, int p0, int p1) {
+    const double *vv = c.v + J.f4_off;
+    const uint8_t *ff = c.fl + J.f4_off;
+    const double dvh = J.dvr4 / 2.;
+    ShrCount w{0., 0., 0, 0, 0};
+    for (int q = p0; q < p1; q++) shr_count_step(w, q == p0, vv[q], ff[q], dvh);
+    return w.emitted + w.fa + w.fb;
+}
+
+// dst[i] = src[i] for i < n by NT threads, U loads in flight per thread
+template <int NT, int U, typename T>
+__device__ __forceinline__ void stage(T *dst, const T *src, int n, int tid) {
+    for (int b = 0; b < n; b += NT * U) {
+        T t[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = b + u * NT + tid;
+            t[u] = i < n ? src[i] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = b + u * NT + tid;
+            if (i < n) dst[i] = t[u];
+        }
+    }
+}
+
+// G(r, L): lines a SHRINQ call emits for the chunk of the L records starting at
+// record r (L = 1..LCAP), one lane per record r walking the positions once (CB
+// positions loaded per batch).  A chunk never takes LIMIN4 new lines or more
+// before its last record, so the walk stops at the first record end after LIMIN4
+// lines (the entries beyond it are never read).
+constexpr int CB = 16;
+__global__ void __launch_bounds__(64) k_chunkG(const Job *jobs, Comp c, int *G) {
+    const int l = blockIdx.y;
+    const Job &J = jobs[l];
+    const int64_t r = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (r >= J.nrec) return;
+    const int p0 = c.recpos[J.rec_off + r];
+    const int pend = rec_end_pos(c, J, l, r + LCAP);
+    int *gout = G + (J.rec_off + r) * LCAP;
+    const double *vv = c.v + J.f4_off;
+    const uint8_t *ff = c.fl + J.f4_off;
+    const double dvh = J.dvr4 / 2.;
+    ShrCount w{0., 0., 0, 0, 0};
+    int L = 0, pe = rec_end_pos(c, J, l, r + 1);
+    for (int base = p0; base < pend; base += CB) {
+        double vb[CB];
+        int fbits[CB];
+#pragma unroll
+        for (int u = 0; u < CB; u++) {
+            const int p = base + u < pend ? base + u : pend - 1;
+            vb[u] = vv[p];
+            fbits[u] = ff[p];
+        }
+#pragma unroll
+        for (int u = 0; u < CB; u++) {
+            const int q = base + u;
+            if (q >= pend) break;
+            bool wrote = false;
+            while (pe <= q) {  // record ends between the previous line and this one
+                gout[L++] = w.emitted + w.fa + w.fb;
+                wrote = true;
+                if (L >= LCAP) return;
+                pe = rec_end_pos(c, J, l, r + 1 + L);
+            }
+            if (wrote && q - p0 >= LIMIN4) return;
+            shr_count_step(w, q == p0, vb[u], fbits[u], dvh);
+        }
+    }
+    for (; L < LCAP; L++) gout[L] = w.emitted + w.fa + w.fb;
 }
 
 // LINF4Q buffer logic (clblm_ODLAY_LineF4.f90:1205-1243): records are appended until
 // the buffer holds LIMIN lines, the new lines are shrunk, and the buffer is written
-// as one TAPE9 block once it holds LIMIN shrunk lines.  One thread per layer walks
-// the records with the precomputed chunk counts.
-__global__ void __launch_bounds__(64) k_sim(const Job *jobs, int nlay, F4Prep f) {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlay) return;
+// as one TAPE9 block once it holds LIMIN shrunk lines.  A chunk starting at record
+// `start` with S shrunk lines in the buffer therefore ends at the first record end
+// where it holds LIMIN - S accepted lines, or after the stop record.  One workgroup
+// per layer stages the record positions and the chunk counts G(r, 1..SIMG) of a
+// window of SIMW records in LDS; one lane walks the chunks.
+constexpr int SIMW = 1024, SIMG = 8;
+__global__ void __launch_bounds__(256) k_sim(const Job *jobs, F4Prep f, Comp c, const int *G) {
+    __shared__ int rp[SIMW + 1];
+    __shared__ short g8[SIMW * SIMG];
+    __shared__ long long st_start, st_out;
+    __shared__ int st_S, st_nch, st_nb, st_done;
+    const int l = blockIdx.x;
     const Job &J = jobs[l];
-    int64_t stop = f.stop[l];
-    int64_t stop_rec = stop < J.f4_n ? (J.f4_lo + stop) / REC - J.rec0 : J.nrec - 1;
-    int S = 0, raw = 0, nch = 0, nb = 0;
-    int64_t start = 0, r = 0, out = 0;
+    const int tid = threadIdx.x;
+    const int64_t stop = f.stop[l];
+    const int64_t stop_rec = stop < J.f4_n ? (J.f4_lo + stop) / REC - J.rec0 : J.nrec - 1;
     int4 *ch = f.chunks + J.rec_off;
     int64_t *be = f.blk9_end + J.rec_off;
-    for (;;) {
-        if (r < J.nrec) raw += f.rec_acc[J.rec_off + r];
-        r++;
-        bool eof = r > stop_rec;
-        if (!eof && S + raw < LIMIN4) continue;
-        int64_t L = r - start;
-        if (raw > 0) {
-            int g;
-            if (L <= LCAP) {
-                g = f.G[(J.rec_off + start) * LCAP + L - 1];
-            } else {  // a chunk of few accepted lines over many records: count it here
-                int64_t k0 = (J.rec0 + start) * REC - J.f4_lo;
-                if (k0 < 0) k0 = 0;
-                int64_t kend = (J.rec0 + start + L) * REC - J.f4_lo;
-                if (kend > J.f4_n) kend = J.f4_n;
-                if (kend > stop + 1) kend = stop + 1;
-                g = 0;
-                shrinq_walk<false>(J, f, k0, kend, kend, 1, &g, Shrunk{}, 0);
+    if (tid == 0) { st_start = 0; st_out = 0; st_S = 0; st_nch = 0; st_nb = 0; st_done = 0; }
+    __syncthreads();
+    while (!st_done) {
+        const int64_t w0 = st_start;
+        {
+            int t[5];
+#pragma unroll
+            for (int u = 0; u < 5; u++) {
+                const int i = u * 256 + tid;
+                t[u] = i <= SIMW ? rec_end_pos(c, J, l, w0 + i) : 0;
             }
-            ch[nch++] = make_int4((int)start, (int)L, (int)(out + S), l);
-            S += g;
+#pragma unroll
+            for (int u = 0; u < 5; u++) {
+                const int i = u * 256 + tid;
+                if (i <= SIMW) rp[i] = t[u];
+            }
         }
-        raw = 0;
-        start = r;
-        if (S < LIMIN4 && !eof) continue;
-        if (S > 0) { out += S; be[nb++] = out; }
-        if (eof) break;
-        S = 0;
+        for (int b = 0; b < SIMW * SIMG; b += 256 * 8) {
+            int t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = b + u * 256 + tid;
+                const int64_t rr = w0 + i / SIMG;
+                t[u] = rr < J.nrec ? G[(J.rec_off + rr) * LCAP + i % SIMG] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) g8[b + u * 256 + tid] = (short)t[u];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int64_t start = st_start, out = st_out;
+            int S = st_S, nch = st_nch, nb = st_nb, done = 0;
+            while (start - w0 < SIMW - LCAP) {
+                const int target = LIMIN4 - S;
+                const int pst = rp[start - w0];
+                int64_t e = start;
+                int pe;
+                for (;;) {
+                    e++;
+                    pe = e - w0 <= SIMW ? rp[e - w0] : rec_end_pos(c, J, l, e);
+                    if (e > stop_rec || pe - pst >= target) break;
+                }
+                const bool eof = e > stop_rec;
+                const int64_t L = e - start;
+                if (pe > pst) {
+                    int g;
+                    if (L <= SIMG) g = g8[(start - w0) * SIMG + L - 1];
+                    else if (L <= LCAP) g = G[(J.rec_off + start) * LCAP + L - 1];
+                    else g = shrinq_count_lane(c, J, pst, pe);
+                    ch[nch++] = make_int4((int)start, (int)L, (int)(out + S), l);
+                    S += g;
+                }
+                start = e;
+                if (S >= LIMIN4 || eof) {
+                    if (S > 0) {
+                        out += S;
+                        be[nb++] = out;
+                    }
+                    if (eof) { done = 1; break; }
+                    S = 0;
+                }
+            }
+            st_start = start; st_out = out; st_S = S; st_nch = nch; st_nb = nb; st_done = done;
+        }
+        __syncthreads();
     }
-    f.nshr[l] = (int)out;
-    f.nchunk[l] = nch;
-    f.nblk9[l] = nb;
+    if (tid == 0) {
+        f.nshr[l] = (int)st_out;
+        f.nchunk[l] = st_nch;
+        f.nblk9[l] = st_nb;
+    }
 }
 
-__global__ void __launch_bounds__(64) k_emit(const Job *jobs, F4Prep f, Shrunk sh, int64_t rec_tot) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= rec_tot) return;
-    int4 c = f.chunks[q];
+// SHRINQ of one chunk, one workgroup per chunk: the centres and flags are staged in
+// LDS, one lane walks them to find where the groups start, then the lanes take the
+// groups, count their effective lines, place them by a prefix sum, and sum them in
+// the reference's order (the non-CO2 part of a group first).
+constexpr int EMAX = LIMIN4 + REC;
+struct ShrinkSums {
+    double v, s, al, ad;
+};
+__global__ void __launch_bounds__(64) k_emit(const Job *jobs, F4Prep f, Comp c, Shrunk sh, int *err) {
+    __shared__ double sv[EMAX];
+    __shared__ uint8_t sf[EMAX];
+    __shared__ short gs[EMAX + 1];
+    __shared__ int ngs;
+    const int64_t q = blockIdx.x;
+    const int4 ck = f.chunks[q];
     // chunk slots are per layer at rec_off; only the first nchunk of each layer are valid
-    if (c.w < 0) return;
-    const Job &J = jobs[c.w];
-    int64_t slot = q - J.rec_off;
-    if (slot >= f.nchunk[c.w]) return;
-    int64_t stop = f.stop[c.w];
-    int64_t k0 = (J.rec0 + c.x) * REC - J.f4_lo;
-    if (k0 < 0) k0 = 0;
-    int64_t kend = (J.rec0 + c.x + c.y) * REC - J.f4_lo;
-    if (kend > J.f4_n) kend = J.f4_n;
-    if (kend > stop + 1) kend = stop + 1;
-    shrinq_walk<true>(J, f, k0, kend, 0, 0, nullptr, sh, c.z);
+    if (ck.w < 0) return;
+    const int l = ck.w;
+    const Job &J = jobs[l];
+    if (q - J.rec_off >= f.nchunk[l]) return;
+    const int tid = threadIdx.x;
+    const int p0 = rec_end_pos(c, J, l, ck.x), p1 = rec_end_pos(c, J, l, (int64_t)ck.x + ck.y);
+    const int n = p1 - p0;
+    if (n <= 0) return;
+    if (n > EMAX) {  // no LINF4Q buffer produces such a chunk: reported as an internal error
+        if (tid == 0) atomicOr(err, 16);
+        return;
+    }
+    const double dvh = J.dvr4 / 2.;
+    stage<64, 8>(sv, c.v + J.f4_off + p0, n, tid);
+    stage<64, 8>(sf, c.fl + J.f4_off + p0, n, tid);
+    __syncthreads();
+    if (tid == 0) {  // group starts: the first line, and every line beyond VLMT
+        double vlmt = sv[0] + dvh, vprev = sv[0];
+        int m = 0;
+        gs[m++] = 0;
+        for (int i0 = 1; i0 < n; i0 += 8) {
+            double xb[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) xb[u] = sv[i0 + u < n ? i0 + u : n - 1];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (i0 + u >= n) break;
+                const double x = xb[u];
+                if (!(x <= vlmt)) {
+                    gs[m++] = (short)(i0 + u);
+                    vlmt = vprev + dvh;
+                }
+                vprev = x;
+            }
+        }
+        gs[m] = (short)n;
+        ngs = m;
+    }
+    __syncthreads();
+    const int m = ngs;
+    const double *ss = c.s + J.f4_off + p0, *aa = c.al + J.f4_off + p0, *dd = c.ad + J.f4_off + p0;
+    int base_out = ck.z;
+    for (int g0 = 0; g0 < m; g0 += 64) {
+        const int g = g0 + tid;
+        int ha = 0, hb = 0;
+        if (g < m) {
+            for (int i = gs[g]; i < gs[g + 1]; i++) {
+                const int fl = sf[i] & 3;
+                ha |= fl == CF_LIVE;
+                hb |= fl == (CF_LIVE | CF_CO2);
+            }
+        }
+        const int cnt = ha + hb;
+        int incl = cnt;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(incl, d, 64);
+            if (tid >= d) incl += o;
+        }
+        if (g < m && cnt) {
+            int64_t o = J.f4_off + base_out + incl - cnt;
+            ShrinkSums a{0, 0, 0, 0}, b{0, 0, 0, 0};
+            for (int i = gs[g]; i < gs[g + 1]; i++) {
+                if (!(sf[i] & CF_LIVE)) continue;
+                const double x = sv[i], sp = ss[i];
+                if (sf[i] & CF_CO2) {
+                    b.v = b.v + x * sp; b.s = b.s + sp; b.al = b.al + sp * aa[i]; b.ad = b.ad + sp * dd[i];
+                } else {
+                    a.v = a.v + x * sp; a.s = a.s + sp; a.al = a.al + sp * aa[i]; a.ad = a.ad + sp * dd[i];
+                }
+            }
+            if (ha) {
+                sh.vnu[o] = a.v / a.s; sh.sp[o] = a.s; sh.alfa[o] = a.al / a.s; sh.epp[o] = a.ad / a.s;
+                sh.spp[o] = 0.; sh.co2[o] = 0;
+                o++;
+            }
+            if (hb) {
+                sh.vnu[o] = b.v / b.s; sh.sp[o] = b.s; sh.alfa[o] = b.al / b.s; sh.epp[o] = b.ad / b.s;
+                sh.spp[o] = 0.; sh.co2[o] = 1;
+            }
+        }
+        base_out += __shfl(incl, 63, 64);
+    }
 }
 
 // Prefix max / suffix min of the shrunk centres (the range searches of the R4
@@ -2028,6 +2222,9 @@ static void finish_slot(clblm_ctx *C, int k, bool wait) {
         if (herr & 1) {
             C->deferred_rc = CLBLM_ERR_UNSUPPORTED;
             C->deferred_msg = "line coupling records (IFLG 1..3) are not supported";
+        } else if (herr & 16) {
+            C->deferred_rc = CLBLM_ERR_HIP;
+            C->deferred_msg = "internal error: a SHRINQ chunk exceeds the LINF4Q buffer";
         } else {
             C->deferred_rc = CLBLM_ERR_CAPACITY;
             C->deferred_msg = "F4 record capacity exceeded";
@@ -2360,8 +2557,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_lp_d = cv.take<double>(4 * (size_t)B.tot_line);
     size_t o_lp_i = cv.take<int>(3 * (size_t)B.tot_line);
     size_t o_A = cv.take<double>((size_t)B.tot_a);
-    size_t o_f4_b = cv.take<int8_t>(2 * (size_t)B.tot_f4);
-    size_t o_f4_d = cv.take<double>(4 * (size_t)B.tot_f4);
+    size_t o_f4_b = cv.take<int8_t>((size_t)B.tot_f4);
     size_t o_sh_d = cv.take<double>(8 * (size_t)B.tot_f4);
     size_t o_sh_b = cv.take<int8_t>((size_t)B.tot_f4);
     size_t o_lay = cv.take<int>(4 * (size_t)nlay);
@@ -2370,6 +2566,10 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_htot = cv.take<int>((size_t)std::max<int64_t>(1, hcap * nlay));
     size_t o_ftot = cv.take<double>((size_t)std::max<int64_t>(1, 2 * fcap * nlay));
     size_t o_G = cv.take<int>((size_t)B.tot_rec * LCAP);
+    size_t o_cd = cv.take<double>(4 * (size_t)B.tot_f4);
+    size_t o_cf = cv.take<uint8_t>((size_t)B.tot_f4);
+    size_t o_cr = cv.take<int>((size_t)B.tot_rec);
+    size_t o_cn = cv.take<int>((size_t)nlay);
     size_t o_ch = cv.take<int4>((size_t)B.tot_rec);
     size_t o_be = cv.take<int64_t>((size_t)B.tot_rec);
     size_t o_R4 = cv.take<double>((size_t)B.tot_r4);
@@ -2452,16 +2652,21 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     lp.iz = lpi; lp.pmin = lpi + B.tot_line; lp.home = lpi + 2 * B.tot_line;
     double *d_A = at<double>(base, o_A);
     F4Prep f;
-    int8_t *f4b = at<int8_t>(base, o_f4_b);
-    double *f4d = at<double>(base, o_f4_d);
-    f.acc = f4b; f.co2 = f4b + B.tot_f4;
-    f.vnu = f4d; f.sp = f4d + B.tot_f4; f.alfa = f4d + 2 * B.tot_f4; f.epp = f4d + 3 * B.tot_f4;
+    f.acc = at<int8_t>(base, o_f4_b);
     int *lay = at<int>(base, o_lay);
     f.stop = lay; f.nshr = lay + nlay; f.nchunk = lay + 2 * nlay; f.nblk9 = lay + 3 * nlay;
     f.rec_acc = at<int>(base, o_racc);
     f.G = at<int>(base, o_G);
     f.chunks = at<int4>(base, o_ch);
     f.blk9_end = at<int64_t>(base, o_be);
+    Comp cmp;
+    {
+        double *cd = at<double>(base, o_cd);
+        cmp.v = cd; cmp.s = cd + B.tot_f4; cmp.al = cd + 2 * B.tot_f4; cmp.ad = cd + 3 * B.tot_f4;
+        cmp.fl = at<uint8_t>(base, o_cf);
+        cmp.recpos = at<int>(base, o_cr);
+        cmp.nacc = at<int>(base, o_cn);
+    }
     Shrunk sh;
     double *shd = at<double>(base, o_sh_d);
     sh.vnu = shd; sh.sp = shd + B.tot_f4; sh.alfa = shd + 2 * B.tot_f4; sh.epp = shd + 3 * B.tot_f4;
@@ -2484,12 +2689,13 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     HIPCHK(hipStreamWaitEvent(s4, S.fork, 0));
     HIPCHK(hipEventRecord(S.ev[7], s4));
     if (B.max_f4 > 0) {
-        hipLaunchKernelGGL(k_f4prep, dim3(cdiv(B.max_f4, 256), nlay), dim3(256), 0, s4, d_jobs, L.dev, f, d_err);
-        hipLaunchKernelGGL(k_reccount, dim3(cdiv(B.max_rec, 256), nlay), dim3(256), 0, s4, d_jobs, f);
-        hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, s4, d_jobs, f, sh);
-        hipLaunchKernelGGL(k_sim, dim3(cdiv(nlay, 64)), dim3(64), 0, s4, d_jobs, nlay, f);
-        hipLaunchKernelGGL(k_emit, dim3(cdiv(std::max<int64_t>(1, B.tot_rec), 64)), dim3(64), 0, s4, d_jobs, f, sh,
-                           B.tot_rec);
+        hipLaunchKernelGGL(k_f4acc, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, d_err);
+        hipLaunchKernelGGL(k_recpos, dim3(nlay), dim3(256), 0, s4, d_jobs, f, cmp);
+        hipLaunchKernelGGL(k_f4prep, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, cmp);
+        hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, s4, d_jobs, cmp, f.G);
+        hipLaunchKernelGGL(k_sim, dim3(nlay), dim3(256), 0, s4, d_jobs, f, cmp, (const int *)f.G);
+        hipLaunchKernelGGL(k_emit, dim3((unsigned)std::max<int64_t>(1, B.tot_rec)), dim3(64), 0, s4, d_jobs, f, cmp,
+                           sh, d_err);
         double *ftot = at<double>(base, o_ftot);
         if (fcap > 0) {
             hipLaunchKernelGGL(k_shr_local, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, s4, d_jobs, f, sh, ftot,
