@@ -17,6 +17,7 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_timing", "clblm_work", "clblm_tuning"]
 TUNE_F4_XCD_CHUNK = 1
 TUNE_F4_FAR_FIELD = 2
+TUNE_F4_STREAM = 3
 
 
 class Layer(ctypes.Structure):

@@ -1019,16 +1019,31 @@ __global__ void __launch_bounds__(SCAN_T) k_shr_fix(const Job *jobs, F4Prep f, S
     else
         scan_fix<double, false, true>(sh.smin, J.f4_off, n, 1e300, tot + (2 * blockIdx.y + 1) * cap);
 }
-__global__ void k_shr_blk9(const Job *jobs, int nlay, F4Prep f, Shrunk sh) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlay) return;
+// Prefix max of the TAPE9 block maxima (the last shrunk centre of each block), one
+// workgroup per layer.
+__global__ void __launch_bounds__(256) k_shr_blk9(const Job *jobs, F4Prep f, Shrunk sh) {
+    __shared__ double wmax[4];
+    const int l = blockIdx.x;
     const Job &J = jobs[l];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const double *v = sh.vnu + J.f4_off;
-    double m = -1e300;
-    for (int b = 0; b < f.nblk9[l]; b++) {
-        int64_t e = f.blk9_end[J.rec_off + b];
-        m = fmax(m, v[e - 1]);
-        sh.bvmax[J.f4_off + b] = m;
+    const int nb = f.nblk9[l];
+    double carry = -1e300;
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+        const int b = b0 + tid;
+        const double x = b < nb ? v[f.blk9_end[J.rec_off + b] - 1] : -1e300;
+        double incl = x;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl = fmax(incl, o);
+        }
+        if (lane == 63) wmax[wid] = incl;
+        __syncthreads();
+        double pre = carry;
+        for (int w = 0; w < wid; w++) pre = fmax(pre, wmax[w]);
+        if (b < nb) sh.bvmax[J.f4_off + b] = fmax(pre, incl);
+        carry = fmax(fmax(carry, wmax[0]), fmax(wmax[1], fmax(wmax[2], wmax[3])));
+        __syncthreads();
     }
 }
 
@@ -2149,6 +2164,7 @@ struct clblm_ctx {
     hipStream_t f4stream = nullptr;
     int f4_chunk = 16;       // consecutive F4 tiles per XCD run (xcd_tile)
     int f4_far = 1;          // far-field evaluation in the FP32 gather: 1 on, 2 off
+    int f4_stream_mode = 1;  // 1: the F4 chain on its own stream; 2: on the batch's stream
     double *far_w = nullptr; // Lagrange weights of the far-field nodes, [64 F4K32][F4NF]
     float *far_tau = nullptr;   // node positions, tile points from the tile's first point
     double *tab = nullptr;   // F1,F2,F3,FG,XVER consecutive, NFMX+1 each
@@ -2684,7 +2700,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     const int64_t nblk = (int64_t)B.blks.size();
 
     // fork: the F4 chain runs on the F4 stream from here to the join before pass 2
-    const hipStream_t s4 = C->f4stream;
+    const hipStream_t s4 = C->f4_stream_mode == 1 ? C->f4stream : st;
     HIPCHK(hipEventRecord(S.fork, st));
     HIPCHK(hipStreamWaitEvent(s4, S.fork, 0));
     HIPCHK(hipEventRecord(S.ev[7], s4));
@@ -2703,7 +2719,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
             hipLaunchKernelGGL(k_shr_fix, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, s4, d_jobs, f, sh, ftot,
                                fcap);
         }
-        hipLaunchKernelGGL(k_shr_blk9, dim3(cdiv(nlay, 64)), dim3(64), 0, s4, d_jobs, nlay, f, sh);
+        hipLaunchKernelGGL(k_shr_blk9, dim3(nlay), dim3(256), 0, s4, d_jobs, f, sh);
     } else {
         HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, s4));
     }
@@ -2988,6 +3004,7 @@ int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value) {
     switch (key) {
     case CLBLM_TUNE_F4_XCD_CHUNK: slot = &c->f4_chunk; lo = 1; hi = 4096; break;
     case CLBLM_TUNE_F4_FAR_FIELD: slot = &c->f4_far; lo = 1; hi = 2; break;
+    case CLBLM_TUNE_F4_STREAM: slot = &c->f4_stream_mode; lo = 1; hi = 2; break;
     default: return set_err(CLBLM_ERR_ARG, "unknown tuning key");
     }
     if (old_value) *old_value = *slot;

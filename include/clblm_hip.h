@@ -107,9 +107,13 @@ int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
  *   CLBLM_TUNE_F4_XCD_CHUNK  consecutive F4 tiles given to one XCD (1-4096).
  *   CLBLM_TUNE_F4_FAR_FIELD  1: CLBLM_FP32C evaluates far-wing records at 16
  *                            Chebyshev nodes per F4 tile (default); 2: at every
- *                            point.  CLBLM_FP64 always sums every point. */
+ *                            point.  CLBLM_FP64 always sums every point.
+ *   CLBLM_TUNE_F4_STREAM     1: the F4 wing chain of a batch runs on a stream of
+ *                            the context beside pass 1 (default); 2: every kernel
+ *                            of the batch on the batch's stream, one after another. */
 #define CLBLM_TUNE_F4_XCD_CHUNK 1
 #define CLBLM_TUNE_F4_FAR_FIELD 2
+#define CLBLM_TUNE_F4_STREAM 3
 int clblm_tuning(clblm_ctx *ctx, int key, int value, int *old_value);
 
 /* Waits for every batch enqueued on the context and returns the first device-found
