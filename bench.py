@@ -152,6 +152,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison run")
     ap.add_argument("--no-jac", action="store_true", help="skip the Jacobian pass")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shapes")
     ap.add_argument("--serial", action="store_true",
                     help="run every kernel of a batch on one stream, one after another (for a kernel "
                          "trace whose durations do not overlap)")
@@ -184,6 +185,15 @@ def main():
         ctx.tuning(TUNE_F4_STREAM, 2)
     h = ctx.upload_lines(lines)
     mk = lambda p: [Context.params(V1, V2, dv, precision=p, continuum=c) for dv, c in zip(dvs, conts)]
+    from clblm_amd import _lib
+    tuned = {}
+    if not args.no_autotune:   # untimed set-up: the library's autotuner picks the launch shapes
+        keys = None if not args.serial else [_lib.TUNE_F4_POINTS_FP64 if prec == FP64 else _lib.TUNE_F4_POINTS_FP32C,
+                                             _lib.TUNE_CONV_UNROLL, _lib.TUNE_F4_XCD_CHUNK]
+        chosen, _ = ctx.autotune(h, layers, mk(prec), keys=keys, reps=2)
+        tuned = {"f4_points_per_lane": chosen[_lib.TUNE_F4_POINTS_FP64 if prec == FP64 else _lib.TUNE_F4_POINTS_FP32C],
+                 "conv_unroll": chosen[_lib.TUNE_CONV_UNROLL], "f4_xcd_chunk": chosen[_lib.TUNE_F4_XCD_CHUNK],
+                 "f4_stream": "own" if chosen[_lib.TUNE_F4_STREAM] == 1 else "batch"}
     dt, w, tm, off = run_od(ctx, h, layers, mk(prec), args.steps, args.warmup)
     value = E / dt
     conv_tf = w["conv_evals"] * FLOP_CONV / (w["ms_conv"] * 1e-3) / 1e12 if w["ms_conv"] > 0 else 0.0
@@ -202,7 +212,7 @@ def main():
         "config": {"workload": "60-layer full-profile OD: 200k lines of 7 species over 500-2500 cm-1, "
                                "1013-0.1 hPa, DV = mean Voigt HWHM/4, spline continuum",
                    "layers": len(layers), "lines": lines.n, "points": int(off[-1]),
-                   "precision": args.precision, "line_evals_per_step": E},
+                   "precision": args.precision, "line_evals_per_step": E, "launch_shapes": tuned},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
                      "frac": ach / peak,
                      "traffic": measured_traffic(dom + ("64" if prec == FP64 else "32")),
@@ -212,6 +222,8 @@ def main():
                             "conv_stage": tm["conv"], "f4_stage": tm["f4"], "panel_stage": tm["panel"]}},
     }
     if prec != FP64 and not args.no_fp64:
+        if not args.no_autotune:
+            ctx.autotune(h, layers, mk(FP64), keys=[_lib.TUNE_F4_POINTS_FP64], reps=2)
         dt64, _, _, _ = run_od(ctx, h, layers, mk(FP64), args.steps, args.warmup)
         res["fp64_gpu"] = {"value": E / dt64, "unit": "line-evals/s", "ms_per_step": dt64 * 1e3,
                            "fp32c_over_fp64": dt64 / dt}

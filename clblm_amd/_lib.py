@@ -14,10 +14,13 @@ FP64, FP32C = 0, 1
 EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
            "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
-           "clblm_timing", "clblm_work", "clblm_tuning"]
+           "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune"]
 TUNE_F4_XCD_CHUNK = 1
 TUNE_F4_FAR_FIELD = 2
 TUNE_F4_STREAM = 3
+TUNE_F4_POINTS_FP32C = 4
+TUNE_F4_POINTS_FP64 = 5
+TUNE_CONV_UNROLL = 6
 
 
 class Layer(ctypes.Structure):
@@ -85,6 +88,8 @@ def load():
     lib.clblm_work.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.clblm_tuning.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.clblm_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double)]
     _lib = lib
     return lib
 

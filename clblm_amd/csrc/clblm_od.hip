@@ -354,8 +354,9 @@ __global__ void k_ranges(const Job *jobs, Panel *panels, int64_t npanel_tot, Lin
 // FP64 atomics and the panel's arrays go to HBM.  Local index 1..MAX is kept; lower
 // indices are never read by the reference's panel logic.
 // ------------------------------------------------------------------------------------
-constexpr int CONV_U = 4;   // R1 points per step of the pass-1 walk
+// CONV_U: R1 points per step of the pass-1 walk (a launch-shape setting, 2, 4 or 8)
 
+template <int CONV_U>
 __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *panels, LinePrep lp,
                                                 Tables T, double *A, unsigned long long *work) {
     extern __shared__ double sm[];
@@ -448,6 +449,7 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
 // cheaper than a compensated FP32 sum, which needs the returned old value of every
 // add; an FP64 accumulator of FP32 terms is at least as accurate as a Kahan sum.
 // ------------------------------------------------------------------------------------
+template <int CONV_U>
 __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *panels, LinePrep lp, Tables T,
                                                 double *A, unsigned long long *work) {
     extern __shared__ double sm[];
@@ -1341,20 +1343,19 @@ __device__ __forceinline__ float f4_eval32(const F4Rec32 &l, float xm, float dpt
     return __fmaf_rn(xm * dptrat, f, f);
 }
 
-// points per lane of the F4 gathers (tiles of 64 F4K points), measured best on the
-// 60-layer workload: 3 for the FP32 shapes, 2 for FP64
-constexpr int F4K32 = 3, F4K64 = 2;
+// F4K: points per lane of the F4 gathers (tiles of 64 F4K points), a launch-shape
+// setting of the context (defaults 3 for the FP32 shapes, 2 for FP64)
 // F4 records summed in FP32 before each FP64 accumulation (FP32 gather)
 constexpr int F4CHUNK = 16;
-// Far field of the FP32 gather.  A record whose line centre lies at least F4DFAR
+// Far field of the FP32 gather.  A record whose line centre lies at least half a tile
 // points outside the tile, and whose shape over the whole tile is the plain Lorentz
 // form (covered, no shift, no mirror), is a smooth function there: it is evaluated
 // at F4NF Chebyshev nodes of the tile instead of at every point, and the node sums
 // are interpolated to the points once per tile (Lagrange weights tabulated in the
-// context, F4Far).  For 1/(a^2 + x^2) with its centre F4DFAR = half a tile outside
+// context per tile width).  For 1/(a^2 + x^2) with its centre half a tile outside
 // the tile, 16 nodes interpolate to within 1.4e-8 of its maximum over the tile.
 constexpr int F4NF = 16;
-constexpr int F4DFAR = 32 * F4K32;
+constexpr int F4KMIN = 2, F4KMAX = 4;   // points per lane the gathers are built for
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int RAMP_CAP = 256;  // ramp pieces per panel held in LDS
 
@@ -1556,7 +1557,7 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
             const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
             F4Plan P;
             f4_plan(P, lane < nb, (int)pre[1].y, (int)pre[1].z, (int)pre[2].x, (int)pre[2].y, (int)pre[1].w, j0, j1,
-                    (int)pre[1].x, far_tau ? F4DFAR : 0);
+                    (int)pre[1].x, far_tau ? 32 * F4K : 0);
             if (lane < nb) nev += f4_count((int)pre[1].y, (int)pre[1].z, j0, j1);
             if (P.far) {
                 anyfar = true;
@@ -2151,6 +2152,7 @@ struct Batch {
     int64_t max_line = 0, max_f4 = 0, max_rec = 0, tot_line = 0, tot_f4 = 0, tot_rec = 0;
     int64_t tot_a = 0, tot_r4 = 0, tot_nat = 0, tot_out = 0, tot_f4rec = 0;
     int max_stride = 0;
+    int f4k = 3;                 // F4 gather points per lane (tile width 64 f4k)
     bool any_jrad1 = false, any_regrid = false;
 };
 
@@ -2165,8 +2167,12 @@ struct clblm_ctx {
     int f4_chunk = 16;       // consecutive F4 tiles per XCD run (xcd_tile)
     int f4_far = 1;          // far-field evaluation in the FP32 gather: 1 on, 2 off
     int f4_stream_mode = 1;  // 1: the F4 chain on its own stream; 2: on the batch's stream
-    double *far_w = nullptr; // Lagrange weights of the far-field nodes, [64 F4K32][F4NF]
-    float *far_tau = nullptr;   // node positions, tile points from the tile's first point
+    // far-field nodes per tile width 64 F4K (F4K = F4KMIN..F4KMAX): Lagrange weights
+    // [64 F4K][F4NF] and node positions (tile points from the tile's first point)
+    double *far_w[F4KMAX + 1] = {};
+    float *far_tau[F4KMAX + 1] = {};
+    int f4k32 = 3, f4k64 = 2;   // F4 gather points per lane, FP32C / FP64
+    int conv_u = 4;             // pass-1 R1 points per walk step
     double *tab = nullptr;   // F1,F2,F3,FG,XVER consecutive, NFMX+1 each
     float *tabs = nullptr;
     double4 *tabg = nullptr;
@@ -2296,9 +2302,10 @@ static int ctx_tables(clblm_ctx *c) {
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cf3), CLBLM_CF3, sizeof(CLBLM_CF3)));
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cer), CLBLM_CER, sizeof(CLBLM_CER)));
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_maxiso), CLBLM_MAXISO, sizeof(CLBLM_MAXISO)));
-    {   // far-field nodes of the FP32 gather: Chebyshev points of [0, W-1] (rounded to
+    for (int fk = F4KMIN; fk <= F4KMAX; fk++) {
+        // far-field nodes of the FP32 gather: Chebyshev points of [0, W-1] (rounded to
         // FP32, as the kernel uses them) and the Lagrange weights of every tile point
-        const int W = 64 * F4K32;
+        const int W = 64 * fk;
         const double PI = 3.14159265358979323846;
         std::vector<float> tau(F4NF);
         std::vector<double> td(F4NF), w((size_t)W * F4NF);
@@ -2313,10 +2320,10 @@ static int ctx_tables(clblm_ctx *c) {
                     if (k != i) v *= ((double)t - td[k]) / (td[i] - td[k]);
                 w[(size_t)t * F4NF + i] = v;
             }
-        HIPCHK(hipMalloc(&c->far_w, w.size() * sizeof(double)));
-        HIPCHK(hipMalloc(&c->far_tau, tau.size() * sizeof(float)));
-        HIPCHK(hipMemcpy(c->far_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->far_tau, tau.data(), tau.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&c->far_w[fk], w.size() * sizeof(double)));
+        HIPCHK(hipMalloc(&c->far_tau[fk], tau.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(c->far_w[fk], w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->far_tau[fk], tau.data(), tau.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -2493,8 +2500,9 @@ static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od
 }
 
 static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, const clblm_od_params *pars,
-                       int32_t *jrad_state, Batch &B, int precision) {
+                       int32_t *jrad_state, Batch &B, int precision, int f4k) {
     B = Batch{};
+    B.f4k = f4k;
     B.jobs.reserve(nlay);
     for (int l = 0; l < nlay; l++) {
         if (pars[l].precision != precision)
@@ -2539,7 +2547,7 @@ static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, 
             B.tot_r4 += LIMOUT4 + 2;
             int bi = (int)B.blks.size();
             B.blks.push_back(b);
-            const int tile = 64 * (precision == CLBLM_FP64 ? F4K64 : F4K32);
+            const int tile = 64 * B.f4k;
             for (int t = 0; t < (LIMOUT4 + 2 + tile - 1) / tile; t++) B.tiles.push_back(make_int2(bi, t));
         }
         B.max_line = std::max(B.max_line, J.line_n);
@@ -2741,13 +2749,21 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     const int xchunk = C->f4_chunk > 0 ? C->f4_chunk : 1;
     const unsigned f4grid = (unsigned)(8 * (int64_t)xchunk * ((ntile + 8 * xchunk - 1) / (8 * xchunk)));
     if (ntile > 0) {
-        if (precision == CLBLM_FP64)
-            hipLaunchKernelGGL(k_f4conv64<F4K64>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,
-                               at<F4Rec64>(base, o_rec), d_R4, C->d_work, ntile, xchunk, d_xjj);
-        else
-            hipLaunchKernelGGL(k_f4conv32<F4K32>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,
-                               at<F4Rec32>(base, o_rec), d_R4, C->d_work, ntile, xchunk, C->far_w,
-                               C->f4_far == 1 ? C->far_tau : nullptr);
+        const int fk = B.f4k;
+        const float *tau = C->f4_far == 1 ? C->far_tau[fk] : nullptr;
+#define F4_LAUNCH(K)                                                                                              \
+    if (precision == CLBLM_FP64)                                                                                  \
+        hipLaunchKernelGGL(k_f4conv64<K>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,              \
+                           at<F4Rec64>(base, o_rec), d_R4, C->d_work, ntile, xchunk, d_xjj);                      \
+    else                                                                                                          \
+        hipLaunchKernelGGL(k_f4conv32<K>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,              \
+                           at<F4Rec32>(base, o_rec), d_R4, C->d_work, ntile, xchunk, C->far_w[fk], tau);
+        switch (fk) {
+        case 2: F4_LAUNCH(2) break;
+        case 4: F4_LAUNCH(4) break;
+        default: F4_LAUNCH(3) break;
+        }
+#undef F4_LAUNCH
     }
     HIPCHK(hipEventRecord(S.ev[6], s4));
     if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, s4, d_jobs, d_blk, d_R4);
@@ -2762,12 +2778,20 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     }
     hipLaunchKernelGGL(k_ranges, dim3(cdiv(npan, 256)), dim3(256), 0, st, d_jobs, d_pan, npan, lp);
     HIPCHK(hipEventRecord(S.ev[1], st));
-    if (precision == CLBLM_FP64) {
-        hipLaunchKernelGGL(k_conv64, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
-                           d_jobs, d_pan, lp, T, d_A, C->d_work);
-    } else {
-        hipLaunchKernelGGL(k_conv32, dim3((unsigned)npan), dim3(256), (size_t)B.max_stride * sizeof(double), st,
-                           d_jobs, d_pan, lp, T, d_A, C->d_work);
+    {
+        const dim3 g((unsigned)npan), b(256);
+        const size_t lds = (size_t)B.max_stride * sizeof(double);
+#define CONV_LAUNCH(U)                                                                                          \
+    if (precision == CLBLM_FP64)                                                                                \
+        hipLaunchKernelGGL(k_conv64<U>, g, b, lds, st, d_jobs, d_pan, lp, T, d_A, C->d_work);                   \
+    else                                                                                                        \
+        hipLaunchKernelGGL(k_conv32<U>, g, b, lds, st, d_jobs, d_pan, lp, T, d_A, C->d_work);
+        switch (C->conv_u) {
+        case 2: CONV_LAUNCH(2) break;
+        case 8: CONV_LAUNCH(8) break;
+        default: CONV_LAUNCH(4) break;
+        }
+#undef CONV_LAUNCH
     }
     HIPCHK(hipEventRecord(S.ev[2], st));
     HIPCHK(hipStreamWaitEvent(st, S.join, 0));
@@ -2864,8 +2888,10 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     (void)hipFree(c->tabs);
     (void)hipFree(c->tabg);
     (void)hipFree(c->tabgs);
-    (void)hipFree(c->far_w);
-    (void)hipFree(c->far_tau);
+    for (int fk = F4KMIN; fk <= F4KMAX; fk++) {
+        (void)hipFree(c->far_w[fk]);
+        (void)hipFree(c->far_tau[fk]);
+    }
     (void)hipFree(c->d_err);
     (void)hipFree(c->d_work);
     if (c->rt_state) (void)hipFree(c->rt_state);
@@ -2975,7 +3001,8 @@ int clblm_od_batch(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
         c->plan_valid = false;
     } else {
         c->plan_valid = false;
-        int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec);
+        int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec,
+                             prec == CLBLM_FP64 ? c->f4k64 : c->f4k32);
         if (rc) return rc;
     }
     for (int l = 0; l < nlay; l++) offsets[l] = B.jobs[l].out_off;
@@ -2997,6 +3024,98 @@ int clblm_od_batch(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
     return 0;
 }
 
+int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value);
+int clblm_sync(clblm_ctx *c);
+
+// Coordinate search over launch-shape settings: each key in turn takes each of its
+// candidate values, the batch is run (one untimed run, then reps timed runs on the
+// context's stream) and the fastest value is kept.
+int clblm_autotune(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers, const clblm_od_params *pars,
+                   const int *keys, int nkeys, int reps, double *ms_best) {
+    if (!c || !layers || !pars || nlay <= 0 || nkeys < 0 || (nkeys > 0 && !keys))
+        return set_err(CLBLM_ERR_ARG, "null argument or bad sizes");
+    if (reps < 1) reps = 1;
+    const int prec = pars[0].precision;
+    std::vector<int> klist;
+    if (nkeys > 0) klist.assign(keys, keys + nkeys);
+    else klist = {prec == CLBLM_FP64 ? CLBLM_TUNE_F4_POINTS_FP64 : CLBLM_TUNE_F4_POINTS_FP32C, CLBLM_TUNE_CONV_UNROLL,
+                  CLBLM_TUNE_F4_XCD_CHUNK, CLBLM_TUNE_F4_STREAM};
+    auto candidates = [](int key) -> std::vector<int> {
+        switch (key) {
+        case CLBLM_TUNE_F4_POINTS_FP32C:
+        case CLBLM_TUNE_F4_POINTS_FP64: return {2, 3, 4};
+        case CLBLM_TUNE_CONV_UNROLL: return {2, 4, 8};
+        case CLBLM_TUNE_F4_XCD_CHUNK: return {4, 16, 64};
+        case CLBLM_TUNE_F4_STREAM: return {1, 2};
+        case CLBLM_TUNE_F4_FAR_FIELD: return {1, 2};
+        default: return {};
+        }
+    };
+    for (int k : klist)
+        if (candidates(k).empty()) return set_err(CLBLM_ERR_ARG, "unknown tuning key");
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<int64_t> off(nlay + 1);
+    int32_t jr0 = 1, jr = jr0;
+    int rc = clblm_od_batch(c, handle, nlay, layers, pars, &jr, off.data(), nullptr, nullptr);
+    if (rc) return rc;
+    double *d_od = nullptr;
+    if (hipMalloc(&d_od, sizeof(double) * (size_t)std::max<int64_t>(1, off[nlay])) != hipSuccess)
+        return set_err(CLBLM_ERR_HIP, "autotune output allocation failed");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(d_od);
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        return set_err(CLBLM_ERR_HIP, "autotune events");
+    }
+    auto run = [&](double *ms) -> int {
+        jr = jr0;
+        int r = clblm_od_batch(c, handle, nlay, layers, pars, &jr, off.data(), d_od, c->stream);
+        if (r) return r;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return set_err(CLBLM_ERR_HIP, "autotune event record");
+        for (int i = 0; i < reps; i++) {
+            jr = jr0;
+            r = clblm_od_batch(c, handle, nlay, layers, pars, &jr, off.data(), d_od, c->stream);
+            if (r) return r;
+        }
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return set_err(CLBLM_ERR_HIP, "autotune event wait");
+        r = clblm_sync(c);
+        if (r) return r;
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        *ms = t / reps;
+        return 0;
+    };
+    double best = 0.;
+    rc = run(&best);
+    for (int key : klist) {
+        if (rc) break;
+        int cur = 0;
+        rc = clblm_tuning(c, key, 0, &cur);
+        if (rc) break;
+        int bestv = cur;
+        for (int v : candidates(key)) {
+            if (v == cur) continue;
+            rc = clblm_tuning(c, key, v, nullptr);
+            if (rc) break;
+            double t = 0.;
+            rc = run(&t);
+            if (rc) break;
+            if (t < best) { best = t; bestv = v; }
+        }
+        if (rc) break;
+        rc = clblm_tuning(c, key, bestv, nullptr);
+    }
+    cleanup();
+    if (rc) return rc;
+    if (ms_best) *ms_best = best;
+    return 0;
+}
+
 int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value) {
     if (!c) return set_err(CLBLM_ERR_ARG, "ctx is NULL");
     int *slot = nullptr;
@@ -3005,11 +3124,15 @@ int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value) {
     case CLBLM_TUNE_F4_XCD_CHUNK: slot = &c->f4_chunk; lo = 1; hi = 4096; break;
     case CLBLM_TUNE_F4_FAR_FIELD: slot = &c->f4_far; lo = 1; hi = 2; break;
     case CLBLM_TUNE_F4_STREAM: slot = &c->f4_stream_mode; lo = 1; hi = 2; break;
+    case CLBLM_TUNE_F4_POINTS_FP32C: slot = &c->f4k32; lo = F4KMIN; hi = F4KMAX; break;
+    case CLBLM_TUNE_F4_POINTS_FP64: slot = &c->f4k64; lo = F4KMIN; hi = F4KMAX; break;
+    case CLBLM_TUNE_CONV_UNROLL: slot = &c->conv_u; lo = 2; hi = 8; break;
     default: return set_err(CLBLM_ERR_ARG, "unknown tuning key");
     }
     if (old_value) *old_value = *slot;
     if (value == 0) return 0;   // query only
-    if (value < lo || value > hi) return set_err(CLBLM_ERR_ARG, "tuning value out of range");
+    if (value < lo || value > hi || (key == CLBLM_TUNE_CONV_UNROLL && value != 2 && value != 4 && value != 8))
+        return set_err(CLBLM_ERR_ARG, "tuning value out of range");
     std::lock_guard<std::mutex> g(c->mu);
     *slot = value;
     c->plan_valid = false;

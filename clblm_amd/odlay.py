@@ -148,6 +148,27 @@ class Context:
         check(self.lib.clblm_tuning(self.ptr, int(key), int(value), ctypes.byref(old)))
         return old.value
 
+    def autotune(self, handle, layers, params, keys=None, reps=2):
+        """Runs the library's autotuner on this batch (clblm_autotune) and returns the
+        chosen settings {key: value} and the time per batch in ms."""
+        nl = len(layers)
+        lay = (_lib.Layer * nl)(*[_layer_struct(l) for l in layers])
+        par = (ODParams * nl)(*params)
+        karr = None
+        nk = 0
+        if keys:
+            karr = (ctypes.c_int * len(keys))(*keys)
+            nk = len(keys)
+        ms = ctypes.c_double()
+        check(self.lib.clblm_autotune(self.ptr, handle, nl, ctypes.cast(lay, ctypes.c_void_p),
+                                      ctypes.cast(par, ctypes.c_void_p),
+                                      ctypes.cast(karr, ctypes.c_void_p) if karr is not None else None, nk,
+                                      int(reps), ctypes.byref(ms)))
+        chosen = {k: self.tuning(k) for k in (_lib.TUNE_F4_POINTS_FP32C, _lib.TUNE_F4_POINTS_FP64,
+                                              _lib.TUNE_CONV_UNROLL, _lib.TUNE_F4_XCD_CHUNK,
+                                              _lib.TUNE_F4_STREAM, _lib.TUNE_F4_FAR_FIELD)}
+        return chosen, ms.value
+
     def sync(self):
         """Wait for the enqueued batches; raises the first error found on the device."""
         check(self.lib.clblm_sync(self.ptr))

@@ -110,11 +110,28 @@ int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
  *                            point.  CLBLM_FP64 always sums every point.
  *   CLBLM_TUNE_F4_STREAM     1: the F4 wing chain of a batch runs on a stream of
  *                            the context beside pass 1 (default); 2: every kernel
- *                            of the batch on the batch's stream, one after another. */
+ *                            of the batch on the batch's stream, one after another.
+ *   CLBLM_TUNE_F4_POINTS_FP32C, CLBLM_TUNE_F4_POINTS_FP64  R4 points per lane of the
+ *                            F4 gather (2-4; a tile is 64 times that), per precision.
+ *   CLBLM_TUNE_CONV_UNROLL   R1 points per step of the pass-1 walk (2, 4 or 8). */
 #define CLBLM_TUNE_F4_XCD_CHUNK 1
 #define CLBLM_TUNE_F4_FAR_FIELD 2
 #define CLBLM_TUNE_F4_STREAM 3
+#define CLBLM_TUNE_F4_POINTS_FP32C 4
+#define CLBLM_TUNE_F4_POINTS_FP64 5
+#define CLBLM_TUNE_CONV_UNROLL 6
 int clblm_tuning(clblm_ctx *ctx, int key, int value, int *old_value);
+
+/* Autotuner of the launch shapes (no counterpart in the reference).  Runs the batch
+ * (layers, pars as for clblm_od_batch; the precision is pars[0].precision) with each
+ * candidate value of each key in turn and keeps the fastest, so later batches of the
+ * context use it.  keys = NULL (nkeys 0) tunes the F4 points per lane of that
+ * precision, CLBLM_TUNE_CONV_UNROLL, CLBLM_TUNE_F4_XCD_CHUNK and CLBLM_TUNE_F4_STREAM.
+ * reps timed runs per candidate after one untimed run; *ms_best (if non-NULL) receives
+ * the time per batch of the chosen settings.  Query them with clblm_tuning(key, 0, &v).
+ * Allocates one output buffer for the duration of the call.  Synchronous. */
+int clblm_autotune(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *layers,
+                   const clblm_od_params *pars, const int *keys, int nkeys, int reps, double *ms_best);
 
 /* Waits for every batch enqueued on the context and returns the first device-found
  * error among them (0 if none). */
