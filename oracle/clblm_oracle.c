@@ -1943,3 +1943,166 @@ void orc_fft_apod(const orc_scan_plan *p, double *h) {
         }
     }
 }
+
+/* ---------------------------------------------------------------------------
+ * clblm_SCANFN_struct (clblm_ScanFilter.f90:70-544) for functID 1 (JFN 0, the
+ * rectangle of adjacent boxes, CNVRCT :1478-1698), with panelInput (:2320-2404,
+ * NREN = 0) and panelOutput (:2411-2506).  The spectrum x[0..n) lies on
+ * v1 + i dv.  This is the scan output_JacArray applies to Jacobian arrays when a
+ * boxcar pre-average is asked for (clblm_noScattJac.f90:1341-1409).
+ * Returns the output count (writes min(count, cap) values) and the output grid in
+ * *v1res, *dvres; -1 on error.  When IRATIO < 2 the reference returns the input
+ * unchanged: so does this (count n, grid v1, dv).
+ * ------------------------------------------------------------------------- */
+typedef struct {   /* CNVRCT's SAVE variables and flags */
+    double sumj, rnj;
+    int jn, nb, ipc;
+} rect_state;
+
+static void rect_cnvrct(const double *S, double HWHM, double *R1, double V1I, double V2I, double DVI, int *NNI,
+                        int NLO, int NHI, double V1, double V2, double DVO, int IDATA, int *IPANEL, int *JFLG,
+                        rect_state *st) {
+    if (*IPANEL == -1) {
+        st->sumj = 0.; st->rnj = 0.; st->jn = NLO;
+        if (*JFLG == -1) { st->nb = 1; st->ipc = 1; }
+    }
+    int NNIV2 = (int)((V2I - V1I) / DVI + 1.0001);
+    if (*NNI > NNIV2) *NNI = NNIV2;
+    while (NLO <= NHI) {
+        double VCNT = V1 + (double)(st->nb - 1) * DVO;
+        if (VCNT > V2) { *IPANEL = 1; return; }
+        double VLFT = VCNT - HWHM, VRGT = VCNT + HWHM;
+        double RL = (VLFT - V1I) / DVI + 1, RR = (VRGT - V1I) / DVI + 1;
+        int IL = (int)(RL + 0.5), IH = (int)(RR + 0.5);
+        double VLBLR = (V1I + (double)(IL - 1) * DVI) + DVI / 2.;
+        double WGTL = (VLBLR - VLFT) / DVI;
+        double VLBLL = (V1I + (double)(IH - 1) * DVI) - DVI / 2.;
+        double WGTR = (VRGT - VLBLL) / DVI;
+        int NEP = 2;
+        if (IH > *NNI) {
+            IH = *NNI;
+            *JFLG = 1;
+            WGTR = 1.;
+            if (IL == IH) { WGTR = 0.; NEP = 1; }
+        }
+        if (IL <= 1) {
+            IL = 1;
+            WGTL = 1.;
+            if (IL >= IH) { WGTL = 0.; NEP = 1; }
+        }
+        if (*JFLG == 1 && IDATA == 0 && IL > *NNI) { *IPANEL = 0; *JFLG = 0; return; }
+        if (*JFLG == 1 && IDATA == 1) return;
+        for (int I = IL + 1; I <= IH - 1; I++) st->sumj = st->sumj + S[I];
+        st->sumj = st->sumj + S[IL] * WGTL;
+        st->sumj = st->sumj + S[IH] * WGTR;
+        st->rnj = st->rnj + (double)(IH - IL + 1 - NEP) + WGTL + WGTR;
+        if (*JFLG == 1 && IDATA == 0) { *IPANEL = 0; *JFLG = 0; return; }
+        R1[st->jn] = st->sumj / st->rnj;
+        st->jn = st->jn + 1;
+        st->sumj = 0.; st->rnj = 0.;
+        if (st->nb == st->ipc * (NHI - NLO + 1)) {
+            *IPANEL = 1;
+            st->ipc = st->ipc + 1;
+            st->nb = st->nb + 1;
+            return;
+        }
+        st->nb = st->nb + 1;
+    }
+}
+
+int64_t orc_scanfn_rect(const double *x, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
+                        double hwhm, double *out, int64_t cap, double *v1res, double *dvres) {
+    if (!(hwhm > 0.) || n < 1) return fail("rectangle scan: bad half width or empty input");
+    const double DVINT = hwhm / 12.;
+    const double SAMPLE = hwhm / fabs(dvout);
+    double *spec = NULL;
+    int64_t ns = n;
+    double sv1 = v1, sdv = dv;
+    if (dv > DVINT) {   /* clblm_INTERP onto DVINT first (:331-334) */
+        double v2 = v1 + dv * (double)(n - 1), r1 = 0.;
+        ns = orc_scan_interp(x, n, v1, dv, v1, v2, DVINT, NULL, 0, &r1);
+        if (ns <= 0) return -1;
+        spec = (double *)malloc(sizeof(double) * (size_t)ns);
+        orc_scan_interp(x, n, v1, dv, v1, v2, DVINT, spec, ns, &r1);
+        sv1 = r1;
+        sdv = DVINT;
+    }
+    const double *sp = spec ? spec : x;
+    const double DVI0 = sdv;
+    double DVO = hwhm / SAMPLE;
+    const int IRATIO = (int)(DVO / DVI0 + 0.5);
+    DVO = (double)IRATIO * DVI0;
+    if (IRATIO < 2) {   /* no scan: the (interpolated) input is the output */
+        for (int64_t i = 0; i < ns && i < cap; i++) out[i] = sp[i];
+        *v1res = sv1; *dvres = sdv;
+        free(spec);
+        return ns;
+    }
+    const double HWF = 1.;
+    const int NLIMF = 2401, NSHIFT = 32;
+    const int NBOUND = (int)((2. * HWF) * SAMPLE + 0.01);
+    const double BOUND = (double)NBOUND * DVO / 2.;
+    const int MAXF = NLIMF + 2 * NBOUND + NSHIFT;
+    const double V1 = v1out, V2 = v2out;
+    int NLO = NSHIFT + 1, NHI = NLIMF + NSHIFT - 1;
+    double *R1 = (double *)calloc((size_t)MAXF + 2, sizeof(double));
+    double *S = (double *)calloc(2402, sizeof(double));
+    int IEOFSC = 1, IDATA = -1, IPANEL = -1, JFLG = -1, NNI = 0, ISTOP = 0;
+    double VFT = V1 - (double)NSHIFT * DVO;
+    const double VBOT = V1 - BOUND, VTOP = V2 + BOUND;
+    int64_t done = 0, nout = 0;
+    double done_v2 = sv1 - sdv;
+    double V1I = 0., V2I = 0., DVI = sdv;
+    rect_state st = {0., 0., 0, 1, 1};
+    int state = 40;
+    for (;;) {
+        if (state == 40) {
+            if (IEOFSC <= 0) { state = 60; continue; }
+            /* panelInput with NREN = 0 */
+            int nlim;
+            for (;;) {
+                nlim = 2400;
+                if (done + nlim > ns) nlim = (int)(ns - done);
+                if (nlim == 0) { IEOFSC = 0; break; }
+                DVI = sdv;
+                V1I = done_v2 + DVI;
+                V2I = V1I + (double)(nlim - 1) * DVI;
+                done += nlim;
+                done_v2 = V2I;
+                if (V2I >= VBOT) break;
+            }
+            if (IEOFSC <= 0) { state = 60; continue; }
+            NNI = nlim;
+            IDATA = 0;
+            for (int i = 0; i < nlim; i++) S[1 + i] = sp[done - nlim + i];
+            if (V2I > VTOP) IDATA = 1;
+            state = 50;
+        }
+        if (state == 50) {
+            rect_cnvrct(S, hwhm, R1, V1I, V2I, DVI, &NNI, NLO, NHI, V1, V2, DVO, IDATA, &IPANEL, &JFLG, &st);
+            if (IPANEL == 0) { state = 40; continue; }
+            state = 60;
+        }
+        /* 60: panelOutput */
+        ISTOP = 0;
+        int NNHI = (int)((V2 - VFT) / DVO + 1.5);
+        if (NHI >= NNHI) ISTOP = 1;
+        if (ISTOP == 1) NHI = NNHI;
+        for (int j = NLO; j <= NHI; j++, nout++)
+            if (nout < cap) out[nout] = R1[j];
+        VFT = VFT + (double)(NLIMF - 1) * DVO;
+        if (ISTOP != 1) {
+            for (int J = NLIMF; J <= MAXF; J++) R1[J - NLIMF + 1] = R1[J];
+            for (int J = MAXF - NLIMF + 2; J <= MAXF; J++) R1[J] = 0.;
+            NLO = NSHIFT + 1;
+        }
+        IPANEL = -1;
+        if (ISTOP != 1 && IEOFSC < 0) { state = 60; continue; }
+        if (ISTOP != 1 && IEOFSC > 0) { state = 50; continue; }
+        break;
+    }
+    *v1res = V1;
+    *dvres = DVO;
+    free(R1); free(S); free(spec);
+    return nout;
+}

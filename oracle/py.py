@@ -270,3 +270,33 @@ def level_jac(od_var, dr_dtau, dr_dtlev, wtot, tave, dt, dw_frac, dt_dtup, dt_dt
     L.orc_level_jac(nlay, nmol, n, _p(od_var), a[0], a[1], a[2], a[3], dt, dw_frac, a[4], a[5], a[6], a[7], _p(jt),
                     _p(jq))
     return jt, jq
+
+
+def scanfn_rect(x, v1, dv, v1out, v2out, dvout, hwhm):
+    """orc_scanfn_rect: the rectangle (boxcar) spectral-domain scan; returns (values, v1, dv)."""
+    L = lib()
+    L.orc_scanfn_rect.restype = ctypes.c_int64
+    L.orc_scanfn_rect.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_double] * 6 + \
+        [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    x = np.ascontiguousarray(x, np.float64)
+    a, b = ctypes.c_double(), ctypes.c_double()
+    n = L.orc_scanfn_rect(x.ctypes.data, x.size, v1, dv, v1out, v2out, dvout, hwhm, None, 0, ctypes.byref(a),
+                          ctypes.byref(b))
+    if n < 0:
+        raise RuntimeError(L.orc_error().decode())
+    out = np.zeros(n)
+    L.orc_scanfn_rect(x.ctypes.data, x.size, v1, dv, v1out, v2out, dvout, hwhm, out.ctypes.data, n, ctypes.byref(a),
+                      ctypes.byref(b))
+    return out, a.value, b.value
+
+
+def prebox(x, v1, dv, boxcar_hw):
+    """output_JacArray's boxcar pre-average (clblm_noScattJac.f90:1341-1409): the
+    rectangle scan with M = int(2 boxcarHW / dv) points per box onto
+    v1 + (M-1) dv/2 .. v2 - (M-1) dv/2 at (M-1) dv; the input unchanged when M <= 1
+    or boxcarHW <= dv."""
+    m = int(2.0 * boxcar_hw / dv)
+    if not (boxcar_hw > 0.0 and boxcar_hw > dv) or m <= 1:
+        return np.asarray(x, np.float64).copy(), v1, dv
+    v2 = v1 + dv * (len(x) - 1)
+    return scanfn_rect(x, v1, dv, v1 + (m - 1) * dv / 2.0, v2 - (m - 1) * dv / 2.0, (m - 1) * dv, boxcar_hw)

@@ -10,8 +10,32 @@ module ref_bind
    use Module_Spectrum,        only: XINT, interpSpectrum, CLBLM_Spectrum, CLBLM_Spectrum_init
    use Module_EMLAY,           only: planck, planckFnDrv, rlin
    use Module_FFT,             only: clblm_FFTSCN, Intpdr
+   use Module_ScanFilter,      only: clblm_SCANFN
    implicit none
 contains
+
+   ! clblm_SCANFN_array (clblm_ScanFilter.f90:33-67): spectral-domain scan of sp(1:np)
+   ! on v1 + (i-1) dv with scanning function functid of half width hwhm onto
+   ! v1out..v2out at dvout.
+   subroutine ref_scanfn(np, sp, v1, v2, dv, v1out, v2out, dvout, functid, hwhm, cap, out, nout, &
+                         v1o, v2o, dvo) bind(C, name="ref_scanfn")
+      integer(c_int), value :: np, functid, cap
+      real(c_double), value :: v1, v2, dv, v1out, v2out, dvout, hwhm
+      real(c_double), intent(in) :: sp(np)
+      real(c_double), intent(out) :: out(cap)
+      integer(c_int), intent(out) :: nout
+      real(c_double), intent(out) :: v1o, v2o, dvo
+      real, allocatable :: d(:)
+      real(r8) :: a, b, c1, c2
+      real :: dvs, dvx, hw
+      integer :: n
+      a = v1; b = v2; c1 = v1out; c2 = v2out; dvs = dv; dvx = dvout; hw = hwhm
+      n = 0
+      call clblm_SCANFN(sp, a, b, dvs, np, d, c1, c2, dvx, n, functid, hw)
+      nout = n
+      v1o = c1; v2o = c2; dvo = dvx
+      out(1:min(n, cap)) = d(1:min(n, cap))
+   end subroutine
 
    ! Intpdr (clblm_FFT.f90:2392-2443): sp(1:np) on v1 + (i-1) dv onto vv1..vv2 at dvv
    subroutine ref_intpdr(np, sp, v1, dv, vv1, vv2, dvv, cap, out, nout, v1o) bind(C, name="ref_intpdr")

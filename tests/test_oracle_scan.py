@@ -198,3 +198,60 @@ def test_functions_the_reference_cannot_run_are_rejected(fid):
     from oracle.py import fft_scan
     with pytest.raises(RuntimeError):
         fft_scan(np.ones(20001), 600.0, 0.01, 620.0, 780.0, 0.0, fid, 0.6)
+
+
+def _ref_scanfn(lib, x, v1, dv, v1o, v2o, dvo, fid, hw):
+    lib.ref_scanfn.argtypes = [I, P, D, D, D, D, D, D, I, D, I, P, ctypes.POINTER(I), ctypes.POINTER(D),
+                               ctypes.POINTER(D), ctypes.POINTER(D)]
+    out = np.zeros(400000)
+    n, a, b, c = I(), D(), D(), D()
+    lib.ref_scanfn(x.size, x.ctypes.data, v1, v1 + dv * (x.size - 1), dv, v1o, v2o, dvo, fid, hw, out.size,
+                   out.ctypes.data, ctypes.byref(n), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return out[:n.value], a.value, c.value
+
+
+RECT_CASES = [  # (points, dv, boxcar half width): boxcar pre-averages of Jacobian arrays
+    (20001, 0.01, 0.05), (20001, 0.01, 0.253), (9001, 0.002, 0.0155), (7001, 0.01, 0.07),
+    (30001, 0.004, 0.61), (5001, 0.01, 0.0151), (3001, 0.01, 0.012), (12345, 0.0031, 0.04),
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("n,dv,hw", RECT_CASES)
+def test_rect_scan_matches_reference(ref, n, dv, hw):
+    """The rectangle scan of output_JacArray's pre-average (clblm_SCANFN, functID 1,
+    CNVRCT) against the reference's own routine: identical grid and bit-identical
+    values, across input panels of 2400 points, with the interpolation onto HWHM/12
+    when the input is coarser (hw = 0.07 at dv = 0.01) and without a scan when the
+    boxes hold fewer than two points (hw = 0.012 at dv = 0.01)."""
+    from oracle.py import prebox
+    x = _spectrum(n, seed=11)[1]
+    v1 = 600.0
+    m = int(2.0 * hw / dv)
+    v2 = v1 + dv * (n - 1)
+    got, g1, gd = prebox(x, v1, dv, hw)
+    want, w1, wd = _ref_scanfn(ref, x, v1, dv, v1 + (m - 1) * dv / 2.0, v2 - (m - 1) * dv / 2.0, (m - 1) * dv, 1, hw)
+    assert (got.size, g1, gd) == (want.size, w1, wd), ((got.size, g1, gd), (want.size, w1, wd))
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+
+
+def test_rect_scan_closed_forms():
+    """A constant spectrum stays constant inside the data (the weights sum to the
+    count).  On a binary-exact grid (dv = 0.0625, half width 1.25, boxes of 40 points
+    whose edges fall on cell boundaries) every box is the plain mean of its 40
+    points, so a linear ramp keeps its value at each box centre, across the
+    2400-point input panels as well."""
+    from oracle.py import prebox
+    n, dv, hw = 20001, 0.01, 0.1
+    v1 = 600.0
+    got, g1, gd = prebox(np.full(n, 3.25), v1, dv, hw)
+    inner = got[5:-5]
+    assert np.allclose(inner, 3.25, rtol=1e-14, atol=0)
+    n, dv, hw = 8001, 0.0625, 1.25
+    ramp = 2.0 + 0.5 * (v1 + dv * np.arange(n))
+    got, g1, gd = prebox(ramp, v1, dv, hw)
+    assert gd == 2.4375 and g1 == 601.21875
+    centres = g1 + gd * np.arange(got.size)
+    ok = (centres > v1 + 2 * hw) & (centres < v1 + dv * (n - 1) - 2 * hw)
+    assert ok.sum() > 150
+    assert np.allclose(got[ok], 2.0 + 0.5 * centres[ok], rtol=1e-14, atol=0)
