@@ -14,7 +14,7 @@ FP64, FP32C = 0, 1
 EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
            "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
-           "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune"]
+           "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune", "clblm_scanfn"]
 TUNE_F4_XCD_CHUNK = 1
 TUNE_F4_FAR_FIELD = 2
 TUNE_F4_STREAM = 3
@@ -88,6 +88,9 @@ def load():
     lib.clblm_work.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.clblm_tuning.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.clblm_scanfn.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_double] * 5 + \
+        [ctypes.c_int, ctypes.c_double, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), P]
     lib.clblm_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_double)]
     _lib = lib

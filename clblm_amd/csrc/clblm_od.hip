@@ -2845,6 +2845,11 @@ extern "C" int clblm_fft_scan_impl(void *state, const double *d_in, int64_t n, d
                                    double pre_box_hw, int deconv_pre_box, int precision, double *d_out, int64_t *nout,
                                    double *v1res, double *dvres, void *stream);
 
+extern "C" int clblm_scanfn_impl(void *state, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in,
+                                 double v1, double dv, double v1out, double v2out, double dvout, int funct_id,
+                                 double hwhm, double *d_out, int64_t ld_out, int64_t *nout, double *v1res,
+                                 double *dvres, void *stream);
+
 extern "C" {
 
 int clblm_version(void) { return 1; }
@@ -3171,6 +3176,24 @@ int clblm_odlay(clblm_ctx *c, int handle, const clblm_layer *layer, const clblm_
     }
     (void)hipFree(d);
     return rc;
+}
+
+int clblm_scanfn(clblm_ctx *c, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1, double dv,
+                 double v1out, double v2out, double dvout, int funct_id, double hwhm, double *d_out, int64_t ld_out,
+                 int64_t *nout, double *v1res, double *dvres, void *stream) {
+    if (!c || !nout || !v1res || !dvres) return set_err(CLBLM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->scan) c->scan = clblm_scan_state_new();
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (d_out && c->last_stream && c->last_stream != st && c->ncalls > 0) {  // after the batches that made d_in
+        const int kp = (int)((c->ncalls + clblm_ctx::NSLOT - 1) % clblm_ctx::NSLOT);
+        HIPCHK(hipStreamWaitEvent(st, c->slot[kp].done, 0));
+    }
+    int rc = clblm_scanfn_impl(c->scan, d_in, n, nrows, ld_in, v1, dv, v1out, v2out, dvout, funct_id, hwhm, d_out,
+                               ld_out, nout, v1res, dvres, st);
+    if (rc) return set_err(rc, clblm_scan_error());
+    return 0;
 }
 
 int clblm_fft_scan(clblm_ctx *c, const double *d_in, int64_t n, double v1, double dv, double v1out, double v2out,

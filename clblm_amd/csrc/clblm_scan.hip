@@ -840,3 +840,280 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_fft_scan_impl(
     SCHK(hipGetLastError());
     return 0;
 }
+
+// ==================================================================================
+// Spectral-domain rectangle scan: clblm_SCANFN_struct (clblm_ScanFilter.f90:70-544)
+// for functID 1 (JFN 0): adjacent boxes of width 2 HWHM every DVO, each the weighted
+// mean of the points it covers (CNVRCT :1478-1698), the input read in 2400-point
+// panels (panelInput :2320-2404, NREN = 0) and the output written in 2400-point
+// panels (panelOutput :2411-2506).  It is the boxcar pre-average output_JacArray
+// applies to Jacobian arrays (clblm_noScattJac.f90:1341-1409).
+//
+//   host    CNVRCT's control flow depends on the grids only: it is walked here, and
+//           each box becomes its parts (a box split by an input panel edge has two),
+//           each part the input range it sums and its two end weights, in the
+//           reference's order of additions
+//   device  one lane per box: the sum in that order, divided by the weight sum
+// ==================================================================================
+namespace {
+
+struct RPart {
+    int64_t il, ih;    // global input indices of the part's end points
+    double wl, wr;     // their weights; the points between have weight 1
+};
+struct RBox {
+    int64_t out;       // output index
+    int32_t p0, np;    // its parts
+    double rnj;        // sum of the weights
+};
+
+struct RectPlan {
+    bool scan = true;          // false: IRATIO < 2, the input is the output
+    double v1 = 0., dvo = 0.;
+    int64_t nout = 0;
+    std::vector<RPart> parts;
+    std::vector<RBox> boxes;
+};
+
+// The reference's walk with the data left out: panelInput's accumulated panel
+// frequencies, CNVRCT's boxes and partial sums, panelOutput's panels.
+void plan_rect(int64_t ns, double sv1, double sdv, double V1, double V2, double dvout, double HWHM, RectPlan &rp) {
+    const double SAMPLE = HWHM / std::fabs(dvout);
+    double DVO = HWHM / SAMPLE;
+    const int IRATIO = (int)(DVO / sdv + 0.5);
+    DVO = (double)IRATIO * sdv;
+    if (IRATIO < 2) {
+        rp.scan = false;
+        rp.v1 = sv1;
+        rp.dvo = sdv;
+        rp.nout = ns;
+        return;
+    }
+    const int NLIMF = 2401, NSHIFT = 32;
+    const int NBOUND = (int)(2. * SAMPLE + 0.01);
+    const double BOUND = (double)NBOUND * DVO / 2.;
+    int NLO = NSHIFT + 1, NHI = NLIMF + NSHIFT - 1;
+    int IEOFSC = 1, IDATA = -1, IPANEL = -1, JFLG = -1, NNI = 0;
+    double VFT = V1 - (double)NSHIFT * DVO;
+    const double VBOT = V1 - BOUND, VTOP = V2 + BOUND;
+    int64_t done = 0, panel0 = 0, nout = 0;
+    double done_v2 = sv1 - sdv, V1I = 0., V2I = 0., DVI = sdv;
+    // CNVRCT's SAVE state; the current box's parts and weight sum
+    int JN = 0, NB = 1, IPC = 1;
+    double RNJ = 0.;
+    std::vector<RPart> cur;
+    int64_t panel_out0 = 0;    // output index of R1(NLO) of the current output panel
+    auto cnvrct = [&]() {
+        if (IPANEL == -1) {
+            cur.clear();
+            RNJ = 0.;
+            JN = NLO;
+            if (JFLG == -1) { NB = 1; IPC = 1; }
+        }
+        const int NNIV2 = (int)((V2I - V1I) / DVI + 1.0001);
+        if (NNI > NNIV2) NNI = NNIV2;
+        while (NLO <= NHI) {
+            const double VCNT = V1 + (double)(NB - 1) * DVO;
+            if (VCNT > V2) { IPANEL = 1; return; }
+            const double VLFT = VCNT - HWHM, VRGT = VCNT + HWHM;
+            const double RL = (VLFT - V1I) / DVI + 1, RR = (VRGT - V1I) / DVI + 1;
+            int IL = (int)(RL + 0.5), IH = (int)(RR + 0.5);
+            const double VLBLR = (V1I + (double)(IL - 1) * DVI) + DVI / 2.;
+            double WGTL = (VLBLR - VLFT) / DVI;
+            const double VLBLL = (V1I + (double)(IH - 1) * DVI) - DVI / 2.;
+            double WGTR = (VRGT - VLBLL) / DVI;
+            int NEP = 2;
+            if (IH > NNI) {
+                IH = NNI;
+                JFLG = 1;
+                WGTR = 1.;
+                if (IL == IH) { WGTR = 0.; NEP = 1; }
+            }
+            if (IL <= 1) {
+                IL = 1;
+                WGTL = 1.;
+                if (IL >= IH) { WGTL = 0.; NEP = 1; }
+            }
+            if (JFLG == 1 && IDATA == 0 && IL > NNI) { IPANEL = 0; JFLG = 0; return; }
+            if (JFLG == 1 && IDATA == 1) return;
+            cur.push_back(RPart{panel0 + IL - 1, panel0 + IH - 1, WGTL, WGTR});
+            RNJ = RNJ + (double)(IH - IL + 1 - NEP) + WGTL + WGTR;
+            if (JFLG == 1 && IDATA == 0) { IPANEL = 0; JFLG = 0; return; }
+            RBox b;
+            b.out = panel_out0 + (JN - (NSHIFT + 1));
+            b.p0 = (int32_t)rp.parts.size();
+            b.np = (int32_t)cur.size();
+            b.rnj = RNJ;
+            rp.parts.insert(rp.parts.end(), cur.begin(), cur.end());
+            rp.boxes.push_back(b);
+            JN = JN + 1;
+            cur.clear();
+            RNJ = 0.;
+            if (NB == IPC * (NHI - NLO + 1)) {
+                IPANEL = 1;
+                IPC = IPC + 1;
+                NB = NB + 1;
+                return;
+            }
+            NB = NB + 1;
+        }
+    };
+    int state = 40;
+    for (;;) {
+        if (state == 40) {
+            if (IEOFSC <= 0) { state = 60; continue; }
+            int nlim;
+            for (;;) {
+                nlim = 2400;
+                if (done + nlim > ns) nlim = (int)(ns - done);
+                if (nlim == 0) { IEOFSC = 0; break; }
+                DVI = sdv;
+                V1I = done_v2 + DVI;
+                V2I = V1I + (double)(nlim - 1) * DVI;
+                done += nlim;
+                done_v2 = V2I;
+                if (V2I >= VBOT) break;
+            }
+            if (IEOFSC <= 0) { state = 60; continue; }
+            NNI = nlim;
+            panel0 = done - nlim;
+            IDATA = V2I > VTOP ? 1 : 0;
+            state = 50;
+        }
+        if (state == 50) {
+            cnvrct();
+            if (IPANEL == 0) { state = 40; continue; }
+        }
+        // panelOutput
+        int ISTOP = 0;
+        const int NNHI = (int)((V2 - VFT) / DVO + 1.5);
+        if (NHI >= NNHI) ISTOP = 1;
+        if (ISTOP == 1) NHI = NNHI;
+        const int64_t cnt = NHI >= NLO ? NHI - NLO + 1 : 0;
+        nout += cnt;
+        panel_out0 += cnt;
+        VFT = VFT + (double)(NLIMF - 1) * DVO;
+        if (ISTOP != 1) NLO = NSHIFT + 1;
+        IPANEL = -1;
+        if (ISTOP != 1 && IEOFSC < 0) { state = 60; continue; }
+        if (ISTOP != 1 && IEOFSC > 0) { state = 50; continue; }
+        break;
+    }
+    // boxes past the last output panel's end are never written out
+    std::vector<RBox> kept;
+    kept.reserve(rp.boxes.size());
+    for (const RBox &b : rp.boxes)
+        if (b.out >= 0 && b.out < nout) kept.push_back(b);
+    rp.boxes.swap(kept);
+    rp.v1 = V1;
+    rp.dvo = DVO;
+    rp.nout = nout;
+}
+
+// row blockIdx.y of the arrays: input at x + y ldx, output at out + y ldo
+__global__ void k_rect_fill(double *out, int64_t n, int64_t ldo) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[blockIdx.y * ldo + i] = 0.;
+}
+
+__global__ void k_rect_boxes(const RBox *boxes, int64_t nbox, const RPart *parts, const double *x, int64_t ldx,
+                             double *out, int64_t ldo) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nbox) return;
+    x += blockIdx.y * ldx;
+    out += blockIdx.y * ldo;
+    const RBox b = boxes[q];
+    double sum = 0.;
+    for (int k = 0; k < b.np; k++) {
+        const RPart p = parts[b.p0 + k];
+        for (int64_t i = p.il + 1; i <= p.ih - 1; i++) sum = sum + x[i];
+        sum = sum + x[p.il] * p.wl;
+        sum = sum + x[p.ih] * p.wr;
+    }
+    out[b.out] = sum / b.rnj;
+}
+
+__global__ void k_rect_copy(const double *x, int64_t ldx, int64_t n, double *out, int64_t ldo) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[blockIdx.y * ldo + i] = x[blockIdx.y * ldx + i];
+}
+
+}  // namespace
+
+extern "C" __attribute__((visibility("hidden"))) int clblm_scanfn_impl(
+    void *state, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1, double dv, double v1out,
+    double v2out, double dvout, int funct_id, double hwhm, double *d_out, int64_t ld_out, int64_t *nout,
+    double *v1res, double *dvres, void *stream) {
+    if (funct_id != 1)
+        return fail(CLBLM_ERR_UNSUPPORTED, "only the rectangle (functID 1) spectral-domain scan is supported");
+    if (!(hwhm > 0.) || !(dv > 0.) || !(dvout != 0.) || n < 8 || nrows < 1 || nrows > 65535 || ld_in < n)
+        return fail(CLBLM_ERR_ARG, "hwhm, dv must be positive, dvout nonzero, n >= 8, 1 <= nrows <= 65535, ld_in >= n");
+    // clblm_INTERP onto HWHM/12 when the input is coarser (clblm_ScanFilter.f90:331-334)
+    const double DVINT = hwhm / 12.;
+    IPlan ip;
+    const bool pre = dv > DVINT;
+    int64_t ns = n;
+    double sv1 = v1, sdv = dv;
+    if (pre) {
+        int rc = plan_interp(n, v1, dv, v1, v1 + dv * (double)(n - 1), DVINT, ip);
+        if (rc) return rc;
+        ns = ip.nout;
+        sv1 = ip.v1;
+        sdv = DVINT;
+    }
+    RectPlan rp;
+    plan_rect(ns, sv1, sdv, v1out, v2out, dvout, hwhm, rp);
+    for (const RPart &q : rp.parts)   // every box reads inside the spectrum
+        if (q.il < 0 || q.ih < 0 || q.il >= ns || q.ih >= ns)
+            return fail(CLBLM_ERR_ARG, "rectangle scan: a box reaches outside the spectrum");
+    *nout = rp.nout;
+    *v1res = rp.v1;
+    *dvres = rp.dvo;
+    if (!d_out) return 0;
+    if (!d_in) return fail(CLBLM_ERR_ARG, "input spectrum is NULL");
+    if (ld_out < (rp.scan ? rp.nout : ns)) return fail(CLBLM_ERR_ARG, "ld_out is below the output length");
+    ScanState &S = *(ScanState *)state;
+    hipStream_t st = (hipStream_t)stream;
+    auto up = [](size_t o) { return (o + 255) & ~(size_t)255; };
+    const size_t o_cf = 0, o_seg = up(sizeof(ICoef));
+    const size_t o_box = up(o_seg + sizeof(ISeg) * ip.segs.size());
+    const size_t o_part = up(o_box + sizeof(RBox) * rp.boxes.size());
+    const size_t meta = o_part + sizeof(RPart) * rp.parts.size() + 256;
+    SCHK(hipStreamSynchronize(st));  // the metadata buffer may still be read by a previous scan
+    int rc = grow(&S.meta, &S.meta_cap, meta);
+    if (rc) return rc;
+    std::vector<char> host(meta, 0);
+    const ICoef cf = make_coef();
+    std::memcpy(host.data() + o_cf, &cf, sizeof cf);
+    if (!ip.segs.empty()) std::memcpy(host.data() + o_seg, ip.segs.data(), sizeof(ISeg) * ip.segs.size());
+    if (!rp.boxes.empty()) std::memcpy(host.data() + o_box, rp.boxes.data(), sizeof(RBox) * rp.boxes.size());
+    if (!rp.parts.empty()) std::memcpy(host.data() + o_part, rp.parts.data(), sizeof(RPart) * rp.parts.size());
+    SCHK(hipMemcpyAsync(S.meta, host.data(), meta, hipMemcpyHostToDevice, st));
+    SCHK(hipStreamSynchronize(st));  // the host copy goes out of scope
+    char *mb = (char *)S.meta;
+    const double *xs = d_in;
+    int64_t ldx = ld_in;
+    if (pre) {
+        rc = grow((void **)&S.xs, &S.xs_cap, sizeof(double) * (size_t)ns * (size_t)nrows);
+        if (rc) return rc;
+        for (int64_t r = 0; r < nrows; r++)
+            hipLaunchKernelGGL(k_scan_interp<double>, dim3(cdiv((int64_t)ip.segs.size(), 64)), dim3(64), 0, st,
+                               (const ISeg *)(mb + o_seg), (int64_t)ip.segs.size(), d_in + r * ld_in,
+                               (const ICoef *)(mb + o_cf), ip.vratio, S.xs + r * ns);
+        xs = S.xs;
+        ldx = ns;
+    }
+    const unsigned ny = (unsigned)nrows;
+    if (!rp.scan) {
+        hipLaunchKernelGGL(k_rect_copy, dim3(cdiv(ns, 256), ny), dim3(256), 0, st, xs, ldx, ns, d_out, ld_out);
+    } else {
+        hipLaunchKernelGGL(k_rect_fill, dim3(cdiv(std::max<int64_t>(1, rp.nout), 256), ny), dim3(256), 0, st, d_out,
+                           rp.nout, ld_out);
+        const int64_t nb = (int64_t)rp.boxes.size();
+        if (nb > 0)
+            hipLaunchKernelGGL(k_rect_boxes, dim3(cdiv(nb, 64), ny), dim3(64), 0, st, (const RBox *)(mb + o_box), nb,
+                               (const RPart *)(mb + o_part), xs, ldx, d_out, ld_out);
+    }
+    SCHK(hipGetLastError());
+    return 0;
+}

@@ -10,7 +10,9 @@ clblm_setDV.f90:203-208):
    molecule alone (drv_LevelConc, :1156-1166);
 2. clblm_rt_jac: the radiance and its per-point derivatives (Jacobians, :346-716);
 3. clblm_level_jac: the level Jacobians dR/dT_lev and dR/dln q_lev (:899-1203) with
-   level2layer's factors (:1208-1279, clblm_level2layer, computed once).
+   level2layer's factors (:1208-1279, clblm_level2layer, computed once);
+4. with a boxcar wider than DVout, output_JacArray's pre-average (:1341-1409,
+   clblm_scanfn) of the radiance and every Jacobian array of the block.
 
 The down path is the view path scaled by sec_dn (the air-mass-scaled form of
 pathOD_up_dn_sun, :826-838).  The OD variants are recomputed per block rather than
@@ -37,7 +39,7 @@ class UpwellingJacobian:
 
     def __init__(self, ctx: Context, handle: int, layers, t_lev, dvs, *, p_lev, q_lev=None, mols=None, emis=0.98,
                  tskin=295.0, lin_in_tau=1, sec_dn=1.0, precision=FP32C, dt=1.0, dw_frac=0.01, continuum=None,
-                 dvout=None):
+                 dvout=None, boxcar_hw=-1.0):
         import torch
         self.ctx, self.h, self.layers = ctx, handle, list(layers)
         L = len(self.layers)
@@ -50,6 +52,11 @@ class UpwellingJacobian:
         self.prec, self.dt, self.dw_frac = precision, float(dt), float(dw_frac)
         self.continuum = continuum   # callable(layer, v1, v2) -> (absrb, v1abs, npt) or None
         self.dvout = min(self.dvs) if dvout is None else dvout
+        # doPreBox (clblm_noScattJac.f90:187-189): a boxcar wider than DVout spanning
+        # more than one point pre-averages the outputs
+        self.boxcar_hw = float(boxcar_hw)
+        self.prebox = self.boxcar_hw > 0.0 and self.boxcar_hw > self.dvout and \
+            int(2.0 * self.boxcar_hw / self.dvout) > 1
         q_lev = q_lev or {}
         self.mols = sorted(q_lev) if mols is None else list(mols)
         missing = [m for m in self.mols if m not in q_lev]
@@ -98,7 +105,9 @@ class UpwellingJacobian:
         """Radiance and Jacobians on [v1, v2] (uniform grid DVout).  Returns torch tensors:
         rad [N]; jac_t [L+1, N] = dR/dT_lev; jac_lnq [M, L+1, N] = dR/dln q_lev;
         dr_dtskin, dr_demis [N]; and the per-point derivatives dr_dtau [L, N],
-        dr_dtlev [L+1, N] and the layer ODs od [L, N]."""
+        dr_dtlev [L+1, N] and the layer ODs od [L, N].  grid = (v1, dv) of rad, jac_t,
+        jac_lnq, dr_dtskin and dr_demis: the block's grid, or the boxcar's output grid
+        when the outputs are pre-averaged."""
         import torch
         jobs, pars = self._jobs(v1, v2)
         self.ctx.jrad = 1
@@ -122,8 +131,16 @@ class UpwellingJacobian:
         check(lib.clblm_level_jac(self.ctx.ptr, L, M, n, p(od), p(dr_dtau), p(dr_dtlev), p(D["wtot"]), p(D["tlay"]),
                                   ctypes.c_double(self.dt), ctypes.c_double(self.dw_frac), p(D["up"]), p(D["lo"]),
                                   p(D["wu"]), p(D["wl"]), p(jac_t), p(jac_q), st))
+        grid = (float(v1), float(self.dvout))
+        if self.prebox:   # output_JacArray (:1341-1409), per block as the reference does
+            box = lambda t: self.ctx.prebox(t, v1, self.dvout, self.boxcar_hw)
+            rad, g1, gd = box(rad)
+            jac_t = box(jac_t)[0]
+            jac_q = box(jac_q)[0] if M else None
+            dts, de = box(dts)[0], box(de)[0]
+            grid = (g1, gd)
         return dict(rad=rad, jac_t=jac_t, jac_lnq=jac_q, dr_dtskin=dts, dr_demis=de, dr_dtau=dr_dtau,
-                    dr_dtlev=dr_dtlev, od=od[0], od_var=od, n=n)
+                    dr_dtlev=dr_dtlev, od=od[0], od_var=od, n=n, grid=grid)
 
 
 def upwelling_radiance(ctx, od, t_lay, t_lev, v1, dv, emis=0.98, tskin=295.0, lin_in_tau=1, sec_dn=1.0):

@@ -141,6 +141,39 @@ class Context:
                                       ctypes.byref(n), ctypes.byref(a), ctypes.byref(b), ctypes.c_void_p(stream)))
         return out, a.value, b.value
 
+    def scanfn(self, x, v1, dv, v1out, v2out, dvout, funct_id=1, hwhm=0.1, stream=None):
+        """clblm_SCANFN_array (src/clblm_src/clblm_ScanFilter.f90:33-67), rectangle
+        (funct_id 1) only, on a float64 cuda tensor x of spectra on v1 + i*dv along its
+        last axis: returns (scanned tensor, its v1, its dv)."""
+        import torch
+        assert x.dtype == torch.float64 and x.is_cuda
+        shape = x.shape
+        x2 = x.contiguous().reshape(-1, shape[-1])
+        rows, n = x2.shape
+        nout, a, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        args = (int(n), int(rows), int(n), float(v1), float(dv), float(v1out), float(v2out), float(dvout),
+                int(funct_id), float(hwhm))
+        check(self.lib.clblm_scanfn(self.ptr, None, *args, None, 0, ctypes.byref(nout), ctypes.byref(a),
+                                    ctypes.byref(b), None))
+        out = torch.empty((rows, max(1, nout.value)), dtype=torch.float64, device=x.device)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        check(self.lib.clblm_scanfn(self.ptr, ctypes.c_void_p(x2.data_ptr()), *args, ctypes.c_void_p(out.data_ptr()),
+                                    int(out.shape[1]), ctypes.byref(nout), ctypes.byref(a), ctypes.byref(b),
+                                    ctypes.c_void_p(stream)))
+        return out[:, :nout.value].reshape(*shape[:-1], nout.value), a.value, b.value
+
+    def prebox(self, x, v1, dv, boxcar_hw):
+        """output_JacArray's boxcar pre-average (clblm_noScattJac.f90:1341-1409) of a
+        float64 cuda tensor of spectra on v1 + i*dv (last axis): the rectangle scan with M = int(2 boxcarHW/dv)
+        points per box onto v1 + (M-1)dv/2 .. v2 - (M-1)dv/2 at (M-1)dv; the input
+        itself when boxcarHW <= dv or M <= 1.  Returns (tensor, v1, dv)."""
+        m = int(2.0 * boxcar_hw / dv)
+        if not (boxcar_hw > 0.0 and boxcar_hw > dv) or m <= 1:
+            return x, v1, dv
+        v2 = v1 + dv * (x.shape[-1] - 1)
+        return self.scanfn(x, v1, dv, v1 + (m - 1) * dv / 2.0, v2 - (m - 1) * dv / 2.0, (m - 1) * dv, 1, boxcar_hw)
+
     def tuning(self, key, value=0):
         """Sets launch-shape parameter ``key`` (``_lib.TUNE_*``) to ``value`` (0: query
         only); returns the previous setting."""

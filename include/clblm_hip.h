@@ -196,6 +196,20 @@ int clblm_level_jac(clblm_ctx *ctx, int nlay, int nmol, int64_t npts, const doub
                     const double *d_dw_dqup, const double *d_dw_dqlo, double *d_jac_t, double *d_jac_q,
                     void *stream);
 
+/* Spectral-domain scan: clblm_SCANFN_array (src/clblm_src/clblm_ScanFilter.f90:33-67)
+ * for functID 1, the rectangle of adjacent boxes of width 2 hwhm every dvout
+ * (CNVRCT, :1478-1698), which output_JacArray applies as the boxcar pre-average of
+ * Jacobian arrays (clblm_noScattJac.f90:1341-1409).  d_in: nrows spectra of n
+ * device doubles on v1 + i dv, row r at d_in + r ld_in; output row r at
+ * d_out + r ld_out (ld_out >= *nout); output limits v1out..v2out.  The input is first interpolated onto
+ * hwhm/12 when dv is coarser; the output step is dvout rounded to a whole number of
+ * input steps; with fewer than two input steps per box the (interpolated) input is
+ * returned unchanged, as in the reference.  Two-phase: d_out == NULL returns *nout,
+ * *v1res, *dvres only.  Other scanning functions return CLBLM_ERR_UNSUPPORTED. */
+int clblm_scanfn(clblm_ctx *ctx, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1,
+                 double dv, double v1out, double v2out, double dvout, int funct_id, double hwhm, double *d_out,
+                 int64_t ld_out, int64_t *nout, double *v1res, double *dvres, void *stream);
+
 /* ---- instrument scan ------------------------------------------------------------
  * Drop-in for clblm_FFTSCN_array (clblm_FFT.f90:308-349): the spectrum d_in[0..n)
  * (device) on v1 + i*dv is scanned with function funct_id of half width hwhm between

@@ -183,12 +183,12 @@ def _scene(nlay=12, n_lines=3000):
     return lines, layers, t_lev, p_lev, q_lev, dvs
 
 
-def _jac_setup(ctx, prec, nlay=12, mols=None):
+def _jac_setup(ctx, prec, nlay=12, mols=None, boxcar_hw=-1.0):
     from clblm_amd.jacobian import UpwellingJacobian
     lines, layers, t_lev, p_lev, q_lev, dvs = _scene(nlay)
     h = ctx.upload_lines(lines)
     return UpwellingJacobian(ctx, h, layers, t_lev, dvs, p_lev=p_lev, q_lev=q_lev, mols=mols, precision=prec,
-                             continuum=lambda lay, a, b: W.continuum_absrb(lay, a, b))
+                             continuum=lambda lay, a, b: W.continuum_absrb(lay, a, b), boxcar_hw=boxcar_hw)
 
 
 def test_jacobian_fp32c_vs_fp64(ctx):
@@ -239,3 +239,32 @@ def test_level_jacobians_match_the_oracle_pipeline(ctx):
         err = np.abs(a - b).max() / np.abs(b).max()
         print("%s max err / max = %.2e" % (name, err))
         assert err < tol, (name, err)
+
+
+@pytest.mark.parametrize("hw_in_dv", [6.3, 40.0])
+def test_jacobian_boxcar_pre_average(ctx, hw_in_dv):
+    """doPreBox (clblm_noScattJac.f90:187-189, output_JacArray :1341-1409): with a boxcar
+    wider than DVout every output array of a block is pre-averaged by the rectangle
+    scan.  The pre-averaged block equals the oracle's rectangle scan (itself
+    bit-identical to the reference's clblm_SCANFN) applied to the same block's plain
+    outputs, bit for bit (the plain block is computed again, FP64)."""
+    from clblm_amd._lib import FP64
+    from oracle.py import prebox
+    plain = _jac_setup(ctx, FP64, nlay=6, mols=[1, 2])
+    hw = hw_in_dv * plain.dvout
+    boxed = _jac_setup(ctx, FP64, nlay=6, mols=[1, 2], boxcar_hw=hw)
+    assert boxed.prebox
+    v1, v2 = 640.0, 652.0
+    a = plain.block(v1, v2)
+    b = boxed.block(v1, v2)
+    for name in ("rad", "jac_t", "jac_lnq", "dr_dtskin", "dr_demis"):
+        x = a[name].cpu().numpy()
+        rows = x.reshape(-1, x.shape[-1])
+        want = [prebox(r, v1, plain.dvout, hw) for r in rows]
+        g1, gd = b["grid"]
+        assert (g1, gd) == (want[0][1], want[0][2]), name
+        got = b[name].cpu().numpy().reshape(rows.shape[0], -1)
+        # the plain outputs of the two runs agree to the pass-1 atomic reordering
+        np.testing.assert_allclose(got, np.array([w[0] for w in want]), rtol=1e-12,
+                                   atol=1e-12 * np.abs(rows).max(), err_msg=name)
+    print("boxcar %.4g cm-1: %d -> %d points per block" % (hw, a["rad"].numel(), b["rad"].numel()))
