@@ -103,7 +103,29 @@ def run_od(ctx, h, layers, pars, steps, warmup):
     ctx.sync()   # raises any error found on the device during the timed batches
     w = ctx.work(reset=False)
     tm = ctx.timing(reset=True)
-    return dt, w, tm, off
+    return dt, w, tm, off, out.cpu().numpy()
+
+
+def run_graph(ctx, h, layers, pars, steps, warmup, ref):
+    """Time `steps` replays of the same batch captured once in a hipGraph (the plan is
+    built once at capture; each replay recomputes every kernel from the layer inputs).
+    Returns (seconds per replay, max relative difference of the last replay from the
+    eager result `ref`)."""
+    import torch
+    ctx.jrad = 1
+    g = ctx.od_graph(h, layers, pars)
+    for _ in range(max(1, warmup)):
+        g.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    g.check()
+    od = g.out.cpu().numpy()
+    g.close()
+    return dt, float((np.abs(od - ref) / np.maximum(np.abs(ref), 1e-300)).max())
 
 
 def run_jacobian(ctx, h, layers, dvs, precision, block=100.0):
@@ -152,6 +174,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison run")
     ap.add_argument("--no-jac", action="store_true", help="skip the Jacobian pass")
+    ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay timing")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shapes")
     ap.add_argument("--serial", action="store_true",
                     help="run every kernel of a batch on one stream, one after another (for a kernel "
@@ -194,7 +217,7 @@ def main():
         tuned = {"f4_points_per_lane": chosen[_lib.TUNE_F4_POINTS_FP64 if prec == FP64 else _lib.TUNE_F4_POINTS_FP32C],
                  "conv_unroll": chosen[_lib.TUNE_CONV_UNROLL], "f4_xcd_chunk": chosen[_lib.TUNE_F4_XCD_CHUNK],
                  "f4_stream": "own" if chosen[_lib.TUNE_F4_STREAM] == 1 else "batch"}
-    dt, w, tm, off = run_od(ctx, h, layers, mk(prec), args.steps, args.warmup)
+    dt, w, tm, off, od_eager = run_od(ctx, h, layers, mk(prec), args.steps, args.warmup)
     value = E / dt
     conv_tf = w["conv_evals"] * FLOP_CONV / (w["ms_conv"] * 1e-3) / 1e12 if w["ms_conv"] > 0 else 0.0
     f4_tf = w["f4_evals"] * FLOP_F4 / (w["ms_f4"] * 1e-3) / 1e12 if w["ms_f4"] > 0 else 0.0
@@ -221,10 +244,15 @@ def main():
                      "ms": {"conv_kernel": w["ms_conv"], "f4_kernel": w["ms_f4"], "prep": tm["prep"],
                             "conv_stage": tm["conv"], "f4_stage": tm["f4"], "panel_stage": tm["panel"]}},
     }
+    if not args.no_graph:
+        dtg, dg = run_graph(ctx, h, layers, mk(prec), args.steps, args.warmup, od_eager)
+        res["graph"] = {"value": E / dtg, "unit": "line-evals/s", "ms_per_step": dtg * 1e3,
+                        "max_rel_diff_vs_eager": dg,
+                        "note": "the same batch captured once in a hipGraph and replayed; planned once"}
     if prec != FP64 and not args.no_fp64:
         if not args.no_autotune:
             ctx.autotune(h, layers, mk(FP64), keys=[_lib.TUNE_F4_POINTS_FP64], reps=2)
-        dt64, _, _, _ = run_od(ctx, h, layers, mk(FP64), args.steps, args.warmup)
+        dt64, _, _, _, _ = run_od(ctx, h, layers, mk(FP64), args.steps, args.warmup)
         res["fp64_gpu"] = {"value": E / dt64, "unit": "line-evals/s", "ms_per_step": dt64 * 1e3,
                            "fp32c_over_fp64": dt64 / dt}
     if not args.no_jac:

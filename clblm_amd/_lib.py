@@ -14,7 +14,8 @@ FP64, FP32C = 0, 1
 EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
            "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
-           "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune", "clblm_scanfn"]
+           "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune", "clblm_scanfn",
+           "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy"]
 TUNE_F4_XCD_CHUNK = 1
 TUNE_F4_FAR_FIELD = 2
 TUNE_F4_STREAM = 3
@@ -93,6 +94,11 @@ def load():
          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), P]
     lib.clblm_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_double)]
+    lib.clblm_od_graph_create.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int32),
+                                          P, P, ctypes.POINTER(ctypes.c_void_p)]
+    lib.clblm_od_graph_launch.argtypes = [P, P, P]
+    lib.clblm_od_graph_check.argtypes = [P, P]
+    lib.clblm_od_graph_destroy.argtypes = [P, P]
     _lib = lib
     return lib
 

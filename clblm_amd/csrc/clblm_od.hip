@@ -2222,6 +2222,20 @@ struct clblm_ctx {
     std::mutex mu;
 };
 
+// The status code and message for a nonzero device error word.
+static int device_error(int herr, std::string &msg) {
+    if (herr & 1) {
+        msg = "line coupling records (IFLG 1..3) are not supported";
+        return CLBLM_ERR_UNSUPPORTED;
+    }
+    if (herr & 16) {
+        msg = "internal error: a SHRINQ chunk exceeds the LINF4Q buffer";
+        return CLBLM_ERR_HIP;
+    }
+    msg = "F4 record capacity exceeded";
+    return CLBLM_ERR_CAPACITY;
+}
+
 // Reads back a finished slot: stage times and the device error word.
 static void finish_slot(clblm_ctx *C, int k, bool wait) {
     clblm_ctx::Slot &S = C->slot[k];
@@ -2240,18 +2254,7 @@ static void finish_slot(clblm_ctx *C, int k, bool wait) {
     C->ms_f4_k += t56;
     C->launches++;
     int herr = C->h_err[k];
-    if (herr && !C->deferred_rc) {
-        if (herr & 1) {
-            C->deferred_rc = CLBLM_ERR_UNSUPPORTED;
-            C->deferred_msg = "line coupling records (IFLG 1..3) are not supported";
-        } else if (herr & 16) {
-            C->deferred_rc = CLBLM_ERR_HIP;
-            C->deferred_msg = "internal error: a SHRINQ chunk exceeds the LINF4Q buffer";
-        } else {
-            C->deferred_rc = CLBLM_ERR_CAPACITY;
-            C->deferred_msg = "F4 record capacity exceeded";
-        }
-    }
+    if (herr && !C->deferred_rc) C->deferred_rc = device_error(herr, C->deferred_msg);
 }
 
 // Reads back every finished slot in call order (all of them when wait is set).
@@ -2564,11 +2567,33 @@ static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, 
 template <typename T>
 static T *at(char *base, size_t off) { return reinterpret_cast<T *>(base + off); }
 
-static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, hipStream_t st, int precision);
+// A batch captured in a hipGraph (clblm_od_graph_create): it owns its arena, its
+// pinned plan staging, its error word and its fork/join events and F4 stream, so
+// replays do not touch the context's ring of batches.
+struct clblm_od_graph {
+    DevArena arena;
+    char *pin = nullptr;
+    int *d_err = nullptr, *h_err = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t s4 = nullptr, cap = nullptr;
+    hipEvent_t done = nullptr;  // recorded after each launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool launched = false;
+};
+static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, hipStream_t st, int precision,
+                     clblm_od_graph *G = nullptr);
 
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
-static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, hipStream_t st, int precision) {
+// Fills n bytes with the byte v (a kernel, so that it can be captured in a graph).
+__global__ void k_fill_bytes(uint8_t *p, size_t n, uint8_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, hipStream_t st, int precision,
+                     clblm_od_graph *G) {
     const int nlay = (int)B.jobs.size();
     if (nlay == 0) return 0;
     // scratch layout
@@ -2623,37 +2648,65 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         o_rg = cv.take<int>(std::max<size_t>(1, rg_lay.size()));
         o_rgf = cv.take<int64_t>(nlay);
     }
-    // stream order with the previous batch (the arena is shared), then the slot
-    const int k = (int)(C->ncalls % clblm_ctx::NSLOT);
-    finish_slot(C, k, true);
-    if (C->last_stream && C->last_stream != st) {
-        const int kp = (int)((C->ncalls + clblm_ctx::NSLOT - 1) % clblm_ctx::NSLOT);
-        HIPCHK(hipStreamWaitEvent(st, C->slot[kp].done, 0));
-    }
-    clblm_ctx::Slot &S = C->slot[k];
-    if (cv.off > C->arena.cap) {  // growing frees the old arena: let every batch finish first
-        drain(C, true);
-        HIPCHK(hipDeviceSynchronize());
-    }
-    if (C->arena.ensure(cv.off)) return set_err(CLBLM_ERR_HIP, "device scratch allocation failed");
-    char *base = C->arena.p;
     // host-built tables, staged in pinned memory: the header region [0, o_lp_d) as it
     // lies in the arena, then the per-layer stop indices and the re-grid units
     const size_t h_stop = (o_lp_d + 255) & ~(size_t)255;
     const size_t h_rg = (h_stop + sizeof(int) * nlay + 255) & ~(size_t)255;
     const size_t h_rgf = (h_rg + sizeof(int) * rg_lay.size() + 255) & ~(size_t)255;
     const size_t h_size = h_rgf + sizeof(int64_t) * nlay;
-    const int ps = C->pin_next;
-    C->pin_next ^= 1;
-    HIPCHK(hipEventSynchronize(C->pin_ev[ps]));
-    if (C->pin_cap[ps] < h_size) {
-        if (C->pin[ps]) HIPCHK(hipHostFree(C->pin[ps]));
-        C->pin[ps] = nullptr;
-        C->pin_cap[ps] = 0;
-        HIPCHK(hipHostMalloc((void **)&C->pin[ps], h_size + h_size / 4, hipHostMallocDefault));
-        C->pin_cap[ps] = h_size + h_size / 4;
+    // where this batch lives: the context's arena, staging and slot ring, or (under
+    // graph capture) the graph's own, with no stage events
+    const int k = (int)(C->ncalls % clblm_ctx::NSLOT);
+    clblm_ctx::Slot &S = C->slot[k];
+    char *base = nullptr, *hp = nullptr;
+    int ps = 0;
+    int *d_err = nullptr;
+    hipStream_t s4 = st;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    if (!G) {
+        // stream order with the previous batch (the arena is shared), then the slot
+        finish_slot(C, k, true);
+        if (C->last_stream && C->last_stream != st) {
+            const int kp = (int)((C->ncalls + clblm_ctx::NSLOT - 1) % clblm_ctx::NSLOT);
+            HIPCHK(hipStreamWaitEvent(st, C->slot[kp].done, 0));
+        }
+        if (cv.off > C->arena.cap) {  // growing frees the old arena: let every batch finish first
+            drain(C, true);
+            HIPCHK(hipDeviceSynchronize());
+        }
+        if (C->arena.ensure(cv.off)) return set_err(CLBLM_ERR_HIP, "device scratch allocation failed");
+        base = C->arena.p;
+        ps = C->pin_next;
+        C->pin_next ^= 1;
+        HIPCHK(hipEventSynchronize(C->pin_ev[ps]));
+        if (C->pin_cap[ps] < h_size) {
+            if (C->pin[ps]) HIPCHK(hipHostFree(C->pin[ps]));
+            C->pin[ps] = nullptr;
+            C->pin_cap[ps] = 0;
+            HIPCHK(hipHostMalloc((void **)&C->pin[ps], h_size + h_size / 4, hipHostMallocDefault));
+            C->pin_cap[ps] = h_size + h_size / 4;
+        }
+        hp = C->pin[ps];
+        d_err = C->d_err + k;
+        s4 = C->f4_stream_mode == 1 ? C->f4stream : st;
+        ev_fork = S.fork;
+        ev_join = S.join;
+    } else {
+        if (G->arena.ensure(cv.off)) return set_err(CLBLM_ERR_HIP, "graph scratch allocation failed");
+        base = G->arena.p;
+        HIPCHK(hipHostMalloc((void **)&G->pin, h_size, hipHostMallocDefault));
+        hp = G->pin;
+        d_err = G->d_err;
+        s4 = C->f4_stream_mode == 1 ? G->s4 : st;
+        ev_fork = G->fork;
+        ev_join = G->join;
+        // everything from here to the end of the batch is recorded into the graph;
+        // the F4 stream joins the capture through the fork event
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     }
-    char *hp = C->pin[ps];
+    auto stage_event = [&](int i, hipStream_t s) -> hipError_t {
+        return G ? hipSuccess : hipEventRecord(S.ev[i], s);
+    };
     std::memcpy(hp + o_jobs, B.jobs.data(), sizeof(Job) * nlay);
     std::memcpy(hp + o_pan, B.panels.data(), sizeof(Panel) * B.panels.size());
     std::memcpy(hp + o_blk, B.blks.data(), sizeof(R4Blk) * B.blks.size());
@@ -2700,18 +2753,20 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     double *d_R4 = at<double>(base, o_R4);
     double *d_nat = at<double>(base, o_nat);
     HIPCHK(hipMemcpyAsync(f.stop, hp + h_stop, sizeof(int) * nlay, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(f.chunks, 0xff, sizeof(int4) * std::max<int64_t>(1, B.tot_rec), st));
-    int *d_err = C->d_err + k;
-    HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int), st));
+    {
+        const size_t nb = sizeof(int4) * std::max<int64_t>(1, B.tot_rec);
+        hipLaunchKernelGGL(k_fill_bytes, dim3(cdiv((int64_t)nb, 256)), dim3(256), 0, st, (uint8_t *)f.chunks, nb,
+                           (uint8_t)0xff);
+        hipLaunchKernelGGL(k_fill_bytes, dim3(1), dim3(64), 0, st, (uint8_t *)d_err, sizeof(int), (uint8_t)0);
+    }
     Tables T = ctx_tabview(C);
     const int64_t npan = (int64_t)B.panels.size();
     const int64_t nblk = (int64_t)B.blks.size();
 
     // fork: the F4 chain runs on the F4 stream from here to the join before pass 2
-    const hipStream_t s4 = C->f4_stream_mode == 1 ? C->f4stream : st;
-    HIPCHK(hipEventRecord(S.fork, st));
-    HIPCHK(hipStreamWaitEvent(s4, S.fork, 0));
-    HIPCHK(hipEventRecord(S.ev[7], s4));
+    HIPCHK(hipEventRecord(ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(s4, ev_fork, 0));
+    HIPCHK(stage_event(7, s4));
     if (B.max_f4 > 0) {
         hipLaunchKernelGGL(k_f4acc, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, d_err);
         hipLaunchKernelGGL(k_recpos, dim3(nlay), dim3(256), 0, s4, d_jobs, f, cmp);
@@ -2729,7 +2784,8 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         }
         hipLaunchKernelGGL(k_shr_blk9, dim3(nlay), dim3(256), 0, s4, d_jobs, f, sh);
     } else {
-        HIPCHK(hipMemsetAsync(f.nshr, 0, sizeof(int) * 3 * nlay, s4));
+        hipLaunchKernelGGL(k_fill_bytes, dim3(cdiv((int64_t)(sizeof(int) * 3 * nlay), 256)), dim3(256), 0, s4,
+                           (uint8_t *)f.nshr, sizeof(int) * 3 * (size_t)nlay, (uint8_t)0);
     }
     double *d_xjj = at<double>(base, o_xjj);
     hipLaunchKernelGGL(k_xjj, dim3(cdiv(nlay, 64)), dim3(64), 0, s4, d_jobs, nlay, d_xjj);
@@ -2744,7 +2800,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
             hipLaunchKernelGGL(k_f4setup<F4Rec32>, dim3(gx, (unsigned)nblk), dim3(256), 0, s4, d_jobs, d_blk, nblk,
                                sh, at<F4Rec32>(base, o_rec), d_xjj);
     }
-    HIPCHK(hipEventRecord(S.ev[5], s4));
+    HIPCHK(stage_event(5, s4));
     const int64_t ntile = (int64_t)B.tiles.size();
     const int xchunk = C->f4_chunk > 0 ? C->f4_chunk : 1;
     const unsigned f4grid = (unsigned)(8 * (int64_t)xchunk * ((ntile + 8 * xchunk - 1) / (8 * xchunk)));
@@ -2765,11 +2821,11 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         }
 #undef F4_LAUNCH
     }
-    HIPCHK(hipEventRecord(S.ev[6], s4));
+    HIPCHK(stage_event(6, s4));
     if (B.any_jrad1 && nblk > 0) hipLaunchKernelGGL(k_r4rad, dim3((unsigned)nblk), dim3(256), 0, s4, d_jobs, d_blk, d_R4);
-    HIPCHK(hipEventRecord(S.ev[3], s4));
-    HIPCHK(hipEventRecord(S.join, s4));
-    HIPCHK(hipEventRecord(S.ev[0], st));
+    HIPCHK(stage_event(3, s4));
+    HIPCHK(hipEventRecord(ev_join, s4));
+    HIPCHK(stage_event(0, st));
     if (B.max_line > 0) {
         hipLaunchKernelGGL(k_prep, dim3(cdiv(B.max_line, 256), nlay), dim3(256), 0, st, d_jobs, d_pan, L.dev, lp, d_err);
         int *htot = at<int>(base, o_htot);
@@ -2777,7 +2833,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         hipLaunchKernelGGL(k_home_fix, dim3((unsigned)hcap, nlay), dim3(SCAN_T), 0, st, d_jobs, lp, htot, hcap);
     }
     hipLaunchKernelGGL(k_ranges, dim3(cdiv(npan, 256)), dim3(256), 0, st, d_jobs, d_pan, npan, lp);
-    HIPCHK(hipEventRecord(S.ev[1], st));
+    HIPCHK(stage_event(1, st));
     {
         const dim3 g((unsigned)npan), b(256);
         const size_t lds = (size_t)B.max_stride * sizeof(double);
@@ -2793,9 +2849,9 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         }
 #undef CONV_LAUNCH
     }
-    HIPCHK(hipEventRecord(S.ev[2], st));
-    HIPCHK(hipStreamWaitEvent(st, S.join, 0));
-    HIPCHK(hipEventRecord(S.ev[8], st));
+    HIPCHK(stage_event(2, st));
+    HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
+    HIPCHK(stage_event(8, st));
     hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
                        d_abs, d_out, d_nat);
     if (B.any_regrid) {
@@ -2806,7 +2862,13 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         hipLaunchKernelGGL(k_regrid, dim3(cdiv((int64_t)rg_lay.size(), 64)), dim3(64), 0, st, d_jobs, d_rg, d_rgf,
                            d_nat, d_out, (int64_t)rg_lay.size());
     }
-    HIPCHK(hipEventRecord(S.ev[4], st));
+    HIPCHK(stage_event(4, st));
+    if (G) {
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(G->h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamEndCapture(st, &G->graph));
+        return 0;
+    }
     HIPCHK(hipEventRecord(C->pin_ev[ps], st));
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(C->h_err + k, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -3026,6 +3088,103 @@ int clblm_od_batch(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
     rc = run_batch(c, *c->lists[handle], B, d_od, st, prec);
     if (rc) return rc;
     *jrad_state = jr;
+    return 0;
+}
+
+static void graph_free(clblm_od_graph *g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->arena.p) (void)hipFree(g->arena.p);
+    if (g->d_err) (void)hipFree(g->d_err);
+    if (g->h_err) (void)hipHostFree(g->h_err);
+    if (g->pin) (void)hipHostFree(g->pin);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    if (g->join) (void)hipEventDestroy(g->join);
+    if (g->done) (void)hipEventDestroy(g->done);
+    if (g->s4) (void)hipStreamDestroy(g->s4);
+    if (g->cap) (void)hipStreamDestroy(g->cap);
+    delete g;
+}
+
+int clblm_od_graph_create(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers,
+                          const clblm_od_params *pars, int32_t *jrad_state, int64_t *offsets, double *d_od,
+                          clblm_od_graph **graph) {
+    if (!c || !layers || !pars || !jrad_state || !offsets || !d_od || !graph || nlay <= 0)
+        return set_err(CLBLM_ERR_ARG, "null argument or nlay <= 0");
+    *graph = nullptr;
+    if (handle < 0 || handle >= (int)c->lists.size() || !c->lists[handle])
+        return set_err(CLBLM_ERR_ARG, "bad line list handle");
+    const int prec = pars[0].precision;
+    if (prec != CLBLM_FP64 && prec != CLBLM_FP32C) return set_err(CLBLM_ERR_ARG, "unknown precision");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> lk(c->mu);
+    int32_t jr = *jrad_state;
+    Batch B;
+    int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec, prec == CLBLM_FP64 ? c->f4k64 : c->f4k32);
+    if (rc) return rc;
+    clblm_od_graph *g = new clblm_od_graph;
+    auto fail = [&](int r) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (g->cap && hipStreamIsCapturing(g->cap, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            hipGraph_t part = nullptr;
+            (void)hipStreamEndCapture(g->cap, &part);
+            if (part) (void)hipGraphDestroy(part);
+        }
+        graph_free(g);
+        return r;
+    };
+    if (hipMalloc(&g->d_err, sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&g->h_err, sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->done, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&g->s4, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(CLBLM_ERR_HIP, "graph resources could not be allocated"));
+    *g->h_err = 0;
+    rc = run_batch(c, *c->lists[handle], B, d_od, g->cap, prec, g);
+    if (rc) return fail(rc);
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        return fail(set_err(CLBLM_ERR_HIP, "hipGraphInstantiate failed"));
+    for (int l = 0; l < nlay; l++) offsets[l] = B.jobs[l].out_off;
+    offsets[nlay] = B.tot_out;
+    *jrad_state = jr;
+    *graph = g;
+    return 0;
+}
+
+int clblm_od_graph_launch(clblm_ctx *c, clblm_od_graph *g, void *stream) {
+    if (!c || !g) return set_err(CLBLM_ERR_ARG, "ctx or graph is NULL");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    // a launch after an earlier one on another stream runs after it (one arena)
+    if (g->launched) HIPCHK(hipStreamWaitEvent(st, g->done, 0));
+    HIPCHK(hipGraphLaunch(g->exec, st));
+    HIPCHK(hipEventRecord(g->done, st));
+    g->launched = true;
+    return 0;
+}
+
+int clblm_od_graph_check(clblm_ctx *c, clblm_od_graph *g) {
+    if (!c || !g) return set_err(CLBLM_ERR_ARG, "ctx or graph is NULL");
+    HIPCHK(hipSetDevice(c->device));
+    if (!g->launched) return 0;
+    HIPCHK(hipEventSynchronize(g->done));
+    if (*g->h_err) {
+        std::string msg;
+        const int rc = device_error(*g->h_err, msg);
+        return set_err(rc, msg);
+    }
+    return 0;
+}
+
+int clblm_od_graph_destroy(clblm_ctx *c, clblm_od_graph *g) {
+    if (!c || !g) return 0;
+    (void)hipSetDevice(c->device);
+    if (g->launched) (void)hipEventSynchronize(g->done);
+    graph_free(g);
     return 0;
 }
 

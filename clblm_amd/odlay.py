@@ -29,6 +29,31 @@ def _layer_struct(layer) -> _lib.Layer:
     return s
 
 
+class ODGraph:
+    """A batch captured by ``Context.od_graph``: ``launch()`` replays it on ``stream``
+    (default: torch's current stream), ``check()`` waits for the last replay and
+    raises a device error, ``close()`` frees it.  ``out`` and ``offsets`` are those
+    of ``Context.od_batch``."""
+
+    def __init__(self, ctx, g, out, offsets):
+        self.ctx, self.g, self.out, self.offsets = ctx, g, out, offsets
+
+    def launch(self, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        check(self.ctx.lib.clblm_od_graph_launch(self.ctx.ptr, self.g, ctypes.c_void_p(stream)))
+        return self.out
+
+    def check(self):
+        check(self.ctx.lib.clblm_od_graph_check(self.ctx.ptr, self.g))
+
+    def close(self):
+        if self.g:
+            check(self.ctx.lib.clblm_od_graph_destroy(self.ctx.ptr, self.g))
+            self.g = None
+
+
 class Context:
     def __init__(self, device: int = 0):
         self.lib = _lib.load()
@@ -57,6 +82,9 @@ class Context:
         check(self.lib.clblm_linelist_upload(self.ptr, int(lines.n), *[a.ctypes.data for a in arrs],
                                              ctypes.byref(h)))
         return h.value
+
+    def free_lines(self, handle):
+        check(self.lib.clblm_linelist_free(self.ptr, int(handle)))
 
     # -- one layer (host buffers) ----------------------------------------------
     @staticmethod
@@ -119,6 +147,24 @@ class Context:
                                       off.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(out.data_ptr()), st))
         self.jrad = jr.value
         return out, off
+
+    def od_graph(self, handle, layers, params, out=None):
+        """Captures the batch of ``od_batch`` in one hipGraph (nothing runs yet):
+        returns an ODGraph whose ``launch()`` fills ``out`` (the graph's float64 cuda
+        tensor) anew on each replay."""
+        import torch
+        lay, par, off = self.od_batch_plan(handle, layers, params)
+        if out is None:
+            out = torch.empty(int(off[-1]), dtype=torch.float64, device="cuda")
+        assert out.dtype == torch.float64 and out.is_cuda and out.numel() >= off[-1]
+        jr = ctypes.c_int32(self.jrad)
+        g = ctypes.c_void_p()
+        check(self.lib.clblm_od_graph_create(self.ptr, handle, len(layers), ctypes.cast(lay, ctypes.c_void_p),
+                                             ctypes.cast(par, ctypes.c_void_p), ctypes.byref(jr),
+                                             off.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(out.data_ptr()),
+                                             ctypes.byref(g)))
+        self.jrad = jr.value
+        return ODGraph(self, g, out, off)
 
     # -- instrument scan ---------------------------------------------------------------
     def fft_scan(self, x, v1, dv, v1out, v2out, dvout=0.0, funct_id=5, hwhm=0.6034, fill=0.0, precision=FP64,

@@ -101,6 +101,28 @@ int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
                    const clblm_od_params *pars, int32_t *jrad_state, int64_t *offsets,
                    double *d_od, void *stream);
 
+/* ---- a batch captured in a hipGraph ----------------------------------------------
+ * (no counterpart in the reference, whose layer loop calls ODLAY one layer at a
+ * time; for callers that compute the same layer set again and again, e.g. a
+ * retrieval iterating on one scene geometry).  clblm_od_graph_create plans the
+ * batch exactly as clblm_od_batch would (same arguments; d_od must be given and
+ * offsets/jrad_state are filled as there) and records all of its launches, on the
+ * F4 wing stream too, into one hipGraph with its own device scratch.  Nothing runs
+ * yet.  clblm_od_graph_launch replays it on `stream` (NULL = the context's stream)
+ * and returns once it is enqueued; each replay rewrites d_od from the layer inputs
+ * and the line list as they were at creation (the line list must stay uploaded;
+ * tuning changes made later do not apply).  Replays of one graph run in launch
+ * order, also across streams; replays and other batches writing d_od must be
+ * ordered by the caller's streams.  clblm_od_graph_check waits for the last replay
+ * and returns the device error of clblm_od_batch (line coupling, capacity), if any. */
+typedef struct clblm_od_graph clblm_od_graph;
+int clblm_od_graph_create(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *layers,
+                          const clblm_od_params *pars, int32_t *jrad_state, int64_t *offsets,
+                          double *d_od, clblm_od_graph **graph);
+int clblm_od_graph_launch(clblm_ctx *ctx, clblm_od_graph *graph, void *stream);
+int clblm_od_graph_check(clblm_ctx *ctx, clblm_od_graph *graph);
+int clblm_od_graph_destroy(clblm_ctx *ctx, clblm_od_graph *graph);
+
 /* Launch-shape tuning of a context (no counterpart in the reference, whose
  * loops are sequential).  key selects the parameter; value 0 only queries it;
  * *old_value (if non-NULL) receives the setting before the call.
