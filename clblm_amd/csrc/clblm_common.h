@@ -30,7 +30,11 @@ constexpr int MAXCHUNK_REC = 6;       // records a shrink chunk can span (1000/2
 // Per-layer job descriptor (device resident).
 struct Job {
     double v1, v2, dv, dvr2, dvr3, dvr4, alfmax, vft0, bound4;
-    double xkt, betacr, tratio, rhorat, pavp0;   // LNCORQ constants
+    double xkt, betacr, tratio, rhorat, pavp0, pavp2;   // LNCORQ constants
+    // line coupling at TAVE: coefficient = X[anchor] + ((X[lo + 1] - X[lo]) rect) dif over the
+    // nodes X[0..3] at 200/250/296/340 K, with LNCORQ's bracket (lcq) and LINF4Q's (lc4)
+    int lcq_lo, lcq_anchor, lc4_lo, lc4_anchor;
+    double lcq_rect, lcq_dif, lc4_rect, lc4_dif;
     double beta4, betacr4;                       // LINF4Q constants
     double v1l4, v2l4, tave;
     double w[NMOL + 1], rhoslf[NMOL + 1];

@@ -155,6 +155,9 @@ __host__ __device__ inline double radfni(double vi, double dvi, double xkt, doub
     return r;
 }
 
+// CNVFNQ's coupling factors CLC = HWF / (NX - 1), NX = 2001 (clblm_ODLAY.f90:2117-2119)
+constexpr double CLC1 = 4. / 2000., CLC2 = 16. / 2000., CLC3 = 64. / 2000.;
+
 __device__ __forceinline__ double avrat_lerp(const double *tab, int iz, double zetdif) {
     return tab[iz - 1] + zetdif * (tab[iz] - tab[iz - 1]);
 }
@@ -181,13 +184,34 @@ __device__ inline int first_fit_panel(const Panel *P, int np, double vnu, double
 // ------------------------------------------------------------------------------------
 struct LinesDev {
     int64_t n;
-    const double *nu;
+    const double *nu;   // centres; a record with IFLG < 0 carries its predecessor's (sorted)
     const float *s, *a_air, *a_self, *e_lo, *tmpalf, *shift;
     const int32_t *mol, *iflg;
+    const double *nu_rec;   // the VNU field as given (for IFLG < 0 records: coupling Y(200 K))
 };
 
+// Line-coupling coefficients of the IFLG 1 / 3 line at i at the layer temperature.
+// The record after the line (IFLG = -1) holds them at 200, 250, 296 and 340 K
+// (clblm_ODLAY.f90:1821-1829, clblm_ODLAY_LineF4.f90:971-978):
+//   A(1..4) = its VNU, ALFA, MOL (the bits of a REAL*4), TMPALF: Y (IFLAG 1) or GAMMA1 (3);
+//   B(1..4) = its SP, EPP, HWHM, PSHIFT: G (IFLAG 1) or GAMMA2 (3).
+// Linear in temperature on one bracket: value = X(anchor) + ((X(lo+1) - X(lo)) rect) dif,
+// where LNCORQ anchors at the lower node and LINF4Q at the upper one.
+struct LcPair {
+    double a, b;
+};
+__device__ inline LcPair lc_coeffs(const LinesDev &L, int64_t i, int lo, int anchor, double rect, double dif) {
+    const int64_t k = i + 1;
+    const double A[4] = {L.nu_rec[k], (double)L.a_air[k], (double)__int_as_float(L.mol[k]), (double)L.tmpalf[k]};
+    const double B[4] = {(double)L.s[k], (double)L.e_lo[k], (double)L.a_self[k], (double)L.shift[k]};
+    LcPair r;
+    r.a = A[anchor] + ((A[lo + 1] - A[lo]) * rect) * dif;
+    r.b = B[anchor] + ((B[lo + 1] - B[lo]) * rect) * dif;
+    return r;
+}
+
 struct LinePrep {
-    double *vnu, *sp, *recalf, *zeta;
+    double *vnu, *sp, *recalf, *zeta, *sppsp;
     int *iz, *pmin, *home;
 };
 
@@ -196,19 +220,21 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= J.line_n) return;
     int64_t i = J.line_lo + k, o = J.lp_off + k;
-    double sp = 0., vnu = L.nu[i], recalf = 0., zeta = 0.;
+    double sp = 0., vnu = L.nu[i], recalf = 0., zeta = 0., sppsp = 0.;
     int iz = 1;
     int pm = 0x3fffffff;
-    int fl = L.iflg[i];
-    if (fl != 0 && fl >= 0) atomicOr(err, 1);
+    const int fl = L.iflg[i];   // IFLG < 0: not a line (RDLIN's IOUT skips it)
     int mol = L.mol[i];
     int m = mol % 100, iso = (mol % 1000) / 100;
     if (iso == 0) iso = 10;
-    bool ok = fl == 0 && m >= 1 && m <= NMOL && J.sel[m] && iso <= c_maxiso[m - 1];
+    bool ok = fl >= 0 && m >= 1 && m <= NMOL && J.sel[m] && iso <= c_maxiso[m - 1];
     if (ok) {
         double sui = (double)L.s[i] * J.w[m];
         if (J.jrad == 1) sui = sui * radfn(vnu, J.xkt);
         if (sui != 0.) {
+            // Y, G (IFLAG 1) or GAMMA1, GAMMA2 (IFLAG 3); IFLAG 2 carries none
+            LcPair lc{0., 0.};
+            if (fl == 1 || fl == 3) lc = lc_coeffs(L, i, J.lcq_lo, J.lcq_anchor, J.lcq_rect, J.lcq_dif);
             vnu = vnu + J.rhorat * (double)L.shift[i];
             double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
             double a_air = (double)L.a_air[i];
@@ -219,6 +245,7 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
             double alfa0i = a_air * tmpcor;
             double hwhmsi = (double)L.a_self[i] * tmpcor;
             double alfl = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+            if (fl == 3) alfl = alfl * (1.0 - lc.a * J.pavp0 - lc.b * J.pavp2);   // reduced width
             double alfad = vnu * J.alfd1[m][iso];
             zeta = alfl / (alfl + alfad);
             double fzeta = 100. * zeta;
@@ -233,8 +260,13 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
                 if (srad <= -1.001) srad = fabs(srad);
                 if (srad > -0.999)
                     sui = sui * J.scor[m][iso] * exp(-(srad * J.betacr)) * (1. + exp(-(vnu / J.xkt)));
-                if (sui * recalf > 0.) {
-                    sp = sui;
+                // SP = SUI (1 + G P^2), SPPSP = SUI Y P / SP (clblm_ODLAY.f90:1937-1940); only
+                // uncoupled lines (IFLAG 0) pass the depth test (DPTMIN = 0, :1992-2013)
+                const double spl = (fl == 1) ? sui * (1. + lc.b * J.pavp2) : sui;
+                const double spp = (fl == 1) ? sui * lc.a * J.pavp0 : 0.;
+                if (fl != 0 ? spl != 0. : sui * recalf > 0.) {
+                    sp = spl;
+                    sppsp = spp / spl;
                     const double DXF1 = 0.002, HWF1 = 4.;
                     pm = first_fit_panel(panels + J.panel_off, J.npanel, vnu, recalf, J.dv, J.dv / DXF1,
                                          HWF1 / DXF1, J.max1);
@@ -244,6 +276,7 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
     }
     lp.vnu[o] = vnu;
     lp.sp[o] = sp;
+    lp.sppsp[o] = sppsp;
     lp.recalf[o] = recalf;
     lp.zeta[o] = zeta;
     lp.iz[o] = iz;
@@ -393,9 +426,10 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
         int j2shft = ftrunc(zint * (1. - conf2) + rshft);
         int j3shft = ftrunc(zint * (1. - conf3) + rshft);
         int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
-        double zf1 = ((double)(jmin1 - 2) - zint) * zslope;
-        double zf2 = ((double)(jmin2 - 2) - zint * conf2) * zslope;
-        double zf3 = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+        const double zf1l = ((double)(jmin1 - 2) - zint) * zslope;
+        const double zf2l = ((double)(jmin2 - 2) - zint * conf2) * zslope;
+        const double zf3l = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+        double zf1 = zf1l, zf2 = zf2l, zf3 = zf3l;
         nev += (unsigned long long)(jmax1 - jmin1 + 1);
         // CONV_U R1 points at a time (see k_conv32)
         int j1 = jmin1;
@@ -432,6 +466,27 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
             if (j1 >= 1) {
                 const double4 g = T.g1[iz1];
                 atomicAdd(&a1[j1], s1 * g.x + sd * g.y + sv * g.z);
+            }
+        }
+        const double dptrat = lp.sppsp[o];
+        if (dptrat != 0.) {
+            // line coupling (clblm_ODLAY.f90:2195-2235): the same walk again, each
+            // term scaled by CLC DPTRAT and by its table argument ZF
+            const double c1 = s1 * CLC1 * dptrat, c2 = s2 * CLC2 * dptrat, c3 = s3 * CLC3 * dptrat;
+            const double cd = sd * CLC1 * dptrat, cv = sv * CLC1 * dptrat;
+            double y1 = zf1l, y2 = zf2l, y3 = zf3l;
+            for (int j1 = jmin1; j1 <= jmax1; j1++) {
+                const int j2 = j1 - j2shft, j3 = j1 - j3shft;
+                y3 = y3 + zslope;
+                y2 = y2 + zslope;
+                y1 = y1 + zslope;
+                const int iz3 = ftrunc(fabs(y3) + 1.5), iz2 = ftrunc(fabs(y2) + 1.5), iz1 = ftrunc(fabs(y1) + 1.5);
+                if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], c3 * T.f3[iz3] * y3);
+                if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], c2 * T.f2[iz2] * y2);
+                if (j1 >= 1) {
+                    const double4 g = T.g1[iz1];
+                    atomicAdd(&a1[j1], (c1 * g.x + cd * g.y + cv * g.z) * y1);
+                }
             }
         }
     }
@@ -487,9 +542,10 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
         int j2shft = ftrunc(zint * (1. - conf2) + rshft);
         int j3shft = ftrunc(zint * (1. - conf3) + rshft);
         int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
-        double zf1 = ((double)(jmin1 - 2) - zint) * zslope;
-        double zf2 = ((double)(jmin2 - 2) - zint * conf2) * zslope;
-        double zf3 = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+        const double zf1l = ((double)(jmin1 - 2) - zint) * zslope;
+        const double zf2l = ((double)(jmin2 - 2) - zint * conf2) * zslope;
+        const double zf3l = ((double)(jmin3 - 2) - zint * conf3) * zslope;
+        double zf1 = zf1l, zf2 = zf2l, zf3 = zf3l;
         nev += (unsigned long long)(jmax1 - jmin1 + 1);
         // CONV_U R1 points at a time, so that their table loads are in flight together;
         // the ZF arguments are still accumulated one step at a time.  Indices of points
@@ -530,6 +586,26 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
                 atomicAdd(&a1[j1], (double)__fmaf_rn(sv, g.z, __fmaf_rn(sd, g.y, s1 * g.x)));
             }
         }
+        const double dptrat = lp.sppsp[o];
+        if (dptrat != 0.) {   // line coupling, as in k_conv64
+            const float c1 = (float)((double)s1 * CLC1 * dptrat), c2 = (float)((double)s2 * CLC2 * dptrat);
+            const float c3 = (float)((double)s3 * CLC3 * dptrat), cd = (float)((double)sd * CLC1 * dptrat);
+            const float cv = (float)((double)sv * CLC1 * dptrat);
+            double y1 = zf1l, y2 = zf2l, y3 = zf3l;
+            for (int j1 = jmin1; j1 <= jmax1; j1++) {
+                const int j2 = j1 - j2shft, j3 = j1 - j3shft;
+                y3 = y3 + zslope;
+                y2 = y2 + zslope;
+                y1 = y1 + zslope;
+                const int iz3 = ftrunc(fabs(y3) + 1.5), iz2 = ftrunc(fabs(y2) + 1.5), iz1 = ftrunc(fabs(y1) + 1.5);
+                if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(c3 * t3[iz3] * (float)y3));
+                if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(c2 * t2[iz2] * (float)y2));
+                if (j1 >= 1) {
+                    const float4 g = T.g1s[iz1];
+                    atomicAdd(&a1[j1], (double)(__fmaf_rn(cv, g.z, __fmaf_rn(cd, g.y, c1 * g.x)) * (float)y1));
+                }
+            }
+        }
     }
     for (int d = 32; d > 0; d >>= 1) nev += __shfl_down(nev, d, 64);
     if ((threadIdx.x & 63) == 0 && nev) atomicAdd(work, nev);
@@ -567,7 +643,7 @@ __device__ __forceinline__ bool f4_accept(const Job &J, const LinesDev &L, int64
     if (iso == 0) iso = 10;
     *m_out = m;
     *iso_out = iso;
-    if (!(L.iflg[i] == 0 && m >= 1 && m <= NMOL && J.sel[m] && iso <= c_maxiso[m - 1])) return false;
+    if (!(L.iflg[i] >= 0 && m >= 1 && m <= NMOL && J.sel[m] && iso <= c_maxiso[m - 1])) return false;
     const double sui = (double)L.s[i] * J.w[m];
     return sui != 0. && !(L.nu[i] < J.v1l4);
 }
@@ -584,7 +660,6 @@ __global__ void __launch_bounds__(256) k_f4acc(const Job *jobs, LinesDev L, F4Pr
     bool on = false;
     if (k < b) {
         const int64_t i = J.f4_lo + k;
-        if (L.iflg[i] > 0) atomicOr(err, 1);
         int m, iso;
         on = f4_accept(J, L, i, &m, &iso);
         if (on && L.nu[i] + J.rhorat * (double)L.shift[i] > J.v2l4) atomicMin(&f.stop[blockIdx.y], (int)k);
@@ -608,12 +683,15 @@ __global__ void __launch_bounds__(256) k_f4acc(const Job *jobs, LinesDev L, F4Pr
 // strength sum is positive: strengths below 1e-35 are zeroed (they still belong to
 // the group), so a part is emitted iff it holds a line of strength >= 1e-35.
 struct Comp {
-    double *v, *s, *al, *ad;   // per layer at f4_off + position
-    uint8_t *fl;               // bit 0: CO2; bit 1: strength >= 1e-35
+    double *v, *s, *al, *ad, *sp2;   // per layer at f4_off + position (sp2: coupling SPP)
+    uint8_t *fl;               // bit 0: CO2; bit 1: strength >= 1e-35; bit 2: that and SPP != 0
     int *recpos;               // per layer record: position of its first accepted line
     int *nacc;                 // per layer: accepted lines up to the stop line
 };
-constexpr uint8_t CF_CO2 = 1, CF_LIVE = 2;
+// A coupled line (CF_CPL) is not shrunk: SHRINQ copies it out as it comes (MOL 0
+// unless CO2) and it joins no sum, but it takes part in the VLMT chain like any line
+// (clblm_ODLAY_LineF4.f90:1563-1574).
+constexpr uint8_t CF_CO2 = 1, CF_LIVE = 2, CF_CPL = 4;
 
 // Exclusive prefix of the per-record accepted counts truncated after the stop line
 // (the stop record counts its lines up to the stop line, later records none), one
@@ -689,6 +767,9 @@ __global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4P
         alfa = (alfa - rvmr * (double)L.a_self[i]) / (1.0 - rvmr);
     }
     double epp = (double)L.e_lo[i];
+    const int fl = L.iflg[i];
+    LcPair lc{0., 0.};   // Y, G (IFLAG 1) or GAMMA1, GAMMA2 (IFLAG 3), LINF4Q's bracket
+    if (fl == 1 || fl == 3) lc = lc_coeffs(L, i, J.lc4_lo, J.lc4_anchor, J.lc4_rect, J.lc4_dif);
     if (vnu == 0.) sui = 2. * sui;
     if (epp <= -1.001) epp = fabs(epp);
     if (epp > -0.999) sui = sui * J.scor[m][iso] * exp(-(epp * J.betacr4)) * (1. + exp(-(vnu * J.beta4)));
@@ -696,11 +777,17 @@ __global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4P
     const double alfa0i = alfa * tmpcor;
     const double hwhmsi = (double)L.a_self[i] * tmpcor;
     alfa = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+    if (fl == 3) alfa = alfa * (1.0 - lc.a * J.pavp0 - lc.b * J.pavp2);
+    // SP = SUI (1 + G P^2), SPP = SUI Y P (:1130-1131); Y = G = 0 unless IFLAG 1
+    const double sp = fl == 1 ? sui * (1. + lc.b * J.pavp2) : sui;
+    const double spp = fl == 1 ? sui * lc.a * J.pavp0 : 0.;
+    const bool live = !(sp < 1.0e-35);
     c.v[o] = vnu;
-    c.s[o] = sui;
+    c.s[o] = sp;
+    c.sp2[o] = spp;
     c.al[o] = alfa;
     c.ad[o] = vnu * J.alfd1[m][iso];
-    c.fl[o] = (uint8_t)((m == 2 ? CF_CO2 : 0) | (sui < 1.0e-35 ? 0 : CF_LIVE));
+    c.fl[o] = (uint8_t)((m == 2 ? CF_CO2 : 0) | (live ? CF_LIVE : 0) | (live && spp != 0. ? CF_CPL : 0));
 }
 
 // record end e (relative record index) of a layer as a position
@@ -726,7 +813,8 @@ __device__ __forceinline__ void shr_count_step(ShrCount &w, bool first, double x
         w.fa = 0; w.fb = 0;
         w.vlmt = w.vprev + dvh;
     }
-    fl &= 3;
+    fl &= 7;
+    w.emitted += (fl & CF_CPL) ? 1 : 0;
     w.fa |= fl == CF_LIVE;
     w.fb |= fl == (CF_LIVE | CF_CO2);
     w.vprev = x;
@@ -956,18 +1044,20 @@ __global__ void __launch_bounds__(64) k_emit(const Job *jobs, F4Prep f, Comp c, 
     __syncthreads();
     const int m = ngs;
     const double *ss = c.s + J.f4_off + p0, *aa = c.al + J.f4_off + p0, *dd = c.ad + J.f4_off + p0;
+    const double *pp = c.sp2 + J.f4_off + p0;
     int base_out = ck.z;
     for (int g0 = 0; g0 < m; g0 += 64) {
         const int g = g0 + tid;
-        int ha = 0, hb = 0;
+        int ha = 0, hb = 0, hc = 0;
         if (g < m) {
             for (int i = gs[g]; i < gs[g + 1]; i++) {
-                const int fl = sf[i] & 3;
+                const int fl = sf[i] & 7;
                 ha |= fl == CF_LIVE;
                 hb |= fl == (CF_LIVE | CF_CO2);
+                hc += (fl & CF_CPL) ? 1 : 0;
             }
         }
-        const int cnt = ha + hb;
+        const int cnt = hc + ha + hb;
         int incl = cnt;
         for (int d = 1; d < 64; d <<= 1) {
             const int o = __shfl_up(incl, d, 64);
@@ -977,6 +1067,12 @@ __global__ void __launch_bounds__(64) k_emit(const Job *jobs, F4Prep f, Comp c, 
             int64_t o = J.f4_off + base_out + incl - cnt;
             ShrinkSums a{0, 0, 0, 0}, b{0, 0, 0, 0};
             for (int i = gs[g]; i < gs[g + 1]; i++) {
+                if (sf[i] & CF_CPL) {   // copied out as it comes, before the group's sums
+                    sh.vnu[o] = sv[i]; sh.sp[o] = ss[i]; sh.alfa[o] = aa[i]; sh.epp[o] = dd[i];
+                    sh.spp[o] = pp[i]; sh.co2[o] = (sf[i] & CF_CO2) ? 1 : 0;
+                    o++;
+                    continue;
+                }
                 if (!(sf[i] & CF_LIVE)) continue;
                 const double x = sv[i], sp = ss[i];
                 if (sf[i] & CF_CO2) {
@@ -2401,6 +2497,31 @@ static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od
     J.tratio = T / TEMP0;
     J.rhorat = (Pv / P0) * (TEMP0 / T);
     J.pavp0 = Pv / P0;
+    J.pavp2 = J.pavp0 * J.pavp0;
+    // line-coupling brackets over TEMPLC = 200, 250, 296, 340 K.  LNCORQ
+    // (clblm_ODLAY.f90:1660-1667): the first ILC = 1..3 with TAVE < TEMPLC(ILC+1)
+    // (else 3), anchored at TEMPLC(ILC).  LINF4Q (clblm_ODLAY_LineF4.f90:860-866): the
+    // first ILC = 1..4 with TAVE <= TEMPLC(ILC), moved into 2..4, anchored at TEMPLC(ILC).
+    {
+        const double TEMPLC[4] = {200.0, 250.0, 296.0, 340.0};
+        int ilc = 1;
+        for (int il = 1; il <= 3; il++) {
+            ilc = il;
+            if (T < TEMPLC[ilc]) break;
+        }
+        J.lcq_lo = ilc - 1;
+        J.lcq_anchor = ilc - 1;
+        J.lcq_rect = 1.0 / (TEMPLC[ilc] - TEMPLC[ilc - 1]);
+        J.lcq_dif = T - TEMPLC[ilc - 1];
+        int k4 = 1;
+        while (k4 <= 4 && !(T <= TEMPLC[k4 - 1])) k4++;
+        if (k4 == 1) k4 = 2;
+        if (k4 == 5) k4 = 4;
+        J.lc4_lo = k4 - 2;
+        J.lc4_anchor = k4 - 1;
+        J.lc4_rect = 1.0 / (TEMPLC[k4 - 1] - TEMPLC[k4 - 2]);
+        J.lc4_dif = T - TEMPLC[k4 - 1];
+    }
     // LINF4Q constants (clblm_ODLAY_LineF4.f90:945-951)
     J.beta4 = RADCN2 / T;
     J.betacr4 = J.beta4 - RADCN2 / TEMP0;
@@ -2603,7 +2724,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_blk = cv.take<R4Blk>(B.blks.size());
     size_t o_tiles = cv.take<int2>(B.tiles.size());
     size_t o_abs = cv.take<double>(std::max<size_t>(1, B.absrb.size()));
-    size_t o_lp_d = cv.take<double>(4 * (size_t)B.tot_line);
+    size_t o_lp_d = cv.take<double>(5 * (size_t)B.tot_line);
     size_t o_lp_i = cv.take<int>(3 * (size_t)B.tot_line);
     size_t o_A = cv.take<double>((size_t)B.tot_a);
     size_t o_f4_b = cv.take<int8_t>((size_t)B.tot_f4);
@@ -2615,7 +2736,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_htot = cv.take<int>((size_t)std::max<int64_t>(1, hcap * nlay));
     size_t o_ftot = cv.take<double>((size_t)std::max<int64_t>(1, 2 * fcap * nlay));
     size_t o_G = cv.take<int>((size_t)B.tot_rec * LCAP);
-    size_t o_cd = cv.take<double>(4 * (size_t)B.tot_f4);
+    size_t o_cd = cv.take<double>(5 * (size_t)B.tot_f4);
     size_t o_cf = cv.take<uint8_t>((size_t)B.tot_f4);
     size_t o_cr = cv.take<int>((size_t)B.tot_rec);
     size_t o_cn = cv.take<int>((size_t)nlay);
@@ -2726,6 +2847,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     double *lpd = at<double>(base, o_lp_d);
     int *lpi = at<int>(base, o_lp_i);
     lp.vnu = lpd; lp.sp = lpd + B.tot_line; lp.recalf = lpd + 2 * B.tot_line; lp.zeta = lpd + 3 * B.tot_line;
+    lp.sppsp = lpd + 4 * B.tot_line;
     lp.iz = lpi; lp.pmin = lpi + B.tot_line; lp.home = lpi + 2 * B.tot_line;
     double *d_A = at<double>(base, o_A);
     F4Prep f;
@@ -2740,6 +2862,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     {
         double *cd = at<double>(base, o_cd);
         cmp.v = cd; cmp.s = cd + B.tot_f4; cmp.al = cd + 2 * B.tot_f4; cmp.ad = cd + 3 * B.tot_f4;
+        cmp.sp2 = cd + 4 * B.tot_f4;
         cmp.fl = at<uint8_t>(base, o_cf);
         cmp.recpos = at<int>(base, o_cr);
         cmp.nacc = at<int>(base, o_cn);
@@ -2986,22 +3109,42 @@ int clblm_linelist_upload(clblm_ctx *c, int64_t n, const double *nu, const float
     if (!c || !handle) return set_err(CLBLM_ERR_ARG, "ctx or handle is NULL");
     if (n <= 1 || !nu || !s || !a_air || !a_self || !e_lo || !tmpalf || !shift || !mol || !iflg)
         return set_err(CLBLM_ERR_ARG, "line list needs at least two lines and every field");
-    for (int64_t i = 1; i < n; i++)
-        if (!(nu[i] >= nu[i - 1])) return set_err(CLBLM_ERR_ARG, "line centres must be sorted ascending");
-    for (int64_t i = 0; i < n; i++)
-        if (iflg[i] > 0) return set_err(CLBLM_ERR_UNSUPPORTED, "line coupling records (IFLG 1..3) are not supported");
+    // Records with IFLG < 0 are not lines (the coupling coefficients of the line before
+    // them, or records RDLIN skips); their VNU field is kept as given (nu_rec) and the
+    // searches see the centre of the line before them, so that the centres are sorted.
+    std::vector<double> cen(nu, nu + n);
+    int64_t first = n;   // the first line
+    double prev = 0.;
+    for (int64_t i = 0; i < n; i++) {
+        if (iflg[i] == 1 || iflg[i] == 3) {
+            if (i + 1 >= n || (i + 1) / REC != i / REC || iflg[i + 1] >= 0)
+                return set_err(CLBLM_ERR_ARG, "a line with IFLG 1 or 3 needs its coefficient record (IFLG < 0) "
+                                              "right after it in the same 250-line record");
+        }
+        if (iflg[i] < 0) {
+            cen[i] = prev;
+            continue;
+        }
+        if (first < n && !(nu[i] >= prev)) return set_err(CLBLM_ERR_ARG, "line centres must be sorted ascending");
+        if (first == n) {
+            first = i;
+            for (int64_t k = 0; k < i; k++) cen[k] = nu[i];   // leading non-line records
+        }
+        prev = nu[i];
+    }
     HIPCHK(hipSetDevice(c->device));
     LineListH *L = new LineListH;
     L->n = n;
-    L->nu.assign(nu, nu + n);
-    size_t bytes = (size_t)n * (8 + 6 * 4 + 2 * 4) + 8 * 256;
+    L->nu = cen;
+    size_t bytes = (size_t)n * (8 + 8 + 6 * 4 + 2 * 4) + 9 * 256;
     HIPCHK(hipMalloc(&L->mem, bytes));
     Carve cv;
     char *b = (char *)L->mem;
     size_t onu = cv.take<double>(n), os = cv.take<float>(n), oa = cv.take<float>(n), osf = cv.take<float>(n);
     size_t oe = cv.take<float>(n), ot = cv.take<float>(n), osh = cv.take<float>(n), om = cv.take<int32_t>(n);
-    size_t oi = cv.take<int32_t>(n);
-    HIPCHK(hipMemcpy(b + onu, nu, 8 * n, hipMemcpyHostToDevice));
+    size_t oi = cv.take<int32_t>(n), onr = cv.take<double>(n);
+    HIPCHK(hipMemcpy(b + onu, cen.data(), 8 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + onr, nu, 8 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b + os, s, 4 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b + oa, a_air, 4 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b + osf, a_self, 4 * n, hipMemcpyHostToDevice));
@@ -3020,6 +3163,7 @@ int clblm_linelist_upload(clblm_ctx *c, int64_t n, const double *nu, const float
     L->dev.shift = at<float>(b, osh);
     L->dev.mol = at<int32_t>(b, om);
     L->dev.iflg = at<int32_t>(b, oi);
+    L->dev.nu_rec = at<double>(b, onr);
     std::lock_guard<std::mutex> g(c->mu);
     c->lists.push_back(L);
     *handle = (int)c->lists.size() - 1;

@@ -51,6 +51,51 @@ class LineList:
         return int(self.nu.shape[0])
 
 
+def couple_lines(lines: LineList, coupled, iflag, a_nodes, b_nodes) -> LineList:
+    """The line list with the lines at indices ``coupled`` flagged IFLG = ``iflag`` (1:
+    Y and G, 3: reduced widths GAMMA1 and GAMMA2; one value or one per coupled line)
+    and followed by their coefficient
+    records, laid out as LNFL writes them: the record after a coupled line has IFLG -1
+    and holds the coefficients at 200, 250, 296 and 340 K, ``a_nodes`` (Y / GAMMA1) in
+    its VNU, A_AIR, MOL (as REAL*4 bits) and TMPALF fields and ``b_nodes`` (G / GAMMA2)
+    in its S, E_LO, A_SELF and SHIFT fields.  A line and its coefficient record stay in
+    one 250-line record: where only one slot is left, a filler record (IFLG -1, not a
+    line) is put first.  a_nodes / b_nodes: shape (4,) or (len(coupled), 4)."""
+    coupled = list(coupled)
+    cset = {int(i): k for k, i in enumerate(coupled)}
+    flags = np.broadcast_to(np.asarray(iflag, np.int32), (len(coupled),))
+    A = np.broadcast_to(np.asarray(a_nodes, np.float64), (len(coupled), 4))
+    Bn = np.broadcast_to(np.asarray(b_nodes, np.float64), (len(coupled), 4))
+    fields = ("nu", "s", "a_air", "a_self", "e_lo", "tmpalf", "shift", "mol", "iflg")
+    out = {f: [] for f in fields}
+
+    def put(vals):
+        for f in fields:
+            out[f].append(vals[f])
+
+    for i in range(lines.n):
+        row = {f: getattr(lines, f)[i] for f in fields}
+        if i in cset:
+            if len(out["nu"]) % 250 == 249:   # the pair would straddle two records
+                filler = dict(row)
+                filler["iflg"] = np.int32(-1)
+                put(filler)
+            k = cset[i]
+            row = dict(row)
+            row["iflg"] = np.int32(flags[k])
+            put(row)
+            a, b = A[k], Bn[k]
+            put({"nu": np.float64(a[0]), "a_air": np.float32(a[1]),
+                 "mol": np.float32(a[2]).view(np.int32), "tmpalf": np.float32(a[3]),
+                 "s": np.float32(b[0]), "e_lo": np.float32(b[1]), "a_self": np.float32(b[2]),
+                 "shift": np.float32(b[3]), "iflg": np.int32(-1)})
+        else:
+            put(row)
+    dt = dict(nu=np.float64, s=np.float32, a_air=np.float32, a_self=np.float32, e_lo=np.float32,
+              tmpalf=np.float32, shift=np.float32, mol=np.int32, iflg=np.int32)
+    return LineList(**{f: np.asarray(out[f], dtype=dt[f]) for f in fields})
+
+
 @dataclasses.dataclass
 class Layer:
     pave: float             # hPa

@@ -64,10 +64,18 @@ int clblm_version(void);
 
 /* ---- line list --------------------------------------------------------------
  * Replaces the TAPE3 reads of RDLIN (clblm_ODLAY.f90:2571-2798) and RDLNFL
- * (clblm_ODLAY_LineF4.f90:1355-1498).  Host arrays of n lines sorted by nu;
- * consecutive groups of 250 lines form the TAPE3 records.  mol = m + 100*iso.
- * iflg: 0 for a plain line, < 0 for a record the readers skip (line coupling,
- * IFLG 1..3, is not supported and is rejected). */
+ * (clblm_ODLAY_LineF4.f90:1355-1498).  Host arrays of n records in TAPE3 order;
+ * consecutive groups of 250 form the TAPE3 records.  mol = m + 100*iso.
+ * iflg: 0 for a plain line; 1 for a line-coupled line (Y and G) and 3 for one with
+ * reduced widths (GAMMA1, GAMMA2), each followed in the same 250-record by its
+ * coefficient record; 2 for a line treated as plain (LNCORQ reads no coefficients
+ * for it); < 0 for a record that is not a line (a coefficient record, or one the
+ * readers skip).  A coefficient record holds the coefficients at 200, 250, 296 and
+ * 340 K: Y / GAMMA1 in its nu, a_air, mol (the bits of a float) and tmpalf fields,
+ * G / GAMMA2 in its s, e_lo, a_self and shift fields (clblm_ODLAY.f90:1821-1829).
+ * The centres of the lines (iflg >= 0) must be sorted; those of the other records
+ * are free.  Returns CLBLM_ERR_ARG for unsorted lines or a coupled line without its
+ * coefficient record. */
 int clblm_linelist_upload(clblm_ctx *ctx, int64_t n, const double *nu, const float *s,
                           const float *a_air, const float *a_self, const float *e_lo,
                           const float *tmpalf, const float *shift, const int32_t *mol,

@@ -542,6 +542,70 @@ static int shrinq(sline *B, int ilo, int ihi, double dvr4) {
     return j;
 }
 
+/* Line coupling.  The record after an IFLG 1 / 3 line (IFLG = -1) holds its
+ * coefficients at TEMPLC = 200, 250, 296, 340 K: A(1..4) in its VNU, ALFA, MOL (the bits
+ * of a REAL*4) and TMPALF fields, B(1..4) in its SP, EPP, HWHM and PSHIFT fields; A is
+ * Y (IFLAG 1) or GAMMA1 (IFLAG 3), B is G or GAMMA2 (clblm_ODLAY.f90:1821-1829,
+ * clblm_ODLAY_LineF4.f90:971-978). */
+static const double TEMPLC[4] = {200.0, 250.0, 296.0, 340.0};
+
+static void lc_nodes(const orc_lines *Ls, int64_t i, double A[5], double B[5]) {
+    const int64_t k = i + 1;
+    float am;
+    int32_t mb = Ls->mol[k];
+    memcpy(&am, &mb, sizeof am);
+    A[1] = Ls->nu[k]; A[2] = Ls->a_air[k]; A[3] = am; A[4] = Ls->tmpalf[k];
+    B[1] = Ls->s[k]; B[2] = Ls->e_lo[k]; B[3] = Ls->a_self[k]; B[4] = Ls->shift[k];
+}
+
+/* LNCORQ, clblm_ODLAY.f90:1660-1667 and 1833-1845 (1-based A, B). */
+static void lc_lncorq(const orc_lines *Ls, int64_t i, double tave, double *a, double *b) {
+    double A[5], B[5];
+    lc_nodes(Ls, i, A, B);
+    int ILC = 1;
+    for (int IL = 1; IL <= 3; IL++) {
+        ILC = IL;
+        if (tave < TEMPLC[ILC + 1 - 1]) break;
+    }
+    if (ILC == 4) ILC = 3;
+    double RECTLC = 1.0 / (TEMPLC[ILC + 1 - 1] - TEMPLC[ILC - 1]);
+    double TMPDIF = tave - TEMPLC[ILC - 1];
+    double SLOPEA = (A[ILC + 1] - A[ILC]) * RECTLC;
+    double SLOPEB = (B[ILC + 1] - B[ILC]) * RECTLC;
+    *a = A[ILC] + SLOPEA * TMPDIF;
+    *b = B[ILC] + SLOPEB * TMPDIF;
+}
+
+/* LINF4Q, clblm_ODLAY_LineF4.f90:860-866 and 971-988. */
+static void lc_linf4q(const orc_lines *Ls, int64_t i, double tave, double *a, double *b) {
+    double A[5], B[5];
+    lc_nodes(Ls, i, A, B);
+    int ILC;
+    for (ILC = 1; ILC <= 4; ILC++)
+        if (tave <= TEMPLC[ILC - 1]) break;
+    if (ILC == 1) ILC = 2;
+    if (ILC == 5) ILC = 4;
+    double RECTLC = 1.0 / (TEMPLC[ILC - 1] - TEMPLC[ILC - 1 - 1]);
+    double TMPDIF = tave - TEMPLC[ILC - 1];
+    double SLOPEY = (A[ILC] - A[ILC - 1]) * RECTLC;
+    double SLOPEG = (B[ILC] - B[ILC - 1]) * RECTLC;
+    *a = A[ILC] + SLOPEY * TMPDIF;
+    *b = B[ILC] + SLOPEG * TMPDIF;
+}
+
+/* The record bounds VMIN / VMAX of the TAPE3 record header: the centres of its first
+ * and last lines (records with IFLG < 0 are not lines). */
+static double rec_vmin(const orc_lines *Ls, int64_t a, int64_t b) {
+    for (int64_t i = a; i < b; i++)
+        if (Ls->iflg[i] >= 0) return Ls->nu[i];
+    return -1e300;
+}
+static double rec_vmax(const orc_lines *Ls, int64_t a, int64_t b) {
+    for (int64_t i = b - 1; i >= a; i--)
+        if (Ls->iflg[i] >= 0) return Ls->nu[i];
+    return -1e300;
+}
+
 /* Layer-wide constants shared by LINF4Q and LNCORQ. */
 typedef struct {
     double scor[48 * 11], rhoslf[48], alfd1[48 * 11];
@@ -575,7 +639,8 @@ static double read_alfa(const orc_lines *Ls, int64_t i) {
 static int linf4q(const orc_lines *Ls, const orc_layer *L, const molec_t *M,
                   double v1l4, double v2l4, double dvr4, shrunk_t *out) {
     const int LIMIN = 1000;
-    double PAVP0 = L->pave / P0; (void)PAVP0;
+    double PAVP0 = L->pave / P0;
+    double PAVP2 = PAVP0 * PAVP0;
     double TRATIO = L->tave / TEMP0;
     double RHORAT = (L->pave / P0) * (TEMP0 / L->tave);
     double BETA = RADCN2 / L->tave;
@@ -593,11 +658,11 @@ static int linf4q(const orc_lines *Ls, const orc_layer *L, const molec_t *M,
         while (rec < nrec) {
             int64_t a = rec * 250, b = a + 250 < Ls->n ? a + 250 : Ls->n;
             rec++;
-            if (Ls->nu[b - 1] < VLO) continue; /* VMAX < VLO: skip record */
+            if (rec_vmax(Ls, a, b) < VLO) continue; /* VMAX < VLO: skip record */
             got = 1;
             for (int64_t i = a; i < b; i++) {
                 if (Ls->iflg[i] < 0) continue;
-                if (Ls->iflg[i] != 0) { free(B); return fail("line coupling (IFLG 1..3) not supported"); }
+                const int IFLAG = Ls->iflg[i];
                 int m = Ls->mol[i] % 100;
                 int iso = (Ls->mol[i] % 1000) / 100;
                 if (iso == 0) iso = 10;
@@ -606,6 +671,9 @@ static int linf4q(const orc_lines *Ls, const orc_layer *L, const molec_t *M,
                 if (sui == 0.) continue;
                 if (Ls->nu[i] < VLO) continue;
                 IJ++;
+                double YI = 0., GI = 0., GAMMA1 = 0., GAMMA2 = 0.;
+                if (IFLAG == 1) lc_linf4q(Ls, i, L->tave, &YI, &GI);
+                if (IFLAG == 3) lc_linf4q(Ls, i, L->tave, &GAMMA1, &GAMMA2);
                 sline *S = &B[IJ];
                 S->vnu = Ls->nu[i] + RHORAT * (double)Ls->shift[i];
                 S->alfa = read_alfa(Ls, i);
@@ -619,9 +687,10 @@ static int linf4q(const orc_lines *Ls, const orc_layer *L, const molec_t *M,
                 double alfa0i = S->alfa * tmpcor;
                 double hwhmsi = (double)Ls->a_self[i] * tmpcor;
                 S->alfa = alfa0i * (RHORAT - M->rhoslf[m]) + hwhmsi * M->rhoslf[m];
+                if (IFLAG == 3) S->alfa = S->alfa * (1.0 - GAMMA1 * PAVP0 - GAMMA2 * PAVP2);
                 S->epp = S->vnu * M->alfd1[m * 11 + iso];
-                S->sp = sui * (1. + 0. * (PAVP0 * PAVP0));
-                S->spp = sui * 0. * PAVP0;
+                S->sp = sui * (1. + GI * PAVP2);
+                S->spp = sui * YI * PAVP0;
                 if (S->vnu > VHI) { ieof = 1; break; }
             }
             break;
@@ -845,10 +914,9 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
     double RHORAT = (PAVE / P0) * (TEMP0 / TAVE);
     double PAVP0 = PAVE / P0;
     double PAVP2 = PAVP0 * PAVP0;
-    double CLC1 = 4. / ((double)(NX1 - 1)); (void)CLC1;
+    double CLC1 = 4. / ((double)(NX1 - 1)), CLC2 = 16. / ((double)(NX2 - 1)), CLC3 = 64. / ((double)(NX3 - 1));
     double WAVDXF = DV / DXF1, HWDXF = HWF1 / DXF1;
     double CONF2 = DV / DVR2, CONF3 = DV / DVR3;
-    (void)NX2; (void)NX3;
 
     int64_t numTot = 0;
     int IEOF = 0, IDATA = 0, ILO = 0, IHI = -999, IPANEL = 0, ISTOP = 0, IFPAN = 1;
@@ -868,7 +936,7 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
             while (rec < nrec) {
                 a = rec * 250; b = a + 250 < Ls->n ? a + 250 : Ls->n;
                 rec++;
-                if (Ls->nu[b - 1] < VBOT) continue;
+                if (rec_vmax(Ls, a, b) < VBOT) continue;
                 found = 1;
                 break;
             }
@@ -879,13 +947,10 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
             nrec_lines = (int)(b - a);
             for (int64_t i = a; i < b; i++) {
                 int k = (int)(i - a) + 1;
-                if (Ls->iflg[i] >= 0) {
-                    if (Ls->iflg[i] != 0) { ret = fail("line coupling (IFLG 1..3) not supported"); goto done; }
-                    iout[++IJ] = k;
-                }
+                if (Ls->iflg[i] >= 0) iout[++IJ] = k;
             }
             ILO = 1;
-            if (Ls->nu[a] < VBOT) {
+            if (rec_vmin(Ls, a, b) < VBOT) {
                 for (int j = 1; j <= IJ; j++) {
                     ILO = j;
                     if (Ls->nu[a + iout[j] - 1] >= VBOT) break;
@@ -912,11 +977,16 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
                 double sui = (double)Ls->s[i] * L->w[m];
                 if (JRAD == 1) sui = sui * orc_radfn(h->vnu, XKT);
                 if (sui == 0.) continue;
+                const int IFLAG = Ls->iflg[i];
+                double YI = 0., GI = 0., GAMMA1 = 0., GAMMA2 = 0.;
+                if (IFLAG == 1) lc_lncorq(Ls, i, TAVE, &YI, &GI);
+                if (IFLAG == 3) lc_lncorq(Ls, i, TAVE, &GAMMA1, &GAMMA2);
                 h->vnu = h->vnu + RHORAT * (double)Ls->shift[i];
                 double tmpcor = pow(TRATIO, (double)Ls->tmpalf[i]);
                 double alfa0i = read_alfa(Ls, i) * tmpcor;
                 double hwhmsi = (double)Ls->a_self[i] * tmpcor;
                 double alfl = alfa0i * (RHORAT - M.rhoslf[m]) + hwhmsi * M.rhoslf[m];
+                if (IFLAG == 3) alfl = alfl * (1.0 - GAMMA1 * PAVP0 - GAMMA2 * PAVP2);
                 double alfad = h->vnu * M.alfd1[m * 11 + iso];
                 double zeta = alfl / (alfl + alfad);
                 h->zetai = zeta;
@@ -933,12 +1003,14 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
                 if (srad <= -1.001) srad = fabs(srad);
                 if (srad > -0.999)
                     sui = sui * M.scor[m * 11 + iso] * exp(-(srad * BETACR)) * (1. + exp(-(h->vnu / XKT)));
-                h->sp = sui * (1. + 0. * PAVP2);
-                double sppi = sui * 0. * PAVP0;
+                h->sp = sui * (1. + GI * PAVP2);
+                double sppi = sui * YI * PAVP0;
                 h->sppsp = sppi / h->sp;
                 h->srad = 0.0;
-                /* rejection with DPTMIN = DPTFAC = 0 */
-                if (sui * h->recalf <= 0. && h->sppsp == 0.) { h->sp = 0.; h->sppsp = 0.; continue; }
+                /* rejection of uncoupled lines with DPTMIN = DPTFAC = 0 (:1992-2013) */
+                if (IFLAG == 0 && sui * h->recalf <= 0. && h->sppsp == 0.) {
+                    h->sp = 0.; h->sppsp = 0.; continue;
+                }
             }
             /* 70..: convolve, panel out, loop */
             for (;;) {
@@ -968,9 +1040,10 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
                         int j2shft = ftrunc(zint * (1. - CONF2) + rshft);
                         int j3shft = ftrunc(zint * (1. - CONF3) + rshft);
                         int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
-                        double zf1 = ((double)(jmin1 - 2) - zint) * zslope;
-                        double zf2 = ((double)(jmin2 - 2) - zint * CONF2) * zslope;
-                        double zf3 = ((double)(jmin3 - 2) - zint * CONF3) * zslope;
+                        double zf1l = ((double)(jmin1 - 2) - zint) * zslope;
+                        double zf2l = ((double)(jmin2 - 2) - zint * CONF2) * zslope;
+                        double zf3l = ((double)(jmin3 - 2) - zint * CONF3) * zslope;
+                        double zf1 = zf1l, zf2 = zf2l, zf3 = zf3l;
                         for (int j1 = jmin1; j1 <= jmax1; j1++) {
                             int j2 = j1 - j2shft, j3 = j1 - j3shft;
                             zf3 = zf3 + zslope;
@@ -984,6 +1057,27 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
                             R3[j3] = R3[j3] + s3 * F3[iz3];
                             R2[j2] = R2[j2] + s2 * F2[iz2];
                             R1[j1] = R1[j1] + s1 * F1[iz1] + sd * FG[iz1] + sv * XVER[iz1];
+                        }
+                        if (h->sppsp != 0.) {
+                            /* line coupling, :2195-2235 */
+                            double dptrat = h->sppsp;
+                            s3 = s3 * CLC3 * dptrat;
+                            s2 = s2 * CLC2 * dptrat;
+                            s1 = s1 * CLC1 * dptrat;
+                            sd = sd * CLC1 * dptrat;
+                            sv = sv * CLC1 * dptrat;
+                            for (int j1 = jmin1; j1 <= jmax1; j1++) {
+                                int j2 = j1 - j2shft, j3 = j1 - j3shft;
+                                zf3l = zf3l + zslope;
+                                zf2l = zf2l + zslope;
+                                zf1l = zf1l + zslope;
+                                int iz3 = ftrunc(fabs(zf3l) + 1.5);
+                                int iz2 = ftrunc(fabs(zf2l) + 1.5);
+                                int iz1 = ftrunc(fabs(zf1l) + 1.5);
+                                R3[j3] = R3[j3] + s3 * F3[iz3] * zf3l;
+                                R2[j2] = R2[j2] + s2 * F2[iz2] * zf2l;
+                                R1[j1] = R1[j1] + (s1 * F1[iz1] + sd * FG[iz1] + sv * XVER[iz1]) * zf1l;
+                            }
                         }
                     }
                     if (IPANEL == -1) { ILAST = IHI; IPANEL = IDATA; }

@@ -21,7 +21,7 @@ extern "C" {
 /* TAPE3-equivalent line list (one record = 250 consecutive lines). */
 typedef struct {
     int64_t n;
-    const double *nu;      /* line centre, cm-1, sorted ascending */
+    const double *nu;      /* line centre, cm-1, sorted ascending over the lines */
     const float *s;        /* intensity as stored in TAPE3 */
     const float *a_air;    /* air-broadened HWHM at 296 K, 1 atm */
     const float *e_lo;     /* lower-state energy, cm-1 */
@@ -29,7 +29,8 @@ typedef struct {
     const float *a_self;   /* self-broadened HWHM */
     const float *tmpalf;   /* TAPE3 temperature exponent of the width */
     const float *shift;    /* pressure shift, cm-1/atm */
-    const int32_t *iflg;   /* 0 = plain line, <0 = skipped record */
+    const int32_t *iflg;   /* 0 plain line; 1 / 3 coupled (coefficients in the next
+                              record); 2 plain; < 0 not a line (coefficients, skipped) */
 } orc_lines;
 
 typedef struct {
