@@ -15,7 +15,8 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
            "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune", "clblm_scanfn",
-           "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy"]
+           "clblm_linelist_broadening", "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy"]
+OD_SPECIES_BROAD = 1
 TUNE_F4_XCD_CHUNK = 1
 TUNE_F4_FAR_FIELD = 2
 TUNE_F4_STREAM = 3
@@ -36,7 +37,7 @@ class ODParams(ctypes.Structure):
     _fields_ = [("v1", ctypes.c_double), ("v2", ctypes.c_double), ("dvnormal", ctypes.c_double),
                 ("dvnarrow", ctypes.c_double), ("sample", ctypes.c_double), ("dvout", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("contnm", ctypes.c_int32),
-                ("continuum", ctypes.c_void_p), ("nptabs", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("continuum", ctypes.c_void_p), ("nptabs", ctypes.c_int32), ("flags", ctypes.c_int32)]
 
 
 _lib = None
@@ -67,6 +68,7 @@ def load():
     P = ctypes.c_void_p
     lib.clblm_linelist_upload.argtypes = [P, ctypes.c_int64] + [P] * 9 + [ctypes.POINTER(ctypes.c_int)]
     lib.clblm_linelist_free.argtypes = [P, ctypes.c_int]
+    lib.clblm_linelist_broadening.argtypes = [P, ctypes.c_int, P, P, P, P]
     lib.clblm_odlay.argtypes = [P, ctypes.c_int, ctypes.POINTER(Layer), ctypes.POINTER(ODParams),
                                 ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(ctypes.c_double), P]

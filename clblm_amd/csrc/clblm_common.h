@@ -48,6 +48,7 @@ struct Job {
     int64_t rec0, nrec, rec_off;                 // records covering the LINF4Q range
     int64_t chunk_off;                           // scratch for chunk table (nrec entries)
     int contnm, nptabs;
+    int species_broad;                           // odCtrl%speciesBroad
     double v1abs;
     int64_t absrb_off;
     int64_t r4_off;                              // start of this layer's R4 blocks

@@ -158,6 +158,13 @@ __host__ __device__ inline double radfni(double vi, double dvi, double xkt, doub
 // CNVFNQ's coupling factors CLC = HWF / (NX - 1), NX = 2001 (clblm_ODLAY.f90:2117-2119)
 constexpr double CLC1 = 4. / 2000., CLC2 = 16. / 2000., CLC3 = 64. / 2000.;
 
+// IZ = int(100 zeta + ONEPL) (clblm_ODLAY.f90:1893), kept within the 101 table
+// intervals when zeta is not in [0, 1] (only for widths the line shape does not define)
+__device__ __forceinline__ int zeta_index(double fzeta) {
+    const double x = fzeta + ONEPL;
+    return x >= 1. && x < 102. ? (int)x : (x >= 102. ? 101 : 1);
+}
+
 __device__ __forceinline__ double avrat_lerp(const double *tab, int iz, double zetdif) {
     return tab[iz - 1] + zetdif * (tab[iz] - tab[iz - 1]);
 }
@@ -188,7 +195,58 @@ struct LinesDev {
     const float *s, *a_air, *a_self, *e_lo, *tmpalf, *shift;
     const int32_t *mol, *iflg;
     const double *nu_rec;   // the VNU field as given (for IFLG < 0 records: coupling Y(200 K))
+    // species broadening, 7 per record (broadener j at 7 i + j - 1), or null
+    const int32_t *brd_flg;
+    const float *brd_hw, *brd_tmp, *brd_shft;
 };
+
+// PSHIFT as RDLIN / RDLNFL read it: an O2 line with its own species data gets its
+// foreign shift (clblm_ODLAY.f90:2730-2738, clblm_ODLAY_LineF4.f90:1463-1468).
+__device__ inline double read_pshift(const LinesDev &L, int64_t i, int m) {
+    double ps = (double)L.shift[i];
+    if (m == 7 && L.brd_flg && L.brd_flg[7 * i + 6] > 0) {
+        const double rvmr = 0.21;
+        ps = (ps - rvmr * (double)L.brd_shft[7 * i + 6]) / (1.0 - rvmr);
+    }
+    return ps;
+}
+// Species broadening applies to line i: the layer asks for it and the flags sum > 0.
+__device__ inline bool brd_sum(const Job &J, const LinesDev &L, int64_t i) {
+    if (!J.species_broad || !L.brd_flg) return false;
+    int t = 0;
+    for (int j = 0; j < 7; j++) t += L.brd_flg[7 * i + j];
+    return t > 0;
+}
+// The line centre after the pressure shifts: VNU + RHORAT PSHIFT, plus the
+// species-weighted shift differences (LNCORQ :1849-1856, LINF4Q :994-999).
+__device__ inline double shifted_vnu(const Job &J, const LinesDev &L, int64_t i, double vnu, double pshift) {
+    double v = vnu + J.rhorat * pshift;
+    if (brd_sum(J, L, i)) {
+        double acc = 0.;
+        for (int j = 0; j < 7; j++)
+            acc = acc + (J.rhoslf[j + 1] * (double)L.brd_flg[7 * i + j]) * ((double)L.brd_shft[7 * i + j] - pshift);
+        v = v + acc;
+    }
+    return v;
+}
+// The Lorentz width: air and self parts, or with species broadening the
+// species-weighted widths (LNCORQ :1861-1883, LINF4Q :1032-1054).
+__device__ inline double lorentz_width(const Job &J, const LinesDev &L, int64_t i, int m, double alfa0i,
+                                       double hwhmsi) {
+    double alfl = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+    if (brd_sum(J, L, i)) {
+        double alfsum = 0., rs = 0.;
+        for (int j = 0; j < 7; j++) {
+            const double at = (double)L.brd_hw[7 * i + j] * pow(J.tratio, (double)L.brd_tmp[7 * i + j]);
+            alfsum = alfsum + (J.rhoslf[j + 1] * (double)L.brd_flg[7 * i + j]) * at;
+        }
+        for (int j = 0; j < 7; j++) rs = rs + J.rhoslf[j + 1] * (double)L.brd_flg[7 * i + j];
+        alfl = (J.rhorat - rs) * alfa0i + alfsum;
+        if (L.brd_flg[7 * i + m - 1] == 0) alfl = alfl + J.rhoslf[m] * (hwhmsi - alfa0i);
+    }
+    if (J.species_broad && L.brd_flg && L.brd_flg[7 * i + m - 1] == -1) alfl = hwhmsi;
+    return alfl;
+}
 
 // Line-coupling coefficients of the IFLG 1 / 3 line at i at the layer temperature.
 // The record after the line (IFLG = -1) holds them at 200, 250, 296 and 340 K
@@ -235,7 +293,7 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
             // Y, G (IFLAG 1) or GAMMA1, GAMMA2 (IFLAG 3); IFLAG 2 carries none
             LcPair lc{0., 0.};
             if (fl == 1 || fl == 3) lc = lc_coeffs(L, i, J.lcq_lo, J.lcq_anchor, J.lcq_rect, J.lcq_dif);
-            vnu = vnu + J.rhorat * (double)L.shift[i];
+            vnu = shifted_vnu(J, L, i, vnu, read_pshift(L, i, m));
             double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
             double a_air = (double)L.a_air[i];
             if (m == 7) {
@@ -244,12 +302,16 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
             }
             double alfa0i = a_air * tmpcor;
             double hwhmsi = (double)L.a_self[i] * tmpcor;
-            double alfl = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+            double alfl = lorentz_width(J, L, i, m, alfa0i, hwhmsi);
             if (fl == 3) alfl = alfl * (1.0 - lc.a * J.pavp0 - lc.b * J.pavp2);   // reduced width
             double alfad = vnu * J.alfd1[m][iso];
             zeta = alfl / (alfl + alfad);
+            // a Lorentz width <= 0 leaves zeta outside [0, 1], where the line shape is
+            // undefined (the reference indexes past its tables): reported, and the
+            // index kept inside the tables
+            if (!(alfl > 0.)) atomicOr(err, 32);
             double fzeta = 100. * zeta;
-            iz = ftrunc(fzeta + ONEPL);
+            iz = zeta_index(fzeta);
             double zetdif = fzeta - (double)(iz - 1);
             double alfv = avrat_lerp(c_avrat, iz, zetdif) * (alfl + alfad);
             if (alfv < J.dv) alfv = J.dv;
@@ -662,7 +724,8 @@ __global__ void __launch_bounds__(256) k_f4acc(const Job *jobs, LinesDev L, F4Pr
         const int64_t i = J.f4_lo + k;
         int m, iso;
         on = f4_accept(J, L, i, &m, &iso);
-        if (on && L.nu[i] + J.rhorat * (double)L.shift[i] > J.v2l4) atomicMin(&f.stop[blockIdx.y], (int)k);
+        if (on && shifted_vnu(J, L, i, L.nu[i], read_pshift(L, i, m)) > J.v2l4)
+            atomicMin(&f.stop[blockIdx.y], (int)k);
         f.acc[J.f4_off + k] = on ? 1 : 0;
     }
     const unsigned long long bal = __ballot(on);
@@ -737,7 +800,7 @@ __global__ void __launch_bounds__(256) k_recpos(const Job *jobs, F4Prep f, Comp 
 
 // LINF4Q per-line physics (clblm_ODLAY_LineF4.f90:951-1170) for the accepted lines of
 // one TAPE3 record up to the stop line, written at their positions.
-__global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4Prep f, Comp c) {
+__global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4Prep f, Comp c, int *err) {
     const Job &J = jobs[blockIdx.y];
     const int64_t r = blockIdx.x;
     if (r >= J.nrec) return;
@@ -760,7 +823,7 @@ __global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4P
     int m, iso;
     (void)f4_accept(J, L, i, &m, &iso);
     double sui = (double)L.s[i] * J.w[m];
-    const double vnu = L.nu[i] + J.rhorat * (double)L.shift[i];
+    const double vnu = shifted_vnu(J, L, i, L.nu[i], read_pshift(L, i, m));
     double alfa = (double)L.a_air[i];
     if (m == 7) {
         const double rvmr = 0.21;
@@ -776,7 +839,7 @@ __global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4P
     const double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
     const double alfa0i = alfa * tmpcor;
     const double hwhmsi = (double)L.a_self[i] * tmpcor;
-    alfa = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+    alfa = lorentz_width(J, L, i, m, alfa0i, hwhmsi);
     if (fl == 3) alfa = alfa * (1.0 - lc.a * J.pavp0 - lc.b * J.pavp2);
     // SP = SUI (1 + G P^2), SPP = SUI Y P (:1130-1131); Y = G = 0 unless IFLAG 1
     const double sp = fl == 1 ? sui * (1. + lc.b * J.pavp2) : sui;
@@ -787,6 +850,7 @@ __global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4P
     c.sp2[o] = spp;
     c.al[o] = alfa;
     c.ad[o] = vnu * J.alfd1[m][iso];
+    if (!(alfa > 0.)) atomicOr(err, 32);
     c.fl[o] = (uint8_t)((m == 2 ? CF_CO2 : 0) | (live ? CF_LIVE : 0) | (live && spp != 0. ? CF_CPL : 0));
 }
 
@@ -1195,7 +1259,7 @@ __device__ inline void f4_line_setup(F4Line &o, const Job &J, const R4Blk &B, do
     if (sp == 0. && spp == 0.) return;
     double alfadi = epp, alfali = alfa;
     double zetai = alfali / (alfali + alfadi);
-    int iz = (int)(100. * zetai + ONEPL);
+    int iz = zeta_index(100. * zetai);
     double zetdif = 100. * zetai - (double)(iz - 1);
     double alfavi = avrat_lerp(c_avrat, iz, zetdif) * (alfali + alfadi);
     double ralfvi = 1. / alfavi;
@@ -2233,6 +2297,7 @@ struct LineListH {
     int64_t n = 0;
     std::vector<double> nu;
     void *mem = nullptr;
+    void *brd_mem = nullptr;   // species-broadening fields, if attached
     LinesDev dev{};
 };
 
@@ -2320,9 +2385,9 @@ struct clblm_ctx {
 
 // The status code and message for a nonzero device error word.
 static int device_error(int herr, std::string &msg) {
-    if (herr & 1) {
-        msg = "line coupling records (IFLG 1..3) are not supported";
-        return CLBLM_ERR_UNSUPPORTED;
+    if (herr & 32) {
+        msg = "a line's Lorentz width is not positive (its line shape is undefined)";
+        return CLBLM_ERR_ARG;
     }
     if (herr & 16) {
         msg = "internal error: a SHRINQ chunk exceeds the LINF4Q buffer";
@@ -2607,6 +2672,7 @@ static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od
     }
     // continuum on the OPDPTH ABSRB grid (clblm_ODLAY.f90:881-896)
     J.contnm = par.contnm ? 1 : 0;
+    J.species_broad = (par.flags & CLBLM_OD_SPECIES_BROAD) ? 1 : 0;
     if (J.contnm) {
         double v1abs = (double)(int)V1;
         if (V1 < 0.) v1abs = v1abs - 1.;
@@ -2893,7 +2959,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     if (B.max_f4 > 0) {
         hipLaunchKernelGGL(k_f4acc, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, d_err);
         hipLaunchKernelGGL(k_recpos, dim3(nlay), dim3(256), 0, s4, d_jobs, f, cmp);
-        hipLaunchKernelGGL(k_f4prep, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, cmp);
+        hipLaunchKernelGGL(k_f4prep, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, cmp, d_err);
         hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, s4, d_jobs, cmp, f.G);
         hipLaunchKernelGGL(k_sim, dim3(nlay), dim3(256), 0, s4, d_jobs, f, cmp, (const int *)f.G);
         hipLaunchKernelGGL(k_emit, dim3((unsigned)std::max<int64_t>(1, B.tot_rec)), dim3(64), 0, s4, d_jobs, f, cmp,
@@ -3072,7 +3138,11 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     for (auto *l : c->lists)
-        if (l) { (void)hipFree(l->mem); delete l; }
+        if (l) {
+            (void)hipFree(l->mem);
+            if (l->brd_mem) (void)hipFree(l->brd_mem);
+            delete l;
+        }
     if (c->arena.p) (void)hipFree(c->arena.p);
     (void)hipFree(c->tab);
     (void)hipFree(c->tabs);
@@ -3170,10 +3240,41 @@ int clblm_linelist_upload(clblm_ctx *c, int64_t n, const double *nu, const float
     return 0;
 }
 
+int clblm_linelist_broadening(clblm_ctx *c, int handle, const int32_t *flg, const float *hw, const float *tmp,
+                              const float *shft) {
+    if (!c || !flg || !hw || !tmp || !shft) return set_err(CLBLM_ERR_ARG, "null argument");
+    if (handle < 0 || handle >= (int)c->lists.size() || !c->lists[handle])
+        return set_err(CLBLM_ERR_ARG, "bad line list handle");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    drain(c, true);   // batches in flight may read the fields being replaced
+    LineListH *L = c->lists[handle];
+    const size_t n7 = (size_t)L->n * 7;
+    if (L->brd_mem) {
+        HIPCHK(hipDeviceSynchronize());
+        (void)hipFree(L->brd_mem);
+        L->brd_mem = nullptr;
+    }
+    Carve cv;
+    const size_t of = cv.take<int32_t>(n7), oh = cv.take<float>(n7), ot = cv.take<float>(n7), os = cv.take<float>(n7);
+    HIPCHK(hipMalloc(&L->brd_mem, cv.off));
+    char *b = (char *)L->brd_mem;
+    HIPCHK(hipMemcpy(b + of, flg, 4 * n7, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + oh, hw, 4 * n7, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + ot, tmp, 4 * n7, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + os, shft, 4 * n7, hipMemcpyHostToDevice));
+    L->dev.brd_flg = at<int32_t>(b, of);
+    L->dev.brd_hw = at<float>(b, oh);
+    L->dev.brd_tmp = at<float>(b, ot);
+    L->dev.brd_shft = at<float>(b, os);
+    return 0;
+}
+
 int clblm_linelist_free(clblm_ctx *c, int handle) {
     if (!c || handle < 0 || handle >= (int)c->lists.size() || !c->lists[handle])
         return set_err(CLBLM_ERR_ARG, "bad line list handle");
     (void)hipFree(c->lists[handle]->mem);
+    if (c->lists[handle]->brd_mem) (void)hipFree(c->lists[handle]->brd_mem);
     delete c->lists[handle];
     c->lists[handle] = nullptr;
     return 0;

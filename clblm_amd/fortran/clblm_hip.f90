@@ -24,7 +24,7 @@ module clblm_hip
       real(c_double) :: v1, v2, dvnormal, dvnarrow, sample, dvout
       integer(c_int32_t) :: precision, contnm
       type(c_ptr) :: continuum
-      integer(c_int32_t) :: nptabs, reserved
+      integer(c_int32_t) :: nptabs, flags
    end type
 
    interface
@@ -129,7 +129,7 @@ contains
       end do
       par%v1 = v1; par%v2 = v2; par%dvnormal = dvnormal; par%dvnarrow = dvnarrow; par%sample = sample
       par%dvout = dvout; par%precision = precision; par%contnm = 0; par%continuum = c_null_ptr
-      par%nptabs = 0; par%reserved = 0
+      par%nptabs = 0; par%flags = 0
       if (present(continuum)) then
          par%contnm = 1
          par%continuum = c_loc(continuum)

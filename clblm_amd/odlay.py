@@ -81,6 +81,11 @@ class Context:
         h = ctypes.c_int()
         check(self.lib.clblm_linelist_upload(self.ptr, int(lines.n), *[a.ctypes.data for a in arrs],
                                              ctypes.byref(h)))
+        if getattr(lines, "brd_flg", None) is not None:
+            b = [np.ascontiguousarray(getattr(lines, f), dt).reshape(lines.n, 7)
+                 for f, dt in (("brd_flg", np.int32), ("brd_hw", np.float32), ("brd_tmp", np.float32),
+                               ("brd_shft", np.float32))]
+            check(self.lib.clblm_linelist_broadening(self.ptr, h.value, *[a.ctypes.data for a in b]))
         return h.value
 
     def free_lines(self, handle):
@@ -88,8 +93,10 @@ class Context:
 
     # -- one layer (host buffers) ----------------------------------------------
     @staticmethod
-    def params(v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, precision=FP64, continuum=None):
+    def params(v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, precision=FP64, continuum=None,
+               species_broad=False):
         p = ODParams()
+        p.flags = _lib.OD_SPECIES_BROAD if species_broad else 0
         p.v1, p.v2, p.dvnormal = float(v1), float(v2), float(dvnormal)
         p.dvnarrow = float(dvnormal if dvnarrow is None else dvnarrow)
         p.sample, p.dvout, p.precision = float(sample), float(dvout), int(precision)

@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List
+from typing import List, Optional
 
 import numpy as np
 
@@ -45,6 +45,11 @@ class LineList:
     shift: np.ndarray       # float32
     mol: np.ndarray         # int32, m + 100*iso
     iflg: np.ndarray        # int32
+    # species broadening (BRD_MOL_FLG / BRD_MOL_DAT), (n, 7) each, or None
+    brd_flg: Optional[np.ndarray] = None    # int32
+    brd_hw: Optional[np.ndarray] = None     # float32
+    brd_tmp: Optional[np.ndarray] = None    # float32
+    brd_shft: Optional[np.ndarray] = None   # float32
 
     @property
     def n(self) -> int:
@@ -67,7 +72,12 @@ def couple_lines(lines: LineList, coupled, iflag, a_nodes, b_nodes) -> LineList:
     A = np.broadcast_to(np.asarray(a_nodes, np.float64), (len(coupled), 4))
     Bn = np.broadcast_to(np.asarray(b_nodes, np.float64), (len(coupled), 4))
     fields = ("nu", "s", "a_air", "a_self", "e_lo", "tmpalf", "shift", "mol", "iflg")
+    brd = ("brd_flg", "brd_hw", "brd_tmp", "brd_shft")
+    if lines.brd_flg is not None:
+        fields = fields + brd
     out = {f: [] for f in fields}
+    zero7 = {"brd_flg": np.zeros(7, np.int32), "brd_hw": np.zeros(7, np.float32),
+             "brd_tmp": np.zeros(7, np.float32), "brd_shft": np.zeros(7, np.float32)}
 
     def put(vals):
         for f in fields:
@@ -85,14 +95,17 @@ def couple_lines(lines: LineList, coupled, iflag, a_nodes, b_nodes) -> LineList:
             row["iflg"] = np.int32(flags[k])
             put(row)
             a, b = A[k], Bn[k]
-            put({"nu": np.float64(a[0]), "a_air": np.float32(a[1]),
-                 "mol": np.float32(a[2]).view(np.int32), "tmpalf": np.float32(a[3]),
-                 "s": np.float32(b[0]), "e_lo": np.float32(b[1]), "a_self": np.float32(b[2]),
-                 "shift": np.float32(b[3]), "iflg": np.int32(-1)})
+            rec = {"nu": np.float64(a[0]), "a_air": np.float32(a[1]),
+                   "mol": np.float32(a[2]).view(np.int32), "tmpalf": np.float32(a[3]),
+                   "s": np.float32(b[0]), "e_lo": np.float32(b[1]), "a_self": np.float32(b[2]),
+                   "shift": np.float32(b[3]), "iflg": np.int32(-1)}
+            rec.update({f: zero7[f] for f in fields if f in zero7})
+            put(rec)
         else:
             put(row)
     dt = dict(nu=np.float64, s=np.float32, a_air=np.float32, a_self=np.float32, e_lo=np.float32,
-              tmpalf=np.float32, shift=np.float32, mol=np.int32, iflg=np.int32)
+              tmpalf=np.float32, shift=np.float32, mol=np.int32, iflg=np.int32, brd_flg=np.int32,
+              brd_hw=np.float32, brd_tmp=np.float32, brd_shft=np.float32)
     return LineList(**{f: np.asarray(out[f], dtype=dt[f]) for f in fields})
 
 

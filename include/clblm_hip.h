@@ -53,8 +53,12 @@ typedef struct {
     int32_t contnm;              /* nonzero: continuum[] supplied on the 1 cm-1 ABSRB grid */
     const double *continuum;     /* host pointer, nptabs values from V1ABS = INT(V1)-3 */
     int32_t nptabs;
-    int32_t reserved;
+    int32_t flags;               /* CLBLM_OD_SPECIES_BROAD: odCtrl%speciesBroad */
 } clblm_od_params;
+/* Species-by-species broadening and shifts (clblm_Config.f90:89, used by LNCORQ
+ * clblm_ODLAY.f90:1853-1883 and LINF4Q clblm_ODLAY_LineF4.f90:996-1054) from the data
+ * attached by clblm_linelist_broadening. */
+#define CLBLM_OD_SPECIES_BROAD 1
 
 /* ---- context ---------------------------------------------------------------- */
 int clblm_ctx_create(int device, clblm_ctx **out);
@@ -81,6 +85,15 @@ int clblm_linelist_upload(clblm_ctx *ctx, int64_t n, const double *nu, const flo
                           const float *tmpalf, const float *shift, const int32_t *mol,
                           const int32_t *iflg, int *handle);
 int clblm_linelist_free(clblm_ctx *ctx, int handle);
+/* The species-broadening fields of the TAPE3 records (BRD_MOL_FLG, and BRD_MOL_DAT
+ * split into widths, temperature exponents and shifts: clblm_LineData.f90:67-69,
+ * RDLIN clblm_ODLAY.f90:2697-2706) for the n records of a list already uploaded:
+ * arrays of n x 7 values, record-major (entry 7 i + j - 1 for broadener j = 1..7).
+ * RDLIN's read-time correction of an O2 line's shift by its own species data
+ * (:2730-2738) applies from then on; the broadening itself applies to batches whose
+ * parameters set CLBLM_OD_SPECIES_BROAD. */
+int clblm_linelist_broadening(clblm_ctx *ctx, int handle, const int32_t *flg, const float *hw,
+                              const float *tmp, const float *shft);
 
 /* ---- one layer, host buffers -------------------------------------------------
  * Drop-in for ODLAY_nonStruct (clblm_ODLAY.f90:129-611).  Two-phase sizing:

@@ -611,6 +611,55 @@ typedef struct {
     double scor[48 * 11], rhoslf[48], alfd1[48 * 11];
 } molec_t;
 
+/* PSHIFT as RDLIN / RDLNFL read it (clblm_ODLAY.f90:2730-2738,
+ * clblm_ODLAY_LineF4.f90:1463-1468): an O2 line with its own species data. */
+static double read_pshift(const orc_lines *Ls, int64_t i) {
+    double pshift = Ls->shift[i];
+    int m = Ls->mol[i] % 100;
+    if (m == 7 && Ls->iflg[i] >= 0 && Ls->brd_flg && Ls->brd_flg[7 * i + 7 - 1] > 0) {
+        double rvmr = 0.21;
+        pshift = (pshift - rvmr * (double)Ls->brd_shft[7 * i + 7 - 1]) / (1.0 - rvmr);
+    }
+    return pshift;
+}
+
+/* sum(brd_mol_flg(:,i)) > 0 .and. speciesBroad */
+static int brd_active(const orc_lines *Ls, int64_t i, int species_broad) {
+    if (!species_broad || !Ls->brd_flg) return 0;
+    int t = 0;
+    for (int j = 1; j <= 7; j++) t += Ls->brd_flg[7 * i + j - 1];
+    return t > 0;
+}
+
+/* vnu(i) + sum(rhoslf(1:7) * brd_mol_flg(:,i) * (brd_mol_shft(:,i) - pshift(i)))
+ * (clblm_ODLAY.f90:1853-1856, clblm_ODLAY_LineF4.f90:996-999) */
+static double brd_shift(const orc_lines *Ls, int64_t i, const molec_t *M, double vnu, double pshift) {
+    double s = 0.;
+    for (int j = 1; j <= 7; j++)
+        s = s + (M->rhoslf[j] * (double)Ls->brd_flg[7 * i + j - 1]) * ((double)Ls->brd_shft[7 * i + j - 1] - pshift);
+    return vnu + s;
+}
+
+/* ALFL with species broadening, clblm_ODLAY.f90:1866-1883 and
+ * clblm_ODLAY_LineF4.f90:1038-1054. */
+static double brd_width(const orc_lines *Ls, int64_t i, const molec_t *M, int m, double RHORAT, double TRATIO,
+                        double alfa0i, double hwhmsi, double alfl, int species_broad) {
+    if (brd_active(Ls, i, species_broad)) {
+        double tmpcor_arr[8], alfa_tmp[8];
+        for (int j = 1; j <= 7; j++) {
+            tmpcor_arr[j] = pow(TRATIO, (double)Ls->brd_tmp[7 * i + j - 1]);
+            alfa_tmp[j] = (double)Ls->brd_hw[7 * i + j - 1] * tmpcor_arr[j];
+        }
+        double alfsum = 0., rsum = 0.;
+        for (int j = 1; j <= 7; j++) alfsum = alfsum + (M->rhoslf[j] * (double)Ls->brd_flg[7 * i + j - 1]) * alfa_tmp[j];
+        for (int j = 1; j <= 7; j++) rsum = rsum + M->rhoslf[j] * (double)Ls->brd_flg[7 * i + j - 1];
+        alfl = (RHORAT - rsum) * alfa0i + alfsum;
+        if (Ls->brd_flg[7 * i + m - 1] == 0) alfl = alfl + M->rhoslf[m] * (hwhmsi - alfa0i);
+    }
+    if (species_broad && Ls->brd_flg && m <= 7 && Ls->brd_flg[7 * i + m - 1] == -1) alfl = hwhmsi;
+    return alfl;
+}
+
 static int sel_ok(const orc_layer *L, int m, int iso) {
     if (m < 1 || m > MXMOL || !L->sel[m]) return 0;
     if (m > CLBLM_NMOL_DATA) return 0;
@@ -637,7 +686,7 @@ static double read_alfa(const orc_lines *Ls, int64_t i) {
 /* LINF4Q + RDLNFL + SHRINQ, clblm_ODLAY_LineF4.f90:647-1263,1355-1498,1516-1647.
  * Records of 250 lines; the shrink buffer holds LIMIN=1000 lines (+250 slack). */
 static int linf4q(const orc_lines *Ls, const orc_layer *L, const molec_t *M,
-                  double v1l4, double v2l4, double dvr4, shrunk_t *out) {
+                  double v1l4, double v2l4, double dvr4, int species_broad, shrunk_t *out) {
     const int LIMIN = 1000;
     double PAVP0 = L->pave / P0;
     double PAVP2 = PAVP0 * PAVP0;
@@ -675,7 +724,9 @@ static int linf4q(const orc_lines *Ls, const orc_layer *L, const molec_t *M,
                 if (IFLAG == 1) lc_linf4q(Ls, i, L->tave, &YI, &GI);
                 if (IFLAG == 3) lc_linf4q(Ls, i, L->tave, &GAMMA1, &GAMMA2);
                 sline *S = &B[IJ];
-                S->vnu = Ls->nu[i] + RHORAT * (double)Ls->shift[i];
+                double pshift = read_pshift(Ls, i);
+                S->vnu = Ls->nu[i] + RHORAT * pshift;
+                if (brd_active(Ls, i, species_broad)) S->vnu = brd_shift(Ls, i, M, S->vnu, pshift);
                 S->alfa = read_alfa(Ls, i);
                 S->epp = Ls->e_lo[i];
                 S->mol = m;
@@ -687,6 +738,7 @@ static int linf4q(const orc_lines *Ls, const orc_layer *L, const molec_t *M,
                 double alfa0i = S->alfa * tmpcor;
                 double hwhmsi = (double)Ls->a_self[i] * tmpcor;
                 S->alfa = alfa0i * (RHORAT - M->rhoslf[m]) + hwhmsi * M->rhoslf[m];
+                S->alfa = brd_width(Ls, i, M, m, RHORAT, TRATIO, alfa0i, hwhmsi, S->alfa, species_broad);
                 if (IFLAG == 3) S->alfa = S->alfa * (1.0 - GAMMA1 * PAVP0 - GAMMA2 * PAVP2);
                 S->epp = S->vnu * M->alfd1[m * 11 + iso];
                 S->sp = sui * (1. + GI * PAVP2);
@@ -871,7 +923,7 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
     if (orc_molec(L, M.scor, M.rhoslf, M.alfd1)) return -1;
 
     shrunk_t sh = {0};
-    if (linf4q(Ls, L, &M, V1L4, V2L4, DVR4, &sh)) { shr_free(&sh); return -1; }
+    if (linf4q(Ls, L, &M, V1L4, V2L4, DVR4, P->species_broad, &sh)) { shr_free(&sh); return -1; }
 
     /* HIRACQ clblm_ODLAY.f90:1002-1562 */
     static double F1[NFMX + 1], F2[NFMX + 1], F3[NFMX + 1], FG[NFMX + 1], XVER[NFMX + 1];
@@ -981,11 +1033,14 @@ int orc_odlay(const orc_lines *Ls, const orc_layer *L, orc_params *P,
                 double YI = 0., GI = 0., GAMMA1 = 0., GAMMA2 = 0.;
                 if (IFLAG == 1) lc_lncorq(Ls, i, TAVE, &YI, &GI);
                 if (IFLAG == 3) lc_lncorq(Ls, i, TAVE, &GAMMA1, &GAMMA2);
-                h->vnu = h->vnu + RHORAT * (double)Ls->shift[i];
+                double pshift = read_pshift(Ls, i);
+                h->vnu = h->vnu + RHORAT * pshift;
+                if (brd_active(Ls, i, P->species_broad)) h->vnu = brd_shift(Ls, i, &M, h->vnu, pshift);
                 double tmpcor = pow(TRATIO, (double)Ls->tmpalf[i]);
                 double alfa0i = read_alfa(Ls, i) * tmpcor;
                 double hwhmsi = (double)Ls->a_self[i] * tmpcor;
                 double alfl = alfa0i * (RHORAT - M.rhoslf[m]) + hwhmsi * M.rhoslf[m];
+                alfl = brd_width(Ls, i, &M, m, RHORAT, TRATIO, alfa0i, hwhmsi, alfl, P->species_broad);
                 if (IFLAG == 3) alfl = alfl * (1.0 - GAMMA1 * PAVP0 - GAMMA2 * PAVP2);
                 double alfad = h->vnu * M.alfd1[m * 11 + iso];
                 double zeta = alfl / (alfl + alfad);

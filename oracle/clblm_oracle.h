@@ -31,6 +31,10 @@ typedef struct {
     const float *shift;    /* pressure shift, cm-1/atm */
     const int32_t *iflg;   /* 0 plain line; 1 / 3 coupled (coefficients in the next
                               record); 2 plain; < 0 not a line (coefficients, skipped) */
+    /* species broadening BRD_MOL_FLG and BRD_MOL_DAT (widths, exponents, shifts) of
+     * broadener j = 1..7 at 7 i + j - 1; NULL when the list has none */
+    const int32_t *brd_flg;
+    const float *brd_hw, *brd_tmp, *brd_shft;
 } orc_lines;
 
 typedef struct {
@@ -47,6 +51,7 @@ typedef struct {
     int contnm;              /* nonzero: absrb[] holds the continuum on the OPDPTH 1 cm-1 grid */
     const double *absrb;
     int nptabs;
+    int species_broad;       /* odCtrl%speciesBroad */
 } orc_params;
 
 /* Builds the line-shape tables (SHAPEL/SHAPEG/VERFN).  Arrays are 1-based

@@ -10,7 +10,8 @@ _lib = None
 
 class _Lines(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64)] + [(k, ctypes.c_void_p) for k in
-                ("nu", "s", "a_air", "e_lo", "mol", "a_self", "tmpalf", "shift", "iflg")]
+                ("nu", "s", "a_air", "e_lo", "mol", "a_self", "tmpalf", "shift", "iflg",
+                 "brd_flg", "brd_hw", "brd_tmp", "brd_shft")]
 
 
 class _Layer(ctypes.Structure):
@@ -22,7 +23,7 @@ class _Params(ctypes.Structure):
     _fields_ = [("v1", ctypes.c_double), ("v2", ctypes.c_double), ("dvnormal", ctypes.c_double),
                 ("dvnarrow", ctypes.c_double), ("sample", ctypes.c_double), ("dvout", ctypes.c_double),
                 ("jrad", ctypes.c_int), ("contnm", ctypes.c_int), ("absrb", ctypes.c_void_p),
-                ("nptabs", ctypes.c_int)]
+                ("nptabs", ctypes.c_int), ("species_broad", ctypes.c_int)]
 
 
 def lib():
@@ -89,10 +90,17 @@ class Oracle:
         arrs = [lines.nu, lines.s, lines.a_air, lines.e_lo, lines.mol, lines.a_self, lines.tmpalf,
                 lines.shift, lines.iflg]
         self._keep = [np.ascontiguousarray(a) for a in arrs]
-        self._L = _Lines(lines.n, *[a.ctypes.data for a in self._keep])
+        brd = [None] * 4
+        if getattr(lines, "brd_flg", None) is not None:
+            self._brd = [np.ascontiguousarray(getattr(lines, f), dt).reshape(lines.n, 7)
+                         for f, dt in (("brd_flg", np.int32), ("brd_hw", np.float32), ("brd_tmp", np.float32),
+                                       ("brd_shft", np.float32))]
+            brd = [a.ctypes.data for a in self._brd]
+        self._L = _Lines(lines.n, *[a.ctypes.data for a in self._keep], *brd)
         self.jrad = 1
 
-    def odlay(self, layer, v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, continuum=None):
+    def odlay(self, layer, v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, continuum=None,
+              species_broad=False):
         """Returns (od, dv_used).  JRAD persists across calls like the reference's SAVE."""
         lay = _Layer()
         lay.pave, lay.tave, lay.wtot = layer.pave, layer.tave, layer.wtot
@@ -106,7 +114,7 @@ class Oracle:
             npt = absrb.size
         par = _Params(v1, v2, dvnormal, dvnormal if dvnarrow is None else dvnarrow, sample, dvout,
                       self.jrad, 1 if absrb is not None else 0,
-                      absrb.ctypes.data if absrb is not None else None, npt)
+                      absrb.ctypes.data if absrb is not None else None, npt, 1 if species_broad else 0)
         ns, dvu = ctypes.c_int64(), ctypes.c_double()
         L = lib()
         if L.orc_odlay(ctypes.byref(self._L), ctypes.byref(lay), ctypes.byref(par), None,

@@ -91,3 +91,93 @@ def test_upload_checks_coefficient_records(ctx):
     assert ok.iflg[249] == -1 and ok.iflg[250] == 1 and ok.iflg[251] == -1
     h = ctx.upload_lines(ok)
     ctx.free_lines(h)
+
+
+def broadened_list(seed=12):
+    """Every line with species data: each of the 7 broadeners flagged with
+    probability 1/2 (the line's own molecule -1 for one line in 17), widths,
+    exponents and shifts drawn per broadener; every fifth line also coupled."""
+    lines = W.make_lines(20000, 560.0, 740.0, seed=seed, mols=(1, 2, 3, 4, 5, 6, 7))
+    rng = np.random.default_rng(seed)
+    n = lines.n
+    # O2's air width is rescaled to a foreign width on read, (a_air - 0.21 a_self) / 0.79;
+    # with self widths 5x air it turns negative, and the species-weighted width of
+    # such a line can reach zero, where the line shape is undefined.  Give O2 lines
+    # their air width as self width, as in the line lists.
+    o2 = lines.mol % 100 == 7
+    lines.a_self[o2] = lines.a_air[o2]
+    flg = (rng.uniform(size=(n, 7)) < 0.5).astype(np.int32)
+    own = (lines.mol % 100) - 1
+    minus = np.arange(n) % 17 == 3
+    flg[np.arange(n)[minus], own[minus]] = -1
+    lines.brd_flg = flg
+    lines.brd_hw = rng.uniform(0.02, 0.12, (n, 7)).astype(np.float32)
+    lines.brd_tmp = rng.uniform(0.4, 0.9, (n, 7)).astype(np.float32)
+    lines.brd_shft = rng.uniform(-0.01, 0.01, (n, 7)).astype(np.float32)
+    idx = np.arange(2, n, 5)
+    return W.couple_lines(lines, idx, 1, rng.uniform(-0.05, 0.05, (idx.size, 4)),
+                          rng.uniform(-0.01, 0.01, (idx.size, 4)))
+
+
+@pytest.mark.parametrize("species_broad", [True, False])
+def test_species_broadening_matches_oracle(ctx, species_broad):
+    import torch
+    from clblm_amd.odlay import Context
+    from oracle.py import Oracle
+    lines = broadened_list()
+    h = ctx.upload_lines(lines)
+    layers = _layers()
+    dvs = [W.calculate_dv(650.0, l.pave, l.tave, l.w[1] / l.wtot) for l in layers]
+    res = {}
+    for prec in (0, 1):
+        ctx.jrad = 1
+        pars = [Context.params(600.0, 700.0, dv, precision=prec, species_broad=species_broad) for dv in dvs]
+        out, off = ctx.od_batch(h, layers, pars)
+        torch.cuda.synchronize()
+        ctx.sync()
+        res[prec] = (out.cpu().numpy(), off)
+    od64, off = res[0]
+    od32, _ = res[1]
+    orc = Oracle(lines)
+    orc.jrad = 1
+    worst, worst32 = 0.0, 0.0
+    for k, (lay, dv) in enumerate(zip(layers, dvs)):
+        ref, _ = orc.odlay(lay, 600.0, 700.0, dv, species_broad=species_broad)
+        got = od64[off[k]:off[k + 1]]
+        assert ref.size == got.size
+        fl = 1e-4 * np.abs(ref).max()
+        worst = max(worst, float((np.abs(got - ref) / np.maximum(np.abs(ref), fl)).max()))
+        g32 = od32[off[k]:off[k + 1]]
+        worst32 = max(worst32, float((np.abs(g32 - got) / np.maximum(np.abs(got), fl)).max()))
+    print("species broadening %s: FP64 vs oracle %.3e, FP32C vs FP64 %.3e" % (species_broad, worst, worst32))
+    assert worst <= 1e-10
+    assert worst32 <= 1e-5
+    ctx.free_lines(h)
+
+
+def test_nonpositive_width_is_reported(ctx):
+    """An O2 line whose foreign width reads negative ((a_air - 0.21 a_self) / 0.79 with
+    a_self = 5 a_air) and whose species data give it no other width has a Lorentz
+    width below zero: the batch reports it instead of reading past the tables."""
+    import torch
+    from clblm_amd._lib import ClblmError
+    from clblm_amd.odlay import Context
+    lines = W.make_lines(600, 600.0, 700.0, seed=9, mols=(1, 7))
+    n = lines.n
+    lines.brd_flg = np.zeros((n, 7), np.int32)
+    o2 = np.where(lines.mol % 100 == 7)[0]
+    lines.brd_flg[o2, 6] = 1                      # broadened by O2 only, with width 0
+    lines.brd_hw = np.zeros((n, 7), np.float32)
+    lines.brd_tmp = np.full((n, 7), 0.7, np.float32)
+    lines.brd_shft = np.zeros((n, 7), np.float32)
+    h = ctx.upload_lines(lines)
+    lay = W.Layer(pave=500.0, tave=250.0, wtot=1e24, w={1: 1e22, 7: 2.1e23})
+    ctx.jrad = 1
+    ctx.od_batch(h, [lay], [Context.params(640.0, 660.0, 0.002, species_broad=True)])
+    torch.cuda.synchronize()
+    with pytest.raises(ClblmError, match="Lorentz width"):
+        ctx.sync()
+    ctx.jrad = 1   # without species broadening the same lines are valid
+    ctx.od_batch(h, [lay], [Context.params(640.0, 660.0, 0.002, species_broad=False)])
+    ctx.sync()
+    ctx.free_lines(h)
