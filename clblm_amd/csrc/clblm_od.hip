@@ -41,6 +41,10 @@ static int set_err(int code, const std::string &m) {
     g_err = m;
     return code;
 }
+// set_err for the library's other translation units (not exported)
+extern "C" __attribute__((visibility("hidden"))) int clblm_set_error(int code, const char *msg) {
+    return set_err(code, std::string(msg));
+}
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
         hipError_t e_ = (x);                                                               \

@@ -88,6 +88,16 @@ class Context:
             check(self.lib.clblm_linelist_broadening(self.ptr, h.value, *[a.ctypes.data for a in b]))
         return h.value
 
+    def load_tape3(self, path, vlo=1.0, vhi=0.0) -> int:
+        """Reads a TAPE3 line file (clblm_linelist_load_tape3) and uploads its panels
+        that overlap [vlo, vhi] (every panel when vlo > vhi): give the range of the
+        runs' V1 - 25 - 3 DVR4 to V2 + 25 + 3 DVR4, the reach of LINF4Q
+        (clblm_ODLAY.f90:913-924).  Returns the handle."""
+        h, n = ctypes.c_int(), ctypes.c_int64()
+        check(self.lib.clblm_linelist_load_tape3(self.ptr, str(path).encode(), float(vlo), float(vhi),
+                                                 ctypes.byref(h), ctypes.byref(n)))
+        return h.value
+
     def free_lines(self, handle):
         check(self.lib.clblm_linelist_free(self.ptr, int(handle)))
 

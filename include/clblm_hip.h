@@ -85,6 +85,28 @@ int clblm_linelist_upload(clblm_ctx *ctx, int64_t n, const double *nu, const flo
                           const float *tmpalf, const float *shift, const int32_t *mol,
                           const int32_t *iflg, int *handle);
 int clblm_linelist_free(clblm_ctx *ctx, int handle);
+/* ---- TAPE3 line files ---------------------------------------------------------
+ * The binary line file LNFL writes, read as CLBLM_LineData_init
+ * (clblm_LineData.f90:99-256), RDLIN (clblm_ODLAY.f90:2571-2798) and RDLNFL
+ * (clblm_ODLAY_LineF4.f90:1355-1498) read it: the header record(s), then per panel
+ * its header (VMIN, VMAX, NREC, NWDS) and its 250-line block (clblm_LineData.f90:52-70),
+ * up to the end of the file or BUFIN_sgl's -99 end mark.  The panels overlapping
+ * [vlo, vhi] are read (every panel when vlo > vhi), and the first panel beyond vhi;
+ * each becomes a 250-entry record, padded with entries that are not lines (IFLG -1)
+ * when it holds fewer.  O2 widths and shifts are corrected on use, as RDLIN does.
+ *
+ * clblm_tape3_read: host arrays in clblm_linelist_upload's layout plus the
+ * species-broadening fields (n x 7 each).  Call with nu == NULL to get *n, then with
+ * arrays of *n records (*n x 7 for the broadening fields; any array may be NULL).
+ * clblm_linelist_load_tape3: reads and uploads in one call (the broadening fields too
+ * when any flag is set); *n (if non-NULL) receives the record count. */
+int clblm_tape3_read(const char *path, double vlo, double vhi, int64_t *n, double *nu, float *s,
+                     float *a_air, float *a_self, float *e_lo, float *tmpalf, float *shift,
+                     int32_t *mol, int32_t *iflg, int32_t *brd_flg, float *brd_hw, float *brd_tmp,
+                     float *brd_shft);
+int clblm_linelist_load_tape3(clblm_ctx *ctx, const char *path, double vlo, double vhi, int *handle,
+                              int64_t *n);
+
 /* The species-broadening fields of the TAPE3 records (BRD_MOL_FLG, and BRD_MOL_DAT
  * split into widths, temperature exponents and shifts: clblm_LineData.f90:67-69,
  * RDLIN clblm_ODLAY.f90:2697-2706) for the n records of a list already uploaded:

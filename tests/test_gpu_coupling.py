@@ -181,3 +181,50 @@ def test_nonpositive_width_is_reported(ctx):
     ctx.od_batch(h, [lay], [Context.params(640.0, 660.0, 0.002, species_broad=False)])
     ctx.sync()
     ctx.free_lines(h)
+
+
+def test_tape3_file_gives_the_same_od(ctx, tmp_path):
+    """A TAPE3 file (short panels, coupling, species data) loaded by the library's
+    reader gives the optical depth of the same records uploaded from memory (up to
+    the order of pass 1's atomics) and of the oracle."""
+    import torch
+    from clblm_amd.odlay import Context
+    from clblm_amd.tape3 import read_tape3, write_tape3
+    lines = broadened_list()
+    sizes, a = [], 0
+    while a < lines.n:
+        k = min(200 if (len(sizes) % 3) else 250, lines.n - a)
+        if a + k < lines.n and lines.iflg[a + k - 1] in (1, 3):
+            k -= 1
+        sizes.append(k)
+        a += k
+    p = tmp_path / "tape3"
+    write_tape3(p, lines, panel_sizes=sizes)
+    h_file = ctx.load_tape3(p, 560.0, 740.0)
+    h_mem = ctx.upload_lines(read_tape3(p, 560.0, 740.0))
+    layers = _layers()[:6]
+    dvs = [W.calculate_dv(650.0, l.pave, l.tave, l.w[1] / l.wtot) for l in layers]
+    pars = [Context.params(600.0, 700.0, dv, species_broad=True) for dv in dvs]
+    outs = []
+    for h in (h_file, h_mem):
+        ctx.jrad = 1
+        out, off = ctx.od_batch(h, layers, pars)
+        torch.cuda.synchronize()
+        ctx.sync()
+        outs.append(out.cpu().numpy())
+    # the same records and kernels: equal up to the order of pass 1's LDS atomics
+    d = np.abs(outs[0] - outs[1]) / np.maximum(np.abs(outs[1]), 1e-4 * np.abs(outs[1]).max())
+    assert float(d.max()) <= 1e-13
+    from oracle.py import Oracle
+    orc = Oracle(read_tape3(p, 560.0, 740.0))
+    orc.jrad = 1
+    worst = 0.0
+    for k, (lay, dv) in enumerate(zip(layers, dvs)):
+        ref, _ = orc.odlay(lay, 600.0, 700.0, dv, species_broad=True)
+        got = outs[0][off[k]:off[k + 1]]
+        fl = 1e-4 * np.abs(ref).max()
+        worst = max(worst, float((np.abs(got - ref) / np.maximum(np.abs(ref), fl)).max()))
+    print("TAPE3 file with short panels: FP64 vs oracle %.3e" % worst)
+    assert worst <= 1e-10
+    ctx.free_lines(h_file)
+    ctx.free_lines(h_mem)
