@@ -218,36 +218,50 @@ __device__ inline double read_pshift(const LinesDev &L, int64_t i, int m) {
 __device__ inline bool brd_sum(const Job &J, const LinesDev &L, int64_t i) {
     if (!J.species_broad || !L.brd_flg) return false;
     int t = 0;
+#pragma unroll 1
     for (int j = 0; j < 7; j++) t += L.brd_flg[7 * i + j];
     return t > 0;
 }
 // The line centre after the pressure shifts: VNU + RHORAT PSHIFT, plus the
 // species-weighted shift differences (LNCORQ :1849-1856, LINF4Q :994-999).
+__device__ inline double brd_shift_sum(const Job &J, const LinesDev &L, int64_t i, double pshift) {
+    double acc = 0.;
+#pragma unroll 1
+    for (int j = 0; j < 7; j++)
+        acc = acc + (J.rhoslf[j + 1] * (double)L.brd_flg[7 * i + j]) * ((double)L.brd_shft[7 * i + j] - pshift);
+    return acc;
+}
+template <bool BRD>
 __device__ inline double shifted_vnu(const Job &J, const LinesDev &L, int64_t i, double vnu, double pshift) {
     double v = vnu + J.rhorat * pshift;
-    if (brd_sum(J, L, i)) {
-        double acc = 0.;
-        for (int j = 0; j < 7; j++)
-            acc = acc + (J.rhoslf[j + 1] * (double)L.brd_flg[7 * i + j]) * ((double)L.brd_shft[7 * i + j] - pshift);
-        v = v + acc;
-    }
+    if (BRD && brd_sum(J, L, i)) v = v + brd_shift_sum(J, L, i, pshift);
     return v;
 }
 // The Lorentz width: air and self parts, or with species broadening the
 // species-weighted widths (LNCORQ :1861-1883, LINF4Q :1032-1054).
-__device__ inline double lorentz_width(const Job &J, const LinesDev &L, int64_t i, int m, double alfa0i,
-                                       double hwhmsi) {
-    double alfl = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
-    if (brd_sum(J, L, i)) {
+__device__ inline double brd_width(const Job &J, const LinesDev &L, int64_t i, int m, double alfa0i,
+                                         double hwhmsi) {
+    double alfl;
+    {
         double alfsum = 0., rs = 0.;
+#pragma unroll 1
         for (int j = 0; j < 7; j++) {
             const double at = (double)L.brd_hw[7 * i + j] * pow(J.tratio, (double)L.brd_tmp[7 * i + j]);
             alfsum = alfsum + (J.rhoslf[j + 1] * (double)L.brd_flg[7 * i + j]) * at;
         }
+#pragma unroll 1
         for (int j = 0; j < 7; j++) rs = rs + J.rhoslf[j + 1] * (double)L.brd_flg[7 * i + j];
         alfl = (J.rhorat - rs) * alfa0i + alfsum;
         if (L.brd_flg[7 * i + m - 1] == 0) alfl = alfl + J.rhoslf[m] * (hwhmsi - alfa0i);
     }
+    return alfl;
+}
+template <bool BRD>
+__device__ inline double lorentz_width(const Job &J, const LinesDev &L, int64_t i, int m, double alfa0i,
+                                       double hwhmsi) {
+    double alfl = alfa0i * (J.rhorat - J.rhoslf[m]) + hwhmsi * J.rhoslf[m];
+    if (!BRD) return alfl;
+    if (brd_sum(J, L, i)) alfl = brd_width(J, L, i, m, alfa0i, hwhmsi);
     if (J.species_broad && L.brd_flg && L.brd_flg[7 * i + m - 1] == -1) alfl = hwhmsi;
     return alfl;
 }
@@ -262,13 +276,20 @@ __device__ inline double lorentz_width(const Job &J, const LinesDev &L, int64_t 
 struct LcPair {
     double a, b;
 };
+__device__ inline double lc_node_a(const LinesDev &L, int64_t k, int t) {
+    return t == 0 ? L.nu_rec[k]
+                  : t == 1 ? (double)L.a_air[k] : t == 2 ? (double)__int_as_float(L.mol[k]) : (double)L.tmpalf[k];
+}
+__device__ inline double lc_node_b(const LinesDev &L, int64_t k, int t) {
+    return t == 0 ? (double)L.s[k] : t == 1 ? (double)L.e_lo[k] : t == 2 ? (double)L.a_self[k] : (double)L.shift[k];
+}
 __device__ inline LcPair lc_coeffs(const LinesDev &L, int64_t i, int lo, int anchor, double rect, double dif) {
     const int64_t k = i + 1;
-    const double A[4] = {L.nu_rec[k], (double)L.a_air[k], (double)__int_as_float(L.mol[k]), (double)L.tmpalf[k]};
-    const double B[4] = {(double)L.s[k], (double)L.e_lo[k], (double)L.a_self[k], (double)L.shift[k]};
+    const double a0 = lc_node_a(L, k, lo), a1 = lc_node_a(L, k, lo + 1);
+    const double b0 = lc_node_b(L, k, lo), b1 = lc_node_b(L, k, lo + 1);
     LcPair r;
-    r.a = A[anchor] + ((A[lo + 1] - A[lo]) * rect) * dif;
-    r.b = B[anchor] + ((B[lo + 1] - B[lo]) * rect) * dif;
+    r.a = (anchor == lo ? a0 : a1) + ((a1 - a0) * rect) * dif;
+    r.b = (anchor == lo ? b0 : b1) + ((b1 - b0) * rect) * dif;
     return r;
 }
 
@@ -277,6 +298,9 @@ struct LinePrep {
     int *iz, *pmin, *home;
 };
 
+// BRD: the batch uses species broadening (a separate instantiation, so that its
+// registers stay out of the common one)
+template <bool BRD>
 __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePrep lp, int *err) {
     const Job &J = jobs[blockIdx.y];
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,7 +321,7 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
             // Y, G (IFLAG 1) or GAMMA1, GAMMA2 (IFLAG 3); IFLAG 2 carries none
             LcPair lc{0., 0.};
             if (fl == 1 || fl == 3) lc = lc_coeffs(L, i, J.lcq_lo, J.lcq_anchor, J.lcq_rect, J.lcq_dif);
-            vnu = shifted_vnu(J, L, i, vnu, read_pshift(L, i, m));
+            vnu = shifted_vnu<BRD>(J, L, i, vnu, read_pshift(L, i, m));
             double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
             double a_air = (double)L.a_air[i];
             if (m == 7) {
@@ -306,7 +330,7 @@ __global__ void k_prep(const Job *jobs, const Panel *panels, LinesDev L, LinePre
             }
             double alfa0i = a_air * tmpcor;
             double hwhmsi = (double)L.a_self[i] * tmpcor;
-            double alfl = lorentz_width(J, L, i, m, alfa0i, hwhmsi);
+            double alfl = lorentz_width<BRD>(J, L, i, m, alfa0i, hwhmsi);
             if (fl == 3) alfl = alfl * (1.0 - lc.a * J.pavp0 - lc.b * J.pavp2);   // reduced width
             double alfad = vnu * J.alfd1[m][iso];
             zeta = alfl / (alfl + alfad);
@@ -455,7 +479,9 @@ __global__ void k_ranges(const Job *jobs, Panel *panels, int64_t npanel_tot, Lin
 // ------------------------------------------------------------------------------------
 // CONV_U: R1 points per step of the pass-1 walk (a launch-shape setting, 2, 4 or 8)
 
-template <int CONV_U>
+// CPL: the line list holds coupled lines (IFLG 1); the coupling walk is compiled
+// only into that instantiation, so that the common one keeps its occupancy
+template <int CONV_U, bool CPL>
 __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *panels, LinePrep lp,
                                                 Tables T, double *A, unsigned long long *work) {
     extern __shared__ double sm[];
@@ -534,8 +560,8 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
                 atomicAdd(&a1[j1], s1 * g.x + sd * g.y + sv * g.z);
             }
         }
-        const double dptrat = lp.sppsp[o];
-        if (dptrat != 0.) {
+        const double dptrat = CPL ? lp.sppsp[o] : 0.;
+        if (CPL && dptrat != 0.) {
             // line coupling (clblm_ODLAY.f90:2195-2235): the same walk again, each
             // term scaled by CLC DPTRAT and by its table argument ZF
             const double c1 = s1 * CLC1 * dptrat, c2 = s2 * CLC2 * dptrat, c3 = s3 * CLC3 * dptrat;
@@ -570,7 +596,7 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
 // cheaper than a compensated FP32 sum, which needs the returned old value of every
 // add; an FP64 accumulator of FP32 terms is at least as accurate as a Kahan sum.
 // ------------------------------------------------------------------------------------
-template <int CONV_U>
+template <int CONV_U, bool CPL>
 __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *panels, LinePrep lp, Tables T,
                                                 double *A, unsigned long long *work) {
     extern __shared__ double sm[];
@@ -652,8 +678,8 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
                 atomicAdd(&a1[j1], (double)__fmaf_rn(sv, g.z, __fmaf_rn(sd, g.y, s1 * g.x)));
             }
         }
-        const double dptrat = lp.sppsp[o];
-        if (dptrat != 0.) {   // line coupling, as in k_conv64
+        const double dptrat = CPL ? lp.sppsp[o] : 0.;
+        if (CPL && dptrat != 0.) {   // line coupling, as in k_conv64
             const float c1 = (float)((double)s1 * CLC1 * dptrat), c2 = (float)((double)s2 * CLC2 * dptrat);
             const float c3 = (float)((double)s3 * CLC3 * dptrat), cd = (float)((double)sd * CLC1 * dptrat);
             const float cv = (float)((double)sv * CLC1 * dptrat);
@@ -713,6 +739,7 @@ __device__ __forceinline__ bool f4_accept(const Job &J, const LinesDev &L, int64
     const double sui = (double)L.s[i] * J.w[m];
     return sui != 0. && !(L.nu[i] < J.v1l4);
 }
+template <bool BRD>
 __global__ void __launch_bounds__(256) k_f4acc(const Job *jobs, LinesDev L, F4Prep f, int *err) {
     const Job &J = jobs[blockIdx.y];
     const int64_t r = blockIdx.x;
@@ -728,7 +755,7 @@ __global__ void __launch_bounds__(256) k_f4acc(const Job *jobs, LinesDev L, F4Pr
         const int64_t i = J.f4_lo + k;
         int m, iso;
         on = f4_accept(J, L, i, &m, &iso);
-        if (on && shifted_vnu(J, L, i, L.nu[i], read_pshift(L, i, m)) > J.v2l4)
+        if (on && shifted_vnu<BRD>(J, L, i, L.nu[i], read_pshift(L, i, m)) > J.v2l4)
             atomicMin(&f.stop[blockIdx.y], (int)k);
         f.acc[J.f4_off + k] = on ? 1 : 0;
     }
@@ -804,6 +831,7 @@ __global__ void __launch_bounds__(256) k_recpos(const Job *jobs, F4Prep f, Comp 
 
 // LINF4Q per-line physics (clblm_ODLAY_LineF4.f90:951-1170) for the accepted lines of
 // one TAPE3 record up to the stop line, written at their positions.
+template <bool BRD>
 __global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4Prep f, Comp c, int *err) {
     const Job &J = jobs[blockIdx.y];
     const int64_t r = blockIdx.x;
@@ -827,7 +855,7 @@ __global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4P
     int m, iso;
     (void)f4_accept(J, L, i, &m, &iso);
     double sui = (double)L.s[i] * J.w[m];
-    const double vnu = shifted_vnu(J, L, i, L.nu[i], read_pshift(L, i, m));
+    const double vnu = shifted_vnu<BRD>(J, L, i, L.nu[i], read_pshift(L, i, m));
     double alfa = (double)L.a_air[i];
     if (m == 7) {
         const double rvmr = 0.21;
@@ -843,7 +871,7 @@ __global__ void __launch_bounds__(256) k_f4prep(const Job *jobs, LinesDev L, F4P
     const double tmpcor = pow(J.tratio, (double)L.tmpalf[i]);
     const double alfa0i = alfa * tmpcor;
     const double hwhmsi = (double)L.a_self[i] * tmpcor;
-    alfa = lorentz_width(J, L, i, m, alfa0i, hwhmsi);
+    alfa = lorentz_width<BRD>(J, L, i, m, alfa0i, hwhmsi);
     if (fl == 3) alfa = alfa * (1.0 - lc.a * J.pavp0 - lc.b * J.pavp2);
     // SP = SUI (1 + G P^2), SPP = SUI Y P (:1130-1131); Y = G = 0 unless IFLAG 1
     const double sp = fl == 1 ? sui * (1. + lc.b * J.pavp2) : sui;
@@ -2302,6 +2330,7 @@ struct LineListH {
     std::vector<double> nu;
     void *mem = nullptr;
     void *brd_mem = nullptr;   // species-broadening fields, if attached
+    bool has_cpl = false;      // some line has IFLG 1 (Y and G)
     LinesDev dev{};
 };
 
@@ -2957,13 +2986,16 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     const int64_t nblk = (int64_t)B.blks.size();
 
     // fork: the F4 chain runs on the F4 stream from here to the join before pass 2
+    bool brd = false;   // species broadening anywhere in the batch, with the fields attached
+    if (L.dev.brd_flg)
+        for (const Job &Jb : B.jobs) brd = brd || Jb.species_broad != 0;
     HIPCHK(hipEventRecord(ev_fork, st));
     HIPCHK(hipStreamWaitEvent(s4, ev_fork, 0));
     HIPCHK(stage_event(7, s4));
     if (B.max_f4 > 0) {
-        hipLaunchKernelGGL(k_f4acc, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, d_err);
+        hipLaunchKernelGGL(brd ? k_f4acc<true> : k_f4acc<false>, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, d_err);
         hipLaunchKernelGGL(k_recpos, dim3(nlay), dim3(256), 0, s4, d_jobs, f, cmp);
-        hipLaunchKernelGGL(k_f4prep, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, cmp, d_err);
+        hipLaunchKernelGGL(brd ? k_f4prep<true> : k_f4prep<false>, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, cmp, d_err);
         hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, s4, d_jobs, cmp, f.G);
         hipLaunchKernelGGL(k_sim, dim3(nlay), dim3(256), 0, s4, d_jobs, f, cmp, (const int *)f.G);
         hipLaunchKernelGGL(k_emit, dim3((unsigned)std::max<int64_t>(1, B.tot_rec)), dim3(64), 0, s4, d_jobs, f, cmp,
@@ -3020,7 +3052,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     HIPCHK(hipEventRecord(ev_join, s4));
     HIPCHK(stage_event(0, st));
     if (B.max_line > 0) {
-        hipLaunchKernelGGL(k_prep, dim3(cdiv(B.max_line, 256), nlay), dim3(256), 0, st, d_jobs, d_pan, L.dev, lp, d_err);
+        hipLaunchKernelGGL(brd ? k_prep<true> : k_prep<false>, dim3(cdiv(B.max_line, 256), nlay), dim3(256), 0, st, d_jobs, d_pan, L.dev, lp, d_err);
         int *htot = at<int>(base, o_htot);
         hipLaunchKernelGGL(k_home_local, dim3((unsigned)hcap, nlay), dim3(SCAN_T), 0, st, d_jobs, lp, htot, hcap);
         hipLaunchKernelGGL(k_home_fix, dim3((unsigned)hcap, nlay), dim3(SCAN_T), 0, st, d_jobs, lp, htot, hcap);
@@ -3031,10 +3063,11 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         const dim3 g((unsigned)npan), b(256);
         const size_t lds = (size_t)B.max_stride * sizeof(double);
 #define CONV_LAUNCH(U)                                                                                          \
-    if (precision == CLBLM_FP64)                                                                                \
-        hipLaunchKernelGGL(k_conv64<U>, g, b, lds, st, d_jobs, d_pan, lp, T, d_A, C->d_work);                   \
-    else                                                                                                        \
-        hipLaunchKernelGGL(k_conv32<U>, g, b, lds, st, d_jobs, d_pan, lp, T, d_A, C->d_work);
+    {                                                                                                           \
+        auto kc = precision == CLBLM_FP64 ? (L.has_cpl ? k_conv64<U, true> : k_conv64<U, false>)                \
+                                          : (L.has_cpl ? k_conv32<U, true> : k_conv32<U, false>);               \
+        hipLaunchKernelGGL(kc, g, b, lds, st, d_jobs, d_pan, lp, T, d_A, C->d_work);                            \
+    }
         switch (C->conv_u) {
         case 2: CONV_LAUNCH(2) break;
         case 8: CONV_LAUNCH(8) break;
@@ -3210,6 +3243,7 @@ int clblm_linelist_upload(clblm_ctx *c, int64_t n, const double *nu, const float
     LineListH *L = new LineListH;
     L->n = n;
     L->nu = cen;
+    for (int64_t i = 0; i < n && !L->has_cpl; i++) L->has_cpl = iflg[i] == 1;
     size_t bytes = (size_t)n * (8 + 8 + 6 * 4 + 2 * 4) + 9 * 256;
     HIPCHK(hipMalloc(&L->mem, bytes));
     Carve cv;
