@@ -15,7 +15,7 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
            "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune", "clblm_scanfn",
-           "clblm_linelist_broadening", "clblm_tape3_read", "clblm_linelist_load_tape3",
+           "clblm_linelist_broadening", "clblm_rt_down", "clblm_tape3_read", "clblm_linelist_load_tape3",
            "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy"]
 OD_SPECIES_BROAD = 1
 TUNE_F4_XCD_CHUNK = 1
@@ -70,6 +70,8 @@ def load():
     lib.clblm_linelist_upload.argtypes = [P, ctypes.c_int64] + [P] * 9 + [ctypes.POINTER(ctypes.c_int)]
     lib.clblm_linelist_free.argtypes = [P, ctypes.c_int]
     lib.clblm_linelist_broadening.argtypes = [P, ctypes.c_int, P, P, P, P]
+    lib.clblm_rt_down.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double, P, P, P,
+                                  ctypes.c_int, P, P, P, P]
     lib.clblm_tape3_read.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
                                      ctypes.POINTER(ctypes.c_int64)] + [P] * 13
     lib.clblm_linelist_load_tape3.argtypes = [P, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,

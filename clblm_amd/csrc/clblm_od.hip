@@ -3108,6 +3108,9 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
 // ====================================================================================
 // C-ABI
 // ====================================================================================
+extern "C" int clblm_rt_down_impl(int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                                  const double *d_tlay, const double *d_tlev, int lin, const double *d_sol,
+                                  double *d_rad, double *d_tx, void *stream);
 extern "C" int clblm_rt_up_impl(int nlay, int64_t npts, double v1, double dv, const double *d_od,
                                 const double *d_tlay, const double *d_tlev, double emis, double tskin, int lin,
                                 double sec_dn, double *d_rad, double *d_dr_dod, double *d_dr_dt,
@@ -3757,6 +3760,19 @@ int clblm_level_jac(clblm_ctx *c, int nlay, int nmol, int64_t npts, const double
     int rc = clblm_level_jac_impl(nlay, nmol, npts, d_od_var, d_dr_dtau, d_dr_dtlev, d_wtot, d_tlay, dt, dw_frac,
                                   d_dt_dtup, d_dt_dtlo, d_dw_dqup, d_dw_dqlo, d_jac_t, d_jac_q, st);
     if (rc) return set_err(rc, "level Jacobian kernel launch failed");
+    return 0;
+}
+
+int clblm_rt_down(clblm_ctx *c, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                  const double *d_tlay, const double *d_tlev, int lin_in_tau, const double *d_sol_toa,
+                  double *d_rad, double *d_tx, void *stream) {
+    if (!c || nlay <= 0 || npts <= 0 || !d_od || !d_tlay || !d_tlev || !d_rad)
+        return set_err(CLBLM_ERR_ARG, "null argument or empty grid");
+    if (lin_in_tau < 0 || lin_in_tau > 2) return set_err(CLBLM_ERR_ARG, "lin_in_tau must be 0, 1 or 2");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = clblm_rt_down_impl(nlay, npts, v1, dv, d_od, d_tlay, d_tlev, lin_in_tau, d_sol_toa, d_rad, d_tx, st);
+    if (rc) return set_err(rc, "radiance kernel launch failed");
     return 0;
 }
 

@@ -184,6 +184,54 @@ __global__ void k_rt_up(int nlay, int64_t npts, double v1, double dv, const doub
 // nlay+1) and `sbar` (layer-mean source derivative of the downwelling pass, nlay),
 // all [k][n]; dr_dtlev holds the downwelling level terms until the upwelling pass
 // folds them in.
+// Down-welling radiance at the lowest level of a look-up path: mode3_lookUp_mergeUp
+// (clblm_noScattRT.f90:843-1108).  Layers from the observer up: each layer's
+// downward emission (layerEmis with Tbot, Ttop, :972-974) is added through the
+// transmittance of the layers below it, then that transmittance takes the layer's
+// (:975-990); the top-of-atmosphere solar radiance, if given, comes through the
+// whole path (:1021-1027).  One lane per spectral point.
+__global__ void __launch_bounds__(256) k_rt_down(int nlay, int64_t npts, double v1, double dv,
+                                                 const double *__restrict__ od, const double *__restrict__ t_lay,
+                                                 const double *__restrict__ t_lev, int lin,
+                                                 const double *__restrict__ sol, double *__restrict__ rad,
+                                                 double *__restrict__ tx_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const double v = v1 + (double)i * dv;
+    double r = 0., tacc = 1.;
+    for (int l = 0; l < nlay; l++) {
+        const double o = od[(int64_t)l * npts + i];
+        const double tx = exp(-o);
+        const double tbot = t_lev[l], ttop = t_lev[l + 1], tave = t_lay[l];
+        const double bbar = planck_x(v, tave / RADCN2);
+        double em;
+        if (lin == 0) {
+            em = (1 - tx) * bbar;
+        } else {
+            double h, dh;
+            rlin(o, &h, &dh);
+            const double ba = planck_x(v, tbot / RADCN2);
+            em = lin == 1 ? (1 - tx) * (ba + 2 * (bbar - ba) * h)
+                          : (1 - tx) * (ba + (planck_x(v, ttop / RADCN2) - ba) * h);
+        }
+        r = r + em * tacc;
+        tacc = tacc * tx;
+    }
+    if (sol) r = r + tacc * sol[i];
+    rad[i] = r;
+    if (tx_out) tx_out[i] = tacc;
+}
+
+extern "C" __attribute__((visibility("hidden"))) int clblm_rt_down_impl(int nlay, int64_t npts, double v1, double dv,
+                                                                        const double *d_od, const double *d_tlay,
+                                                                        const double *d_tlev, int lin,
+                                                                        const double *d_sol, double *d_rad,
+                                                                        double *d_tx, void *stream) {
+    hipLaunchKernelGGL(k_rt_down, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream, nlay,
+                       npts, v1, dv, d_od, d_tlay, d_tlev, lin, d_sol, d_rad, d_tx);
+    return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}
+
 __global__ void __launch_bounds__(256) k_rt_jac(int L, int64_t n, double v1, double dv, const double *__restrict__ od,
                                                 const double *__restrict__ t_lay, const double *__restrict__ t_lev,
                                                 double emis, double tskin, int lin, double sec,

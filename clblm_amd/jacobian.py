@@ -186,3 +186,23 @@ try:
                     None, None)
 except ImportError:  # pragma: no cover - torch is part of the stack
     _Radiance = None
+
+
+def downwelling_radiance(ctx, od, t_lay, t_lev, v1, dv, lin_in_tau=1, sol_toa=None):
+    """Down-welling radiance at the observer of a look-up path (mode3_lookUp_mergeUp,
+    src/clblm_src/clblm_noScattRT.f90:843-1108) from float64 cuda tensors: od [L, N]
+    (layer 0 at the observer), t_lay [L], t_lev [L+1]; sol_toa [N] (optional) the solar
+    radiance at the top.  Returns (radiance [N], path transmittance [N])."""
+    import torch
+    od = od.contiguous().double()
+    L, n = od.shape
+    t_lay = t_lay.contiguous().double()
+    t_lev = t_lev.contiguous().double()
+    rad = torch.empty(n, dtype=torch.float64, device=od.device)
+    tx = torch.empty(n, dtype=torch.float64, device=od.device)
+    p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    sol = sol_toa.contiguous().double() if sol_toa is not None else None
+    check(ctx.lib.clblm_rt_down(ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(dv), p(od), p(t_lay), p(t_lev),
+                                int(lin_in_tau), p(sol), p(rad), p(tx),
+                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return rad, tx

@@ -202,6 +202,17 @@ int clblm_autotune(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
  * error among them (0 if none). */
 int clblm_sync(clblm_ctx *ctx);
 
+/* Down-welling radiance at the observer of a look-up path (mode3_lookUp_mergeUp,
+ * clblm_noScattRT.f90:843-1108): nlay layer ODs on one grid (d_od[l*npts + i], l = 0
+ * at the observer), layer and level temperatures as for clblm_rt_up, layerEmis with
+ * linInTau 0/1/2 (clblm_EMLAY.f90:104-178).  d_sol_toa (nullable): the solar
+ * radiance at the top on the same grid, added through the whole path.  d_tx
+ * (nullable) receives the path transmittance.  Enqueued on `stream`; returns once
+ * launched. */
+int clblm_rt_down(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                  const double *d_tlay, const double *d_tlev, int lin_in_tau, const double *d_sol_toa,
+                  double *d_rad, double *d_tx, void *stream);
+
 /* ---- radiance and its derivatives ---------------------------------------------
  * Upwelling radiance at the top of nlay layers (layer 0 at the surface) on a
  * common grid v1 + i*dv, i < npts: the merge-up recursion of mode2_lookDn_mergeUp /

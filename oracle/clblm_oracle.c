@@ -1404,6 +1404,26 @@ static double layer_emis(double v, double ta, double tb, double tave, double tx,
  * layer below the observer: mode2_lookDn_mergeUp / layerMerge_mode2
  * (clblm_noScattRT.f90:1287-1296, 1704-1794) on a common grid.  od[l*npts+i] is the
  * layer OD along the view path; the down path uses od * sec_dn. */
+/* mode3_lookUp_mergeUp, clblm_noScattRT.f90:843-1108: down-welling radiance at the
+ * observer, layers from the observer up; sol (nullable) the solar radiance at the top. */
+void orc_rt_down(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
+                 const double *t_lev, int lin, const double *sol, double *rad, double *tx) {
+    for (int64_t i = 0; i < npts; i++) {
+        double v = v1 + (double)i * dv;
+        double r = 0., tacc = 1.;
+        for (int l = 0; l < nlay; l++) {
+            double o = od[(int64_t)l * npts + i];
+            double t = exp(-o);
+            double em = layer_emis(v, t_lev[l], t_lev[l + 1], t_lay[l], t, o, lin);
+            r = r + em * tacc;
+            tacc = tacc * t;
+        }
+        if (sol) r = r + tacc * sol[i];
+        rad[i] = r;
+        if (tx) tx[i] = tacc;
+    }
+}
+
 void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
                const double *t_lev, double emis, double tskin, int lin, double sec_dn, double *rad) {
     for (int64_t i = 0; i < npts; i++) {
