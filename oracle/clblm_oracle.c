@@ -1,7 +1,7 @@
 /*
  * CLBLM oracle -- sequential CPU restatement of the reference's ODLAY path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  The product never links or calls it.
+ * TEST INFRASTRUCTURE ONLY (it checks results).  The product never links or calls it.
  *
  * Each routine follows the reference routine named in its comment, statement by
  * statement and in the reference's operation order, because the FP64 GPU path is

@@ -3,7 +3,7 @@
  * path (ODLAY -> OPDPTH -> LINF4Q/SHRINQ/LBLF4Q -> HIRACQ/LNCORQ/CNVFNQ/panelOut).
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker.
+ * cpu_baseline leg of bench.py may load this library, and only to check results.
  * The product (clblm_amd/) never links, loads or calls it.
  *
  * All arithmetic is IEEE double with no FMA contraction (build with
