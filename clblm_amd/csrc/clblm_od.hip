@@ -2331,6 +2331,7 @@ struct LineListH {
     void *mem = nullptr;
     void *brd_mem = nullptr;   // species-broadening fields, if attached
     bool has_cpl = false;      // some line has IFLG 1 (Y and G)
+    int graphs = 0;            // captured graphs that read this list's device arrays
     LinesDev dev{};
 };
 
@@ -2800,6 +2801,7 @@ struct clblm_od_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool launched = false;
+    LineListH *list = nullptr;  // the line list whose device pointers the graph holds
 };
 static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, hipStream_t st, int precision,
                      clblm_od_graph *G = nullptr);
@@ -3288,8 +3290,10 @@ int clblm_linelist_broadening(clblm_ctx *c, int handle, const int32_t *flg, cons
         return set_err(CLBLM_ERR_ARG, "bad line list handle");
     HIPCHK(hipSetDevice(c->device));
     std::lock_guard<std::mutex> g(c->mu);
-    drain(c, true);   // batches in flight may read the fields being replaced
     LineListH *L = c->lists[handle];
+    if (L->graphs > 0)
+        return set_err(CLBLM_ERR_ARG, "line list is read by a captured graph: destroy the graph first");
+    drain(c, true);   // batches in flight may read the fields being replaced
     const size_t n7 = (size_t)L->n * 7;
     if (L->brd_mem) {
         HIPCHK(hipDeviceSynchronize());
@@ -3312,8 +3316,14 @@ int clblm_linelist_broadening(clblm_ctx *c, int handle, const int32_t *flg, cons
 }
 
 int clblm_linelist_free(clblm_ctx *c, int handle) {
-    if (!c || handle < 0 || handle >= (int)c->lists.size() || !c->lists[handle])
+    if (!c) return set_err(CLBLM_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (handle < 0 || handle >= (int)c->lists.size() || !c->lists[handle])
         return set_err(CLBLM_ERR_ARG, "bad line list handle");
+    if (c->lists[handle]->graphs > 0)
+        return set_err(CLBLM_ERR_ARG, "line list is read by a captured graph: destroy the graph first");
+    HIPCHK(hipSetDevice(c->device));
+    drain(c, true);   // batches in flight may read the list
     (void)hipFree(c->lists[handle]->mem);
     if (c->lists[handle]->brd_mem) (void)hipFree(c->lists[handle]->brd_mem);
     delete c->lists[handle];
@@ -3436,6 +3446,8 @@ int clblm_od_graph_create(clblm_ctx *c, int handle, int nlay, const clblm_layer 
     for (int l = 0; l < nlay; l++) offsets[l] = B.jobs[l].out_off;
     offsets[nlay] = B.tot_out;
     *jrad_state = jr;
+    g->list = c->lists[handle];
+    g->list->graphs++;
     *graph = g;
     return 0;
 }
@@ -3470,6 +3482,10 @@ int clblm_od_graph_destroy(clblm_ctx *c, clblm_od_graph *g) {
     if (!c || !g) return 0;
     (void)hipSetDevice(c->device);
     if (g->launched) (void)hipEventSynchronize(g->done);
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (g->list) g->list->graphs--;
+    }
     graph_free(g);
     return 0;
 }

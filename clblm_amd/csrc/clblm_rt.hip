@@ -171,19 +171,6 @@ __global__ void k_rt_up(int nlay, int64_t npts, double v1, double dv, const doub
     if (dr_demis) dr_demis[i] = rb * bs - rhob;
 }
 
-// ---------------------------------------------------------------------------------
-// Jacobians (src/clblm_src/clblm_noScattJac.f90:346-716) per spectral point, for a
-// down-looking sensor above the top layer (obsLev = nlay + 1), thermal only: the view
-// OD of layer l is od[l*n+i] (amfView = 1) and the down-path OD is sec_dn times it
-// (amfDown = sec_dn, the air-mass-scaled path of pathOD_up_dn_sun :826-838).  The
-// reference's cumulative-transmittance form with its summation order, including the
-// rule that the reflected downwelling term is dropped when the total view
-// transmittance is 1e-6 or less (:480).  Layer 0 is at the surface; level k is the
-// bottom of layer k.  Per-layer states the loops pass to each other live in `sa`
-// (view transmittance to the top, nlay+1 levels), `sb` (two-path transmittance,
-// nlay+1) and `sbar` (layer-mean source derivative of the downwelling pass, nlay),
-// all [k][n]; dr_dtlev holds the downwelling level terms until the upwelling pass
-// folds them in.
 // Down-welling radiance at the lowest level of a look-up path: mode3_lookUp_mergeUp
 // (clblm_noScattRT.f90:843-1108).  Layers from the observer up: each layer's
 // downward emission (layerEmis with Tbot, Ttop, :972-974) is added through the
@@ -232,6 +219,19 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_rt_down_impl(int nlay
     return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
 }
 
+// ---------------------------------------------------------------------------------
+// Jacobians (src/clblm_src/clblm_noScattJac.f90:346-716) per spectral point, for a
+// down-looking sensor above the top layer (obsLev = nlay + 1), thermal only: the view
+// OD of layer l is od[l*n+i] (amfView = 1) and the down-path OD is sec_dn times it
+// (amfDown = sec_dn, the air-mass-scaled path of pathOD_up_dn_sun :826-838).  The
+// reference's cumulative-transmittance form with its summation order, including the
+// rule that the reflected downwelling term is dropped when the total view
+// transmittance is 1e-6 or less (:480).  Layer 0 is at the surface; level k is the
+// bottom of layer k.  Per-layer states the loops pass to each other live in `sa`
+// (view transmittance to the top, nlay+1 levels), `sb` (two-path transmittance,
+// nlay+1) and `sbar` (layer-mean source derivative of the downwelling pass, nlay),
+// all [k][n]; dr_dtlev holds the downwelling level terms until the upwelling pass
+// folds them in.
 __global__ void __launch_bounds__(256) k_rt_jac(int L, int64_t n, double v1, double dv, const double *__restrict__ od,
                                                 const double *__restrict__ t_lay, const double *__restrict__ t_lev,
                                                 double emis, double tskin, int lin, double sec,

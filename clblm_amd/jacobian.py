@@ -195,13 +195,20 @@ def downwelling_radiance(ctx, od, t_lay, t_lev, v1, dv, lin_in_tau=1, sol_toa=No
     radiance at the top.  Returns (radiance [N], path transmittance [N])."""
     import torch
     od = od.contiguous().double()
+    if od.dim() != 2 or not od.is_cuda:
+        raise ValueError("od must be a [layers, points] cuda tensor")
     L, n = od.shape
     t_lay = t_lay.contiguous().double()
     t_lev = t_lev.contiguous().double()
+    sol = sol_toa.contiguous().double() if sol_toa is not None else None
+    # the kernel reads t_lay[L], t_lev[L+1] and sol[N] on the device: check them here
+    for name, t, want in (("t_lay", t_lay, L), ("t_lev", t_lev, L + 1), ("sol_toa", sol, n)):
+        if t is not None and (t.numel() != want or t.device != od.device):
+            raise ValueError("%s must hold %d values on %s (has %d on %s)" % (name, want, od.device, t.numel(),
+                                                                            t.device))
     rad = torch.empty(n, dtype=torch.float64, device=od.device)
     tx = torch.empty(n, dtype=torch.float64, device=od.device)
     p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
-    sol = sol_toa.contiguous().double() if sol_toa is not None else None
     check(ctx.lib.clblm_rt_down(ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(dv), p(od), p(t_lay), p(t_lev),
                                 int(lin_in_tau), p(sol), p(rad), p(tx),
                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))

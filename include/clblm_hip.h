@@ -84,6 +84,8 @@ int clblm_linelist_upload(clblm_ctx *ctx, int64_t n, const double *nu, const flo
                           const float *a_air, const float *a_self, const float *e_lo,
                           const float *tmpalf, const float *shift, const int32_t *mol,
                           const int32_t *iflg, int *handle);
+/* Frees a line list once the batches that read it are done; CLBLM_ERR_ARG while a
+ * captured graph (clblm_od_graph_create) holds it. */
 int clblm_linelist_free(clblm_ctx *ctx, int handle);
 /* ---- TAPE3 line files ---------------------------------------------------------
  * The binary line file LNFL writes, read as CLBLM_LineData_init
@@ -113,7 +115,8 @@ int clblm_linelist_load_tape3(clblm_ctx *ctx, const char *path, double vlo, doub
  * arrays of n x 7 values, record-major (entry 7 i + j - 1 for broadener j = 1..7).
  * RDLIN's read-time correction of an O2 line's shift by its own species data
  * (:2730-2738) applies from then on; the broadening itself applies to batches whose
- * parameters set CLBLM_OD_SPECIES_BROAD. */
+ * parameters set CLBLM_OD_SPECIES_BROAD.  CLBLM_ERR_ARG while a captured graph holds
+ * the list (the graph reads the fields in place). */
 int clblm_linelist_broadening(clblm_ctx *ctx, int handle, const int32_t *flg, const float *hw,
                               const float *tmp, const float *shft);
 
@@ -153,8 +156,9 @@ int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
  * F4 wing stream too, into one hipGraph with its own device scratch.  Nothing runs
  * yet.  clblm_od_graph_launch replays it on `stream` (NULL = the context's stream)
  * and returns once it is enqueued; each replay rewrites d_od from the layer inputs
- * and the line list as they were at creation (the line list must stay uploaded;
- * tuning changes made later do not apply).  Replays of one graph run in launch
+ * and the line list as they were at creation (tuning changes made later do not
+ * apply); until the graph is destroyed, clblm_linelist_free and
+ * clblm_linelist_broadening refuse its line list.  Replays of one graph run in launch
  * order, also across streams; replays and other batches writing d_od must be
  * ordered by the caller's streams.  clblm_od_graph_check waits for the last replay
  * and returns the device error of clblm_od_batch (line coupling, capacity), if any. */

@@ -1400,12 +1400,19 @@ static double layer_emis(double v, double ta, double tb, double tave, double tx,
     return abso * (bba + (bbb - bba) * hlf);
 }
 
-/* Upwelling radiance at the top of nlay layers (layer 0 at the surface), every
- * layer below the observer: mode2_lookDn_mergeUp / layerMerge_mode2
- * (clblm_noScattRT.f90:1287-1296, 1704-1794) on a common grid.  od[l*npts+i] is the
- * layer OD along the view path; the down path uses od * sec_dn. */
+/* layerEmis at every point of v1 + i dv, transmittance exp(-od): the entry point
+ * the tests compare with the reference's own layerEmis. */
+void orc_layer_emis(int64_t npts, const double *od, double v1, double dv, double ta, double tb, double tave,
+                    int lin, double *out) {
+    for (int64_t i = 0; i < npts; i++)
+        out[i] = layer_emis(v1 + (double)i * dv, ta, tb, tave, exp(-od[i]), od[i], lin);
+}
+
 /* mode3_lookUp_mergeUp, clblm_noScattRT.f90:843-1108: down-welling radiance at the
- * observer, layers from the observer up; sol (nullable) the solar radiance at the top. */
+ * observer, layers from the observer up (layer 0 nearest), merged upward: each
+ * layer's downward emission (layerEmis from its lower level, :972-974) is attenuated
+ * by the layers below it (:981-984); sol (nullable) the solar radiance at the top,
+ * added through the whole path (:1021-1027). */
 void orc_rt_down(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
                  const double *t_lev, int lin, const double *sol, double *rad, double *tx) {
     for (int64_t i = 0; i < npts; i++) {
@@ -1424,6 +1431,10 @@ void orc_rt_down(int nlay, int64_t npts, double v1, double dv, const double *od,
     }
 }
 
+/* Upwelling radiance at the top of nlay layers (layer 0 at the surface), every
+ * layer below the observer: mode2_lookDn_mergeUp / layerMerge_mode2
+ * (clblm_noScattRT.f90:1287-1296, 1704-1794) on a common grid.  od[l*npts+i] is the
+ * layer OD along the view path; the down path uses od * sec_dn. */
 void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
                const double *t_lev, double emis, double tskin, int lin, double sec_dn, double *rad) {
     for (int64_t i = 0; i < npts; i++) {

@@ -83,6 +83,8 @@ int orc_molec(const orc_layer *layer, double *scor /*[48*11]*/, double *rhoslf /
 double orc_planck(double v, double t);
 void orc_planck_drv(double v, double t, double *b, double *dbdt);
 void orc_rlin(double tau, double *w, double *dw);
+void orc_layer_emis(int64_t npts, const double *od, double v1, double dv, double ta, double tb, double tave,
+                    int lin, double *out);
 void orc_rt_down(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
                  const double *t_lev, int lin, const double *sol, double *rad, double *tx);
 void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,

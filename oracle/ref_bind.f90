@@ -8,11 +8,70 @@ module ref_bind
    use Module_ODLAY_CommonSub, only: MOLEC, RADFN, RADFNI, AVRAT, CGAUSS, CF1, CF2, CF3, CER
    use Module_TIPS,            only: TIPS_2003
    use Module_Spectrum,        only: XINT, interpSpectrum, CLBLM_Spectrum, CLBLM_Spectrum_init
-   use Module_EMLAY,           only: planck, planckFnDrv, rlin
+   use Module_EMLAY,           only: planck, planckFnDrv, rlin, layerEmis
    use Module_FFT,             only: clblm_FFTSCN, Intpdr
    use Module_ScanFilter,      only: clblm_SCANFN
+   use Module_LineData,        only: INPUT_HEADER, INPUT_BLOCK, NLINEREC, MXBRDMOL, MXBRDMOL3
    implicit none
+   external :: BUFIN_sgl
 contains
+
+   ! A TAPE3 line file read with the reference's own record types (INPUT_HEADER and
+   ! INPUT_BLOCK, clblm_LineData.f90:52-70) and record reader (BUFIN_sgl,
+   ! util_linux_intel.f90:121-150), in the order RDLIN and RDLNFL read it
+   ! (clblm_ODLAY.f90:2626-2658, clblm_ODLAY_LineF4.f90:1408-1418): the header record
+   ! (as lnfilhd_4 reads it, clblm_ODLAY_LineF4.f90:1302-1309), the negative-E'' record
+   ! when HLINID(7) ends in '^', then per panel its 6-word header and its NWDS-word
+   ! block, until the end of the file.  Every panel is returned, unmodified (RDLIN's
+   ! O2/N2 width correction is applied after the read).  Panel q (1-based) holds
+   ! entries 250*(q-1)+1 .. 250*(q-1)+nrec(q) of the per-entry arrays.
+   subroutine ref_tape3(path, cap, npanel, nrec, nwds, vmin, vmax, vnu, sp, alfa, epp, mol, hwhm, tmpalf, &
+                        pshift, iflg, brdflg, brddat, sdep, hdr_i, hdr_f) bind(C, name="ref_tape3")
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: cap
+      integer(c_int), intent(out) :: npanel, nrec(cap), nwds(cap), mol(NLINEREC, cap), iflg(NLINEREC, cap)
+      integer(c_int), intent(out) :: brdflg(MXBRDMOL, NLINEREC, cap), hdr_i(5)
+      real(c_double), intent(out) :: vmin(cap), vmax(cap), vnu(NLINEREC, cap)
+      real(c_float), intent(out) :: sp(NLINEREC, cap), alfa(NLINEREC, cap), epp(NLINEREC, cap)
+      real(c_float), intent(out) :: hwhm(NLINEREC, cap), tmpalf(NLINEREC, cap), pshift(NLINEREC, cap)
+      real(c_float), intent(out) :: brddat(MXBRDMOL3, NLINEREC, cap), sdep(NLINEREC, cap), hdr_f(2)
+      type(INPUT_HEADER) :: pnl
+      type(INPUT_BLOCK) :: buf
+      character(8) :: hlinid(10), bmolid(64), hid1(2)
+      integer*4 :: molcnt(64), mcntlc(64), mcntnl(64), linmol, lincnt, ilinlc, ilinnl, irec, irectl
+      integer*4 :: nneg(64), nres(64)
+      real*4 :: sumstr(64), flinlo, flinhi, xspace(4096)
+      character(len=4096) :: fname
+      integer :: lun, leof, n, q
+      n = 0
+      do while (path(n + 1) /= c_null_char .and. n < len(fname))
+         n = n + 1
+         fname(n:n) = path(n)
+      end do
+      open(newunit=lun, file=fname(1:n), status='OLD', form='UNFORMATTED')
+      read(lun) hlinid, bmolid, molcnt, mcntlc, mcntnl, sumstr, linmol, flinlo, flinhi, lincnt, ilinlc, &
+                ilinnl, irec, irectl, hid1
+      hdr_i = [linmol, lincnt, ilinlc, 0, 0]
+      hdr_f = [flinlo, flinhi]
+      if (hlinid(7)(8:8) == '^') then
+         hdr_i(4) = 1
+         read(lun) nneg, nres, xspace
+      end if
+      if (hlinid(10)(1:1) == 'I') hdr_i(5) = 1
+      npanel = 0
+      do q = 1, cap
+         call BUFIN_sgl(lun, leof, pnl, 6)
+         if (leof == 0) exit
+         call BUFIN_sgl(lun, leof, buf, pnl%NWDS)
+         npanel = q
+         nrec(q) = pnl%NREC; nwds(q) = pnl%NWDS; vmin(q) = pnl%VMIN; vmax(q) = pnl%VMAX
+         vnu(:, q) = buf%VNU; sp(:, q) = buf%SP; alfa(:, q) = buf%ALFA; epp(:, q) = buf%EPP
+         mol(:, q) = buf%MOL; hwhm(:, q) = buf%HWHM; tmpalf(:, q) = buf%TMPALF; pshift(:, q) = buf%PSHIFT
+         iflg(:, q) = buf%IFLG; brdflg(:, :, q) = buf%BRD_MOL_FLG_IN; brddat(:, :, q) = buf%BRD_MOL_DAT
+         sdep(:, q) = buf%SPEED_DEP
+      end do
+      close(lun)
+   end subroutine
 
    ! clblm_SCANFN_array (clblm_ScanFilter.f90:33-67): spectral-domain scan of sp(1:np)
    ! on v1 + (i-1) dv with scanning function functid of half width hwhm onto
@@ -183,6 +242,25 @@ contains
       real(r8) :: vv
       vv = v
       call planckFnDrv(vv, t, b, dbdt)
+   end subroutine
+
+   ! layerEmis (clblm_EMLAY.f90:104-178): emission of a layer of optical depth od(1:np)
+   ! on v1 + (i-1) dv from its Ta side, transmittance exp(-od), linInTau lin.
+   subroutine ref_layer_emis(np, od, v1, dv, ta, tb, tave, lin, out) bind(C, name="ref_layer_emis")
+      integer(c_int), value :: np, lin
+      real(c_double), value :: v1, dv, ta, tb, tave
+      real(c_double), intent(in) :: od(np)
+      real(c_double), intent(out) :: out(np)
+      type(CLBLM_Spectrum) :: em, tx, sod
+      real(r8) :: a
+      real :: d
+      a = v1; d = dv
+      call CLBLM_Spectrum_init(sod, a, d, np)
+      call CLBLM_Spectrum_init(tx, a, d, np)
+      sod%spect(sod%indV1:sod%indV2) = od
+      tx%spect(tx%indV1:tx%indV2) = exp(-od)
+      call layerEmis(em, ta, tb, tave, tx, sod, lin)
+      out = em%spect(em%indV1:em%indV1 + np - 1)
    end subroutine
 
    subroutine ref_rlin(od, hlf, dhlf) bind(C, name="ref_rlin")

@@ -1,11 +1,13 @@
 """Down-welling radiance of a look-up path (mode3_lookUp_mergeUp,
 src/clblm_src/clblm_noScattRT.f90:843-1108).  The oracle's restatement (orc_rt_down)
 is pinned by closed forms: an isothermal path gives B(T)(1 - prod tx) for every
-linInTau (the linear-in-tau source is B itself there), the solar term arrives through
-the whole path's transmittance, and an optically thin layer adds its own emission
-unattenuated only when it is the lowest.  The layer emission itself (layerEmis) is
-pinned against the reference in tests/test_oracle_ref.py.  The GPU kernel
-(clblm_rt_down) matches the oracle."""
+linInTau (the linear-in-tau source is B itself there); the solar term arrives through
+the whole path's transmittance; an opaque layer next to the observer hides every
+layer beyond it and emits from its lower (observer-side) level; a thin layer with
+linInTau 2 gives the explicit (1 - tx)(B(Tbot) + (B(Ttop) - B(Tbot)) h(tau)) with
+Planck and h written out here; and the oracle's layer emission equals the
+reference's own layerEmis (oracle/_ref, when built).  The GPU kernel (clblm_rt_down)
+matches the oracle."""
 import ctypes
 
 import numpy as np
@@ -64,6 +66,73 @@ def test_solar_through_the_path_and_order():
     a, _ = orc_down(od2, t_lay, t_lev, 800.0, 0.1, 1)
     b, _ = orc_down(od2[1:], t_lay[1:], t_lev[1:], 800.0, 0.1, 1)
     assert np.allclose(a, b, rtol=1e-13)
+
+
+RADCN1, RADCN2 = 1.191042722e-12, 1.4387752     # clblm_ConstParam.f90:70-71
+
+
+def _planck(v, t):
+    return RADCN1 * v ** 3 / np.expm1(RADCN2 * v / t)
+
+
+def _h(tau):
+    return 1.0 / tau - 1.0 / np.expm1(tau)
+
+
+@pytest.mark.parametrize("lin", [0, 1, 2])
+def test_opaque_first_layer_hides_the_rest(lin):
+    """Layer 0 (next to the observer) with OD 50: the radiance is its own emission,
+    B_eff from its lower level, whatever lies beyond it.  Merging in the other order
+    or emitting from the upper level would show layer 1 or B(Ttop)."""
+    n = 300
+    v1, dv = 700.0, 1.0
+    v = v1 + dv * np.arange(n)
+    od = np.vstack([np.full(n, 50.0), np.full(n, 0.7)])
+    t_lev = np.array([290.0, 240.0, 200.0])
+    t_lay = np.array([262.0, 218.0])
+    a, tx = orc_down(od, t_lay, t_lev, v1, dv, lin)
+    od2 = od.copy()
+    od2[1] = 3.0
+    b, _ = orc_down(od2, t_lay, np.array([290.0, 240.0, 150.0]), v1, dv, lin)
+    assert np.array_equal(a, b)
+    h = _h(50.0)
+    ba, bb, bt = _planck(v, 290.0), _planck(v, 240.0), _planck(v, 262.0)
+    want = {0: bt, 1: ba + 2 * (bt - ba) * h, 2: ba + (bb - ba) * h}[lin] * (1 - np.exp(-50.0))
+    assert np.allclose(a, want, rtol=1e-12)
+    assert np.all(tx < 1e-21)
+
+
+def test_thin_layer_linear_in_tau_2():
+    n = 200
+    v1, dv = 500.0, 2.5
+    v = v1 + dv * np.arange(n)
+    tau = np.linspace(0.002, 0.3, n)
+    r, tx = orc_down(tau[None, :], np.array([250.0]), np.array([280.0, 230.0]), v1, dv, 2)
+    ba, bb = _planck(v, 280.0), _planck(v, 230.0)
+    want = -np.expm1(-tau) * (ba + (bb - ba) * _h(tau))
+    assert np.allclose(r, want, rtol=1e-11)
+    assert np.allclose(tx, np.exp(-tau), rtol=1e-15)
+
+
+@pytest.mark.parametrize("lin", [0, 1, 2])
+def test_layer_emission_matches_the_reference(lin):
+    import os
+    ref_so = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                          "libclblm_ref.so")
+    if not os.path.exists(ref_so):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    ref = ctypes.CDLL(ref_so, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    ref.ref_layer_emis.argtypes = [ctypes.c_int, ctypes.c_void_p, D, D, D, D, D, ctypes.c_int, ctypes.c_void_p]
+    L = lib()
+    L.orc_layer_emis.argtypes = [ctypes.c_int64, ctypes.c_void_p, D, D, D, D, D, ctypes.c_int, ctypes.c_void_p]
+    rng = np.random.default_rng(7 + lin)
+    n = 4000
+    od = 10.0 ** rng.uniform(-6.0, 1.7, n)       # both rlin limits and the exact form
+    for ta, tb, tave in ((285.0, 262.0, 273.0), (215.0, 240.0, 226.5)):
+        a, b = np.zeros(n), np.zeros(n)
+        ref.ref_layer_emis(n, od.ctypes.data, 600.0, 0.37, ta, tb, tave, lin, a.ctypes.data)
+        L.orc_layer_emis(n, od.ctypes.data, 600.0, 0.37, ta, tb, tave, lin, b.ctypes.data)
+        assert np.array_equal(a, b), np.abs(a - b).max() / np.abs(a).max()
 
 
 @pytest.mark.gpu
