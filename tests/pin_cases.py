@@ -131,11 +131,17 @@ def _lay():
     return W.Layer(pave=100.0, tave=250.0, wtot=1e24, w={1: 1e21, 2: 1e21, 4: 1e21})
 
 
+def _padded(mols, nus, ss):
+    # lines of zero strength at 701 and 749 cm-1 are rejected by LNCORQ and LINF4Q
+    # alike; they keep every list at two lines or more (the library's minimum)
+    return lone([1] + mols + [1], [701.0] + nus + [749.0], [0.0] + ss + [0.0])
+
+
 def _pair(od, mols, nus, ss):
     lay = _lay()
-    ab = od(lone(mols, nus, ss), lay, 710.0, 740.0, DV)
-    a = od(lone(mols[:1], nus[:1], ss[:1]), lay, 710.0, 740.0, DV)
-    b = od(lone(mols[1:], nus[1:], ss[1:]), lay, 710.0, 740.0, DV)
+    ab = od(_padded(mols, nus, ss), lay, 710.0, 740.0, DV)
+    a = od(_padded(mols[:1], nus[:1], ss[:1]), lay, 710.0, 740.0, DV)
+    b = od(_padded(mols[1:], nus[1:], ss[1:]), lay, 710.0, 740.0, DV)
     return ab, a, b
 
 
