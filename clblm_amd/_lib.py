@@ -16,7 +16,9 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
            "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune", "clblm_scanfn",
            "clblm_linelist_broadening", "clblm_rt_down", "clblm_tape3_read", "clblm_linelist_load_tape3",
-           "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy"]
+           "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy",
+           "clblm_rt_jac_noalloc"]
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_CAPACITY = 0, 1, 2, 3, 4
 OD_SPECIES_BROAD = 1
 TUNE_F4_XCD_CHUNK = 1
 TUNE_F4_FAR_FIELD = 2
@@ -88,6 +90,7 @@ def load():
     lib.clblm_level2layer.argtypes = [ctypes.c_int, ctypes.c_int] + [P] * 8
     lib.clblm_rt_jac.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double, P, P, P, P, P,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, P, P, P, P, P, P]
+    lib.clblm_rt_jac_noalloc.argtypes = lib.clblm_rt_jac.argtypes
     lib.clblm_level_jac.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P, P, P, ctypes.c_double,
                                     ctypes.c_double, P, P, P, P, P, P, P]
     lib.clblm_fft_scan.argtypes = [P, P, ctypes.c_int64] + [ctypes.c_double] * 5 + [ctypes.c_int] + \

@@ -3125,6 +3125,12 @@ extern "C" int clblm_rt_jac_impl(int nlay, int64_t npts, double v1, double dv, c
                                  const double *d_dt_dtlo, double emis, double tskin, int lin, double sec_dn,
                                  double *d_rad, double *d_dr_dtau, double *d_dr_dtlev, double *d_dr_dtskin,
                                  double *d_dr_demis, double *d_state, void *stream);
+extern "C" int clblm_rt_scratch_init(int device);
+extern "C" int clblm_rt_jac_scratch(int device, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                                    const double *d_tlay, const double *d_tlev, const double *d_dt_dtup,
+                                    const double *d_dt_dtlo, double emis, double tskin, int lin, double sec_dn,
+                                    double *d_rad, double *d_dr_dtau, double *d_dr_dtlev, double *d_dr_dtskin,
+                                    double *d_dr_demis, void *stream);
 extern "C" int clblm_level_jac_impl(int nlay, int nmol, int64_t npts, const double *d_od_var,
                                     const double *d_dr_dtau, const double *d_dr_dtlev, const double *d_wtot,
                                     const double *d_tlay, double dt, double dw_frac, const double *d_dt_dtup,
@@ -3154,6 +3160,8 @@ int clblm_ctx_create(int device, clblm_ctx **out) {
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return set_err(CLBLM_ERR_ARG, "no such device");
     HIPCHK(hipSetDevice(device));
+    // the device address of clblm_rt_jac_noalloc's scratch array on this device
+    if (clblm_rt_scratch_init(device)) return set_err(CLBLM_ERR_HIP, "hipGetSymbolAddress of the scratch array failed");
     clblm_ctx *c = new clblm_ctx;
     c->device = device;
     HIPCHK(hipStreamCreate(&c->stream));
@@ -3756,6 +3764,33 @@ int clblm_rt_jac(clblm_ctx *c, int nlay, int64_t npts, double v1, double dv, con
                                c->rt_state, st);
     if (rc) return set_err(rc, "Jacobian kernel launch failed");
     HIPCHK(hipStreamSynchronize(st));   // the scratch states are reused by the next call
+    return 0;
+}
+
+int clblm_rt_jac_noalloc(clblm_ctx *c, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                         const double *d_tlay, const double *d_tlev, const double *d_dt_dtup,
+                         const double *d_dt_dtlo, double surf_emis, double t_skin, int lin_in_tau, double sec_dn,
+                         double *d_rad, double *d_dr_dtau, double *d_dr_dtlev, double *d_dr_dtskin,
+                         double *d_dr_demis, void *stream) {
+    if (!c || nlay <= 0 || npts <= 0 || !d_od || !d_tlay || !d_tlev || !d_rad || !d_dr_dtau || !d_dr_dtlev)
+        return set_err(CLBLM_ERR_ARG, "null argument or empty grid");
+    if (lin_in_tau < 0 || lin_in_tau > 2) return set_err(CLBLM_ERR_ARG, "lin_in_tau must be 0, 1 or 2");
+    if (lin_in_tau != 2 && (!d_dt_dtup || !d_dt_dtlo))
+        return set_err(CLBLM_ERR_ARG, "lin_in_tau 0 and 1 need dT/dT_lev of the layer means");
+    const double need = 8.0 * (3.0 * (double)nlay + 2.0) * (double)npts;
+    if (need > (double)CLBLM_RT_JAC_NOALLOC_BYTES) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "clblm_rt_jac_noalloc: %d layers x %lld points need %.0f bytes of scratch, "
+                 "more than its %lld (8 (3 nlay + 2) npts <= that)", nlay, (long long)npts, need,
+                 (long long)CLBLM_RT_JAC_NOALLOC_BYTES);
+        return set_err(CLBLM_ERR_CAPACITY, msg);
+    }
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = clblm_rt_jac_scratch(c->device, nlay, npts, v1, dv, d_od, d_tlay, d_tlev, d_dt_dtup, d_dt_dtlo,
+                                  surf_emis, t_skin, lin_in_tau, sec_dn, d_rad, d_dr_dtau, d_dr_dtlev, d_dr_dtskin,
+                                  d_dr_demis, st);
+    if (rc) return set_err(rc, "Jacobian kernel launch on the static scratch failed");
     return 0;
 }
 

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/clblm_hip.h"
@@ -477,6 +478,55 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_rt_jac_impl(
                        d_tlev, emis, tskin, lin, sec_dn, d_dt_dtup, d_dt_dtlo, d_rad, d_dr_dtau, d_dr_dtlev,
                        d_dr_dtskin, d_dr_demis, sa, sb, sbar);
     return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}
+
+// Scratch of clblm_rt_jac_noalloc: k_rt_jac's per-level states (sa, sb, sbar) in an
+// array the library owns for the life of the process, one copy on each GPU it runs
+// on, so that the entry point allocates nothing.  Host code uses the per-device
+// address that hipGetSymbolAddress returns, never the array's name (which in host
+// code is a host-side stand-in, not a device address).  Every state a lane reads is
+// written by that lane earlier in the same launch, so nothing needs clearing.
+static __device__ __align__(256) double scratch[CLBLM_RT_JAC_NOALLOC_BYTES / sizeof(double)];
+
+namespace {
+constexpr int SCRATCH_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[SCRATCH_MAX_DEV] = {};
+std::mutex g_scratch_call_mu[SCRATCH_MAX_DEV];
+}  // namespace
+
+// Once per device, with that device current (clblm_ctx_create): the device address of
+// `scratch` on it.
+extern "C" __attribute__((visibility("hidden"))) int clblm_rt_scratch_init(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEV) return CLBLM_ERR_ARG;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CLBLM_ERR_HIP;
+    g_scratch_ptr[device] = (double *)p;
+    return 0;
+}
+
+// clblm_rt_jac with its states in `scratch`: the caller must have checked the size.
+// Calls on one device are serialized by a per-device lock held until the kernel has
+// finished, because they share the one array.
+extern "C" __attribute__((visibility("hidden"))) int clblm_rt_jac_scratch(
+    int device, int nlay, int64_t npts, double v1, double dv, const double *d_od, const double *d_tlay,
+    const double *d_tlev, const double *d_dt_dtup, const double *d_dt_dtlo, double emis, double tskin, int lin,
+    double sec_dn, double *d_rad, double *d_dr_dtau, double *d_dr_dtlev, double *d_dr_dtskin, double *d_dr_demis,
+    void *stream) {
+    if (device < 0 || device >= SCRATCH_MAX_DEV) return CLBLM_ERR_ARG;
+    double *ptr;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        ptr = g_scratch_ptr[device];
+    }
+    if (!ptr) return CLBLM_ERR_HIP;
+    std::lock_guard<std::mutex> call(g_scratch_call_mu[device]);
+    int rc = clblm_rt_jac_impl(nlay, npts, v1, dv, d_od, d_tlay, d_tlev, d_dt_dtup, d_dt_dtlo, emis, tskin, lin,
+                               sec_dn, d_rad, d_dr_dtau, d_dr_dtlev, d_dr_dtskin, d_dr_demis, ptr, stream);
+    if (rc) return rc;
+    return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : CLBLM_ERR_HIP;
 }
 
 extern "C" __attribute__((visibility("hidden"))) int clblm_level_jac_impl(

@@ -261,6 +261,25 @@ int clblm_rt_jac(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, c
                  double surf_emis, double t_skin, int lin_in_tau, double sec_dn, double *d_rad, double *d_dr_dtau,
                  double *d_dr_dtlev, double *d_dr_dtskin, double *d_dr_demis, void *stream);
 
+/* clblm_rt_jac without any allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernel and results as
+ * clblm_rt_jac, but k_rt_jac's per-level states live in a static device array of
+ * CLBLM_RT_JAC_NOALLOC_BYTES that the library holds on each GPU until the process
+ * ends, instead of context memory grown with hipMalloc.  It needs
+ * 8 (3 nlay + 2) npts <= CLBLM_RT_JAC_NOALLOC_BYTES (e.g. 60 layers x 46 000 points)
+ * and returns CLBLM_ERR_CAPACITY beyond that.  All calls on one GPU share the array,
+ * which keeps the last call's states: the library serializes them (a per-GPU lock held
+ * until the kernel has finished, like clblm_rt_jac synchronous), so calls made at
+ * the same time from several threads or contexts wait for each other instead of
+ * racing.  Calls no hipMalloc, hipFree or allocating routine. */
+#define CLBLM_RT_JAC_NOALLOC_BYTES ((int64_t)64 << 20)
+int clblm_rt_jac_noalloc(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                         const double *d_tlay, const double *d_tlev, const double *d_dt_dtup,
+                         const double *d_dt_dtlo, double surf_emis, double t_skin, int lin_in_tau, double sec_dn,
+                         double *d_rad, double *d_dr_dtau, double *d_dr_dtlev, double *d_dr_dtskin,
+                         double *d_dr_demis, void *stream);
+
 /* drv_LevelTemp and drv_LevelConc (clblm_noScattJac.f90:899-1203): level Jacobians
  * from clblm_rt_jac's derivatives and the layer-OD variants the reference computes.
  * d_od_var holds 3 + nmol device blocks of nlay*npts: the layer OD, the OD at
