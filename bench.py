@@ -67,21 +67,102 @@ def line_evals(lines, layers, dvs):
     return tot, per
 
 
-def cpu_baseline(lines, layers, dvs, conts, per_layer_evals, budget_s=20.0):
-    """Oracle (sequential C restatement, one core) on a bounded sample of layers."""
+# ---- CPU baseline: the oracle on every host core (one process per core) -------------
+_W = {}
+
+
+def _cpu_init():
     from oracle.py import Oracle
-    order = [k for s in range(10) for k in range(s, len(layers), 10)]
-    t_tot, e_tot, used = 0.0, 0, []
-    orc = Oracle(lines)
-    for k in order:
+    lines, layers, dvs, conts = workload()
+    _W.update(layers=layers, dvs=dvs, conts=conts, orc=Oracle(lines))
+
+
+def _cpu_ready(_):
+    time.sleep(0.2)
+    return os.getpid()
+
+
+def _cpu_layer(k):
+    """One layer of the OD workload on the oracle; returns (k, seconds)."""
+    o = _W["orc"]
+    o.jrad = 1
+    t0 = time.perf_counter()
+    o.odlay(_W["layers"][k], V1, V2, _W["dvs"][k], continuum=_W["conts"][k])
+    return k, time.perf_counter() - t0
+
+
+def _cpu_od_job(job):
+    """One OD variant of a Jacobian block on the oracle, re-gridded to dvout."""
+    lay, a, b, dv, dvout, cont, jrad = job
+    o = _W["orc"]
+    o.jrad = jrad
+    return o.odlay(lay, a, b, dv, dvout=dvout, continuum=cont)[0]
+
+
+def cpu_workers():
+    """Worker processes for the CPU baseline: every host core, at most 16 (the host CPU
+    share of one GPU on the benchmark machine)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_pool(n):
+    import multiprocessing as mp
+    pool = mp.get_context("spawn").Pool(n, initializer=_cpu_init)
+    pool.map(_cpu_ready, range(4 * n), chunksize=1)      # every worker up and initialised
+    return pool
+
+
+def cpu_baseline(pool, nworkers, layers, per_layer_evals):
+    """The oracle (sequential C restatement, -O2, one layer per process at a time) on
+    all 60 layers of the workload over the full range, the costliest layers first.
+    Returns (line-evals/s over the wall time, line-evals/s per core, seconds)."""
+    order = sorted(range(len(layers)), key=lambda k: -layers[k].pave ** -0.5)
+    t0 = time.perf_counter()
+    res = list(pool.imap_unordered(_cpu_layer, order, chunksize=1))
+    wall = time.perf_counter() - t0
+    busy = sum(t for _, t in res)
+    e = sum(per_layer_evals)
+    return e / wall, e / busy, wall
+
+
+def cpu_jacobian(pool, layers, dvs, t_lev, p_lev, q_lev, blocks, lin=1, emis=0.98, tskin=295.0):
+    """The Jacobian block pipeline on the CPU: the (3 + M) x L oracle ODs of each block
+    spread over the pool, then the oracle's Jacobians and level assembly (one core).
+    Returns seconds per block."""
+    from clblm_amd import workloads as W
+    from clblm_amd.jacobian import od_variants
+    from oracle import py as O
+    mols = sorted(q_lev)
+    L = len(layers)
+    t_lay = np.array([l.tave for l in layers])
+    up, lo, wu, wl = O.level2layer(p_lev, t_lev, t_lay, np.array([q_lev[m] for m in mols]))
+    dvout = min(dvs)
+    times = []
+    for a, b in blocks:
+        jobs, jr = [], 1
+        for k, (lv, cont) in enumerate(od_variants(layers, mols)):
+            if a / (lv.tave / 1.4387752) < 5.0 and jr != -1:   # the SAVEd JRAD, successive calls
+                jr = 0
+            c = W.continuum_absrb(lv, a, b)[0] if cont else None
+            jobs.append((lv, a, b, dvs[k % L], dvout, c, jr))
         t0 = time.perf_counter()
-        orc.odlay(layers[k], V1, V2, dvs[k], continuum=conts[k])
-        t_tot += time.perf_counter() - t0
-        e_tot += per_layer_evals[k]
-        used.append(k)
-        if t_tot > budget_s:
-            break
-    return e_tot / t_tot, used, t_tot
+        ods = pool.map(_cpu_od_job, jobs, chunksize=1)
+        od_var = np.stack(ods).reshape(3 + len(mols), L, -1)
+        rad, dtau, dtlev, _, _ = O.jacobians(od_var[0], t_lay, t_lev, a, dvout, emis=emis, tskin=tskin, lin=lin,
+                                             dt_dtup=up, dt_dtlo=lo)
+        O.level_jac(od_var, dtau, dtlev, [l.wtot for l in layers], t_lay, 1.0, 0.01, up, lo, wu, wl)
+        times.append(time.perf_counter() - t0)
+    return float(np.mean(times))
 
 
 def run_od(ctx, h, layers, pars, steps, warmup):
@@ -128,9 +209,9 @@ def run_graph(ctx, h, layers, pars, steps, warmup, ref):
     return dt, float((np.abs(od - ref) / np.maximum(np.abs(ref), 1e-300)).max())
 
 
-def run_jacobian(ctx, h, layers, dvs, precision, block=100.0):
+def run_jacobian(ctx, h, layers, dvs, precision, block=100.0, compare=None):
     """Radiance + dR/dT_lev + dR/dln q_lev,m over V1..V2 in spectral blocks; returns
-    (seconds, spectra, points per spectrum, blocks)."""
+    (seconds, spectra, points per spectrum, blocks, first block's outputs on the host)."""
     import torch
     from clblm_amd import workloads as W
     from clblm_amd.jacobian import UpwellingJacobian
@@ -139,7 +220,8 @@ def run_jacobian(ctx, h, layers, dvs, precision, block=100.0):
                             p_lev=W.level_pressures(nlay, 0.1), q_lev=W.level_densities(nlay, 0.1),
                             precision=precision, continuum=lambda l, a, b: W.continuum_absrb(l, a, b))
     edges = np.arange(V1, V2 + 0.5 * block, block)
-    jac.block(edges[0], edges[1])          # warm-up
+    r = jac.block(edges[0], edges[1])          # warm-up; its outputs are kept for the error figures
+    first = {k: r[k].cpu().numpy() for k in ("rad", "jac_t", "jac_lnq")}
     torch.cuda.synchronize()
     npts = 0
     t0 = time.perf_counter()
@@ -150,7 +232,12 @@ def run_jacobian(ctx, h, layers, dvs, precision, block=100.0):
     t = time.perf_counter() - t0
     ctx.sync()
     spectra = (nlay + 1) * (1 + len(jac.mols))    # dR/dT_lev and dR/dln q_lev of every level
-    return t, spectra, npts, len(edges) - 1
+    return t, spectra, npts, len(edges) - 1, first
+
+
+def rel_err(a, b):
+    """max |a - b| / max |b| (the FP64 reference b)."""
+    return float(np.abs(a - b).max() / np.abs(b).max())
 
 
 def measured_traffic(kernel):
@@ -252,19 +339,59 @@ def main():
     if prec != FP64 and not args.no_fp64:
         if not args.no_autotune:
             ctx.autotune(h, layers, mk(FP64), keys=[_lib.TUNE_F4_POINTS_FP64], reps=2)
-        dt64, _, _, _, _ = run_od(ctx, h, layers, mk(FP64), args.steps, args.warmup)
+        dt64, _, _, _, od64 = run_od(ctx, h, layers, mk(FP64), args.steps, args.warmup)
+        pt = np.abs(od_eager - od64) / np.maximum(np.abs(od64), 1e-300)
         res["fp64_gpu"] = {"value": E / dt64, "unit": "line-evals/s", "ms_per_step": dt64 * 1e3,
-                           "fp32c_over_fp64": dt64 / dt}
+                           "fp32c_over_fp64": dt64 / dt,
+                           "fp32c_od_err": {"max_rel_per_point": float(pt.max()),
+                                            "max_abs_over_max": rel_err(od_eager, od64),
+                                            "bar": 1e-5}}
     if not args.no_jac:
-        t, spectra, npts, nblk = run_jacobian(ctx, h, layers, dvs, prec)
+        t, spectra, npts, nblk, first = run_jacobian(ctx, h, layers, dvs, prec)
         res["jacobian"] = {"spectra_per_s": spectra / t, "points_per_s": spectra * npts / t, "seconds": t,
                            "spectra": spectra, "points_per_spectrum": npts, "blocks": nblk,
                            "precision": args.precision}
+        if prec != FP64 and not args.no_fp64:
+            t64, _, _, _, first64 = run_jacobian(ctx, h, layers, dvs, FP64)
+            res["jacobian"]["fp64_gpu"] = {"spectra_per_s": spectra / t64, "seconds": t64,
+                                           "fp32c_over_fp64": t64 / t}
+            rmax = float(np.abs(first64["rad"]).max())
+            dq = float(np.abs(first["jac_lnq"] - first64["jac_lnq"]).max())
+            res["jacobian"]["fp32c_err_vs_fp64"] = {
+                "block": [V1, V1 + 100.0], "radiance": rel_err(first["rad"], first64["rad"]),
+                "dR_dT_lev": rel_err(first["jac_t"], first64["jac_t"]),
+                "dR_dlnq_lev": rel_err(first["jac_lnq"], first64["jac_lnq"]),
+                "dR_dlnq_lev_over_max_radiance": dq / rmax,
+                "max_dR_dlnq_over_max_radiance": float(np.abs(first64["jac_lnq"]).max()) / rmax,
+                "measure": "max |FP32C - FP64| / max |FP64| per quantity; dR/dln q also over the radiance "
+                           "(same units): above an opaque troposphere the isothermal upper layers leave dR/dln q "
+                           "at rounding level, where the first ratio compares noise", "bar": 1e-4}
     if not args.no_cpu:
-        v, used, t = cpu_baseline(lines, layers, dvs, conts, per)
-        res["cpu_baseline"] = {"value": v, "unit": "line-evals/s", "cores": 1, "kind": "port",
-                               "sample": "oracle (sequential C restatement of ODLAY, -O2) on layers %s of the same "
-                                         "workload, full 500-2500 cm-1 range, %.1f s" % (used, t)}
+        from clblm_amd import workloads as W
+        nw = cpu_workers()
+        pool = cpu_pool(nw)
+        try:
+            v, v1core, wall = cpu_baseline(pool, nw, layers, per)
+            res["cpu_baseline"] = {"value": v, "unit": "line-evals/s", "cores": nw, "kind": "port",
+                                   "per_core": v1core, "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+                                   "sample": "oracle (sequential C restatement of ODLAY, -O2), %d processes, one "
+                                             "layer each at a time, all 60 layers of the same workload over "
+                                             "500-2500 cm-1: %.1f s wall" % (nw, wall)}
+            if not args.no_jac:
+                nlay = len(layers)
+                blocks = [(V1, V1 + 100.0), (1500.0, 1600.0)]
+                tb = cpu_jacobian(pool, layers, dvs, W.level_temperatures(nlay, 0.1), W.level_pressures(nlay, 0.1),
+                                  W.level_densities(nlay, 0.1), blocks)
+                nblk = res["jacobian"]["blocks"]
+                res["jacobian"]["cpu_baseline"] = {
+                    "spectra_per_s": res["jacobian"]["spectra"] / (tb * nblk), "seconds_per_block": tb,
+                    "cores": nw, "kind": "port",
+                    "sample": "oracle ODLAY for the (3+7)x60 OD variants over %d processes, then the oracle's "
+                              "Jacobians and level assembly on one core, blocks %s; the %d-block time is the "
+                              "mean block time x %d" % (nw, blocks, nblk, nblk)}
+        finally:
+            pool.terminate()
+            pool.join()
     print(json.dumps(res))
 
 

@@ -29,6 +29,26 @@ from ._lib import FP32C, check
 from .odlay import Context
 
 
+def od_variants(layers, mols, dt=1.0, dw_frac=0.01):
+    """The (3 + M) x L layer-OD jobs of one Jacobian block, variant-major, as
+    (layer, takes the continuum): the layers (pathOD_up_dn_sun :796-838), the layers at
+    Tave + dt (drv_LevelTemp :1005-1009), at Wtot + dw_frac Wtot (:1018-1024), and for
+    each Jacobian molecule the layers with every other amount zero (drv_LevelConc
+    :1156-1166; only water keeps the continuum, whose absorber is water)."""
+    out = []
+    for lay in layers:
+        out.append((lay, True))
+    for lay in layers:
+        out.append((dataclasses.replace(lay, tave=lay.tave + dt), True))
+    for lay in layers:
+        out.append((dataclasses.replace(lay, wtot=lay.wtot + dw_frac * lay.wtot), True))
+    for m in mols:
+        for lay in layers:
+            w = {mm: (v if mm == m else 0.0) for mm, v in lay.w.items()}
+            out.append((dataclasses.replace(lay, w=w), m == 1))
+    return out
+
+
 class UpwellingJacobian:
     """Radiance and level Jacobians of one scene.
 
@@ -76,19 +96,7 @@ class UpwellingJacobian:
                        wu=d(wu.reshape(-1)) if M else None, wl=d(wl.reshape(-1)) if M else None)
 
     def _variants(self):
-        """The (3 + M) x L layer-OD jobs of one block, variant-major."""
-        out = []
-        for lay in self.layers:
-            out.append((lay, True))
-        for lay in self.layers:
-            out.append((dataclasses.replace(lay, tave=lay.tave + self.dt), True))
-        for lay in self.layers:
-            out.append((dataclasses.replace(lay, wtot=lay.wtot + self.dw_frac * lay.wtot), True))
-        for m in self.mols:
-            for lay in self.layers:
-                w = {mm: (v if mm == m else 0.0) for mm, v in lay.w.items()}
-                out.append((dataclasses.replace(lay, w=w), m == 1))
-        return out
+        return od_variants(self.layers, self.mols, self.dt, self.dw_frac)
 
     def _jobs(self, v1, v2):
         jobs, pars = [], []

@@ -17,6 +17,7 @@ algebra, clblm_ODLAY.f90:1805-1864,1928-1935):
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 from typing import List, Optional
 
@@ -263,16 +264,25 @@ def continuum_absrb(layer: Layer, v1: float, v2: float, seed=1001, nknots=50):
     (clblm_ODLAY.f90:881-896): V1ABS = INT(V1)-3, V2ABS = INT(V2+3.5).
     log10 of the coefficient is a natural cubic spline through ``nknots`` seeded
     knots in [-27, -25] cm2/molec; the absorber column is the layer's H2O."""
-    from scipy.interpolate import CubicSpline
-    rng = np.random.default_rng(seed + 7919)
     v1abs = float(int(v1)) - (1.0 if v1 < 0 else 0.0) - 3.0
     v2abs = float(int(v2 + 3.0 + 0.5))
     npt = int((v2abs - v1abs) / 1.0 + 1.5)
+    return _continuum_coef(v1abs, v2abs, npt, seed, nknots) * layer.w.get(1, 0.0), v1abs, npt
+
+
+@functools.lru_cache(maxsize=256)
+def _continuum_coef(v1abs, v2abs, npt, seed, nknots):
+    """The continuum coefficient (cm2/molec) on v1abs + i, i < npt: the seeded spline
+    depends on the range and the seed only, so it is evaluated once per block."""
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(seed + 7919)
     knots = np.linspace(v1abs, v2abs, nknots)
     vals = rng.uniform(-27.0, -25.0, nknots)
     spl = CubicSpline(knots, vals, bc_type="natural")
     grid = v1abs + np.arange(npt, dtype=np.float64)
-    return (10.0 ** spl(grid)) * layer.w.get(1, 0.0), v1abs, npt
+    out = 10.0 ** spl(grid)
+    out.setflags(write=False)
+    return out
 
 
 # --------------------------------------------------------------------------
