@@ -17,7 +17,9 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune", "clblm_scanfn",
            "clblm_linelist_broadening", "clblm_rt_down", "clblm_tape3_read", "clblm_linelist_load_tape3",
            "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy",
-           "clblm_rt_jac_noalloc"]
+           "clblm_rt_jac_noalloc", "clblm_calculate_dv", "clblm_set_dv", "clblm_interp_spectrum",
+           "clblm_interp_to_finest", "clblm_rt_up_grids"]
+GRID_EXACT, GRID_FIXED_RATIO, GRID_UNIFORM = -1, 0, 1
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_CAPACITY = 0, 1, 2, 3, 4
 OD_SPECIES_BROAD = 1
 TUNE_F4_XCD_CHUNK = 1
@@ -91,6 +93,14 @@ def load():
     lib.clblm_rt_jac.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double, P, P, P, P, P,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, P, P, P, P, P, P]
     lib.clblm_rt_jac_noalloc.argtypes = lib.clblm_rt_jac.argtypes
+    D, I64 = ctypes.c_double, ctypes.c_int64
+    lib.clblm_calculate_dv.argtypes = [ctypes.c_int, P, P, P, P, D, D, D, P]
+    lib.clblm_set_dv.argtypes = [ctypes.c_int, P, P, P, P, D, D, D, D, D, ctypes.c_int, D, P, P, P]
+    lib.clblm_interp_spectrum.argtypes = [P, P, I64, D, D, P, I64, D, P]
+    lib.clblm_interp_to_finest.argtypes = [P, ctypes.c_int, D, P, P, P, P, ctypes.POINTER(D),
+                                           ctypes.POINTER(I64), P]
+    lib.clblm_rt_up_grids.argtypes = [P, ctypes.c_int, D, P, P, P, P, P, P, D, D, D, I64, ctypes.c_int, D, P,
+                                      ctypes.POINTER(D), ctypes.POINTER(I64), P]
     lib.clblm_level_jac.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P, P, P, ctypes.c_double,
                                     ctypes.c_double, P, P, P, P, P, P, P]
     lib.clblm_fft_scan.argtypes = [P, P, ctypes.c_int64] + [ctypes.c_double] * 5 + [ctypes.c_int] + \

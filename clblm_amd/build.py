@@ -3,7 +3,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("clblm_od.hip", "clblm_rt.hip", "clblm_scan.hip", "clblm_tape3.cpp")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("clblm_od.hip", "clblm_rt.hip", "clblm_scan.hip", "clblm_tape3.cpp", "clblm_setdv.cpp")]
 OUT = os.path.join(HERE, "libclblm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -28,6 +28,7 @@
 #include "../../include/clblm_hip.h"
 #include "clblm_common.h"
 #include "seq_ramp.h"
+#include "spectrum_interp.h"
 
 using namespace clblm;
 
@@ -2119,91 +2120,14 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
 // blocks of 2400 output points with a sequentially accumulated fraction P, whose
 // rounding the reference's quantised coefficient index depends on; one lane walks
 // one block.
-__device__ inline int regrid_type(double DVin, double DVout, double *ratio, int *lenIn, int *lenOut) {
-    const double TOL = 1.e-8;
-    double dvRatio = DVout / DVin;
-    if (fabs(DVout - DVin) < TOL) dvRatio = 1.;
-    const double hi2lo[5] = {2. / 1., 3. / 2., 4. / 3., 5. / 4., 6. / 5.};
-    const double lo2hi[5] = {1. / 2., 2. / 3., 3. / 4., 4. / 5., 5. / 6.};
-    bool anyhi = false, anylo = false;
-    for (int k = 0; k < 5; k++) {
-        if (fabs(dvRatio - hi2lo[k]) < TOL) anyhi = true;
-        if (fabs(dvRatio - lo2hi[k]) < TOL) anylo = true;
-    }
-    *ratio = dvRatio;
-    if (fabs(dvRatio - 1.) < 2.220446049250313e-16) { *lenIn = 1; *lenOut = 1; return 0; }
-    if (anyhi) { *lenIn = (int)(DVout / (DVout - DVin) + 0.5); *lenOut = *lenIn - 1; return *lenOut - 1; }
-    if (anylo) { *lenIn = (int)(DVout / (DVin - DVout) + 0.5); *lenOut = *lenIn + 1; return *lenOut - 1; }
-    *lenIn = 0; *lenOut = 0;
-    return -1;
-}
-
 __global__ void k_regrid(const Job *jobs, const int *lay_of_unit, const int64_t *unit_first,
                          const double *nat, double *out, int64_t nunits) {
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunits) return;
     int l = lay_of_unit[u];
     const Job &J = jobs[l];
-    int64_t ul = u - unit_first[l];
-    const double *X = nat + J.nat_off - 1;  // X[1..nnat]
-    const int64_t NIN = J.nnat;
-    auto ext = [&](int64_t i) -> double {  // spInExt
-        if (i >= 1 && i <= NIN) return X[i];
-        if (i == 0 || i == -1) return 2.0 * X[1] - X[2];
-        if (i == NIN + 1 || i == NIN + 2) return 2.0 * X[NIN] - X[NIN - 1];
-        return 0.;
-    };
-    const double DVin = J.dv, DVout = J.dvout, V1 = J.v1;
-    double dvRatio;
-    int lenIn, lenOut;
-    int ITYPE = regrid_type(DVin, DVout, &dvRatio, &lenIn, &lenOut);
-    double *o = out + J.out_off - 1;  // o[1..npts]
-    if (ITYPE == 0) {
-        int64_t I = ul + 1;
-        int64_t step = (int64_t)llround(dvRatio);
-        o[I] = ext(1 - step + I * step);
-    } else if (ITYPE > 0) {
-        int64_t I = ul + 1;
-        int64_t nl = (I - 1) % lenOut + 1, m = (I - 1) / lenOut;
-        int64_t iIn = nl - lenIn;
-        if (nl > 1 && DVin > DVout) iIn = iIn - 1;
-        iIn += (m + 1) * lenIn;
-        if (nl == 1) {
-            o[I] = ext(iIn);
-        } else {
-            double AP = 1.0 / ((double)ITYPE + 1.0);
-            double pp = AP * (double)(nl - 1);
-            if (DVin > DVout) pp = 1.0 - pp;
-            double B1 = -pp * (pp - 1.0) * (pp - 2.0) / 6.0;
-            double B2 = (pp * pp - 1.0) * (pp - 2.0) * 0.5;
-            double B3 = -pp * (pp + 1.0) * (pp - 2.0) * 0.5;
-            double B4 = pp * (pp * pp - 1.0) / 6.0;
-            o[I] = B1 * ext(iIn - 1) + B2 * ext(iIn) + B3 * ext(iIn + 1) + B4 * ext(iIn + 2);
-        }
-    } else {
-        const double XNUMCF = 201.;
-        int64_t J1 = ul * 2400 + 1;
-        double Vo = ul == 0 ? V1 : V1 + (double)(J1 - 1) * DVout;
-        int64_t iIn = (int64_t)((Vo - V1) / DVin + 1.00001);
-        double Vi = V1 + (double)(iIn - 1) * DVin;
-        double P = (Vo - Vi) / DVin;
-        int64_t J2 = J1 + 2400 - 1 < J.npts ? J1 + 2400 - 1 : J.npts;
-        for (int64_t j = J1; j <= J2; j++) {
-            int ip = (int)(P * XNUMCF + 1.00001);
-            double p = ((double)ip - 1.0) / (XNUMCF - 1.0);
-            double pp = p * p;
-            double A1 = -p / 2.0 * ((1 - p) * (1 - p));
-            double A2 = 1.0 - pp * (3.0 - 2.0 * p) + pp / 2.0 * (1.0 - p);
-            double A3 = pp * (3.0 - 2.0 * p) + p / 2.0 * ((1.0 - p) * (1.0 - p));
-            double A4 = -pp / 2 * (1.0 - p);
-            o[j] = A1 * ext(iIn - 1) + A2 * ext(iIn) + A3 * ext(iIn + 1) + A4 * ext(iIn + 2);
-            P = P + dvRatio;
-            if (P >= 1.0) {
-                iIn = iIn + (int64_t)P;
-                P = P - (double)(int64_t)P;
-            }
-        }
-    }
+    clblm_interp::unit(nat + J.nat_off - 1, J.nnat, J.v1, J.dv, J.dvout, out + J.out_off - 1, J.npts,
+                       u - unit_first[l]);
 }
 
 // ====================================================================================
@@ -2378,6 +2302,8 @@ struct clblm_ctx {
     unsigned long long *d_work = nullptr;   // [conv evaluations, F4 evaluations]
     double *rt_state = nullptr;             // per-layer recursion states for the reverse sweep
     size_t rt_state_cap = 0;
+    double *grid_state = nullptr;           // merged and layer arrays of the merge on layer grids
+    size_t grid_state_cap = 0;
     // Batches are asynchronous: each call takes the next slot of a ring, which holds
     // its stage events and its device error word (copied to pinned host memory at
     // the end of the call).  A slot is read back when it is reused, by clblm_sync,
@@ -2857,14 +2783,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
             rg_first[l] = (int64_t)rg_lay.size();
             const Job &Jl = B.jobs[l];
             if (!(Jl.dvout > 0.)) continue;
-            double ratio = Jl.dvout / Jl.dv;
-            if (std::fabs(Jl.dvout - Jl.dv) < 1e-8) ratio = 1.;
-            bool aligned = std::fabs(ratio - 1.) < 2.220446049250313e-16;
-            for (int k = 0; k < 5; k++) {
-                double a = (k + 2.) / (k + 1.), b = (k + 1.) / (k + 2.);
-                if (std::fabs(ratio - a) < 1e-8 || std::fabs(ratio - b) < 1e-8) aligned = true;
-            }
-            int64_t units = aligned ? Jl.npts : (Jl.npts + 2399) / 2400;
+            int64_t units = clblm_interp::units(Jl.dv, Jl.dvout, Jl.npts);
             for (int64_t u = 0; u < units; u++) rg_lay.push_back(l);
         }
         o_rg = cv.take<int>(std::max<size_t>(1, rg_lay.size()));
@@ -3126,6 +3045,16 @@ extern "C" int clblm_rt_jac_impl(int nlay, int64_t npts, double v1, double dv, c
                                  double *d_rad, double *d_dr_dtau, double *d_dr_dtlev, double *d_dr_dtskin,
                                  double *d_dr_demis, double *d_state, void *stream);
 extern "C" int clblm_rt_scratch_init(int device);
+extern "C" int clblm_interp_spectrum_impl(const double *d_in, int64_t nin, double v1, double dvin, double *d_out,
+                                          int64_t nout, double dvout, void *stream);
+extern "C" int clblm_finest_grid(int nspec, const double *dv, const int64_t *nlim);
+extern "C" void clblm_rt_grids_final(int nlay, double v1, const double *dv, const int64_t *nlim, double surf_dv,
+                                     int64_t surf_nlim, double *dv_out, int64_t *nlim_out, int64_t *max_n);
+extern "C" int clblm_rt_up_grids_impl(int nlay, double v1, const double *dv, const int64_t *nlim,
+                                      const int64_t *od_off, const double *d_od, const double *t_lay,
+                                      const double *t_lev, double emis, double tskin, double surf_dv,
+                                      int64_t surf_nlim, int lin, double sec, double *scratch, int64_t max_n,
+                                      double *d_rad, void *stream);
 extern "C" int clblm_rt_jac_scratch(int device, int nlay, int64_t npts, double v1, double dv, const double *d_od,
                                     const double *d_tlay, const double *d_tlev, const double *d_dt_dtup,
                                     const double *d_dt_dtlo, double emis, double tskin, int lin, double sec_dn,
@@ -3205,6 +3134,7 @@ int clblm_ctx_destroy(clblm_ctx *c) {
     (void)hipFree(c->d_err);
     (void)hipFree(c->d_work);
     if (c->rt_state) (void)hipFree(c->rt_state);
+    if (c->grid_state) (void)hipFree(c->grid_state);
     for (auto &S : c->slot) {
         for (auto &e : S.ev) (void)hipEventDestroy(e);
         (void)hipEventDestroy(S.fork);
@@ -3764,6 +3694,79 @@ int clblm_rt_jac(clblm_ctx *c, int nlay, int64_t npts, double v1, double dv, con
                                c->rt_state, st);
     if (rc) return set_err(rc, "Jacobian kernel launch failed");
     HIPCHK(hipStreamSynchronize(st));   // the scratch states are reused by the next call
+    return 0;
+}
+
+int clblm_interp_spectrum(clblm_ctx *c, const double *d_in, int64_t nin, double v1, double dvin, double *d_out,
+                          int64_t nout, double dvout, void *stream) {
+    if (!c || !d_in || !d_out || nin < 2 || nout < 1) return set_err(CLBLM_ERR_ARG, "null argument or too few points");
+    if (d_in == d_out) return set_err(CLBLM_ERR_ARG, "input and output must not overlap");
+    if (!(dvin > 0.) || !(dvout > 0.)) return set_err(CLBLM_ERR_ARG, "DVs must be > 0");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (clblm_interp_spectrum_impl(d_in, nin, v1, dvin, d_out, nout, dvout, st))
+        return set_err(CLBLM_ERR_HIP, "interpolation kernel launch failed");
+    return 0;
+}
+
+int clblm_interp_to_finest(clblm_ctx *c, int nspec, double v1, const double *dv, const int64_t *nlim,
+                           const double *const *d_in, double *const *d_out, double *dv_out, int64_t *nlim_out,
+                           void *stream) {
+    if (!c || nspec <= 0 || !dv || !nlim || !dv_out || !nlim_out) return set_err(CLBLM_ERR_ARG, "null argument");
+    for (int k = 0; k < nspec; k++)
+        if (!(dv[k] > 0.) || nlim[k] < 2) return set_err(CLBLM_ERR_ARG, "every spectrum needs DV > 0 and 2 points");
+    const int loc = clblm_finest_grid(nspec, dv, nlim);
+    *dv_out = dv[loc];
+    *nlim_out = nlim[loc];
+    if (!d_out) return 0;
+    if (!d_in) return set_err(CLBLM_ERR_ARG, "null input pointers");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    for (int k = 0; k < nspec; k++) {
+        if (!d_in[k] || !d_out[k]) return set_err(CLBLM_ERR_ARG, "null spectrum pointer");
+        const bool same = k == loc || clblm_interp::same_grid(v1, dv[k], nlim[k], dv[loc], nlim[loc]);
+        if (same) {
+            // spectraHaveSameGrid: left as it is (copied where the output is elsewhere)
+            if (d_out[k] != d_in[k])
+                HIPCHK(hipMemcpyAsync(d_out[k], d_in[k], sizeof(double) * (size_t)nlim[k], hipMemcpyDeviceToDevice,
+                                      st));
+            continue;
+        }
+        if (d_out[k] == d_in[k]) return set_err(CLBLM_ERR_ARG, "a spectrum to interpolate must have its own output");
+        if (clblm_interp_spectrum_impl(d_in[k], nlim[k], v1, dv[k], d_out[k], nlim[loc], dv[loc], st))
+            return set_err(CLBLM_ERR_HIP, "interpolation kernel launch failed");
+    }
+    return 0;
+}
+
+int clblm_rt_up_grids(clblm_ctx *c, int nlay, double v1, const double *dv, const int64_t *nlim, const int64_t *od_off,
+                      const double *d_od, const double *t_lay, const double *t_lev, double surf_emis, double t_skin,
+                      double surf_dv, int64_t surf_nlim, int lin_in_tau, double sec_dn, double *d_rad, double *dv_out,
+                      int64_t *nlim_out, void *stream) {
+    if (!c || nlay <= 0 || !dv || !nlim || !dv_out || !nlim_out) return set_err(CLBLM_ERR_ARG, "null argument");
+    if (lin_in_tau < 0 || lin_in_tau > 2) return set_err(CLBLM_ERR_ARG, "lin_in_tau must be 0, 1 or 2");
+    if (!(surf_dv > 0.) || surf_nlim < 2) return set_err(CLBLM_ERR_ARG, "surface grid needs DV > 0 and 2 points");
+    for (int l = 0; l < nlay; l++)
+        if (!(dv[l] > 0.) || nlim[l] < 2) return set_err(CLBLM_ERR_ARG, "every layer needs DV > 0 and 2 points");
+    int64_t max_n = 0;
+    clblm_rt_grids_final(nlay, v1, dv, nlim, surf_dv, surf_nlim, dv_out, nlim_out, &max_n);
+    if (!d_rad) return 0;
+    if (!od_off || !d_od || !t_lay || !t_lev) return set_err(CLBLM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    const size_t bytes = sizeof(double) * 12 * (size_t)max_n;
+    if (c->grid_state_cap < bytes) {
+        if (c->grid_state) (void)hipFree(c->grid_state);
+        c->grid_state = nullptr;
+        c->grid_state_cap = 0;
+        HIPCHK(hipMalloc(&c->grid_state, bytes));
+        c->grid_state_cap = bytes;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = clblm_rt_up_grids_impl(nlay, v1, dv, nlim, od_off, d_od, t_lay, t_lev, surf_emis, t_skin, surf_dv,
+                                    surf_nlim, lin_in_tau, sec_dn, c->grid_state, max_n, d_rad, st);
+    if (rc) return set_err(rc, "radiance kernel launch failed");
+    HIPCHK(hipStreamSynchronize(st));   // the scratch arrays are reused by the next call
     return 0;
 }
 

@@ -13,11 +13,13 @@
 // kept in a scratch buffer between the two sweeps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <string>
 
 #include "../../include/clblm_hip.h"
+#include "spectrum_interp.h"
 
 namespace {
 constexpr double RADCN1 = 1.191042722E-12;
@@ -541,3 +543,187 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_level_jac_impl(
                        d_dw_dqup, d_dw_dqlo, d_jac_t, d_jac_q);
     return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
 }
+
+// ---------------------------------------------------------------------------------
+// Merge-up on the layers' own grids: mode2_lookDn_mergeUp / layerMerge_mode2
+// (clblm_noScattRT.f90:1287-1296, 1700-1794), where every layer's OD comes on its own
+// DV and interpToFinestGrid (clblm_Spectrum.f90:733-795) brings the merged radiance and
+// reflectance and the layer's transmittances and emissions onto the finest grid of
+// the set before each merge.  All grids start at V1.
+
+// interpSpectrum of X[0..nin) on v1 + i dvin onto o[0..nout) on v1 + i dvout.
+__global__ void __launch_bounds__(256) k_interp_spec(const double *__restrict__ X, int64_t nin, double v1,
+                                                     double dvin, double dvout, double *__restrict__ o,
+                                                     int64_t nout, int64_t nunits) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    clblm_interp::unit(X - 1, nin, v1, dvin, dvout, o - 1, nout, u);
+}
+
+// The surface (calcSurfThmEmis_array, clblm_Scene.f90): R = B(Tskin) eps, rho = 1 - eps.
+__global__ void __launch_bounds__(256) k_grid_surface(double *__restrict__ r, double *__restrict__ rho, int64_t n,
+                                                      double v1, double dv, double emis, double tskin) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = v1 + (double)i * dv;
+    r[i] = planck_x(v, tskin / RADCN2) * emis;
+    rho[i] = 1.0 - emis;
+}
+
+// One layer's view and down transmittances and its upward and downward emission
+// (layerEmis, clblm_EMLAY.f90:104-178) at point i of its own grid.
+__device__ inline void grid_layer_point(double v, double oda, double sec, double tbot, double ttop, double tave,
+                                        int lin, double *txa, double *txb, double *emup, double *emdn) {
+    const double odb = oda * sec;
+    *txa = exp(-oda);
+    *txb = exp(-odb);
+    const double bbar = planck_x(v, tave / RADCN2);
+    if (lin == 0) {
+        *emdn = (1 - *txb) * bbar;
+        *emup = *emdn;            // lyrEmUp = lyrEmDn (:1752)
+    } else if (lin == 1) {
+        double h, dh;
+        rlin(odb, &h, &dh);
+        const double ba = planck_x(v, tbot / RADCN2);
+        *emdn = (1 - *txb) * (ba + 2 * (bbar - ba) * h);
+        rlin(oda, &h, &dh);
+        const double bu = planck_x(v, ttop / RADCN2);
+        *emup = (1 - *txa) * (bu + 2 * (bbar - bu) * h);
+    } else {
+        double h, dh;
+        rlin(odb, &h, &dh);
+        const double ba = planck_x(v, tbot / RADCN2), bb = planck_x(v, ttop / RADCN2);
+        *emdn = (1 - *txb) * (ba + (bb - ba) * h);
+        rlin(oda, &h, &dh);
+        *emup = (1 - *txa) * (bb + (ba - bb) * h);
+    }
+}
+
+// The layer's quantities on its grid, merged at once (the merged arrays are on it).
+__global__ void __launch_bounds__(256) k_grid_layer_merge(const double *__restrict__ od, int64_t n, double v1,
+                                                          double dv, double sec, double tbot, double ttop,
+                                                          double tave, int lin, double *__restrict__ r,
+                                                          double *__restrict__ rho) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double txa, txb, emup, emdn;
+    grid_layer_point(v1 + (double)i * dv, od[i], sec, tbot, ttop, tave, lin, &txa, &txb, &emup, &emdn);
+    r[i] = emup + txa * (r[i] + emdn * rho[i]);
+    rho[i] = txa * txb * rho[i];
+}
+
+// The layer's quantities on its grid, stored for interpolation onto a finer one.
+__global__ void __launch_bounds__(256) k_grid_layer(const double *__restrict__ od, int64_t n, double v1, double dv,
+                                                    double sec, double tbot, double ttop, double tave, int lin,
+                                                    double *__restrict__ q) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    grid_layer_point(v1 + (double)i * dv, od[i], sec, tbot, ttop, tave, lin, q + i, q + n + i, q + 2 * n + i,
+                     q + 3 * n + i);
+}
+
+// R = EmUp + TxA (R + EmDn rho), rho = TxA TxB rho (:1773-1786) with the layer's
+// quantities already on the merged grid (q: TxA, TxB, EmUp, EmDn, n each).
+__global__ void __launch_bounds__(256) k_grid_merge(const double *__restrict__ q, int64_t n, double *__restrict__ r,
+                                                    double *__restrict__ rho) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double txa = q[i], txb = q[n + i], emup = q[2 * n + i], emdn = q[3 * n + i];
+    r[i] = emup + txa * (r[i] + emdn * rho[i]);
+    rho[i] = txa * txb * rho[i];
+}
+
+namespace {
+inline unsigned blocks_of(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+void launch_interp(const double *X, int64_t nin, double v1, double dvin, double dvout, double *o, int64_t nout,
+                   hipStream_t st) {
+    const int64_t nu = clblm_interp::units(dvin, dvout, nout);
+    hipLaunchKernelGGL(k_interp_spec, dim3(blocks_of(nu, 256)), dim3(256), 0, st, X, nin, v1, dvin, dvout, o, nout,
+                       nu);
+}
+}  // namespace
+
+// interpSpectrum of one device spectrum onto another grid from the same V1.
+extern "C" __attribute__((visibility("hidden"))) int clblm_interp_spectrum_impl(const double *d_in, int64_t nin,
+                                                                                double v1, double dvin,
+                                                                                double *d_out, int64_t nout,
+                                                                                double dvout, void *stream) {
+    launch_interp(d_in, nin, v1, dvin, dvout, d_out, nout, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}
+
+// The finest grid of a set (interpToFinestGrid: the first spectrum of smallest DV
+// gives DV and NLIM).
+extern "C" __attribute__((visibility("hidden"))) int clblm_finest_grid(int nspec, const double *dv,
+                                                                       const int64_t *nlim) {
+    int loc = 0;
+    for (int k = 1; k < nspec; k++)
+        if (dv[k] < dv[loc]) loc = k;
+    return loc;
+}
+
+// The grids the merge passes through: the surface grid, then before layer l the
+// finest of the merged grid and the layer's.  Returns the final grid.
+extern "C" __attribute__((visibility("hidden"))) void clblm_rt_grids_final(int nlay, double v1, const double *dv,
+                                                                          const int64_t *nlim, double surf_dv,
+                                                                          int64_t surf_nlim, double *dv_out,
+                                                                          int64_t *nlim_out, int64_t *max_n) {
+    double cdv = surf_dv;
+    int64_t cn = surf_nlim, mx = surf_nlim;
+    for (int l = 0; l < nlay; l++) {
+        mx = std::max(mx, nlim[l]);
+        if (dv[l] < cdv && !clblm_interp::same_grid(v1, dv[l], nlim[l], cdv, cn)) {   // the layer's TxA leads the set
+            cdv = dv[l];
+            cn = nlim[l];
+        }
+    }
+    *dv_out = cdv;
+    *nlim_out = cn;
+    *max_n = mx;
+}
+
+// scratch: 12 * max_n doubles (merged R and rho, their refined copies, the layer's
+// four quantities on its grid and on the merged grid).
+extern "C" __attribute__((visibility("hidden"))) int clblm_rt_up_grids_impl(
+    int nlay, double v1, const double *dv, const int64_t *nlim, const int64_t *od_off, const double *d_od,
+    const double *t_lay, const double *t_lev, double emis, double tskin, double surf_dv, int64_t surf_nlim, int lin,
+    double sec, double *scratch, int64_t max_n, double *d_rad, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    double *r = scratch, *rho = scratch + max_n, *r2 = scratch + 2 * max_n, *rho2 = scratch + 3 * max_n;
+    double *q = scratch + 4 * max_n, *qi = scratch + 8 * max_n;
+    double cdv = surf_dv;
+    int64_t cn = surf_nlim;
+    hipLaunchKernelGGL(k_grid_surface, dim3(blocks_of(cn, 256)), dim3(256), 0, st, r, rho, cn, v1, cdv, emis, tskin);
+    for (int l = 0; l < nlay; l++) {
+        const double ldv = dv[l];
+        const int64_t ln = nlim[l];
+        const double *od = d_od + od_off[l];
+        const double tbot = t_lev[l], ttop = t_lev[l + 1], tave = t_lay[l];
+        if (clblm_interp::same_grid(v1, ldv, ln, cdv, cn)) {
+            // one grid: no interpolation (the merged arrays keep their DV)
+            hipLaunchKernelGGL(k_grid_layer_merge, dim3(blocks_of(cn, 256)), dim3(256), 0, st, od, cn, v1, ldv, sec,
+                               tbot, ttop, tave, lin, r, rho);
+        } else if (ldv < cdv) {
+            // the layer's grid is the finest: the merged arrays are interpolated onto it
+            launch_interp(r, cn, v1, cdv, ldv, r2, ln, st);
+            launch_interp(rho, cn, v1, cdv, ldv, rho2, ln, st);
+            std::swap(r, r2);
+            std::swap(rho, rho2);
+            cdv = ldv;
+            cn = ln;
+            hipLaunchKernelGGL(k_grid_layer_merge, dim3(blocks_of(cn, 256)), dim3(256), 0, st, od, cn, v1, cdv, sec,
+                               tbot, ttop, tave, lin, r, rho);
+        } else {
+            // the merged grid is the finest (or as fine, with another length): the
+            // layer's four quantities are interpolated onto it
+            hipLaunchKernelGGL(k_grid_layer, dim3(blocks_of(ln, 256)), dim3(256), 0, st, od, ln, v1, ldv, sec, tbot,
+                               ttop, tave, lin, q);
+            for (int k = 0; k < 4; k++) launch_interp(q + k * ln, ln, v1, ldv, cdv, qi + k * cn, cn, st);
+            hipLaunchKernelGGL(k_grid_merge, dim3(blocks_of(cn, 256)), dim3(256), 0, st, qi, cn, r, rho);
+        }
+    }
+    if (hipMemcpyAsync(d_rad, r, sizeof(double) * (size_t)cn, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return CLBLM_ERR_HIP;
+    return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}
+

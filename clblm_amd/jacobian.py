@@ -151,6 +151,41 @@ class UpwellingJacobian:
                     dr_dtlev=dr_dtlev, od=od[0], od_var=od, n=n, grid=grid)
 
 
+def upwelling_radiance_grids(ctx, od, offsets, dvs, v1, t_lay, t_lev, emis=0.98, tskin=295.0, lin_in_tau=1,
+                             sec_dn=1.0, surf_dv=None, surf_nlim=None):
+    """Upwelling radiance of layers each on its own grid (clblm_rt_up_grids: the merge
+    of mode2_lookDn_mergeUp with interpToFinestGrid, src/clblm_src/clblm_noScattRT.f90:
+    1700-1794).  od: float64 cuda tensor of the layer ODs back to back, layer l at
+    offsets[l]..offsets[l+1] on v1 + i*dvs[l] (clblm_od_batch's output without DVout);
+    t_lay [L], t_lev [L+1] host sequences (K).  The surface grid defaults to layer 0's.
+    Returns (radiance tensor, its DV)."""
+    import torch
+    off = np.ascontiguousarray(offsets, np.int64)
+    L = off.size - 1
+    nl = np.ascontiguousarray(np.diff(off), np.int64)
+    dv = np.ascontiguousarray(dvs, np.float64)
+    tl, tv = np.ascontiguousarray(t_lay, np.float64), np.ascontiguousarray(t_lev, np.float64)
+    if dv.size != L or tl.size != L or tv.size != L + 1:
+        raise ValueError("dvs and t_lay need one value per layer, t_lev one per level")
+    if not od.is_cuda or od.dtype != torch.float64 or od.numel() < off[-1]:
+        raise ValueError("od must be a float64 cuda tensor holding every layer")
+    sdv = float(dv[0] if surf_dv is None else surf_dv)
+    sn = int(nl[0] if surf_nlim is None else surf_nlim)
+    dvo, no = ctypes.c_double(), ctypes.c_int64()
+    args = [ctx.ptr, L, ctypes.c_double(v1), dv.ctypes.data, nl.ctypes.data, off.ctypes.data]
+    check(ctx.lib.clblm_rt_up_grids(*args, None, tl.ctypes.data, tv.ctypes.data, ctypes.c_double(emis),
+                                    ctypes.c_double(tskin), ctypes.c_double(sdv), sn, int(lin_in_tau),
+                                    ctypes.c_double(sec_dn), None, ctypes.byref(dvo), ctypes.byref(no), None))
+    rad = torch.empty(no.value, dtype=torch.float64, device=od.device)
+    od = od.contiguous()
+    check(ctx.lib.clblm_rt_up_grids(*args, ctypes.c_void_p(od.data_ptr()), tl.ctypes.data, tv.ctypes.data,
+                                    ctypes.c_double(emis), ctypes.c_double(tskin), ctypes.c_double(sdv), sn,
+                                    int(lin_in_tau), ctypes.c_double(sec_dn), ctypes.c_void_p(rad.data_ptr()),
+                                    ctypes.byref(dvo), ctypes.byref(no),
+                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return rad, dvo.value
+
+
 def upwelling_radiance(ctx, od, t_lay, t_lev, v1, dv, emis=0.98, tskin=295.0, lin_in_tau=1, sec_dn=1.0):
     """Differentiable radiance R(OD, T_lay) (torch.autograd.Function on ROCm tensors)."""
     return _Radiance.apply(od, t_lay, ctx, t_lev, v1, dv, emis, tskin, lin_in_tau, sec_dn)

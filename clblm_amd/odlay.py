@@ -237,6 +237,41 @@ class Context:
         v2 = v1 + dv * (x.shape[-1] - 1)
         return self.scanfn(x, v1, dv, v1 + (m - 1) * dv / 2.0, v2 - (m - 1) * dv / 2.0, (m - 1) * dv, 1, boxcar_hw)
 
+    def interp_spectrum(self, x, v1, dvin, dvout, nout, stream=None):
+        """interpSpectrum (src/clblm_src/clblm_Spectrum.f90:326-590) of the float64 cuda
+        tensor x on v1 + i*dvin onto nout points on v1 + i*dvout."""
+        import torch
+        assert x.dtype == torch.float64 and x.is_cuda
+        x = x.contiguous()
+        out = torch.empty(int(nout), dtype=torch.float64, device=x.device)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        check(self.lib.clblm_interp_spectrum(self.ptr, ctypes.c_void_p(x.data_ptr()), int(x.numel()), float(v1),
+                                             float(dvin), ctypes.c_void_p(out.data_ptr()), int(nout), float(dvout),
+                                             ctypes.c_void_p(stream)))
+        return out
+
+    def interp_to_finest(self, spectra, dvs, v1, stream=None):
+        """interpToFinestGrid (clblm_Spectrum.f90:733-795) of float64 cuda tensors from v1
+        on dvs: returns (the spectra on the finest grid, its DV)."""
+        import torch
+        n = len(spectra)
+        xs = [t.contiguous() for t in spectra]
+        dv = np.ascontiguousarray(dvs, np.float64)
+        nl = np.array([t.numel() for t in xs], np.int64)
+        dvo, no = ctypes.c_double(), ctypes.c_int64()
+        check(self.lib.clblm_interp_to_finest(self.ptr, n, float(v1), dv.ctypes.data, nl.ctypes.data, None, None,
+                                              ctypes.byref(dvo), ctypes.byref(no), None))
+        outs = [torch.empty(no.value, dtype=torch.float64, device=x.device) for x in xs]
+        pin = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
+        pout = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        check(self.lib.clblm_interp_to_finest(self.ptr, n, float(v1), dv.ctypes.data, nl.ctypes.data,
+                                              ctypes.cast(pin, ctypes.c_void_p), ctypes.cast(pout, ctypes.c_void_p),
+                                              ctypes.byref(dvo), ctypes.byref(no), ctypes.c_void_p(stream)))
+        return outs, dvo.value
+
     def tuning(self, key, value=0):
         """Sets launch-shape parameter ``key`` (``_lib.TUNE_*``) to ``value`` (0: query
         only); returns the previous setting."""

@@ -217,6 +217,59 @@ int clblm_rt_down(clblm_ctx *ctx, int nlay, int64_t npts, double v1, double dv, 
                   const double *d_tlay, const double *d_tlev, int lin_in_tau, const double *d_sol_toa,
                   double *d_rad, double *d_tx, void *stream);
 
+/* ---- layer grids (setDV, clblm_setDV.f90:114-382) ---------------------------------
+ * Host only.  Layer l has pressure pave[l] (hPa), temperature tave[l] (K), broadening
+ * column wtot[l] and water column w_h2o[l] (molec/cm2).
+ * clblm_calculate_dv: calculateDV (:221-271), the exact DV of every layer for the mean
+ * wavenumber vbar, ALFAL0 and SAMPLE.
+ * clblm_set_dv: setDV_struct (:114-215).  grid_type CLBLM_GRID_EXACT (-1): the exact
+ * DVs; CLBLM_GRID_FIXED_RATIO (0): fixDVRatio (:277-349), the first DV to three
+ * significant figures with an even last digit and each next one the previous times
+ * 1 or n/(n+1) so that adjacent grids nest (CLBLM_ERR_ARG where the exact DVs of two
+ * adjacent layers differ by more than 2.5x, where the reference stops);
+ * CLBLM_GRID_UNIFORM (1): the exact DVs and a uniform output DV.  For types 0 and 1
+ * a user_dv > 0 within 20 % of the last layer's DV rescales all of them (DVSET,
+ * :356-382).  Outputs: dvnormal[nlay] (ALFAL0), dvnarrow[nlay] (alfal0_fine) and
+ * *dvout: user_dv (> 0) or the smallest dvnarrow for type 1, -999.9 otherwise. */
+#define CLBLM_GRID_EXACT (-1)
+#define CLBLM_GRID_FIXED_RATIO 0
+#define CLBLM_GRID_UNIFORM 1
+int clblm_calculate_dv(int nlay, const double *pave, const double *tave, const double *wtot, const double *w_h2o,
+                       double vbar, double alfal0, double sample, double *dv);
+int clblm_set_dv(int nlay, const double *pave, const double *tave, const double *wtot, const double *w_h2o,
+                 double v1, double v2, double alfal0, double alfal0_fine, double sample, int grid_type,
+                 double user_dv, double *dvnormal, double *dvnarrow, double *dvout);
+
+/* ---- spectra on different grids (clblm_Spectrum.f90) -----------------------------
+ * clblm_interp_spectrum: interpSpectrum_array2array (:326-590) of the device spectrum
+ * d_in[0..nin) on v1 + i dvin onto d_out[0..nout) on v1 + i dvout (nin >= 2; the
+ * reference's linear extrapolation for the two points past either end, zero beyond).
+ * clblm_interp_to_finest: interpToFinestGrid (:733-795) over nspec device spectra from
+ * one v1, spectrum k on dv[k] with nlim[k] points: *dv_out and *nlim_out receive the
+ * grid of the first spectrum of smallest DV (query with d_out == NULL); then every
+ * spectrum is written to d_out[k] on that grid (nlim_out points; d_out[k] may be
+ * d_in[k] for spectra already on it).  Enqueued on `stream`. */
+int clblm_interp_spectrum(clblm_ctx *ctx, const double *d_in, int64_t nin, double v1, double dvin, double *d_out,
+                          int64_t nout, double dvout, void *stream);
+int clblm_interp_to_finest(clblm_ctx *ctx, int nspec, double v1, const double *dv, const int64_t *nlim,
+                           const double *const *d_in, double *const *d_out, double *dv_out, int64_t *nlim_out,
+                           void *stream);
+
+/* Upwelling radiance at the top of nlay layers each on its own grid, as the reference
+ * merges them (mode2_lookDn_mergeUp / layerMerge_mode2, clblm_noScattRT.f90:1287-1296,
+ * 1700-1794): layer l's OD (d_od + od_off[l], nlim[l] points on v1 + i dv[l], layer 0
+ * at the surface, e.g. from clblm_od_batch without DVout) gives its transmittances and
+ * emissions (layerEmis, the down path sec_dn times the view OD) on its grid; before
+ * each merge interpToFinestGrid brings the merged radiance and reflectance, or the
+ * layer's four arrays, onto the finer grid.  The surface, eps B(t_skin) and 1 - eps,
+ * starts on v1 + i surf_dv, surf_nlim points.  t_lay[nlay], t_lev[nlay+1] are HOST
+ * arrays (K).  Query with d_rad == NULL: *dv_out, *nlim_out give the final grid; then
+ * d_rad (device, *nlim_out doubles) is filled.  Synchronous. */
+int clblm_rt_up_grids(clblm_ctx *ctx, int nlay, double v1, const double *dv, const int64_t *nlim,
+                      const int64_t *od_off, const double *d_od, const double *t_lay, const double *t_lev,
+                      double surf_emis, double t_skin, double surf_dv, int64_t surf_nlim, int lin_in_tau,
+                      double sec_dn, double *d_rad, double *dv_out, int64_t *nlim_out, void *stream);
+
 /* ---- radiance and its derivatives ---------------------------------------------
  * Upwelling radiance at the top of nlay layers (layer 0 at the surface) on a
  * common grid v1 + i*dv, i < npts: the merge-up recursion of mode2_lookDn_mergeUp /

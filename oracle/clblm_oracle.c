@@ -1455,6 +1455,92 @@ void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, c
     }
 }
 
+/* spectraHaveSameGrid (clblm_Spectrum.f90) for two spectra from one V1: DV,
+ * V2 = V1 + DV (NLIM - 1) (CLBLM_Spectrum_init) and NLIM equal, to epsilon(1d0). */
+static int same_grid(double v1, double dva, int64_t na, double dvb, int64_t nb) {
+    const double EPS = 2.220446049250313e-16;
+    return fabs(dva - dvb) < EPS && na == nb &&
+           fabs((v1 + dva * (double)(na - 1)) - (v1 + dvb * (double)(nb - 1))) < EPS;
+}
+
+/* interpToFinestGrid_array, clblm_Spectrum.f90:799-866, for spectra from one V1: the
+ * first spectrum of smallest DV sets the grid; every other one not already on it is
+ * interpolated onto it (interpSpectrum).  out[k] receives spectrum k on its final
+ * grid; returns the index of the finest. */
+int orc_interp_to_finest(int nspec, double v1, const double *dv, const int64_t *nlim, const double *const *in,
+                         double *const *out) {
+    int loc = 0;
+    for (int k = 1; k < nspec; k++)
+        if (dv[k] < dv[loc]) loc = k;
+    for (int k = 0; k < nspec; k++) {
+        if (k == loc || same_grid(v1, dv[k], nlim[k], dv[loc], nlim[loc])) {
+            memmove(out[k], in[k], sizeof(double) * (size_t)nlim[k]);
+            continue;
+        }
+        orc_interp(in[k], dv[k], v1, (int)nlim[k], out[k], dv[loc], v1, (int)nlim[loc]);
+    }
+    return loc;
+}
+
+/* mode2_lookDn_mergeUp / layerMerge_mode2 (clblm_noScattRT.f90:1287-1296, 1700-1794)
+ * with every layer on its own grid: layer l's OD is od + off[l], nlim[l] points on
+ * v1 + i dv[l]; the surface starts on surf_dv, surf_nlim.  Before each merge
+ * interpToFinestGrid (mrgRadUp, mrgThmRefl, lyrTxA, lyrTxB, lyrEmUp, lyrEmDn) puts
+ * all six on the finest grid.  rad (capacity cap) receives the radiance; returns its
+ * length (*dv_out its DV), or -1. */
+int64_t orc_rt_up_grids(int nlay, double v1, const double *dv, const int64_t *nlim, const int64_t *off,
+                        const double *od, const double *t_lay, const double *t_lev, double emis, double tskin,
+                        double surf_dv, int64_t surf_nlim, int lin, double sec_dn, double *rad, int64_t cap,
+                        double *dv_out) {
+    int64_t mx = surf_nlim;
+    for (int l = 0; l < nlay; l++) mx = nlim[l] > mx ? nlim[l] : mx;
+    double *R = malloc(sizeof(double) * mx), *RH = malloc(sizeof(double) * mx);
+    double *Q = malloc(sizeof(double) * 4 * mx), *T = malloc(sizeof(double) * 6 * mx);
+    double cdv = surf_dv;
+    int64_t cn = surf_nlim;
+    for (int64_t i = 0; i < cn; i++) {
+        double v = v1 + (double)i * cdv;
+        R[i] = orc_planck(v, tskin) * emis;
+        RH[i] = 1.0 - emis;
+    }
+    for (int l = 0; l < nlay; l++) {
+        const double ldv = dv[l];
+        const int64_t ln = nlim[l];
+        const double *o = od + off[l];
+        double *txa = Q, *txb = Q + ln, *emup = Q + 2 * ln, *emdn = Q + 3 * ln;
+        for (int64_t i = 0; i < ln; i++) {
+            double v = v1 + (double)i * ldv;
+            double oda = o[i], odb = oda * sec_dn;
+            txa[i] = exp(-oda);
+            txb[i] = exp(-odb);
+            emdn[i] = layer_emis(v, t_lev[l], t_lev[l + 1], t_lay[l], txb[i], odb, lin);
+            emup[i] = lin != 0 ? layer_emis(v, t_lev[l + 1], t_lev[l], t_lay[l], txa[i], oda, lin) : emdn[i];
+        }
+        double dvs[6] = {cdv, cdv, ldv, ldv, ldv, ldv};
+        int64_t ns[6] = {cn, cn, ln, ln, ln, ln};
+        const double *in[6] = {R, RH, txa, txb, emup, emdn};
+        double *outp[6];
+        for (int k = 0; k < 6; k++) outp[k] = T + k * mx;
+        int loc = orc_interp_to_finest(6, v1, dvs, ns, in, outp);
+        /* the merged arrays take the grid they end on (their own when already on it) */
+        int64_t rn = (loc == 0 || same_grid(v1, cdv, cn, dvs[loc], ns[loc])) ? cn : ns[loc];
+        if (!(loc == 0 || same_grid(v1, cdv, cn, dvs[loc], ns[loc]))) cdv = dvs[loc];
+        cn = rn;
+        for (int64_t i = 0; i < cn; i++) {
+            double r = outp[0][i], rho = outp[1][i];
+            r = outp[4][i] + outp[2][i] * (r + outp[5][i] * rho);
+            rho = outp[2][i] * outp[3][i] * rho;
+            R[i] = r;
+            RH[i] = rho;
+        }
+    }
+    int64_t n = cn < cap ? cn : cap;
+    memcpy(rad, R, sizeof(double) * (size_t)n);
+    *dv_out = cdv;
+    free(R); free(RH); free(Q); free(T);
+    return cn;
+}
+
 /* ==== Jacobians ================================================================
  * level2layer (clblm_noScattJac.f90:1208-1279): derivatives of the layer mean
  * temperature and of ln(layer amount) with respect to the bounding levels' T and

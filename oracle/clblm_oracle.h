@@ -87,6 +87,12 @@ void orc_layer_emis(int64_t npts, const double *od, double v1, double dv, double
                     int lin, double *out);
 void orc_rt_down(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
                  const double *t_lev, int lin, const double *sol, double *rad, double *tx);
+int orc_interp_to_finest(int nspec, double v1, const double *dv, const int64_t *nlim, const double *const *in,
+                         double *const *out);
+int64_t orc_rt_up_grids(int nlay, double v1, const double *dv, const int64_t *nlim, const int64_t *off,
+                        const double *od, const double *t_lay, const double *t_lev, double emis, double tskin,
+                        double surf_dv, int64_t surf_nlim, int lin, double sec_dn, double *rad, int64_t cap,
+                        double *dv_out);
 void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, const double *t_lay,
                const double *t_lev, double emis, double tskin, int lin, double sec_dn, double *rad);
 

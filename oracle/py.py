@@ -308,3 +308,49 @@ def prebox(x, v1, dv, boxcar_hw):
         return np.asarray(x, np.float64).copy(), v1, dv
     v2 = v1 + dv * (len(x) - 1)
     return scanfn_rect(x, v1, dv, v1 + (m - 1) * dv / 2.0, v2 - (m - 1) * dv / 2.0, (m - 1) * dv, boxcar_hw)
+
+
+def interp_to_finest(spectra, dvs, v1):
+    """orc_interp_to_finest: spectra (list of 1-D arrays) from v1 on dvs; returns (the
+    spectra on the finest grid, index of the finest)."""
+    L = lib()
+    n = len(spectra)
+    ins = [np.ascontiguousarray(s, np.float64) for s in spectra]
+    nl = np.array([s.size for s in ins], np.int64)
+    dv = np.ascontiguousarray(dvs, np.float64)
+    loc = int(np.argmin(dv))
+    outs = [np.zeros(max(int(nl[loc]), int(nl[k]))) for k in range(n)]
+    P = ctypes.c_void_p
+    L.orc_interp_to_finest.restype = ctypes.c_int
+    L.orc_interp_to_finest.argtypes = [ctypes.c_int, ctypes.c_double, P, P, P, P]
+    pin = (P * n)(*[s.ctypes.data for s in ins])
+    pout = (P * n)(*[o.ctypes.data for o in outs])
+    loc = L.orc_interp_to_finest(n, v1, dv.ctypes.data, nl.ctypes.data, pin, pout)
+    # every spectrum ends on the finest grid, so every one has its length
+    return [outs[k][:nl[loc]] for k in range(n)], loc
+
+
+def rt_up_grids(ods, dvs, v1, t_lay, t_lev, emis=0.98, tskin=295.0, surf_dv=None, surf_nlim=None, lin=1,
+                sec_dn=1.0):
+    """orc_rt_up_grids: ods (list of per-layer OD arrays on v1 + i dvs[l], layer 0 at the
+    surface); the surface grid defaults to layer 0's.  Returns (radiance, its DV)."""
+    L = lib()
+    nlay = len(ods)
+    nl = np.array([o.size for o in ods], np.int64)
+    off = np.concatenate([[0], np.cumsum(nl)[:-1]]).astype(np.int64)
+    od = np.ascontiguousarray(np.concatenate(ods), np.float64)
+    dv = np.ascontiguousarray(dvs, np.float64)
+    sdv = float(dv[0] if surf_dv is None else surf_dv)
+    sn = int(nl[0] if surf_nlim is None else surf_nlim)
+    cap = int(max(nl.max(), sn))
+    rad = np.zeros(cap)
+    dvo = ctypes.c_double()
+    D, P = ctypes.c_double, ctypes.c_void_p
+    L.orc_rt_up_grids.restype = ctypes.c_int64
+    L.orc_rt_up_grids.argtypes = [ctypes.c_int, D, P, P, P, P, P, P, D, D, D, ctypes.c_int64, ctypes.c_int, D, P,
+                                  ctypes.c_int64, ctypes.POINTER(D)]
+    tl = np.ascontiguousarray(t_lay, np.float64)
+    tv = np.ascontiguousarray(t_lev, np.float64)
+    n = L.orc_rt_up_grids(nlay, v1, dv.ctypes.data, nl.ctypes.data, off.ctypes.data, od.ctypes.data, tl.ctypes.data,
+                          tv.ctypes.data, emis, tskin, sdv, sn, lin, sec_dn, rad.ctypes.data, cap, ctypes.byref(dvo))
+    return rad[:n], dvo.value

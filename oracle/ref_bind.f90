@@ -7,7 +7,7 @@ module ref_bind
    use Module_ConstParam,      only: r8 => kind_r8, clblmMolNames, MXMOL, MXISOTPL
    use Module_ODLAY_CommonSub, only: MOLEC, RADFN, RADFNI, AVRAT, CGAUSS, CF1, CF2, CF3, CER
    use Module_TIPS,            only: TIPS_2003
-   use Module_Spectrum,        only: XINT, interpSpectrum, CLBLM_Spectrum, CLBLM_Spectrum_init
+   use Module_Spectrum,        only: XINT, interpSpectrum, CLBLM_Spectrum, CLBLM_Spectrum_init, interpToFinestGrid
    use Module_EMLAY,           only: planck, planckFnDrv, rlin, layerEmis
    use Module_FFT,             only: clblm_FFTSCN, Intpdr
    use Module_ScanFilter,      only: clblm_SCANFN
@@ -261,6 +261,38 @@ contains
       tx%spect(tx%indV1:tx%indV2) = exp(-od)
       call layerEmis(em, ta, tb, tave, tx, sod, lin)
       out = em%spect(em%indV1:em%indV1 + np - 1)
+   end subroutine
+
+   ! interpToFinestGrid (clblm_Spectrum.f90:799-866) of nspec spectra from v1: spectrum k
+   ! holds nlim(k) values at data(off(k)+1 ..) on dv(k).  On return spectrum k is on the
+   ! finest grid: nlim_out(k) values at out(k*cap+1 ..), dv_out(k).
+   subroutine ref_interp_to_finest(nspec, v1, dv, nlim, off, data, cap, out, nlim_out, dv_out) &
+         bind(C, name="ref_interp_to_finest")
+      integer(c_int), value :: nspec, cap
+      real(c_double), value :: v1
+      real(c_double), intent(in) :: dv(nspec), data(*)
+      integer(c_int64_t), intent(in) :: nlim(nspec), off(nspec)
+      real(c_double), intent(out) :: out(cap, nspec), dv_out(nspec)
+      integer(c_int64_t), intent(out) :: nlim_out(nspec)
+      type(CLBLM_Spectrum), allocatable :: sp(:)
+      real(r8) :: a
+      real :: d
+      integer :: k, n
+      allocate(sp(nspec))
+      a = v1
+      do k = 1, nspec
+         d = dv(k)
+         n = int(nlim(k))
+         call CLBLM_Spectrum_init(sp(k), a, d, n)
+         sp(k)%spect(sp(k)%indV1:sp(k)%indV2) = data(off(k) + 1:off(k) + n)
+      end do
+      call interpToFinestGrid(sp)
+      do k = 1, nspec
+         n = min(sp(k)%NLIM, cap)
+         nlim_out(k) = sp(k)%NLIM
+         dv_out(k) = sp(k)%DV
+         out(1:n, k) = sp(k)%spect(sp(k)%indV1:sp(k)%indV1 + n - 1)
+      end do
    end subroutine
 
    subroutine ref_rlin(od, hlf, dhlf) bind(C, name="ref_rlin")
