@@ -28,6 +28,7 @@ TUNE_F4_STREAM = 3
 TUNE_F4_POINTS_FP32C = 4
 TUNE_F4_POINTS_FP64 = 5
 TUNE_CONV_UNROLL = 6
+TUNE_CONV_SPLIT = 7
 
 
 class Layer(ctypes.Structure):

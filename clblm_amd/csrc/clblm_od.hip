@@ -58,6 +58,9 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_set_error(int code, c
 // ------------------------------------------------------------------------------------
 __constant__ double c_avrat[102], c_cgauss[102], c_cf1[102], c_cf2[102], c_cf3[102], c_cer[102];
 __constant__ int c_maxiso[NMOL];
+// SHAPEL's constants (clblm_ODLAY.f90:2822-2833): 1/pi and A(z), B(z) of the quadratic
+// continuations at z = 4, 16, 64: {recpi, a1, b1, a2, b2, a3, b3}
+__constant__ double c_shape[8];
 
 struct Tables {
     const double *f1, *f2, *f3, *fg, *xver;   // 1-based, NFMX+1 slots
@@ -472,19 +475,77 @@ __global__ void k_ranges(const Job *jobs, Panel *panels, int64_t npanel_tot, Lin
 }
 
 // ------------------------------------------------------------------------------------
-// Pass 1, FP64: CNVFNQ (clblm_ODLAY.f90:2067-2258) for the lines homed in one panel.
-// One lane per line walks its support in the reference's order (the table argument
+// Pass 1: CNVFNQ (clblm_ODLAY.f90:2067-2258) for the lines homed in one panel.  Each
+// lane walks part of one line's support in the reference's order (the table argument
 // ZF is accumulated exactly as the reference does); contributions land in LDS with
 // FP64 atomics and the panel's arrays go to HBM.  Local index 1..MAX is kept; lower
 // indices are never read by the reference's panel logic.
+// Work items.  A panel of fewer lines than lanes splits every line's walk into 2 or 4
+// consecutive parts, one per lane, so that its waves are filled (the upper layers'
+// panels hold about 80 lines for 256 lanes); a part starting inside the walk first
+// replays the ZF sums up to its first point, so the table indices stay the
+// reference's.  The parts of one line sit in neighbouring lanes, with equal lengths.
 // ------------------------------------------------------------------------------------
-// CONV_U: R1 points per step of the pass-1 walk (a launch-shape setting, 2, 4 or 8)
-
+// CONV_U: R1 points per step of the walk (a launch-shape setting, 2, 4 or 8)
 // CPL: the line list holds coupled lines (IFLG 1); the coupling walk is compiled
 // only into that instantiation, so that the common one keeps its occupancy
+
+// F2[iz] and F3[iz] of SHAPEL (clblm_ODLAY.f90:2822-2833, the tables of build_tables)
+// from their closed forms in FP32, for the FP32 line shapes of pass 1: on x = (iz-1) dx,
+// the difference of two quadratic continuations up to the inner boundary (iz <= lim),
+// then the Lorentzian minus the outer quadratic; zero from iz = NFPTS on.  Evaluating
+// them costs a few VALU operations against a gather from the tables (pass 1 waits on
+// those loads).
+struct Shape23 {
+    float recpi, pa, pb, qa, qb, dx;
+    int lim;
+};
+__device__ __forceinline__ float shape23(const Shape23 &c, int iz) {
+    const float x = (float)(iz - 1) * c.dx, xs = x * x;
+    const float in = iz <= c.lim ? __fmaf_rn(c.pb, xs, c.pa) : __builtin_amdgcn_rcpf(1.f + xs);
+    const float v = c.recpi * (in - __fmaf_rn(c.qb, xs, c.qa));
+    return iz < NFPTS ? v : 0.f;
+}
+
+// One line's CNVFNQ constants in the panel (:2120-2158).
+struct ConvLine {
+    double s1, s2, s3, sd, sv, zslope, zf1l, zf2l, zf3l;
+    int jmin1, jmax1, j2shft, j3shft;
+};
+__device__ __forceinline__ void conv_line(ConvLine &c, const LinePrep &lp, int64_t o, double sp, double DV,
+                                          double wavdxf, double hwdxf, double conf2, double conf3, double vft) {
+    const double recalf = lp.recalf[o];
+    const double depth = sp * recalf;
+    const int izm = lp.iz[o];
+    const double zetdif = 100. * lp.zeta[o] - (double)(izm - 1);
+    c.s1 = depth * avrat_lerp(c_cf1, izm, zetdif);
+    c.s2 = depth * avrat_lerp(c_cf2, izm, zetdif);
+    c.s3 = depth * avrat_lerp(c_cf3, izm, zetdif);
+    c.sd = depth * avrat_lerp(c_cgauss, izm, zetdif);
+    c.sv = depth * avrat_lerp(c_cer, izm, zetdif);
+    c.zslope = recalf * wavdxf;
+    const double zint = (lp.vnu[o] - vft) / DV;
+    const double bhwdxf = hwdxf / c.zslope;
+    c.jmax1 = ftrunc(zint + bhwdxf + 1.5);
+    c.jmin1 = ftrunc(zint - bhwdxf + 1.5);
+    const double rshft = zint < 0.0 ? -0.5 : 0.5;
+    c.j2shft = ftrunc(zint * (1. - conf2) + rshft);
+    c.j3shft = ftrunc(zint * (1. - conf3) + rshft);
+    const int jmin2 = c.jmin1 - c.j2shft, jmin3 = c.jmin1 - c.j3shft;
+    c.zf1l = ((double)(c.jmin1 - 2) - zint) * c.zslope;
+    c.zf2l = ((double)(jmin2 - 2) - zint * conf2) * c.zslope;
+    c.zf3l = ((double)(jmin3 - 2) - zint * conf3) * c.zslope;
+}
+
+// log2 of the parts per line of a panel with nl lines for nt lanes, at most 2^maxlg.
+__device__ __forceinline__ int conv_parts_lg(int nl, int nt, int maxlg) {
+    const int lg = nl >= nt ? 0 : (2 * nl >= nt ? 1 : 2);
+    return lg < maxlg ? lg : maxlg;
+}
+
 template <int CONV_U, bool CPL>
 __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *panels, LinePrep lp,
-                                                Tables T, double *A, unsigned long long *work) {
+                                                Tables T, double *A, unsigned long long *work, int split_lg) {
     extern __shared__ double sm[];
     const Panel &P = panels[blockIdx.x];
     const Job &J = jobs[P.layer];
@@ -498,35 +559,30 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
     const double conf2 = DV / J.dvr2, conf3 = DV / J.dvr3;
     const double vft = P.vft;
     unsigned long long nev = 0;
-    for (int64_t o = P.lstart + threadIdx.x; o < P.lend; o += blockDim.x) {
-        double sp = lp.sp[o];
+    const int nl = (int)(P.lend - P.lstart);
+    const int lg = conv_parts_lg(nl, blockDim.x, split_lg), parts = 1 << lg;
+    for (int it = threadIdx.x; it < (nl << lg); it += blockDim.x) {
+        const int64_t o = P.lstart + (it >> lg);
+        const int part = it & (parts - 1);
+        const double sp = lp.sp[o];
         if (sp == 0.) continue;
-        double recalf = lp.recalf[o];
-        double depth = sp * recalf;
-        int izm = lp.iz[o];
-        double zetdif = 100. * lp.zeta[o] - (double)(izm - 1);
-        double s1 = depth * avrat_lerp(c_cf1, izm, zetdif);
-        double s2 = depth * avrat_lerp(c_cf2, izm, zetdif);
-        double s3 = depth * avrat_lerp(c_cf3, izm, zetdif);
-        double sd = depth * avrat_lerp(c_cgauss, izm, zetdif);
-        double sv = depth * avrat_lerp(c_cer, izm, zetdif);
-        double zslope = recalf * wavdxf;
-        double zint = (lp.vnu[o] - vft) / DV;
-        double bhwdxf = hwdxf / zslope;
-        int jmax1 = ftrunc(zint + bhwdxf + 1.5);
-        int jmin1 = ftrunc(zint - bhwdxf + 1.5);
-        double rshft = zint < 0.0 ? -0.5 : 0.5;
-        int j2shft = ftrunc(zint * (1. - conf2) + rshft);
-        int j3shft = ftrunc(zint * (1. - conf3) + rshft);
-        int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
-        const double zf1l = ((double)(jmin1 - 2) - zint) * zslope;
-        const double zf2l = ((double)(jmin2 - 2) - zint * conf2) * zslope;
-        const double zf3l = ((double)(jmin3 - 2) - zint * conf3) * zslope;
-        double zf1 = zf1l, zf2 = zf2l, zf3 = zf3l;
-        nev += (unsigned long long)(jmax1 - jmin1 + 1);
+        ConvLine c;
+        conv_line(c, lp, o, sp, DV, wavdxf, hwdxf, conf2, conf3, vft);
+        const double s1 = c.s1, s2 = c.s2, s3 = c.s3, sd = c.sd, sv = c.sv, zslope = c.zslope;
+        const int j2shft = c.j2shft, j3shft = c.j3shft;
+        const int len = c.jmax1 - c.jmin1 + 1;
+        const int ja = c.jmin1 + (part * len) / parts, jb = c.jmin1 + ((part + 1) * len) / parts - 1;
+        double zf1 = c.zf1l, zf2 = c.zf2l, zf3 = c.zf3l;
+        for (int j = c.jmin1; j < ja; j++) {   // replay the sums up to this part
+            zf3 = zf3 + zslope;
+            zf2 = zf2 + zslope;
+            zf1 = zf1 + zslope;
+        }
+        const double y1s = zf1, y2s = zf2, y3s = zf3;
+        nev += (unsigned long long)(jb >= ja ? jb - ja + 1 : 0);
         // CONV_U R1 points at a time (see k_conv32)
-        int j1 = jmin1;
-        for (; j1 + (CONV_U - 1) <= jmax1; j1 += CONV_U) {
+        int j1 = ja;
+        for (; j1 + (CONV_U - 1) <= jb; j1 += CONV_U) {
             double4 g[CONV_U];
             double v2[CONV_U], v3[CONV_U];
 #pragma unroll
@@ -543,10 +599,10 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
                 const int jj = j1 + u, j2 = jj - j2shft, j3 = jj - j3shft;
                 if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], s3 * v3[u]);
                 if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], s2 * v2[u]);
-                if (jj >= 1) atomicAdd(&a1[jj], s1 * g[u].x + sd * g[u].y + sv * g[u].z);
+                if (jj >= 1 && jj <= max1) atomicAdd(&a1[jj], s1 * g[u].x + sd * g[u].y + sv * g[u].z);
             }
         }
-        for (; j1 <= jmax1; j1++) {
+        for (; j1 <= jb; j1++) {
             int j2 = j1 - j2shft, j3 = j1 - j3shft;
             zf3 = zf3 + zslope;
             zf2 = zf2 + zslope;
@@ -556,7 +612,7 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
             int iz1 = ftrunc(fabs(zf1) + 1.5);
             if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], s3 * T.f3[iz3]);
             if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], s2 * T.f2[iz2]);
-            if (j1 >= 1) {
+            if (j1 >= 1 && j1 <= max1) {
                 const double4 g = T.g1[iz1];
                 atomicAdd(&a1[j1], s1 * g.x + sd * g.y + sv * g.z);
             }
@@ -567,8 +623,8 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
             // term scaled by CLC DPTRAT and by its table argument ZF
             const double c1 = s1 * CLC1 * dptrat, c2 = s2 * CLC2 * dptrat, c3 = s3 * CLC3 * dptrat;
             const double cd = sd * CLC1 * dptrat, cv = sv * CLC1 * dptrat;
-            double y1 = zf1l, y2 = zf2l, y3 = zf3l;
-            for (int j1 = jmin1; j1 <= jmax1; j1++) {
+            double y1 = y1s, y2 = y2s, y3 = y3s;
+            for (int j1 = ja; j1 <= jb; j1++) {
                 const int j2 = j1 - j2shft, j3 = j1 - j3shft;
                 y3 = y3 + zslope;
                 y2 = y2 + zslope;
@@ -576,7 +632,7 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
                 const int iz3 = ftrunc(fabs(y3) + 1.5), iz2 = ftrunc(fabs(y2) + 1.5), iz1 = ftrunc(fabs(y1) + 1.5);
                 if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], c3 * T.f3[iz3] * y3);
                 if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], c2 * T.f2[iz2] * y2);
-                if (j1 >= 1) {
+                if (j1 >= 1 && j1 <= max1) {
                     const double4 g = T.g1[iz1];
                     atomicAdd(&a1[j1], (c1 * g.x + cd * g.y + cv * g.z) * y1);
                 }
@@ -599,13 +655,15 @@ __global__ void __launch_bounds__(256) k_conv64(const Job *jobs, const Panel *pa
 // ------------------------------------------------------------------------------------
 template <int CONV_U, bool CPL>
 __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *panels, LinePrep lp, Tables T,
-                                                double *A, unsigned long long *work) {
+                                                double *A, unsigned long long *work, int split_lg) {
     extern __shared__ double sm[];
     const Panel &P = panels[blockIdx.x];
     const Job &J = jobs[P.layer];
     const int max1 = J.max1, max2 = J.max2, max3 = J.max3, stride = J.stride;
     double *a1 = sm - 1, *a2 = sm + max1 - 1, *a3 = sm + max1 + max2 - 1;
-    const float *__restrict__ t2 = T.f2s, *__restrict__ t3 = T.f3s;
+    const float recpi = (float)c_shape[0];
+    const Shape23 sh2{recpi, (float)c_shape[1], (float)c_shape[2], (float)c_shape[3], (float)c_shape[4], 0.008f, 501};
+    const Shape23 sh3{recpi, (float)c_shape[3], (float)c_shape[4], (float)c_shape[5], (float)c_shape[6], 0.032f, 501};
     for (int t = threadIdx.x; t < stride; t += blockDim.x) sm[t] = 0.;
     __syncthreads();
     const double DXF1 = 0.002, HWF1 = 4.;
@@ -614,37 +672,33 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
     const double conf2 = DV / J.dvr2, conf3 = DV / J.dvr3;
     const double vft = P.vft;
     unsigned long long nev = 0;
-    for (int64_t o = P.lstart + threadIdx.x; o < P.lend; o += blockDim.x) {
-        double sp = lp.sp[o];
+    const int nl = (int)(P.lend - P.lstart);
+    const int lg = conv_parts_lg(nl, blockDim.x, split_lg), parts = 1 << lg;
+    for (int it = threadIdx.x; it < (nl << lg); it += blockDim.x) {
+        const int64_t o = P.lstart + (it >> lg);
+        const int part = it & (parts - 1);
+        const double sp = lp.sp[o];
         if (sp == 0.) continue;
-        double recalf = lp.recalf[o];
-        double depth = sp * recalf;
-        int izm = lp.iz[o];
-        double zetdif = 100. * lp.zeta[o] - (double)(izm - 1);
-        float s1 = (float)(depth * avrat_lerp(c_cf1, izm, zetdif));
-        float s2 = (float)(depth * avrat_lerp(c_cf2, izm, zetdif));
-        float s3 = (float)(depth * avrat_lerp(c_cf3, izm, zetdif));
-        float sd = (float)(depth * avrat_lerp(c_cgauss, izm, zetdif));
-        float sv = (float)(depth * avrat_lerp(c_cer, izm, zetdif));
-        double zslope = recalf * wavdxf;
-        double zint = (lp.vnu[o] - vft) / DV;
-        double bhwdxf = hwdxf / zslope;
-        int jmax1 = ftrunc(zint + bhwdxf + 1.5);
-        int jmin1 = ftrunc(zint - bhwdxf + 1.5);
-        double rshft = zint < 0.0 ? -0.5 : 0.5;
-        int j2shft = ftrunc(zint * (1. - conf2) + rshft);
-        int j3shft = ftrunc(zint * (1. - conf3) + rshft);
-        int jmin2 = jmin1 - j2shft, jmin3 = jmin1 - j3shft;
-        const double zf1l = ((double)(jmin1 - 2) - zint) * zslope;
-        const double zf2l = ((double)(jmin2 - 2) - zint * conf2) * zslope;
-        const double zf3l = ((double)(jmin3 - 2) - zint * conf3) * zslope;
-        double zf1 = zf1l, zf2 = zf2l, zf3 = zf3l;
-        nev += (unsigned long long)(jmax1 - jmin1 + 1);
+        ConvLine c;
+        conv_line(c, lp, o, sp, DV, wavdxf, hwdxf, conf2, conf3, vft);
+        const float s1 = (float)c.s1, s2 = (float)c.s2, s3 = (float)c.s3, sd = (float)c.sd, sv = (float)c.sv;
+        const double zslope = c.zslope;
+        const int j2shft = c.j2shft, j3shft = c.j3shft;
+        const int len = c.jmax1 - c.jmin1 + 1;
+        const int ja = c.jmin1 + (part * len) / parts, jb = c.jmin1 + ((part + 1) * len) / parts - 1;
+        double zf1 = c.zf1l, zf2 = c.zf2l, zf3 = c.zf3l;
+        for (int j = c.jmin1; j < ja; j++) {   // replay the sums up to this part
+            zf3 = zf3 + zslope;
+            zf2 = zf2 + zslope;
+            zf1 = zf1 + zslope;
+        }
+        const double y1s = zf1, y2s = zf2, y3s = zf3;
+        nev += (unsigned long long)(jb >= ja ? jb - ja + 1 : 0);
         // CONV_U R1 points at a time, so that their table loads are in flight together;
         // the ZF arguments are still accumulated one step at a time.  Indices of points
         // outside the R2/R3 buffers are clamped into the tables (their values unused).
-        int j1 = jmin1;
-        for (; j1 + (CONV_U - 1) <= jmax1; j1 += CONV_U) {
+        int j1 = ja;
+        for (; j1 + (CONV_U - 1) <= jb; j1 += CONV_U) {
             float4 g[CONV_U];
             float v2[CONV_U], v3[CONV_U];
 #pragma unroll
@@ -653,18 +707,19 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
                 zf2 = zf2 + zslope;
                 zf1 = zf1 + zslope;
                 g[u] = T.g1s[min(ftrunc(fabs(zf1) + 1.5), NFMX)];
-                v2[u] = t2[min(ftrunc(fabs(zf2) + 1.5), NFMX)];
-                v3[u] = t3[min(ftrunc(fabs(zf3) + 1.5), NFMX)];
+                v2[u] = shape23(sh2, ftrunc(fabs(zf2) + 1.5));
+                v3[u] = shape23(sh3, ftrunc(fabs(zf3) + 1.5));
             }
 #pragma unroll
             for (int u = 0; u < CONV_U; u++) {
                 const int jj = j1 + u, j2 = jj - j2shft, j3 = jj - j3shft;
                 if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(s3 * v3[u]));
                 if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(s2 * v2[u]));
-                if (jj >= 1) atomicAdd(&a1[jj], (double)__fmaf_rn(sv, g[u].z, __fmaf_rn(sd, g[u].y, s1 * g[u].x)));
+                if (jj >= 1 && jj <= max1)
+                    atomicAdd(&a1[jj], (double)__fmaf_rn(sv, g[u].z, __fmaf_rn(sd, g[u].y, s1 * g[u].x)));
             }
         }
-        for (; j1 <= jmax1; j1++) {
+        for (; j1 <= jb; j1++) {
             int j2 = j1 - j2shft, j3 = j1 - j3shft;
             zf3 = zf3 + zslope;
             zf2 = zf2 + zslope;
@@ -672,9 +727,9 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
             int iz3 = ftrunc(fabs(zf3) + 1.5);
             int iz2 = ftrunc(fabs(zf2) + 1.5);
             int iz1 = ftrunc(fabs(zf1) + 1.5);
-            if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(s3 * t3[iz3]));
-            if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(s2 * t2[iz2]));
-            if (j1 >= 1) {
+            if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(s3 * shape23(sh3, iz3)));
+            if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(s2 * shape23(sh2, iz2)));
+            if (j1 >= 1 && j1 <= max1) {
                 const float4 g = T.g1s[iz1];
                 atomicAdd(&a1[j1], (double)__fmaf_rn(sv, g.z, __fmaf_rn(sd, g.y, s1 * g.x)));
             }
@@ -684,16 +739,16 @@ __global__ void __launch_bounds__(256) k_conv32(const Job *jobs, const Panel *pa
             const float c1 = (float)((double)s1 * CLC1 * dptrat), c2 = (float)((double)s2 * CLC2 * dptrat);
             const float c3 = (float)((double)s3 * CLC3 * dptrat), cd = (float)((double)sd * CLC1 * dptrat);
             const float cv = (float)((double)sv * CLC1 * dptrat);
-            double y1 = zf1l, y2 = zf2l, y3 = zf3l;
-            for (int j1 = jmin1; j1 <= jmax1; j1++) {
+            double y1 = y1s, y2 = y2s, y3 = y3s;
+            for (int j1 = ja; j1 <= jb; j1++) {
                 const int j2 = j1 - j2shft, j3 = j1 - j3shft;
                 y3 = y3 + zslope;
                 y2 = y2 + zslope;
                 y1 = y1 + zslope;
                 const int iz3 = ftrunc(fabs(y3) + 1.5), iz2 = ftrunc(fabs(y2) + 1.5), iz1 = ftrunc(fabs(y1) + 1.5);
-                if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(c3 * t3[iz3] * (float)y3));
-                if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(c2 * t2[iz2] * (float)y2));
-                if (j1 >= 1) {
+                if (j3 >= 1 && j3 <= max3) atomicAdd(&a3[j3], (double)(c3 * shape23(sh3, iz3) * (float)y3));
+                if (j2 >= 1 && j2 <= max2) atomicAdd(&a2[j2], (double)(c2 * shape23(sh2, iz2) * (float)y2));
+                if (j1 >= 1 && j1 <= max1) {
                     const float4 g = T.g1s[iz1];
                     atomicAdd(&a1[j1], (double)(__fmaf_rn(cv, g.z, __fmaf_rn(cd, g.y, c1 * g.x)) * (float)y1));
                 }
@@ -2137,6 +2192,15 @@ namespace {
 
 // Line-shape tables (SHAPEL clblm_ODLAY.f90:2804-2909, SHAPEG :2914-2953, VERFN
 // :2959-3016), built on the host in the reference's operation order.
+// The constants of build_tables: 1/pi and A, B at z = 4, 16, 64.
+void shape_constants(double *sc) {
+    auto Af = [](double z) { return (1. + 2. * z * z) / ((1. + z * z) * (1. + z * z)); };
+    auto Bf = [](double z) { return -1. / ((1. + z * z) * (1. + z * z)); };
+    sc[0] = 1. / (2. * std::asin(1.));
+    sc[1] = Af(4.); sc[2] = Bf(4.); sc[3] = Af(16.); sc[4] = Bf(16.); sc[5] = Af(64.); sc[6] = Bf(64.);
+    sc[7] = 0.;
+}
+
 void build_tables(std::vector<double> &F1, std::vector<double> &F2, std::vector<double> &F3,
                   std::vector<double> &FG, std::vector<double> &XV) {
     const double HWF1 = 4., HWF2 = 16., HWF3 = 64., DXF1 = 0.002, DXF2 = 0.008, DXF3 = 0.032;
@@ -2292,6 +2356,7 @@ struct clblm_ctx {
     float *far_tau[F4KMAX + 1] = {};
     int f4k32 = 3, f4k64 = 2;   // F4 gather points per lane, FP32C / FP64
     int conv_u = 4;             // pass-1 R1 points per walk step
+    int conv_split = 0;         // log2 of the most parts a pass-1 line walk is split into
     double *tab = nullptr;   // F1,F2,F3,FG,XVER consecutive, NFMX+1 each
     float *tabs = nullptr;
     double4 *tabg = nullptr;
@@ -2426,6 +2491,11 @@ static int ctx_tables(clblm_ctx *c) {
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cf3), CLBLM_CF3, sizeof(CLBLM_CF3)));
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_cer), CLBLM_CER, sizeof(CLBLM_CER)));
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_maxiso), CLBLM_MAXISO, sizeof(CLBLM_MAXISO)));
+    {
+        double sc[8];
+        shape_constants(sc);
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_shape), sc, sizeof(sc)));
+    }
     for (int fk = F4KMIN; fk <= F4KMAX; fk++) {
         // far-field nodes of the FP32 gather: Chebyshev points of [0, W-1] (rounded to
         // FP32, as the kernel uses them) and the Lagrange weights of every tile point
@@ -2985,9 +3055,12 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         const size_t lds = (size_t)B.max_stride * sizeof(double);
 #define CONV_LAUNCH(U)                                                                                          \
     {                                                                                                           \
-        auto kc = precision == CLBLM_FP64 ? (L.has_cpl ? k_conv64<U, true> : k_conv64<U, false>)                \
-                                          : (L.has_cpl ? k_conv32<U, true> : k_conv32<U, false>);               \
-        hipLaunchKernelGGL(kc, g, b, lds, st, d_jobs, d_pan, lp, T, d_A, C->d_work);                            \
+        auto k64 = L.has_cpl ? k_conv64<U, true> : k_conv64<U, false>;                                         \
+        auto k32 = L.has_cpl ? k_conv32<U, true> : k_conv32<U, false>;                                         \
+        if (precision == CLBLM_FP64)                                                                            \
+            hipLaunchKernelGGL(k64, g, b, lds, st, d_jobs, d_pan, lp, T, d_A, C->d_work, C->conv_split);      \
+        else                                                                                                    \
+            hipLaunchKernelGGL(k32, g, b, lds, st, d_jobs, d_pan, lp, T, d_A, C->d_work, C->conv_split);       \
     }
         switch (C->conv_u) {
         case 2: CONV_LAUNCH(2) break;
@@ -3443,12 +3516,13 @@ int clblm_autotune(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
     std::vector<int> klist;
     if (nkeys > 0) klist.assign(keys, keys + nkeys);
     else klist = {prec == CLBLM_FP64 ? CLBLM_TUNE_F4_POINTS_FP64 : CLBLM_TUNE_F4_POINTS_FP32C, CLBLM_TUNE_CONV_UNROLL,
-                  CLBLM_TUNE_F4_XCD_CHUNK, CLBLM_TUNE_F4_STREAM};
+                  CLBLM_TUNE_CONV_SPLIT, CLBLM_TUNE_F4_XCD_CHUNK, CLBLM_TUNE_F4_STREAM};
     auto candidates = [](int key) -> std::vector<int> {
         switch (key) {
         case CLBLM_TUNE_F4_POINTS_FP32C:
         case CLBLM_TUNE_F4_POINTS_FP64: return {2, 3, 4};
         case CLBLM_TUNE_CONV_UNROLL: return {2, 4, 8};
+        case CLBLM_TUNE_CONV_SPLIT: return {1, 2, 4};
         case CLBLM_TUNE_F4_XCD_CHUNK: return {4, 16, 64};
         case CLBLM_TUNE_F4_STREAM: return {1, 2};
         case CLBLM_TUNE_F4_FAR_FIELD: return {1, 2};
@@ -3523,6 +3597,7 @@ int clblm_autotune(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
 int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value) {
     if (!c) return set_err(CLBLM_ERR_ARG, "ctx is NULL");
     int *slot = nullptr;
+    int split_parts = 1 << c->conv_split;   // the key counts parts; the context keeps log2
     int lo = 0, hi = 0;
     switch (key) {
     case CLBLM_TUNE_F4_XCD_CHUNK: slot = &c->f4_chunk; lo = 1; hi = 4096; break;
@@ -3531,14 +3606,17 @@ int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value) {
     case CLBLM_TUNE_F4_POINTS_FP32C: slot = &c->f4k32; lo = F4KMIN; hi = F4KMAX; break;
     case CLBLM_TUNE_F4_POINTS_FP64: slot = &c->f4k64; lo = F4KMIN; hi = F4KMAX; break;
     case CLBLM_TUNE_CONV_UNROLL: slot = &c->conv_u; lo = 2; hi = 8; break;
+    case CLBLM_TUNE_CONV_SPLIT: slot = &split_parts; lo = 1; hi = 4; break;
     default: return set_err(CLBLM_ERR_ARG, "unknown tuning key");
     }
     if (old_value) *old_value = *slot;
     if (value == 0) return 0;   // query only
-    if (value < lo || value > hi || (key == CLBLM_TUNE_CONV_UNROLL && value != 2 && value != 4 && value != 8))
+    if (value < lo || value > hi || (key == CLBLM_TUNE_CONV_UNROLL && value != 2 && value != 4 && value != 8) ||
+        (key == CLBLM_TUNE_CONV_SPLIT && value == 3))
         return set_err(CLBLM_ERR_ARG, "tuning value out of range");
     std::lock_guard<std::mutex> g(c->mu);
-    *slot = value;
+    if (key == CLBLM_TUNE_CONV_SPLIT) c->conv_split = value == 4 ? 2 : value - 1;
+    else *slot = value;
     c->plan_valid = false;
     return 0;
 }

@@ -296,7 +296,7 @@ class Context:
                                       ctypes.cast(karr, ctypes.c_void_p) if karr is not None else None, nk,
                                       int(reps), ctypes.byref(ms)))
         chosen = {k: self.tuning(k) for k in (_lib.TUNE_F4_POINTS_FP32C, _lib.TUNE_F4_POINTS_FP64,
-                                              _lib.TUNE_CONV_UNROLL, _lib.TUNE_F4_XCD_CHUNK,
+                                              _lib.TUNE_CONV_UNROLL, _lib.TUNE_CONV_SPLIT, _lib.TUNE_F4_XCD_CHUNK,
                                               _lib.TUNE_F4_STREAM, _lib.TUNE_F4_FAR_FIELD)}
         return chosen, ms.value
 

@@ -182,13 +182,16 @@ int clblm_od_graph_destroy(clblm_ctx *ctx, clblm_od_graph *graph);
  *                            of the batch on the batch's stream, one after another.
  *   CLBLM_TUNE_F4_POINTS_FP32C, CLBLM_TUNE_F4_POINTS_FP64  R4 points per lane of the
  *                            F4 gather (2-4; a tile is 64 times that), per precision.
- *   CLBLM_TUNE_CONV_UNROLL   R1 points per step of the pass-1 walk (2, 4 or 8). */
+ *   CLBLM_TUNE_CONV_UNROLL   R1 points per step of the pass-1 walk (2, 4 or 8).
+ *   CLBLM_TUNE_CONV_SPLIT    the most parts (1, 2 or 4) a pass-1 line walk is split into
+ *                            in panels with fewer lines than lanes. */
 #define CLBLM_TUNE_F4_XCD_CHUNK 1
 #define CLBLM_TUNE_F4_FAR_FIELD 2
 #define CLBLM_TUNE_F4_STREAM 3
 #define CLBLM_TUNE_F4_POINTS_FP32C 4
 #define CLBLM_TUNE_F4_POINTS_FP64 5
 #define CLBLM_TUNE_CONV_UNROLL 6
+#define CLBLM_TUNE_CONV_SPLIT 7
 int clblm_tuning(clblm_ctx *ctx, int key, int value, int *old_value);
 
 /* Autotuner of the launch shapes (no counterpart in the reference).  Runs the batch
