@@ -3663,7 +3663,7 @@ int clblm_scanfn(clblm_ctx *c, const double *d_in, int64_t n, int64_t nrows, int
     std::lock_guard<std::mutex> g(c->mu);
     if (!c->scan) c->scan = clblm_scan_state_new();
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-    if (d_out && c->last_stream && c->last_stream != st && c->ncalls > 0) {  // after the batches that made d_in
+    if (d_in && c->last_stream && c->last_stream != st && c->ncalls > 0) {  // after the batches that made d_in
         const int kp = (int)((c->ncalls + clblm_ctx::NSLOT - 1) % clblm_ctx::NSLOT);
         HIPCHK(hipStreamWaitEvent(st, c->slot[kp].done, 0));
     }

@@ -457,6 +457,8 @@ struct ScanState {
     size_t win_cap = 0;
     void *meta = nullptr;          // segments, apodization blocks, coefficients
     size_t meta_cap = 0;
+    void *conv = nullptr;          // spectral-domain scan: tables and per-box arrays
+    size_t conv_cap = 0;
     ~ScanState() {
         if (fwd) hipfftDestroy(fwd);
         if (inv) hipfftDestroy(inv);
@@ -465,6 +467,7 @@ struct ScanState {
         (void)hipFree(xs);
         (void)hipFree(win);
         (void)hipFree(meta);
+        (void)hipFree(conv);
     }
 };
 
@@ -1038,16 +1041,487 @@ __global__ void k_rect_copy(const double *x, int64_t ldx, int64_t n, double *out
     if (i < n) out[blockIdx.y * ldo + i] = x[blockIdx.y * ldx + i];
 }
 
+// ==================================================================================
+// Spectral-domain scan with a tabulated scanning function: clblm_SCANFN_struct
+// (clblm_ScanFilter.f90:70-544) for functIDs 2..5 (JFN 1..4: triangle, Gaussian,
+// sinc**2, sinc).  The function is tabulated every DXF half widths (SHAPET, SHAPEG,
+// SINCSQ, SINC :551-723), the input compressed into boxes of IRATSH points (SHRKSC
+// :732-850), and CONVSC (:856-955) adds SI * XF(IT) into R1(JMIN..JMAX) for each box,
+// IT = INT(|ZF| + 1.5) with ZF accumulated step by step from JMIN.  R1 is written and
+// shifted by 2400 points whenever a box reaches past MAXF (panelOutput :2411-2506).
+//
+//   host    the walk over input panels (panelInput :2320-2404 with SHRKSC's carried
+//           remainder), the output panels' VFT (accumulated as the reference does),
+//           and the table, evaluated with the C library's sin/exp as the reference's
+//           build does
+//   device  one lane per box: its sum, its output panel k (the first whose JMAX fits
+//           MAXF: boxes only move forward, so this is the panel the reference is on
+//           when it reaches the box), JMIN, JMAX and ZF(JMIN); then one wave per 64
+//           outputs, walking the boxes that reach them in the reference's order: each
+//           lane replays one box's ZF from JMIN to the tile (the same additions), the
+//           table indices go through LDS, and each lane adds its output's terms box by
+//           box, so every output is the reference's sum, bit for bit
+// Output G = 33 + o of the global R1 index G = JF + 2400 k; a term at JF < 33 falls on a
+// position already written (or before V1) and is dropped, as in the reference.
+// The reference's output count depends on the data in one case: when the input ends
+// before V2 + bound, the last panel written is the one of the last nonzero box (CONVSC
+// skips zero boxes, :906); that case reads one integer per row back.
+// ==================================================================================
+struct ScanFnTab { double hwf, dxf; int nf, nfmax; double smpl, xscal; };
+constexpr ScanFnTab SCAN_FN[5] = {   // HWJ, DXJ, NJ, NJMX, SMPLJ, XSCAL (clblm_ScanFilter.f90:182-207)
+    {1., 0.0, 0, 0, .5, 0.},
+    {2., 0.02, 101, 251, 2., 0.},
+    {4., 0.02, 201, 251, 4., 0.},
+    {54.1826, 0.02, 2710, 2760, 4., 1.391557377},
+    {119.332818, 0.02, 5968, 6018, 4., 1.89549425},
+};
+constexpr int CONV_NLIMF = 2401, CONV_NSHIFT = 32, CONV_R1 = 5000, CONV_TILE = 64;
+constexpr int CONV_DEAD = 0x3fffffff;   // G of a box past the last output panel
+
+// xf[0..nfmax], xf[0] unused: SHAPET, SHAPEG, SINCSQ, SINC (clblm_ScanFilter.f90:551-723)
+void scan_shape(int jfn, std::vector<double> &xf) {
+    const ScanFnTab &f = SCAN_FN[jfn];
+    const double PI = 2. * std::asin(1.);
+    xf.assign((size_t)f.nfmax + 1, 0.);
+    if (jfn == 1) {
+        xf[1] = 0.5;
+        for (int i = 2; i <= f.nf; i++) {
+            const double X = (double)(i - 1) * f.dxf;
+            xf[i] = 0.5 * (1. - 0.5 * X);
+        }
+    } else if (jfn == 2) {
+        const double FLN2 = std::log(2.), RECPI = 1. / (2. * std::asin(1.)), FGNORM = std::sqrt(FLN2 * RECPI);
+        xf[1] = FGNORM * std::exp(-FLN2 * 0.);
+        for (int j = 2; j <= f.nf; j++) {
+            const double X = (double)(j - 1) * f.dxf, XSQ = X * X;
+            xf[j] = FGNORM * std::exp(-FLN2 * XSQ);
+        }
+        xf[f.nf] = 0.;
+    } else {
+        const double XNORM = f.xscal / PI;
+        xf[1] = XNORM;
+        for (int i = 2; i <= f.nf; i++) {
+            const double X = (double)(i - 1) * f.dxf, T = X * f.xscal, q = std::sin(T) / T;
+            xf[i] = (jfn == 3) ? XNORM * (q * q) : XNORM * q;
+        }
+    }
+}
+
+struct CPanel {
+    int64_t q0;    // global box index of the panel's box 1
+    double v1i;    // V1I after SHRKSC
+};
+
+struct ConvPlan {
+    int jfn = 0, nfmax = 0;
+    double hwhm = 0., dvo = 0., dvi = 0., ratio = 0., zslope = 0., zbound = 0., xnorm = 0., sratio = 1.;
+    int maxf = 0, iratsh = 1;
+    int64_t raw0 = 0;               // input index of box 0's first point
+    int64_t qlo = 0, qhi = -1;      // boxes CONVSC reaches
+    std::vector<CPanel> panels;     // the panels holding them, in order
+    std::vector<double> vft;        // VFT of output panels 0..kstar+1
+    int kstar = 0, nnhi = 0;        // the panel where panelOutput sets ISTOP, and its NNHI
+    bool exhausted = false;         // the input ends before V2 + bound
+    std::vector<double> xf;
+    double v1 = 0.;
+    int64_t nout_of(int kex) const {  // output count when the last panel written is kex
+        if (!exhausted || kex >= kstar) return (int64_t)2400 * kstar + nnhi - (CONV_NSHIFT);
+        return (int64_t)2400 * (kex + 1);
+    }
+};
+
+int plan_conv(int64_t ns, double sv1, double sdv, double V1, double V2, double dvout, int jfn, double HWHM,
+              ConvPlan &cp) {
+    const ScanFnTab &F = SCAN_FN[jfn];
+    cp.jfn = jfn;
+    cp.nfmax = F.nfmax;
+    cp.hwhm = HWHM;
+    cp.v1 = V1;
+    const double SAMPL = -std::fabs(dvout);
+    double SAMPLE = F.smpl;
+    if (SAMPL < 0.) SAMPLE = HWHM / (-SAMPL);
+    const double DVO = HWHM / SAMPLE;
+    cp.dvo = DVO;
+    const double HWF = F.hwf, DXF = F.dxf;
+    const int NBOUND = (int)((2. * HWF) * SAMPLE + 0.01);
+    const double BOUND = (double)NBOUND * DVO / 2.;
+    cp.maxf = CONV_NLIMF + 2 * NBOUND + CONV_NSHIFT;
+    if (cp.maxf > CONV_R1)
+        return fail(CLBLM_ERR_ARG, "scan: output spacing too fine (the reference's R1 holds 5000 points)");
+    if ((HWF / DVO * HWHM + 0.5) * (DVO / DXF / HWHM) * (1. + 1e-9) + 1.5 >= (double)(F.nfmax + 1))
+        return fail(CLBLM_ERR_ARG, "scan: output spacing too coarse for the scanning function's table");
+    scan_shape(jfn, cp.xf);
+    const double VBOT = V1 - BOUND, VTOP = V2 + BOUND;
+    // output panels: VFT accumulated, ISTOP where NNHI <= NHI
+    const int NHI = CONV_NLIMF + CONV_NSHIFT - 1;
+    double VFT = V1 - (double)CONV_NSHIFT * DVO;
+    for (int k = 0;; k++) {
+        cp.vft.push_back(VFT);
+        const double z = (V2 - VFT) / DVO + 1.5;
+        if (z < (double)NHI + 1.) {
+            cp.kstar = k;
+            cp.nnhi = (int)z;
+            break;
+        }
+        if (k > 400000) return fail(CLBLM_ERR_ARG, "scan: output range too long");
+        VFT = VFT + (double)(CONV_NLIMF - 1) * DVO;
+    }
+    cp.vft.push_back(VFT + (double)(CONV_NLIMF - 1) * DVO);
+    if ((int64_t)2400 * cp.kstar + cp.nnhi - CONV_NSHIFT < 1) return fail(CLBLM_ERR_ARG, "scan: empty output range");
+    // the input panels (panelInput + SHRKSC), data left out
+    static const int JRATIO[24] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 25, 30, 32, 40, 48, 50, 60, 75, 80, 100, 120};
+    int64_t done = 0, qnext = 0;
+    double done_v2 = sv1 - sdv;
+    int NREN = 0, INIT = 0, IRATSH = 1;
+    double DVSC = 0., V1SHFT = 0.;
+    bool first = true;
+    cp.dvi = sdv;
+    for (;;) {
+        int nlim;
+        double VMIN = 0., VMAX = 0.;
+        for (;;) {
+            nlim = 2400;
+            if (done + nlim > ns) nlim = (int)(ns - done);
+            if (nlim == 0) break;
+            VMIN = done_v2 + sdv;
+            VMAX = VMIN + (double)(nlim - 1) * sdv;
+            done += nlim;
+            done_v2 = VMAX;
+            if (VMAX >= VBOT) break;
+        }
+        if (nlim == 0) { cp.exhausted = true; break; }
+        const int NLOW = NREN > 0 ? NREN + 1 : 1;
+        double V1I = VMIN - (double)(NLOW - 1) * sdv;
+        int NNI = nlim + NLOW - 1;
+        const bool IDATA = VMAX > VTOP;
+        int ILO = 1, IHI = IDATA ? (int)((VTOP - V1I) / sdv + 1.5) : NNI;
+        if (first) cp.raw0 = done - nlim;
+        NREN = 0;
+        bool skip = false;
+        if (INIT == 0) {
+            DVSC = HWHM / 12.;
+            IRATSH = (int)(DVSC / sdv + 0.5);
+            for (int i = 2; i <= 24; i++)
+                if (JRATIO[i - 1] > IRATSH) { IRATSH = JRATIO[i - 2]; break; }
+            if (IRATSH > JRATIO[23]) IRATSH = JRATIO[23];
+            if (IRATSH <= 1) skip = true;
+            else {
+                DVSC = (double)IRATSH * sdv;
+                V1SHFT = (double)(IRATSH - 1) * sdv / 2.;
+                INIT = 1;
+            }
+        }
+        int nbox = NNI;
+        if (!skip && IRATSH > 1) {
+            NREN = NNI - (NNI / IRATSH) * IRATSH;
+            V1I = V1I + V1SHFT;
+            nbox = NNI / IRATSH;
+            ILO = (int)((VBOT - V1I) / DVSC + 1.5);
+            if (ILO < 1) ILO = 1;
+            IHI = (int)((VTOP - V1I) / DVSC + 1.5);
+            if (IHI > nbox) IHI = nbox;
+            cp.iratsh = IRATSH;
+            cp.sratio = IRATSH;
+            cp.dvi = DVSC;
+        } else {
+            cp.iratsh = 1;
+        }
+        const int64_t q0 = qnext;
+        qnext += nbox;
+        if (ILO > IHI) break;     // CONVSC reaches nothing here and writes panels up to ISTOP
+        if (!first && ILO != 1)
+            return fail(CLBLM_ERR_UNSUPPORTED, "scan: a later input panel starts below the scan bound");
+        cp.panels.push_back({q0, V1I});
+        if (first) cp.qlo = q0 + ILO - 1;
+        cp.qhi = q0 + IHI - 1;
+        first = false;
+        if (IDATA) break;         // the last panel CONVSC reads
+    }
+    if (cp.qhi >= cp.qlo && cp.raw0 + (cp.qhi + 1) * (int64_t)cp.iratsh > ns)
+        return fail(CLBLM_ERR_UNSUPPORTED, "scan: a box reaches past the input");
+    cp.ratio = cp.dvi / DVO;
+    cp.zslope = (DVO / DXF) / HWHM;
+    cp.zbound = (HWF / DVO) * HWHM;
+    cp.xnorm = cp.dvi / HWHM;
+    return 0;
+}
+
+struct ConvArgs {
+    const CPanel *panels;
+    int npanel;
+    const double *vft;
+    int kstar, maxf, iratsh;
+    double dvo, ratio, zbound, zslope, xnorm, sratio;
+    int64_t qlo, nq, raw0;
+};
+
+// One lane per box q = qlo + t: its output panel, JMIN, JMAX and ZF(JMIN).
+__global__ void k_conv_boxes(ConvArgs a, int32_t *g0, int32_t *ge, int32_t *kk, double *zf0) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nq) return;
+    const int64_t q = a.qlo + t;
+    int lo = 0, hi = a.npanel - 1;            // the panel holding q
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (a.panels[mid].q0 <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    const double V1I = a.panels[lo].v1i;
+    const double ip = (double)(q - a.panels[lo].q0) * a.ratio;   // REAL(I-1) * RATIO
+    const double lim = (double)a.maxf + 1.;                      // INT(x) <= MAXF  <=>  x < MAXF + 1
+    auto top = [&](int k) { return ((ip + (V1I - a.vft[k]) / a.dvo) + a.zbound) + 1.5; };
+    int k = (int)fmax(0., ceil((top(0) - lim) / 2400.));
+    if (k > a.kstar + 1) k = a.kstar + 1;
+    while (k > 0 && top(k - 1) < lim) k--;
+    while (k <= a.kstar && !(top(k) < lim)) k++;
+    if (k > a.kstar) {                        // reached only after the last panel is written
+        g0[t] = CONV_DEAD;
+        ge[t] = CONV_DEAD;
+        kk[t] = k;
+        zf0[t] = 0.;
+        return;
+    }
+    const double ZPEAK = ip + (V1I - a.vft[k]) / a.dvo;
+    const int JMAX = (int)((ZPEAK + a.zbound) + 1.5);
+    int JMIN = (int)((ZPEAK - a.zbound) + 1.5);
+    if (JMIN < 1) JMIN = 1;
+    g0[t] = JMIN + 2400 * k;
+    ge[t] = JMAX + 2400 * k;
+    kk[t] = k;
+    zf0[t] = ((double)(JMIN - 1) - ZPEAK) * a.zslope;
+}
+
+__global__ void k_conv_zero(int32_t *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// One lane per box and row: the box sum (SHRKSC), SI = XNORM S, and the last output
+// panel of a nonzero box.
+__global__ void k_conv_si(ConvArgs a, const double *x, int64_t ldx, const int32_t *kk, double *si, int32_t *kex) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nq) return;
+    const int r = blockIdx.y;
+    const double *src = x + r * ldx + a.raw0 + (a.qlo + t) * (int64_t)a.iratsh;
+    double S;
+    if (a.iratsh > 1) {
+        double SUMK = 0.;
+        for (int j = 0; j < a.iratsh; j++) SUMK = SUMK + src[j];
+        S = SUMK / a.sratio;
+    } else {
+        S = src[0];
+    }
+    si[r * a.nq + t] = a.xnorm * S;
+    if (S != 0. && kk[t] <= a.kstar) atomicMax(kex + r, kk[t]);
+}
+
+// One wave per 64 outputs: the boxes that reach them, in order, 64 at a time.
+__global__ __launch_bounds__(64) void k_conv_sum(ConvArgs a, const int32_t *g0, const int32_t *ge,
+                                                  const int32_t *kk, const double *zf0, const double *si,
+                                                  const double *xf, int nfmax, double *out, int64_t ldo,
+                                                  int64_t nout) {
+    __shared__ int it[CONV_TILE][CONV_TILE + 1];
+    const int lane = threadIdx.x;
+    const int r = blockIdx.y;
+    const int64_t o0 = (int64_t)blockIdx.x * CONV_TILE;
+    const int Ga = (int)(o0 + CONV_NSHIFT + 1);
+    const int Gb = Ga + CONV_TILE - 1;
+    // first box whose last output reaches Ga (ge is nondecreasing up to 1: a box on a
+    // later output panel may round one position lower), last box starting by Gb
+    int64_t lo = 0, hi = a.nq;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (ge[mid] >= Ga - 2) hi = mid;
+        else lo = mid + 1;
+    }
+    const int64_t qa = lo;
+    lo = qa - 1;
+    hi = a.nq - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (g0[mid] <= Gb + 2) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t qb = lo;
+    const double *sir = si + r * a.nq;
+    double acc = 0.;
+    for (int64_t base = qa; base <= qb; base += CONV_TILE) {
+        const int64_t t = base + lane;
+        double s = 0.;
+        for (int g = 0; g < CONV_TILE; g++) it[lane][g] = -1;
+        if (t <= qb) {
+            s = sir[t];
+            const int G0 = g0[t], GE = ge[t];
+            const int Gs = max(G0, 2400 * kk[t] + CONV_NSHIFT + 1);
+            const int gl = max(Gs, Ga), gh = min(GE, Gb);
+            if (G0 != CONV_DEAD && gl <= gh) {
+                const double zs = a.zslope;
+                double zf = zf0[t];
+                for (int g = G0; g < gl; g++) zf = zf + zs;
+                for (int g = gl; g <= gh; g++) {
+                    int idx = (int)(fabs(zf) + 1.5);
+                    it[lane][g - Ga] = idx < nfmax ? idx : nfmax;
+                    zf = zf + zs;
+                }
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < CONV_TILE; j++) {
+            const double sj = __shfl(s, j);
+            const int idx = it[j][lane];
+            if (idx >= 0) acc = acc + sj * xf[idx];
+        }
+        __syncthreads();
+    }
+    const int64_t o = o0 + lane;
+    if (o < nout) out[r * ldo + o] = acc;
+}
+
+int run_conv(ScanState &S, const ConvPlan &cp, const double *xs, int64_t ldx, int64_t nrows, bool count_only,
+             double *d_out, int64_t ld_out, int64_t *nout, hipStream_t st) {
+    const int64_t nq = cp.qhi - cp.qlo + 1;
+    if (nq <= 0) {
+        *nout = cp.nout_of(0);
+        if (count_only) return 0;
+        if (ld_out < *nout) return fail(CLBLM_ERR_ARG, "ld_out is below the output length");
+        hipLaunchKernelGGL(k_rect_fill, dim3(cdiv(*nout, 256), (unsigned)nrows), dim3(256), 0, st, d_out, *nout,
+                           ld_out);
+        SCHK(hipGetLastError());
+        return 0;
+    }
+    if (!cp.exhausted) {
+        *nout = cp.nout_of(cp.kstar);
+        if (count_only) return 0;
+    }
+    if (!xs) return fail(CLBLM_ERR_ARG, "input spectrum is NULL");
+    auto up = [](size_t o) { return (o + 255) & ~(size_t)255; };
+    const size_t o_xf = 0, o_vft = up(sizeof(double) * cp.xf.size());
+    const size_t o_pan = up(o_vft + sizeof(double) * cp.vft.size());
+    const size_t o_g0 = up(o_pan + sizeof(CPanel) * cp.panels.size());
+    const size_t o_ge = up(o_g0 + 4 * (size_t)nq), o_kk = up(o_ge + 4 * (size_t)nq);
+    const size_t o_zf = up(o_kk + 4 * (size_t)nq), o_si = up(o_zf + 8 * (size_t)nq);
+    const size_t o_kex = up(o_si + 8 * (size_t)nq * (size_t)nrows);
+    const size_t total = o_kex + 4 * (size_t)nrows + 256;
+    SCHK(hipStreamSynchronize(st));   // the buffer may still be read by a previous scan
+    int rc = grow(&S.conv, &S.conv_cap, total);
+    if (rc) return rc;
+    char *b = (char *)S.conv;
+    std::vector<char> host(o_g0, 0);
+    std::memcpy(host.data() + o_xf, cp.xf.data(), sizeof(double) * cp.xf.size());
+    std::memcpy(host.data() + o_vft, cp.vft.data(), sizeof(double) * cp.vft.size());
+    std::memcpy(host.data() + o_pan, cp.panels.data(), sizeof(CPanel) * cp.panels.size());
+    SCHK(hipMemcpyAsync(b, host.data(), o_g0, hipMemcpyHostToDevice, st));
+    ConvArgs a{(const CPanel *)(b + o_pan), (int)cp.panels.size(), (const double *)(b + o_vft), cp.kstar, cp.maxf,
+               cp.iratsh, cp.dvo, cp.ratio, cp.zbound, cp.zslope, cp.xnorm, cp.sratio, cp.qlo, nq, cp.raw0};
+    int32_t *g0 = (int32_t *)(b + o_g0), *ge = (int32_t *)(b + o_ge), *kk = (int32_t *)(b + o_kk);
+    int32_t *kex = (int32_t *)(b + o_kex);
+    double *zf0 = (double *)(b + o_zf), *si = (double *)(b + o_si);
+    hipLaunchKernelGGL(k_conv_zero, dim3(cdiv(nrows, 64)), dim3(64), 0, st, kex, nrows);
+    hipLaunchKernelGGL(k_conv_boxes, dim3(cdiv(nq, 256)), dim3(256), 0, st, a, g0, ge, kk, zf0);
+    hipLaunchKernelGGL(k_conv_si, dim3(cdiv(nq, 256), (unsigned)nrows), dim3(256), 0, st, a, xs, ldx, kk, si, kex);
+    SCHK(hipGetLastError());
+    if (cp.exhausted) {   // the count follows the last nonzero box of each row
+        std::vector<int32_t> kx((size_t)nrows);
+        SCHK(hipMemcpyAsync(kx.data(), kex, 4 * (size_t)nrows, hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        *nout = cp.nout_of(kx[0]);
+        for (int64_t r = 1; r < nrows; r++)
+            if (cp.nout_of(kx[r]) != *nout)
+                return fail(CLBLM_ERR_ARG, "scan: the rows end on different output counts (the input ends before "
+                                           "V2 + bound and trailing zeros differ); scan them separately");
+        if (count_only) return 0;
+    }
+    if (ld_out < *nout) return fail(CLBLM_ERR_ARG, "ld_out is below the output length");
+    hipLaunchKernelGGL(k_conv_sum, dim3(cdiv(*nout, CONV_TILE), (unsigned)nrows), dim3(CONV_TILE), 0, st, a, g0, ge,
+                       kk, zf0, si, (const double *)(b + o_xf), cp.nfmax, d_out, ld_out, *nout);
+    SCHK(hipGetLastError());
+    return 0;
+}
+
+// clblm_INTERP of each row onto the plan's grid (HWHM/12) into S.xs.
+int regrid_rows(ScanState &S, const IPlan &ip, const double *d_in, int64_t nrows, int64_t ld_in, hipStream_t st) {
+    const int64_t ns = ip.nout;
+    auto up = [](size_t o) { return (o + 255) & ~(size_t)255; };
+    const size_t o_seg = up(sizeof(ICoef));
+    const size_t meta = o_seg + sizeof(ISeg) * ip.segs.size() + 256;
+    SCHK(hipStreamSynchronize(st));  // the metadata buffer may still be read by a previous scan
+    int rc = grow(&S.meta, &S.meta_cap, meta);
+    if (rc) return rc;
+    rc = grow((void **)&S.xs, &S.xs_cap, sizeof(double) * (size_t)ns * (size_t)nrows);
+    if (rc) return rc;
+    std::vector<char> host(meta, 0);
+    const ICoef cf = make_coef();
+    std::memcpy(host.data(), &cf, sizeof cf);
+    if (!ip.segs.empty()) std::memcpy(host.data() + o_seg, ip.segs.data(), sizeof(ISeg) * ip.segs.size());
+    SCHK(hipMemcpyAsync(S.meta, host.data(), meta, hipMemcpyHostToDevice, st));
+    SCHK(hipStreamSynchronize(st));  // the host copy goes out of scope
+    char *mb = (char *)S.meta;
+    for (int64_t r = 0; r < nrows; r++)
+        hipLaunchKernelGGL(k_scan_interp<double>, dim3(cdiv((int64_t)ip.segs.size(), 64)), dim3(64), 0, st,
+                           (const ISeg *)(mb + o_seg), (int64_t)ip.segs.size(), d_in + r * ld_in, (const ICoef *)mb,
+                           ip.vratio, S.xs + r * ns);
+    SCHK(hipGetLastError());
+    return 0;
+}
+
+// functIDs 2..5, -4, -5 (JFN 1..4, clblm_ScanFilter.f90:239-334)
+int scanfn_conv(ScanState &S, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1, double dv,
+                double v1out, double v2out, double dvout, int funct_id, double hwhm, double *d_out, int64_t ld_out,
+                int64_t *nout, double *v1res, double *dvres, hipStream_t st) {
+    int jfn = std::abs(funct_id) - 1;
+    if (jfn < 1 || jfn > 4) return fail(CLBLM_ERR_UNSUPPORTED, "scan: functID must be 1..5, -4 or -5");
+    double HWHM = hwhm;
+    if (funct_id < 0) {   // the first zero crossing in place of the half width (:272-280)
+        if (jfn != 3 && jfn != 4) return fail(CLBLM_ERR_ARG, "scan: a negative functID is only for -4 and -5");
+        HWHM = HWHM / (2. * std::asin(1.) / SCAN_FN[jfn].xscal);
+    }
+    const double DVINT = HWHM / 12.;
+    IPlan ip;
+    const bool pre = dv > DVINT;
+    int64_t ns = n;
+    double sv1 = v1, sdv = dv;
+    if (pre) {
+        int rc = plan_interp(n, v1, dv, v1, v1 + dv * (double)(n - 1), DVINT, ip);
+        if (rc) return rc;
+        ns = ip.nout;
+        sv1 = ip.v1;
+        sdv = DVINT;
+    }
+    ConvPlan cp;
+    int rc = plan_conv(ns, sv1, sdv, v1out, v2out, dvout, jfn, HWHM, cp);
+    if (rc) return rc;
+    *v1res = v1out;
+    *dvres = cp.dvo;
+    const bool count_only = d_out == nullptr;
+    const bool need_data = !count_only || (cp.exhausted && cp.qhi >= cp.qlo);
+    const double *xs = d_in;
+    int64_t ldx = ld_in;
+    if (need_data) {
+        if (!d_in) return fail(CLBLM_ERR_ARG, "input spectrum is NULL (the output count depends on it here)");
+        if (pre) {
+            rc = regrid_rows(S, ip, d_in, nrows, ld_in, st);
+            if (rc) return rc;
+            xs = S.xs;
+            ldx = ns;
+        }
+    }
+    return run_conv(S, cp, need_data ? xs : nullptr, ldx, nrows, count_only, d_out, ld_out, nout, st);
+}
+
 }  // namespace
 
 extern "C" __attribute__((visibility("hidden"))) int clblm_scanfn_impl(
     void *state, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1, double dv, double v1out,
     double v2out, double dvout, int funct_id, double hwhm, double *d_out, int64_t ld_out, int64_t *nout,
     double *v1res, double *dvres, void *stream) {
+    if (!(hwhm > 0.) || !(dv > 0.) || n < 8 || nrows < 1 || nrows > 65535 || ld_in < n)
+        return fail(CLBLM_ERR_ARG, "hwhm, dv must be positive, n >= 8, 1 <= nrows <= 65535, ld_in >= n");
+    ScanState &S = *(ScanState *)state;
+    hipStream_t st = (hipStream_t)stream;
     if (funct_id != 1)
-        return fail(CLBLM_ERR_UNSUPPORTED, "only the rectangle (functID 1) spectral-domain scan is supported");
-    if (!(hwhm > 0.) || !(dv > 0.) || !(dvout != 0.) || n < 8 || nrows < 1 || nrows > 65535 || ld_in < n)
-        return fail(CLBLM_ERR_ARG, "hwhm, dv must be positive, dvout nonzero, n >= 8, 1 <= nrows <= 65535, ld_in >= n");
+        return scanfn_conv(S, d_in, n, nrows, ld_in, v1, dv, v1out, v2out, dvout, funct_id, hwhm, d_out, ld_out,
+                           nout, v1res, dvres, st);
+    if (!(dvout != 0.)) return fail(CLBLM_ERR_ARG, "the rectangle scan needs a nonzero dvout");
     // clblm_INTERP onto HWHM/12 when the input is coarser (clblm_ScanFilter.f90:331-334)
     const double DVINT = hwhm / 12.;
     IPlan ip;
@@ -1072,37 +1546,27 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_scanfn_impl(
     if (!d_out) return 0;
     if (!d_in) return fail(CLBLM_ERR_ARG, "input spectrum is NULL");
     if (ld_out < (rp.scan ? rp.nout : ns)) return fail(CLBLM_ERR_ARG, "ld_out is below the output length");
-    ScanState &S = *(ScanState *)state;
-    hipStream_t st = (hipStream_t)stream;
-    auto up = [](size_t o) { return (o + 255) & ~(size_t)255; };
-    const size_t o_cf = 0, o_seg = up(sizeof(ICoef));
-    const size_t o_box = up(o_seg + sizeof(ISeg) * ip.segs.size());
-    const size_t o_part = up(o_box + sizeof(RBox) * rp.boxes.size());
-    const size_t meta = o_part + sizeof(RPart) * rp.parts.size() + 256;
-    SCHK(hipStreamSynchronize(st));  // the metadata buffer may still be read by a previous scan
-    int rc = grow(&S.meta, &S.meta_cap, meta);
-    if (rc) return rc;
-    std::vector<char> host(meta, 0);
-    const ICoef cf = make_coef();
-    std::memcpy(host.data() + o_cf, &cf, sizeof cf);
-    if (!ip.segs.empty()) std::memcpy(host.data() + o_seg, ip.segs.data(), sizeof(ISeg) * ip.segs.size());
-    if (!rp.boxes.empty()) std::memcpy(host.data() + o_box, rp.boxes.data(), sizeof(RBox) * rp.boxes.size());
-    if (!rp.parts.empty()) std::memcpy(host.data() + o_part, rp.parts.data(), sizeof(RPart) * rp.parts.size());
-    SCHK(hipMemcpyAsync(S.meta, host.data(), meta, hipMemcpyHostToDevice, st));
-    SCHK(hipStreamSynchronize(st));  // the host copy goes out of scope
-    char *mb = (char *)S.meta;
     const double *xs = d_in;
     int64_t ldx = ld_in;
     if (pre) {
-        rc = grow((void **)&S.xs, &S.xs_cap, sizeof(double) * (size_t)ns * (size_t)nrows);
+        int rc = regrid_rows(S, ip, d_in, nrows, ld_in, st);
         if (rc) return rc;
-        for (int64_t r = 0; r < nrows; r++)
-            hipLaunchKernelGGL(k_scan_interp<double>, dim3(cdiv((int64_t)ip.segs.size(), 64)), dim3(64), 0, st,
-                               (const ISeg *)(mb + o_seg), (int64_t)ip.segs.size(), d_in + r * ld_in,
-                               (const ICoef *)(mb + o_cf), ip.vratio, S.xs + r * ns);
         xs = S.xs;
         ldx = ns;
     }
+    auto up = [](size_t o) { return (o + 255) & ~(size_t)255; };
+    const size_t o_box = 0;
+    const size_t o_part = up(o_box + sizeof(RBox) * rp.boxes.size());
+    const size_t meta = o_part + sizeof(RPart) * rp.parts.size() + 256;
+    SCHK(hipStreamSynchronize(st));  // the buffer may still be read by a previous scan
+    int rc = grow(&S.conv, &S.conv_cap, meta);
+    if (rc) return rc;
+    std::vector<char> host(meta, 0);
+    if (!rp.boxes.empty()) std::memcpy(host.data() + o_box, rp.boxes.data(), sizeof(RBox) * rp.boxes.size());
+    if (!rp.parts.empty()) std::memcpy(host.data() + o_part, rp.parts.data(), sizeof(RPart) * rp.parts.size());
+    SCHK(hipMemcpyAsync(S.conv, host.data(), meta, hipMemcpyHostToDevice, st));
+    SCHK(hipStreamSynchronize(st));  // the host copy goes out of scope
+    char *mb = (char *)S.conv;
     const unsigned ny = (unsigned)nrows;
     if (!rp.scan) {
         hipLaunchKernelGGL(k_rect_copy, dim3(cdiv(ns, 256), ny), dim3(256), 0, st, xs, ldx, ns, d_out, ld_out);

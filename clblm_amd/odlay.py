@@ -205,9 +205,11 @@ class Context:
         return out, a.value, b.value
 
     def scanfn(self, x, v1, dv, v1out, v2out, dvout, funct_id=1, hwhm=0.1, stream=None):
-        """clblm_SCANFN_array (src/clblm_src/clblm_ScanFilter.f90:33-67), rectangle
-        (funct_id 1) only, on a float64 cuda tensor x of spectra on v1 + i*dv along its
-        last axis: returns (scanned tensor, its v1, its dv)."""
+        """clblm_SCANFN_array (src/clblm_src/clblm_ScanFilter.f90:33-67) on a float64 cuda
+        tensor x of spectra on v1 + i*dv along its last axis: funct_id 1 rectangle, 2
+        triangle, 3 Gaussian, 4 sinc**2, 5 sinc (-4, -5: hwhm is the first zero
+        crossing); dvout 0 takes the function's default sampling.  Returns (scanned
+        tensor, its v1, its dv)."""
         import torch
         assert x.dtype == torch.float64 and x.is_cuda
         shape = x.shape
@@ -216,11 +218,12 @@ class Context:
         nout, a, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
         args = (int(n), int(rows), int(n), float(v1), float(dv), float(v1out), float(v2out), float(dvout),
                 int(funct_id), float(hwhm))
-        check(self.lib.clblm_scanfn(self.ptr, None, *args, None, 0, ctypes.byref(nout), ctypes.byref(a),
-                                    ctypes.byref(b), None))
-        out = torch.empty((rows, max(1, nout.value)), dtype=torch.float64, device=x.device)
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
+        # the count can depend on the data (functions 2..5), so the query reads x too
+        check(self.lib.clblm_scanfn(self.ptr, ctypes.c_void_p(x2.data_ptr()), *args, None, 0, ctypes.byref(nout),
+                                    ctypes.byref(a), ctypes.byref(b), ctypes.c_void_p(stream)))
+        out = torch.empty((rows, max(1, nout.value)), dtype=torch.float64, device=x.device)
         check(self.lib.clblm_scanfn(self.ptr, ctypes.c_void_p(x2.data_ptr()), *args, ctypes.c_void_p(out.data_ptr()),
                                     int(out.shape[1]), ctypes.byref(nout), ctypes.byref(a), ctypes.byref(b),
                                     ctypes.c_void_p(stream)))

@@ -351,16 +351,27 @@ int clblm_level_jac(clblm_ctx *ctx, int nlay, int nmol, int64_t npts, const doub
                     const double *d_dw_dqup, const double *d_dw_dqlo, double *d_jac_t, double *d_jac_q,
                     void *stream);
 
-/* Spectral-domain scan: clblm_SCANFN_array (src/clblm_src/clblm_ScanFilter.f90:33-67)
- * for functID 1, the rectangle of adjacent boxes of width 2 hwhm every dvout
- * (CNVRCT, :1478-1698), which output_JacArray applies as the boxcar pre-average of
- * Jacobian arrays (clblm_noScattJac.f90:1341-1409).  d_in: nrows spectra of n
- * device doubles on v1 + i dv, row r at d_in + r ld_in; output row r at
- * d_out + r ld_out (ld_out >= *nout); output limits v1out..v2out.  The input is first interpolated onto
- * hwhm/12 when dv is coarser; the output step is dvout rounded to a whole number of
- * input steps; with fewer than two input steps per box the (interpolated) input is
- * returned unchanged, as in the reference.  Two-phase: d_out == NULL returns *nout,
- * *v1res, *dvres only.  Other scanning functions return CLBLM_ERR_UNSUPPORTED. */
+/* Spectral-domain scan: clblm_SCANFN_array (src/clblm_src/clblm_ScanFilter.f90:33-67).
+ * d_in: nrows spectra of n device doubles on v1 + i dv, row r at d_in + r ld_in;
+ * output row r at d_out + r ld_out (ld_out >= *nout); output limits v1out..v2out.
+ * The input is first interpolated onto hwhm/12 when dv is coarser (:331-334).
+ *   funct_id 1  the rectangle of adjacent boxes of width 2 hwhm every dvout (CNVRCT,
+ *            :1478-1698), which output_JacArray applies as the boxcar pre-average of
+ *            Jacobian arrays (clblm_noScattJac.f90:1341-1409): the output step is dvout
+ *            rounded to a whole number of input steps; with fewer than two input steps
+ *            per box the (interpolated) input is returned unchanged.  dvout != 0.
+ *   2..5     triangle, Gaussian, sinc**2, sinc of half width hwhm, tabulated and
+ *            convolved as SHRKSC / CONVSC do (:551-955), bit for bit the reference's
+ *            sums; -4 and -5 take hwhm as the first zero crossing (:272-280).  Output
+ *            step |dvout|, or hwhm / (the function's default sampling) when dvout is 0.
+ *            CLBLM_ERR_ARG where the reference would overrun its 5000-point R1 buffer
+ *            or its table (dvout too fine or too coarse).  When the input ends before
+ *            v2out + the function's bound, the reference writes panels of 2400 points
+ *            only up to the one holding its last nonzero input: then the count depends
+ *            on the data, the query reads d_in (and waits for it on `stream`), and rows
+ *            whose counts differ return CLBLM_ERR_ARG.
+ * Two-phase: d_out == NULL returns *nout, *v1res, *dvres only (d_in may be NULL for
+ * functID 1 and whenever the count does not depend on the data). */
 int clblm_scanfn(clblm_ctx *ctx, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1,
                  double dv, double v1out, double v2out, double dvout, int funct_id, double hwhm, double *d_out,
                  int64_t ld_out, int64_t *nout, double *v1res, double *dvres, void *stream);

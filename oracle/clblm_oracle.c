@@ -2372,3 +2372,241 @@ int64_t orc_scanfn_rect(const double *x, int64_t n, double v1, double dv, double
     free(R1); free(S); free(spec);
     return nout;
 }
+
+/* ---------------------------------------------------------------------------
+ * clblm_SCANFN_struct (clblm_ScanFilter.f90:70-544) for functIDs 2..5 (JFN 1..4:
+ * triangle, Gaussian, sinc**2, sinc; functID -4 / -5 take the first zero crossing
+ * in place of the half width, :272-280): the scanning function tabulated by
+ * SHAPET / SHAPEG / SINCSQ / SINC (:551-723), the input compressed by SHRKSC
+ * (:732-850, boxes of IRATSH points carried across input panels), convolved by
+ * CONVSC (:856-955) into the 5000-point R1 buffer, read by panelInput (:2320-2404)
+ * and written by panelOutput (:2411-2506).
+ * CONVSC entered with ILO > IHI reads an unset ILAST (:942); this restatement takes
+ * that call as processing nothing (ILO stays above IHI), which is what reaching
+ * label 60 without a loop pass means for every later call.
+ * Returns the output count (writes min(count, cap) values), -1 on error.
+ * ------------------------------------------------------------------------- */
+typedef struct { double hwf, dxf; int nf, nfmax; double smpl, xscal; } scan_fn_t;
+static const scan_fn_t SCN_FN[5] = {   /* HWJ, DXJ, NJ, NJMX, SMPLJ, XSCAL (:182-207) */
+    {1., 0.0, 0, 0, .5, 0.},
+    {2., 0.02, 101, 251, 2., 0.},
+    {4., 0.02, 201, 251, 4., 0.},
+    {54.1826, 0.02, 2710, 2760, 4., 1.391557377},
+    {119.332818, 0.02, 5968, 6018, 4., 1.89549425},
+};
+
+/* SHAPET, SHAPEG, SINCSQ, SINC: xf[1..nfmax] (1-based), zero past nf. */
+void orc_scan_shape(int jfn, double *xf) {
+    const scan_fn_t *f = &SCN_FN[jfn];
+    const double PI = 2. * asin(1.);
+    for (int i = 1; i <= f->nfmax; i++) xf[i] = 0.;
+    if (jfn == 1) {
+        xf[1] = 0.5;
+        for (int i = 2; i <= f->nf; i++) {
+            double X = (double)(i - 1) * f->dxf;
+            xf[i] = 0.5 * (1. - 0.5 * X);
+        }
+    } else if (jfn == 2) {
+        const double FLN2 = log(2.), RECPI = 1. / (2. * asin(1.)), FGNORM = sqrt(FLN2 * RECPI);
+        xf[1] = FGNORM * exp(-FLN2 * 0.);
+        for (int j = 2; j <= f->nf; j++) {
+            double X = (double)(j - 1) * f->dxf, XSQ = X * X;
+            xf[j] = FGNORM * exp(-FLN2 * XSQ);
+        }
+        xf[f->nf] = 0.;
+    } else {
+        const double XNORM = f->xscal / PI;
+        xf[1] = XNORM;
+        for (int i = 2; i <= f->nf; i++) {
+            double X = (double)(i - 1) * f->dxf, T = X * f->xscal, q = sin(T) / T;
+            xf[i] = (jfn == 3) ? XNORM * (q * q) : XNORM * q;
+        }
+    }
+}
+
+int64_t orc_scanfn_conv(const double *x, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
+                        int funct_id, double hwhm, double *out, int64_t cap, double *v1res, double *dvres) {
+    const double PI = 2. * asin(1.);
+    int JFN = abs(funct_id) - 1;
+    if (funct_id < 0) JFN = -JFN;
+    if (!(hwhm > 0.) || n < 1) return fail("scan: bad half width or empty input");
+    if (JFN == 0 || abs(JFN) > 4) return fail("scan: functID must be 2..5 or -4, -5");
+    const scan_fn_t *F = &SCN_FN[abs(JFN)];
+    double HWHM = hwhm;
+    if (JFN < 0) {
+        JFN = -JFN;
+        if (JFN != 3 && JFN != 4) return fail("scan: negative functID only for sinc**2 and sinc");
+        HWHM = HWHM / (PI / F->xscal);
+    }
+    const double DVINT = HWHM / 12.;
+    const double SAMPL = -fabs(dvout);
+    double SAMPLE = F->smpl;
+    if (SAMPL < 0.) SAMPLE = HWHM / (-SAMPL);
+    double XF[6019];
+    orc_scan_shape(JFN, XF);
+    double *spec = NULL;
+    int64_t ns = n;
+    double sv1 = v1, sdv = dv;
+    if (dv > DVINT) {   /* clblm_INTERP onto DVINT (:331-334) */
+        double v2 = v1 + dv * (double)(n - 1), r1 = 0.;
+        ns = orc_scan_interp(x, n, v1, dv, v1, v2, DVINT, NULL, 0, &r1);
+        if (ns <= 0) return -1;
+        spec = (double *)malloc(sizeof(double) * (size_t)ns);
+        orc_scan_interp(x, n, v1, dv, v1, v2, DVINT, spec, ns, &r1);
+        sv1 = r1;
+        sdv = DVINT;
+    }
+    const double *sp = spec ? spec : x;
+    const double DVO = HWHM / SAMPLE;
+    const double HWF = F->hwf, DXF = F->dxf;
+    const int NLIMF = 2401, NSHIFT = 32;
+    const int NBOUND = (int)((2. * HWF) * SAMPLE + 0.01);
+    const double BOUND = (double)NBOUND * DVO / 2.;
+    const int MAXF = NLIMF + 2 * NBOUND + NSHIFT;
+    if (MAXF > 5000) { free(spec); return fail("scan: output spacing too fine for R1(5000)"); }
+    /* CONVSC's table index stays below (ZBOUND + 1/2) ZSLOPE + 1.5; past NJMX the
+     * reference reads outside its table */
+    if ((HWF / DVO * HWHM + 0.5) * (DVO / DXF / HWHM) * (1. + 1e-9) + 1.5 >= (double)(F->nfmax + 1)) {
+        free(spec);
+        return fail("scan: output spacing too coarse for the scanning function's table");
+    }
+    const double V1 = v1out, V2 = v2out;
+    int NLO = NSHIFT + 1, NHI = NLIMF + NSHIFT - 1;
+    double *R1 = (double *)calloc((size_t)MAXF + 2, sizeof(double));
+    double *S = (double *)calloc(3851, sizeof(double)), *SS = (double *)calloc(5001, sizeof(double));
+    int IEOFSC = 1, IDATA = -1, IPANEL = -1, ISTOP = 0, INIT = 0, NREN = 0, ILO = 1, IHI = 0, NNI = 0;
+    double VFT = V1 - (double)NSHIFT * DVO;
+    const double VBOT = V1 - BOUND, VTOP = V2 + BOUND;
+    int64_t done = 0, nout = 0;
+    double done_v2 = sv1 - sdv;
+    double V1I = 0., V2I = 0., DVI = sdv;
+    /* SHRKSC's SAVE variables */
+    double DVSC = 0., SRATIO = 1., V1SHFT = 0.;
+    int IRATSH = 1, IRATM1 = 0;
+    static const int JRATIO[24] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 25, 30, 32, 40, 48, 50, 60, 75, 80, 100, 120};
+    int state = 40;
+    for (;;) {
+        if (state == 40) {
+            if (IEOFSC <= 0) { state = 60; continue; }
+            /* panelInput */
+            int nlim;
+            for (;;) {
+                nlim = 2400;
+                if (done + nlim > ns) nlim = (int)(ns - done);
+                if (nlim == 0) { IEOFSC = 0; break; }
+                DVI = sdv;
+                V1I = done_v2 + DVI;
+                V2I = V1I + (double)(nlim - 1) * DVI;
+                done += nlim;
+                done_v2 = V2I;
+                if (V2I >= VBOT) break;
+            }
+            if (IEOFSC <= 0) { state = 60; continue; }
+            const int NLOW = NREN > 0 ? NREN + 1 : 1;
+            V1I = V1I - (double)(NLOW - 1) * DVI;
+            NNI = nlim + NLOW - 1;
+            IDATA = 0;
+            for (int i = 0; i < nlim; i++) S[NLOW + i] = sp[done - nlim + i];
+            ILO = 1;
+            IHI = NNI;
+            if (V2I > VTOP) {
+                IHI = (int)((VTOP - V1I) / DVI + 1.5);
+                IDATA = 1;
+            }
+            /* SHRKSC */
+            {
+                const int NLIMS = NNI;
+                for (int i = 1; i <= NREN; i++) S[i] = SS[i];
+                NREN = 0;
+                int skip = 0;
+                if (INIT == 0) {
+                    DVSC = HWHM / 12.;
+                    IRATSH = (int)(DVSC / DVI + 0.5);
+                    for (int i = 2; i <= 24; i++)
+                        if (JRATIO[i - 1] > IRATSH) { IRATSH = JRATIO[i - 2]; break; }
+                    if (IRATSH > JRATIO[23]) IRATSH = JRATIO[23];
+                    if (IRATSH <= 1) skip = 1;
+                    else {
+                        DVSC = (double)IRATSH * DVI;
+                        V1SHFT = (double)(IRATSH - 1) * DVI / 2.;
+                        SRATIO = IRATSH;
+                        IRATM1 = IRATSH - 1;
+                        INIT = 1;
+                    }
+                }
+                if (!skip && IRATSH > 1) {
+                    NREN = NNI - (NNI / IRATSH) * IRATSH;
+                    V1I = V1I + V1SHFT;
+                    const int IMAX = NNI - IRATM1 - NREN;
+                    int K = 0;
+                    for (int I = 1; I <= IMAX; I += IRATSH) {
+                        double SUMK = 0.;
+                        K = K + 1;
+                        for (int J = I; J <= I + IRATM1; J++) SUMK = SUMK + S[J];
+                        S[K] = SUMK / SRATIO;
+                    }
+                    V2I = V1I + DVSC * (double)(K - 1);
+                    NNI = K;
+                    DVI = DVSC;
+                    ILO = (int)((VBOT - V1I) / DVI + 1.5);
+                    if (ILO < 1) ILO = 1;
+                    IHI = (int)((VTOP - V1I) / DVI + 1.5);
+                    if (IHI > NNI) IHI = NNI;
+                    for (int i = 1; i <= NREN; i++) SS[i] = S[NLIMS - NREN + i];
+                }
+            }
+            state = 50;
+        }
+        if (state == 50) {
+            /* CONVSC (JVAR = 0) */
+            if (ILO > IHI) {
+                IPANEL = 1;
+            } else {
+                const double RATIO = DVI / DVO, DVODX = DVO / DXF, HWBND = HWF / DVO;
+                const double ZINT = (V1I - VFT) / DVO;
+                const double ZSLOPE = DVODX / HWHM, ZBOUND = HWBND * HWHM, XNORM = DVI / HWHM;
+                int ILAST = IHI, full = 0;
+                for (int I = ILO; I <= IHI; I++) {
+                    if (S[I] == 0.) continue;
+                    const double ZPEAK = (double)(I - 1) * RATIO + ZINT;
+                    int JMAX = (int)(ZPEAK + ZBOUND + 1.5);
+                    if (JMAX > MAXF) { ILAST = I - 1; full = 1; break; }
+                    int JMIN = (int)(ZPEAK - ZBOUND + 1.5);
+                    if (JMIN < 1) JMIN = 1;
+                    const double SI = XNORM * S[I];
+                    double ZF = ((double)(JMIN - 1) - ZPEAK) * ZSLOPE;
+                    for (int JF = JMIN; JF <= JMAX; JF++) {
+                        int IT = (int)(fabs(ZF) + 1.5);
+                        R1[JF] = R1[JF] + SI * XF[IT];
+                        ZF = ZF + ZSLOPE;
+                    }
+                }
+                IPANEL = full ? 1 : IDATA;
+                ILO = ILAST + 1;
+            }
+            if (IPANEL == 0) { state = 40; continue; }
+            state = 60;
+        }
+        /* 60: panelOutput */
+        ISTOP = 0;
+        int NNHI = (int)((V2 - VFT) / DVO + 1.5);
+        if (NHI >= NNHI) ISTOP = 1;
+        if (ISTOP == 1) NHI = NNHI;
+        for (int j = NLO; j <= NHI; j++, nout++)
+            if (nout < cap) out[nout] = R1[j];
+        if (NHI < NLO) nout += NHI - NLO + 1;   /* a negative count shortens what came before */
+        VFT = VFT + (double)(NLIMF - 1) * DVO;
+        if (ISTOP != 1) {
+            for (int J = NLIMF; J <= MAXF; J++) R1[J - NLIMF + 1] = R1[J];
+            for (int J = MAXF - NLIMF + 2; J <= MAXF; J++) R1[J] = 0.;
+            NLO = NSHIFT + 1;
+        }
+        if (ISTOP != 1 && IEOFSC < 0) { state = 60; continue; }
+        if (ISTOP != 1 && IEOFSC > 0) { state = 50; continue; }
+        break;
+    }
+    *v1res = V1;
+    *dvres = DVO;
+    free(R1); free(S); free(SS); free(spec);
+    return nout;
+}

@@ -130,6 +130,9 @@ typedef struct {
 } orc_scan_plan;
 int64_t orc_scan_interp(const double *x, int64_t n, double v1, double dv, double vv1, double vv2, double dvo,
                         double *out, int64_t cap, double *v1res);
+void orc_scan_shape(int jfn, double *xf);
+int64_t orc_scanfn_conv(const double *x, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
+                        int funct_id, double hwhm, double *out, int64_t cap, double *v1res, double *dvres);
 int64_t orc_scanfn_rect(const double *x, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
                         double hwhm, double *out, int64_t cap, double *v1res, double *dvres);
 int orc_fftscn_plan2(double v1c, double dv, int64_t n, double v1out, double v2out, int funct_id, double hwhm,

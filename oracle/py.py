@@ -298,6 +298,37 @@ def scanfn_rect(x, v1, dv, v1out, v2out, dvout, hwhm):
     return out, a.value, b.value
 
 
+def scanfn(x, v1, dv, v1out, v2out, dvout, funct_id, hwhm):
+    """orc_scanfn_rect (functID 1) or orc_scanfn_conv (2..5, -4, -5): the spectral-domain
+    scan clblm_SCANFN_array; returns (values, v1, dv)."""
+    if funct_id == 1:
+        return scanfn_rect(x, v1, dv, v1out, v2out, dvout, hwhm)
+    L = lib()
+    L.orc_scanfn_conv.restype = ctypes.c_int64
+    L.orc_scanfn_conv.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_double] * 5 + \
+        [ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+         ctypes.POINTER(ctypes.c_double)]
+    x = np.ascontiguousarray(x, np.float64)
+    a, b = ctypes.c_double(), ctypes.c_double()
+    n = L.orc_scanfn_conv(x.ctypes.data, x.size, v1, dv, v1out, v2out, dvout, funct_id, hwhm, None, 0,
+                          ctypes.byref(a), ctypes.byref(b))
+    if n < 0:
+        raise RuntimeError(L.orc_error().decode())
+    out = np.zeros(max(n, 1))
+    L.orc_scanfn_conv(x.ctypes.data, x.size, v1, dv, v1out, v2out, dvout, funct_id, hwhm, out.ctypes.data, n,
+                      ctypes.byref(a), ctypes.byref(b))
+    return out[:n], a.value, b.value
+
+
+def scan_shape(jfn):
+    """orc_scan_shape: the tabulated scanning function XF(1..NJMX) of JFN 1..4."""
+    L = lib()
+    L.orc_scan_shape.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    xf = np.zeros(6019)
+    L.orc_scan_shape(jfn, xf.ctypes.data)
+    return xf[1:{1: 252, 2: 252, 3: 2761, 4: 6019}[jfn]]
+
+
 def prebox(x, v1, dv, boxcar_hw):
     """output_JacArray's boxcar pre-average (clblm_noScattJac.f90:1341-1409): the
     rectangle scan with M = int(2 boxcarHW / dv) points per box onto

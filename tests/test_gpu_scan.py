@@ -142,3 +142,100 @@ def test_rect_scan_matches_oracle(ctx, n, dv, hw):
     got = got.cpu().numpy()
     assert (got.size, g1, gd) == (want.size, w1, wd), ((got.size, g1, gd), (want.size, w1, wd))
     assert np.array_equal(got, want), float(np.abs(got - want).max())
+
+
+def _conv_gpu(ctx, x, v1, dv, a, b, dvo, fid, hw):
+    import torch
+    y, g1, gd = ctx.scanfn(torch.tensor(x, device="cuda"), v1, dv, a, b, dvo, fid, hw)
+    torch.cuda.synchronize()
+    return y.cpu().numpy(), g1, gd
+
+
+def _conv_cases():
+    from test_oracle_scan import CONV_CASES
+    return CONV_CASES
+
+
+@pytest.mark.parametrize("n,dv,v1o,v2o,dvo,fid,hw,tail", _conv_cases())
+def test_conv_scan_matches_oracle_bitwise(ctx, n, dv, v1o, v2o, dvo, fid, hw, tail):
+    """Triangle, Gaussian, sinc**2 and sinc scans (clblm_SCANFN functIDs 2..5, -4,
+    -5) on the GPU: the oracle's grid and count and bit-identical values, the oracle
+    being bit-identical to the reference's routine on these cases."""
+    from oracle.py import scanfn
+    from test_oracle_scan import _conv_input
+    x = _conv_input(n, dv, tail)
+    print("case", n, dv, v1o, v2o, dvo, fid, hw, tail, flush=True)
+    want, w1, wd = scanfn(x, 600.0, dv, v1o, v2o, dvo, fid, hw)
+    got, g1, gd = _conv_gpu(ctx, x, 600.0, dv, v1o, v2o, dvo, fid, hw)
+    assert (got.size, g1, gd) == (want.size, w1, wd), ((got.size, g1, gd), (want.size, w1, wd))
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+
+
+def test_conv_scan_random_cases_bitwise(ctx):
+    """Thirty seeded random scans, each bit-identical to the oracle (count and grid
+    included): zero runs, zero tails, all-zero inputs, every function and sampling."""
+    from oracle.py import scanfn
+    rng = np.random.default_rng(77)
+    done = 0
+    while done < 30:
+        fid = int(rng.choice([2, 3, 4, 5, -4, -5]))
+        n = int(rng.integers(50, 60000))
+        dv = float(rng.choice([0.001, 0.002, 0.0037, 0.01, 0.05]))
+        hw = float(rng.uniform(0.02, 1.5))
+        v1 = 600.0 + float(rng.uniform(-1, 1))
+        span = dv * (n - 1)
+        a = v1 + float(rng.uniform(-0.2, 0.8)) * span
+        b = a + float(rng.uniform(0.05, 1.2)) * span
+        dvo = float(rng.choice([0.0, hw / rng.uniform(1.6, 5.0)]))
+        x = 1.0 + 0.5 * np.sin(np.arange(n) * dv / 0.7) + 0.1 * rng.normal(size=n)
+        kind = int(rng.integers(0, 4))
+        if kind == 1:
+            x[-int(rng.integers(1, n // 2 + 1)):] = 0.0
+        elif kind == 2:
+            x[rng.uniform(size=n) < 0.3] = 0.0
+        elif kind == 3 and rng.uniform() < 0.3:
+            x[:] = 0.0
+        try:
+            want, w1, wd = scanfn(x, v1, dv, a, b, dvo, fid, hw)
+        except RuntimeError:
+            continue
+        print("case", fid, n, dv, hw, a, b, dvo, kind, flush=True)
+        got, g1, gd = _conv_gpu(ctx, x, v1, dv, a, b, dvo, fid, hw)
+        assert (got.size, g1, gd) == (want.size, w1, wd), (fid, n, dv, hw, a, b, dvo)
+        assert np.array_equal(got, want), (fid, n, dv, hw, a, b, dvo, float(np.abs(got - want).max()))
+        done += 1
+
+
+def test_conv_scan_rows(ctx):
+    """Three rows scanned in one call equal three single scans; rows whose counts
+    differ (the input ends before V2 + bound and one row ends in zeros) are refused."""
+    import torch
+    from clblm_amd._lib import ClblmError
+    from oracle.py import scanfn
+    rng = np.random.default_rng(8)
+    n, dv = 40001, 0.002
+    xs = 1.0 + 0.5 * np.sin(np.arange(n)[None, :] * dv / np.array([[0.7], [0.3], [1.9]])) + \
+        0.05 * rng.normal(size=(3, n))
+    y, g1, gd = ctx.scanfn(torch.tensor(xs, device="cuda"), 600.0, dv, 610.0, 670.0, 0.02, 5, 0.08)
+    y = y.cpu().numpy()
+    for r in range(3):
+        want, w1, wd = scanfn(xs[r], 600.0, dv, 610.0, 670.0, 0.02, 5, 0.08)
+        assert (y.shape[1], g1, gd) == (want.size, w1, wd)
+        assert np.array_equal(y[r], want)
+    xs[1, -n // 2:] = 0.0
+    with pytest.raises(ClblmError):
+        ctx.scanfn(torch.tensor(xs, device="cuda"), 600.0, dv, 610.0, 700.0, 0.02, 3, 0.08)
+
+
+def test_conv_scan_large(ctx):
+    """A 2 M-point spectrum at 0.001 cm-1 scanned by the sinc (HWHM 0.5, 72 zero
+    crossings each side): bit-identical to the oracle."""
+    from oracle.py import scanfn
+    rng = np.random.default_rng(12)
+    n, dv = 2000001, 0.001
+    v = 500.0 + dv * np.arange(n)
+    x = 1.0 + 0.5 * np.sin(v / 3.0) + 0.2 * np.exp(-((v - 1500.0) / 0.05) ** 2) + 0.05 * rng.normal(size=n)
+    want, w1, wd = scanfn(x, 500.0, dv, 600.0, 2400.0, 0.125, 5, 0.5)
+    got, g1, gd = _conv_gpu(ctx, x, 500.0, dv, 600.0, 2400.0, 0.125, 5, 0.5)
+    assert (got.size, g1, gd) == (want.size, w1, wd)
+    assert np.array_equal(got, want), float(np.abs(got - want).max())

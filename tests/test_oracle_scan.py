@@ -255,3 +255,132 @@ def test_rect_scan_closed_forms():
     ok = (centres > v1 + 2 * hw) & (centres < v1 + dv * (n - 1) - 2 * hw)
     assert ok.sum() > 150
     assert np.allclose(got[ok], 2.0 + 0.5 * centres[ok], rtol=1e-14, atol=0)
+
+
+# (points, dv, v1out, v2out, dvout, functID, hwhm, tail): the spectral-domain scans
+# with a tabulated scanning function.  tail: "zeros" ends the input in zeros (the
+# reference's count then follows the last nonzero box), "short" leaves a last input
+# panel shorter than one SHRKSC box.
+CONV_CASES = [
+    (20001, 0.01, 620., 780., 0.05, 2, 0.2, None),
+    (20001, 0.01, 620., 780., 0.05, 3, 0.2, None),
+    (20001, 0.01, 620., 780., 0.05, 4, 0.2, None),
+    (20001, 0.01, 620., 780., 0.05, 5, 0.2, None),          # input ends before V2 + bound
+    (20001, 0.01, 620., 780., 0.0, 5, 0.6, None),           # default sampling
+    (20001, 0.01, 620., 780., 0.1, 3, 1.0, None),           # SHRKSC ratio 8
+    (20001, 0.01, 650., 750., 0.0, -5, 0.6, None),          # first zero crossing given
+    (20001, 0.01, 650., 750., 0.0, -4, 0.6, None),
+    (20001, 0.01, 550., 850., 0.05, 3, 0.25, None),         # output range wider than the input
+    (5001, 0.2, 650., 1500., 0.3, 5, 1.2, None),            # coarse input interpolated onto HWHM/12
+    (30001, 0.01, 620., 900., 0.05, 2, 0.2, "zeros"),
+    (72003, 0.001, 600.5, 675.003, 0.025, 2, 0.1, "short"),
+    (72008, 0.001, 600.5, 675.008, 0.025, 5, 0.1, None),
+    (9001, 0.002, 601., 615., 0.0, 3, 0.02, None),          # no compression (IRATSH 1)
+]
+
+
+def _conv_input(n, dv, tail, seed=5):
+    rng = np.random.default_rng(seed)
+    x = 1.0 + 0.5 * np.sin(np.arange(n) * dv / 0.7) + 0.1 * rng.normal(size=n)
+    if tail == "zeros":
+        x[-n // 3:] = 0.0
+        x[rng.uniform(size=n) < 0.2] = 0.0
+    return x
+
+
+@needs_ref
+@pytest.mark.parametrize("n,dv,v1o,v2o,dvo,fid,hw,tail", CONV_CASES)
+def test_conv_scan_matches_reference_bitwise(ref, n, dv, v1o, v2o, dvo, fid, hw, tail):
+    """clblm_SCANFN for the triangle, Gaussian, sinc**2 and sinc against the
+    reference's own routine: identical grid and count and bit-identical values,
+    through SHRKSC's boxes carried across input panels, CONVSC's sequential ZF and
+    R1 shifts, and the reference's count rules at the end of the input."""
+    from oracle.py import scanfn
+    x = _conv_input(n, dv, tail)
+    got, g1, gd = scanfn(x, 600.0, dv, v1o, v2o, dvo, fid, hw)
+    want, w1, wd = _ref_scanfn(ref, x, 600.0, dv, v1o, v2o, dvo, fid, hw)
+    assert (got.size, g1, gd) == (want.size, w1, wd)
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+
+
+@needs_ref
+def test_conv_scan_random_cases_match_reference(ref):
+    """Forty seeded random scans (function, half width, grids, output range, zero runs
+    and all-zero inputs): every one bit-identical to the reference."""
+    from oracle.py import scanfn
+    rng = np.random.default_rng(2024)
+    done = 0
+    while done < 40:
+        fid = int(rng.choice([2, 3, 4, 5, -4, -5]))
+        n = int(rng.integers(50, 30000))
+        dv = float(rng.choice([0.001, 0.002, 0.0037, 0.01, 0.05]))
+        hw = float(rng.uniform(0.02, 1.5))
+        v1 = 600.0 + float(rng.uniform(-1, 1))
+        span = dv * (n - 1)
+        a = v1 + float(rng.uniform(-0.2, 0.8)) * span
+        b = a + float(rng.uniform(0.05, 1.2)) * span
+        dvo = float(rng.choice([0.0, hw / rng.uniform(1.6, 5.0)]))
+        x = 1.0 + 0.5 * np.sin(np.arange(n) * dv / 0.7) + 0.1 * rng.normal(size=n)
+        kind = int(rng.integers(0, 4))
+        if kind == 1:
+            x[-int(rng.integers(1, n // 2 + 1)):] = 0.0
+        elif kind == 2:
+            x[rng.uniform(size=n) < 0.3] = 0.0
+        elif kind == 3 and rng.uniform() < 0.3:
+            x[:] = 0.0
+        try:
+            got, g1, gd = scanfn(x, v1, dv, a, b, dvo, fid, hw)
+        except RuntimeError:
+            continue    # outside R1 or the table: the reference overruns its arrays there
+        want, w1, wd = _ref_scanfn(ref, x, v1, dv, a, b, dvo, fid, hw)
+        assert (got.size, g1, gd) == (want.size, w1, wd), (fid, n, dv, hw, a, b, dvo)
+        assert np.array_equal(got, want), (fid, n, dv, hw, a, b, dvo)
+        done += 1
+
+
+@pytest.mark.parametrize("jfn", [1, 2, 3, 4])
+def test_scan_tables_closed_forms(jfn):
+    """The tabulated functions every 0.02 half widths: the triangle 0.5 (1 - x/2) and
+    the Gaussian sqrt(ln2/pi) exp(-ln2 x^2) (both zero at their last point), the
+    sinc**2 and sinc XSCALE/pi (sin(xXSCALE)/(xXSCALE))**k; each is 1/2 of its peak at
+    one half width, has unit area, and is zero past NJ."""
+    from oracle.py import scan_shape
+    f = scan_shape(jfn)
+    nf = {1: 101, 2: 201, 3: 2710, 4: 5968}[jfn]
+    x = 0.02 * np.arange(nf)
+    xs = {3: 1.391557377, 4: 1.89549425}.get(jfn)
+    if jfn == 1:
+        want = 0.5 * (1 - 0.5 * x)
+    elif jfn == 2:
+        want = np.sqrt(np.log(2) / np.pi) * np.exp(-np.log(2) * x * x)
+        want[-1] = 0.0
+    else:
+        t = np.where(x == 0, 1.0, x * xs)
+        s = np.where(x == 0, 1.0, np.sin(t) / t)
+        want = xs / np.pi * (s * s if jfn == 3 else s)
+    np.testing.assert_allclose(f[:nf], want, rtol=1e-14, atol=1e-17)
+    assert np.all(f[nf:] == 0.0)
+    assert abs(f[50] / f[0] - 0.5) < 1e-6                # x = 1: half maximum
+    area = 0.02 * (2.0 * f.sum() - f[0])
+    assert abs(area - 1.0) < {1: 1e-12, 2: 1e-5, 3: 5e-3, 4: 5e-3}[jfn]   # the tails past HWJ are cut
+
+
+@pytest.mark.parametrize("fid,hw", [(2, 0.1), (3, 0.1), (3, 0.3)])
+def test_conv_scan_of_a_narrow_line(fid, hw):
+    """A Gaussian line of width sigma << HWHM and area A scans (triangle, Gaussian)
+    to the scanning function of the same area: the output integrates to A, peaks
+    at the line, and its half maximum lies at the line +- HWHM (sigma adds in
+    quadrature for the Gaussian)."""
+    from oracle.py import scanfn
+    dv, A, v0, sig = 0.001, 3.0, 650.0037, 0.002
+    n = 100001
+    v = 600.0 + dv * np.arange(n)
+    x = A * np.exp(-0.5 * ((v - v0) / sig) ** 2) / (sig * np.sqrt(2 * np.pi))
+    y, y1, dy = scanfn(x, 600.0, dv, 640.0, 660.0, hw / 20.0, fid, hw)
+    vy = y1 + dy * np.arange(y.size)
+    assert abs(y.sum() * dy - A) < 1e-3 * A
+    assert abs(vy[np.argmax(y)] - v0) <= dy
+    half = vy[y >= 0.5 * y.max()]
+    width = 0.5 * (half.max() - half.min() + dy)
+    expect = hw if fid == 2 else np.sqrt(hw ** 2 + 2 * np.log(2) * sig ** 2)
+    assert abs(width - expect) < 0.06 * hw
