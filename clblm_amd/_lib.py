@@ -18,10 +18,11 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_linelist_broadening", "clblm_rt_down", "clblm_tape3_read", "clblm_linelist_load_tape3",
            "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy",
            "clblm_rt_jac_noalloc", "clblm_calculate_dv", "clblm_set_dv", "clblm_interp_spectrum",
-           "clblm_interp_to_finest", "clblm_rt_up_grids"]
+           "clblm_interp_to_finest", "clblm_rt_up_grids", "clblm_mtckd_load", "clblm_mtckd_absrb"]
 GRID_EXACT, GRID_FIXED_RATIO, GRID_UNIFORM = -1, 0, 1
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_CAPACITY = 0, 1, 2, 3, 4
 OD_SPECIES_BROAD = 1
+OD_MTCKD = 2
 TUNE_F4_XCD_CHUNK = 1
 TUNE_F4_FAR_FIELD = 2
 TUNE_F4_STREAM = 3
@@ -115,6 +116,8 @@ def load():
     lib.clblm_scanfn.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_double] * 5 + \
         [ctypes.c_int, ctypes.c_double, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), P]
+    lib.clblm_mtckd_load.argtypes = [P, ctypes.c_int, P, P, P, P] + [ctypes.c_double] * 4
+    lib.clblm_mtckd_absrb.argtypes = [P, ctypes.c_int, P, P, P, P, P, ctypes.c_int64, P, P]
     lib.clblm_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_double)]
     lib.clblm_od_graph_create.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int32),

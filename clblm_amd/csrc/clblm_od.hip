@@ -2323,12 +2323,149 @@ struct LineListH {
     LinesDev dev{};
 };
 
+// ---- MT_CKD water-vapour continuum ------------------------------------------------
+// CONTNM's H2O part (clblm_ODLAY_Continuum.f90:356-404) with mt_ckd_h2o_absco
+// (mt_ckd_h2o_module.f90:27-103): per layer, the coefficient window around
+// [V1ABS, V2ABS] scaled to the layer (the self coefficients by (T0/T)**texp, both by
+// the radiation term unless JRAD is 0) is built on the host with the C library's
+// pow and exp, as the reference's build evaluates them; the 4-point interpolation
+// onto the 1 cm-1 ABSRB grid (myxint :231-271, bounds by pre_xint :273-292) and the
+// sum absrb + W_H2O sh2o xself + W_H2O fh2o xfrgn run on the device.
+struct MtckdTable {
+    std::vector<double> wvn, self, forr, texp;
+    double ref_t = 0., ref_p = 0., xself = 1., xfrgn = 1.;
+    int64_t gen = 0;             // bumped by every load
+};
+
+struct MtJob {
+    int64_t coef;     // its self window in the batch's coefficient array, foreign after it
+    int64_t abs_off;  // its ABSRB in the batch's array
+    int32_t ncoeff, ilo, ihi, pad;
+    double v1a, v2a, dva, vft, wk1, xself, xfrgn;
+};
+
+// OPDPTH's ABSRB grid (clblm_ODLAY.f90:880-887)
+static void absrb_grid(double V1, double V2, double *v1abs, int *npt) {
+    double a = (double)(int)V1;
+    if (V1 < 0.) a = a - 1.;
+    a = a - 3.;
+    const double b = (double)(int)(V2 + 3. + 0.5);
+    *v1abs = a;
+    *npt = (int)((b - a) / 1. + 1.5);
+}
+
+// One layer's MT_CKD job; returns 0 with *has = false where CONTNM adds nothing.
+static int mtckd_plan(const MtckdTable &T, const clblm_layer &lay, double V1, double V2, int jrad, MtJob &mj,
+                      std::vector<double> &coef, bool *has) {
+    *has = false;
+    if (T.wvn.size() < 4) return set_err(CLBLM_ERR_ARG, "MT_CKD continuum asked for but no table loaded");
+    double V1ABS;
+    int NPTABS;
+    absrb_grid(V1, V2, &V1ABS, &NPTABS);
+    if (NPTABS > 5050) return set_err(CLBLM_ERR_ARG, "MT_CKD continuum: ABSRB longer than 5050 points");
+    if (!((V2 > -20.0) && (V1 < 20000.) && ((T.xself > 0.) || (T.xfrgn > 0.)))) return 0;
+    const double V2ABS = V1ABS + (double)(NPTABS - 1);
+    const double *wvn = T.wvn.data();
+    const int n = (int)T.wvn.size();
+    const double h2o_fac = lay.w[1] / lay.wtot;
+    const bool radflag = jrad != 0;
+    const double dvc = wvn[1] - wvn[0];
+    int i = 1;
+    while (i <= n && wvn[i - 1] <= (V1ABS - 2 * dvc)) i++;
+    int i1 = i - 1;
+    if (i1 == 0) i1 = 1;
+    while (i <= n && wvn[i - 1] < (V2ABS + 2 * dvc)) i++;
+    if (i > n) return set_err(CLBLM_ERR_ARG, "MT_CKD continuum: the ABSRB grid reaches past the coefficient table");
+    const int i2 = i, ncoeff = i2 - i1 + 1;
+    const double xkt = lay.tave / RADCN2;
+    const double rho_rat = (lay.pave / T.ref_p) * (T.ref_t / lay.tave);
+    mj.coef = (int64_t)coef.size();
+    coef.resize(coef.size() + 2 * (size_t)ncoeff);
+    double *sh = coef.data() + mj.coef, *fh = sh + ncoeff;
+    for (int k = 0; k < ncoeff; k++) {
+        const int m = i1 - 1 + k;
+        double sv = T.self[m] * std::pow(T.ref_t / lay.tave, T.texp[m]);
+        sv = sv * h2o_fac * rho_rat;
+        double fv = T.forr[m];
+        fv = fv * (1 - h2o_fac) * rho_rat;
+        if (radflag) {   // myradfn (:196-226)
+            const double xvi = wvn[m];
+            double rad = xvi;
+            const double xviokt = xvi / xkt;
+            if (xviokt <= 0.01) {
+                rad = 0.5 * xviokt * xvi;
+            } else if (xviokt <= 10.) {
+                const double expvkt = std::exp(-xviokt);
+                rad = xvi * (1 - expvkt) / (1. + expvkt);
+            }
+            sv = sv * rad;
+            fv = fv * rad;
+        }
+        sh[k] = sv;
+        fh[k] = fv;
+    }
+    // pre_xint over the whole table, then myxint's own bounds
+    const int nptabs = (int)((V2ABS - V1ABS) / 1. + 1);
+    const int nbnd_v1c = (int)(2 + (wvn[0] - V1ABS) / 1. + 1.e-5);
+    const int ist = std::max(1, nbnd_v1c);
+    const int nbnd_v2c = (int)(1 + (wvn[n - 1] - V1ABS) / 1. + 1.e-5);
+    const int lst = std::min(nptabs, nbnd_v2c);
+    const double V1A = wvn[i1 - 1], V2A = wvn[i2 - 1];
+    int ILO = (int)((V1A + dvc - V1ABS) / 1. + 1. + 0.999);
+    ILO = std::max(ILO, ist);
+    int IHI = (int)((V2A - dvc - V1ABS) / 1. + 0.999);
+    IHI = std::min(IHI, lst);
+    mj.ncoeff = ncoeff;
+    mj.ilo = ILO;
+    mj.ihi = IHI;
+    mj.pad = 0;
+    mj.v1a = V1A;
+    mj.v2a = V2A;
+    mj.dva = dvc;
+    mj.vft = V1ABS;
+    mj.wk1 = lay.w[1];
+    mj.xself = T.xself;
+    mj.xfrgn = T.xfrgn;
+    *has = ILO <= IHI;
+    if (!*has) coef.resize((size_t)mj.coef);
+    return 0;
+}
+
+// myxint of both windows at ABSRB point I (1-based) of job blockIdx.y, then CONTNM's
+// absrb = absrb + cself + cfor.  Outside [ILO, IHI] the reference adds zeros.
+__global__ void k_mtckd(const MtJob *jobs, const double *coef, double *absrb) {
+    const MtJob J = jobs[blockIdx.y];
+    const int I = J.ilo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (I > J.ihi) return;
+    const double *A = coef + J.coef - 1, *F = A + J.ncoeff;   // 1-based windows
+    const double RECDVA = 1. / J.dva;
+    const double VI = J.vft + 1. * (double)(I - 1);
+    const int j = (int)((VI - J.v1a) * RECDVA + 1.001);
+    const double VJ = J.v1a + J.dva * (double)(j - 1);
+    const double P = RECDVA * (VI - VJ);
+    const double C = (3. - 2. * P) * P * P;
+    const double B = 0.5 * P * (1. - P);
+    const double B1 = B * (1. - P);
+    const double B2 = B * P;
+    const bool in3 = j + 2 <= J.ncoeff;   // one past the window only where B2 = 0
+    const double s = -A[j - 1] * B1 + A[j] * (1. - C + B2) + A[j + 1] * (C + B1) - (in3 ? A[j + 2] : 0.) * B2;
+    const double f = -F[j - 1] * B1 + F[j] * (1. - C + B2) + F[j + 1] * (C + B1) - (in3 ? F[j + 2] : 0.) * B2;
+    const double sh2o = 0. + s * 1.0, fh2o = 0. + f * 1.0;
+    double *out = absrb + J.abs_off + (I - 1);
+    const double cself = J.wk1 * sh2o * J.xself;
+    const double cfor = J.wk1 * fh2o * J.xfrgn;
+    *out = *out + cself + cfor;
+}
+
 struct Batch {
     std::vector<Job> jobs;
     std::vector<Panel> panels;
     std::vector<R4Blk> blks;
     std::vector<int2> tiles;
     std::vector<double> absrb;
+    std::vector<MtJob> mtjobs;   // MT_CKD water-vapour continuum added to absrb on the device
+    std::vector<double> mtcoef;
+    int64_t mt_maxpts = 0;
     std::vector<int> stop0;
     std::vector<int> regrid_lay;
     std::vector<int64_t> regrid_first;
@@ -2405,6 +2542,9 @@ struct clblm_ctx {
     double ms[4] = {0, 0, 0, 0};
     int64_t launches = 0;
     void *scan = nullptr;                   // instrument-scan state (FFT plans, buffers)
+    MtckdTable mtckd;                       // MT_CKD water-vapour continuum coefficients
+    void *mt_buf = nullptr;                 // clblm_mtckd_absrb's jobs and windows
+    size_t mt_cap = 0;
     std::mutex mu;
 };
 
@@ -2542,10 +2682,13 @@ struct Plan {
     std::vector<Panel> panels;
     std::vector<R4Blk> blks;
     std::vector<double> absrb;
+    bool mt = false;             // MT_CKD water-vapour continuum: its job and windows
+    MtJob mj{};
+    std::vector<double> mtc;
 };
 
 static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od_params &par,
-                      int32_t *jrad_state, Plan &pl) {
+                      int32_t *jrad_state, Plan &pl, const MtckdTable *mt) {
     Job &J = pl.job;
     std::memset(&J, 0, sizeof J);
     if (!(par.v2 > par.v1)) return set_err(CLBLM_ERR_ARG, "v2 must exceed v1");
@@ -2700,27 +2843,36 @@ static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od
         if (V1 + (double)(J.npts - 1) * par.dvout > V1 + (double)(J.nnat - 1) * DV + DV)
             return set_err(CLBLM_ERR_ARG, "DVout grid extends beyond the native grid");
     }
-    // continuum on the OPDPTH ABSRB grid (clblm_ODLAY.f90:881-896)
-    J.contnm = par.contnm ? 1 : 0;
+    // continuum on the OPDPTH ABSRB grid (clblm_ODLAY.f90:881-896): the caller's
+    // array, the MT_CKD water-vapour continuum, or both (the caller's first)
+    const bool mtckd = (par.flags & CLBLM_OD_MTCKD) != 0;
+    J.contnm = (par.contnm || mtckd) ? 1 : 0;
     J.species_broad = (par.flags & CLBLM_OD_SPECIES_BROAD) ? 1 : 0;
     if (J.contnm) {
-        double v1abs = (double)(int)V1;
-        if (V1 < 0.) v1abs = v1abs - 1.;
-        v1abs = v1abs - 3.;
-        double v2abs = (double)(int)(V2 + 3. + 0.5);
-        int npt = (int)((v2abs - v1abs) / 1. + 1.5);
-        if (par.nptabs != npt || !par.continuum)
-            return set_err(CLBLM_ERR_ARG, "continuum needs " + std::to_string(npt) + " points from " +
-                                              std::to_string(v1abs) + " cm-1 at 1 cm-1");
+        double v1abs;
+        int npt;
+        absrb_grid(V1, V2, &v1abs, &npt);
         J.v1abs = v1abs;
         J.nptabs = npt;
-        pl.absrb.assign(par.continuum, par.continuum + npt);
+        if (par.contnm) {
+            if (par.nptabs != npt || !par.continuum)
+                return set_err(CLBLM_ERR_ARG, "continuum needs " + std::to_string(npt) + " points from " +
+                                                  std::to_string(v1abs) + " cm-1 at 1 cm-1");
+            pl.absrb.assign(par.continuum, par.continuum + npt);
+        } else {
+            pl.absrb.assign((size_t)npt, 0.);
+        }
+        if (mtckd) {
+            if (!mt) return set_err(CLBLM_ERR_ARG, "MT_CKD continuum asked for but no table loaded");
+            int rc = mtckd_plan(*mt, lay, V1, V2, J.jrad, pl.mj, pl.mtc, &pl.mt);
+            if (rc) return rc;
+        }
     }
     return 0;
 }
 
 static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, const clblm_od_params *pars,
-                       int32_t *jrad_state, Batch &B, int precision, int f4k) {
+                       int32_t *jrad_state, Batch &B, int precision, int f4k, const MtckdTable *mt) {
     B = Batch{};
     B.f4k = f4k;
     B.jobs.reserve(nlay);
@@ -2728,7 +2880,7 @@ static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, 
         if (pars[l].precision != precision)
             return set_err(CLBLM_ERR_ARG, "all layers of a batch must use the same precision");
         Plan pl;
-        int rc = plan_layer(L, layers[l], pars[l], jrad_state, pl);
+        int rc = plan_layer(L, layers[l], pars[l], jrad_state, pl, mt);
         if (rc) return rc;
         Job J = pl.job;
         J.panel_off = (int64_t)B.panels.size();
@@ -2743,6 +2895,14 @@ static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, 
         B.tot_rec += J.nrec;
         J.absrb_off = (int64_t)B.absrb.size();
         B.absrb.insert(B.absrb.end(), pl.absrb.begin(), pl.absrb.end());
+        if (pl.mt) {
+            MtJob mj = pl.mj;
+            mj.coef = (int64_t)B.mtcoef.size();
+            mj.abs_off = J.absrb_off;
+            B.mtcoef.insert(B.mtcoef.end(), pl.mtc.begin(), pl.mtc.end());
+            B.mtjobs.push_back(mj);
+            B.mt_maxpts = std::max<int64_t>(B.mt_maxpts, mj.ihi - mj.ilo + 1);
+        }
         J.nat_off = B.tot_nat;
         if (J.dvout > 0.) {
             B.tot_nat += J.nnat;
@@ -2821,6 +2981,8 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_blk = cv.take<R4Blk>(B.blks.size());
     size_t o_tiles = cv.take<int2>(B.tiles.size());
     size_t o_abs = cv.take<double>(std::max<size_t>(1, B.absrb.size()));
+    size_t o_mtj = cv.take<MtJob>(std::max<size_t>(1, B.mtjobs.size()));
+    size_t o_mtc = cv.take<double>(std::max<size_t>(1, B.mtcoef.size()));
     size_t o_lp_d = cv.take<double>(5 * (size_t)B.tot_line);
     size_t o_lp_i = cv.take<int>(3 * (size_t)B.tot_line);
     size_t o_A = cv.take<double>((size_t)B.tot_a);
@@ -2923,6 +3085,10 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     std::memcpy(hp + o_blk, B.blks.data(), sizeof(R4Blk) * B.blks.size());
     std::memcpy(hp + o_tiles, B.tiles.data(), sizeof(int2) * B.tiles.size());
     if (!B.absrb.empty()) std::memcpy(hp + o_abs, B.absrb.data(), sizeof(double) * B.absrb.size());
+    if (!B.mtjobs.empty()) {
+        std::memcpy(hp + o_mtj, B.mtjobs.data(), sizeof(MtJob) * B.mtjobs.size());
+        std::memcpy(hp + o_mtc, B.mtcoef.data(), sizeof(double) * B.mtcoef.size());
+    }
     std::memcpy(hp + h_stop, B.stop0.data(), sizeof(int) * nlay);
     if (!rg_lay.empty()) std::memcpy(hp + h_rg, rg_lay.data(), sizeof(int) * rg_lay.size());
     std::memcpy(hp + h_rgf, rg_first.data(), sizeof(int64_t) * nlay);
@@ -2933,6 +3099,9 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     double *d_abs = at<double>(base, o_abs);
     HIPCHK(hipMemcpyAsync(base, hp, o_lp_d, hipMemcpyHostToDevice, st));
     (void)d_jobs; (void)d_pan; (void)d_blk; (void)d_tiles; (void)d_abs;
+    if (!B.mtjobs.empty())   // the water-vapour continuum joins ABSRB before pass 2 reads it
+        hipLaunchKernelGGL(k_mtckd, dim3(cdiv(B.mt_maxpts, 256), (unsigned)B.mtjobs.size()), dim3(256), 0, st,
+                           at<MtJob>(base, o_mtj), at<double>(base, o_mtc), d_abs);
     LinePrep lp;
     double *lpd = at<double>(base, o_lp_d);
     int *lpi = at<int>(base, o_lp_i);
@@ -3196,6 +3365,7 @@ int clblm_ctx_destroy(clblm_ctx *c) {
             delete l;
         }
     if (c->arena.p) (void)hipFree(c->arena.p);
+    (void)hipFree(c->mt_buf);
     (void)hipFree(c->tab);
     (void)hipFree(c->tabs);
     (void)hipFree(c->tabg);
@@ -3364,6 +3534,7 @@ int clblm_od_batch(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
     put(jrad_state, sizeof *jrad_state);
     put(layers, sizeof(clblm_layer) * nlay);
     put(pars, sizeof(clblm_od_params) * nlay);
+    put(&c->mtckd.gen, sizeof c->mtckd.gen);
     for (int l = 0; l < nlay; l++)
         if (pars[l].contnm && pars[l].continuum && pars[l].nptabs > 0)
             put(pars[l].continuum, sizeof(double) * (size_t)pars[l].nptabs);
@@ -3376,7 +3547,7 @@ int clblm_od_batch(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
     } else {
         c->plan_valid = false;
         int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec,
-                             prec == CLBLM_FP64 ? c->f4k64 : c->f4k32);
+                             prec == CLBLM_FP64 ? c->f4k64 : c->f4k32, &c->mtckd);
         if (rc) return rc;
     }
     for (int l = 0; l < nlay; l++) offsets[l] = B.jobs[l].out_off;
@@ -3428,7 +3599,8 @@ int clblm_od_graph_create(clblm_ctx *c, int handle, int nlay, const clblm_layer 
     std::lock_guard<std::mutex> lk(c->mu);
     int32_t jr = *jrad_state;
     Batch B;
-    int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec, prec == CLBLM_FP64 ? c->f4k64 : c->f4k32);
+    int rc = build_batch(*c->lists[handle], nlay, layers, pars, &jr, B, prec, prec == CLBLM_FP64 ? c->f4k64 : c->f4k32,
+                         &c->mtckd);
     if (rc) return rc;
     clblm_od_graph *g = new clblm_od_graph;
     auto fail = [&](int r) {
@@ -3653,6 +3825,81 @@ int clblm_odlay(clblm_ctx *c, int handle, const clblm_layer *layer, const clblm_
     }
     (void)hipFree(d);
     return rc;
+}
+
+int clblm_mtckd_load(clblm_ctx *c, int n, const double *wavenumber, const double *self_absco_ref,
+                     const double *for_absco_ref, const double *self_texp, double ref_temp, double ref_press,
+                     double xself, double xfrgn) {
+    if (!c || !wavenumber || !self_absco_ref || !for_absco_ref || !self_texp || n < 4)
+        return set_err(CLBLM_ERR_ARG, "null argument or fewer than 4 coefficients");
+    if (!(ref_temp > 0.) || !(ref_press > 0.)) return set_err(CLBLM_ERR_ARG, "reference T and p must be positive");
+    for (int i = 1; i < n; i++)
+        if (!(wavenumber[i] > wavenumber[i - 1])) return set_err(CLBLM_ERR_ARG, "wavenumbers must increase");
+    std::lock_guard<std::mutex> g(c->mu);
+    MtckdTable &T = c->mtckd;
+    T.wvn.assign(wavenumber, wavenumber + n);
+    T.self.assign(self_absco_ref, self_absco_ref + n);
+    T.forr.assign(for_absco_ref, for_absco_ref + n);
+    T.texp.assign(self_texp, self_texp + n);
+    T.ref_t = ref_temp;
+    T.ref_p = ref_press;
+    T.xself = xself;
+    T.xfrgn = xfrgn;
+    T.gen++;
+    c->plan_valid = false;
+    return 0;
+}
+
+int clblm_mtckd_absrb(clblm_ctx *c, int nlay, const clblm_layer *layers, const double *v1, const double *v2,
+                      const int32_t *jrad, double *d_absrb, int64_t ld, int32_t *nptabs, void *stream) {
+    if (!c || !layers || !v1 || !v2 || !jrad || !nptabs || nlay <= 0 || nlay > 65535)
+        return set_err(CLBLM_ERR_ARG, "null argument or nlay outside 1..65535");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    std::vector<MtJob> jobs;
+    std::vector<double> coef;
+    int64_t maxpts = 1;
+    for (int l = 0; l < nlay; l++) {
+        if (!(layers[l].tave > 0.) || !(layers[l].wtot > 0.))
+            return set_err(CLBLM_ERR_ARG, "tave and wtot must be positive");
+        double v1abs;
+        absrb_grid(v1[l], v2[l], &v1abs, &nptabs[l]);
+        if (d_absrb && ld < nptabs[l]) return set_err(CLBLM_ERR_ARG, "ld is below a layer's ABSRB length");
+        MtJob mj{};
+        bool has = false;
+        int rc = mtckd_plan(c->mtckd, layers[l], v1[l], v2[l], jrad[l], mj, coef, &has);
+        if (rc) return rc;
+        if (has) {
+            mj.abs_off = (int64_t)l * ld;
+            jobs.push_back(mj);
+            maxpts = std::max<int64_t>(maxpts, mj.ihi - mj.ilo + 1);
+        }
+    }
+    if (!d_absrb) return 0;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    const size_t oj = 0, oc = (sizeof(MtJob) * std::max<size_t>(1, jobs.size()) + 255) & ~(size_t)255;
+    const size_t bytes = oc + sizeof(double) * std::max<size_t>(1, coef.size());
+    HIPCHK(hipStreamSynchronize(st));   // the buffer may still be read by a previous call
+    if (bytes > c->mt_cap) {
+        (void)hipFree(c->mt_buf);
+        c->mt_buf = nullptr;
+        c->mt_cap = 0;
+        HIPCHK(hipMalloc(&c->mt_buf, bytes));
+        c->mt_cap = bytes;
+    }
+    char *b = (char *)c->mt_buf;
+    if (!jobs.empty()) {
+        HIPCHK(hipMemcpyAsync(b + oj, jobs.data(), sizeof(MtJob) * jobs.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(b + oc, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
+    }
+    const size_t nb = sizeof(double) * (size_t)ld * (size_t)nlay;
+    hipLaunchKernelGGL(k_fill_bytes, dim3(cdiv((int64_t)nb, 256)), dim3(256), 0, st, (uint8_t *)d_absrb, nb, (uint8_t)0);
+    if (!jobs.empty())
+        hipLaunchKernelGGL(k_mtckd, dim3(cdiv(maxpts, 256), (unsigned)jobs.size()), dim3(256), 0, st,
+                           (const MtJob *)(b + oj), (const double *)(b + oc), d_absrb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));   // the host job list goes out of scope
+    return 0;
 }
 
 int clblm_scanfn(clblm_ctx *c, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1, double dv,

@@ -98,15 +98,45 @@ class Context:
                                                  ctypes.byref(h), ctypes.byref(n)))
         return h.value
 
+    # -- MT_CKD water-vapour continuum ---------------------------------------------
+    def mtckd_load(self, table, xself=1.0, xfrgn=1.0):
+        """clblm_mtckd_load from a table of clblm_amd.continuum.load_mtckd."""
+        arr = [np.ascontiguousarray(table[k], np.float64) for k in
+               ("wavenumbers", "self_absco_ref", "for_absco_ref", "self_texp")]
+        check(self.lib.clblm_mtckd_load(self.ptr, int(arr[0].size), *[a.ctypes.data for a in arr],
+                                        float(table["ref_temp"]), float(table["ref_press"]), float(xself),
+                                        float(xfrgn)))
+
+    def mtckd_absrb(self, layers, v1, v2, jrad, stream=None):
+        """clblm_mtckd_absrb: the water-vapour continuum of each layer on its ABSRB grid;
+        returns a list of float64 cuda tensors."""
+        import torch
+        n = len(layers)
+        ls = (_lib.Layer * n)(*[_layer_struct(lay) for lay in layers])
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v1, np.float64), (n,)))
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(v2, np.float64), (n,)))
+        j = np.ascontiguousarray(np.broadcast_to(np.asarray(jrad, np.int32), (n,)))
+        npt = np.zeros(n, np.int32)
+        check(self.lib.clblm_mtckd_absrb(self.ptr, n, ls, a.ctypes.data, b.ctypes.data, j.ctypes.data, None, 0,
+                                         npt.ctypes.data, None))
+        ld = int(npt.max())
+        out = torch.empty((n, ld), dtype=torch.float64, device="cuda")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        check(self.lib.clblm_mtckd_absrb(self.ptr, n, ls, a.ctypes.data, b.ctypes.data, j.ctypes.data,
+                                         ctypes.c_void_p(out.data_ptr()), ld, npt.ctypes.data,
+                                         ctypes.c_void_p(stream)))
+        return [out[l, :npt[l]] for l in range(n)]
+
     def free_lines(self, handle):
         check(self.lib.clblm_linelist_free(self.ptr, int(handle)))
 
     # -- one layer (host buffers) ----------------------------------------------
     @staticmethod
     def params(v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, precision=FP64, continuum=None,
-               species_broad=False):
+               species_broad=False, mtckd=False):
         p = ODParams()
-        p.flags = _lib.OD_SPECIES_BROAD if species_broad else 0
+        p.flags = (_lib.OD_SPECIES_BROAD if species_broad else 0) | (_lib.OD_MTCKD if mtckd else 0)
         p.v1, p.v2, p.dvnormal = float(v1), float(v2), float(dvnormal)
         p.dvnarrow = float(dvnormal if dvnarrow is None else dvnarrow)
         p.sample, p.dvout, p.precision = float(sample), float(dvout), int(precision)
@@ -117,9 +147,9 @@ class Context:
         return p
 
     def odlay(self, handle, layer, v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0,
-              precision=FP64, continuum=None):
+              precision=FP64, continuum=None, mtckd=False):
         lay = _layer_struct(layer)
-        par = self.params(v1, v2, dvnormal, dvnarrow, sample, dvout, precision, continuum)
+        par = self.params(v1, v2, dvnormal, dvnarrow, sample, dvout, precision, continuum, mtckd=mtckd)
         jr = ctypes.c_int32(self.jrad)
         n = ctypes.c_int64()
         dvu = ctypes.c_double()

@@ -59,6 +59,12 @@ typedef struct {
  * clblm_ODLAY.f90:1853-1883 and LINF4Q clblm_ODLAY_LineF4.f90:996-1054) from the data
  * attached by clblm_linelist_broadening. */
 #define CLBLM_OD_SPECIES_BROAD 1
+/* The MT_CKD water-vapour continuum (self and foreign, CONTNM's H2O part,
+ * clblm_ODLAY_Continuum.f90:356-404 with mt_ckd_h2o_absco, mt_ckd_h2o_module.f90:27-103)
+ * from the table given to clblm_mtckd_load, added to ABSRB (after the caller's
+ * continuum[] when contnm is also set) and carried into the OD as the reference's
+ * CONTNM output is (XINT into R3). */
+#define CLBLM_OD_MTCKD 2
 
 /* ---- context ---------------------------------------------------------------- */
 int clblm_ctx_create(int device, clblm_ctx **out);
@@ -375,6 +381,26 @@ int clblm_level_jac(clblm_ctx *ctx, int nlay, int nmol, int64_t npts, const doub
 int clblm_scanfn(clblm_ctx *ctx, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1,
                  double dv, double v1out, double v2out, double dvout, int funct_id, double hwhm, double *d_out,
                  int64_t ld_out, int64_t *nout, double *v1res, double *dvres, void *stream);
+
+/* ---- continuum ------------------------------------------------------------------
+ * The MT_CKD water-vapour continuum coefficients mt_ckd_h2o_absco reads from
+ * absco-ref_wv-mt-ckd.nc (mt_ckd_h2o_module.f90:34-41, read_module.f90:50-101): n
+ * wavenumbers (increasing, uniform), the self and foreign coefficients at ref_temp K and
+ * ref_press hPa, the self temperature exponent; xself and xfrgn are CONTNM's XSELF and
+ * XFRGN scale factors (1 by default).  Host arrays, copied.  Layers whose params set
+ * CLBLM_OD_MTCKD then carry the continuum. */
+int clblm_mtckd_load(clblm_ctx *ctx, int n, const double *wavenumber, const double *self_absco_ref,
+                     const double *for_absco_ref, const double *self_texp, double ref_temp, double ref_press,
+                     double xself, double xfrgn);
+
+/* The same continuum alone, as CONTNM adds it to a zeroed ABSRB (clblm_ODLAY.f90:880-896):
+ * for layer l, on V1ABS = INT(v1[l]) - 3 at 1 cm-1 (nptabs[l] points, returned), with the
+ * radiation term unless jrad[l] is 0 (jrad: OPDPTH's JRAD after its flip,
+ * clblm_ODLAY.f90:877-878).  d_absrb (device) row l at d_absrb + l ld, zero past the
+ * continuum's points; d_absrb == NULL returns nptabs only.  Enqueued on `stream`, waited
+ * for before returning. */
+int clblm_mtckd_absrb(clblm_ctx *ctx, int nlay, const clblm_layer *layers, const double *v1, const double *v2,
+                      const int32_t *jrad, double *d_absrb, int64_t ld, int32_t *nptabs, void *stream);
 
 /* ---- instrument scan ------------------------------------------------------------
  * Drop-in for clblm_FFTSCN_array (clblm_FFT.f90:308-349): the spectrum d_in[0..n)

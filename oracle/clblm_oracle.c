@@ -2610,3 +2610,108 @@ int64_t orc_scanfn_conv(const double *x, int64_t n, double v1, double dv, double
     free(R1); free(S); free(SS); free(spec);
     return nout;
 }
+
+/* ---------------------------------------------------------------------------
+ * MT_CKD water-vapour continuum: the H2O part of CONTNM
+ * (clblm_ODLAY_Continuum.f90:356-404) with mt_ckd_h2o_absco, myradfn, myxint and
+ * pre_xint (mt_ckd_h2o_module.f90:27-294), on OPDPTH's ABSRB grid
+ * (clblm_ODLAY.f90:880-887: V1ABS = INT(V1) - 3, 1 cm-1, to INT(V2 + 3.5)).  The
+ * coefficient table (wavenumber, self and foreign coefficients at ref_temp /
+ * ref_press, self temperature exponent) is the caller's, read from the reference's
+ * absco-ref_wv-mt-ckd.nc.  jrad is OPDPTH's JRAD after its flip; the radiation term
+ * is applied unless JRAD is 0 (:365-369).  absrb[0..nptabs) is ADDED to.
+ * Returns nptabs, -1 on error.
+ * ------------------------------------------------------------------------- */
+static void mtckd_xint(double V1A, double V2A, double DVA, const double *A /* 1-based, n */, int n, double AFACT,
+                       double VFT, double DVR3, double *R3 /* 1-based */, int N1R3, int N2R3) {
+    const double onepl = 1.001, onemi = 0.999;   /* the module's own (mt_ckd_h2o_module.f90:21) */
+    const double RECDVA = 1. / DVA;
+    int ILO = (int)((V1A + DVA - VFT) / DVR3 + 1. + onemi);
+    if (ILO < N1R3) ILO = N1R3;
+    int IHI = (int)((V2A - DVA - VFT) / DVR3 + onemi);
+    if (IHI > N2R3) IHI = N2R3;
+    for (int I = ILO; I <= IHI; I++) {
+        const double VI = VFT + DVR3 * (double)(I - 1);
+        const int J = (int)((VI - V1A) * RECDVA + onepl);
+        const double VJ = V1A + DVA * (double)(J - 1);
+        const double P = RECDVA * (VI - VJ);
+        const double C = (3. - 2. * P) * P * P;
+        const double B = 0.5 * P * (1. - P);
+        const double B1 = B * (1. - P);
+        const double B2 = B * P;
+        const double a3 = (J + 2 <= n) ? A[J + 2] : 0.;   /* past the window only where B2 = 0 */
+        const double CONTI = -A[J - 1] * B1 + A[J] * (1. - C + B2) + A[J + 1] * (C + B1) - a3 * B2;
+        R3[I] = R3[I] + CONTI * AFACT;
+    }
+}
+
+int orc_mtckd_h2o(const double *wvn, const double *self_ref, const double *for_ref, const double *texp, int ncoeffin,
+                  double ref_temp, double ref_press, double pave, double tave, double wk1, double wtot, double v1,
+                  double v2, int jrad, double xself, double xfrgn, double *absrb, int cap) {
+    /* OPDPTH's ABSRB grid */
+    double V1ABS = (double)(int)v1;
+    if (v1 < 0.) V1ABS = V1ABS - 1.;
+    V1ABS = V1ABS - 3.;
+    const double V2ABS = (double)(int)(v2 + 3. + 0.5);
+    const int NPTABS = (int)((V2ABS - V1ABS) / 1. + 1.5);
+    if (NPTABS > cap || NPTABS > 5050) return fail("mtckd: ABSRB longer than 5050 points");
+    if (!((v2 > -20.0) && (v1 < 20000.) && ((xself > 0.) || (xfrgn > 0.)))) return NPTABS;
+    const double h2o_fac = wk1 / wtot;
+    const int radflag = jrad != 0;
+    /* mt_ckd_h2o_absco */
+    const double dvabs = 1.;
+    const double dvc = wvn[1] - wvn[0];
+    int i = 1;                                    /* 1-based over wvn */
+    while (i <= ncoeffin && wvn[i - 1] <= (V1ABS - 2 * dvc)) i++;
+    int i1 = i - 1;
+    if (i1 == 0) i1 = 1;
+    while (i <= ncoeffin && wvn[i - 1] < (V2ABS + 2 * dvc)) i++;
+    if (i > ncoeffin) return fail("mtckd: the ABSRB grid reaches past the coefficient table");
+    const int i2 = i;
+    const int ncoeff = i2 - i1 + 1;
+    double *sh = (double *)calloc((size_t)ncoeff + 2, sizeof(double));
+    double *fh = (double *)calloc((size_t)ncoeff + 2, sizeof(double));
+    const double xkt = tave / RADCN2;
+    const double rho_rat = (pave / ref_press) * (ref_temp / tave);
+    for (int k = 1; k <= ncoeff; k++) {
+        const int m = i1 + k - 1;
+        double s = self_ref[m - 1] * pow(ref_temp / tave, texp[m - 1]);
+        s = s * h2o_fac * rho_rat;
+        double f = for_ref[m - 1];
+        f = f * (1 - h2o_fac) * rho_rat;
+        if (radflag) {   /* myradfn */
+            const double xvi = wvn[m - 1];
+            double rad = xvi;
+            const double xviokt = xvi / xkt;
+            if (xviokt <= 0.01) {
+                rad = 0.5 * xviokt * xvi;
+            } else if (xviokt <= 10.) {
+                const double expvkt = exp(-xviokt);
+                rad = xvi * (1 - expvkt) / (1. + expvkt);
+            }
+            s = s * rad;
+            f = f * rad;
+        }
+        sh[k] = s;
+        fh[k] = f;
+    }
+    const int nptabs = (int)((V2ABS - V1ABS) / dvabs + 1);
+    /* pre_xint over the whole table */
+    const double v1ss = wvn[0], v2ss = wvn[ncoeffin - 1];
+    const int nbnd_v1c = (int)(2 + (v1ss - V1ABS) / dvabs + 1.e-5);
+    const int ist = nbnd_v1c > 1 ? nbnd_v1c : 1;
+    const int nbnd_v2c = (int)(1 + (v2ss - V1ABS) / dvabs + 1.e-5);
+    const int lst = nptabs < nbnd_v2c ? nptabs : nbnd_v2c;
+    double *self_abs = (double *)calloc((size_t)NPTABS + 2, sizeof(double));
+    double *for_abs = (double *)calloc((size_t)NPTABS + 2, sizeof(double));
+    mtckd_xint(wvn[i1 - 1], wvn[i2 - 1], dvc, sh, ncoeff, 1.0, V1ABS, dvabs, self_abs, ist, lst);
+    mtckd_xint(wvn[i1 - 1], wvn[i2 - 1], dvc, fh, ncoeff, 1.0, V1ABS, dvabs, for_abs, ist, lst);
+    /* CONTNM: cself = wk(1) sh2o xself, cfor = wk(1) fh2o xfrgn, absrb = absrb + cself + cfor */
+    for (int k = 1; k <= NPTABS; k++) {
+        const double cself = wk1 * self_abs[k] * xself;
+        const double cfor = wk1 * for_abs[k] * xfrgn;
+        absrb[k - 1] = absrb[k - 1] + cself + cfor;
+    }
+    free(sh); free(fh); free(self_abs); free(for_abs);
+    return NPTABS;
+}

@@ -130,6 +130,9 @@ typedef struct {
 } orc_scan_plan;
 int64_t orc_scan_interp(const double *x, int64_t n, double v1, double dv, double vv1, double vv2, double dvo,
                         double *out, int64_t cap, double *v1res);
+int orc_mtckd_h2o(const double *wvn, const double *self_ref, const double *for_ref, const double *texp, int ncoeffin,
+                  double ref_temp, double ref_press, double pave, double tave, double wk1, double wtot, double v1,
+                  double v2, int jrad, double xself, double xfrgn, double *absrb, int cap);
 void orc_scan_shape(int jfn, double *xf);
 int64_t orc_scanfn_conv(const double *x, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
                         int funct_id, double hwhm, double *out, int64_t cap, double *v1res, double *dvres);

@@ -100,8 +100,10 @@ class Oracle:
         self.jrad = 1
 
     def odlay(self, layer, v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, continuum=None,
-              species_broad=False):
-        """Returns (od, dv_used).  JRAD persists across calls like the reference's SAVE."""
+              species_broad=False, mtckd=None):
+        """Returns (od, dv_used).  JRAD persists across calls like the reference's SAVE.
+        mtckd: a continuum table (clblm_amd.continuum.load_mtckd); its water-vapour
+        continuum is added to ABSRB after `continuum`, with OPDPTH's JRAD after the flip."""
         lay = _Layer()
         lay.pave, lay.tave, lay.wtot = layer.pave, layer.tave, layer.wtot
         for m, v in layer.w.items():
@@ -109,6 +111,12 @@ class Oracle:
             lay.sel[m] = 1
         absrb = None
         npt = 0
+        if mtckd is not None:
+            jr = self.jrad
+            if v1 / (layer.tave / 1.4387752) < 5. and jr != -1:   # clblm_ODLAY.f90:877-878
+                jr = 0
+            continuum = mtckd_h2o(mtckd, layer.pave, layer.tave, layer.w.get(1, 0.0), layer.wtot, v1, v2, jr,
+                                  absrb=continuum)
         if continuum is not None:
             absrb = np.ascontiguousarray(continuum, np.float64)
             npt = absrb.size
@@ -318,6 +326,26 @@ def scanfn(x, v1, dv, v1out, v2out, dvout, funct_id, hwhm):
     L.orc_scanfn_conv(x.ctypes.data, x.size, v1, dv, v1out, v2out, dvout, funct_id, hwhm, out.ctypes.data, n,
                       ctypes.byref(a), ctypes.byref(b))
     return out[:n], a.value, b.value
+
+
+def mtckd_h2o(table, pave, tave, wk1, wtot, v1, v2, jrad, xself=1.0, xfrgn=1.0, absrb=None):
+    """orc_mtckd_h2o: the MT_CKD water-vapour continuum of one layer on OPDPTH's ABSRB
+    grid (V1ABS = INT(V1) - 3 at 1 cm-1), added to absrb (zeros by default)."""
+    L = lib()
+    D, P = ctypes.c_double, ctypes.c_void_p
+    L.orc_mtckd_h2o.restype = ctypes.c_int
+    L.orc_mtckd_h2o.argtypes = [P, P, P, P, ctypes.c_int] + [D] * 8 + [ctypes.c_int, D, D, P, ctypes.c_int]
+    t = {k: np.ascontiguousarray(table[k], np.float64) for k in
+         ("wavenumbers", "self_absco_ref", "for_absco_ref", "self_texp")}
+    out = np.zeros(5050) if absrb is None else np.concatenate([np.asarray(absrb, np.float64),
+                                                               np.zeros(5050 - len(absrb))])
+    n = L.orc_mtckd_h2o(t["wavenumbers"].ctypes.data, t["self_absco_ref"].ctypes.data,
+                        t["for_absco_ref"].ctypes.data, t["self_texp"].ctypes.data, t["wavenumbers"].size,
+                        float(table["ref_temp"]), float(table["ref_press"]), pave, tave, wk1, wtot, v1, v2, int(jrad),
+                        xself, xfrgn, out.ctypes.data, out.size)
+    if n < 0:
+        raise RuntimeError(L.orc_error().decode())
+    return out[:n]
 
 
 def scan_shape(jfn):

@@ -1,0 +1,114 @@
+"""The MT_CKD water-vapour continuum on the GPU (clblm_mtckd_absrb and the
+CLBLM_OD_MTCKD path of the OD) against the oracle: the continuum itself bit for bit,
+the OD with the continuum within the FP64 tolerance of the other OD tests, and
+the OD the same continuum gives when the caller passes it in."""
+import os
+
+import numpy as np
+import pytest
+
+from clblm_amd import workloads as W
+from clblm_amd.continuum import load_mtckd
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def table():
+    return load_mtckd(os.path.join(HERE, "golden", "mtckd_h2o.npz"))
+
+
+@pytest.fixture(scope="module")
+def ctx(table):
+    from clblm_amd.odlay import Context
+    c = Context(0)
+    c.mtckd_load(table)
+    yield c
+    c.close()
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("jrad", [1, 0, -1])
+def test_absrb_bitwise(ctx, table, jrad):
+    from oracle.py import mtckd_h2o
+    layers = W.atmosphere()[::7]
+    bands = [(500.0, 2500.0), (10.3, 350.7), (2499.5, 3100.2)]
+    for v1, v2 in bands:
+        got = ctx.mtckd_absrb(layers, v1, v2, jrad)
+        for lay, g in zip(layers, got):
+            want = mtckd_h2o(table, lay.pave, lay.tave, lay.w.get(1, 0.0), lay.wtot, v1, v2, jrad)
+            g = g.cpu().numpy()
+            assert g.size == want.size
+            assert np.array_equal(g, want), (v1, v2, lay.pave, _rel(g, want))
+
+
+@pytest.mark.parametrize("k", [0, 20, 45])
+def test_od_with_mtckd_matches_oracle(ctx, table, k):
+    from oracle.py import Oracle, mtckd_h2o
+    from test_gpu_od_fp64 import _cmp
+    lines = W.profile_lines(20000, 500.0, 800.0, seed=1001)
+    lay = W.atmosphere()[k]
+    dv = W.calculate_dv(650.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+    h = ctx.upload_lines(lines)
+    ctx.jrad = 1
+    od, _ = ctx.odlay(h, lay, 600.0, 700.0, dv, mtckd=True)
+    orc = Oracle(lines)
+    ref, _ = orc.odlay(lay, 600.0, 700.0, dv, mtckd=table)
+    print("layer %d: max rel err %.3e" % (k, _cmp(od, ref)))
+    # the same continuum passed in by the caller gives the same OD (to the rounding of
+    # pass 1's atomic sums, whose order varies from run to run)
+    ctx.jrad = 1
+    jr = 0 if 600.0 / (lay.tave / 1.4387752) < 5.0 else 1
+    cont = mtckd_h2o(table, lay.pave, lay.tave, lay.w[1], lay.wtot, 600.0, 700.0, jr)
+    od2, _ = ctx.odlay(h, lay, 600.0, 700.0, dv, continuum=cont)
+    assert _rel(od, od2) <= 1e-13
+    ctx.free_lines(h)
+
+
+def test_batch_mixes_caller_continuum_and_mtckd(ctx, table):
+    """A batch where layers carry the MT_CKD continuum, a caller continuum, both (the
+    caller's first, then MT_CKD added to it) or neither, in FP64 and FP32C."""
+    import torch
+    from clblm_amd._lib import FP32C, FP64
+    from clblm_amd.odlay import Context
+    from oracle.py import Oracle
+    lines = W.profile_lines(20000, 500.0, 800.0, seed=1001)
+    layers = W.atmosphere()[::15]
+    h = ctx.upload_lines(lines)
+    modes = [(True, False), (False, True), (True, True), (False, False)]
+    for prec in (FP64, FP32C):
+        pars, refs = [], []
+        orc = Oracle(lines)
+        ctx.jrad = 1
+        for lay, (mt, cc) in zip(layers, modes):
+            dv = W.calculate_dv(650.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+            cont = W.continuum_absrb(lay, 600.0, 700.0)[0] if cc else None
+            pars.append(Context.params(600.0, 700.0, dv, precision=prec, continuum=cont, mtckd=mt))
+            refs.append(orc.odlay(lay, 600.0, 700.0, dv, continuum=cont, mtckd=table if mt else None)[0])
+        out, off = ctx.od_batch(h, layers, pars)
+        torch.cuda.synchronize()
+        out = out.cpu().numpy()
+        for l, ref in enumerate(refs):
+            od = out[off[l]:off[l + 1]]
+            err = _rel(od, ref)
+            print("prec %d layer %d modes %s: max err / max %.3e" % (prec, l, modes[l], err))
+            assert err <= (1e-10 if prec == FP64 else 1e-5)
+    ctx.free_lines(h)
+
+
+def test_mtckd_needs_a_table():
+    from clblm_amd._lib import ClblmError
+    from clblm_amd.odlay import Context
+    c = Context(0)
+    try:
+        lines = W.profile_lines(2000, 500.0, 800.0, seed=1)
+        h = c.upload_lines(lines)
+        lay = W.atmosphere()[3]
+        with pytest.raises(ClblmError):
+            c.odlay(h, lay, 600.0, 700.0, 0.002, mtckd=True)
+    finally:
+        c.close()
