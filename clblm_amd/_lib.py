@@ -18,7 +18,8 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_linelist_broadening", "clblm_rt_down", "clblm_tape3_read", "clblm_linelist_load_tape3",
            "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy",
            "clblm_rt_jac_noalloc", "clblm_calculate_dv", "clblm_set_dv", "clblm_interp_spectrum",
-           "clblm_interp_to_finest", "clblm_rt_up_grids", "clblm_mtckd_load", "clblm_mtckd_absrb"]
+           "clblm_interp_to_finest", "clblm_rt_up_grids", "clblm_mtckd_load", "clblm_mtckd_absrb",
+           "clblm_rt_mode1"]
 GRID_EXACT, GRID_FIXED_RATIO, GRID_UNIFORM = -1, 0, 1
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_CAPACITY = 0, 1, 2, 3, 4
 OD_SPECIES_BROAD = 1
@@ -37,6 +38,14 @@ class Layer(ctypes.Structure):
     _fields_ = [("pave", ctypes.c_double), ("tave", ctypes.c_double), ("wtot", ctypes.c_double),
                 ("zbot", ctypes.c_double), ("ztop", ctypes.c_double),
                 ("w", ctypes.c_double * (MAXMOL + 1)), ("sel", ctypes.c_int32 * (MAXMOL + 1))]
+
+
+class Mode1Params(ctypes.Structure):
+    """clblm_mode1_params"""
+    _fields_ = [("nlay", ctypes.c_int32), ("obs_level", ctypes.c_int32), ("lin_in_tau", ctypes.c_int32),
+                ("thermal", ctypes.c_int32), ("solar", ctypes.c_int32), ("v1", ctypes.c_double),
+                ("v2", ctypes.c_double), ("surf_emis", ctypes.c_double), ("t_skin", ctypes.c_double),
+                ("surf_refl", ctypes.c_double)]
 
 
 class ODParams(ctypes.Structure):
@@ -118,6 +127,8 @@ def load():
          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), P]
     lib.clblm_mtckd_load.argtypes = [P, ctypes.c_int, P, P, P, P] + [ctypes.c_double] * 4
     lib.clblm_mtckd_absrb.argtypes = [P, ctypes.c_int, P, P, P, P, P, ctypes.c_int64, P, P]
+    lib.clblm_rt_mode1.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, P, P, ctypes.POINTER(D),
+                                   ctypes.POINTER(I64), P]
     lib.clblm_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_double)]
     lib.clblm_od_graph_create.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int32),

@@ -3315,6 +3315,13 @@ extern "C" int clblm_fft_scan_impl(void *state, const double *d_in, int64_t n, d
                                    double pre_box_hw, int deconv_pre_box, int precision, double *d_out, int64_t *nout,
                                    double *v1res, double *dvres, void *stream);
 
+extern "C" int clblm_rt_mode1_impl(int nlay, double v1, double v2, const double *dv, const int64_t *nlim,
+                                   const int64_t *od_off, const double *d_od_up, const double *d_od_dn,
+                                   const double *d_od_sun, const double *fac_up, const double *fac_dn,
+                                   const double *fac_sun, const double *t_lay, const double *t_lev, int obs_level,
+                                   int lin, int thermal, int solar, double emis, double tskin, double refl,
+                                   const double *d_sol, int64_t sol_n, double sol_dv, double *scratch, int64_t max_n,
+                                   double *d_rad, double *d_tx, double *dv_out, int64_t *n_out, void *stream);
 extern "C" int clblm_scanfn_impl(void *state, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in,
                                  double v1, double dv, double v1out, double v2out, double dvout, int funct_id,
                                  double hwhm, double *d_out, int64_t ld_out, int64_t *nout, double *v1res,
@@ -4091,6 +4098,57 @@ int clblm_rt_up_grids(clblm_ctx *c, int nlay, double v1, const double *dv, const
     int rc = clblm_rt_up_grids_impl(nlay, v1, dv, nlim, od_off, d_od, t_lay, t_lev, surf_emis, t_skin, surf_dv,
                                     surf_nlim, lin_in_tau, sec_dn, c->grid_state, max_n, d_rad, st);
     if (rc) return set_err(rc, "radiance kernel launch failed");
+    HIPCHK(hipStreamSynchronize(st));   // the scratch arrays are reused by the next call
+    return 0;
+}
+
+int clblm_rt_mode1(clblm_ctx *c, const clblm_mode1_params *p, const double *dv, const int64_t *nlim,
+                   const int64_t *od_off, const double *d_od_up, const double *d_od_dn, const double *d_od_sun,
+                   const double *fac_up, const double *fac_dn, const double *fac_sun, const double *t_lay,
+                   const double *t_lev, const double *d_sol, int64_t sol_n, double sol_dv, double *d_rad,
+                   double *d_tx, double *dv_out, int64_t *nlim_out, void *stream) {
+    if (!c || !p || !dv || !nlim || !dv_out || !nlim_out || p->nlay <= 0) return set_err(CLBLM_ERR_ARG, "null argument");
+    const int nlay = p->nlay;
+    if (p->lin_in_tau < 0 || p->lin_in_tau > 2) return set_err(CLBLM_ERR_ARG, "lin_in_tau must be 0, 1 or 2");
+    if (!(p->v2 > p->v1)) return set_err(CLBLM_ERR_ARG, "v2 must exceed v1");
+    if (p->obs_level < 0 || p->obs_level > nlay) return set_err(CLBLM_ERR_ARG, "obs_level must lie in 0..nlay");
+    if (!p->thermal && p->surf_emis < 1.)
+        return set_err(CLBLM_ERR_ARG, "a reflecting surface needs the thermal down path");
+    if (!(p->surf_emis < 1.) && !(p->t_skin > 0.))
+        return set_err(CLBLM_ERR_ARG, "no surface term: emissivity 1 and no skin temperature");
+    if (p->solar && (!(sol_dv > 0.) || sol_n < 2)) return set_err(CLBLM_ERR_ARG, "solar spectrum needs DV > 0 and 2 points");
+    int64_t max_n = 2;
+    for (int l = 0; l < nlay; l++) {
+        if (!(dv[l] > 0.) || nlim[l] < 2) return set_err(CLBLM_ERR_ARG, "every layer needs DV > 0 and 2 points");
+        max_n = std::max(max_n, nlim[l]);
+    }
+    if (p->solar) max_n = std::max(max_n, sol_n);
+    std::vector<double> one((size_t)nlay, 1.0);
+    const double *fu = fac_up ? fac_up : one.data(), *fd = fac_dn ? fac_dn : one.data();
+    const double *fs = fac_sun ? fac_sun : one.data();
+    if (!d_rad)
+        return clblm_rt_mode1_impl(nlay, p->v1, p->v2, dv, nlim, od_off, d_od_up, d_od_dn, d_od_sun, fu, fd, fs,
+                                   t_lay, t_lev, p->obs_level, p->lin_in_tau, p->thermal, p->solar, p->surf_emis,
+                                   p->t_skin, p->surf_refl, d_sol, sol_n, sol_dv, nullptr, max_n, nullptr, nullptr,
+                                   dv_out, nlim_out, nullptr);
+    if (!od_off || !t_lay || !t_lev || !d_od_up || (p->thermal && !d_od_dn) || (p->solar && (!d_od_sun || !d_sol)))
+        return set_err(CLBLM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    const size_t bytes = sizeof(double) * 12 * (size_t)max_n;
+    if (c->grid_state_cap < bytes) {
+        if (c->grid_state) (void)hipFree(c->grid_state);
+        c->grid_state = nullptr;
+        c->grid_state_cap = 0;
+        HIPCHK(hipMalloc(&c->grid_state, bytes));
+        c->grid_state_cap = bytes;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = clblm_rt_mode1_impl(nlay, p->v1, p->v2, dv, nlim, od_off, d_od_up, d_od_dn, d_od_sun, fu, fd, fs, t_lay,
+                                 t_lev, p->obs_level, p->lin_in_tau, p->thermal, p->solar, p->surf_emis, p->t_skin,
+                                 p->surf_refl, d_sol, sol_n, sol_dv, c->grid_state, max_n, d_rad, d_tx, dv_out,
+                                 nlim_out, st);
+    if (rc) return set_err(rc, "merge-down radiance failed");
     HIPCHK(hipStreamSynchronize(st));   // the scratch arrays are reused by the next call
     return 0;
 }

@@ -727,3 +727,277 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_rt_up_grids_impl(
     return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
 }
 
+
+// ---------------------------------------------------------------------------------
+// Merge-down on the layers' own grids: mode1_lookDn_mergeDn / layerMerge_mode1
+// (clblm_noScattRT.f90:42-842) with addThermalBoundary and addSolarBoundary
+// (:1945-2148).  From the top layer down: the down radiance gains the layer's
+// downward emission under the merged down-path radiance (mrgRadDn = EmDn + mrgRadDn
+// TxB), the up radiance gains the layer's upward emission under the merged view
+// transmittance (mrgRadUp = mrgRadUp + mrgTxA EmUp), then mrgTxA and the solar
+// transmittance take the layer's factors.  Each step is preceded by the
+// reference's interpToFinestGrid over the spectra it combines; the grids involved
+// depend on the DVs alone, so the host walks them (also to size the output) and the
+// device does the arithmetic.
+
+// The layer's view and down transmittances, emissions and solar transmittance on its
+// grid (the up emission only for linInTau 1 and 2: for 0 it is the down one).
+__global__ void __launch_bounds__(256) k_m1_layer(const double *__restrict__ od_up, const double *__restrict__ od_dn,
+                                                  const double *__restrict__ od_sun, int64_t n, double v1, double dv,
+                                                  double fu, double fd, double fs, double tbot, double ttop,
+                                                  double tave, int lin, int below, int thermal, int solar,
+                                                  double *__restrict__ txa, double *__restrict__ txb,
+                                                  double *__restrict__ emdn, double *__restrict__ emup,
+                                                  double *__restrict__ sol) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = v1 + (double)i * dv;
+    double oda = 0., ta = 1.;
+    if (below) {
+        oda = od_up[i] * fu;
+        ta = exp(-oda);
+        txa[i] = ta;
+    }
+    if (thermal) {
+        const double odb = od_dn[i] * fd;
+        const double tb = exp(-odb);
+        txb[i] = tb;
+        const double bbar = planck_x(v, tave / RADCN2);
+        double h, dh;
+        if (lin == 0) {
+            emdn[i] = (1 - tb) * bbar;
+        } else if (lin == 1) {
+            rlin(odb, &h, &dh);
+            const double ba = planck_x(v, tbot / RADCN2);
+            emdn[i] = (1 - tb) * (ba + 2 * (bbar - ba) * h);
+            if (below) {
+                rlin(oda, &h, &dh);
+                const double bu = planck_x(v, ttop / RADCN2);
+                emup[i] = (1 - ta) * (bu + 2 * (bbar - bu) * h);
+            }
+        } else {
+            rlin(odb, &h, &dh);
+            const double ba = planck_x(v, tbot / RADCN2), bb = planck_x(v, ttop / RADCN2);
+            emdn[i] = (1 - tb) * (ba + (bb - ba) * h);
+            if (below) {
+                rlin(oda, &h, &dh);
+                emup[i] = (1 - ta) * (bb + (ba - bb) * h);
+            }
+        }
+    }
+    if (solar) sol[i] = exp(-(od_sun[i] * fs));
+}
+
+__global__ void __launch_bounds__(256) k_m1_fill(double *__restrict__ x, int64_t n, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// mrgRadDn = EmDn + mrgRadDn TxB
+__global__ void __launch_bounds__(256) k_m1_dn(const double *__restrict__ em, const double *__restrict__ tb,
+                                               double *__restrict__ r, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = em[i] + r[i] * tb[i];
+}
+
+// mrgRadUp = mrgRadUp + mrgTxA EmUp
+__global__ void __launch_bounds__(256) k_m1_up(const double *__restrict__ ta, const double *__restrict__ em,
+                                               double *__restrict__ r, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = r[i] + ta[i] * em[i];
+}
+
+// a = a b
+__global__ void __launch_bounds__(256) k_m1_mul(double *__restrict__ a, const double *__restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = a[i] * b[i];
+}
+
+// addThermalBoundary: emission plus the reflected down radiance (refl) or emission only.
+__global__ void __launch_bounds__(256) k_m1_surface(const double *__restrict__ up, const double *__restrict__ ta,
+                                                    const double *__restrict__ dn, int64_t n, double v1, double dv,
+                                                    double emis, double tskin, int refl, double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = v1 + (double)i * dv;
+    const double srad = planck_x(v, tskin / RADCN2) * emis;
+    if (refl) out[i] = up[i] + ta[i] * (srad + (1.0 - emis) * dn[i]);
+    else out[i] = up[i] + ta[i] * srad;
+}
+
+// addSolarBoundary: tot = tot + TxA refl Tsun S0
+__global__ void __launch_bounds__(256) k_m1_solar(double *__restrict__ tot, const double *__restrict__ ta,
+                                                  const double *__restrict__ ts, const double *__restrict__ s0,
+                                                  int64_t n, double refl) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tot[i] = tot[i] + ta[i] * refl * ts[i] * s0[i];
+}
+
+namespace {
+struct M1Spec {
+    double *d = nullptr;   // device values (nullptr in the sizing walk)
+    double dv = 0.;
+    int64_t n = 0;
+};
+
+// The walk: run == false follows the grids only.
+struct M1Walk {
+    double v1 = 0.;
+    bool run = false;
+    hipStream_t st = nullptr;
+    std::vector<double *> free_bufs;
+    double *spare = nullptr;   // handed out if the pool ran dry (then the call fails)
+    bool dry = false;
+    double *take() {
+        if (!run) return nullptr;
+        if (free_bufs.empty()) {
+            dry = true;
+            return spare;
+        }
+        double *p = free_bufs.back();
+        free_bufs.pop_back();
+        return p;
+    }
+    void give(double *p) {
+        if (run && p) free_bufs.push_back(p);
+    }
+    // interpToFinestGrid over s[0..ns)
+    void finest(int ns, M1Spec **s) {
+        int loc = 0;
+        for (int k = 1; k < ns; k++)
+            if (s[k]->dv < s[loc]->dv) loc = k;
+        const double dvl = s[loc]->dv;
+        const int64_t nl = s[loc]->n;
+        for (int k = 0; k < ns; k++) {
+            if (k == loc || clblm_interp::same_grid(v1, s[k]->dv, s[k]->n, dvl, nl)) continue;
+            double *o = take();
+            if (run) launch_interp(s[k]->d, s[k]->n, v1, s[k]->dv, dvl, o, nl, st);
+            give(s[k]->d);
+            s[k]->d = o;
+            s[k]->dv = dvl;
+            s[k]->n = nl;
+        }
+    }
+};
+}  // namespace
+
+// scratch: 12 * max_n doubles, max_n >= every layer's points, sol_n and 2.
+extern "C" __attribute__((visibility("hidden"))) int clblm_rt_mode1_impl(
+    int nlay, double v1, double v2, const double *dv, const int64_t *nlim, const int64_t *od_off,
+    const double *d_od_up, const double *d_od_dn, const double *d_od_sun, const double *fac_up, const double *fac_dn,
+    const double *fac_sun, const double *t_lay, const double *t_lev, int obs_level, int lin, int thermal, int solar,
+    double emis, double tskin, double refl, const double *d_sol, int64_t sol_n, double sol_dv, double *scratch,
+    int64_t max_n, double *d_rad, double *d_tx, double *dv_out, int64_t *n_out, void *stream) {
+    M1Walk W;
+    W.v1 = v1;
+    W.run = d_rad != nullptr;
+    W.st = (hipStream_t)stream;
+    if (W.run) {
+        for (int k = 11; k >= 0; k--) W.free_bufs.push_back(scratch + (size_t)k * max_n);
+        W.spare = scratch;
+    }
+    auto fill = [&](M1Spec &s, double v) {
+        s.d = W.take();
+        if (W.run) hipLaunchKernelGGL(k_m1_fill, dim3(blocks_of(s.n, 256)), dim3(256), 0, W.st, s.d, s.n, v);
+    };
+    M1Spec TxA{nullptr, v2 - v1, 2}, RadUp{nullptr, v2 - v1, 2}, RadDn{nullptr, v2 - v1, 2}, SolTx{nullptr, v2 - v1, 2};
+    fill(TxA, 1.);
+    fill(RadUp, 0.);
+    if (thermal) fill(RadDn, 0.);
+    if (solar) fill(SolTx, 1.);
+    for (int l = nlay - 1; l >= 0; l--) {
+        const int below = l < obs_level;
+        const double ldv = dv[l];
+        const int64_t ln = nlim[l];
+        M1Spec lTxA{nullptr, ldv, ln}, lTxB{nullptr, ldv, ln}, lEmDn{nullptr, ldv, ln}, lEmUp{nullptr, ldv, ln},
+            lSol{nullptr, ldv, ln};
+        if (below) lTxA.d = W.take();
+        if (thermal) {
+            lTxB.d = W.take();
+            lEmDn.d = W.take();
+            if (below && lin != 0) lEmUp.d = W.take();
+        }
+        if (solar) lSol.d = W.take();
+        if (W.run)
+            hipLaunchKernelGGL(k_m1_layer, dim3(blocks_of(ln, 256)), dim3(256), 0, W.st, d_od_up + od_off[l],
+                               d_od_dn + od_off[l], d_od_sun + od_off[l], ln, v1, ldv, fac_up[l], fac_dn[l], fac_sun[l],
+                               t_lev[l], t_lev[l + 1], t_lay[l], lin, below, thermal, solar, lTxA.d, lTxB.d, lEmDn.d,
+                               lEmUp.d, lSol.d);
+        if (thermal) {
+            M1Spec *a[3] = {&lEmDn, &lTxB, &RadDn};
+            W.finest(3, a);
+            if (W.run)
+                hipLaunchKernelGGL(k_m1_dn, dim3(blocks_of(RadDn.n, 256)), dim3(256), 0, W.st, lEmDn.d, lTxB.d,
+                                   RadDn.d, RadDn.n);
+            if (below) {
+                if (lin == 0) {   // lyrEmUp = lyrEmDn on its grid after the merge above
+                    lEmUp = M1Spec{W.take(), lEmDn.dv, lEmDn.n};
+                    if (W.run &&
+                        hipMemcpyAsync(lEmUp.d, lEmDn.d, sizeof(double) * (size_t)lEmDn.n, hipMemcpyDeviceToDevice,
+                                       W.st) != hipSuccess)
+                        return CLBLM_ERR_HIP;
+                }
+                M1Spec *b[3] = {&lEmUp, &TxA, &RadUp};
+                W.finest(3, b);
+                if (W.run)
+                    hipLaunchKernelGGL(k_m1_up, dim3(blocks_of(RadUp.n, 256)), dim3(256), 0, W.st, TxA.d, lEmUp.d,
+                                       RadUp.d, RadUp.n);
+            }
+        }
+        if (below) {
+            M1Spec *c[2] = {&lTxA, &TxA};
+            W.finest(2, c);
+            if (W.run)
+                hipLaunchKernelGGL(k_m1_mul, dim3(blocks_of(TxA.n, 256)), dim3(256), 0, W.st, TxA.d, lTxA.d, TxA.n);
+        }
+        if (solar) {
+            M1Spec *d[2] = {&lSol, &SolTx};
+            W.finest(2, d);
+            if (W.run)
+                hipLaunchKernelGGL(k_m1_mul, dim3(blocks_of(SolTx.n, 256)), dim3(256), 0, W.st, SolTx.d, lSol.d,
+                                   SolTx.n);
+        }
+        W.give(lTxA.d);
+        W.give(lTxB.d);
+        W.give(lEmDn.d);
+        W.give(lEmUp.d);
+        W.give(lSol.d);
+    }
+    // addThermalBoundary (flux off)
+    const bool reflecting = emis < 1.;
+    M1Spec Tot;
+    if (reflecting) {
+        M1Spec *a[3] = {&TxA, &RadUp, &RadDn};
+        W.finest(3, a);
+    } else {
+        M1Spec *a[2] = {&RadUp, &TxA};
+        W.finest(2, a);
+    }
+    Tot = M1Spec{W.take(), RadUp.dv, RadUp.n};
+    if (W.run)
+        hipLaunchKernelGGL(k_m1_surface, dim3(blocks_of(Tot.n, 256)), dim3(256), 0, W.st, RadUp.d, TxA.d, RadDn.d,
+                           Tot.n, v1, Tot.dv, emis, tskin, reflecting ? 1 : 0, Tot.d);
+    if (solar) {   // addSolarBoundary
+        M1Spec S0{nullptr, sol_dv, sol_n};
+        if (W.run) {
+            S0.d = W.take();
+            if (hipMemcpyAsync(S0.d, d_sol, sizeof(double) * (size_t)sol_n, hipMemcpyDeviceToDevice, W.st) !=
+                hipSuccess)
+                return CLBLM_ERR_HIP;
+        }
+        M1Spec *a[4] = {&TxA, &Tot, &SolTx, &S0};
+        W.finest(4, a);
+        if (W.run)
+            hipLaunchKernelGGL(k_m1_solar, dim3(blocks_of(Tot.n, 256)), dim3(256), 0, W.st, Tot.d, TxA.d, SolTx.d,
+                               S0.d, Tot.n, refl);
+    }
+    *dv_out = Tot.dv;
+    *n_out = Tot.n;
+    if (!W.run) return 0;
+    if (W.dry) return CLBLM_ERR_CAPACITY;   // more spectra alive than the scratch holds
+    if (hipMemcpyAsync(d_rad, Tot.d, sizeof(double) * (size_t)Tot.n, hipMemcpyDeviceToDevice, W.st) != hipSuccess)
+        return CLBLM_ERR_HIP;
+    if (d_tx && hipMemcpyAsync(d_tx, TxA.d, sizeof(double) * (size_t)TxA.n, hipMemcpyDeviceToDevice, W.st) != hipSuccess)
+        return CLBLM_ERR_HIP;
+    return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}

@@ -25,6 +25,7 @@ import dataclasses
 
 import numpy as np
 
+from . import _lib
 from ._lib import FP32C, check
 from .odlay import Context
 
@@ -184,6 +185,52 @@ def upwelling_radiance_grids(ctx, od, offsets, dvs, v1, t_lay, t_lev, emis=0.98,
                                     ctypes.byref(dvo), ctypes.byref(no),
                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
     return rad, dvo.value
+
+
+def radiance_mode1(ctx, od, offsets, dvs, v1, v2, t_lay, t_lev, od_dn=None, od_sun=None, fac_up=None, fac_dn=None,
+                   fac_sun=None, obs_level=None, lin_in_tau=1, thermal=True, solar=False, emis=0.98, tskin=295.0,
+                   refl=0.0, sol=None, sol_dv=None):
+    """Radiance at an observer above the layers, merged from the top down
+    (clblm_rt_mode1: mode1_lookDn_mergeDn / layerMerge_mode1 with the thermal and solar
+    boundary terms, src/clblm_src/clblm_noScattRT.f90:42-842, 1945-2148).  od: float64
+    cuda tensor of the view-path layer ODs back to back (layer l at offsets[l], on
+    v1 + i*dvs[l], layer 0 at the surface); od_dn / od_sun the down and solar path ODs
+    laid out alike (default: od), each scaled by its per-layer factors (default 1).
+    sol: cuda tensor of the solar radiance at the top on v1 + i*sol_dv.  Returns
+    (radiance, total view transmittance, DV)."""
+    import torch
+    off = np.ascontiguousarray(offsets, np.int64)
+    L = off.size - 1
+    nl = np.ascontiguousarray(np.diff(off), np.int64)
+    dv = np.ascontiguousarray(dvs, np.float64)
+    tl, tv = np.ascontiguousarray(t_lay, np.float64), np.ascontiguousarray(t_lev, np.float64)
+    if dv.size != L or tl.size != L or tv.size != L + 1:
+        raise ValueError("dvs and t_lay need one value per layer, t_lev one per level")
+    for t in (od, od_dn, od_sun):
+        if t is not None and (not t.is_cuda or t.dtype != torch.float64 or t.numel() < off[-1]):
+            raise ValueError("the ODs must be float64 cuda tensors holding every layer")
+    p = _lib.Mode1Params()
+    p.nlay, p.obs_level, p.lin_in_tau = L, L if obs_level is None else int(obs_level), int(lin_in_tau)
+    p.thermal, p.solar = int(bool(thermal)), int(bool(solar))
+    p.v1, p.v2, p.surf_emis, p.t_skin, p.surf_refl = float(v1), float(v2), float(emis), float(tskin), float(refl)
+    fac = [None if f is None else np.ascontiguousarray(f, np.float64) for f in (fac_up, fac_dn, fac_sun)]
+    fp = [None if f is None else f.ctypes.data for f in fac]
+    od = od.contiguous()
+    dn = od if od_dn is None else od_dn.contiguous()
+    sn = od if od_sun is None else od_sun.contiguous()
+    s0 = sol.contiguous() if sol is not None else None
+    sdv = float(dv[-1] if sol_dv is None else sol_dv)
+    sol_n = 0 if s0 is None else s0.numel()
+    dvo, no = ctypes.c_double(), ctypes.c_int64()
+    ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    head = [ctx.ptr, ctypes.byref(p), dv.ctypes.data, nl.ctypes.data, off.ctypes.data, ptr(od), ptr(dn), ptr(sn),
+            *fp, tl.ctypes.data, tv.ctypes.data, ptr(s0), sol_n, ctypes.c_double(sdv)]
+    check(ctx.lib.clblm_rt_mode1(*head, None, None, ctypes.byref(dvo), ctypes.byref(no), None))
+    rad = torch.empty(no.value, dtype=torch.float64, device=od.device)
+    tx = torch.empty(no.value, dtype=torch.float64, device=od.device)
+    check(ctx.lib.clblm_rt_mode1(*head, ptr(rad), ptr(tx), ctypes.byref(dvo), ctypes.byref(no),
+                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return rad, tx, dvo.value
 
 
 def upwelling_radiance(ctx, od, t_lay, t_lev, v1, dv, emis=0.98, tskin=295.0, lin_in_tau=1, sec_dn=1.0):

@@ -402,6 +402,38 @@ int clblm_mtckd_load(clblm_ctx *ctx, int n, const double *wavenumber, const doub
 int clblm_mtckd_absrb(clblm_ctx *ctx, int nlay, const clblm_layer *layers, const double *v1, const double *v2,
                       const int32_t *jrad, double *d_absrb, int64_t ld, int32_t *nptabs, void *stream);
 
+/* ---- merge-down radiance -----------------------------------------------------------
+ * mode1_lookDn_mergeDn with layerMerge_mode1 (src/clblm_src/clblm_noScattRT.f90:42-842),
+ * addThermalBoundary and addSolarBoundary (:1945-2148), flux off: the radiance at an
+ * observer above the surface, merged from the top layer down, every layer's ODs on its
+ * own grid (nlim[l] points on v1 + i dv[l], device arrays at od_off[l]; layer 0 at the
+ * surface).  The view, down and solar path ODs of layer l are d_od_up, d_od_dn and
+ * d_od_sun times fac_up[l], fac_dn[l], fac_sun[l] (the reference's airmass-scaled form
+ * scalUpFac, scalDownFac, scalSolarFac; NULL factors are 1, for ODs computed along each
+ * path).  Layers l < obs_level lie below the observer and enter the view path; the
+ * others only the down and solar paths.  Before each merge the reference's
+ * interpToFinestGrid (clblm_Spectrum.f90:733-795) brings the spectra it combines onto
+ * the finest of their grids; the merged spectra start on the two-point grid v1, v2.
+ * Surface: constant emissivity (reflecting below 1, which needs thermal), skin
+ * temperature, solar reflectance; d_sol: the solar radiance at the top (S0 umu0/pi) on
+ * v1 + i sol_dv, covering no more than the layers' grids do (past them the reference's
+ * interpolation reads outside its extended input; zeros here).  Outputs (device, on the final grid *dv_out, *nlim_out points): d_rad
+ * the radiance and, if non-NULL, d_tx the total view transmittance.  d_rad == NULL
+ * returns the final grid only.  Waits for the stream before returning. */
+typedef struct {
+    int32_t nlay;
+    int32_t obs_level;           /* levels from 0 (surface) to nlay (top) */
+    int32_t lin_in_tau;          /* 0, 1 or 2 (layerEmis) */
+    int32_t thermal, solar;      /* sources */
+    double v1, v2;
+    double surf_emis, t_skin, surf_refl;
+} clblm_mode1_params;
+int clblm_rt_mode1(clblm_ctx *ctx, const clblm_mode1_params *p, const double *dv, const int64_t *nlim,
+                   const int64_t *od_off, const double *d_od_up, const double *d_od_dn, const double *d_od_sun,
+                   const double *fac_up, const double *fac_dn, const double *fac_sun, const double *t_lay,
+                   const double *t_lev, const double *d_sol, int64_t sol_n, double sol_dv, double *d_rad,
+                   double *d_tx, double *dv_out, int64_t *nlim_out, void *stream);
+
 /* ---- instrument scan ------------------------------------------------------------
  * Drop-in for clblm_FFTSCN_array (clblm_FFT.f90:308-349): the spectrum d_in[0..n)
  * (device) on v1 + i*dv is scanned with function funct_id of half width hwhm between

@@ -389,8 +389,11 @@ void orc_interp(const double *spIn, double DVin, double V1in, int NLIMin,
     } else {
         lenIn = 0; lenOut = 0; ITYPE = -1;
     }
-    /* spInExt(1-4 : NLIMin+4) */
-    double *ext = (double *)calloc((size_t)NLIMin + 9, sizeof(double));
+    /* spInExt(1-4 : NLIMin+4); an output grid reaching further past the input reads
+     * beyond it in the reference (undefined there): zeros here, as on the GPU */
+    int64_t reach = (int64_t)((V2out - V1in) / DVin) + 8;
+    if (reach < NLIMin) reach = NLIMin;
+    double *ext = (double *)calloc((size_t)reach + 9, sizeof(double));
     double *X = ext + 4; /* X[i] == spInExt(i) for i in -3..NLIMin+4 */
     for (int i = 1; i <= NLIMin; i++) X[i] = spIn[i - 1];
     X[0] = 2.0 * X[1] - X[2];
@@ -2714,4 +2717,154 @@ int orc_mtckd_h2o(const double *wvn, const double *self_ref, const double *for_r
     }
     free(sh); free(fh); free(self_abs); free(for_abs);
     return NPTABS;
+}
+
+/* ---------------------------------------------------------------------------
+ * mode1_lookDn_mergeDn / layerMerge_mode1 (clblm_noScattRT.f90:42-528, 529-842) with
+ * addThermalBoundary and addSolarBoundary (:1945-2148): the radiance at an observer
+ * above the layers, merged from the top layer down, every layer on its own grid
+ * (od_* + off[l], nlim[l] points on v1 + i dv[l]).  Layer l is below the observer when
+ * l < obs_level (levels from 0 at the surface).  The view, down and solar path ODs of
+ * layer l are od_up, od_dn, od_sun times fac_up[l], fac_dn[l], fac_sun[l] (the
+ * airmass-scaled form; factor 1 for ODs computed along each path).  The merged
+ * transmittance, up and down radiance and solar transmittance start on the two-point
+ * grid V1, V2 (:217-245), and every interpToFinestGrid of the reference is applied in
+ * its order.  Surface: constant emissivity emis (reflecting when below 1), skin
+ * temperature tskin, solar reflectance refl; sol: the solar radiance at the top on
+ * v1 + i sol_dv (sol_n points).  rad and tx (capacity cap) receive the radiance and
+ * the total transmittance on the final grid; returns its length (*dv_out its DV), or -1.
+ * ------------------------------------------------------------------------- */
+typedef struct { double *x; double dv; int64_t n; } m1spec;
+
+static void m1_new(m1spec *s, double dv, int64_t n) {
+    s->x = (double *)calloc((size_t)n, sizeof(double));
+    s->dv = dv;
+    s->n = n;
+}
+
+/* interpToFinestGrid on the spectra s[0..ns) in place */
+static void m1_finest(double v1, int ns, m1spec **s) {
+    int loc = 0;
+    for (int k = 1; k < ns; k++)
+        if (s[k]->dv < s[loc]->dv) loc = k;
+    const double dvl = s[loc]->dv;
+    const int64_t nl = s[loc]->n;
+    for (int k = 0; k < ns; k++) {
+        if (k == loc || same_grid(v1, s[k]->dv, s[k]->n, dvl, nl)) continue;
+        double *o = (double *)calloc((size_t)nl, sizeof(double));
+        orc_interp(s[k]->x, s[k]->dv, v1, (int)s[k]->n, o, dvl, v1, (int)nl);
+        free(s[k]->x);
+        s[k]->x = o;
+        s[k]->dv = dvl;
+        s[k]->n = nl;
+    }
+}
+
+int64_t orc_rt_mode1(int nlay, double v1, double v2, const double *dv, const int64_t *nlim, const int64_t *off,
+                     const double *od_up, const double *od_dn, const double *od_sun, const double *fac_up,
+                     const double *fac_dn, const double *fac_sun, const double *t_lay, const double *t_lev,
+                     int obs_level, int lin, int thermal, int solar, double emis, double tskin, double refl,
+                     const double *sol, int64_t sol_n, double sol_dv, double *rad, double *tx, int64_t cap,
+                     double *dv_out) {
+    if (!thermal && emis < 1.) return fail("mode1: a reflecting surface needs the thermal down path");
+    if (!(emis < 1.) && !(tskin > 0.)) return fail("mode1: no surface term (emissivity 1 and no skin temperature)");
+    m1spec TxA, RadUp, RadDn, SolTx;
+    m1_new(&TxA, v2 - v1, 2);
+    m1_new(&RadUp, v2 - v1, 2);
+    m1_new(&RadDn, v2 - v1, 2);
+    m1_new(&SolTx, v2 - v1, 2);
+    for (int i = 0; i < 2; i++) { TxA.x[i] = 1.; SolTx.x[i] = 1.; }
+    for (int l = nlay - 1; l >= 0; l--) {
+        const int below = l < obs_level;
+        const double ldv = dv[l];
+        const int64_t ln = nlim[l];
+        const double tbot = t_lev[l], ttop = t_lev[l + 1], tave = t_lay[l];
+        m1spec lTxA, lTxB, lEmDn, lEmUp, lSol;
+        m1_new(&lTxA, ldv, ln);
+        m1_new(&lTxB, ldv, ln);
+        m1_new(&lEmDn, ldv, ln);
+        m1_new(&lEmUp, ldv, ln);
+        m1_new(&lSol, ldv, ln);
+        double *oda = (double *)calloc((size_t)ln, sizeof(double));
+        for (int64_t i = 0; i < ln; i++) {
+            const double v = v1 + (double)i * ldv;
+            if (below) {
+                oda[i] = od_up[off[l] + i] * fac_up[l];
+                lTxA.x[i] = exp(-oda[i]);
+            }
+            if (thermal) {
+                const double odb = od_dn[off[l] + i] * fac_dn[l];
+                lTxB.x[i] = exp(-odb);
+                lEmDn.x[i] = layer_emis(v, tbot, ttop, tave, lTxB.x[i], odb, lin);
+            }
+            if (solar) lSol.x[i] = exp(-(od_sun[off[l] + i] * fac_sun[l]));
+        }
+        if (thermal) {
+            m1spec *a[3] = {&lEmDn, &lTxB, &RadDn};
+            m1_finest(v1, 3, a);
+            for (int64_t i = 0; i < RadDn.n; i++) RadDn.x[i] = lEmDn.x[i] + RadDn.x[i] * lTxB.x[i];
+            if (below) {
+                if (lin != 0) {
+                    for (int64_t i = 0; i < ln; i++)
+                        lEmUp.x[i] = layer_emis(v1 + (double)i * ldv, ttop, tbot, tave, lTxA.x[i], oda[i], lin);
+                } else {            /* lyrEmUp = lyrEmDn, on its grid after the merge above */
+                    free(lEmUp.x);
+                    m1_new(&lEmUp, lEmDn.dv, lEmDn.n);
+                    memcpy(lEmUp.x, lEmDn.x, sizeof(double) * (size_t)lEmDn.n);
+                }
+                m1spec *b[3] = {&lEmUp, &TxA, &RadUp};
+                m1_finest(v1, 3, b);
+                for (int64_t i = 0; i < RadUp.n; i++) RadUp.x[i] = RadUp.x[i] + TxA.x[i] * lEmUp.x[i];
+            }
+        }
+        if (below) {
+            m1spec *c[2] = {&lTxA, &TxA};
+            m1_finest(v1, 2, c);
+            for (int64_t i = 0; i < TxA.n; i++) TxA.x[i] = TxA.x[i] * lTxA.x[i];
+        }
+        if (solar) {
+            m1spec *d[2] = {&lSol, &SolTx};
+            m1_finest(v1, 2, d);
+            for (int64_t i = 0; i < SolTx.n; i++) SolTx.x[i] = SolTx.x[i] * lSol.x[i];
+        }
+        free(oda); free(lTxA.x); free(lTxB.x); free(lEmDn.x); free(lEmUp.x); free(lSol.x);
+    }
+    /* addThermalBoundary (flux off) */
+    m1spec Tot;
+    if (emis < 1.) {
+        m1spec *a[3] = {&TxA, &RadUp, &RadDn};
+        m1_finest(v1, 3, a);
+        m1_new(&Tot, RadUp.dv, RadUp.n);
+        for (int64_t i = 0; i < Tot.n; i++) {
+            const double v = v1 + (double)i * RadUp.dv;
+            const double srad = planck_x(v, tskin / RADCN2) * emis, srefl = 1.0 - emis;
+            Tot.x[i] = RadUp.x[i] + TxA.x[i] * (srad + srefl * RadDn.x[i]);
+        }
+    } else {
+        m1spec *a[2] = {&RadUp, &TxA};
+        m1_finest(v1, 2, a);
+        m1_new(&Tot, RadUp.dv, RadUp.n);
+        for (int64_t i = 0; i < Tot.n; i++) {
+            const double v = v1 + (double)i * RadUp.dv;
+            Tot.x[i] = RadUp.x[i] + TxA.x[i] * (planck_x(v, tskin / RADCN2) * emis);
+        }
+    }
+    /* addSolarBoundary */
+    if (solar) {
+        m1spec S0;
+        m1_new(&S0, sol_dv, sol_n);
+        memcpy(S0.x, sol, sizeof(double) * (size_t)sol_n);
+        m1spec *a[4] = {&TxA, &Tot, &SolTx, &S0};
+        m1_finest(v1, 4, a);
+        for (int64_t i = 0; i < TxA.n; i++) Tot.x[i] = Tot.x[i] + TxA.x[i] * refl * SolTx.x[i] * S0.x[i];
+        free(S0.x);
+    }
+    const int64_t n = Tot.n;
+    for (int64_t i = 0; i < n && i < cap; i++) {
+        rad[i] = Tot.x[i];
+        if (tx) tx[i] = TxA.n == n ? TxA.x[i] : 0.;
+    }
+    *dv_out = Tot.dv;
+    free(TxA.x); free(RadUp.x); free(RadDn.x); free(SolTx.x); free(Tot.x);
+    return n;
 }

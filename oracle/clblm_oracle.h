@@ -133,6 +133,12 @@ int64_t orc_scan_interp(const double *x, int64_t n, double v1, double dv, double
 int orc_mtckd_h2o(const double *wvn, const double *self_ref, const double *for_ref, const double *texp, int ncoeffin,
                   double ref_temp, double ref_press, double pave, double tave, double wk1, double wtot, double v1,
                   double v2, int jrad, double xself, double xfrgn, double *absrb, int cap);
+int64_t orc_rt_mode1(int nlay, double v1, double v2, const double *dv, const int64_t *nlim, const int64_t *off,
+                     const double *od_up, const double *od_dn, const double *od_sun, const double *fac_up,
+                     const double *fac_dn, const double *fac_sun, const double *t_lay, const double *t_lev,
+                     int obs_level, int lin, int thermal, int solar, double emis, double tskin, double refl,
+                     const double *sol, int64_t sol_n, double sol_dv, double *rad, double *tx, int64_t cap,
+                     double *dv_out);
 void orc_scan_shape(int jfn, double *xf);
 int64_t orc_scanfn_conv(const double *x, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
                         int funct_id, double hwhm, double *out, int64_t cap, double *v1res, double *dvres);

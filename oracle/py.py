@@ -328,6 +328,43 @@ def scanfn(x, v1, dv, v1out, v2out, dvout, funct_id, hwhm):
     return out[:n], a.value, b.value
 
 
+def rt_mode1(ods, dvs, v1, v2, t_lay, t_lev, od_dn=None, od_sun=None, fac_up=None, fac_dn=None, fac_sun=None,
+             obs_level=None, lin=1, thermal=True, solar=False, emis=0.98, tskin=295.0, refl=0.0, sol=None,
+             sol_dv=None):
+    """orc_rt_mode1: the merge-down radiance at an observer above the layers (ods: list of
+    per-layer view-path OD arrays on v1 + i dvs[l], layer 0 at the surface).  Returns
+    (radiance, total transmittance, DV)."""
+    L = lib()
+    nlay = len(ods)
+    nl = np.array([o.size for o in ods], np.int64)
+    off = np.concatenate([[0], np.cumsum(nl)[:-1]]).astype(np.int64)
+    cat = lambda a: np.ascontiguousarray(np.concatenate(a), np.float64)
+    up = cat(ods)
+    dn = up if od_dn is None else cat(od_dn)
+    sn = up if od_sun is None else cat(od_sun)
+    one = np.ones(nlay)
+    fu, fd, fs = [np.ascontiguousarray(one if f is None else f, np.float64) for f in (fac_up, fac_dn, fac_sun)]
+    dv = np.ascontiguousarray(dvs, np.float64)
+    s0 = np.zeros(2) if sol is None else np.ascontiguousarray(sol, np.float64)
+    cap = int(max(nl.max(), s0.size, 2))
+    rad, tx = np.zeros(cap), np.zeros(cap)
+    dvo = ctypes.c_double()
+    D, P, I = ctypes.c_double, ctypes.c_void_p, ctypes.c_int
+    L.orc_rt_mode1.restype = ctypes.c_int64
+    L.orc_rt_mode1.argtypes = [I, D, D, P, P, P, P, P, P, P, P, P, P, P, I, I, I, I, D, D, D, P, ctypes.c_int64, D,
+                               P, P, ctypes.c_int64, ctypes.POINTER(D)]
+    tl = np.ascontiguousarray(t_lay, np.float64)
+    tv = np.ascontiguousarray(t_lev, np.float64)
+    n = L.orc_rt_mode1(nlay, v1, v2, dv.ctypes.data, nl.ctypes.data, off.ctypes.data, up.ctypes.data, dn.ctypes.data,
+                       sn.ctypes.data, fu.ctypes.data, fd.ctypes.data, fs.ctypes.data, tl.ctypes.data, tv.ctypes.data,
+                       nlay if obs_level is None else obs_level, lin, int(thermal), int(solar), emis, tskin, refl,
+                       s0.ctypes.data, s0.size, float(dv[-1] if sol_dv is None else sol_dv), rad.ctypes.data,
+                       tx.ctypes.data, cap, ctypes.byref(dvo))
+    if n < 0:
+        raise RuntimeError(L.orc_error().decode())
+    return rad[:n], tx[:n], dvo.value
+
+
 def mtckd_h2o(table, pave, tave, wk1, wtot, v1, v2, jrad, xself=1.0, xfrgn=1.0, absrb=None):
     """orc_mtckd_h2o: the MT_CKD water-vapour continuum of one layer on OPDPTH's ABSRB
     grid (V1ABS = INT(V1) - 3 at 1 cm-1), added to absrb (zeros by default)."""
