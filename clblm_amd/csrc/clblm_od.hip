@@ -3322,6 +3322,9 @@ extern "C" int clblm_rt_mode1_impl(int nlay, double v1, double v2, const double 
                                    int lin, int thermal, int solar, double emis, double tskin, double refl,
                                    const double *d_sol, int64_t sol_n, double sol_dv, double *scratch, int64_t max_n,
                                    double *d_rad, double *d_tx, double *dv_out, int64_t *n_out, void *stream);
+extern "C" int clblm_filter_impl(void *state, const double *d_in, int64_t n, double v1, double dv, int nchan,
+                                 const double *v1f, const double *dvf, const int32_t *nptf, const int64_t *xf_off,
+                                 const double *xf, double *d_out, void *stream);
 extern "C" int clblm_scanfn_impl(void *state, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in,
                                  double v1, double dv, double v1out, double v2out, double dvout, int funct_id,
                                  double hwhm, double *d_out, int64_t ld_out, int64_t *nout, double *v1res,
@@ -3906,6 +3909,23 @@ int clblm_mtckd_absrb(clblm_ctx *c, int nlay, const clblm_layer *layers, const d
                            (const MtJob *)(b + oj), (const double *)(b + oc), d_absrb);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));   // the host job list goes out of scope
+    return 0;
+}
+
+int clblm_filter(clblm_ctx *c, const double *d_in, int64_t n, double v1, double dv, int nchan, const double *v1f,
+                 const double *dvf, const int32_t *nptf, const int64_t *xf_off, const double *xf, double *d_out,
+                 void *stream) {
+    if (!c) return set_err(CLBLM_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->scan) c->scan = clblm_scan_state_new();
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (c->last_stream && c->last_stream != st && c->ncalls > 0) {   // after the batches that made d_in
+        const int kp = (int)((c->ncalls + clblm_ctx::NSLOT - 1) % clblm_ctx::NSLOT);
+        HIPCHK(hipStreamWaitEvent(st, c->slot[kp].done, 0));
+    }
+    int rc = clblm_filter_impl(c->scan, d_in, n, v1, dv, nchan, v1f, dvf, nptf, xf_off, xf, d_out, st);
+    if (rc) return set_err(rc, clblm_scan_error());
     return 0;
 }
 

@@ -459,6 +459,8 @@ struct ScanState {
     size_t meta_cap = 0;
     void *conv = nullptr;          // spectral-domain scan: tables and per-box arrays
     size_t conv_cap = 0;
+    void *filt = nullptr;          // SRF filter: channels, filter values, panel bounds
+    size_t filt_cap = 0;
     ~ScanState() {
         if (fwd) hipfftDestroy(fwd);
         if (inv) hipfftDestroy(inv);
@@ -468,6 +470,7 @@ struct ScanState {
         (void)hipFree(win);
         (void)hipFree(meta);
         (void)hipFree(conv);
+        (void)hipFree(filt);
     }
 };
 
@@ -1508,7 +1511,155 @@ int scanfn_conv(ScanState &S, const double *d_in, int64_t n, int64_t nrows, int6
     return run_conv(S, cp, need_data ? xs : nullptr, ldx, nrows, count_only, d_out, ld_out, nout, st);
 }
 
+// ==================================================================================
+// Spectral response function filter: clblm_FLTRFN with CNVFLT
+// (clblm_ScanFilter.f90:1703-1900) for each channel of postProc_filter_singleStruct
+// (clblm_PostProc.f90:289-336).  One lane per channel walks the input panels the
+// channel reads (panelInput's accumulated panel bounds come from the host) and sums
+// its points in the reference's order, so the result is the reference's to the bit.
+// ==================================================================================
+struct FChan {
+    double v1f, v2f, dvf, sumflt;
+    int64_t xf;        // the channel's filter values (1-based at xf + 1, end values halved)
+    int32_t npts, bad;
+};
+
+__global__ void k_filter(const FChan *ch, int nchan, const double *xfv, const double *x, const double *vmin,
+                         const double *vmax, const int32_t *plen, int npanel, double dv, double *out, int *err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchan) return;
+    const FChan C = ch[c];
+    if (C.bad) {          // V1F < 0: the reference returns without a value
+        out[c] = 0.;
+        return;
+    }
+    const double *XF = xfv + C.xf;
+    const double V1F = C.v1f, V2F = C.v2f, DVF = C.dvf, VBOT = V1F, VTOP = V2F;
+    double RFILTR = 0.0;
+    int p = 0;
+    int64_t start = 0;
+    while (p < npanel && vmax[p] < VBOT) start += plen[p++];
+    for (; p < npanel; start += plen[p++]) {
+        const double VMIN = vmin[p], VMAX = vmax[p];
+        const int nlim = plen[p];
+        const double *S = x + start - 1;
+        int ILO = 1, IHI = nlim, IDATA = 0;
+        if (VMAX > VTOP) {
+            IHI = (int)((VTOP - VMIN) / dv + 1.5);
+            IDATA = 1;
+        }
+        const double DVI = dv, V1I = VMIN;
+        int IMIN = (int)((V1F - V1I) / DVI + 1.0001);
+        if (IMIN < ILO) IMIN = ILO;
+        int IMAX = (int)((V2F + V1F - V1I) / DVI + 1.0001);
+        if (IMAX > IHI) IMAX = IHI;
+        const double XIF0 = (V1I - V1F) / DVF + 1.0001;
+        const double XDVIF = DVI / DVF;
+        if ((int)(XIF0 + XDVIF * (double)IMAX) + 1 > C.npts) {
+            for (int i = IMAX - 1; i >= IMIN; i--)
+                if ((int)(XIF0 + XDVIF * (double)i) + 1 <= C.npts) {
+                    IMAX = i;
+                    break;
+                }
+        }
+        if ((int)(XIF0 + XDVIF * (double)IMAX) + 1 > C.npts) {   // the reference stops here
+            atomicOr(err, 1);
+            out[c] = __longlong_as_double(0x7ff8000000000000LL);
+            return;
+        }
+        for (int I = IMIN; I <= IMAX; I++) {
+            const int IFL = (int)(XIF0 + XDVIF * (double)I);
+            const double v2s = V1I + (double)I * DVI;
+            const double vxf1 = V1F + (double)(IFL - 1) * DVF;
+            const double vxf2 = V1F + (double)IFL * DVF;
+            const double pp = (vxf2 - v2s) / (vxf2 - vxf1);
+            RFILTR = RFILTR + S[I] * (pp * XF[IFL] + (1. - pp) * XF[IFL + 1]);
+        }
+        if (IDATA == 1) break;
+    }
+    out[c] = (RFILTR * dv) / C.sumflt;
+}
+
 }  // namespace
+
+extern "C" __attribute__((visibility("hidden"))) int clblm_filter_impl(void *state, const double *d_in, int64_t n,
+                                                                       double v1, double dv, int nchan,
+                                                                       const double *v1f, const double *dvf,
+                                                                       const int32_t *nptf, const int64_t *xf_off,
+                                                                       const double *xf, double *d_out, void *stream) {
+    if (!d_in || !d_out || !v1f || !dvf || !nptf || !xf_off || !xf || n < 2 || nchan < 1 || !(dv > 0.))
+        return fail(CLBLM_ERR_ARG, "filter: null argument, fewer than 2 points or no channel");
+    ScanState &S = *(ScanState *)state;
+    hipStream_t st = (hipStream_t)stream;
+    // the channels (the filter values copied with their end values halved) and their sums
+    std::vector<FChan> ch((size_t)nchan);
+    std::vector<double> vals;
+    for (int c = 0; c < nchan; c++) {
+        const int np = nptf[c];
+        if (np < 2 || np > 3001) return fail(CLBLM_ERR_ARG, "filter: 2..3001 points per channel (NFLTPT)");
+        FChan &C = ch[(size_t)c];
+        double DVF = dvf[c], V1F = v1f[c];
+        C.bad = V1F < 0 ? 1 : 0;
+        if (DVF < 0.) {
+            DVF = std::fabs(DVF);
+            if ((np - 1) % 2 != 0) return fail(CLBLM_ERR_ARG, "filter: a centre frequency needs an odd number of points");
+            V1F = V1F - DVF * (double)((np - 1) / 2);
+        }
+        if (!(DVF > 0.)) return fail(CLBLM_ERR_ARG, "filter: the filter step must be nonzero");
+        C.v1f = V1F;
+        C.dvf = DVF;
+        C.v2f = V1F + DVF * (double)(np - 1);
+        C.npts = np;
+        C.xf = (int64_t)vals.size();
+        vals.push_back(0.);   // 1-based
+        const double *f = xf + xf_off[c];
+        for (int i = 0; i < np; i++) vals.push_back(f[i]);
+        vals[(size_t)C.xf + 1] = 0.5 * vals[(size_t)C.xf + 1];
+        vals[(size_t)C.xf + (size_t)np] = 0.5 * vals[(size_t)C.xf + (size_t)np];
+        double sum = 0.0;
+        for (int i = 1; i <= np; i++) sum = sum + vals[(size_t)C.xf + (size_t)i];
+        C.sumflt = sum * DVF;
+    }
+    // panelInput's panel bounds, accumulated as the reference does
+    std::vector<double> vmin, vmax;
+    std::vector<int32_t> plen;
+    double done_v2 = v1 - dv;
+    for (int64_t done = 0; done < n;) {
+        const int nlim = (int)std::min<int64_t>(2400, n - done);
+        const double a = done_v2 + dv, b = a + (double)(nlim - 1) * dv;
+        vmin.push_back(a);
+        vmax.push_back(b);
+        plen.push_back(nlim);
+        done += nlim;
+        done_v2 = b;
+    }
+    auto up = [](size_t o) { return (o + 255) & ~(size_t)255; };
+    const size_t o_ch = 0, o_v = up(sizeof(FChan) * ch.size()), o_a = up(o_v + sizeof(double) * vals.size());
+    const size_t o_b = up(o_a + sizeof(double) * vmin.size()), o_l = up(o_b + sizeof(double) * vmax.size());
+    const size_t o_e = up(o_l + sizeof(int32_t) * plen.size()), total = o_e + 256;
+    SCHK(hipStreamSynchronize(st));   // the buffer may still be read by a previous call
+    int rc = grow(&S.filt, &S.filt_cap, total);
+    if (rc) return rc;
+    std::vector<char> host(o_e + sizeof(int), 0);
+    std::memcpy(host.data() + o_ch, ch.data(), sizeof(FChan) * ch.size());
+    std::memcpy(host.data() + o_v, vals.data(), sizeof(double) * vals.size());
+    std::memcpy(host.data() + o_a, vmin.data(), sizeof(double) * vmin.size());
+    std::memcpy(host.data() + o_b, vmax.data(), sizeof(double) * vmax.size());
+    std::memcpy(host.data() + o_l, plen.data(), sizeof(int32_t) * plen.size());
+    char *b = (char *)S.filt;
+    SCHK(hipMemcpyAsync(b, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_filter, dim3(cdiv(nchan, 64)), dim3(64), 0, st, (const FChan *)(b + o_ch), nchan,
+                       (const double *)(b + o_v), d_in, (const double *)(b + o_a), (const double *)(b + o_b),
+                       (const int32_t *)(b + o_l), (int)plen.size(), dv, d_out, (int *)(b + o_e));
+    SCHK(hipGetLastError());
+    int herr = 0;
+    SCHK(hipMemcpyAsync(&herr, b + o_e, sizeof(int), hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));   // the host copies go out of scope
+    if (herr)
+        return fail(CLBLM_ERR_ARG, "filter: a channel reads past its filter (CNVFLT's subscript check, where the "
+                                   "reference stops); its output is NaN, the other channels are filled");
+    return 0;
+}
 
 extern "C" __attribute__((visibility("hidden"))) int clblm_scanfn_impl(
     void *state, const double *d_in, int64_t n, int64_t nrows, int64_t ld_in, double v1, double dv, double v1out,

@@ -259,6 +259,29 @@ class Context:
                                     ctypes.c_void_p(stream)))
         return out[:, :nout.value].reshape(*shape[:-1], nout.value), a.value, b.value
 
+    def filter(self, x, v1, dv, channels, stream=None):
+        """postProc_filter_singleStruct (src/clblm_src/clblm_PostProc.f90:289-336) with
+        clblm_FLTRFN: x a float64 cuda tensor (one spectrum on v1 + i*dv); channels a list
+        of (v1f, dvf, filter values), dvf < 0 meaning v1f is the centre.  Returns a
+        float64 cuda tensor of one value per channel."""
+        import torch
+        assert x.dtype == torch.float64 and x.is_cuda and x.dim() == 1
+        n = len(channels)
+        v1f = np.array([c[0] for c in channels], np.float64)
+        dvf = np.array([c[1] for c in channels], np.float64)
+        vals = [np.ascontiguousarray(c[2], np.float64) for c in channels]
+        nptf = np.array([v.size for v in vals], np.int32)
+        off = np.concatenate([[0], np.cumsum(nptf)[:-1]]).astype(np.int64)
+        xf = np.ascontiguousarray(np.concatenate(vals))
+        out = torch.empty(n, dtype=torch.float64, device=x.device)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        x = x.contiguous()
+        check(self.lib.clblm_filter(self.ptr, ctypes.c_void_p(x.data_ptr()), int(x.numel()), float(v1), float(dv),
+                                    n, v1f.ctypes.data, dvf.ctypes.data, nptf.ctypes.data, off.ctypes.data,
+                                    xf.ctypes.data, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)))
+        return out
+
     def prebox(self, x, v1, dv, boxcar_hw):
         """output_JacArray's boxcar pre-average (clblm_noScattJac.f90:1341-1409) of a
         float64 cuda tensor of spectra on v1 + i*dv (last axis): the rectangle scan with M = int(2 boxcarHW/dv)

@@ -434,6 +434,20 @@ int clblm_rt_mode1(clblm_ctx *ctx, const clblm_mode1_params *p, const double *dv
                    const double *t_lev, const double *d_sol, int64_t sol_n, double sol_dv, double *d_rad,
                    double *d_tx, double *dv_out, int64_t *nlim_out, void *stream);
 
+/* Spectral response function filter: clblm_FLTRFN with CNVFLT
+ * (src/clblm_src/clblm_ScanFilter.f90:1703-1900), once per channel as
+ * postProc_filter_singleStruct does (clblm_PostProc.f90:289-336).  d_in: the spectrum (n
+ * device doubles on v1 + i dv); channel c: nptf[c] filter values at xf + xf_off[c] (host)
+ * on v1f[c] + i dvf[c] (dvf < 0: v1f is the centre, nptf odd).  d_out[c] (device): the
+ * spectrum integrated against the filter (end values halved, linear between values)
+ * over the filter's own integral, bit for bit the reference's sum; 0 for v1f < 0, where
+ * the reference returns no value.  A channel whose end falls where CNVFLT's subscript
+ * check stops the reference (:1830-1841) gets NaN and the call returns CLBLM_ERR_ARG
+ * after filling the others.  Waits for `stream` before returning. */
+int clblm_filter(clblm_ctx *ctx, const double *d_in, int64_t n, double v1, double dv, int nchan, const double *v1f,
+                 const double *dvf, const int32_t *nptf, const int64_t *xf_off, const double *xf, double *d_out,
+                 void *stream);
+
 /* ---- instrument scan ------------------------------------------------------------
  * Drop-in for clblm_FFTSCN_array (clblm_FFT.f90:308-349): the spectrum d_in[0..n)
  * (device) on v1 + i*dv is scanned with function funct_id of half width hwhm between

@@ -2868,3 +2868,94 @@ int64_t orc_rt_mode1(int nlay, double v1, double v2, const double *dv, const int
     free(TxA.x); free(RadUp.x); free(RadDn.x); free(SolTx.x); free(Tot.x);
     return n;
 }
+
+/* ---------------------------------------------------------------------------
+ * clblm_FLTRFN with CNVFLT (clblm_ScanFilter.f90:1703-1900), the spectral response
+ * function filter of postProc_filter_singleStruct (clblm_PostProc.f90:289-336): the
+ * spectrum x[0..n) on v1 + i dv, read in 2400-point panels (panelInput :2320-2404,
+ * VBOT = V1F, VTOP = V2F), summed against the filter xf[0..nptf) on v1f + i dvf (its end
+ * values halved, each spectral point weighted by the filter interpolated linearly at
+ * the point's interval), times DV, over the filter's own sum times DVF.  dvf < 0:
+ * v1f is the filter's centre (odd nptf).  Returns 0 (result in *out), -1 on error.
+ * ------------------------------------------------------------------------- */
+int orc_fltrfn(const double *x, int64_t n, double v1, double dv, double v1f, double dvf, int nptf, const double *xf,
+               double *out) {
+    if (nptf < 2 || nptf > 3001) return fail("filter: 2..3001 filter points (NFLTPT)");
+    double DVF = dvf, V1F = v1f;
+    *out = 0.;
+    if (V1F < 0) return 0;   /* the reference returns without setting filtOut */
+    if (DVF < 0.) {
+        DVF = fabs(DVF);
+        if ((nptf - 1) % 2 != 0) return fail("filter: a centre frequency needs an odd number of points");
+        const int half = (nptf - 1) / 2;
+        V1F = V1F - DVF * (double)half;
+    }
+    double *XF = (double *)malloc(sizeof(double) * (size_t)(nptf + 1));   /* 1-based */
+    for (int i = 0; i < nptf; i++) XF[i + 1] = xf[i];
+    const int NPTS = nptf;
+    const double V2F = V1F + DVF * (double)(NPTS - 1);
+    XF[1] = 0.5 * XF[1];
+    XF[NPTS] = 0.5 * XF[NPTS];
+    double SUMFLT = 0.0;
+    for (int i = 1; i <= NPTS; i++) SUMFLT = SUMFLT + XF[i];
+    SUMFLT = SUMFLT * DVF;
+    double RFILTR = 0.0;
+    const double VBOT = V1F, VTOP = V2F;
+    int64_t done = 0;
+    double done_v2 = v1 - dv;
+    for (;;) {
+        /* panelInput, NREN = 0 */
+        int nlim;
+        double VMIN = 0., VMAX = 0.;
+        for (;;) {
+            nlim = 2400;
+            if (done + nlim > n) nlim = (int)(n - done);
+            if (nlim == 0) break;
+            VMIN = done_v2 + dv;
+            VMAX = VMIN + (double)(nlim - 1) * dv;
+            done += nlim;
+            done_v2 = VMAX;
+            if (VMAX >= VBOT) break;
+        }
+        if (nlim == 0) break;                  /* IEOFSC = 0 */
+        const double *S = x + (done - nlim) - 1;   /* S[1..nlim] */
+        int ILO = 1, IHI = nlim, IDATA = 0;
+        if (VMAX > VTOP) {
+            IHI = (int)((VTOP - VMIN) / dv + 1.5);
+            IDATA = 1;
+        }
+        /* CNVFLT */
+        const double DVI = dv, V1I = VMIN;
+        int IMIN = (int)((V1F - V1I) / DVI + 1.0001);
+        if (IMIN < ILO) IMIN = ILO;
+        int IMAX = (int)((V2F + V1F - V1I) / DVI + 1.0001);
+        if (IMAX > IHI) IMAX = IHI;
+        const double XIF0 = (V1I - V1F) / DVF + 1.0001;
+        const double XDVIF = DVI / DVF;
+        if ((int)(XIF0 + XDVIF * (double)IMAX) + 1 > NPTS) {
+            for (int i = IMAX - 1; i >= IMIN; i--)
+                if ((int)(XIF0 + XDVIF * (double)i) + 1 <= NPTS) {
+                    IMAX = i;
+                    break;
+                }
+        }
+        if ((int)(XIF0 + XDVIF * (double)IMAX) + 1 > NPTS) {
+            free(XF);
+            return fail("filter: subscript beyond the filter (CNVFLT)");
+        }
+        for (int I = IMIN; I <= IMAX; I++) {
+            const int IFL = (int)(XIF0 + XDVIF * (double)I);
+            const double v1s = V1I + (double)(I - 1) * DVI;
+            const double v2s = V1I + (double)I * DVI;
+            const double vxf1 = V1F + (double)(IFL - 1) * DVF;
+            const double vxf2 = V1F + (double)IFL * DVF;
+            const double p = (vxf2 - v2s) / (vxf2 - vxf1);
+            (void)v1s;
+            RFILTR = RFILTR + S[I] * (p * XF[IFL] + (1. - p) * XF[IFL + 1]);
+        }
+        if (IDATA == 1) break;
+    }
+    *out = (RFILTR * dv) / SUMFLT;
+    free(XF);
+    return 0;
+}

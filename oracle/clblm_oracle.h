@@ -139,6 +139,8 @@ int64_t orc_rt_mode1(int nlay, double v1, double v2, const double *dv, const int
                      int obs_level, int lin, int thermal, int solar, double emis, double tskin, double refl,
                      const double *sol, int64_t sol_n, double sol_dv, double *rad, double *tx, int64_t cap,
                      double *dv_out);
+int orc_fltrfn(const double *x, int64_t n, double v1, double dv, double v1f, double dvf, int nptf, const double *xf,
+               double *out);
 void orc_scan_shape(int jfn, double *xf);
 int64_t orc_scanfn_conv(const double *x, int64_t n, double v1, double dv, double v1out, double v2out, double dvout,
                         int funct_id, double hwhm, double *out, int64_t cap, double *v1res, double *dvres);

@@ -385,6 +385,21 @@ def mtckd_h2o(table, pave, tave, wk1, wtot, v1, v2, jrad, xself=1.0, xfrgn=1.0, 
     return out[:n]
 
 
+def fltrfn(x, v1, dv, v1f, dvf, xf):
+    """orc_fltrfn: the filtered value of the spectrum x on v1 + i dv through the filter
+    xf on v1f + i dvf (dvf < 0: v1f is the centre)."""
+    L = lib()
+    D, P = ctypes.c_double, ctypes.c_void_p
+    L.orc_fltrfn.restype = ctypes.c_int
+    L.orc_fltrfn.argtypes = [P, ctypes.c_int64, D, D, D, D, ctypes.c_int, P, ctypes.POINTER(D)]
+    x = np.ascontiguousarray(x, np.float64)
+    f = np.ascontiguousarray(xf, np.float64)
+    out = ctypes.c_double()
+    if L.orc_fltrfn(x.ctypes.data, x.size, v1, dv, v1f, dvf, f.size, f.ctypes.data, ctypes.byref(out)):
+        raise RuntimeError(L.orc_error().decode())
+    return out.value
+
+
 def scan_shape(jfn):
     """orc_scan_shape: the tabulated scanning function XF(1..NJMX) of JFN 1..4."""
     L = lib()

@@ -10,7 +10,7 @@ module ref_bind
    use Module_Spectrum,        only: XINT, interpSpectrum, CLBLM_Spectrum, CLBLM_Spectrum_init, interpToFinestGrid
    use Module_EMLAY,           only: planck, planckFnDrv, rlin, layerEmis
    use Module_FFT,             only: clblm_FFTSCN, Intpdr
-   use Module_ScanFilter,      only: clblm_SCANFN
+   use Module_ScanFilter,      only: clblm_SCANFN, clblm_FLTRFN
    use Module_LineData,        only: INPUT_HEADER, INPUT_BLOCK, NLINEREC, MXBRDMOL, MXBRDMOL3
    implicit none
    external :: BUFIN_sgl
@@ -94,6 +94,28 @@ contains
       nout = n
       v1o = c1; v2o = c2; dvo = dvx
       out(1:min(n, cap)) = d(1:min(n, cap))
+   end subroutine
+
+   ! clblm_FLTRFN (clblm_ScanFilter.f90:1703-1860): the spectrum sp(1:np) on v1 + (i-1) dv
+   ! through the filter xf(1:nptf) from v1f at dvf (dvf < 0: v1f is the centre)
+   subroutine ref_fltrfn(np, sp, v1, dv, v1f, dvf, nptf, xf, out) bind(C, name="ref_fltrfn")
+      integer(c_int), value :: np, nptf
+      real(c_double), value :: v1, dv, v1f, dvf
+      real(c_double), intent(in) :: sp(np), xf(nptf)
+      real(c_double), intent(out) :: out
+      type(CLBLM_Spectrum) :: s
+      real(r8) :: a, b
+      real :: d, e, f
+      real, allocatable :: x(:)
+      a = v1; d = dv
+      call CLBLM_Spectrum_init(s, a, d, np)
+      s%spect(s%indV1:s%indV2) = sp
+      b = v1f; e = dvf
+      allocate(x(nptf))
+      x = xf
+      f = 0.
+      call clblm_FLTRFN(s, b, e, nptf, x, f)
+      out = f
    end subroutine
 
    ! Intpdr (clblm_FFT.f90:2392-2443): sp(1:np) on v1 + (i-1) dv onto vv1..vv2 at dvv

@@ -239,3 +239,43 @@ def test_conv_scan_large(ctx):
     got, g1, gd = _conv_gpu(ctx, x, 500.0, dv, 600.0, 2400.0, 0.125, 5, 0.5)
     assert (got.size, g1, gd) == (want.size, w1, wd)
     assert np.array_equal(got, want), float(np.abs(got - want).max())
+
+
+def test_filter_matches_oracle_bitwise(ctx):
+    """The SRF filter (clblm_filter: clblm_FLTRFN per channel) on the GPU, all the
+    seeded channels of the oracle's pin in one call per spectrum, bit for bit; the
+    channels where the reference stops (CNVFLT's subscript check) are refused."""
+    import torch
+    from clblm_amd._lib import ClblmError
+    from oracle.py import fltrfn
+    from test_oracle_scan import filter_cases
+    n_ok = 0
+    for x, v1, dv, v1f, dvf, xf in filter_cases():
+        try:
+            want = fltrfn(x, v1, dv, v1f, dvf, xf)
+        except RuntimeError:
+            with pytest.raises(ClblmError):
+                ctx.filter(torch.tensor(x, device="cuda"), v1, dv, [(v1f, dvf, xf)])
+            continue
+        got = ctx.filter(torch.tensor(x, device="cuda"), v1, dv, [(v1f, dvf, xf)]).cpu().numpy()
+        assert got[0] == want, (x.size, dv, v1f, dvf)
+        n_ok += 1
+    assert n_ok >= 40
+    # many channels over one spectrum in one call; every 24th of these ends where the
+    # reference stops (CNVFLT's subscript check): those are kept for a call of their own
+    rng = np.random.default_rng(5)
+    n, dv = 200001, 0.005
+    x = 1 + 0.3 * np.sin(np.arange(n) * dv / 0.7) + 0.05 * rng.normal(size=n)
+    chans, stop = [], []
+    for k in range(1500):
+        c = (620.0 + 0.5 * k, -0.01, np.exp(-np.linspace(-2.5, 2.5, 201) ** 2))
+        try:
+            chans.append((c, fltrfn(x, 600.0, dv, *c)))
+        except RuntimeError:
+            stop.append(c)
+    assert len(chans) > 1400 and stop
+    xt = torch.tensor(x, device="cuda")
+    got = ctx.filter(xt, 600.0, dv, [c for c, _ in chans]).cpu().numpy()
+    assert np.array_equal(got, np.array([w for _, w in chans]))
+    with pytest.raises(ClblmError):
+        ctx.filter(xt, 600.0, dv, [chans[0][0], stop[0]])
