@@ -1,11 +1,13 @@
-"""Full-size cases, checked through size-independent properties.
+"""Full-size cases, against the oracle and through size-independent properties.
 
 * The full-profile workload (60 layers, 200 000 lines of 7 species, 500-2500 cm-1,
-  spline continuum): FP32C against FP64 within the 1e-5 bar at every point, and the
-  FP64 optical depth linear in the absorber columns (all columns and the broadening
-  column doubled give twice the line optical depth).
+  spline continuum): FP64 against the oracle within 1e-10 on every layer at the full
+  range, FP32C against FP64 within the 1e-5 bar at every point, and the FP64 optical
+  depth linear in the absorber columns (all columns and the broadening column doubled
+  give twice the line optical depth).
 * The large scene (2 000 000 lines over 10-3500 cm-1, 80 layers to 0.01 hPa): FP32C
-  against FP64 for every layer, in batches of 8 layers.
+  against FP64 for every layer, in batches of 8 layers, and FP64 against the oracle at
+  the full range on sampled layers (bottom, middle, top).
 """
 import numpy as np
 import pytest
@@ -91,3 +93,49 @@ def test_large_scene_fp32c_within_1e5_of_fp64(ctx):
               flush=True)
     print("large scene: %d lines, %d points, FP32C max rel err vs FP64 %.3e" % (lines.n, npts, worst))
     assert worst <= 1e-5
+
+
+def test_full_profile_fp64_matches_oracle_every_layer(ctx):
+    """All 60 layers of the full-profile workload, 500-2500 cm-1, the oracle's layers in
+    worker processes (about a minute of CPU)."""
+    from clblm_amd._lib import FP64
+    from tests_helpers_pool import oracle_layers
+    lines = W.profile_lines(200000, V1, V2, seed=1001)
+    layers = W.atmosphere(60, 0.1)
+    dvs = [W.calculate_dv(0.5 * (V1 + V2), l.pave, l.tave, l.w[1] / l.wtot) for l in layers]
+    conts = [W.continuum_absrb(l, V1, V2)[0] for l in layers]
+    refs = oracle_layers("profile", [(l, V1, V2, dv, c) for l, dv, c in zip(layers, dvs, conts)])
+    h = ctx.upload_lines(lines)
+    a, off = _run(ctx, h, layers, dvs, FP64, V1, V2, conts)
+    worst = 0.0
+    for k, ref in enumerate(refs):
+        od = a[off[k]:off[k + 1]]
+        assert od.size == ref.size, k
+        err = float((np.abs(od - ref) / np.maximum(np.abs(ref), 1e-300)).max())
+        worst = max(worst, err)
+        assert err <= 1e-10, (k, err)
+    print("full profile FP64 vs oracle: %d points, max rel err %.3e" % (off[-1], worst))
+    ctx.free_lines(h)
+
+
+def test_large_scene_fp64_matches_oracle_sampled_layers(ctx):
+    from clblm_amd._lib import FP64
+    from tests_helpers_pool import oracle_layers
+    lines = W.large_scene_lines()
+    layers = W.large_scene_atmosphere()
+    pick = [layers[k] for k in (0, 40, 79)]
+    dvs = [W.large_scene_dv(l) for l in pick]
+    refs = oracle_layers("large", [(l, W.LARGE_V1, W.LARGE_V2, dv, None) for l, dv in zip(pick, dvs)], workers=3)
+    h = ctx.upload_lines(lines)
+    a, off = _run(ctx, h, pick, dvs, FP64, W.LARGE_V1, W.LARGE_V2)
+    for k, ref in enumerate(refs):
+        od = a[off[k]:off[k + 1]]
+        assert od.size == ref.size, k
+        rel = np.abs(od - ref) / np.maximum(np.abs(ref), 1e-300)
+        i = int(np.argmax(rel))
+        print("large scene layer %d: %d points, FP64 max rel err vs oracle %.3e at OD %.3e (max OD %.3e); "
+              "max abs err / max OD %.3e; rel err where OD >= 1e-6 max: %.3e"
+              % (k, od.size, rel[i], ref[i], np.abs(ref).max(), np.abs(od - ref).max() / np.abs(ref).max(),
+                 rel[np.abs(ref) >= 1e-6 * np.abs(ref).max()].max()))
+        assert _rel(od, ref, 1e-6) <= 1e-10, k
+    ctx.free_lines(h)

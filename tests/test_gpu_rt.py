@@ -268,3 +268,78 @@ def test_jacobian_boxcar_pre_average(ctx, hw_in_dv):
         np.testing.assert_allclose(got, np.array([w[0] for w in want]), rtol=1e-12,
                                    atol=1e-12 * np.abs(rows).max(), err_msg=name)
     print("boxcar %.4g cm-1: %d -> %d points per block" % (hw, a["rad"].numel(), b["rad"].numel()))
+
+
+def _perturbed_block(ctx, k, dt_lev=0.0, q_fac=None, mol=None, v1=640.0, v2=646.0, nlay=8):
+    """The FP64 radiance of _scene with level k's temperature moved by dt_lev (its
+    layers' mean temperatures by dT/dT_lev and their broadening columns by
+    -Wtot dT/T, the perturbation drv_LevelTemp linearises, clblm_noScattJac.f90:
+    1005-1034) or molecule mol's level density scaled by q_fac (its layers' columns by
+    q_fac ** dlnW/dlnq, drv_LevelConc :1159-1171)."""
+    import copy
+    from clblm_amd._lib import FP64
+    from clblm_amd.jacobian import UpwellingJacobian
+    lines, layers, t_lev, p_lev, q_lev, dvs = _scene(nlay)
+    base = _jac_setup(ctx, FP64, nlay=nlay, mols=[mol] if mol else [1])
+    layers = [copy.deepcopy(l) for l in layers]
+    t_lev = t_lev.copy()
+    q_lev = {m: v.copy() for m, v in q_lev.items()}
+    mi = base.mols.index(mol) if mol else 0
+    for l in (k - 1, k):
+        if l < 0 or l >= nlay:
+            continue
+        upper = l == k - 1          # level k is the upper boundary of layer k - 1
+        if dt_lev:
+            d = (base.dt_dtup[l] if upper else base.dt_dtlo[l]) * dt_lev
+            layers[l].wtot = layers[l].wtot * (1.0 - d / layers[l].tave)
+            layers[l].tave = layers[l].tave + d
+        if q_fac is not None:
+            e = base.dw_dqup[mi, l] if upper else base.dw_dqlo[mi, l]
+            layers[l].w[mol] = layers[l].w[mol] * q_fac ** e
+    t_lev[k] += dt_lev
+    if q_fac is not None:
+        q_lev[mol][k] *= q_fac
+    h = ctx.upload_lines(lines)
+    jb = UpwellingJacobian(ctx, h, layers, t_lev, dvs, p_lev=p_lev, q_lev=q_lev, mols=[mol] if mol else [1],
+                           precision=FP64, continuum=lambda lay, a, b: W.continuum_absrb(lay, a, b))
+    r = jb.block(v1, v2)["rad"].cpu().numpy()
+    ctx.free_lines(h)
+    return r
+
+
+@pytest.mark.parametrize("k", [0, 3, 8])
+def test_jacobian_t_vs_pipeline_central_difference(ctx, k):
+    """dR/dT_lev (the reference's dtau/dT: a 1 K one-sided difference of the layer OD,
+    drv_LevelTemp) against the central difference (h = 0.5 K) of the whole OD + radiance
+    pipeline: within 0.15 of the largest value (measured: 0.09 at the surface level,
+    where the signal is smallest, 0.02 in the middle, 0.002 at the top).  The one-sided
+    difference's O(dT) curvature term bounds the agreement; a wrong level factor, sign
+    or level assignment gives errors of order 1.  Steps much below 1 K are dominated by
+    the OD's table indices stepping (each step a jump of a table interval) in the line
+    wings, in both differences."""
+    from clblm_amd._lib import FP64
+    jb = _jac_setup(ctx, FP64, nlay=8, mols=[1])
+    jac = jb.block(640.0, 646.0)["jac_t"][k].cpu().numpy()
+    h = 0.5
+    fd = (_perturbed_block(ctx, k, dt_lev=h) - _perturbed_block(ctx, k, dt_lev=-h)) / (2 * h)
+    err = np.abs(jac - fd).max() / np.abs(fd).max()
+    print("level %d: max |jac - fd| / max |fd| = %.2e" % (k, err))
+    assert err < 0.15
+
+
+@pytest.mark.parametrize("mol,k", [(2, 2), (3, 6), (1, 4)])
+def test_jacobian_lnq_vs_pipeline_central_difference(ctx, mol, k):
+    """dR/dln q_lev against the central difference (dln q = 1e-4) of the whole pipeline,
+    within 1e-2 of the largest value (measured: 4e-3 for CO2, 2e-5 for O3 and H2O).  The
+    Jacobian takes the OD of the molecule alone (drv_LevelConc) as its derivative; the
+    true OD is not exactly linear in the column (self-broadening, and the F4 wings'
+    1000-line buffers, which change when the other molecules' lines drop out)."""
+    from clblm_amd._lib import FP64
+    jb = _jac_setup(ctx, FP64, nlay=8, mols=[mol])
+    jac = jb.block(640.0, 646.0)["jac_lnq"][0][k].cpu().numpy()
+    e = 1e-4
+    fd = (_perturbed_block(ctx, k, q_fac=np.exp(e), mol=mol) -
+          _perturbed_block(ctx, k, q_fac=np.exp(-e), mol=mol)) / (2 * e)
+    err = np.abs(jac - fd).max() / np.abs(fd).max()
+    print("molecule %d level %d: max |jac - fd| / max |fd| = %.2e" % (mol, k, err))
+    assert err < 1e-2
