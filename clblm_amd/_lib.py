@@ -19,7 +19,8 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_od_graph_create", "clblm_od_graph_launch", "clblm_od_graph_check", "clblm_od_graph_destroy",
            "clblm_rt_jac_noalloc", "clblm_calculate_dv", "clblm_set_dv", "clblm_interp_spectrum",
            "clblm_interp_to_finest", "clblm_rt_up_grids", "clblm_mtckd_load", "clblm_mtckd_absrb",
-           "clblm_rt_mode1", "clblm_filter"]
+           "clblm_rt_mode1", "clblm_filter", "clblm_mtckd_read_nc", "clblm_mtckd_load_nc",
+           "clblm_dev_alloc", "clblm_dev_free", "clblm_dev_put", "clblm_dev_get"]
 GRID_EXACT, GRID_FIXED_RATIO, GRID_UNIFORM = -1, 0, 1
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_CAPACITY = 0, 1, 2, 3, 4
 OD_SPECIES_BROAD = 1
@@ -129,6 +130,9 @@ def load():
     lib.clblm_mtckd_absrb.argtypes = [P, ctypes.c_int, P, P, P, P, P, ctypes.c_int64, P, P]
     lib.clblm_rt_mode1.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, P, P, ctypes.POINTER(D),
                                    ctypes.POINTER(I64), P]
+    lib.clblm_mtckd_read_nc.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), P, P, P, P,
+                                        ctypes.POINTER(D), ctypes.POINTER(D)]
+    lib.clblm_mtckd_load_nc.argtypes = [P, ctypes.c_char_p, D, D]
     lib.clblm_filter.argtypes = [P, P, I64, D, D, ctypes.c_int, P, P, P, P, P, P, P]
     lib.clblm_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_double)]

@@ -3,7 +3,8 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("clblm_od.hip", "clblm_rt.hip", "clblm_scan.hip", "clblm_tape3.cpp", "clblm_setdv.cpp")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("clblm_od.hip", "clblm_rt.hip", "clblm_scan.hip", "clblm_tape3.cpp", "clblm_setdv.cpp",
+                                                        "clblm_netcdf3.cpp")]
 OUT = os.path.join(HERE, "libclblm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -34,13 +35,15 @@ FBIN = os.path.join(FDIR, "_bin")
 
 
 def build_fortran():
-    """The ISO_C_BINDING module and the Fortran test caller (skipped without flang)."""
+    """The ISO_C_BINDING module and the Fortran test callers (skipped without flang)."""
     if not os.path.exists(FLANG):
         return None
     os.makedirs(FBIN, exist_ok=True)
     steps = [
         [FLANG, "-O2", "-J", FBIN, "-c", os.path.join(FDIR, "clblm_hip.f90"), "-o", os.path.join(FBIN, "clblm_hip.o")],
         [FLANG, "-O2", "-J", FBIN, "-o", os.path.join(FBIN, "test_odlay_f"), os.path.join(FDIR, "test_odlay_f.f90"),
+         os.path.join(FBIN, "clblm_hip.o"), "-L" + HERE, "-lclblm_hip", "-Wl,-rpath," + HERE],
+        [FLANG, "-O2", "-J", FBIN, "-o", os.path.join(FBIN, "test_jac_f"), os.path.join(FDIR, "test_jac_f.f90"),
          os.path.join(FBIN, "clblm_hip.o"), "-L" + HERE, "-lclblm_hip", "-Wl,-rpath," + HERE],
     ]
     for cmd in steps:

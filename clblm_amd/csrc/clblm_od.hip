@@ -3912,6 +3912,38 @@ int clblm_mtckd_absrb(clblm_ctx *c, int nlay, const clblm_layer *layers, const d
     return 0;
 }
 
+int clblm_dev_alloc(clblm_ctx *c, int64_t bytes, void **d) {
+    if (!c || !d || bytes <= 0) return set_err(CLBLM_ERR_ARG, "null argument or no bytes");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMalloc(d, (size_t)bytes));
+    return 0;
+}
+
+int clblm_dev_free(clblm_ctx *c, void *d) {
+    if (!c) return set_err(CLBLM_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    drain(c, false);   // batches still reading it finish first
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(d));
+    return 0;
+}
+
+int clblm_dev_put(clblm_ctx *c, void *d, const void *h, int64_t bytes) {
+    if (!c || !d || !h || bytes < 0) return set_err(CLBLM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(d, h, (size_t)bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int clblm_dev_get(clblm_ctx *c, void *h, const void *d, int64_t bytes) {
+    if (!c || !d || !h || bytes < 0) return set_err(CLBLM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(h, d, (size_t)bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int clblm_filter(clblm_ctx *c, const double *d_in, int64_t n, double v1, double dv, int nchan, const double *v1f,
                  const double *dvf, const int32_t *nptf, const int64_t *xf_off, const double *xf, double *d_out,
                  void *stream) {

@@ -6,9 +6,15 @@
 ! drv_LevelTemp, drv_LevelConc) are unchanged.  ODLAY_struct needs the reference's
 ! Module_AtmPath and Module_DV, which come with its netCDF-dependent scene modules;
 ! -DCLBLM_HIP_NO_STRUCT leaves it out for builds without them.
-! The line list is uploaded once with clblm_hip_lines (the TAPE3 reader is not
-! part of this library).  Supported: molecules 1-7, no NLTE, no cross-sections,
-! no MT_CKD continuum (contnmOff must be set), no line coupling.
+! As the reference does (:276-302), the first call reads the line file named by
+! ioFiles%lineFile (a TAPE3 file, line coupling and species-broadening fields
+! included) unless a list was uploaded with clblm_hip_lines.  odCtrl%speciesBroad
+! selects the species-by-species broadening.  With the continuum on
+! (.not. odCtrl%contnmOff) the MT_CKD water-vapour continuum (self and foreign, XSELF
+! and XFRGN) is added from the reference's coefficient file, read on the first call;
+! the other continua of CONTNM (CO2, O3, O2, N2, Rayleigh) are not part of this path,
+! and a nonzero scale factor for them is reported once.  clblm_hip_precision selects
+! FP64 (default) or FP32C.  Not supported: NLTE, cross-sections, lineOff.
 module Module_ODLAY
    use iso_c_binding
    use clblm_hip
@@ -22,6 +28,9 @@ module Module_ODLAY
       module procedure ODLAY_nonStruct
    end interface
    logical, save :: firstPassJRAD = .true.
+   logical, save :: mtckdLoaded = .false., warnedContinua = .false.
+   ! the reference's MT_CKD coefficient file (mt_ckd_h2o_module.f90:16)
+   character(*), parameter :: mtckdFile = 'clblm_data/spectroscopy/absco-ref_wv-mt-ckd.nc'
 contains
 #ifndef CLBLM_HIP_NO_STRUCT
    ! Unpacks the layer, grid and DV records as the reference's wrapper does and
@@ -45,8 +54,8 @@ contains
       if (present(NLTE)) then
          if (NLTE) stop '--- ODLAY(): NLTE is not supported by the HIP path'
       end if
-      ! JacMolName only selects the continuum derivative (clblm_ODLAY.f90:438), and the
-      ! continuum is not part of this path
+      ! JacMolName only selects the continuum derivative of CONTNM's AJ branch
+      ! (clblm_ODLAY.f90:438), which this path does not compute
       L = aLayer%layNo
       dvo = spectGrid%DVout
       call ODLAY_nonStruct(aLayer%nMol, aLayer%molID, aLayer%Pave, aLayer%Tave, aLayer%W, aLayer%Wtot, &
@@ -63,7 +72,7 @@ contains
                               SAMPLE, ALFAL0, IPATH, LayNo, &
                               nSamp, monoOD, NLTE, nlteEmisFac, DVout, ajMolName)
       use Module_ConstParam, only: r8 => kind_r8, molNum
-      use Module_Config, only: CLBLM_OD_Ctrl
+      use Module_Config, only: CLBLM_OD_Ctrl, ioFiles
       integer, intent(in) :: nMol
       character(*), intent(in) :: molID(:)
       real, intent(in) :: Pav, Tav, W(:), Wtot, Zbot, Ztop
@@ -78,13 +87,29 @@ contains
       real, optional, intent(inout) :: DVout
       character(*), optional, intent(in) :: ajMolName
       integer :: mols(nMol), k
+      integer(c_int32_t) :: flags
       real(c_double) :: dvo, dvu
       real(c_double), allocatable :: od(:)
       if (present(NLTE)) then
          if (NLTE) stop '--- ODLAY(): NLTE is not supported by the HIP path'
       end if
-      if (.not. odCtrl%contnmOff) stop '--- ODLAY(): the MT_CKD continuum is not part of the HIP path'
       if (odCtrl%lineOff) stop '--- ODLAY(): lineOff is not supported by the HIP path'
+      if (.not. clblm_hip_have_lines()) call clblm_hip_lines_file(trim(ioFiles%lineFile))
+      flags = 0
+      if (odCtrl%speciesBroad) flags = ior(flags, CLBLM_OD_SPECIES_BROAD)
+      if (.not. odCtrl%contnmOff) then
+         if (.not. mtckdLoaded) then
+            call clblm_hip_mtckd(mtckdFile, real(odCtrl%XSELF, c_double), real(odCtrl%XFRGN, c_double))
+            mtckdLoaded = .true.
+         end if
+         if (.not. warnedContinua .and. (odCtrl%XCO2C > 0. .or. odCtrl%XO3CN > 0. .or. odCtrl%XO2CN > 0. .or. &
+                                         odCtrl%XN2CN > 0. .or. odCtrl%XRAYL > 0.)) then
+            write (*, '(a)') ' ODLAY (HIP): only the MT_CKD water-vapour continuum is added; the CO2, O3, O2, '// &
+                             'N2 and Rayleigh continua are not part of this path'
+            warnedContinua = .true.
+         end if
+         flags = ior(flags, CLBLM_OD_MTCKD)
+      end if
       do k = 1, nMol
          mols(k) = molNum(molID(k))
       end do
@@ -101,7 +126,7 @@ contains
       call clblm_hip_odlay_plain(mols, real(W(1:nMol), c_double), real(Pav, c_double), real(Tav, c_double), &
                                  real(Wtot, c_double), real(V1, c_double), real(V2, c_double), &
                                  real(DVnormal, c_double), real(DVnarrow, c_double), real(SAMPLE, c_double), &
-                                 dvo, CLBLM_FP64, nSamp, od, dvu)
+                                 dvo, clblm_hip_precision, nSamp, od, dvu, flags=flags)
       allocate (monoOD(nSamp))
       monoOD = od
       if (present(DVout)) DVout = dvu

@@ -147,9 +147,10 @@ class Context:
         return p
 
     def odlay(self, handle, layer, v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0,
-              precision=FP64, continuum=None, mtckd=False):
+              precision=FP64, continuum=None, mtckd=False, species_broad=False):
         lay = _layer_struct(layer)
-        par = self.params(v1, v2, dvnormal, dvnarrow, sample, dvout, precision, continuum, mtckd=mtckd)
+        par = self.params(v1, v2, dvnormal, dvnarrow, sample, dvout, precision, continuum,
+                          species_broad=species_broad, mtckd=mtckd)
         jr = ctypes.c_int32(self.jrad)
         n = ctypes.c_int64()
         dvu = ctypes.c_double()

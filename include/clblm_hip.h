@@ -393,6 +393,14 @@ int clblm_mtckd_load(clblm_ctx *ctx, int n, const double *wavenumber, const doub
                      const double *for_absco_ref, const double *self_texp, double ref_temp, double ref_press,
                      double xself, double xfrgn);
 
+/* The table read from the reference's absco-ref_wv-mt-ckd.nc (netCDF classic, CDF-1 or
+ * CDF-2), as read_module.f90's getData reads it (:50-101), with no netCDF library:
+ * clblm_mtckd_read_nc returns *n with wavenumber == NULL, then fills the four arrays of
+ * *n values and the two scalars; clblm_mtckd_load_nc reads and loads in one call. */
+int clblm_mtckd_read_nc(const char *path, int *n, double *wavenumber, double *self_absco_ref,
+                        double *for_absco_ref, double *self_texp, double *ref_temp, double *ref_press);
+int clblm_mtckd_load_nc(clblm_ctx *ctx, const char *path, double xself, double xfrgn);
+
 /* The same continuum alone, as CONTNM adds it to a zeroed ABSRB (clblm_ODLAY.f90:880-896):
  * for layer l, on V1ABS = INT(v1[l]) - 3 at 1 cm-1 (nptabs[l] points, returned), with the
  * radiation term unless jrad[l] is 0 (jrad: OPDPTH's JRAD after its flip,
@@ -472,6 +480,16 @@ int clblm_fft_scan(clblm_ctx *ctx, const double *d_in, int64_t n, double v1, dou
                    double v2out, double dvout, int funct_id, double hwhm, double fill, double pre_box_hw,
                    int deconv_pre_box, int precision, double *d_out, int64_t *nout, double *v1res, double *dvres,
                    void *stream);
+
+/* ---- device memory for callers without a HIP binding ------------------------------
+ * No counterpart in the reference: Fortran callers of the device-array entry points
+ * (clblm_od_batch, clblm_rt_jac, clblm_level_jac ...) get their device arrays here.
+ * clblm_dev_put / clblm_dev_get copy after the device is idle (synchronous);
+ * clblm_dev_free waits for the context's batches. */
+int clblm_dev_alloc(clblm_ctx *ctx, int64_t bytes, void **d);
+int clblm_dev_free(clblm_ctx *ctx, void *d);
+int clblm_dev_put(clblm_ctx *ctx, void *d, const void *h, int64_t bytes);
+int clblm_dev_get(clblm_ctx *ctx, void *h, const void *d, int64_t bytes);
 
 /* ---- timing -------------------------------------------------------------------
  * Average device time (ms) of each pipeline stage over the launches since the

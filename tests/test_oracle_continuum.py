@@ -92,3 +92,27 @@ def test_mtckd_magnitude():
     a = mtckd_h2o(t, 1013.0, 296.0, 1e22, 1e22, 999.0, 1001.0, jrad=1)
     c = a[4] / 1e22                                     # v = 1000, self only (x = 1)
     assert 0.5e-22 < c < 5e-22
+
+
+REF_NC = "/root/reference/clblm_data/spectroscopy/absco-ref_wv-mt-ckd.nc"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_NC), reason="the reference's data directory is absent")
+def test_library_reads_the_reference_netcdf_file():
+    """clblm_mtckd_read_nc (the library's own netCDF classic reader) returns the arrays
+    and scalars scipy reads from the reference's coefficient file, bit for bit."""
+    import ctypes
+    from clblm_amd import _lib
+    lib = _lib.load()
+    t = load_mtckd(REF_NC)
+    n = ctypes.c_int()
+    assert lib.clblm_mtckd_read_nc(REF_NC.encode(), ctypes.byref(n), None, None, None, None, None, None) == 0
+    arr = [np.zeros(n.value) for _ in range(4)]
+    rt, rp = ctypes.c_double(), ctypes.c_double()
+    assert lib.clblm_mtckd_read_nc(REF_NC.encode(), ctypes.byref(n), *[a.ctypes.data for a in arr],
+                                   ctypes.byref(rt), ctypes.byref(rp)) == 0
+    for a, k in zip(arr, ("wavenumbers", "self_absco_ref", "for_absco_ref", "self_texp")):
+        assert np.array_equal(a, t[k]), k
+    assert rt.value == t["ref_temp"] and rp.value == t["ref_press"]
+    assert lib.clblm_mtckd_read_nc(os.path.join(HERE, "golden", "mtckd_h2o.npz").encode(), ctypes.byref(n),
+                                   None, None, None, None, None, None) != 0
