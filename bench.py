@@ -235,6 +235,81 @@ def run_jacobian(ctx, h, layers, dvs, precision, block=100.0, compare=None):
     return t, spectra, npts, len(edges) - 1, first
 
 
+def _timed(fn, reps):
+    """Seconds per call of fn (one untimed call first), device work included."""
+    import torch
+    out = fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps, out
+
+
+def run_rows(ctx, h, layers, dvs, conts, prec, dt_step, reps=3):
+    """The components that take the layer ODs further, each timed on device-resident
+    inputs of the same workload: the OD step with the MT_CKD water-vapour continuum
+    (the reference's coefficient table) and the continuum alone; the upwelling radiance
+    merged on the layers' own grids (mode 2) and merged downward with the surface terms
+    (mode 1); on that radiance (the finest grid) the spectral-domain Gaussian scan, the
+    FFT sinc scan of 0.5 cm maximum OPD and 1000 SRF channels.  `hbm_gbps` is the
+    algorithmic bytes (each input point read once, each output point written once) over
+    the call's time."""
+    import torch
+    from clblm_amd import workloads as W
+    from clblm_amd.continuum import load_mtckd
+    from clblm_amd.jacobian import radiance_mode1, upwelling_radiance_grids
+    from clblm_amd.odlay import Context
+    from clblm_amd._lib import FP64
+    rows = {}
+    nlay = len(layers)
+    ctx.mtckd_load(load_mtckd(os.path.join(ROOT, "tests", "golden", "mtckd_h2o.npz")))
+    pm = [Context.params(V1, V2, dv, precision=prec, continuum=c, mtckd=True) for dv, c in zip(dvs, conts)]
+    out, off = ctx.od_batch(h, layers, pm)
+
+    def od_mt():
+        ctx.jrad = 1
+        return ctx.od_batch(h, layers, pm, out=out)
+    t, _ = _timed(od_mt, reps)
+    ta, ab = _timed(lambda: ctx.mtckd_absrb(layers, V1, V2, 1), reps)
+    npt_abs = sum(int(a.numel()) for a in ab)
+    rows["mtckd"] = {"ms_per_step_with_mtckd": t * 1e3, "ms_per_step_without": dt_step * 1e3,
+                     "ms_continuum_alone": ta * 1e3, "absrb_points": npt_abs}
+    ctx.jrad = 1
+    od, off = ctx.od_batch(h, layers, [Context.params(V1, V2, dv, precision=prec, continuum=c)
+                                       for dv, c in zip(dvs, conts)])
+    torch.cuda.synchronize()
+    t_lay = [l.tave for l in layers]
+    t_lev = W.level_temperatures(nlay, 0.1)
+    npts = int(off[-1])
+    t, (rad, dvr) = _timed(lambda: upwelling_radiance_grids(ctx, od, off, dvs, V1, t_lay, t_lev), reps)
+    rows["rt_mode2_layer_grids"] = {"ms": t * 1e3, "layer_points": npts, "output_points": int(rad.numel()),
+                                    "hbm_gbps": 8.0 * (npts + rad.numel()) / t / 1e9}
+    t, (rad1, _, _) = _timed(lambda: radiance_mode1(ctx, od, off, dvs, V1, V2, t_lay, t_lev, fac_dn=[1.67] * nlay),
+                             reps)
+    rows["rt_mode1_layer_grids"] = {"ms": t * 1e3, "layer_points": npts, "output_points": int(rad1.numel()),
+                                    "hbm_gbps": 8.0 * (npts + rad1.numel()) / t / 1e9}
+    a, b = V1 + 5.0, V2 - 5.0
+    n = int(rad.numel())
+    t, (sg, _, _) = _timed(lambda: ctx.scanfn(rad, V1, dvr, a, b, 0.0, 3, 0.25), reps)
+    rows["scanfn_gaussian"] = {"ms": t * 1e3, "hwhm": 0.25, "input_points": n, "output_points": int(sg.numel()),
+                               "hbm_gbps": 8.0 * (n + sg.numel()) / t / 1e9}
+    t, (ss, _, _) = _timed(lambda: ctx.fft_scan(rad, V1, dvr, a, b, 0.0, 5, 0.6034, precision=prec), reps)
+    rows["fft_scan_sinc"] = {"ms": t * 1e3, "hwhm": 0.6034, "input_points": n, "output_points": int(ss.numel()),
+                             "precision": "fp64" if prec == FP64 else "fp32c"}
+    rng = np.random.default_rng(5)
+    chans = []
+    for c in np.linspace(V1 + 10.0, V2 - 10.0, 1000):
+        npf = 201
+        chans.append((float(c), -0.01, np.exp(-np.linspace(-3.0, 3.0, npf) ** 2) + 0.01 * rng.uniform(size=npf)))
+    t, fv = _timed(lambda: ctx.filter(rad, V1, dvr, chans), reps)
+    rows["srf_filter"] = {"ms": t * 1e3, "channels": len(chans), "channel_width_cm-1": 2.0,
+                          "channels_per_s": len(chans) / t}
+    ctx.sync()
+    return rows
+
+
 def rel_err(a, b):
     """max |a - b| / max |b| (the FP64 reference b)."""
     return float(np.abs(a - b).max() / np.abs(b).max())
@@ -252,6 +327,21 @@ def measured_traffic(kernel):
     return None
 
 
+def measured_counters(kernel):
+    """SQ counter figures per launch of `kernel` from the counter passes summarised under
+    profiles/ (lines `counter <kernel> <name>: <value>`), or None."""
+    d = os.path.join(ROOT, "profiles")
+    out = {}
+    for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
+        if not name.endswith("_sq_counters.md"):
+            continue
+        for ln in open(os.path.join(d, name)):
+            if ln.startswith("counter %s " % kernel):
+                key, val = ln[len("counter %s " % kernel):].split(":")
+                out[key.strip()] = float(val)
+    return out or None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +352,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison run")
     ap.add_argument("--no-jac", action="store_true", help="skip the Jacobian pass")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay timing")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the continuum, RT-mode, scan and filter timings")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shapes")
     ap.add_argument("--serial", action="store_true",
                     help="run every kernel of a batch on one stream, one after another (for a kernel "
@@ -326,6 +418,7 @@ def main():
         "roofline": {"bound": "valu", "kernel": dom, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
                      "frac": ach / peak,
                      "traffic": measured_traffic(dom + ("64" if prec == FP64 else "32")),
+                     "counters": measured_counters(dom + ("64" if prec == FP64 else "32")),
                      "flop_per_unit": {"k_conv": FLOP_CONV, "k_f4conv": FLOP_F4},
                      "units_per_launch": {"k_conv": w["conv_evals"], "k_f4conv": w["f4_evals"]},
                      "ms": {"conv_kernel": w["ms_conv"], "f4_kernel": w["ms_f4"], "prep": tm["prep"],
@@ -346,6 +439,8 @@ def main():
                            "fp32c_od_err": {"max_rel_per_point": float(pt.max()),
                                             "max_abs_over_max": rel_err(od_eager, od64),
                                             "bar": 1e-5}}
+    if not args.no_rows:
+        res["rows"] = run_rows(ctx, h, layers, dvs, conts, prec, dt)
     if not args.no_jac:
         t, spectra, npts, nblk, first = run_jacobian(ctx, h, layers, dvs, prec)
         res["jacobian"] = {"spectra_per_s": spectra / t, "points_per_s": spectra * npts / t, "seconds": t,

@@ -2173,13 +2173,19 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
 // interpSpectrum_array2array (clblm_Spectrum.f90:326-590).  For the n/(n+1) ratios
 // every output point is independent (one lane per point).  The general case walks
 // blocks of 2400 output points with a sequentially accumulated fraction P, whose
-// rounding the reference's quantised coefficient index depends on; one lane walks
-// one block.
-__global__ void k_regrid(const Job *jobs, const int *lay_of_unit, const int64_t *unit_first,
-                         const double *nat, double *out, int64_t nunits) {
+// rounding the reference's quantised coefficient index depends on; a lane takes a
+// piece of a block and replays P from the block's start.  unit_first[l]: the first
+// work unit of layer l (nondecreasing; layers without re-grid have none).
+__global__ void k_regrid(const Job *jobs, int nlay, const int64_t *unit_first, const double *nat, double *out,
+                         int64_t nunits) {
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunits) return;
-    int l = lay_of_unit[u];
+    int l = 0, hi = nlay - 1;          // the last layer whose units start at or before u
+    while (l < hi) {
+        const int mid = (l + hi + 1) / 2;
+        if (unit_first[mid] <= u) l = mid;
+        else hi = mid - 1;
+    }
     const Job &J = jobs[l];
     clblm_interp::unit(nat + J.nat_off - 1, J.nnat, J.v1, J.dv, J.dvout, out + J.out_off - 1, J.npts,
                        u - unit_first[l]);
@@ -3005,27 +3011,22 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_xjj = cv.take<double>((size_t)nlay * XJJ_N);
     size_t o_rec = precision == CLBLM_FP64 ? cv.take<F4Rec64>((size_t)B.tot_f4rec) : cv.take<F4Rec32>((size_t)B.tot_f4rec);
     size_t o_nat = cv.take<double>(std::max<int64_t>(1, B.tot_nat));
-    size_t o_rg = cv.take<int>(1);
     size_t o_rgf = cv.take<int64_t>(nlay);
-    // re-grid work units: points for the n/(n+1) ratios, 2400-point blocks otherwise
-    std::vector<int> rg_lay;
+    // re-grid work units: points for the n/(n+1) ratios, pieces of 2400-point blocks
+    // otherwise; rg_first[l] is layer l's first unit
     std::vector<int64_t> rg_first(nlay, 0);
+    int64_t rg_units = 0;
     if (B.any_regrid) {
         for (int l = 0; l < nlay; l++) {
-            rg_first[l] = (int64_t)rg_lay.size();
+            rg_first[l] = rg_units;
             const Job &Jl = B.jobs[l];
-            if (!(Jl.dvout > 0.)) continue;
-            int64_t units = clblm_interp::units(Jl.dv, Jl.dvout, Jl.npts);
-            for (int64_t u = 0; u < units; u++) rg_lay.push_back(l);
+            if (Jl.dvout > 0.) rg_units += clblm_interp::units(Jl.dv, Jl.dvout, Jl.npts);
         }
-        o_rg = cv.take<int>(std::max<size_t>(1, rg_lay.size()));
-        o_rgf = cv.take<int64_t>(nlay);
     }
     // host-built tables, staged in pinned memory: the header region [0, o_lp_d) as it
     // lies in the arena, then the per-layer stop indices and the re-grid units
     const size_t h_stop = (o_lp_d + 255) & ~(size_t)255;
-    const size_t h_rg = (h_stop + sizeof(int) * nlay + 255) & ~(size_t)255;
-    const size_t h_rgf = (h_rg + sizeof(int) * rg_lay.size() + 255) & ~(size_t)255;
+    const size_t h_rgf = (h_stop + sizeof(int) * nlay + 255) & ~(size_t)255;
     const size_t h_size = h_rgf + sizeof(int64_t) * nlay;
     // where this batch lives: the context's arena, staging and slot ring, or (under
     // graph capture) the graph's own, with no stage events
@@ -3090,7 +3091,6 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         std::memcpy(hp + o_mtc, B.mtcoef.data(), sizeof(double) * B.mtcoef.size());
     }
     std::memcpy(hp + h_stop, B.stop0.data(), sizeof(int) * nlay);
-    if (!rg_lay.empty()) std::memcpy(hp + h_rg, rg_lay.data(), sizeof(int) * rg_lay.size());
     std::memcpy(hp + h_rgf, rg_first.data(), sizeof(int64_t) * nlay);
     Job *d_jobs = at<Job>(base, o_jobs);
     Panel *d_pan = at<Panel>(base, o_pan);
@@ -3244,12 +3244,11 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     hipLaunchKernelGGL(k_pass2<double>, dim3((unsigned)npan), dim3(256), 0, st, d_jobs, d_pan, d_blk, d_A, d_R4,
                        d_abs, d_out, d_nat);
     if (B.any_regrid) {
-        int *d_rg = at<int>(base, o_rg);
         int64_t *d_rgf = at<int64_t>(base, o_rgf);
-        HIPCHK(hipMemcpyAsync(d_rg, hp + h_rg, sizeof(int) * rg_lay.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_rgf, hp + h_rgf, sizeof(int64_t) * nlay, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_regrid, dim3(cdiv((int64_t)rg_lay.size(), 64)), dim3(64), 0, st, d_jobs, d_rg, d_rgf,
-                           d_nat, d_out, (int64_t)rg_lay.size());
+        if (rg_units > 0)
+            hipLaunchKernelGGL(k_regrid, dim3((unsigned)cdiv(rg_units, 64)), dim3(64), 0, st, d_jobs, nlay, d_rgf,
+                               d_nat, d_out, rg_units);
     }
     HIPCHK(stage_event(4, st));
     if (G) {

@@ -6,7 +6,10 @@
 //     one output point per work unit;
 //   * any other ratio: the general form, whose fraction P is accumulated point by
 //     point in blocks of 2400 output points and quantised to 1/200 for the
-//     coefficient index, so one work unit walks one block sequentially.
+//     coefficient index.  A block is cut into SUB pieces of SUBLEN points, one work
+//     unit each; the unit of a piece replays the recurrence of P (the same additions
+//     in the same order, no memory traffic) from the block's start to its first point,
+//     so every point gets the reference's P bit for bit.
 // Both the OD re-grid of ODLAY (DVout) and interpToFinestGrid use it.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -43,12 +46,16 @@ __host__ __device__ inline bool same_grid(double v1, double dva, int64_t na, dou
            fabs((v1 + dva * (double)(na - 1)) - (v1 + dvb * (double)(nb - 1))) < EPS;
 }
 
-// Work units of one interpolation: one per output point for the aligned forms, one
+constexpr int64_t BLOCK = 2400;
+constexpr int64_t SUB = 64;
+constexpr int64_t SUBLEN = (BLOCK + SUB - 1) / SUB;
+
+// Work units of one interpolation: one per output point for the aligned forms, SUB
 // per 2400-point block for the general form.
 __host__ __device__ inline int64_t units(double DVin, double DVout, int64_t NOUT) {
     double r;
     int a, b;
-    return regrid_type(DVin, DVout, &r, &a, &b) >= 0 ? NOUT : (NOUT + 2399) / 2400;
+    return regrid_type(DVin, DVout, &r, &a, &b) >= 0 ? NOUT : (NOUT + BLOCK - 1) / BLOCK * SUB;
 }
 
 // Work unit ul (0-based) of the interpolation of X[1..NIN] (1-based pointer) into
@@ -88,13 +95,29 @@ __device__ inline void unit(const double *X, int64_t NIN, double V1, double DVin
         }
     } else {
         const double XNUMCF = 201.;
-        int64_t J1 = ul * 2400 + 1;
-        double Vo = ul == 0 ? V1 : V1 + (double)(J1 - 1) * DVout;
+        const int64_t blk = ul / SUB;
+        int64_t J1 = blk * BLOCK + 1;
+        int64_t J2 = J1 + BLOCK - 1 < NOUT ? J1 + BLOCK - 1 : NOUT;
+        const int64_t Js = J1 + (ul % SUB) * SUBLEN;
+        if (Js > J2) return;
+        const int64_t Je = Js + SUBLEN - 1 < J2 ? Js + SUBLEN - 1 : J2;
+        double Vo = blk == 0 ? V1 : V1 + (double)(J1 - 1) * DVout;
         int64_t iIn = (int64_t)((Vo - V1) / DVin + 1.00001);
         double Vi = V1 + (double)(iIn - 1) * DVin;
         double P = (Vo - Vi) / DVin;
-        int64_t J2 = J1 + 2400 - 1 < NOUT ? J1 + 2400 - 1 : NOUT;
-        for (int64_t j = J1; j <= J2; j++) {
+        // the block's recurrence up to this piece; for P >= 1, (double)(int64_t)P is
+        // floor(P), so the whole steps are counted in a double (exact below 2^53)
+        double adv = 0.;
+        for (int64_t j = J1; j < Js; j++) {
+            P = P + dvRatio;
+            if (P >= 1.0) {
+                const double f = floor(P);
+                adv = adv + f;
+                P = P - f;
+            }
+        }
+        iIn = iIn + (int64_t)adv;
+        for (int64_t j = Js; j <= Je; j++) {
             int ip = (int)(P * XNUMCF + 1.00001);
             double p = ((double)ip - 1.0) / (XNUMCF - 1.0);
             double pp = p * p;
