@@ -20,11 +20,13 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_rt_jac_noalloc", "clblm_calculate_dv", "clblm_set_dv", "clblm_interp_spectrum",
            "clblm_interp_to_finest", "clblm_rt_up_grids", "clblm_mtckd_load", "clblm_mtckd_absrb",
            "clblm_rt_mode1", "clblm_filter", "clblm_mtckd_read_nc", "clblm_mtckd_load_nc",
-           "clblm_dev_alloc", "clblm_dev_free", "clblm_dev_put", "clblm_dev_get"]
+           "clblm_dev_alloc", "clblm_dev_free", "clblm_dev_put", "clblm_dev_get", "clblm_contnm_scales",
+           "clblm_contnm_absrb"]
 GRID_EXACT, GRID_FIXED_RATIO, GRID_UNIFORM = -1, 0, 1
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_CAPACITY = 0, 1, 2, 3, 4
 OD_SPECIES_BROAD = 1
 OD_MTCKD = 2
+OD_CONTNM = 4
 TUNE_F4_XCD_CHUNK = 1
 TUNE_F4_FAR_FIELD = 2
 TUNE_F4_STREAM = 3
@@ -128,6 +130,8 @@ def load():
          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), P]
     lib.clblm_mtckd_load.argtypes = [P, ctypes.c_int, P, P, P, P] + [ctypes.c_double] * 4
     lib.clblm_mtckd_absrb.argtypes = [P, ctypes.c_int, P, P, P, P, P, ctypes.c_int64, P, P]
+    lib.clblm_contnm_scales.argtypes = [P, D, D, D, D, D]
+    lib.clblm_contnm_absrb.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_int32, P, ctypes.c_int64, P, P]
     lib.clblm_rt_mode1.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, P, P, ctypes.POINTER(D),
                                    ctypes.POINTER(I64), P]
     lib.clblm_mtckd_read_nc.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), P, P, P, P,

@@ -2340,7 +2340,8 @@ struct LineListH {
 struct MtckdTable {
     std::vector<double> wvn, self, forr, texp;
     double ref_t = 0., ref_p = 0., xself = 1., xfrgn = 1.;
-    int64_t gen = 0;             // bumped by every load
+    double cn_scale[5] = {1., 1., 1., 1., 1.};   // XCO2C, XO3CN, XO2CN, XN2CN, XRAYL
+    int64_t gen = 0;             // bumped by every load and every change of the factors
 };
 
 struct MtJob {
@@ -2463,6 +2464,217 @@ __global__ void k_mtckd(const MtJob *jobs, const double *coef, double *absrb) {
     *out = *out + cself + cfor;
 }
 
+// ---- CONTNM's other continua of the thermal infrared -------------------------------
+// (clblm_ODLAY_Continuum.f90:408-1060).  Each tabulated band (CO2 with FRNCO2 :1333-1400
+// and the XFACCO2 scaling :415-457, the O2 fundamental o2_ver_1 :7187-7255, the N2 pure
+// rotation xn2_r :2533-2600 and fundamental n2_ver_1 :2701-2790) is the window of its
+// table around [V1ABS, V2ABS] that the reference's routine builds, scaled to the layer
+// as CONTNM scales it; the windows are built on the host (the C library's pow and exp,
+// as the reference's build evaluates them, and RADFN where JRAD is 1).  The device adds
+// the bands in CONTNM's order into ABSRB, each by XINT (clblm_Spectrum.f90:874-914),
+// then the Rayleigh extinction of every point (:970-1060, no aerosols).
+#include "clblm_contnm.inc"
+
+struct CnTerm {
+    double v1c, v2c, dvc;
+    int64_t coef;      // its window (NPTC values, then one zero) in the batch's array
+    int32_t nptc, pad;
+};
+
+struct CnJob {
+    int64_t abs_off;
+    double v1abs, ray_fac;                  // ray_fac: WTOT CONV_CM2MOL
+    int32_t nptabs, term0, nterm, jrad, ray, pad;
+};
+
+// The table window of CONTNM's band routines around [V1ABS, V2ABS] (the same lines in
+// FRNCO2, xn2_r, n2_ver_1 and o2_ver_1); returns NPTC.
+static int cn_window(double V1ABS, double V2ABS, const double *grid, double *V1C, double *V2C, double *DVC,
+                     int *I1) {
+    const double V1S = grid[0], DVS = grid[2];
+    const int NPTS = (int)grid[3];
+    *DVC = DVS;
+    double v1c = V1ABS - *DVC, v2c = V2ABS + *DVC;
+    const int i1 = v1c < V1S ? -1 : (int)((v1c - V1S) / DVS + 0.01);
+    v1c = V1S + DVS * (double)(i1 - 1);
+    const int i2 = (int)((v2c - V1S) / DVS + 0.01);
+    int nptc = i2 - i1 + 3;
+    if (nptc > NPTS) nptc = NPTS + 4;
+    *V1C = v1c;
+    *V2C = v1c + DVS * (double)(nptc - 1);
+    *I1 = i1;
+    return nptc;
+}
+
+// One layer's bands.  CLBLM_ERR_UNSUPPORTED where a band CONTNM would add beyond these
+// is reached with its factor above 0.
+static int cn_plan(const double *scale, const clblm_layer &lay, double V1, double V2, int jrad, CnJob &cj,
+                   std::vector<CnTerm> &terms, std::vector<double> &coef, bool *has) {
+    *has = false;
+    const double xco2c = scale[0], xo3cn = scale[1], xo2cn = scale[2], xn2cn = scale[3], xrayl = scale[4];
+    const double v1 = V1, v2 = V2;
+    if (xo3cn > 0. && ((v2 > 8920.0 && v1 <= 24665.0) || (v2 > 27370. && v1 < 40800.) || (v2 > 40800. && v1 < 54000.)))
+        return set_err(CLBLM_ERR_UNSUPPORTED, "continuum: the O3 bands (from 8920 cm-1) are not part of this path");
+    if (xo2cn > 0. && ((v2 > 7536.0 && v1 < 8500.) || (v2 > 9100.0 && v1 < 11000.) ||
+                       (v2 > 12961.5 && v1 < 13221.5) || (v2 > 15000.0 && v1 < 29870.) || v2 > 36000.0))
+        return set_err(CLBLM_ERR_UNSUPPORTED, "continuum: the O2 bands from 7536 cm-1 are not part of this path");
+    if (xn2cn > 0. && (v2 > 4340.0) && (v1 < 4910.))
+        return set_err(CLBLM_ERR_UNSUPPORTED, "continuum: the N2 overtone (4340-4910 cm-1) is not part of this path");
+    double V1ABS;
+    int NPTABS;
+    absrb_grid(V1, V2, &V1ABS, &NPTABS);
+    const double V2ABS = V1ABS + (double)(NPTABS - 1);
+    const double P0C = 1013., T0C = 296., XLOSMT = 2.68675E+19;
+    const double pave = lay.pave, tave = lay.tave, wtot = lay.wtot;
+    const double RHOAVE = (pave / P0C) * (T0C / tave);
+    const double XKT = tave / RADCN2;
+    const double amagat = (pave / P0C) * (273. / tave);
+    const double x_vmr_h2o = lay.w[1] / wtot, x_vmr_o2 = lay.w[7] / wtot;
+    const double x_vmr_n2 = 1. - x_vmr_h2o - x_vmr_o2;
+    const double wn2 = x_vmr_n2 * wtot;
+    cj.term0 = (int32_t)terms.size();
+    cj.nterm = 0;
+    auto begin = [&](const double *grid, double *V1C, double *DVC, int *I1) {
+        CnTerm t{};
+        double V2C;
+        t.nptc = cn_window(V1ABS, V2ABS, grid, V1C, &V2C, DVC, I1);
+        t.v1c = *V1C;
+        t.v2c = V2C;
+        t.dvc = *DVC;
+        t.coef = (int64_t)coef.size();
+        return t;
+    };
+    auto finish = [&](CnTerm &t) {
+        coef.push_back(0.);
+        terms.push_back(t);
+        cj.nterm++;
+    };
+    double V1C, DVC;
+    int I1;
+    if ((v2 > -20.0) && (v1 < 10000.) && xco2c > 0) {   // CO2 (:415-457)
+        const double WCO2 = lay.w[2] * RHOAVE * 1.0E-20 * xco2c;
+        CnTerm t = begin(CN_CO2_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_CO2_GRID[3];
+        const double trat = tave / CN_CO2_TEFF[0];
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            double f = 0.;
+            if (I >= 1 && I <= NPTS) {
+                double tcor = 1.;
+                if (I >= 1196 && I <= 1220) tcor = std::pow(trat, CN_CO2_TDEP[I - 1196]);
+                f = tcor * CN_CO2_S[I - 1];
+            }
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double CFAC = 1.;
+            if (VJ >= 2000. && VJ <= 2998.) CFAC = CN_XFACCO2[(int)((VJ - 1998.) / 2. + 0.00001) - 1];
+            f = CFAC * f;
+            double c = f * WCO2;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    if ((v2 > 1340.0) && (v1 < 1850.) && xo2cn > 0.) {   // O2 fundamental (:587-628)
+        const double tau_fac = xo2cn * lay.w[7] * 1.e-20 * amagat;
+        CnTerm t = begin(CN_O2F_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_O2F_GRID[3];
+        const double xktfac = (1. / 296.) - (1. / tave);
+        const double factor = (1.e+20 / 2.68675e+19);
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double c0 = 0.;
+            if (I >= 1 && I <= NPTS) c0 = factor * CN_O2F_X[I - 1] * std::exp(CN_O2F_XT[I - 1] * xktfac) / VJ;
+            double c = tau_fac * c0;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    if ((v2 > -10.0) && (v1 < 350.) && xn2cn > 0.) {   // N2 pure rotation (:807-852)
+        const double a_h2o = 1.;
+        const double tau_fac = xn2cn * (wn2 / XLOSMT) * amagat;
+        CnTerm t = begin(CN_N2R_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_N2R_GRID[3];
+        const double tfac = (tave - 296.) / (220. - 296.);
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double c0 = 0., c1 = 0.;
+            if (I >= 1 && I <= NPTS) {
+                c0 = CN_N2R_C296[I - 1] * std::pow(CN_N2R_C220[I - 1] / CN_N2R_C296[I - 1], tfac);
+                const double sf_T = CN_N2R_SF296[I - 1] * std::pow(CN_N2R_SF220[I - 1] / CN_N2R_SF296[I - 1], tfac);
+                c1 = (sf_T - 1.) * (0.79) / (0.21);
+            }
+            double c = tau_fac * c0 * (x_vmr_n2 + c1 * x_vmr_o2 + a_h2o * x_vmr_h2o);
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    if ((v2 > 2001.77) && (v1 < 2897.59) && xn2cn > 0.) {   // N2 fundamental (:856-909)
+        const double tau_fac = xn2cn * (wn2 / XLOSMT) * amagat;
+        CnTerm t = begin(CN_N2F_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_N2F_GRID[3];
+        const double T = tave;
+        const double xtfac = ((1. / T) - (1. / 272.)) / ((1. / 228.) - (1. / 272.));
+        const double xt_lin = (T - 272.) / (228. - 272.);
+        const double a_o2 = 1.294 - 0.4545 * T / 296.;
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double c0 = 0., c1 = 0., c2 = 0.;
+            if (I >= 1 && I <= NPTS) {
+                const double x272 = CN_N2F_X272[I - 1], x228 = CN_N2F_X228[I - 1];
+                if (x272 > 0. && x228 > 0.) c0 = x272 * std::pow(x228 / x272, xtfac);
+                else c0 = x272 + (x228 - x272) * xt_lin;
+                c0 = c0 / VJ;
+                c1 = a_o2 * c0;
+                c2 = (9. / 7.) * CN_N2F_AH2O[I - 1] * c0;
+            }
+            double c = tau_fac * (x_vmr_n2 * c0 + x_vmr_o2 * c1 + x_vmr_h2o * c2);
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    cj.ray = (v2 >= 820. && xrayl > 0.) ? 1 : 0;   // Rayleigh (:970-1060)
+    cj.ray_fac = wtot * (xrayl * 1.E-20 / (2.68675e-1 * 1.e5));
+    cj.v1abs = V1ABS;
+    cj.nptabs = NPTABS;
+    cj.jrad = jrad;
+    *has = cj.nterm > 0 || cj.ray;
+    return 0;
+}
+
+// CONTNM's bands at ABSRB point I (1-based) of job blockIdx.y, added in order: each
+// band by XINT where I lies in its range, then the Rayleigh extinction.
+__global__ void k_contnm(const CnJob *jobs, const CnTerm *terms, const double *coef, double *absrb) {
+    const CnJob J = jobs[blockIdx.y];
+    const int I = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (I > J.nptabs) return;
+    double *out = absrb + J.abs_off + (I - 1);
+    double a = *out;
+    for (int t = 0; t < J.nterm; t++) {
+        const CnTerm T = terms[J.term0 + t];
+        int ilo = (int)((T.v1c + T.dvc - J.v1abs) / 1. + 1. + ONEMI);
+        if (ilo < 1) ilo = 1;
+        int ihi = (int)((T.v2c - T.dvc - J.v1abs) / 1. + ONEMI);
+        if (ihi > J.nptabs) ihi = J.nptabs;
+        if (I >= ilo && I <= ihi)
+            a = a + xint_point(T.v1c, T.v2c, T.dvc, coef + T.coef - 1, J.v1abs, 1., I, 1, J.nptabs);
+    }
+    if (J.ray) {
+        const double vrayleigh = J.v1abs + (double)(I - 1) * 1.;
+        const double x = vrayleigh / 1.e4;
+        double r = (x * x * x / (9.38076E2 - 10.8426 * (x * x))) * J.ray_fac;
+        if (J.jrad == 1) r = r * x;
+        else if (J.jrad == 0) r = r / 1.e4;
+        a = a + r;
+    }
+    *out = a;
+}
+
 struct Batch {
     std::vector<Job> jobs;
     std::vector<Panel> panels;
@@ -2472,6 +2684,10 @@ struct Batch {
     std::vector<MtJob> mtjobs;   // MT_CKD water-vapour continuum added to absrb on the device
     std::vector<double> mtcoef;
     int64_t mt_maxpts = 0;
+    std::vector<CnJob> cnjobs;   // CONTNM's other bands, added after the water vapour
+    std::vector<CnTerm> cnterms;
+    std::vector<double> cncoef;
+    int64_t cn_maxpts = 0;
     std::vector<int> stop0;
     std::vector<int> regrid_lay;
     std::vector<int64_t> regrid_first;
@@ -2691,6 +2907,10 @@ struct Plan {
     bool mt = false;             // MT_CKD water-vapour continuum: its job and windows
     MtJob mj{};
     std::vector<double> mtc;
+    bool cn = false;             // CONTNM's other bands: the job, its bands and windows
+    CnJob cj{};
+    std::vector<CnTerm> cnt;
+    std::vector<double> cnc;
 };
 
 static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od_params &par,
@@ -2852,7 +3072,8 @@ static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od
     // continuum on the OPDPTH ABSRB grid (clblm_ODLAY.f90:881-896): the caller's
     // array, the MT_CKD water-vapour continuum, or both (the caller's first)
     const bool mtckd = (par.flags & CLBLM_OD_MTCKD) != 0;
-    J.contnm = (par.contnm || mtckd) ? 1 : 0;
+    const bool cnband = (par.flags & CLBLM_OD_CONTNM) != 0;
+    J.contnm = (par.contnm || mtckd || cnband) ? 1 : 0;
     J.species_broad = (par.flags & CLBLM_OD_SPECIES_BROAD) ? 1 : 0;
     if (J.contnm) {
         double v1abs;
@@ -2871,6 +3092,11 @@ static int plan_layer(const LineListH &L, const clblm_layer &lay, const clblm_od
         if (mtckd) {
             if (!mt) return set_err(CLBLM_ERR_ARG, "MT_CKD continuum asked for but no table loaded");
             int rc = mtckd_plan(*mt, lay, V1, V2, J.jrad, pl.mj, pl.mtc, &pl.mt);
+            if (rc) return rc;
+        }
+        if (cnband) {
+            static const double ones[5] = {1., 1., 1., 1., 1.};
+            int rc = cn_plan(mt ? mt->cn_scale : ones, lay, V1, V2, J.jrad, pl.cj, pl.cnt, pl.cnc, &pl.cn);
             if (rc) return rc;
         }
     }
@@ -2908,6 +3134,19 @@ static int build_batch(const LineListH &L, int nlay, const clblm_layer *layers, 
             B.mtcoef.insert(B.mtcoef.end(), pl.mtc.begin(), pl.mtc.end());
             B.mtjobs.push_back(mj);
             B.mt_maxpts = std::max<int64_t>(B.mt_maxpts, mj.ihi - mj.ilo + 1);
+        }
+        if (pl.cn) {
+            CnJob cj = pl.cj;
+            cj.abs_off = J.absrb_off;
+            cj.term0 = (int32_t)B.cnterms.size();
+            const int64_t c0 = (int64_t)B.cncoef.size();
+            for (CnTerm t : pl.cnt) {
+                t.coef += c0;
+                B.cnterms.push_back(t);
+            }
+            B.cncoef.insert(B.cncoef.end(), pl.cnc.begin(), pl.cnc.end());
+            B.cnjobs.push_back(cj);
+            B.cn_maxpts = std::max<int64_t>(B.cn_maxpts, cj.nptabs);
         }
         J.nat_off = B.tot_nat;
         if (J.dvout > 0.) {
@@ -2989,6 +3228,9 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     size_t o_abs = cv.take<double>(std::max<size_t>(1, B.absrb.size()));
     size_t o_mtj = cv.take<MtJob>(std::max<size_t>(1, B.mtjobs.size()));
     size_t o_mtc = cv.take<double>(std::max<size_t>(1, B.mtcoef.size()));
+    size_t o_cnj = cv.take<CnJob>(std::max<size_t>(1, B.cnjobs.size()));
+    size_t o_cnt = cv.take<CnTerm>(std::max<size_t>(1, B.cnterms.size()));
+    size_t o_cnc = cv.take<double>(std::max<size_t>(1, B.cncoef.size()));
     size_t o_lp_d = cv.take<double>(5 * (size_t)B.tot_line);
     size_t o_lp_i = cv.take<int>(3 * (size_t)B.tot_line);
     size_t o_A = cv.take<double>((size_t)B.tot_a);
@@ -3090,6 +3332,11 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         std::memcpy(hp + o_mtj, B.mtjobs.data(), sizeof(MtJob) * B.mtjobs.size());
         std::memcpy(hp + o_mtc, B.mtcoef.data(), sizeof(double) * B.mtcoef.size());
     }
+    if (!B.cnjobs.empty()) {
+        std::memcpy(hp + o_cnj, B.cnjobs.data(), sizeof(CnJob) * B.cnjobs.size());
+        if (!B.cnterms.empty()) std::memcpy(hp + o_cnt, B.cnterms.data(), sizeof(CnTerm) * B.cnterms.size());
+        if (!B.cncoef.empty()) std::memcpy(hp + o_cnc, B.cncoef.data(), sizeof(double) * B.cncoef.size());
+    }
     std::memcpy(hp + h_stop, B.stop0.data(), sizeof(int) * nlay);
     std::memcpy(hp + h_rgf, rg_first.data(), sizeof(int64_t) * nlay);
     Job *d_jobs = at<Job>(base, o_jobs);
@@ -3102,6 +3349,9 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     if (!B.mtjobs.empty())   // the water-vapour continuum joins ABSRB before pass 2 reads it
         hipLaunchKernelGGL(k_mtckd, dim3(cdiv(B.mt_maxpts, 256), (unsigned)B.mtjobs.size()), dim3(256), 0, st,
                            at<MtJob>(base, o_mtj), at<double>(base, o_mtc), d_abs);
+    if (!B.cnjobs.empty())   // then CONTNM's other bands, in the reference's order
+        hipLaunchKernelGGL(k_contnm, dim3(cdiv(B.cn_maxpts, 256), (unsigned)B.cnjobs.size()), dim3(256), 0, st,
+                           at<CnJob>(base, o_cnj), at<CnTerm>(base, o_cnt), at<double>(base, o_cnc), d_abs);
     LinePrep lp;
     double *lpd = at<double>(base, o_lp_d);
     int *lpi = at<int>(base, o_lp_i);
@@ -3859,35 +4109,69 @@ int clblm_mtckd_load(clblm_ctx *c, int n, const double *wavenumber, const double
     return 0;
 }
 
-int clblm_mtckd_absrb(clblm_ctx *c, int nlay, const clblm_layer *layers, const double *v1, const double *v2,
-                      const int32_t *jrad, double *d_absrb, int64_t ld, int32_t *nptabs, void *stream) {
+int clblm_contnm_scales(clblm_ctx *c, double xco2c, double xo3cn, double xo2cn, double xn2cn, double xrayl) {
+    if (!c) return set_err(CLBLM_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    const double v[5] = {xco2c, xo3cn, xo2cn, xn2cn, xrayl};
+    for (int k = 0; k < 5; k++) {
+        if (!(v[k] >= 0.)) return set_err(CLBLM_ERR_ARG, "continuum scale factors must be >= 0");
+        c->mtckd.cn_scale[k] = v[k];
+    }
+    c->mtckd.gen++;
+    c->plan_valid = false;
+    return 0;
+}
+
+int clblm_contnm_absrb(clblm_ctx *c, int nlay, const clblm_layer *layers, const double *v1, const double *v2,
+                       const int32_t *jrad, int32_t flags, double *d_absrb, int64_t ld, int32_t *nptabs,
+                       void *stream) {
     if (!c || !layers || !v1 || !v2 || !jrad || !nptabs || nlay <= 0 || nlay > 65535)
         return set_err(CLBLM_ERR_ARG, "null argument or nlay outside 1..65535");
     HIPCHK(hipSetDevice(c->device));
     std::lock_guard<std::mutex> g(c->mu);
     std::vector<MtJob> jobs;
     std::vector<double> coef;
-    int64_t maxpts = 1;
+    std::vector<CnJob> cjobs;
+    std::vector<CnTerm> cterms;
+    std::vector<double> ccoef;
+    int64_t maxpts = 1, cmax = 1;
     for (int l = 0; l < nlay; l++) {
         if (!(layers[l].tave > 0.) || !(layers[l].wtot > 0.))
             return set_err(CLBLM_ERR_ARG, "tave and wtot must be positive");
         double v1abs;
         absrb_grid(v1[l], v2[l], &v1abs, &nptabs[l]);
         if (d_absrb && ld < nptabs[l]) return set_err(CLBLM_ERR_ARG, "ld is below a layer's ABSRB length");
-        MtJob mj{};
-        bool has = false;
-        int rc = mtckd_plan(c->mtckd, layers[l], v1[l], v2[l], jrad[l], mj, coef, &has);
-        if (rc) return rc;
-        if (has) {
-            mj.abs_off = (int64_t)l * ld;
-            jobs.push_back(mj);
-            maxpts = std::max<int64_t>(maxpts, mj.ihi - mj.ilo + 1);
+        if (flags & CLBLM_OD_MTCKD) {
+            MtJob mj{};
+            bool has = false;
+            int rc = mtckd_plan(c->mtckd, layers[l], v1[l], v2[l], jrad[l], mj, coef, &has);
+            if (rc) return rc;
+            if (has) {
+                mj.abs_off = (int64_t)l * ld;
+                jobs.push_back(mj);
+                maxpts = std::max<int64_t>(maxpts, mj.ihi - mj.ilo + 1);
+            }
+        }
+        if (flags & CLBLM_OD_CONTNM) {
+            CnJob cj{};
+            bool has = false;
+            int rc = cn_plan(c->mtckd.cn_scale, layers[l], v1[l], v2[l], jrad[l], cj, cterms, ccoef, &has);
+            if (rc) return rc;
+            if (has) {
+                cj.abs_off = (int64_t)l * ld;
+                cjobs.push_back(cj);
+                cmax = std::max<int64_t>(cmax, cj.nptabs);
+            }
         }
     }
     if (!d_absrb) return 0;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-    const size_t oj = 0, oc = (sizeof(MtJob) * std::max<size_t>(1, jobs.size()) + 255) & ~(size_t)255;
-    const size_t bytes = oc + sizeof(double) * std::max<size_t>(1, coef.size());
+    auto up = [](size_t o) { return (o + 255) & ~(size_t)255; };
+    const size_t oj = 0, oc = up(sizeof(MtJob) * std::max<size_t>(1, jobs.size()));
+    const size_t ocj = up(oc + sizeof(double) * std::max<size_t>(1, coef.size()));
+    const size_t oct = up(ocj + sizeof(CnJob) * std::max<size_t>(1, cjobs.size()));
+    const size_t occ = up(oct + sizeof(CnTerm) * std::max<size_t>(1, cterms.size()));
+    const size_t bytes = occ + sizeof(double) * std::max<size_t>(1, ccoef.size());
     HIPCHK(hipStreamSynchronize(st));   // the buffer may still be read by a previous call
     if (bytes > c->mt_cap) {
         (void)hipFree(c->mt_buf);
@@ -3901,14 +4185,29 @@ int clblm_mtckd_absrb(clblm_ctx *c, int nlay, const clblm_layer *layers, const d
         HIPCHK(hipMemcpyAsync(b + oj, jobs.data(), sizeof(MtJob) * jobs.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(b + oc, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
     }
+    if (!cjobs.empty()) {
+        HIPCHK(hipMemcpyAsync(b + ocj, cjobs.data(), sizeof(CnJob) * cjobs.size(), hipMemcpyHostToDevice, st));
+        if (!cterms.empty())
+            HIPCHK(hipMemcpyAsync(b + oct, cterms.data(), sizeof(CnTerm) * cterms.size(), hipMemcpyHostToDevice, st));
+        if (!ccoef.empty())
+            HIPCHK(hipMemcpyAsync(b + occ, ccoef.data(), sizeof(double) * ccoef.size(), hipMemcpyHostToDevice, st));
+    }
     const size_t nb = sizeof(double) * (size_t)ld * (size_t)nlay;
     hipLaunchKernelGGL(k_fill_bytes, dim3(cdiv((int64_t)nb, 256)), dim3(256), 0, st, (uint8_t *)d_absrb, nb, (uint8_t)0);
     if (!jobs.empty())
         hipLaunchKernelGGL(k_mtckd, dim3(cdiv(maxpts, 256), (unsigned)jobs.size()), dim3(256), 0, st,
                            (const MtJob *)(b + oj), (const double *)(b + oc), d_absrb);
+    if (!cjobs.empty())
+        hipLaunchKernelGGL(k_contnm, dim3(cdiv(cmax, 256), (unsigned)cjobs.size()), dim3(256), 0, st,
+                           (const CnJob *)(b + ocj), (const CnTerm *)(b + oct), (const double *)(b + occ), d_absrb);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));   // the host job list goes out of scope
+    HIPCHK(hipStreamSynchronize(st));   // the host job lists go out of scope
     return 0;
+}
+
+int clblm_mtckd_absrb(clblm_ctx *c, int nlay, const clblm_layer *layers, const double *v1, const double *v2,
+                      const int32_t *jrad, double *d_absrb, int64_t ld, int32_t *nptabs, void *stream) {
+    return clblm_contnm_absrb(c, nlay, layers, v1, v2, jrad, CLBLM_OD_MTCKD, d_absrb, ld, nptabs, stream);
 }
 
 int clblm_dev_alloc(clblm_ctx *c, int64_t bytes, void **d) {

@@ -16,6 +16,9 @@ Tables written (reference file:line):
   * isotopologue masses for molecules 1-7      src/common/clblm_ConstParam.f90:260-285
   * MaxISOTPL_smass for molecules 1-7          src/common/clblm_ConstParam.f90:246-251
 
+and ``clblm_contnm.inc``: the continuum tables of CONTNM's thermal-infrared bands
+(CO2, N2 pure rotation and fundamental, O2 fundamental), listed at CONTNM_TABLES.
+
 Run in a container that holds the reference sources:
     python clblm_amd/csrc/gen_tables.py /root/reference
 The generated file is committed; the build never needs the reference.
@@ -136,8 +139,73 @@ def main(ref_root, out_inc, out_json):
               open(out_json, "w"))
 
 
+# ---- continuum tables (src/clblm_src/clblm_ODLAY_Continuum.f90) ------------------------
+# The CONTNM bands of the thermal infrared, each tabulated on its own grid
+# (V1S, V2S, DVS, NPTS):
+#   XFACCO2 (CONTNM :145-236), FRNCO2's S, TDEP_BANDHEAD and T_EFF (:1333-2530),
+#   xn2_r's C_296, sf_296, C_220, sf_220 (:2533-2681), n2_ver_1's xn2_272, xn2_228,
+#   a_h2o (:2701-2953), o2_ver_1's xo2, xo2t (:7187-7311).
+CONTNM_TABLES = [
+    # (C name, subroutine, DATA name, count)
+    ("CN_XFACCO2", "CONTNM", "XFACCO2", 500),
+    ("CN_CO2_GRID", "FRNCO2", "V1S", 4),
+    ("CN_CO2_S", "FRNCO2", "S", 5003),
+    ("CN_CO2_TDEP", "FRNCO2", "tdep_bandhead", 25),
+    ("CN_CO2_TEFF", "FRNCO2", "t_eff", 1),
+    ("CN_N2R_GRID", "xn2_r", "V1S", 4),
+    ("CN_N2R_C296", "xn2_r", "C_296", 73),
+    ("CN_N2R_SF296", "xn2_r", "sf_296", 73),
+    ("CN_N2R_C220", "xn2_r", "C_220", 73),
+    ("CN_N2R_SF220", "xn2_r", "sf_220", 73),
+    ("CN_N2F_GRID", "n2_ver_1", "V1S", 4),
+    ("CN_N2F_X272", "n2_ver_1", "xn2_272", 230),
+    ("CN_N2F_X228", "n2_ver_1", "xn2_228", 230),
+    ("CN_N2F_AH2O", "n2_ver_1", "a_h2o", 230),
+    ("CN_O2F_GRID", "o2_ver_1", "V1S", 4),
+    ("CN_O2F_X", "o2_ver_1", "xo2", 103),
+    ("CN_O2F_XT", "o2_ver_1", "xo2t", 103),
+]
+
+
+def subroutine_text(src, name):
+    m = re.search(r"^\s*SUBROUTINE\s+" + name + r"\b.*?END\s+SUBROUTINE", src, re.S | re.I | re.M)
+    if not m:
+        raise RuntimeError("subroutine %s not found" % name)
+    return _strip_comments(m.group(0))
+
+
+def data_values(text, name):
+    """The values of `DATA name/.../`, `DATA name(1:n)/.../` or `DATA (name(i),i=a,b)/.../`."""
+    m = re.search(r"\bDATA\s+\(?\s*" + name + r"\b[^/]*/(.*?)/", text, re.S | re.I)
+    if not m:
+        raise RuntimeError("DATA %s not found" % name)
+    return _numbers(m.group(1).replace("&", " "))
+
+
+def contnm_tables(ref_root):
+    src = open(os.path.join(ref_root, "src/clblm_src/clblm_ODLAY_Continuum.f90")).read()
+    out = {}
+    for cname, sub, dname, n in CONTNM_TABLES:
+        vals = data_values(subroutine_text(src, sub), dname)[:n]
+        if len(vals) != n:
+            raise RuntimeError("%s %s: expected %d values, got %d" % (sub, dname, n, len(vals)))
+        out[cname] = vals
+    return out
+
+
+def main_contnm(ref_root, out_inc):
+    t = contnm_tables(ref_root)
+    lines = ["/* Generated by gen_tables.py from the DATA statements of the reference's",
+             "   clblm_ODLAY_Continuum.f90; do not edit.  Values are the IEEE doubles nearest the",
+             "   reference's decimal literals.  *_GRID: V1S, V2S, DVS, NPTS of the table. */", ""]
+    for cname, _, _, _ in CONTNM_TABLES:
+        lines.append(c_array(cname, t[cname]))
+    open(out_inc, "w").write("\n".join(lines))
+
+
 if __name__ == "__main__":
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     here = os.path.dirname(os.path.abspath(__file__))
     repo = os.path.dirname(os.path.dirname(here))
     main(ref, os.path.join(here, "clblm_data.inc"), os.path.join(repo, "tests", "golden", "tables.json"))
+    main_contnm(ref, os.path.join(here, "clblm_contnm.inc"))

@@ -12,14 +12,16 @@ module clblm_hip
    implicit none
    private
    public :: clblm_layer_t, clblm_od_params_t, CLBLM_FP64, CLBLM_FP32C, CLBLM_OD_SPECIES_BROAD, CLBLM_OD_MTCKD
+   public :: CLBLM_OD_CONTNM
    public :: clblm_hip_init, clblm_hip_lines, clblm_hip_lines_file, clblm_hip_have_lines, clblm_hip_mtckd
+   public :: clblm_hip_contnm_scales
    public :: clblm_hip_odlay_plain, clblm_hip_finalize, clblm_hip_error, clblm_hip_jrad, clblm_hip_precision
    public :: clblm_hip_ctx, clblm_hip_handle
    public :: clblm_hip_dev_alloc, clblm_hip_dev_free, clblm_hip_dev_put, clblm_hip_dev_get
    public :: c_od_batch, c_rt_jac, c_level_jac
 
    integer(c_int), parameter :: CLBLM_FP64 = 0, CLBLM_FP32C = 1
-   integer(c_int32_t), parameter :: CLBLM_OD_SPECIES_BROAD = 1, CLBLM_OD_MTCKD = 2
+   integer(c_int32_t), parameter :: CLBLM_OD_SPECIES_BROAD = 1, CLBLM_OD_MTCKD = 2, CLBLM_OD_CONTNM = 4
    integer, parameter :: MAXMOL = 47
 
    type, bind(C) :: clblm_layer_t
@@ -71,6 +73,12 @@ module clblm_hip
          type(c_ptr), value :: ctx
          character(kind=c_char), intent(in) :: path(*)
          real(c_double), value :: xself, xfrgn
+      end function
+      integer(c_int) function c_contnm_scales(ctx, xco2c, xo3cn, xo2cn, xn2cn, xrayl) &
+            bind(C, name="clblm_contnm_scales")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: xco2c, xo3cn, xo2cn, xn2cn, xrayl
       end function
       integer(c_int) function c_odlay(ctx, handle, layer, par, jrad, nsamp, dv_used, od) bind(C, name="clblm_odlay")
          import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double, clblm_layer_t, clblm_od_params_t
@@ -213,8 +221,17 @@ contains
       call clblm_hip_error(c_mtckd_load_nc(clblm_hip_ctx, trim(path)//c_null_char, xself, xfrgn), 'clblm_hip_mtckd')
    end subroutine
 
+   ! CONTNM's scale factors of the CO2, O3, O2, N2 and Rayleigh continua (layers with
+   ! CLBLM_OD_CONTNM).
+   subroutine clblm_hip_contnm_scales(xco2c, xo3cn, xo2cn, xn2cn, xrayl)
+      real(c_double), intent(in) :: xco2c, xo3cn, xo2cn, xn2cn, xrayl
+      call clblm_hip_init()
+      call clblm_hip_error(c_contnm_scales(clblm_hip_ctx, xco2c, xo3cn, xo2cn, xn2cn, xrayl), &
+                           'clblm_hip_contnm_scales')
+   end subroutine
+
    ! One layer: molecule numbers mols(1:n) with columns w(1:n), p (hPa), T (K), Wtot;
-   ! flags: CLBLM_OD_SPECIES_BROAD, CLBLM_OD_MTCKD.
+   ! flags: CLBLM_OD_SPECIES_BROAD, CLBLM_OD_MTCKD, CLBLM_OD_CONTNM.
    subroutine clblm_hip_odlay_plain(mols, w, pave, tave, wtot, v1, v2, dvnormal, dvnarrow, sample, dvout, &
                                     precision, nsamp, od, dv_used, continuum, flags)
       integer, intent(in) :: mols(:)

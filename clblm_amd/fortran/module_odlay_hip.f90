@@ -10,11 +10,13 @@
 ! ioFiles%lineFile (a TAPE3 file, line coupling and species-broadening fields
 ! included) unless a list was uploaded with clblm_hip_lines.  odCtrl%speciesBroad
 ! selects the species-by-species broadening.  With the continuum on
-! (.not. odCtrl%contnmOff) the MT_CKD water-vapour continuum (self and foreign, XSELF
-! and XFRGN) is added from the reference's coefficient file, read on the first call;
-! the other continua of CONTNM (CO2, O3, O2, N2, Rayleigh) are not part of this path,
-! and a nonzero scale factor for them is reported once.  clblm_hip_precision selects
-! FP64 (default) or FP32C.  Not supported: NLTE, cross-sections, lineOff.
+! (.not. odCtrl%contnmOff) CONTNM's continua are added: the MT_CKD water vapour (self and
+! foreign, XSELF and XFRGN) from the reference's coefficient file, read on the first
+! call, and the CO2, O2 fundamental, N2 and Rayleigh continua of the thermal infrared
+! (XCO2C, XO2CN, XN2CN, XRAYL); a range that reaches the O3, the O2 near-infrared and
+! visible or the N2 overtone bands with their factor above 0 stops with a message.
+! clblm_hip_precision selects FP64 (default) or FP32C.  Not supported: NLTE,
+! cross-sections, lineOff.
 module Module_ODLAY
    use iso_c_binding
    use clblm_hip
@@ -28,7 +30,7 @@ module Module_ODLAY
       module procedure ODLAY_nonStruct
    end interface
    logical, save :: firstPassJRAD = .true.
-   logical, save :: mtckdLoaded = .false., warnedContinua = .false.
+   logical, save :: mtckdLoaded = .false.
    ! the reference's MT_CKD coefficient file (mt_ckd_h2o_module.f90:16)
    character(*), parameter :: mtckdFile = 'clblm_data/spectroscopy/absco-ref_wv-mt-ckd.nc'
 contains
@@ -100,15 +102,12 @@ contains
       if (.not. odCtrl%contnmOff) then
          if (.not. mtckdLoaded) then
             call clblm_hip_mtckd(mtckdFile, real(odCtrl%XSELF, c_double), real(odCtrl%XFRGN, c_double))
+            call clblm_hip_contnm_scales(real(odCtrl%XCO2C, c_double), real(odCtrl%XO3CN, c_double), &
+                                         real(odCtrl%XO2CN, c_double), real(odCtrl%XN2CN, c_double), &
+                                         real(odCtrl%XRAYL, c_double))
             mtckdLoaded = .true.
          end if
-         if (.not. warnedContinua .and. (odCtrl%XCO2C > 0. .or. odCtrl%XO3CN > 0. .or. odCtrl%XO2CN > 0. .or. &
-                                         odCtrl%XN2CN > 0. .or. odCtrl%XRAYL > 0.)) then
-            write (*, '(a)') ' ODLAY (HIP): only the MT_CKD water-vapour continuum is added; the CO2, O3, O2, '// &
-                             'N2 and Rayleigh continua are not part of this path'
-            warnedContinua = .true.
-         end if
-         flags = ior(flags, CLBLM_OD_MTCKD)
+         flags = ior(flags, ior(CLBLM_OD_MTCKD, CLBLM_OD_CONTNM))
       end if
       do k = 1, nMol
          mols(k) = molNum(molID(k))

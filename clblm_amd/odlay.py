@@ -107,9 +107,20 @@ class Context:
                                         float(table["ref_temp"]), float(table["ref_press"]), float(xself),
                                         float(xfrgn)))
 
+    def contnm_scales(self, xco2c=1.0, xo3cn=1.0, xo2cn=1.0, xn2cn=1.0, xrayl=1.0):
+        """clblm_contnm_scales: CONTNM's scale factors for the layers that ask for its
+        other continua (Context.params(..., other_continua=True))."""
+        check(self.lib.clblm_contnm_scales(self.ptr, float(xco2c), float(xo3cn), float(xo2cn), float(xn2cn),
+                                           float(xrayl)))
+
     def mtckd_absrb(self, layers, v1, v2, jrad, stream=None):
         """clblm_mtckd_absrb: the water-vapour continuum of each layer on its ABSRB grid;
         returns a list of float64 cuda tensors."""
+        return self.contnm_absrb(layers, v1, v2, jrad, _lib.OD_MTCKD, stream)
+
+    def contnm_absrb(self, layers, v1, v2, jrad, flags=_lib.OD_MTCKD | _lib.OD_CONTNM, stream=None):
+        """clblm_contnm_absrb: CONTNM's ABSRB of each layer for the continua `flags`
+        selects (OD_MTCKD, OD_CONTNM); returns a list of float64 cuda tensors."""
         import torch
         n = len(layers)
         ls = (_lib.Layer * n)(*[_layer_struct(lay) for lay in layers])
@@ -117,15 +128,15 @@ class Context:
         b = np.ascontiguousarray(np.broadcast_to(np.asarray(v2, np.float64), (n,)))
         j = np.ascontiguousarray(np.broadcast_to(np.asarray(jrad, np.int32), (n,)))
         npt = np.zeros(n, np.int32)
-        check(self.lib.clblm_mtckd_absrb(self.ptr, n, ls, a.ctypes.data, b.ctypes.data, j.ctypes.data, None, 0,
-                                         npt.ctypes.data, None))
+        check(self.lib.clblm_contnm_absrb(self.ptr, n, ls, a.ctypes.data, b.ctypes.data, j.ctypes.data, int(flags),
+                                          None, 0, npt.ctypes.data, None))
         ld = int(npt.max())
         out = torch.empty((n, ld), dtype=torch.float64, device="cuda")
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
-        check(self.lib.clblm_mtckd_absrb(self.ptr, n, ls, a.ctypes.data, b.ctypes.data, j.ctypes.data,
-                                         ctypes.c_void_p(out.data_ptr()), ld, npt.ctypes.data,
-                                         ctypes.c_void_p(stream)))
+        check(self.lib.clblm_contnm_absrb(self.ptr, n, ls, a.ctypes.data, b.ctypes.data, j.ctypes.data, int(flags),
+                                          ctypes.c_void_p(out.data_ptr()), ld, npt.ctypes.data,
+                                          ctypes.c_void_p(stream)))
         return [out[l, :npt[l]] for l in range(n)]
 
     def free_lines(self, handle):
@@ -134,9 +145,10 @@ class Context:
     # -- one layer (host buffers) ----------------------------------------------
     @staticmethod
     def params(v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, precision=FP64, continuum=None,
-               species_broad=False, mtckd=False):
+               species_broad=False, mtckd=False, other_continua=False):
         p = ODParams()
-        p.flags = (_lib.OD_SPECIES_BROAD if species_broad else 0) | (_lib.OD_MTCKD if mtckd else 0)
+        p.flags = ((_lib.OD_SPECIES_BROAD if species_broad else 0) | (_lib.OD_MTCKD if mtckd else 0) |
+                   (_lib.OD_CONTNM if other_continua else 0))
         p.v1, p.v2, p.dvnormal = float(v1), float(v2), float(dvnormal)
         p.dvnarrow = float(dvnormal if dvnarrow is None else dvnarrow)
         p.sample, p.dvout, p.precision = float(sample), float(dvout), int(precision)
@@ -147,10 +159,10 @@ class Context:
         return p
 
     def odlay(self, handle, layer, v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0,
-              precision=FP64, continuum=None, mtckd=False, species_broad=False):
+              precision=FP64, continuum=None, mtckd=False, species_broad=False, other_continua=False):
         lay = _layer_struct(layer)
         par = self.params(v1, v2, dvnormal, dvnarrow, sample, dvout, precision, continuum,
-                          species_broad=species_broad, mtckd=mtckd)
+                          species_broad=species_broad, mtckd=mtckd, other_continua=other_continua)
         jr = ctypes.c_int32(self.jrad)
         n = ctypes.c_int64()
         dvu = ctypes.c_double()

@@ -65,6 +65,14 @@ typedef struct {
  * continuum[] when contnm is also set) and carried into the OD as the reference's
  * CONTNM output is (XINT into R3). */
 #define CLBLM_OD_MTCKD 2
+/* CONTNM's other continua of the thermal infrared (clblm_ODLAY_Continuum.f90:408-1060):
+ * CO2 (FRNCO2 with the XFACCO2 scaling), the O2 collision-induced fundamental, the N2
+ * collision-induced pure rotation and fundamental bands and Rayleigh extinction, scaled
+ * by the context's XCO2C, XO2CN, XN2CN, XRAYL (clblm_contnm_scales), added to ABSRB after
+ * the water vapour in the reference's order.  A layer whose range reaches a band this
+ * path leaves out (O3 from 8920 cm-1, O2 from 7536 cm-1, the N2 overtone from 4340 cm-1)
+ * with that band's factor above 0 returns CLBLM_ERR_UNSUPPORTED. */
+#define CLBLM_OD_CONTNM 4
 
 /* ---- context ---------------------------------------------------------------- */
 int clblm_ctx_create(int device, clblm_ctx **out);
@@ -400,6 +408,18 @@ int clblm_mtckd_load(clblm_ctx *ctx, int n, const double *wavenumber, const doub
 int clblm_mtckd_read_nc(const char *path, int *n, double *wavenumber, double *self_absco_ref,
                         double *for_absco_ref, double *self_texp, double *ref_temp, double *ref_press);
 int clblm_mtckd_load_nc(clblm_ctx *ctx, const char *path, double xself, double xfrgn);
+
+/* CONTNM's scale factors XCO2C, XO3CN, XO2CN, XN2CN, XRAYL (CLBLM_OD_Ctrl,
+ * clblm_Config.f90:98-102) for the layers that set CLBLM_OD_CONTNM; 1 each by default.
+ * XSELF and XFRGN are those of clblm_mtckd_load. */
+int clblm_contnm_scales(clblm_ctx *ctx, double xco2c, double xo3cn, double xo2cn, double xn2cn, double xrayl);
+
+/* CONTNM's ABSRB alone (clblm_ODLAY.f90:880-896) for the continua flags selects
+ * (CLBLM_OD_MTCKD, CLBLM_OD_CONTNM), on the grid and with the JRAD rule of
+ * clblm_mtckd_absrb below. */
+int clblm_contnm_absrb(clblm_ctx *ctx, int nlay, const clblm_layer *layers, const double *v1, const double *v2,
+                       const int32_t *jrad, int32_t flags, double *d_absrb, int64_t ld, int32_t *nptabs,
+                       void *stream);
 
 /* The same continuum alone, as CONTNM adds it to a zeroed ABSRB (clblm_ODLAY.f90:880-896):
  * for layer l, on V1ABS = INT(v1[l]) - 3 at 1 cm-1 (nptabs[l] points, returned), with the

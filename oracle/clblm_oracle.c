@@ -26,6 +26,7 @@
 #include <string.h>
 
 #include "../clblm_amd/csrc/clblm_data.inc"
+#include "../clblm_amd/csrc/clblm_contnm.inc"
 
 /* clblm_ConstParam.f90:49-50,59-71,92,110,112 */
 #define ONEPL 1.01
@@ -2717,6 +2718,171 @@ int orc_mtckd_h2o(const double *wvn, const double *self_ref, const double *for_r
     }
     free(sh); free(fh); free(self_abs); free(for_abs);
     return NPTABS;
+}
+
+/* ---------------------------------------------------------------------------
+ * CONTNM's continua other than water vapour over the thermal infrared
+ * (clblm_ODLAY_Continuum.f90:408-1060): CO2 (FRNCO2 :1333-1400 with the XFACCO2
+ * scaling :415-457), the O2 collision-induced fundamental (o2_ver_1 :7187-7255, CONTNM
+ * :587-628), the N2 collision-induced pure rotation (xn2_r :2533-2600, CONTNM :807-852)
+ * and fundamental (n2_ver_1 :2701-2790, CONTNM :856-909) bands, and Rayleigh extinction
+ * (:970-1060, no aerosols), each added into absrb (nptabs points on v1abs + i at
+ * 1 cm-1, 0-based) in that order, the tabulated bands through XINT.  wk[m]: column of
+ * HITRAN molecule m (wk[1] H2O, wk[2] CO2, wk[7] O2); scale: XCO2C, XO3CN, XO2CN,
+ * XN2CN, XRAYL.  The bands CONTNM adds beyond these (O3, O2 from 7536 cm-1 up, the N2
+ * overtone) are not restated: returns 1 when one of them would add (its factor > 0 and
+ * its range reached), 0 otherwise.
+ * --------------------------------------------------------------------------- */
+
+/* the window every table routine of CONTNM takes around [V1ABS, V2ABS]: returns NPTC */
+static int cn_window(double V1ABS, double V2ABS, const double *grid, double *V1C, double *V2C, double *DVC,
+                     int *I1) {
+    const double V1S = grid[0], DVS = grid[2];
+    const int NPTS = (int)grid[3];
+    *DVC = DVS;
+    double v1c = V1ABS - *DVC, v2c = V2ABS + *DVC;
+    int i1 = v1c < V1S ? -1 : (int)((v1c - V1S) / DVS + 0.01);
+    v1c = V1S + DVS * (double)(i1 - 1);
+    const int i2 = (int)((v2c - V1S) / DVS + 0.01);
+    int nptc = i2 - i1 + 3;
+    if (nptc > NPTS) nptc = NPTS + 4;
+    *V1C = v1c;
+    *V2C = v1c + DVS * (double)(nptc - 1);
+    *I1 = i1;
+    return nptc;
+}
+
+int orc_contnm_ir(double pave, double tave, double wtot, const double *wk, double v1, double v2, int jrad,
+                  const double *scale, double v1abs, int nptabs, double *absrb) {
+    const double P0C = 1013., T0C = 296., XLOSMT = 2.68675E+19;
+    const double xco2c = scale[0], xo3cn = scale[1], xo2cn = scale[2], xn2cn = scale[3], xrayl = scale[4];
+    const double V1ABS = v1abs, V2ABS = v1abs + (double)(nptabs - 1), DVABS = 1.;
+    const double RHOAVE = (pave / P0C) * (T0C / tave);
+    const double XKT = tave / RADCN2;
+    const double amagat = (pave / P0C) * (273. / tave);
+    const double x_vmr_h2o = wk[1] / wtot, x_vmr_o2 = wk[7] / wtot;
+    const double x_vmr_n2 = 1. - x_vmr_h2o - x_vmr_o2;
+    const double wn2 = x_vmr_n2 * wtot;
+    double *R = absrb - 1;   /* 1-based */
+    double C[6000], V1C, V2C, DVC;
+    int I1, NPTC;
+    int beyond = 0;
+
+    /* CO2 */
+    if ((v2 > -20.0) && (v1 < 10000.) && xco2c > 0) {
+        const double WCO2 = wk[2] * RHOAVE * 1.0E-20 * xco2c;
+        NPTC = cn_window(V1ABS, V2ABS, CN_CO2_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_CO2_GRID[3];
+        const double trat = tave / CN_CO2_TEFF[0];
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            double f = 0.;
+            if (I >= 1 && I <= NPTS) {
+                double tcor = 1.;
+                if (I >= 1196 && I <= 1220) tcor = pow(trat, CN_CO2_TDEP[I - 1196]);
+                f = tcor * CN_CO2_S[I - 1];
+            }
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double CFAC = 1.;
+            if (VJ >= 2000. && VJ <= 2998.) CFAC = CN_XFACCO2[(int)((VJ - 1998.) / 2. + 0.00001) - 1];
+            f = CFAC * f;
+            C[J] = f * WCO2;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* O3 (diffuse, Hartley-Huggins, UV) */
+    if (xo3cn > 0. && ((v2 > 8920.0 && v1 <= 24665.0) || (v2 > 27370. && v1 < 40800.) || (v2 > 40800. && v1 < 54000.)))
+        beyond = 1;
+    /* O2 collision-induced fundamental */
+    if ((v2 > 1340.0) && (v1 < 1850.) && xo2cn > 0.) {
+        const double tau_fac = xo2cn * wk[7] * 1.e-20 * amagat;
+        NPTC = cn_window(V1ABS, V2ABS, CN_O2F_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_O2F_GRID[3];
+        const double xktfac = (1. / 296.) - (1. / tave);
+        const double factor = (1.e+20 / 2.68675e+19);
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double c0 = 0.;
+            if (I >= 1 && I <= NPTS) c0 = factor * CN_O2F_X[I - 1] * exp(CN_O2F_XT[I - 1] * xktfac) / VJ;
+            C[J] = tau_fac * c0;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* O2 near-infrared, visible and ultraviolet bands */
+    if (xo2cn > 0. && ((v2 > 7536.0 && v1 < 8500.) || (v2 > 9100.0 && v1 < 11000.) ||
+                       (v2 > 12961.5 && v1 < 13221.5) || (v2 > 15000.0 && v1 < 29870.) || v2 > 36000.0 ||
+                       v2 > 56740.0))
+        beyond = 1;
+    /* N2 collision-induced pure rotation */
+    if ((v2 > -10.0) && (v1 < 350.) && xn2cn > 0.) {
+        const double a_h2o = 1.;
+        const double tau_fac = xn2cn * (wn2 / XLOSMT) * amagat;
+        NPTC = cn_window(V1ABS, V2ABS, CN_N2R_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_N2R_GRID[3];
+        const double tfac = (tave - 296.) / (220. - 296.);
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double c0 = 0., c1 = 0.;
+            if (I >= 1 && I <= NPTS) {
+                c0 = CN_N2R_C296[I - 1] * pow(CN_N2R_C220[I - 1] / CN_N2R_C296[I - 1], tfac);
+                const double sf_T = CN_N2R_SF296[I - 1] * pow(CN_N2R_SF220[I - 1] / CN_N2R_SF296[I - 1], tfac);
+                c1 = (sf_T - 1.) * (0.79) / (0.21);
+            }
+            C[J] = tau_fac * c0 * (x_vmr_n2 + c1 * x_vmr_o2 + a_h2o * x_vmr_h2o);
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* N2 collision-induced fundamental */
+    if ((v2 > 2001.77) && (v1 < 2897.59) && xn2cn > 0.) {
+        const double tau_fac = xn2cn * (wn2 / XLOSMT) * amagat;
+        NPTC = cn_window(V1ABS, V2ABS, CN_N2F_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_N2F_GRID[3];
+        const double T = tave;
+        const double xtfac = ((1. / T) - (1. / 272.)) / ((1. / 228.) - (1. / 272.));
+        const double xt_lin = (T - 272.) / (228. - 272.);
+        const double a_o2 = 1.294 - 0.4545 * T / 296.;
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            double c0 = 0., c1 = 0., c2 = 0.;
+            if (I >= 1 && I <= NPTS) {
+                const double VJ = V1C + DVC * (double)(J - 1);
+                const double x272 = CN_N2F_X272[I - 1], x228 = CN_N2F_X228[I - 1];
+                if (x272 > 0. && x228 > 0.) c0 = x272 * pow(x228 / x272, xtfac);
+                else c0 = x272 + (x228 - x272) * xt_lin;
+                c0 = c0 / VJ;
+                c1 = a_o2 * c0;
+                c2 = (9. / 7.) * CN_N2F_AH2O[I - 1] * c0;
+            }
+            const double VJ = V1C + DVC * (double)(J - 1);
+            C[J] = tau_fac * (x_vmr_n2 * c0 + x_vmr_o2 * c1 + x_vmr_h2o * c2);
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* N2 first overtone */
+    if ((v2 > 4340.0) && (v1 < 4910.) && xn2cn > 0.) beyond = 1;
+    /* Rayleigh extinction (IAERSL 0) */
+    if (v2 >= 820. && xrayl > 0.) {
+        const double conv_cm2mol = xrayl * 1.E-20 / (2.68675e-1 * 1.e5);
+        for (int i = 1; i <= nptabs; i++) {
+            const double vrayleigh = V1ABS + (double)(i - 1) * DVABS;
+            const double x = vrayleigh / 1.e4;
+            double ray_ext = (x * x * x / (9.38076E2 - 10.8426 * (x * x))) * (wtot * conv_cm2mol);
+            if (jrad == 1) ray_ext = ray_ext * x;
+            else if (jrad == 0) ray_ext = ray_ext / 1.e4;
+            R[i] = R[i] + ray_ext;
+        }
+    }
+    return beyond;
 }
 
 /* ---------------------------------------------------------------------------

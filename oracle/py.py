@@ -100,10 +100,12 @@ class Oracle:
         self.jrad = 1
 
     def odlay(self, layer, v1, v2, dvnormal, dvnarrow=None, sample=4.0, dvout=0.0, continuum=None,
-              species_broad=False, mtckd=None):
+              species_broad=False, mtckd=None, contnm=None):
         """Returns (od, dv_used).  JRAD persists across calls like the reference's SAVE.
         mtckd: a continuum table (clblm_amd.continuum.load_mtckd); its water-vapour
-        continuum is added to ABSRB after `continuum`, with OPDPTH's JRAD after the flip."""
+        continuum is added to ABSRB after `continuum`, with OPDPTH's JRAD after the flip.
+        contnm: CONTNM's other thermal-infrared continua (CO2, O2, N2, Rayleigh), the
+        scale factors (XCO2C, XO3CN, XO2CN, XN2CN, XRAYL), added after those."""
         lay = _Layer()
         lay.pave, lay.tave, lay.wtot = layer.pave, layer.tave, layer.wtot
         for m, v in layer.w.items():
@@ -117,6 +119,14 @@ class Oracle:
                 jr = 0
             continuum = mtckd_h2o(mtckd, layer.pave, layer.tave, layer.w.get(1, 0.0), layer.wtot, v1, v2, jr,
                                   absrb=continuum)
+        if contnm is not None:
+            jr = self.jrad
+            if v1 / (layer.tave / 1.4387752) < 5. and jr != -1:
+                jr = 0
+            continuum, beyond = contnm_ir(layer.pave, layer.tave, layer.wtot, layer.w, v1, v2, jr, contnm,
+                                          absrb=continuum)
+            if beyond:
+                raise RuntimeError("contnm: a band beyond the thermal infrared is reached")
         if continuum is not None:
             absrb = np.ascontiguousarray(continuum, np.float64)
             npt = absrb.size
@@ -383,6 +393,37 @@ def mtckd_h2o(table, pave, tave, wk1, wtot, v1, v2, jrad, xself=1.0, xfrgn=1.0, 
     if n < 0:
         raise RuntimeError(L.orc_error().decode())
     return out[:n]
+
+
+def absrb_grid(v1, v2):
+    """OPDPTH's ABSRB grid (clblm_ODLAY.f90:880-887): (V1ABS, NPTABS) at 1 cm-1."""
+    a = float(int(v1))
+    if v1 < 0.:
+        a -= 1.
+    a -= 3.
+    b = float(int(v2 + 3. + 0.5))
+    return a, int((b - a) / 1. + 1.5)
+
+
+def contnm_ir(pave, tave, wtot, wk, v1, v2, jrad, scale=(1.0, 1.0, 1.0, 1.0, 1.0), absrb=None):
+    """orc_contnm_ir: CONTNM's CO2, O2 fundamental, N2 rotation and fundamental and
+    Rayleigh continua of one layer on OPDPTH's ABSRB grid, added to absrb (zeros by
+    default).  wk: {HITRAN molecule: column}.  Returns (absrb, beyond): beyond is 1
+    when a band this restatement leaves out would add."""
+    L = lib()
+    D, P = ctypes.c_double, ctypes.c_void_p
+    L.orc_contnm_ir.restype = ctypes.c_int
+    L.orc_contnm_ir.argtypes = [D, D, D, P, D, D, ctypes.c_int, P, D, ctypes.c_int, P]
+    v1abs, n = absrb_grid(v1, v2)
+    w = np.zeros(48)
+    for m, v in wk.items():
+        w[m] = v
+    sc = np.ascontiguousarray(scale, np.float64)
+    out = np.zeros(n) if absrb is None else np.array(absrb, np.float64)
+    assert out.size == n
+    beyond = L.orc_contnm_ir(pave, tave, wtot, w.ctypes.data, v1, v2, int(jrad), sc.ctypes.data, v1abs, n,
+                             out.ctypes.data)
+    return out, beyond
 
 
 def fltrfn(x, v1, dv, v1f, dvf, xf):

@@ -144,9 +144,9 @@ def test_dropin_odlay_called_as_the_reference_calls_it(cont, brd):
     """A Fortran program calling the reference's generic ODLAY with the reference's own
     CLBLM_OD_Ctrl and ioFiles (Module_Config, Module_ConstParam compiled from the
     reference's sources), against the drop-in Module_ODLAY: the line list comes from the
-    TAPE3 file named by ioFiles%lineFile, speciesBroad and the MT_CKD continuum (from a
-    netCDF file at the reference's data path) follow the control record, and the
-    optical depths equal the Python route's (same library, 1e-13)."""
+    TAPE3 file named by ioFiles%lineFile, speciesBroad and the continua (MT_CKD from a
+    netCDF file at the reference's data path, CO2 and the others) follow the control
+    record, and the optical depths equal the Python route's (same library, 1e-13)."""
     from scipy.io import netcdf_file
     from clblm_amd.continuum import load_mtckd
     from clblm_amd.odlay import Context
@@ -196,7 +196,8 @@ def test_dropin_odlay_called_as_the_reference_calls_it(cont, brd):
         pos += 12
         od_f = np.frombuffer(raw[pos:pos + 8 * n], np.float64)
         pos += 8 * n
-        od, dvp = ctx.odlay(h, l, 600.0, 700.0, dv, mtckd=bool(cont), species_broad=bool(brd))
+        od, dvp = ctx.odlay(h, l, 600.0, 700.0, dv, mtckd=bool(cont), other_continua=bool(cont),
+                            species_broad=bool(brd))
         assert n == od.size and dvo == dvp
         np.testing.assert_allclose(od_f, od, rtol=1e-13, atol=0)
     ctx.close()

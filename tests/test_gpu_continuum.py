@@ -112,3 +112,74 @@ def test_mtckd_needs_a_table():
             c.odlay(h, lay, 600.0, 700.0, 0.002, mtckd=True)
     finally:
         c.close()
+
+
+# ---- CONTNM's other bands of the thermal infrared (CO2, O2, N2, Rayleigh) ----------
+CN_BANDS = [(500.0, 2500.0), (20.3, 340.7), (1300.0, 1900.0), (1990.0, 2910.0), (2300.0, 2480.0), (830.0, 1000.0)]
+
+
+@pytest.mark.parametrize("jrad", [1, 0, -1])
+def test_contnm_absrb_bitwise(ctx, table, jrad):
+    """clblm_contnm_absrb against the oracle: the other bands alone, and after the water
+    vapour as CONTNM adds them, bit for bit."""
+    from clblm_amd import _lib
+    from oracle.py import contnm_ir, mtckd_h2o
+    layers = W.atmosphere()[::9]
+    for v1, v2 in CN_BANDS:
+        alone = ctx.contnm_absrb(layers, v1, v2, jrad, _lib.OD_CONTNM)
+        both = ctx.contnm_absrb(layers, v1, v2, jrad, _lib.OD_MTCKD | _lib.OD_CONTNM)
+        for lay, g1, g2 in zip(layers, alone, both):
+            w1, _ = contnm_ir(lay.pave, lay.tave, lay.wtot, lay.w, v1, v2, jrad)
+            h2o = mtckd_h2o(table, lay.pave, lay.tave, lay.w.get(1, 0.0), lay.wtot, v1, v2, jrad)
+            w2, _ = contnm_ir(lay.pave, lay.tave, lay.wtot, lay.w, v1, v2, jrad, absrb=h2o)
+            assert np.array_equal(g1.cpu().numpy(), w1), (v1, v2, lay.pave)
+            assert np.array_equal(g2.cpu().numpy(), w2), (v1, v2, lay.pave)
+
+
+def test_contnm_scales(ctx):
+    from clblm_amd import _lib
+    from oracle.py import contnm_ir
+    layers = W.atmosphere()[::20]
+    sc = (0.5, 1.0, 2.0, 0.0, 3.0)
+    ctx.contnm_scales(*sc)
+    try:
+        got = ctx.contnm_absrb(layers, 100.0, 2600.0, 1, _lib.OD_CONTNM)
+    finally:
+        ctx.contnm_scales()
+    for lay, g in zip(layers, got):
+        want, _ = contnm_ir(lay.pave, lay.tave, lay.wtot, lay.w, 100.0, 2600.0, 1, sc)
+        assert np.array_equal(g.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("k", [0, 30, 59])
+def test_od_with_all_continua_matches_oracle(ctx, table, k):
+    """The OD with CONTNM's water vapour and other bands against the oracle's OD over
+    the N2 fundamental and the CO2 band head."""
+    from oracle.py import Oracle
+    from test_gpu_od_fp64 import _cmp
+    lines = W.profile_lines(20000, 2000.0, 2600.0, seed=1001)
+    lay = W.atmosphere()[k]
+    dv = W.calculate_dv(2300.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+    h = ctx.upload_lines(lines)
+    ctx.jrad = 1
+    od, _ = ctx.odlay(h, lay, 2250.0, 2450.0, dv, mtckd=True, other_continua=True)
+    orc = Oracle(lines)
+    ref, _ = orc.odlay(lay, 2250.0, 2450.0, dv, mtckd=table, contnm=(1.0, 1.0, 1.0, 1.0, 1.0))
+    err = _cmp(od, ref)
+    print("layer %d: max rel err %.3e" % (k, err))
+    ctx.free_lines(h)
+
+
+def test_contnm_bands_left_out_are_refused(ctx):
+    from clblm_amd._lib import ClblmError
+    lay = W.atmosphere()[0]
+    lines = W.profile_lines(2000, 4300.0, 4500.0, seed=1001)
+    h = ctx.upload_lines(lines)
+    with pytest.raises(ClblmError, match="N2 overtone"):
+        ctx.odlay(h, lay, 4300.0, 4400.0, 0.01, other_continua=True)
+    ctx.contnm_scales(xn2cn=0.0)
+    try:
+        ctx.odlay(h, lay, 4300.0, 4400.0, 0.01, other_continua=True)   # the band's factor is 0: no refusal
+    finally:
+        ctx.contnm_scales()
+    ctx.free_lines(h)
