@@ -3564,6 +3564,10 @@ extern "C" int clblm_fft_scan_impl(void *state, const double *d_in, int64_t n, d
                                    double pre_box_hw, int deconv_pre_box, int precision, double *d_out, int64_t *nout,
                                    double *v1res, double *dvres, void *stream);
 
+extern "C" int clblm_rt_flux_impl(int nlay, int64_t npts, double v1, double dv, const double *d_od,
+                                  const double *d_tlay, const double *d_tlev, int lin, double emis, double tskin,
+                                  int nang, int outinrat, int nout, int lambertian, double *scratch, int64_t blk_pts,
+                                  double *fdn, double *fup, void *stream);
 extern "C" int clblm_rt_mode1_impl(int nlay, double v1, double v2, const double *dv, const int64_t *nlim,
                                    const int64_t *od_off, const double *d_od_up, const double *d_od_dn,
                                    const double *d_od_sun, const double *fac_up, const double *fac_dn,
@@ -4500,6 +4504,62 @@ int clblm_rt_mode1(clblm_ctx *c, const clblm_mode1_params *p, const double *dv, 
                                  nlim_out, st);
     if (rc) return set_err(rc, "merge-down radiance failed");
     HIPCHK(hipStreamSynchronize(st));   // the scratch arrays are reused by the next call
+    return 0;
+}
+
+int clblm_rt_flux(clblm_ctx *c, const clblm_flux_params *p, const double *d_od, int64_t npts, const double *d_tlay,
+                  const double *d_tlev, const double *p_lev, double *flux_up, double *flux_dn, double *net_flux,
+                  double *heating, int32_t *nbin, void *stream) {
+    if (!c || !p || !nbin || p->nlay <= 0) return set_err(CLBLM_ERR_ARG, "null argument");
+    if (p->nang < 1 || p->nang > 3) return set_err(CLBLM_ERR_ARG, "nang must be 1, 2 or 3");
+    if (p->lin_in_tau < 0 || p->lin_in_tau > 2) return set_err(CLBLM_ERR_ARG, "lin_in_tau must be 0, 1 or 2");
+    if (!(p->v2 > p->v1) || !(p->dv > 0.) || !(p->dv_flux > 0.) || npts < 2)
+        return set_err(CLBLM_ERR_ARG, "need v2 > v1, dv > 0, dv_flux > 0 and 2 points");
+    // the number of output groups (clblm_noScattRT.f90:197-202, EPS = 1e-4)
+    const double OUT = (p->v2 - p->v1) / p->dv_flux;
+    const int NOUT = (int)(OUT + 1.E-4);
+    if (std::fabs((double)NOUT - OUT) > 1.E-4 || NOUT < 1)
+        return set_err(CLBLM_ERR_ARG, "V1, V2 and DV_flux are inconsistent (the reference stops)");
+    const int outinrat = (int)(p->dv_flux / p->dv);
+    if (outinrat < 1) return set_err(CLBLM_ERR_ARG, "dv_flux is below dv");
+    *nbin = NOUT;
+    if (!flux_up) return 0;
+    if (!d_od || !d_tlay || !d_tlev || !p_lev || !flux_dn || !net_flux || !heating)
+        return set_err(CLBLM_ERR_ARG, "null argument");
+    const int nlay = p->nlay, nlev = nlay + 1;
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    const int64_t gpb = std::max<int64_t>(1, ((int64_t)1 << 18) / outinrat);
+    const int64_t blk = gpb * outinrat;
+    const size_t nscr = (size_t)(2 * p->nang * nlay + 1) * (size_t)blk;
+    const size_t nout_arr = 2 * (size_t)nlev * (size_t)NOUT;
+    const size_t bytes = sizeof(double) * (nscr + nout_arr);
+    if (c->grid_state_cap < bytes) {
+        if (c->grid_state) (void)hipFree(c->grid_state);
+        c->grid_state = nullptr;
+        c->grid_state_cap = 0;
+        HIPCHK(hipMalloc(&c->grid_state, bytes));
+        c->grid_state_cap = bytes;
+    }
+    double *fdn = c->grid_state + nscr, *fup = fdn + (size_t)nlev * NOUT;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    const size_t zb = sizeof(double) * nout_arr;   // groups no point reaches stay 0
+    hipLaunchKernelGGL(k_fill_bytes, dim3(cdiv((int64_t)zb, 256)), dim3(256), 0, st, (uint8_t *)fdn, zb, (uint8_t)0);
+    int rc = clblm_rt_flux_impl(nlay, npts, p->v1, p->dv, d_od, d_tlay, d_tlev, p->lin_in_tau, p->surf_emis,
+                                p->t_skin, p->nang, outinrat, NOUT, p->lambertian, c->grid_state, blk, fdn, fup, st);
+    if (rc) return set_err(rc, "flux kernels failed");
+    HIPCHK(hipMemcpyAsync(flux_dn, fdn, sizeof(double) * (size_t)nlev * NOUT, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(flux_up, fup, sizeof(double) * (size_t)nlev * NOUT, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // the driver's net flux and heating rate (clblm_Drivers.f90:455-466), HEATFC of
+    // clblm_ConstParam.f90:45
+    const double HEATFC = 8.4391;
+    for (int64_t q = 0; q < (int64_t)nlev * NOUT; q++) net_flux[q] = flux_up[q] - flux_dn[q];
+    for (int j = nlev - 1; j >= 0; j--)
+        for (int b = 0; b < NOUT; b++) {
+            const int64_t q = (int64_t)j * NOUT + b;
+            heating[q] = j == nlev - 1 ? 0. : HEATFC * (net_flux[q] - net_flux[q + NOUT]) / (p_lev[j] - p_lev[j + 1]);
+        }
     return 0;
 }
 

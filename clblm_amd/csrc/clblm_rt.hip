@@ -1001,3 +1001,160 @@ extern "C" __attribute__((visibility("hidden"))) int clblm_rt_mode1_impl(
         return CLBLM_ERR_HIP;
     return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
 }
+
+// ---------------------------------------------------------------------------------
+// Fluxes: the flux branches of mode1_lookDn_mergeDn (clblm_noScattRT.f90:133-422,
+// 482-507) and mode2_lookDn_mergeUp (:1214-1450), every layer on one grid.  Per
+// quadrature angle the ODs are scaled by its secant; the down radiance is merged from
+// the top, the surface term formed, the up radiance merged from the surface.  The
+// points are taken in blocks of whole output groups: k_flux_cols writes every level's
+// down and up radiance of every angle for the block's points (one lane per point),
+// k_flux_groups sums each (level, group) over its points in order (one lane each) and
+// weights the angles, k_flux_surface sums the surface upwelling flux of each group.
+
+struct FluxArgs {
+    int nlay, nang, lin, lambertian;
+    double v1, dv, emis, tskin, factor, pi;
+    double sec[3], w[3];
+    int64_t npts, p0, nblk;   // the block: points p0 .. p0 + nblk - 1
+    int outinrat, nout, g0, ngrp;
+};
+
+__global__ void __launch_bounds__(256) k_flux_cols(FluxArgs A, const double *__restrict__ od,
+                                                   const double *__restrict__ t_lay, const double *__restrict__ t_lev,
+                                                   double *__restrict__ dn, double *__restrict__ up,
+                                                   double *__restrict__ sfc) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.nblk) return;
+    const int64_t i = A.p0 + k;
+    const double v = A.v1 + (double)i * A.dv;
+    const int64_t LB = (int64_t)A.nlay * A.nblk;
+    double r0[3] = {0., 0., 0.};
+    for (int a = 0; a < A.nang; a++) {
+        double r = 0.;
+        for (int l = A.nlay - 1; l >= 0; l--) {
+            const double odb = od[(int64_t)l * A.npts + i] * A.sec[a];
+            double txa, txb, emup, emdn;   // on the down path alone: txa = txb
+            grid_layer_point(v, odb, 1., t_lev[l], t_lev[l + 1], t_lay[l], A.lin, &txa, &txb, &emup, &emdn);
+            r = emdn + r * txb;
+            dn[a * LB + (int64_t)l * A.nblk + k] = r;
+        }
+        r0[a] = r;
+    }
+    double d;   // the downwelling flux density at the surface (:396-403)
+    if (A.nang == 1) d = A.factor * A.w[0] * r0[0];
+    else if (A.nang == 2) d = A.factor * (A.w[0] * r0[0] + A.w[1] * r0[1]);
+    else d = A.factor * (A.w[0] * r0[0] + A.w[1] * r0[1] + A.w[2] * r0[2]);
+    const double srf = planck_x(v, A.tskin / RADCN2) * A.emis;
+    sfc[k] = srf * A.dv * 1.E04 * A.pi + (1. - A.emis) * d;
+    for (int a = 0; a < A.nang; a++) {
+        // the lambertian form for points 1 .. nlim - 1 (:488-495), the angle's own
+        // reflected radiance otherwise (addThermalBoundary's flux branch :2020-2024)
+        double r = (A.lambertian && i < A.npts - 1) ? srf + ((1. - A.emis) * (d / (A.pi * A.dv * 1.E04)))
+                                                    : srf + (1. - A.emis) * r0[a];
+        for (int l = 0; l < A.nlay; l++) {
+            const double oda = od[(int64_t)l * A.npts + i] * A.sec[a];
+            double txa, txb, emup, emdn;
+            grid_layer_point(v, oda, 1., t_lev[l], t_lev[l + 1], t_lay[l], A.lin, &txa, &txb, &emup, &emdn);
+            r = emup + txa * (r + emdn * 0.);
+            up[a * LB + (int64_t)l * A.nblk + k] = r;
+        }
+    }
+}
+
+// (pass, layer, group): pass 0 the down radiance of layer l (the flux at level l),
+// pass 1 its up radiance (the flux at level l + 1).
+__global__ void __launch_bounds__(256) k_flux_groups(FluxArgs A, const double *__restrict__ dn,
+                                                     const double *__restrict__ up, double *__restrict__ fdn,
+                                                     double *__restrict__ fup) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)A.nlay * A.ngrp;
+    if (t >= 2 * per) return;
+    const int pass = (int)(t / per);
+    const int l = (int)((t % per) / A.ngrp);
+    const int gl = (int)(t % A.ngrp);
+    const int g = A.g0 + gl;
+    if (g >= A.nout) return;
+    const double *R = pass == 0 ? dn : up;
+    const int64_t LB = (int64_t)A.nlay * A.nblk;
+    const int64_t k0 = (int64_t)gl * A.outinrat;
+    double S[3] = {0., 0., 0.};
+    for (int a = 0; a < A.nang; a++) {
+        const double *r = R + a * LB + (int64_t)l * A.nblk;
+        double s = 0.;
+        for (int64_t k = k0; k < k0 + A.outinrat && k < A.nblk; k++) s = s + r[k];
+        S[a] = s;
+    }
+    double f;
+    if (A.nang == 1) f = A.w[0] * S[0] * A.factor;
+    else if (A.nang == 2) f = (A.w[0] * S[0] + A.w[1] * S[1]) * A.factor;
+    else f = (A.w[0] * S[0] + A.w[1] * S[1] + A.w[2] * S[2]) * A.factor;
+    if (pass == 0) fdn[(int64_t)l * A.nout + g] = f;
+    else fup[(int64_t)(l + 1) * A.nout + g] = f;
+}
+
+// the surface upwelling flux of each group, over points 1 .. nlim - 1 (:482-507)
+__global__ void __launch_bounds__(256) k_flux_surface(FluxArgs A, const double *__restrict__ sfc,
+                                                      double *__restrict__ fup) {
+    const int gl = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (gl >= A.ngrp) return;
+    const int g = A.g0 + gl;
+    if (g >= A.nout) return;
+    const int64_t k0 = (int64_t)gl * A.outinrat;
+    double s = 0.;
+    for (int64_t k = k0; k < k0 + A.outinrat && k < A.nblk && A.p0 + k < A.npts - 1; k++) s = s + sfc[k];
+    fup[g] = s;
+}
+
+// fdn, fup: device arrays of (nlay + 1) * nout, level-major; scratch: device array of
+// (2 nang nlay + 1) * blk_pts doubles.
+extern "C" __attribute__((visibility("hidden"))) int clblm_rt_flux_impl(
+    int nlay, int64_t npts, double v1, double dv, const double *d_od, const double *d_tlay, const double *d_tlev,
+    int lin, double emis, double tskin, int nang, int outinrat, int nout, int lambertian, double *scratch,
+    int64_t blk_pts, double *fdn, double *fup, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    FluxArgs A{};
+    A.nlay = nlay;
+    A.nang = nang;
+    A.lin = lin;
+    A.lambertian = lambertian;
+    A.v1 = v1;
+    A.dv = dv;
+    A.emis = emis;
+    A.tskin = tskin;
+    A.pi = 3.1415926535898;
+    A.factor = dv * 1.E04 * 2. * A.pi;
+    if (nang == 1) {
+        A.sec[0] = 1. / 0.66666666667;
+        A.w[0] = 0.50;
+    } else if (nang == 2) {
+        A.sec[0] = 1.0 / 0.84494897;
+        A.sec[1] = 1.0 / 0.35505103;
+        A.w[0] = 0.31804138;
+        A.w[1] = 0.18195862;
+    } else {
+        A.sec[0] = 1.0 / 0.91141204;
+        A.sec[1] = 1.0 / 0.59053314;
+        A.sec[2] = 1.0 / 0.21234054;
+        A.w[0] = 0.20093191;
+        A.w[1] = 0.22924111;
+        A.w[2] = 0.06982698;
+    }
+    A.npts = npts;
+    A.outinrat = outinrat;
+    A.nout = nout;
+    const int gpb = (int)(blk_pts / outinrat);   // whole groups per block
+    double *dn = scratch, *up = scratch + (int64_t)nang * nlay * blk_pts, *sfc = up + (int64_t)nang * nlay * blk_pts;
+    for (int64_t p0 = 0, g0 = 0; p0 < npts && g0 < nout; p0 += (int64_t)gpb * outinrat, g0 += gpb) {
+        A.p0 = p0;
+        A.nblk = std::min<int64_t>((int64_t)gpb * outinrat, npts - p0);
+        A.g0 = (int)g0;
+        A.ngrp = (int)((A.nblk + outinrat - 1) / outinrat);
+        hipLaunchKernelGGL(k_flux_cols, dim3((unsigned)((A.nblk + 255) / 256)), dim3(256), 0, st, A, d_od, d_tlay,
+                           d_tlev, dn, up, sfc);
+        const int64_t nt = 2 * (int64_t)nlay * A.ngrp;
+        hipLaunchKernelGGL(k_flux_groups, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, A, dn, up, fdn, fup);
+        hipLaunchKernelGGL(k_flux_surface, dim3((unsigned)((A.ngrp + 255) / 256)), dim3(256), 0, st, A, sfc, fup);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}

@@ -233,6 +233,39 @@ def radiance_mode1(ctx, od, offsets, dvs, v1, v2, t_lay, t_lev, od_dn=None, od_s
     return rad, tx, dvo.value
 
 
+def fluxes(ctx, od, t_lay, t_lev, p_lev, v1, v2, dv, dv_flux, nang=3, lin_in_tau=1, emis=0.98, tskin=295.0,
+           lambertian=True):
+    """Upwelling and downwelling fluxes, net flux and heating rate at every level
+    (clblm_rt_flux: the flux branches of mode1_lookDn_mergeDn and mode2_lookDn_mergeUp,
+    src/clblm_src/clblm_noScattRT.f90:133-507, 1214-1450, and the driver's net flux and
+    heating rate, clblm_Drivers.f90:418-466).  od: float64 cuda tensor [L, N] of the layer
+    ODs on v1 + i*dv (layer 0 at the surface); t_lay [L], t_lev [L+1] (K), p_lev [L+1]
+    (hPa).  Returns a dict of numpy arrays [L+1, nbin] (level 0 at the surface, groups of
+    dv_flux from v1): up, dn, net (W/m2), heating (K/day)."""
+    import torch
+    L, n = od.shape
+    if not od.is_cuda or od.dtype != torch.float64:
+        raise ValueError("od must be a float64 cuda tensor")
+    tl = torch.as_tensor(np.asarray(t_lay, np.float64), device=od.device)
+    tv = torch.as_tensor(np.asarray(t_lev, np.float64), device=od.device)
+    pl = np.ascontiguousarray(p_lev, np.float64)
+    if tl.numel() != L or tv.numel() != L + 1 or pl.size != L + 1:
+        raise ValueError("t_lay needs one value per layer, t_lev and p_lev one per level")
+    p = _lib.FluxParams()
+    p.nlay, p.nang, p.lin_in_tau, p.lambertian = L, int(nang), int(lin_in_tau), int(bool(lambertian))
+    p.v1, p.v2, p.dv, p.dv_flux, p.surf_emis, p.t_skin = float(v1), float(v2), float(dv), float(dv_flux), \
+        float(emis), float(tskin)
+    nb = ctypes.c_int32()
+    od = od.contiguous()
+    args = [ctx.ptr, ctypes.byref(p), ctypes.c_void_p(od.data_ptr()), n, ctypes.c_void_p(tl.data_ptr()),
+            ctypes.c_void_p(tv.data_ptr()), pl.ctypes.data]
+    check(ctx.lib.clblm_rt_flux(*args, None, None, None, None, ctypes.byref(nb), None))
+    out = {k: np.zeros((L + 1, nb.value)) for k in ("up", "dn", "net", "heating")}
+    check(ctx.lib.clblm_rt_flux(*args, *[out[k].ctypes.data for k in ("up", "dn", "net", "heating")],
+                                ctypes.byref(nb), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return out
+
+
 def upwelling_radiance(ctx, od, t_lay, t_lev, v1, dv, emis=0.98, tskin=295.0, lin_in_tau=1, sec_dn=1.0):
     """Differentiable radiance R(OD, T_lay) (torch.autograd.Function on ROCm tensors)."""
     return _Radiance.apply(od, t_lay, ctx, t_lev, v1, dv, emis, tskin, lin_in_tau, sec_dn)

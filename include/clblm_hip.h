@@ -462,6 +462,34 @@ int clblm_rt_mode1(clblm_ctx *ctx, const clblm_mode1_params *p, const double *dv
                    const double *t_lev, const double *d_sol, int64_t sol_n, double sol_dv, double *d_rad,
                    double *d_tx, double *dv_out, int64_t *nlim_out, void *stream);
 
+/* ---- fluxes -------------------------------------------------------------------------
+ * The flux branches of mode1_lookDn_mergeDn (src/clblm_src/clblm_noScattRT.f90:133-422,
+ * 482-507) and mode2_lookDn_mergeUp (:1214-1450) with the driver's net flux and heating
+ * rate (clblm_Drivers.f90:418-466): every layer's OD on one grid (d_od[l npts + i],
+ * layer 0 at the surface, device; v1 + i dv, i < npts), temperatures d_tlay[nlay],
+ * d_tlev[nlay + 1] (device), level pressures p_lev[nlay + 1] (hPa, host).  nang = 1..3
+ * quadrature angles (secants and weights of clblm_ConstParam.f90:31-38 and the RT
+ * routines :155-172); per angle the ODs are scaled by the secant, the down radiance is
+ * merged from the top (layerEmis toward the lower level), the surface radiance is
+ * eps B(t_skin) plus (1 - eps) times the downwelling (lambertian: the quadrature flux
+ * over pi, the reference's ThmReflMode 0; else the angle's own radiance), and the up
+ * radiance is merged from the surface with no reflected term.  The radiances of each
+ * level are summed in groups of int(dv_flux / dv) points from v1 and weighted:
+ * flux = (sum_a w_a S_a) dv 1e4 2 pi (W/m2 per group).  Outputs (host, level-major,
+ * (nlay + 1) x *nbin, level 0 at the surface): flux_up (level 0 the surface upwelling
+ * flux), flux_dn (0 at the top), net_flux = up - dn, heating = 8.4391
+ * (net_j - net_j+1) / (p_j - p_j+1) K/day (0 at the top).  *nbin = NOUT = (v2 - v1) /
+ * dv_flux, which must be whole (to 1e-4) as the reference requires; the reference's
+ * arrays carry one more group, always zero.  flux_up == NULL returns *nbin only.
+ * Synchronous. */
+typedef struct {
+    int32_t nlay, nang, lin_in_tau, lambertian;
+    double v1, v2, dv, dv_flux, surf_emis, t_skin;
+} clblm_flux_params;
+int clblm_rt_flux(clblm_ctx *ctx, const clblm_flux_params *p, const double *d_od, int64_t npts, const double *d_tlay,
+                  const double *d_tlev, const double *p_lev, double *flux_up, double *flux_dn, double *net_flux,
+                  double *heating, int32_t *nbin, void *stream);
+
 /* Spectral response function filter: clblm_FLTRFN with CNVFLT
  * (src/clblm_src/clblm_ScanFilter.f90:1703-1900), once per channel as
  * postProc_filter_singleStruct does (clblm_PostProc.f90:289-336).  d_in: the spectrum (n

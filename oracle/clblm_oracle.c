@@ -1459,6 +1459,139 @@ void orc_rt_up(int nlay, int64_t npts, double v1, double dv, const double *od, c
     }
 }
 
+/* The flux branches of mode1_lookDn_mergeDn (clblm_noScattRT.f90:133-422, 482-507) and
+ * mode2_lookDn_mergeUp (:1214-1450) with the driver's net flux and heating rate
+ * (clblm_Drivers.f90:418-466), every layer on one grid (od[l*npts + i], layer 0 at the
+ * surface, v1 + i dv, i < npts), with the quadrature of nang = 1..3 angles (secants and
+ * weights of Clough et al. 1992, clblm_ConstParam.f90:31-38).  Per angle the ODs are
+ * scaled by the secant; the down radiance is merged from the top (EmDn + R TxB), the
+ * surface gives eps B(Tskin) plus (1 - eps) times the downwelling (lambertian, the
+ * reference's ThmReflMode 0: the quadrature flux over pi, else the angle's own
+ * radiance; :364-376, :482-507 and addThermalBoundary's flux branch :2020-2024), the
+ * up radiance is merged
+ * from the surface (EmUp + TxA R; the reflected term is zero in the flux branch).  Each
+ * level's radiances are summed in groups of OUTINRAT = int(dv_flux / dv) points and
+ * weighted: flux = (sum_a w_a S_a) dv 1e4 2 pi.  Outputs per level j = 0..nlay
+ * (0: surface) and group k < nbin: up[j*nbin + k], dn, net = up - dn and
+ * htr = HEATFC (net_j - net_j+1) / (p_j - p_j+1) (0 at the top).  Returns nbin (NOUT)
+ * or -1 when (v2 - v1) / dv_flux is not a whole number. */
+int orc_rt_flux(int nlay, int64_t npts, double v1, double v2, double dv, const double *od, const double *t_lay,
+                const double *t_lev, const double *p_lev, int lin, double emis, double tskin, int nang,
+                double dv_flux, int lambertian, double *up, double *dn, double *net, double *htr) {
+    const double PIC = 3.1415926535898, EPSC = 1.E-4, HEATFC = 8.4391;
+    double sec[3] = {0., 0., 0.}, w[3] = {0., 0., 0.};
+    if (nang == 1) { sec[0] = 1. / 0.66666666667; w[0] = 0.50; }
+    else if (nang == 2) { sec[0] = 1.0 / 0.84494897; sec[1] = 1.0 / 0.35505103; w[0] = 0.31804138; w[1] = 0.18195862; }
+    else if (nang == 3) {
+        sec[0] = 1.0 / 0.91141204; sec[1] = 1.0 / 0.59053314; sec[2] = 1.0 / 0.21234054;
+        w[0] = 0.20093191; w[1] = 0.22924111; w[2] = 0.06982698;
+    } else return fail("flux: nang must be 1, 2 or 3");
+    const double OUT = (v2 - v1) / dv_flux;
+    const int NOUT = (int)(OUT + EPSC);
+    if (fabs((double)NOUT - OUT) > EPSC) return fail("flux: V1, V2, DV_flux are inconsistent");
+    const int outinrat = (int)(dv_flux / dv);
+    const double FACTOR = dv * 1.E04 * 2. * PIC;
+    const int nlev = nlay + 1;
+    double *radd = (double *)calloc((size_t)nang * nlay * npts, sizeof(double));
+    double *radu = (double *)calloc((size_t)nang * nlay * npts, sizeof(double));
+    double *dflux = (double *)calloc((size_t)npts, sizeof(double));
+    double *srad = (double *)calloc((size_t)(NOUT + 2) * nang, sizeof(double));
+    double *sfcu = (double *)calloc((size_t)NOUT + 2, sizeof(double));
+    /* down, per angle and point */
+    for (int a = 0; a < nang; a++)
+        for (int64_t i = 0; i < npts; i++) {
+            const double v = v1 + (double)i * dv;
+            double r = 0.;
+            for (int l = nlay - 1; l >= 0; l--) {
+                const double odb = od[(int64_t)l * npts + i] * sec[a];
+                const double txb = exp(-odb);
+                const double emdn = layer_emis(v, t_lev[l], t_lev[l + 1], t_lay[l], txb, odb, lin);
+                r = emdn + r * txb;
+                radd[((int64_t)a * nlay + l) * npts + i] = r;
+            }
+        }
+    /* surface: downwelling flux density and the upwelling surface radiance per angle */
+    for (int64_t i = 0; i < npts; i++) {
+        const double *r0 = radd + i;   /* layer 0, angle a at r0[a * nlay * npts] */
+        double d;
+        if (nang == 1) d = FACTOR * w[0] * r0[0];
+        else if (nang == 2) d = FACTOR * (w[0] * r0[0] + w[1] * r0[(int64_t)nlay * npts]);
+        else d = FACTOR * (w[0] * r0[0] + w[1] * r0[(int64_t)nlay * npts] + w[2] * r0[2 * (int64_t)nlay * npts]);
+        dflux[i] = d;
+    }
+    for (int a = 0; a < nang; a++)
+        for (int64_t i = 0; i < npts; i++) {
+            const double v = v1 + (double)i * dv;
+            const double srf = orc_planck(v, tskin) * emis;
+            /* the lambertian form is set for points 1 .. nlim - 1 (:488-495); the last
+             * point keeps addThermalBoundary's, the angle's own reflected radiance */
+            double r = (lambertian && i < npts - 1) ? srf + ((1. - emis) * (dflux[i] / (PIC * dv * 1.E04)))
+                                                    : srf + (1. - emis) * radd[((int64_t)a * nlay + 0) * npts + i];
+            for (int l = 0; l < nlay; l++) {
+                const double oda = od[(int64_t)l * npts + i] * sec[a];
+                const double txa = exp(-oda);
+                const double emdn = layer_emis(v, t_lev[l], t_lev[l + 1], t_lay[l], txa, oda, lin);
+                const double emup = lin != 0 ? layer_emis(v, t_lev[l + 1], t_lev[l], t_lay[l], txa, oda, lin) : emdn;
+                r = emup + txa * (r + emdn * 0.);
+                radu[((int64_t)a * nlay + l) * npts + i] = r;
+            }
+        }
+    for (int64_t q = 0; q < (int64_t)nlev * NOUT; q++) up[q] = dn[q] = 0.;
+    /* groups of each level: layer l's down radiance is the flux at level l, its up
+     * radiance the flux at level l + 1 */
+    for (int pass = 0; pass < 2; pass++)
+        for (int l = 0; l < nlay; l++) {
+            const double *R = pass == 0 ? radd : radu;
+            for (int q = 0; q < (NOUT + 2) * nang; q++) srad[q] = 0.;
+            int iout = 1, icount = 0;
+            for (int64_t k = 0; k < npts; k++) {
+                for (int a = 0; a < nang; a++) srad[(iout - 1) * nang + a] += R[((int64_t)a * nlay + l) * npts + k];
+                icount++;
+                if (icount >= outinrat) {
+                    icount = 0;
+                    iout++;
+                    if (iout > NOUT + 1) iout = NOUT + 1;   /* the reference's arrays end there */
+                }
+            }
+            double *F = pass == 0 ? dn + (int64_t)l * NOUT : up + (int64_t)(l + 1) * NOUT;
+            for (int b = 0; b < NOUT; b++) {
+                const double *S = srad + b * nang;
+                if (nang == 1) F[b] = w[0] * S[0] * FACTOR;
+                else if (nang == 2) F[b] = (w[0] * S[0] + w[1] * S[1]) * FACTOR;
+                else F[b] = (w[0] * S[0] + w[1] * S[1] + w[2] * S[2]) * FACTOR;
+            }
+        }
+    /* surface upwelling flux, points 1 .. nlim - 1 */
+    {
+        int iout = 1, icount = 0;
+        for (int64_t k = 0; k < npts - 1; k++) {
+            const double v = v1 + (double)k * dv;
+            const double srf = orc_planck(v, tskin) * emis;
+            sfcu[iout - 1] = sfcu[iout - 1] + srf * dv * 1.E04 * PIC + (1. - emis) * dflux[k];
+            icount++;
+            if (icount >= outinrat) {
+                icount = 0;
+                iout++;
+                if (iout > NOUT + 1) iout = NOUT + 1;
+            }
+        }
+        for (int b = 0; b < NOUT; b++) up[b] = sfcu[b];
+    }
+    for (int j = 0; j < nlev; j++)
+        for (int b = 0; b < NOUT; b++) {
+            const int64_t q = (int64_t)j * NOUT + b;
+            net[q] = up[q] - dn[q];
+        }
+    for (int j = nlev - 1; j >= 0; j--)
+        for (int b = 0; b < NOUT; b++) {
+            const int64_t q = (int64_t)j * NOUT + b;
+            if (j == nlev - 1) htr[q] = 0.;
+            else htr[q] = HEATFC * (net[q] - net[q + NOUT]) / (p_lev[j] - p_lev[j + 1]);
+        }
+    free(radd); free(radu); free(dflux); free(srad); free(sfcu);
+    return NOUT;
+}
+
 /* spectraHaveSameGrid (clblm_Spectrum.f90) for two spectra from one V1: DV,
  * V2 = V1 + DV (NLIM - 1) (CLBLM_Spectrum_init) and NLIM equal, to epsilon(1d0). */
 static int same_grid(double v1, double dva, int64_t na, double dvb, int64_t nb) {

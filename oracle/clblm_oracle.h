@@ -133,6 +133,9 @@ int64_t orc_scan_interp(const double *x, int64_t n, double v1, double dv, double
 int orc_mtckd_h2o(const double *wvn, const double *self_ref, const double *for_ref, const double *texp, int ncoeffin,
                   double ref_temp, double ref_press, double pave, double tave, double wk1, double wtot, double v1,
                   double v2, int jrad, double xself, double xfrgn, double *absrb, int cap);
+int orc_rt_flux(int nlay, int64_t npts, double v1, double v2, double dv, const double *od, const double *t_lay,
+                const double *t_lev, const double *p_lev, int lin, double emis, double tskin, int nang,
+                double dv_flux, int lambertian, double *up, double *dn, double *net, double *htr);
 int orc_contnm_ir(double pave, double tave, double wtot, const double *wk, double v1, double v2, int jrad,
                   const double *scale, double v1abs, int nptabs, double *absrb);
 int64_t orc_rt_mode1(int nlay, double v1, double v2, const double *dv, const int64_t *nlim, const int64_t *off,

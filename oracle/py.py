@@ -395,6 +395,26 @@ def mtckd_h2o(table, pave, tave, wk1, wtot, v1, v2, jrad, xself=1.0, xfrgn=1.0, 
     return out[:n]
 
 
+def rt_flux(od, t_lay, t_lev, p_lev, v1, v2, dv, dv_flux, nang=3, lin=1, emis=0.98, tskin=295.0, lambertian=True):
+    """orc_rt_flux: fluxes [L+1, nbin] (up, dn, net, heating) from layer ODs [L, N] on one
+    grid (layer 0 at the surface)."""
+    L = lib()
+    D, P, I = ctypes.c_double, ctypes.c_void_p, ctypes.c_int
+    L.orc_rt_flux.restype = I
+    L.orc_rt_flux.argtypes = [I, ctypes.c_int64, D, D, D, P, P, P, P, I, D, D, I, D, I, P, P, P, P]
+    od = np.ascontiguousarray(od, np.float64)
+    nl, n = od.shape
+    tl, tv, pl = (np.ascontiguousarray(x, np.float64) for x in (t_lay, t_lev, p_lev))
+    nout = int((v2 - v1) / dv_flux + 1e-4)
+    out = {k: np.zeros((nl + 1, max(nout, 1))) for k in ("up", "dn", "net", "heating")}
+    r = L.orc_rt_flux(nl, n, v1, v2, dv, od.ctypes.data, tl.ctypes.data, tv.ctypes.data, pl.ctypes.data, lin,
+                      emis, tskin, nang, dv_flux, 1 if lambertian else 0,
+                      *[out[k].ctypes.data for k in ("up", "dn", "net", "heating")])
+    if r < 0:
+        raise RuntimeError(L.orc_error().decode())
+    return out
+
+
 def absrb_grid(v1, v2):
     """OPDPTH's ABSRB grid (clblm_ODLAY.f90:880-887): (V1ABS, NPTABS) at 1 cm-1."""
     a = float(int(v1))
