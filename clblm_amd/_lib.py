@@ -21,7 +21,7 @@ EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_d
            "clblm_interp_to_finest", "clblm_rt_up_grids", "clblm_mtckd_load", "clblm_mtckd_absrb",
            "clblm_rt_mode1", "clblm_filter", "clblm_mtckd_read_nc", "clblm_mtckd_load_nc",
            "clblm_dev_alloc", "clblm_dev_free", "clblm_dev_put", "clblm_dev_get", "clblm_contnm_scales",
-           "clblm_contnm_absrb", "clblm_rt_flux"]
+           "clblm_contnm_absrb", "clblm_rt_flux", "clblm_rt_jac_paths"]
 GRID_EXACT, GRID_FIXED_RATIO, GRID_UNIFORM = -1, 0, 1
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_CAPACITY = 0, 1, 2, 3, 4
 OD_SPECIES_BROAD = 1
@@ -49,6 +49,14 @@ class Mode1Params(ctypes.Structure):
                 ("thermal", ctypes.c_int32), ("solar", ctypes.c_int32), ("v1", ctypes.c_double),
                 ("v2", ctypes.c_double), ("surf_emis", ctypes.c_double), ("t_skin", ctypes.c_double),
                 ("surf_refl", ctypes.c_double)]
+
+
+class JacPathsParams(ctypes.Structure):
+    """clblm_jac_paths_params"""
+    _fields_ = [("nlay", ctypes.c_int32), ("obs_level", ctypes.c_int32), ("uplook", ctypes.c_int32),
+                ("lin_in_tau", ctypes.c_int32), ("solar", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("surf_emis", ctypes.c_double), ("t_skin", ctypes.c_double), ("rsfc_sun", ctypes.c_double),
+                ("amf_view", ctypes.c_void_p), ("amf_down", ctypes.c_void_p), ("amf_sun", ctypes.c_void_p)]
 
 
 class FluxParams(ctypes.Structure):
@@ -140,6 +148,7 @@ def load():
     lib.clblm_mtckd_absrb.argtypes = [P, ctypes.c_int, P, P, P, P, P, ctypes.c_int64, P, P]
     lib.clblm_contnm_scales.argtypes = [P, D, D, D, D, D]
     lib.clblm_rt_flux.argtypes = [P, P, P, ctypes.c_int64, P, P, P, P, P, P, P, P, P]
+    lib.clblm_rt_jac_paths.argtypes = [P, P, ctypes.c_int64, D, D] + [P] * 12
     lib.clblm_contnm_absrb.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_int32, P, ctypes.c_int64, P, P]
     lib.clblm_rt_mode1.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, P, P, ctypes.POINTER(D),
                                    ctypes.POINTER(I64), P]

@@ -3564,6 +3564,12 @@ extern "C" int clblm_fft_scan_impl(void *state, const double *d_in, int64_t n, d
                                    double pre_box_hw, int deconv_pre_box, int precision, double *d_out, int64_t *nout,
                                    double *v1res, double *dvres, void *stream);
 
+extern "C" int clblm_rt_jac_paths_impl(int L, int64_t n, double v1, double dv, const double *d_od, const double *d_av,
+                                       const double *d_ad, const double *d_as, const double *d_tlay,
+                                       const double *d_tlev, double emis, double tskin, int solar, const double *d_sol,
+                                       double rs, int obs, int uplook, int lin, const double *d_dt_dtup,
+                                       const double *d_dt_dtlo, double *d_rad, double *d_dtau, double *d_dtlev,
+                                       double *d_dts, double *d_de, double *d_drs, double *d_state, void *stream);
 extern "C" int clblm_rt_flux_impl(int nlay, int64_t npts, double v1, double dv, const double *d_od,
                                   const double *d_tlay, const double *d_tlev, int lin, double emis, double tskin,
                                   int nang, int outinrat, int nout, int lambertian, double *scratch, int64_t blk_pts,
@@ -4504,6 +4510,46 @@ int clblm_rt_mode1(clblm_ctx *c, const clblm_mode1_params *p, const double *dv, 
                                  nlim_out, st);
     if (rc) return set_err(rc, "merge-down radiance failed");
     HIPCHK(hipStreamSynchronize(st));   // the scratch arrays are reused by the next call
+    return 0;
+}
+
+int clblm_rt_jac_paths(clblm_ctx *c, const clblm_jac_paths_params *p, int64_t npts, double v1, double dv,
+                       const double *d_od, const double *d_tlay, const double *d_tlev, const double *d_dt_dtup,
+                       const double *d_dt_dtlo, const double *d_sol, double *d_rad, double *d_dr_dtau,
+                       double *d_dr_dtlev, double *d_dr_dtskin, double *d_dr_demis, double *d_dr_drsfc,
+                       void *stream) {
+    if (!c || !p || p->nlay <= 0 || npts <= 0 || !d_od || !d_tlay || !d_tlev || !d_rad || !d_dr_dtau || !d_dr_dtlev)
+        return set_err(CLBLM_ERR_ARG, "null argument or empty grid");
+    const int L = p->nlay;
+    if (p->lin_in_tau < 0 || p->lin_in_tau > 2) return set_err(CLBLM_ERR_ARG, "lin_in_tau must be 0, 1 or 2");
+    if (p->obs_level < 0 || p->obs_level > L) return set_err(CLBLM_ERR_ARG, "obs_level must lie in 0..nlay");
+    if (p->lin_in_tau != 2 && (!d_dt_dtup || !d_dt_dtlo))
+        return set_err(CLBLM_ERR_ARG, "lin_in_tau 0 and 1 need dT/dT_lev of the layer means");
+    if (p->solar && !p->uplook && !d_sol) return set_err(CLBLM_ERR_ARG, "the solar term needs the solar radiance");
+    std::vector<double> amf((size_t)3 * L, 1.0);
+    const double *src[3] = {p->amf_view, p->amf_down, p->amf_sun};
+    for (int k = 0; k < 3; k++)
+        if (src[k]) std::copy(src[k], src[k] + L, amf.begin() + (size_t)k * L);
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->mu);
+    const size_t na = ((size_t)3 * L + 31) & ~(size_t)31;
+    const size_t bytes = sizeof(double) * (na + (size_t)5 * (L + 1) * (size_t)npts);
+    if (c->grid_state_cap < bytes) {
+        if (c->grid_state) (void)hipFree(c->grid_state);
+        c->grid_state = nullptr;
+        c->grid_state_cap = 0;
+        HIPCHK(hipMalloc(&c->grid_state, bytes));
+        c->grid_state_cap = bytes;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    double *d_amf = c->grid_state;
+    HIPCHK(hipMemcpyAsync(d_amf, amf.data(), sizeof(double) * amf.size(), hipMemcpyHostToDevice, st));
+    int rc = clblm_rt_jac_paths_impl(L, npts, v1, dv, d_od, d_amf, d_amf + L, d_amf + 2 * L, d_tlay, d_tlev,
+                                     p->surf_emis, p->t_skin, p->solar && !p->uplook ? 1 : 0, d_sol, p->rsfc_sun,
+                                     p->obs_level, p->uplook ? 1 : 0, p->lin_in_tau, d_dt_dtup, d_dt_dtlo, d_rad,
+                                     d_dr_dtau, d_dr_dtlev, d_dr_dtskin, d_dr_demis, d_dr_drsfc, d_amf + na, st);
+    if (rc) return set_err(rc, "Jacobian kernel failed");
+    HIPCHK(hipStreamSynchronize(st));   // the host factors and the scratch are reused by the next call
     return 0;
 }
 

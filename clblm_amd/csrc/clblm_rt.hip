@@ -372,6 +372,209 @@ __global__ void __launch_bounds__(256) k_rt_jac(int L, int64_t n, double v1, dou
     rad_o[i] = rad;
 }
 
+// Jacobians (src/clblm_src/clblm_noScattJac.f90:346-716) with every branch, one lane
+// per point: the path ODs are the vertical OD od[l*n+i] times the per-layer air-mass
+// factors av, ad, as (view, down, sun; pathOD_up_dn_sun's scaled paths :826-838); the
+// observer at level obs (0 = surface .. L = top); up-looking (:436-494) or
+// down-looking with the reflected downwelling, the surface terms, the solar term
+// (sol[i] at the top, surface reflectance rs) and the upwelling of the layers below the
+// observer (:496-678); then the level assembly (:681-715).  The reference's loops and
+// summation order.  States passed between the loops live in `st`, five arrays of
+// (L+1) x n: cumulative view transmittance, cumulative down (two-path) transmittance,
+// the down and up level terms and the layer-mean terms.
+__global__ void __launch_bounds__(256) k_rt_jac_paths(
+    int L, int64_t n, double v1, double dv, const double *__restrict__ od, const double *__restrict__ av,
+    const double *__restrict__ ad, const double *__restrict__ as, const double *__restrict__ t_lay,
+    const double *__restrict__ t_lev, double emis, double tskin, int solar, const double *__restrict__ sol, double rs,
+    int obs, int uplook, int lin, const double *__restrict__ dt_dtup, const double *__restrict__ dt_dtlo,
+    double *__restrict__ rad_o, double *__restrict__ dtau, double *__restrict__ dtlev, double *__restrict__ dts,
+    double *__restrict__ de, double *__restrict__ drs, double *__restrict__ st) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double vn = v1 + (double)i * dv;
+    const int64_t S = (int64_t)(L + 1) * n;
+    double *cA = st, *cB = st + S, *tdn = st + 2 * S, *tup = st + 3 * S, *tbar = st + 4 * S;
+    auto at = [n, i](int k) { return (int64_t)k * n + i; };
+    auto vod = [&](int j) { return od[at(j)] * av[j]; };
+    auto dod = [&](int j) { return od[at(j)] * ad[j]; };
+    auto blev = [&](int k, double *b, double *db) { planck_drv(vn, t_lev[k], b, db); };
+    for (int j = 0; j < L; j++) {
+        dtau[at(j)] = 0.;
+        tbar[at(j)] = 0.;
+    }
+    for (int k = 0; k <= L; k++) {
+        tdn[at(k)] = 0.;
+        tup[at(k)] = 0.;
+        dtlev[at(k)] = 0.;
+    }
+    double rad = 0., dEmis = 0., dTsfc = 0., dRsfc = 0., rsfc = 0.;
+    int obsLay = obs;   // the first layer above the observer
+    if (uplook) {
+        for (int k = 0; k <= obs; k++) cB[at(k)] = 1.0;
+        double cum = 0.0;
+        if (obsLay < 0) obsLay = 0;
+        for (int j = obsLay; j < L; j++) {
+            cum = cum + vod(j);
+            cB[at(j + 1)] = exp(-cum);
+        }
+        for (int j = L - 1; j >= obsLay; j--) {
+            const double dTran = cB[at(j)] - cB[at(j + 1)];
+            double bbar, dbbar, Blt;
+            planck_drv(vn, t_lay[j], &bbar, &dbbar);
+            if (lin != 0) {
+                double h, dh, blo, dblo, bhi, dbhi;
+                rlin(vod(j), &h, &dh);
+                blev(j, &blo, &dblo);
+                blev(j + 1, &bhi, &dbhi);
+                if (lin == 2) {
+                    Blt = blo * (1 - h) + bhi * h;
+                    dtau[at(j)] = (cB[at(j + 1)] * Blt - rad + dTran * dh * (bhi - blo)) * av[j];
+                    tdn[at(j + 1)] = tdn[at(j + 1)] + dTran * dbhi * h;
+                    tdn[at(j)] = dTran * dblo * (1.0 - h);
+                    tbar[at(j)] = 0.0;
+                } else {
+                    Blt = 2.0 * bbar * h + blo * (1.0 - 2.0 * h);
+                    dtau[at(j)] = (cB[at(j + 1)] * Blt - rad + 2.0 * dTran * dh * (bbar - blo)) * av[j];
+                    tdn[at(j)] = dTran * dblo * (1.0 - 2.0 * h);
+                    tbar[at(j)] = dTran * 2.0 * dbbar * h;
+                }
+                rad = rad + dTran * Blt;
+            } else {
+                dtau[at(j)] = (cB[at(j + 1)] * bbar - rad) * av[j];
+                rad = dTran * bbar + rad;
+                tbar[at(j)] = dTran * dbbar;
+            }
+        }
+    } else {
+        // 0- view-path transmittance from the observer down to the surface
+        double cumA = 0.;
+        for (int k = obs; k <= L; k++) cA[at(k)] = 1.;
+        for (int j = obsLay - 1; j >= 0; j--) {
+            cumA = cumA + vod(j);
+            cA[at(j)] = exp(-cumA);
+        }
+        const double totODUp = cumA, totTxUp = cA[at(0)];
+        // 1- downwelling thermal radiance
+        if (totTxUp > 1.e-06) {
+            cB[at(0)] = totTxUp;
+            double cumB = 0.;
+            for (int j = 0; j < L; j++) {
+                cumB = cumB + dod(j);
+                cB[at(j + 1)] = exp(-(totODUp + cumB));
+            }
+            for (int j = L - 1; j >= 0; j--) {
+                const double dTran = cB[at(j)] - cB[at(j + 1)];
+                double bbar, dbbar, Blt;
+                planck_drv(vn, t_lay[j], &bbar, &dbbar);
+                if (lin != 0) {
+                    double h, dh, blo, dblo, bhi, dbhi;
+                    rlin(dod(j), &h, &dh);
+                    blev(j, &blo, &dblo);
+                    blev(j + 1, &bhi, &dbhi);
+                    if (lin == 2) {
+                        Blt = blo + (bhi - blo) * h;
+                        dtau[at(j)] = (cB[at(j + 1)] * Blt - rad + dTran * dh * (bhi - blo)) * ad[j];
+                        tdn[at(j + 1)] = tdn[at(j + 1)] + dTran * dbhi * h;
+                        tdn[at(j)] = dTran * dblo * (1. - h);
+                        tbar[at(j)] = 0.0;
+                    } else {
+                        Blt = 2.0 * bbar * h + blo * (1.0 - 2.0 * h);
+                        dtau[at(j)] = (cB[at(j + 1)] * Blt - rad + 2.0 * dTran * dh * (bbar - blo)) * ad[j];
+                        tdn[at(j)] = dTran * dblo * (1.0 - 2.0 * h);
+                        tbar[at(j)] = dTran * 2.0 * dbbar * h;
+                    }
+                    rad = rad + dTran * Blt;
+                } else {
+                    dtau[at(j)] = (cB[at(j + 1)] * bbar - rad) * ad[j];
+                    rad = rad + dTran * bbar;
+                    tbar[at(j)] = dTran * dbbar;
+                }
+            }
+        }
+        // 2- scale by the surface reflectance
+        rsfc = (1. - emis);
+        for (int j = 0; j < L; j++) {
+            dtau[at(j)] = dtau[at(j)] * rsfc;
+            tbar[at(j)] = tbar[at(j)] * rsfc;
+        }
+        // 3- surface terms
+        double bs, dbs;
+        planck_drv(vn, tskin, &bs, &dbs);
+        dEmis = totTxUp * bs - rad;
+        dTsfc = emis * totTxUp * dbs;
+        const double sfcEmit = emis * totTxUp * bs;
+        rad = rad * rsfc + sfcEmit;
+        // 4- solar
+        if (solar) {
+            double totODSun = 0.;
+            for (int j = 0; j < L; j++) totODSun = totODSun + od[at(j)] * as[j];
+            const double totTxSun2 = exp(-(totODUp + totODSun));
+            dRsfc = totTxSun2 * sol[i];
+            const double radsun = rs * dRsfc;
+            for (int j = 0; j < L; j++) dtau[at(j)] = dtau[at(j)] - radsun * as[j];
+            rad = rad + radsun;
+        }
+        // 5- upwelling thermal radiance of the layers below the observer
+        for (int j = 0; j <= obsLay - 1; j++) {
+            const double dTran = cA[at(j + 1)] - cA[at(j)];
+            double bbar, dbbar, Blt;
+            planck_drv(vn, t_lay[j], &bbar, &dbbar);
+            if (lin != 0) {
+                double h, dh, blo, dblo, bhi, dbhi;
+                rlin(vod(j), &h, &dh);
+                blev(j, &blo, &dblo);
+                blev(j + 1, &bhi, &dbhi);
+                if (lin == 2) {
+                    Blt = bhi * (1 - h) + blo * h;
+                    dtau[at(j)] = dtau[at(j)] + (cA[at(j)] * Blt - rad + dTran * dh * (blo - bhi)) * av[j];
+                    tup[at(j + 1)] = dTran * dbhi * (1 - h);
+                    tup[at(j)] = tup[at(j)] + dTran * dblo * h;
+                } else {
+                    Blt = 2.0 * bbar * h + bhi * (1.0 - 2.0 * h);
+                    dtau[at(j)] = dtau[at(j)] + (cA[at(j)] * Blt - rad + dTran * dh * 2.0 * (bbar - bhi)) * av[j];
+                    tup[at(j + 1)] = dTran * dbhi * (1.0 - 2.0 * h);
+                    tbar[at(j)] = tbar[at(j)] + dTran * 2.0 * dbbar * h;
+                }
+                rad = rad + dTran * Blt;
+            } else {
+                dtau[at(j)] = dtau[at(j)] + (cA[at(j)] * bbar - rad) * av[j];
+                rad = rad + dTran * bbar;
+                tbar[at(j)] = tbar[at(j)] + dTran * dbbar;
+            }
+        }
+    }
+    // level temperature derivatives
+    if (lin != 0) {
+        if (uplook)
+            for (int k = obs; k <= L; k++) dtlev[at(k)] = tdn[at(k)];
+        else
+            for (int k = 0; k <= L; k++) dtlev[at(k)] = tup[at(k)] + tdn[at(k)] * rsfc;
+    }
+    if (lin == 0 || lin == 1) {
+        const int j2 = uplook ? obsLay : 0;
+        for (int j = L - 1; j >= j2; j--) {
+            dtlev[at(j + 1)] = dtlev[at(j + 1)] + tbar[at(j)] * dt_dtup[j];
+            dtlev[at(j)] = dtlev[at(j)] + tbar[at(j)] * dt_dtlo[j];
+        }
+    }
+    rad_o[i] = rad;
+    if (dts) dts[i] = dTsfc;
+    if (de) de[i] = dEmis;
+    if (drs) drs[i] = dRsfc;
+}
+
+extern "C" __attribute__((visibility("hidden"))) int clblm_rt_jac_paths_impl(
+    int L, int64_t n, double v1, double dv, const double *d_od, const double *d_av, const double *d_ad,
+    const double *d_as, const double *d_tlay, const double *d_tlev, double emis, double tskin, int solar,
+    const double *d_sol, double rs, int obs, int uplook, int lin, const double *d_dt_dtup, const double *d_dt_dtlo,
+    double *d_rad, double *d_dtau, double *d_dtlev, double *d_dts, double *d_de, double *d_drs, double *d_state,
+    void *stream) {
+    hipLaunchKernelGGL(k_rt_jac_paths, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, L, n,
+                       v1, dv, d_od, d_av, d_ad, d_as, d_tlay, d_tlev, emis, tskin, solar, d_sol, rs, obs, uplook, lin,
+                       d_dt_dtup, d_dt_dtlo, d_rad, d_dtau, d_dtlev, d_dts, d_de, d_drs, d_state);
+    return hipGetLastError() == hipSuccess ? 0 : CLBLM_ERR_HIP;
+}
+
 // drv_LevelTemp and drv_LevelConc (src/clblm_src/clblm_noScattJac.f90:899-1203): one
 // thread per (point, quantity); quantity 0 is temperature, q = 1..nmol the molecules.
 // od_var holds 3 + nmol blocks of [L][n]: the layer OD, the OD at Tave + dt, the OD at

@@ -233,6 +233,58 @@ def radiance_mode1(ctx, od, offsets, dvs, v1, v2, t_lay, t_lev, od_dn=None, od_s
     return rad, tx, dvo.value
 
 
+def jacobians_paths(ctx, od, t_lay, t_lev, v1, dv, amf_view=None, amf_down=None, amf_sun=None, obs_level=None,
+                    uplook=False, lin_in_tau=1, emis=0.98, tskin=295.0, solar=False, sol=None, rsfc_sun=0.0,
+                    dt_dtup=None, dt_dtlo=None):
+    """Per-point Jacobians with every branch of the reference (clblm_rt_jac_paths:
+    src/clblm_src/clblm_noScattJac.f90:346-716): per-layer air-mass factors of the view,
+    down and solar paths, an observer at any level, up- or down-looking, the solar term.
+    od: float64 cuda tensor [L, N] of vertical layer ODs on v1 + i*dv (layer 0 at the
+    surface); t_lay [L], t_lev [L+1]; sol: cuda tensor [N] of the solar radiance at the
+    top.  Returns a dict of cuda tensors: rad, dtau [L, N], dtlev [L+1, N], dtskin, demis,
+    drsfc."""
+    import torch
+    L, n = od.shape
+    if not od.is_cuda or od.dtype != torch.float64:
+        raise ValueError("od must be a float64 cuda tensor")
+    dev = od.device
+    tl = torch.as_tensor(np.asarray(t_lay, np.float64), device=dev)
+    tv = torch.as_tensor(np.asarray(t_lev, np.float64), device=dev)
+    if tl.numel() != L or tv.numel() != L + 1:
+        raise ValueError("t_lay needs one value per layer, t_lev one per level")
+    lin = int(lin_in_tau)
+    dup = dlo = None
+    if lin != 2:
+        if dt_dtup is None or dt_dtlo is None:
+            raise ValueError("lin_in_tau 0 and 1 need dt_dtup and dt_dtlo")
+        dup = torch.as_tensor(np.asarray(dt_dtup, np.float64), device=dev)
+        dlo = torch.as_tensor(np.asarray(dt_dtlo, np.float64), device=dev)
+        if dup.numel() != L or dlo.numel() != L:
+            raise ValueError("dt_dtup and dt_dtlo need one value per layer")
+    if solar and not uplook:
+        if sol is None or not sol.is_cuda or sol.numel() != n or sol.dtype != torch.float64:
+            raise ValueError("the solar term needs sol, a float64 cuda tensor of one value per point")
+    keep = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (amf_view, amf_down, amf_sun)]
+    for a in keep:
+        if a is not None and a.size != L:
+            raise ValueError("air-mass factors need one value per layer")
+    p = _lib.JacPathsParams()
+    p.nlay, p.obs_level, p.uplook, p.lin_in_tau = L, L if obs_level is None else int(obs_level), int(bool(uplook)), lin
+    p.solar, p.surf_emis, p.t_skin, p.rsfc_sun = int(bool(solar)), float(emis), float(tskin), float(rsfc_sun)
+    p.amf_view, p.amf_down, p.amf_sun = [None if a is None else a.ctypes.data for a in keep]
+    od = od.contiguous()
+    out = dict(rad=torch.empty(n, dtype=torch.float64, device=dev), dtau=torch.empty_like(od),
+               dtlev=torch.empty((L + 1, n), dtype=torch.float64, device=dev),
+               dtskin=torch.empty(n, dtype=torch.float64, device=dev), demis=torch.empty(n, dtype=torch.float64, device=dev),
+               drsfc=torch.empty(n, dtype=torch.float64, device=dev))
+    ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    check(ctx.lib.clblm_rt_jac_paths(ctx.ptr, ctypes.byref(p), n, ctypes.c_double(v1), ctypes.c_double(dv), ptr(od),
+                                     ptr(tl), ptr(tv), ptr(dup), ptr(dlo), ptr(sol if (solar and not uplook) else None),
+                                     *[ptr(out[k]) for k in ("rad", "dtau", "dtlev", "dtskin", "demis", "drsfc")],
+                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return out
+
+
 def fluxes(ctx, od, t_lay, t_lev, p_lev, v1, v2, dv, dv_flux, nang=3, lin_in_tau=1, emis=0.98, tskin=295.0,
            lambertian=True):
     """Upwelling and downwelling fluxes, net flux and heating rate at every level

@@ -462,6 +462,28 @@ int clblm_rt_mode1(clblm_ctx *ctx, const clblm_mode1_params *p, const double *dv
                    const double *t_lev, const double *d_sol, int64_t sol_n, double sol_dv, double *d_rad,
                    double *d_tx, double *dv_out, int64_t *nlim_out, void *stream);
 
+/* Jacobians (clblm_noScattJac.f90:346-716) with every branch of the reference: the
+ * layer's path ODs are its vertical OD d_od[l npts + i] times the air-mass factors
+ * amf_view, amf_down, amf_sun (host arrays [nlay], NULL: 1; pathOD_up_dn_sun's scaled
+ * paths :826-838); the observer at level obs_level (0 the surface .. nlay the top);
+ * up-looking (the down-welling radiance from the layers above the observer, :436-494)
+ * or down-looking (the reflected downwelling, the surface emission, the solar term
+ * d_sol[i] (solar radiance at the top) times rsfc_sun through the sun and view paths
+ * when solar is set, and the upwelling of the layers below the observer, :496-678).
+ * Outputs as clblm_rt_jac's, dR/dtau with respect to the vertical OD, plus
+ * d_dr_drsfc (dR/d solar reflectance); the surface outputs are zero when up-looking.
+ * Device arrays; d_dr_dtskin, d_dr_demis, d_dr_drsfc may be NULL.  Synchronous. */
+typedef struct {
+    int32_t nlay, obs_level, uplook, lin_in_tau, solar, pad;
+    double surf_emis, t_skin, rsfc_sun;
+    const double *amf_view, *amf_down, *amf_sun;
+} clblm_jac_paths_params;
+int clblm_rt_jac_paths(clblm_ctx *ctx, const clblm_jac_paths_params *p, int64_t npts, double v1, double dv,
+                       const double *d_od, const double *d_tlay, const double *d_tlev, const double *d_dt_dtup,
+                       const double *d_dt_dtlo, const double *d_sol, double *d_rad, double *d_dr_dtau,
+                       double *d_dr_dtlev, double *d_dr_dtskin, double *d_dr_demis, double *d_dr_drsfc,
+                       void *stream);
+
 /* ---- fluxes -------------------------------------------------------------------------
  * The flux branches of mode1_lookDn_mergeDn (src/clblm_src/clblm_noScattRT.f90:133-422,
  * 482-507) and mode2_lookDn_mergeUp (:1214-1450) with the driver's net flux and heating

@@ -1840,6 +1840,205 @@ void orc_jacobians(int nlay, int64_t npts, double v1, double dv, const double *o
     free(dTau); free(dTbar); free(dTup); free(dTdn); free(dTlev); free(vOD); free(dOD);
 }
 
+/* Jacobians (clblm_noScattJac.f90:346-716) with every branch: the path ODs are the
+ * layer's vertical OD od[l*npts+i] times the air-mass factors amf_view[l], amf_down[l],
+ * amf_sun[l] (pathOD_up_dn_sun's air-mass-scaled paths, :826-838), the observer at level
+ * obs (0 = surface .. nlay = top of the atmosphere), up-looking (the down-welling
+ * radiance at the observer from the layers above it, :436-494) or down-looking (the
+ * reflected downwelling, the surface terms, the solar term with sol_toa[i] and
+ * rsfc_sun when solar is set, and the upwelling of the layers below the observer,
+ * :496-678), then the level assembly (:681-715).  Layer 0 is at the surface.  Outputs
+ * rad, dr_dtau[l*npts+i] (with respect to the vertical OD), dr_dtlev[k*npts+i],
+ * dr_dtskin, dr_demis, dr_drsfc (each may be NULL).  The summation order is the
+ * reference's. */
+void orc_jacobians_paths(int nlay, int64_t npts, double v1, double dv, const double *od, const double *amf_view,
+                         const double *amf_down, const double *amf_sun, const double *t_lay, const double *t_lev,
+                         double emis, double tskin, int solar, const double *sol_toa, double rsfc_sun, int obs,
+                         int uplook, int lin, const double *dt_dtup, const double *dt_dtlo, double *rad_o,
+                         double *dr_dtau, double *dr_dtlev, double *dr_dtskin, double *dr_demis, double *dr_drsfc) {
+    const int L = nlay, N1 = 1, N2 = nlay;
+    const int obsLev = obs + 1;   /* 1-based level */
+    double *Blev = malloc(sizeof(double) * (L + 2)), *dBlev = malloc(sizeof(double) * (L + 2));
+    double *Bbar = malloc(sizeof(double) * (L + 1)), *dBbar = malloc(sizeof(double) * (L + 1));
+    double *cumTxA = malloc(sizeof(double) * (L + 2)), *cumTxB = malloc(sizeof(double) * (L + 2));
+    double *cumTxB2 = malloc(sizeof(double) * (L + 2));
+    double *dTau = malloc(sizeof(double) * (L + 1)), *dTbar = malloc(sizeof(double) * (L + 1));
+    double *dTup = malloc(sizeof(double) * (L + 2)), *dTdn = malloc(sizeof(double) * (L + 2));
+    double *dTlev = malloc(sizeof(double) * (L + 2));
+    double *vOD = malloc(sizeof(double) * (L + 1)), *dOD = malloc(sizeof(double) * (L + 1));
+    double *sOD = malloc(sizeof(double) * (L + 1));
+    for (int64_t i = 0; i < npts; i++) {
+        const double vn = v1 + (double)i * dv;
+        for (int l = 1; l <= L; l++) {
+            const double o = od[(int64_t)(l - 1) * npts + i];
+            vOD[l] = o * amf_view[l - 1];
+            dOD[l] = o * amf_down[l - 1];
+            sOD[l] = o * amf_sun[l - 1];
+        }
+        if (lin != 0)
+            for (int k = 1; k <= L + 1; k++) orc_planck_drv(vn, t_lev[k - 1], &Blev[k], &dBlev[k]);
+        for (int l = 1; l <= L; l++) orc_planck_drv(vn, t_lay[l - 1], &Bbar[l], &dBbar[l]);
+        double Bsfc, dBsfc;
+        orc_planck_drv(vn, tskin, &Bsfc, &dBsfc);
+        double rad = 0., radsun = 0., dEmis = 0., dTsfc = 0., dRsfc = 0.;
+        for (int l = 1; l <= L; l++) { dTau[l] = 0.; dTbar[l] = 0.; }
+        for (int k = 1; k <= L + 1; k++) { dTup[k] = 0.; dTdn[k] = 0.; dTlev[k] = 0.; }
+        int obsLay = obsLev;
+        double rsfc = 0.;
+        if (uplook) {
+            for (int k = N1; k <= obsLev; k++) cumTxB[k] = 1.0;
+            double cumODB = 0.0;
+            obsLay = obsLay > N1 ? obsLay : N1;
+            for (int l = obsLay; l <= N2; l++) {
+                cumODB = cumODB + vOD[l];
+                cumTxB[l + 1] = exp(-cumODB);
+            }
+            for (int l = N2; l >= obsLay; l--) {
+                const double dTran = cumTxB[l] - cumTxB[l + 1];
+                double Blt;
+                if (lin != 0) {
+                    double h, dh;
+                    orc_rlin(vOD[l], &h, &dh);
+                    if (lin == 2) {
+                        Blt = Blev[l] * (1 - h) + Blev[l + 1] * h;
+                        dTau[l] = (cumTxB[l + 1] * Blt - rad + dTran * dh * (Blev[l + 1] - Blev[l])) * amf_view[l - 1];
+                        dTdn[l + 1] = dTdn[l + 1] + dTran * dBlev[l + 1] * h;
+                        dTdn[l] = dTran * dBlev[l] * (1.0 - h);
+                        dTbar[l] = 0.0;
+                    } else {
+                        Blt = 2.0 * Bbar[l] * h + Blev[l] * (1.0 - 2.0 * h);
+                        dTau[l] = (cumTxB[l + 1] * Blt - rad + 2.0 * dTran * dh * (Bbar[l] - Blev[l])) * amf_view[l - 1];
+                        dTdn[l] = dTran * dBlev[l] * (1.0 - 2.0 * h);
+                        dTbar[l] = dTran * 2.0 * dBbar[l] * h;
+                    }
+                    rad = rad + dTran * Blt;
+                } else {
+                    dTau[l] = (cumTxB[l + 1] * Bbar[l] - rad) * amf_view[l - 1];
+                    rad = dTran * Bbar[l] + rad;
+                    dTbar[l] = dTran * dBbar[l];
+                }
+            }
+        } else {
+            /* 0- view-path transmittance from the observer down to the surface */
+            double cumODA = 0.;
+            for (int k = obsLev; k <= N2 + 1; k++) cumTxA[k] = 1.;
+            for (int l = obsLay - 1; l >= N1; l--) {
+                cumODA = cumODA + vOD[l];
+                cumTxA[l] = exp(-cumODA);
+            }
+            const double totODUp = cumODA, totTxUp = cumTxA[N1];
+            /* 1- downwelling thermal radiance */
+            if (totTxUp > 1.e-06) {
+                cumTxB2[N1] = totTxUp;
+                double cumODB = 0.;
+                for (int l = N1; l <= N2; l++) {
+                    cumODB = cumODB + dOD[l];
+                    cumTxB2[l + 1] = exp(-(totODUp + cumODB));
+                }
+                for (int l = N2; l >= N1; l--) {
+                    const double dTran = cumTxB2[l] - cumTxB2[l + 1];
+                    double Blt;
+                    if (lin != 0) {
+                        double h, dh;
+                        orc_rlin(dOD[l], &h, &dh);
+                        if (lin == 2) {
+                            Blt = Blev[l] + (Blev[l + 1] - Blev[l]) * h;
+                            dTau[l] = (cumTxB2[l + 1] * Blt - rad + dTran * dh * (Blev[l + 1] - Blev[l])) *
+                                      amf_down[l - 1];
+                            dTdn[l + 1] = dTdn[l + 1] + dTran * dBlev[l + 1] * h;
+                            dTdn[l] = dTran * dBlev[l] * (1. - h);
+                            dTbar[l] = 0.0;
+                        } else {
+                            Blt = 2.0 * Bbar[l] * h + Blev[l] * (1.0 - 2.0 * h);
+                            dTau[l] = (cumTxB2[l + 1] * Blt - rad + 2.0 * dTran * dh * (Bbar[l] - Blev[l])) *
+                                      amf_down[l - 1];
+                            dTdn[l] = dTran * dBlev[l] * (1.0 - 2.0 * h);
+                            dTbar[l] = dTran * 2.0 * dBbar[l] * h;
+                        }
+                        rad = rad + dTran * Blt;
+                    } else {
+                        dTau[l] = (cumTxB2[l + 1] * Bbar[l] - rad) * amf_down[l - 1];
+                        rad = rad + dTran * Bbar[l];
+                        dTbar[l] = dTran * dBbar[l];
+                    }
+                }
+            }
+            /* 2- scale by the surface reflectance */
+            rsfc = (1. - emis);
+            for (int l = N1; l <= N2; l++) {
+                dTau[l] = dTau[l] * rsfc;
+                dTbar[l] = dTbar[l] * rsfc;
+            }
+            /* 3- surface terms */
+            dEmis = totTxUp * Bsfc - rad;
+            dTsfc = emis * totTxUp * dBsfc;
+            const double sfcEmit = emis * totTxUp * Bsfc;
+            rad = rad * rsfc + sfcEmit;
+            /* 4- solar */
+            if (solar) {
+                double totODSun = 0.;
+                for (int l = N1; l <= N2; l++) totODSun = totODSun + sOD[l];
+                const double totTxSun2 = exp(-(totODUp + totODSun));
+                dRsfc = totTxSun2 * sol_toa[i];
+                radsun = rsfc_sun * dRsfc;
+                for (int l = N1; l <= N2; l++) dTau[l] = dTau[l] - radsun * amf_sun[l - 1];
+                rad = rad + radsun;
+            }
+            /* 5- upwelling thermal radiance of the layers below the observer */
+            for (int l = N1; l <= obsLay - 1; l++) {
+                const double dTran = cumTxA[l + 1] - cumTxA[l];
+                double Blt;
+                if (lin != 0) {
+                    double h, dh;
+                    orc_rlin(vOD[l], &h, &dh);
+                    if (lin == 2) {
+                        Blt = Blev[l + 1] * (1 - h) + Blev[l] * h;
+                        dTau[l] = dTau[l] + (cumTxA[l] * Blt - rad + dTran * dh * (Blev[l] - Blev[l + 1])) *
+                                                amf_view[l - 1];
+                        dTup[l + 1] = dTran * dBlev[l + 1] * (1 - h);
+                        dTup[l] = dTup[l] + dTran * dBlev[l] * h;
+                    } else {
+                        Blt = 2.0 * Bbar[l] * h + Blev[l + 1] * (1.0 - 2.0 * h);
+                        dTau[l] = dTau[l] + (cumTxA[l] * Blt - rad + dTran * dh * 2.0 * (Bbar[l] - Blev[l + 1])) *
+                                                amf_view[l - 1];
+                        dTup[l + 1] = dTran * dBlev[l + 1] * (1.0 - 2.0 * h);
+                        dTbar[l] = dTbar[l] + dTran * 2.0 * dBbar[l] * h;
+                    }
+                    rad = rad + dTran * Blt;
+                } else {
+                    dTau[l] = dTau[l] + (cumTxA[l] * Bbar[l] - rad) * amf_view[l - 1];
+                    rad = rad + dTran * Bbar[l];
+                    dTbar[l] = dTbar[l] + dTran * dBbar[l];
+                }
+            }
+        }
+        /* level temperature derivatives */
+        if (lin != 0) {
+            if (uplook)
+                for (int k = obsLev; k <= N2 + 1; k++) dTlev[k] = dTdn[k];
+            else
+                for (int k = N1; k <= N2 + 1; k++) dTlev[k] = dTup[k] + dTdn[k] * rsfc;
+        }
+        if (lin == 0 || lin == 1) {
+            const int L2 = uplook ? obsLay : N1;
+            for (int l = N2; l >= L2; l--) {
+                dTlev[l + 1] = dTlev[l + 1] + dTbar[l] * dt_dtup[l - 1];
+                dTlev[l] = dTlev[l] + dTbar[l] * dt_dtlo[l - 1];
+            }
+        }
+        rad_o[i] = rad;
+        if (dr_dtau)
+            for (int l = 1; l <= L; l++) dr_dtau[(int64_t)(l - 1) * npts + i] = dTau[l];
+        if (dr_dtlev)
+            for (int k = 1; k <= L + 1; k++) dr_dtlev[(int64_t)(k - 1) * npts + i] = dTlev[k];
+        if (dr_dtskin) dr_dtskin[i] = dTsfc;
+        if (dr_demis) dr_demis[i] = dEmis;
+        if (dr_drsfc) dr_drsfc[i] = dRsfc;
+    }
+    free(Blev); free(dBlev); free(Bbar); free(dBbar); free(cumTxA); free(cumTxB); free(cumTxB2);
+    free(dTau); free(dTbar); free(dTup); free(dTdn); free(dTlev); free(vOD); free(dOD); free(sOD);
+}
+
 /* drv_LevelTemp and drv_LevelConc (clblm_noScattJac.f90:899-1203): level Jacobians
  * from the per-point derivatives and the layer-OD variants.  od_var holds 3 + nmol
  * blocks of [nlay][npts]: the layer OD, the OD at Tave + dt, the OD at

@@ -103,6 +103,11 @@ void orc_jacobians(int nlay, int64_t npts, double v1, double dv, const double *o
                    const double *t_lev, double emis, double tskin, int lin, double sec_dn, const double *dt_dtup,
                    const double *dt_dtlo, double *rad, double *dr_dtau, double *dr_dtlev, double *dr_dtskin,
                    double *dr_demis);
+void orc_jacobians_paths(int nlay, int64_t npts, double v1, double dv, const double *od, const double *amf_view,
+                         const double *amf_down, const double *amf_sun, const double *t_lay, const double *t_lev,
+                         double emis, double tskin, int solar, const double *sol_toa, double rsfc_sun, int obs,
+                         int uplook, int lin, const double *dt_dtup, const double *dt_dtlo, double *rad_o,
+                         double *dr_dtau, double *dr_dtlev, double *dr_dtskin, double *dr_demis, double *dr_drsfc);
 void orc_level_jac(int nlay, int nmol, int64_t npts, const double *od_var, const double *dr_dtau,
                    const double *dr_dtlev, const double *wtot, const double *tave, double dt, double dw_frac,
                    const double *dt_dtup, const double *dt_dtlo, const double *dw_dqup, const double *dw_dqlo,

@@ -278,6 +278,32 @@ def jacobians(od, t_lay, t_lev, v1, dv, emis=0.98, tskin=295.0, lin=1, sec_dn=1.
     return rad, dtau, dtlev, dts, de
 
 
+def jacobians_paths(od, t_lay, t_lev, v1, dv, amf_view=None, amf_down=None, amf_sun=None, emis=0.98, tskin=295.0,
+                    lin=1, solar=False, sol_toa=None, rsfc_sun=0.0, obs=None, uplook=False, dt_dtup=None,
+                    dt_dtlo=None):
+    """orc_jacobians_paths: Jacobians with every branch (air-mass factors per layer,
+    observer level, up- or down-looking, solar).  Returns a dict: rad [n], dtau
+    [nlay, n], dtlev [nlay+1, n], dtskin, demis, drsfc [n]."""
+    L = lib()
+    od = np.ascontiguousarray(od, np.float64)
+    nlay, n = od.shape
+    f = lambda a, d: np.full(nlay, d) if a is None else np.ascontiguousarray(a, np.float64)
+    av, ad, asun = f(amf_view, 1.0), f(amf_down, 1.0), f(amf_sun, 1.0)
+    up, lo = f(dt_dtup, 0.0), f(dt_dtlo, 0.0)
+    sol = np.zeros(n) if sol_toa is None else np.ascontiguousarray(sol_toa, np.float64)
+    t_lay = np.ascontiguousarray(t_lay, np.float64)
+    t_lev = np.ascontiguousarray(t_lev, np.float64)
+    out = dict(rad=np.zeros(n), dtau=np.zeros((nlay, n)), dtlev=np.zeros((nlay + 1, n)), dtskin=np.zeros(n),
+               demis=np.zeros(n), drsfc=np.zeros(n))
+    D, P, I = ctypes.c_double, ctypes.c_void_p, ctypes.c_int
+    L.orc_jacobians_paths.argtypes = [I, ctypes.c_int64, D, D, P, P, P, P, P, P, D, D, I, P, D, I, I, I, P, P] + [P] * 6
+    L.orc_jacobians_paths(nlay, n, v1, dv, _p(od), _p(av), _p(ad), _p(asun), _p(t_lay), _p(t_lev), emis, tskin,
+                          1 if solar else 0, _p(sol), rsfc_sun, nlay if obs is None else int(obs), 1 if uplook else 0,
+                          lin, _p(up), _p(lo), *[_p(out[k]) for k in ("rad", "dtau", "dtlev", "dtskin", "demis",
+                                                                     "drsfc")])
+    return out
+
+
 def level_jac(od_var, dr_dtau, dr_dtlev, wtot, tave, dt, dw_frac, dt_dtup, dt_dtlo, dw_dqup, dw_dqlo):
     """drv_LevelTemp + drv_LevelConc: od_var [3+nmol, nlay, n].  Returns jac_t [nlay+1, n]
     and jac_q [nmol, nlay+1, n]."""
