@@ -2464,21 +2464,28 @@ __global__ void k_mtckd(const MtJob *jobs, const double *coef, double *absrb) {
     *out = *out + cself + cfor;
 }
 
-// ---- CONTNM's other continua of the thermal infrared -------------------------------
-// (clblm_ODLAY_Continuum.f90:408-1060).  Each tabulated band (CO2 with FRNCO2 :1333-1400
-// and the XFACCO2 scaling :415-457, the O2 fundamental o2_ver_1 :7187-7255, the N2 pure
-// rotation xn2_r :2533-2600 and fundamental n2_ver_1 :2701-2790) is the window of its
-// table around [V1ABS, V2ABS] that the reference's routine builds, scaled to the layer
-// as CONTNM scales it; the windows are built on the host (the C library's pow and exp,
-// as the reference's build evaluates them, and RADFN where JRAD is 1).  The device adds
-// the bands in CONTNM's order into ABSRB, each by XINT (clblm_Spectrum.f90:874-914),
-// then the Rayleigh extinction of every point (:970-1060, no aerosols).
+// ---- CONTNM's continua other than water vapour --------------------------------------
+// (clblm_ODLAY_Continuum.f90:408-1060).  Each tabulated band is the window of its table
+// around [V1ABS, V2ABS] that the reference's routine builds, scaled to the layer as
+// CONTNM scales it: CO2 (FRNCO2 :1333-1400, XFACCO2 :415-457); O3 Chappuis (XO3CHP
+// :3061-3205), Hartley-Huggins (O3HHT0-2 :5210-7085) and UV (O3HHUV :7090-7175); the O2
+// fundamental (o2_ver_1 :7187-7310), 1.27 um (O2INF1 :7327-7488), 1.06 um (O2INF2
+// :7494-7560), A-band (O2INF3 :7573-7682), visible (O2_vis :7699-8090), Herzberg (O2HERZ,
+// HERTDA, HERPRS :8095-8253) and far-UV (O2FUV :8257-8650) continua; the N2 rotation
+// (xn2_r :2533-2600), fundamental (n2_ver_1 :2701-2790) and first overtone
+// (n2_overtone1 :2956-3056).  The windows are built on the host (the C library's pow,
+// exp and log, as the reference's build evaluates them, and RADFN where JRAD is 1).  The
+// device adds the bands in CONTNM's order into ABSRB, each by XINT (clblm_Spectrum.f90
+// :874-914) over the ABSRB points [n1, n2] it may touch: pre_xint's bounds for the A-band
+// and the far UV, and, for ozone, the points CONTNM's save and restore of ABSRB around
+// 40800 cm-1 leave alone (:505-525, :545-565).  Then the Rayleigh extinction of every
+// point (:970-1060, no aerosols).
 #include "clblm_contnm.inc"
 
 struct CnTerm {
     double v1c, v2c, dvc;
     int64_t coef;      // its window (NPTC values, then one zero) in the batch's array
-    int32_t nptc, pad;
+    int32_t nptc, n1, n2, pad;   // XINT's N1R3, N2R3
 };
 
 struct CnJob {
@@ -2488,41 +2495,64 @@ struct CnJob {
 };
 
 // The table window of CONTNM's band routines around [V1ABS, V2ABS] (the same lines in
-// FRNCO2, xn2_r, n2_ver_1 and o2_ver_1); returns NPTC.
-static int cn_window(double V1ABS, double V2ABS, const double *grid, double *V1C, double *V2C, double *DVC,
-                     int *I1) {
-    const double V1S = grid[0], DVS = grid[2];
-    const int NPTS = (int)grid[3];
+// each): returns NPTC.  eps is the offset of the index truncation (0.01; O2FUV 1e-5);
+// O2HERZ does not clip NPTC at its table length (clip false).
+static int cn_window_g(double V1ABS, double V2ABS, double V1S, double DVS, int NPTS, double eps, bool clip,
+                       double *V1C, double *V2C, double *DVC, int *I1) {
     *DVC = DVS;
     double v1c = V1ABS - *DVC, v2c = V2ABS + *DVC;
-    const int i1 = v1c < V1S ? -1 : (int)((v1c - V1S) / DVS + 0.01);
+    const int i1 = v1c < V1S ? -1 : (int)((v1c - V1S) / DVS + eps);
     v1c = V1S + DVS * (double)(i1 - 1);
-    const int i2 = (int)((v2c - V1S) / DVS + 0.01);
+    const int i2 = (int)((v2c - V1S) / DVS + eps);
     int nptc = i2 - i1 + 3;
-    if (nptc > NPTS) nptc = NPTS + 4;
+    if (clip && nptc > NPTS) nptc = NPTS + 4;
     *V1C = v1c;
     *V2C = v1c + DVS * (double)(nptc - 1);
     *I1 = i1;
     return nptc;
 }
 
-// One layer's bands.  CLBLM_ERR_UNSUPPORTED where a band CONTNM would add beyond these
-// is reached with its factor above 0.
+static int cn_window(double V1ABS, double V2ABS, const double *grid, double *V1C, double *V2C, double *DVC,
+                     int *I1) {
+    return cn_window_g(V1ABS, V2ABS, grid[0], grid[2], (int)grid[3], 0.01, true, V1C, V2C, DVC, I1);
+}
+
+// pre_xint (clblm_Spectrum.f90:917-944): the ABSRB points XINT may touch for a table on
+// [v1ss, v2ss].
+static void cn_pre_xint(double v1ss, double v2ss, double v1abs, int nptabs, int32_t *ist, int32_t *last) {
+    const double dvabs = 1.;
+    const int nbnd_v1c = (int)(2. + (v1ss - v1abs) / dvabs + 1.e-5);
+    *ist = std::max(1, nbnd_v1c);
+    const int nbnd_v2c = (int)(1. + (v2ss - v1abs) / dvabs + 1.e-5);
+    *last = std::min(nptabs, nbnd_v2c);
+}
+
+// HERTDA with HERPRS (clblm_ODLAY_Continuum.f90:8164-8253): the Herzberg O2 continuum at
+// V with its pressure dependence.
+static double cn_herzberg(double V, double T, double P) {
+    double HERZ = 0.;
+    if (V > 36000.00) {
+        double CORR = 0.;
+        if (V <= 40000.) CORR = ((40000. - V) / 4000.) * 7.917E-27;
+        const double YRATIO = V / 48811.0;
+        const double ly = std::log(YRATIO);
+        HERZ = 6.884E-04 * (YRATIO)*std::exp(-69.738 * (ly * ly)) - CORR;
+    }
+    const double PO = 1013., TO = 273.16;
+    return HERZ * (1. + .83 * (P / PO) * (TO / T));
+}
+
+// One layer's bands.  CLBLM_ERR_ARG where ABSRB is longer than the reference's N_ABSRB
+// (5050 points), the length of CONTNM's arrays.
 static int cn_plan(const double *scale, const clblm_layer &lay, double V1, double V2, int jrad, CnJob &cj,
                    std::vector<CnTerm> &terms, std::vector<double> &coef, bool *has) {
     *has = false;
     const double xco2c = scale[0], xo3cn = scale[1], xo2cn = scale[2], xn2cn = scale[3], xrayl = scale[4];
     const double v1 = V1, v2 = V2;
-    if (xo3cn > 0. && ((v2 > 8920.0 && v1 <= 24665.0) || (v2 > 27370. && v1 < 40800.) || (v2 > 40800. && v1 < 54000.)))
-        return set_err(CLBLM_ERR_UNSUPPORTED, "continuum: the O3 bands (from 8920 cm-1) are not part of this path");
-    if (xo2cn > 0. && ((v2 > 7536.0 && v1 < 8500.) || (v2 > 9100.0 && v1 < 11000.) ||
-                       (v2 > 12961.5 && v1 < 13221.5) || (v2 > 15000.0 && v1 < 29870.) || v2 > 36000.0))
-        return set_err(CLBLM_ERR_UNSUPPORTED, "continuum: the O2 bands from 7536 cm-1 are not part of this path");
-    if (xn2cn > 0. && (v2 > 4340.0) && (v1 < 4910.))
-        return set_err(CLBLM_ERR_UNSUPPORTED, "continuum: the N2 overtone (4340-4910 cm-1) is not part of this path");
     double V1ABS;
     int NPTABS;
     absrb_grid(V1, V2, &V1ABS, &NPTABS);
+    if (NPTABS > 5050) return set_err(CLBLM_ERR_ARG, "continuum: ABSRB longer than 5050 points");
     const double V2ABS = V1ABS + (double)(NPTABS - 1);
     const double P0C = 1013., T0C = 296., XLOSMT = 2.68675E+19;
     const double pave = lay.pave, tave = lay.tave, wtot = lay.wtot;
@@ -2534,15 +2564,21 @@ static int cn_plan(const double *scale, const clblm_layer &lay, double V1, doubl
     const double wn2 = x_vmr_n2 * wtot;
     cj.term0 = (int32_t)terms.size();
     cj.nterm = 0;
-    auto begin = [&](const double *grid, double *V1C, double *DVC, int *I1) {
+    auto begin_at = [&](double v1c, double v2c, double dvc, int nptc) {
         CnTerm t{};
-        double V2C;
-        t.nptc = cn_window(V1ABS, V2ABS, grid, V1C, &V2C, DVC, I1);
-        t.v1c = *V1C;
-        t.v2c = V2C;
-        t.dvc = *DVC;
+        t.nptc = nptc;
+        t.v1c = v1c;
+        t.v2c = v2c;
+        t.dvc = dvc;
+        t.n1 = 1;
+        t.n2 = NPTABS;
         t.coef = (int64_t)coef.size();
         return t;
+    };
+    auto begin = [&](const double *grid, double *V1C, double *DVC, int *I1) {
+        double V2C;
+        const int nptc = cn_window(V1ABS, V2ABS, grid, V1C, &V2C, DVC, I1);
+        return begin_at(*V1C, V2C, *DVC, nptc);
     };
     auto finish = [&](CnTerm &t) {
         coef.push_back(0.);
@@ -2574,6 +2610,63 @@ static int cn_plan(const double *scale, const clblm_layer &lay, double V1, doubl
         }
         finish(t);
     }
+    if (v2 > 8920.0 && v1 <= 24665.0 && xo3cn > 0.) {   // O3 Chappuis and Wulf (:464-481)
+        const double WO3 = lay.w[3] * 1.0E-20 * xo3cn;
+        CnTerm t = begin(CN_O3C_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_O3C_GRID[3];
+        const double DT = tave - 273.15;
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double c0 = 0., c1 = 0., c2 = 0.;
+            if (I >= 1 && I <= NPTS) {
+                c0 = CN_O3C_X[I - 1] / VJ;
+                c1 = CN_O3C_Y[I - 1] / VJ;
+                c2 = CN_O3C_Z[I - 1] / VJ;
+            }
+            double c = (c0 + (c1 + c2 * DT) * DT) * WO3;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    if (v2 > 27370. && v1 < 40800. && xo3cn > 0.) {   // O3 Hartley-Huggins (:483-525)
+        const double WO3 = lay.w[3] * 1.E-20 * xo3cn;
+        const double TC = tave - 273.15;
+        CnTerm t = begin(CN_O3T_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_O3T_GRID[3];
+        double VJ = 0.;
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const bool in = I >= 1 && I <= NPTS;
+            VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = in ? CN_O3T0_S[I - 1] / VJ : 0.;
+            const double ct1 = in ? CN_O3T1_S[I - 1] : 0., ct2 = in ? CN_O3T2_S[I - 1] : 0.;
+            double c = c0 * WO3;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            c = c * (1. + ct1 * TC + ct2 * TC * TC);
+            coef.push_back(c);
+        }
+        // the points from 40800 cm-1 are saved before XINT and restored after it
+        if ((VJ > 40815.) && (v2 > 40800)) t.n2 = std::min(t.n2, (int32_t)((40800. - V1ABS) / 1. + 1.001) - 1);
+        finish(t);
+    }
+    if (v2 > 40800. && v1 < 54000. && xo3cn > 0.) {   // O3 Hartley-Huggins in the UV (:530-566)
+        const double WO3 = lay.w[3] * xo3cn;
+        CnTerm t = begin(CN_O3UV_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_O3UV_GRID[3];
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_O3UV_S[I - 1] / VJ : 0.;
+            double c = c0 * WO3;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        // the points below 40800 cm-1 are saved before XINT and restored after it
+        if (v1 < 40800) t.n1 = std::max(t.n1, (int32_t)((40800. - V1ABS) / 1. + 1.001));
+        finish(t);
+    }
     if ((v2 > 1340.0) && (v1 < 1850.) && xo2cn > 0.) {   // O2 fundamental (:587-628)
         const double tau_fac = xo2cn * lay.w[7] * 1.e-20 * amagat;
         CnTerm t = begin(CN_O2F_GRID, &V1C, &DVC, &I1);
@@ -2589,6 +2682,120 @@ static int cn_plan(const double *scale, const clblm_layer &lay, double V1, doubl
             if (jrad == 1) c = c * radfn(VJ, XKT);
             coef.push_back(c);
         }
+        finish(t);
+    }
+    if ((v2 > 7536.0) && (v1 < 8500.) && xo2cn > 0.) {   // O2 1.27 um (:633-660)
+        const double a_o2 = 1. / 0.446, a_n2 = 0.3 / 0.446, a_h2o = 1.;
+        const double tau_fac =
+            xo2cn * (lay.w[7] / XLOSMT) * amagat * (a_o2 * x_vmr_o2 + a_n2 * x_vmr_n2 + a_h2o * x_vmr_h2o);
+        CnTerm t = begin(CN_O2I1_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_O2I1_GRID[3];
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_O2I1_X[I - 1] / VJ : 0.;
+            double c = tau_fac * c0;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    if ((v2 > 9100.0) && (v1 < 11000.) && xo2cn > 0.) {   // O2 1.06 um (:670-690), O2INF2's own grid
+        const double V1_osc = 9375., HW1 = 58.96, V2_osc = 9439., HW2 = 45.04;
+        const double S1 = 1.166E-04, S2 = 3.086E-05;
+        const double V1S = 9100., v2s = 11000., DVS = 2.;
+        DVC = DVS;
+        V1C = V1ABS - DVC;
+        double V2C = V2ABS + DVC;
+        if (V1C < V1S) V1C = V1S - 2. * DVS;
+        if (V2C > v2s) V2C = v2s + 2. * DVS;
+        const int nptc = (int)((V2C - V1C) / DVC + 3.01);
+        V2C = V1C + DVC * (double)(nptc - 1);
+        CnTerm t = begin_at(V1C, V2C, DVC, nptc);
+        const double WO2 = xo2cn * (lay.w[7] * 1.e-20) * RHOAVE;
+        const double ADJWO2 = (lay.w[7] / wtot) * (1. / 0.209) * WO2;
+        for (int J = 1; J <= nptc; J++) {
+            double c0 = 0.;
+            const double VJ = V1C + DVC * (double)(J - 1);
+            if ((VJ > V1S) && (VJ < v2s)) {
+                const double DV1 = VJ - V1_osc, DV2 = VJ - V2_osc;
+                const double DAMP1 = DV1 < 0.0 ? std::exp(DV1 / 176.1) : 1.0;
+                const double DAMP2 = DV2 < 0.0 ? std::exp(DV2 / 176.1) : 1.0;
+                const double q1 = DV1 / HW1, q2 = DV2 / HW2;
+                const double O2INF =
+                    0.31831 * (((S1 * DAMP1 / HW1) / (1. + q1 * q1)) + ((S2 * DAMP2 / HW2) / (1. + q2 * q2))) * 1.054;
+                c0 = O2INF / VJ;
+            }
+            double c = c0 * ADJWO2;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    if ((v2 > 12961.5) && (v1 < 13221.5) && xo2cn > 0.) {   // O2 A-band (:700-719)
+        const double tau_fac = xo2cn * (lay.w[7] / XLOSMT) * amagat;
+        CnTerm t = begin(CN_O2I3_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_O2I3_GRID[3];
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_O2I3_X[I - 1] / VJ : 0.;
+            double c = tau_fac * c0;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        cn_pre_xint(CN_O2I3_GRID[0], CN_O2I3_GRID[1], V1ABS, NPTABS, &t.n1, &t.n2);
+        finish(t);
+    }
+    if ((v2 > 15000.0) && (v1 < 29870.) && xo2cn > 0.) {   // O2 visible (:734-758)
+        const double WO2 = lay.w[7] * 1.e-20 * ((pave / 1013.) * (273. / tave)) * xo2cn;
+        const double CHIO2 = lay.w[7] / wtot;
+        const double ADJWO2 = CHIO2 * WO2;
+        const double tq = 55. * 273. / 296.;
+        const double factor = 1. / ((XLOSMT * 1.e-20 * (tq * tq)) * 89.5);
+        CnTerm t = begin(CN_O2V_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_O2V_GRID[3];
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? factor * CN_O2V_S[I - 1] / VJ : 0.;
+            double c = c0 * ADJWO2;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    if (v2 > 36000.0 && xo2cn > 0.) {   // O2 Herzberg (:762-779), no table length
+        const double WO2 = lay.w[7] * 1.e-20 * xo2cn;
+        double V2C;
+        const int nptc = cn_window_g(V1ABS, V2ABS, 36000., 10., 0, 0.01, false, &V1C, &V2C, &DVC, &I1);
+        CnTerm t = begin_at(V1C, V2C, DVC, nptc);
+        for (int J = 1; J <= nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = I >= 1 ? cn_herzberg(VJ, tave, pave) / VJ : 0.;
+            double c = c0 * WO2;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
+    if (v2 > 56740.0 && xo2cn > 0.) {   // O2 far UV (:784-802)
+        const double WO2 = lay.w[7] * 1.e-20 * xo2cn;
+        double V2C;
+        const int nptc = cn_window_g(V1ABS, V2ABS, CN_O2U_GRID[0], CN_O2U_GRID[2], (int)CN_O2U_GRID[3], 1.e-5, true,
+                                     &V1C, &V2C, &DVC, &I1);
+        CnTerm t = begin_at(V1C, V2C, DVC, nptc);
+        const int NPTS = (int)CN_O2U_GRID[3];
+        for (int J = 1; J <= nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_O2U_S[I - 1] / VJ : 0.;
+            double c = c0 * WO2;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        cn_pre_xint(CN_O2U_GRID[0], CN_O2U_GRID[1], V1ABS, NPTABS, &t.n1, &t.n2);
         finish(t);
     }
     if ((v2 > -10.0) && (v1 < 350.) && xn2cn > 0.) {   // N2 pure rotation (:807-852)
@@ -2638,6 +2845,21 @@ static int cn_plan(const double *scale, const clblm_layer &lay, double V1, doubl
         }
         finish(t);
     }
+    if ((v2 > 4340.0) && (v1 < 4910.) && xn2cn > 0.) {   // N2 first overtone (:946-987)
+        const double a_o2 = 1., a_h2o = 1.;
+        const double tau_fac = xn2cn * (wn2 / XLOSMT) * amagat * (x_vmr_n2 + a_o2 * x_vmr_o2 + a_h2o * x_vmr_h2o);
+        CnTerm t = begin(CN_N2O_GRID, &V1C, &DVC, &I1);
+        const int NPTS = (int)CN_N2O_GRID[3];
+        for (int J = 1; J <= t.nptc; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_N2O_X[I - 1] / VJ : 0.;
+            double c = tau_fac * c0;
+            if (jrad == 1) c = c * radfn(VJ, XKT);
+            coef.push_back(c);
+        }
+        finish(t);
+    }
     cj.ray = (v2 >= 820. && xrayl > 0.) ? 1 : 0;   // Rayleigh (:970-1060)
     cj.ray_fac = wtot * (xrayl * 1.E-20 / (2.68675e-1 * 1.e5));
     cj.v1abs = V1ABS;
@@ -2648,7 +2870,7 @@ static int cn_plan(const double *scale, const clblm_layer &lay, double V1, doubl
 }
 
 // CONTNM's bands at ABSRB point I (1-based) of job blockIdx.y, added in order: each
-// band by XINT where I lies in its range, then the Rayleigh extinction.
+// band by XINT where I lies in its range and in [n1, n2], then the Rayleigh extinction.
 __global__ void k_contnm(const CnJob *jobs, const CnTerm *terms, const double *coef, double *absrb) {
     const CnJob J = jobs[blockIdx.y];
     const int I = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -2658,11 +2880,11 @@ __global__ void k_contnm(const CnJob *jobs, const CnTerm *terms, const double *c
     for (int t = 0; t < J.nterm; t++) {
         const CnTerm T = terms[J.term0 + t];
         int ilo = (int)((T.v1c + T.dvc - J.v1abs) / 1. + 1. + ONEMI);
-        if (ilo < 1) ilo = 1;
+        if (ilo < T.n1) ilo = T.n1;
         int ihi = (int)((T.v2c - T.dvc - J.v1abs) / 1. + ONEMI);
-        if (ihi > J.nptabs) ihi = J.nptabs;
+        if (ihi > T.n2) ihi = T.n2;
         if (I >= ilo && I <= ihi)
-            a = a + xint_point(T.v1c, T.v2c, T.dvc, coef + T.coef - 1, J.v1abs, 1., I, 1, J.nptabs);
+            a = a + xint_point(T.v1c, T.v2c, T.dvc, coef + T.coef - 1, J.v1abs, 1., I, T.n1, T.n2);
     }
     if (J.ray) {
         const double vrayleigh = J.v1abs + (double)(I - 1) * 1.;

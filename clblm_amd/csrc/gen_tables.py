@@ -16,8 +16,10 @@ Tables written (reference file:line):
   * isotopologue masses for molecules 1-7      src/common/clblm_ConstParam.f90:260-285
   * MaxISOTPL_smass for molecules 1-7          src/common/clblm_ConstParam.f90:246-251
 
-and ``clblm_contnm.inc``: the continuum tables of CONTNM's thermal-infrared bands
-(CO2, N2 pure rotation and fundamental, O2 fundamental), listed at CONTNM_TABLES.
+and ``clblm_contnm.inc``: the continuum tables of CONTNM's bands other than water
+vapour (CO2; the N2 pure rotation, fundamental and first overtone; the O2 fundamental,
+1.27 um, A-band, visible and far-UV continua; ozone's Chappuis and Hartley-Huggins
+bands), listed at CONTNM_TABLES.
 
 Run in a container that holds the reference sources:
     python clblm_amd/csrc/gen_tables.py /root/reference
@@ -164,7 +166,34 @@ CONTNM_TABLES = [
     ("CN_O2F_GRID", "o2_ver_1", "V1S", 4),
     ("CN_O2F_X", "o2_ver_1", "xo2", 103),
     ("CN_O2F_XT", "o2_ver_1", "xo2t", 103),
+    ("CN_N2O_GRID", "n2_overtone1", "V1S", 4),
+    ("CN_N2O_X", "n2_overtone1", "xn2", 201),
+    ("CN_O2I1_GRID", "O2INF1", "V1S", 4),
+    ("CN_O2I1_X", "O2INF1", "xo2inf1", 483),
+    ("CN_O2I3_GRID", "O2INF3", "V1S", 4),
+    ("CN_O2I3_X", "O2INF3", "xo2inf3", 261),
+    ("CN_O2V_GRID", "O2_vis", "V1s", 4),
+    ("CN_O2V_S", "O2_vis", "S", 1488),
+    ("CN_O2U_GRID", "O2FUV", "V1S", 4),
+    ("CN_O2U_S", "O2FUV", "S", 1512),
+    ("CN_O3C_GRID", "XO3CHP", "V1S", 4),
+    ("CN_O3C_X", "XO3CHP", "X", 3150),
+    ("CN_O3C_Y", "XO3CHP", "Y", 3150),
+    ("CN_O3C_Z", "XO3CHP", "Z", 3150),
+    ("CN_O3T_GRID", "O3HHT0", "V1S", 4),
+    ("CN_O3T0_S", "O3HHT0", "S", 2687),
+    ("CN_O3T1_S", "O3HHT1", "S", 2687),
+    ("CN_O3T2_S", "O3HHT2", "S", 2687),
+    ("CN_O3UV_GRID", "O3HHUV", "V1S", 4),
+    ("CN_O3UV_S", "O3HHUV", "S", 133),
 ]
+
+# Arrays declared with a lower bound other than 1.
+LOWER_BOUND = {"tdep_bandhead": 1196}
+
+# The routines whose table grid must equal another's: O3HHT1 and O3HHT2 fill CT1(J)
+# and CT2(J) on their own windows, which CONTNM indexes with O3HHT0's J.
+SAME_GRID = [("O3HHT1", "O3HHT0"), ("O3HHT2", "O3HHT0")]
 
 
 def subroutine_text(src, name):
@@ -182,14 +211,64 @@ def data_values(text, name):
     return _numbers(m.group(1).replace("&", " "))
 
 
+def _data_items(body):
+    """The constants of a DATA value list, with Fortran's repeat form `r*value` expanded."""
+    out = []
+    for tok in body.replace("&", " ").split(","):
+        tok = tok.strip()
+        if not tok:
+            continue
+        m = re.fullmatch(r"(\d+)\s*\*\s*(%s)" % NUM, tok)
+        if m:
+            out.extend([float(m.group(2).replace("D", "E").replace("d", "e"))] * int(m.group(1)))
+        elif re.fullmatch(NUM, tok):
+            out.append(float(tok.replace("D", "E").replace("d", "e")))
+        else:
+            raise RuntimeError("DATA item %r not understood" % tok)
+    return out
+
+
+def data_array(text, name, n, lo=1):
+    """Array `name(lo:lo+n-1)` from every DATA statement of `text` that fills part of it:
+    `DATA name/.../`, `DATA name(a:b)/.../` and `DATA (name(I),I=a,b)/.../`.  Every element
+    must be set exactly once."""
+    vals = [None] * n
+    pat = (r"\bDATA\s+(?:%s\s*(?:\(\s*(\d+)\s*:\s*(\d+)\s*\))?|\(\s*%s\s*\(\s*(\w+)\s*\)\s*,\s*\3\s*=\s*"
+           r"(\d+)\s*,\s*(\d+)\s*\))\s*/(.*?)/") % (name, name)
+    found = False
+    for m in re.finditer(pat, text, re.S | re.I):
+        found = True
+        a, b = (m.group(1), m.group(2)) if m.group(1) else (m.group(4), m.group(5))
+        a, b = (int(a), int(b)) if a else (lo, lo + n - 1)
+        items = _data_items(m.group(6))
+        if len(items) != b - a + 1 or a < lo or b > lo + n - 1:
+            raise RuntimeError("DATA %s(%d:%d): %d values" % (name, a, b, len(items)))
+        for k, v in enumerate(items):
+            if vals[a - lo + k] is not None:
+                raise RuntimeError("%s(%d) set twice" % (name, a + k))
+            vals[a - lo + k] = v
+    if not found or any(v is None for v in vals):
+        raise RuntimeError("DATA %s: not every element set" % name)
+    return vals
+
+
 def contnm_tables(ref_root):
     src = open(os.path.join(ref_root, "src/clblm_src/clblm_ODLAY_Continuum.f90")).read()
     out = {}
     for cname, sub, dname, n in CONTNM_TABLES:
-        vals = data_values(subroutine_text(src, sub), dname)[:n]
+        text = subroutine_text(src, sub)
+        if cname.endswith("_GRID") or n == 1:
+            vals = data_values(text, dname)[:n]
+        else:
+            vals = data_array(text, dname, n, LOWER_BOUND.get(dname, 1))
         if len(vals) != n:
             raise RuntimeError("%s %s: expected %d values, got %d" % (sub, dname, n, len(vals)))
         out[cname] = vals
+    for sub, like in SAME_GRID:
+        a = data_values(subroutine_text(src, sub), "V1S")[:4]
+        b = data_values(subroutine_text(src, like), "V1S")[:4]
+        if a != b:
+            raise RuntimeError("%s grid %s differs from %s grid %s" % (sub, a, like, b))
     return out
 
 

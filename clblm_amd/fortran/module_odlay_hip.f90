@@ -12,9 +12,8 @@
 ! selects the species-by-species broadening.  With the continuum on
 ! (.not. odCtrl%contnmOff) CONTNM's continua are added: the MT_CKD water vapour (self and
 ! foreign, XSELF and XFRGN) from the reference's coefficient file, read on the first
-! call, and the CO2, O2 fundamental, N2 and Rayleigh continua of the thermal infrared
-! (XCO2C, XO2CN, XN2CN, XRAYL); a range that reaches the O3, the O2 near-infrared and
-! visible or the N2 overtone bands with their factor above 0 stops with a message.
+! call, and CONTNM's other continua: CO2, O3, the O2 and N2 collision-induced and
+! electronic bands and Rayleigh extinction (XCO2C, XO3CN, XO2CN, XN2CN, XRAYL).
 ! clblm_hip_precision selects FP64 (default) or FP32C.  Not supported: NLTE,
 ! cross-sections, lineOff.
 module Module_ODLAY

@@ -65,13 +65,14 @@ typedef struct {
  * continuum[] when contnm is also set) and carried into the OD as the reference's
  * CONTNM output is (XINT into R3). */
 #define CLBLM_OD_MTCKD 2
-/* CONTNM's other continua of the thermal infrared (clblm_ODLAY_Continuum.f90:408-1060):
- * CO2 (FRNCO2 with the XFACCO2 scaling), the O2 collision-induced fundamental, the N2
- * collision-induced pure rotation and fundamental bands and Rayleigh extinction, scaled
- * by the context's XCO2C, XO2CN, XN2CN, XRAYL (clblm_contnm_scales), added to ABSRB after
- * the water vapour in the reference's order.  A layer whose range reaches a band this
- * path leaves out (O3 from 8920 cm-1, O2 from 7536 cm-1, the N2 overtone from 4340 cm-1)
- * with that band's factor above 0 returns CLBLM_ERR_UNSUPPORTED. */
+/* CONTNM's continua other than water vapour (clblm_ODLAY_Continuum.f90:408-1060): CO2
+ * (FRNCO2 with the XFACCO2 scaling); O3 (Chappuis and Wulf, Hartley-Huggins with its
+ * temperature terms, Hartley-Huggins in the UV); O2 (collision-induced fundamental,
+ * 1.27 um, 1.06 um, A-band, visible, Herzberg and far-UV continua); N2 (collision-induced
+ * pure rotation, fundamental and first overtone); Rayleigh extinction; scaled by the
+ * context's XCO2C, XO3CN, XO2CN, XN2CN, XRAYL (clblm_contnm_scales) and added to ABSRB
+ * after the water vapour in the reference's order.  A layer whose ABSRB is longer than
+ * the reference's 5050 points (N_ABSRB) returns CLBLM_ERR_ARG. */
 #define CLBLM_OD_CONTNM 4
 
 /* ---- context ---------------------------------------------------------------- */

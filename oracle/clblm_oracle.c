@@ -3053,35 +3053,77 @@ int orc_mtckd_h2o(const double *wvn, const double *self_ref, const double *for_r
 }
 
 /* ---------------------------------------------------------------------------
- * CONTNM's continua other than water vapour over the thermal infrared
- * (clblm_ODLAY_Continuum.f90:408-1060): CO2 (FRNCO2 :1333-1400 with the XFACCO2
- * scaling :415-457), the O2 collision-induced fundamental (o2_ver_1 :7187-7255, CONTNM
- * :587-628), the N2 collision-induced pure rotation (xn2_r :2533-2600, CONTNM :807-852)
- * and fundamental (n2_ver_1 :2701-2790, CONTNM :856-909) bands, and Rayleigh extinction
- * (:970-1060, no aerosols), each added into absrb (nptabs points on v1abs + i at
- * 1 cm-1, 0-based) in that order, the tabulated bands through XINT.  wk[m]: column of
- * HITRAN molecule m (wk[1] H2O, wk[2] CO2, wk[7] O2); scale: XCO2C, XO3CN, XO2CN,
- * XN2CN, XRAYL.  The bands CONTNM adds beyond these (O3, O2 from 7536 cm-1 up, the N2
- * overtone) are not restated: returns 1 when one of them would add (its factor > 0 and
- * its range reached), 0 otherwise.
+ * CONTNM's continua other than water vapour (clblm_ODLAY_Continuum.f90:408-1060), each
+ * added into absrb (nptabs points on v1abs + i at 1 cm-1, 0-based) in CONTNM's order:
+ *   CO2 (FRNCO2 :1333-1400 with the XFACCO2 scaling, CONTNM :415-457);
+ *   O3: the Chappuis and Wulf bands (XO3CHP :3061-3205, CONTNM :464-481), the
+ *     Hartley-Huggins bands with their temperature terms (O3HHT0-2 :5210-7085, CONTNM
+ *     :483-525) and in the UV (O3HHUV :7090-7175, CONTNM :530-566), with CONTNM's save and restore
+ *     of ABSRB around 40800 cm-1;
+ *   O2: the collision-induced fundamental (o2_ver_1 :7187-7255, CONTNM :582-615), the
+ *     1.27 um band (O2INF1 :7327-7488, CONTNM :633-660), the 1.06 um band (O2INF2
+ *     :7494-7560, CONTNM :670-690), the A-band (O2INF3 :7573-7682, CONTNM :700-719, XINT
+ *     bounded by pre_xint), the visible (O2_vis :7699-8090, CONTNM :734-758), the
+ *     Herzberg continuum (O2HERZ :8095-8145 with HERTDA and HERPRS, CONTNM :762-779) and
+ *     the far UV (O2FUV :8257-8650, CONTNM :784-802, bounded by pre_xint);
+ *   N2: the collision-induced pure rotation (xn2_r :2533-2600, CONTNM :831-865), the
+ *     fundamental (n2_ver_1 :2701-2790, CONTNM :889-931) and the first overtone
+ *     (n2_overtone1 :2956-3056, CONTNM :946-987);
+ *   Rayleigh extinction (:970-1060, no aerosols).
+ * The tabulated bands go through XINT.  Parity unpinned: HERTDA's correction term (at
+ * most 7.9e-27, below double precision against the continuum it corrects) and O2FUV's
+ * 1e-5 truncation offset (the same window as 0.01 for integer V1ABS) cannot change a
+ * result, so no check reaches them.  wk[m]: column of HITRAN molecule m (wk[1] H2O,
+ * wk[2] CO2, wk[3] O3, wk[7] O2); scale: XCO2C, XO3CN, XO2CN, XN2CN, XRAYL.  Returns 0,
+ * or -1 when ABSRB is longer than the reference's N_ABSRB (5050 points).
  * --------------------------------------------------------------------------- */
 
-/* the window every table routine of CONTNM takes around [V1ABS, V2ABS]: returns NPTC */
-static int cn_window(double V1ABS, double V2ABS, const double *grid, double *V1C, double *V2C, double *DVC,
-                     int *I1) {
-    const double V1S = grid[0], DVS = grid[2];
-    const int NPTS = (int)grid[3];
+/* the window a table routine of CONTNM takes around [V1ABS, V2ABS]: returns NPTC.  eps
+ * is the offset of the index truncation (0.01; O2FUV 1e-5); O2HERZ does not clip NPTC
+ * at its table length (clip 0). */
+static int cn_window_g(double V1ABS, double V2ABS, double V1S, double DVS, int NPTS, double eps, int clip,
+                       double *V1C, double *V2C, double *DVC, int *I1) {
     *DVC = DVS;
     double v1c = V1ABS - *DVC, v2c = V2ABS + *DVC;
-    int i1 = v1c < V1S ? -1 : (int)((v1c - V1S) / DVS + 0.01);
+    int i1 = v1c < V1S ? -1 : (int)((v1c - V1S) / DVS + eps);
     v1c = V1S + DVS * (double)(i1 - 1);
-    const int i2 = (int)((v2c - V1S) / DVS + 0.01);
+    const int i2 = (int)((v2c - V1S) / DVS + eps);
     int nptc = i2 - i1 + 3;
-    if (nptc > NPTS) nptc = NPTS + 4;
+    if (clip && nptc > NPTS) nptc = NPTS + 4;
     *V1C = v1c;
     *V2C = v1c + DVS * (double)(nptc - 1);
     *I1 = i1;
     return nptc;
+}
+
+static int cn_window(double V1ABS, double V2ABS, const double *grid, double *V1C, double *V2C, double *DVC,
+                     int *I1) {
+    return cn_window_g(V1ABS, V2ABS, grid[0], grid[2], (int)grid[3], 0.01, 1, V1C, V2C, DVC, I1);
+}
+
+/* pre_xint (clblm_Spectrum.f90:917-944): the ABSRB points [ist, last] that XINT may
+ * touch for a table on [v1ss, v2ss] */
+static void cn_pre_xint(double v1ss, double v2ss, double v1abs, double dvabs, int nptabs, int *ist, int *last) {
+    const int nbnd_v1c = (int)(2. + (v1ss - v1abs) / dvabs + 1.e-5);
+    *ist = nbnd_v1c > 1 ? nbnd_v1c : 1;
+    const int nbnd_v2c = (int)(1. + (v2ss - v1abs) / dvabs + 1.e-5);
+    *last = nptabs < nbnd_v2c ? nptabs : nbnd_v2c;
+}
+
+/* HERTDA and HERPRS (clblm_ODLAY_Continuum.f90:8164-8253): the Herzberg O2 continuum at
+ * v and its pressure dependence */
+static double cn_hertda(double V) {
+    if (V <= 36000.00) return 0.;
+    double CORR = 0.;
+    if (V <= 40000.) CORR = ((40000. - V) / 4000.) * 7.917E-27;
+    const double YRATIO = V / 48811.0;
+    const double ly = log(YRATIO);
+    return 6.884E-04 * (YRATIO)*exp(-69.738 * (ly * ly)) - CORR;
+}
+
+static double cn_herprs(double HERZ, double T, double P) {
+    const double PO = 1013., TO = 273.16;
+    return HERZ * (1. + .83 * (P / PO) * (TO / T));
 }
 
 int orc_contnm_ir(double pave, double tave, double wtot, const double *wk, double v1, double v2, int jrad,
@@ -3098,7 +3140,7 @@ int orc_contnm_ir(double pave, double tave, double wtot, const double *wk, doubl
     double *R = absrb - 1;   /* 1-based */
     double C[6000], V1C, V2C, DVC;
     int I1, NPTC;
-    int beyond = 0;
+    if (nptabs > 5050) return -1;
 
     /* CO2 */
     if ((v2 > -20.0) && (v1 < 10000.) && xco2c > 0) {
@@ -3124,9 +3166,87 @@ int orc_contnm_ir(double pave, double tave, double wtot, const double *wk, doubl
         C[NPTC + 1] = 0.;
         orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
     }
-    /* O3 (diffuse, Hartley-Huggins, UV) */
-    if (xo3cn > 0. && ((v2 > 8920.0 && v1 <= 24665.0) || (v2 > 27370. && v1 < 40800.) || (v2 > 40800. && v1 < 54000.)))
-        beyond = 1;
+    /* O3: Chappuis and Wulf bands */
+    if (v2 > 8920.0 && v1 <= 24665.0 && xo3cn > 0.) {
+        const double WO3 = wk[3] * 1.0E-20 * xo3cn;
+        NPTC = cn_window(V1ABS, V2ABS, CN_O3C_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_O3C_GRID[3];
+        const double DT = tave - 273.15;
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double c0 = 0., c1 = 0., c2 = 0.;
+            if (I >= 1 && I <= NPTS) {
+                c0 = CN_O3C_X[I - 1] / VJ;
+                c1 = CN_O3C_Y[I - 1] / VJ;
+                c2 = CN_O3C_Z[I - 1] / VJ;
+            }
+            C[J] = (c0 + (c1 + c2 * DT) * DT) * WO3;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* O3: Hartley-Huggins bands, 27370-40800 cm-1, with their temperature terms; the
+     * points from 40800 cm-1 keep their value when the window reaches past 40815 */
+    if (v2 > 27370. && v1 < 40800. && xo3cn > 0.) {
+        const double WO3 = wk[3] * 1.E-20 * xo3cn;
+        const double TC = tave - 273.15;
+        NPTC = cn_window(V1ABS, V2ABS, CN_O3T_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_O3T_GRID[3];
+        double VJ = 0.;
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const int in = I >= 1 && I <= NPTS;
+            VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = in ? CN_O3T0_S[I - 1] / VJ : 0.;
+            const double ct1 = in ? CN_O3T1_S[I - 1] : 0., ct2 = in ? CN_O3T2_S[I - 1] : 0.;
+            C[J] = c0 * WO3;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+            C[J] = C[J] * (1. + ct1 * TC + ct2 * TC * TC);
+        }
+        C[NPTC + 1] = 0.;
+        const int fix = (VJ > 40815.) && (v2 > 40800);
+        int I_FIX = 0;
+        double *save = NULL;
+        if (fix) {
+            I_FIX = (int)((40800. - V1ABS) / DVABS + 1.001);
+            save = (double *)malloc(sizeof(double) * (size_t)(nptabs + 1));
+            for (int i = I_FIX; i <= nptabs; i++) save[i] = R[i];
+        }
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+        if (fix) {
+            for (int i = I_FIX; i <= nptabs; i++) R[i] = save[i];
+            free(save);
+        }
+    }
+    /* O3: Hartley-Huggins in the UV, 40800-54000 cm-1; the points below 40800 cm-1 keep
+     * their value */
+    if (v2 > 40800. && v1 < 54000. && xo3cn > 0.) {
+        const double WO3 = wk[3] * xo3cn;
+        NPTC = cn_window(V1ABS, V2ABS, CN_O3UV_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_O3UV_GRID[3];
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_O3UV_S[I - 1] / VJ : 0.;
+            C[J] = c0 * WO3;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        int I_FIX = 0;
+        double *save = NULL;
+        if (v1 < 40800) {
+            I_FIX = (int)((40800. - V1ABS) / DVABS + 1.001);
+            save = (double *)malloc(sizeof(double) * (size_t)(nptabs + 1));
+            for (int i = 1; i <= I_FIX - 1; i++) save[i] = R[i];
+        }
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+        if (v1 < 40800) {
+            for (int i = 1; i <= I_FIX - 1; i++) R[i] = save[i];
+            free(save);
+        }
+    }
     /* O2 collision-induced fundamental */
     if ((v2 > 1340.0) && (v1 < 1850.) && xo2cn > 0.) {
         const double tau_fac = xo2cn * wk[7] * 1.e-20 * amagat;
@@ -3145,11 +3265,125 @@ int orc_contnm_ir(double pave, double tave, double wtot, const double *wk, doubl
         C[NPTC + 1] = 0.;
         orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
     }
-    /* O2 near-infrared, visible and ultraviolet bands */
-    if (xo2cn > 0. && ((v2 > 7536.0 && v1 < 8500.) || (v2 > 9100.0 && v1 < 11000.) ||
-                       (v2 > 12961.5 && v1 < 13221.5) || (v2 > 15000.0 && v1 < 29870.) || v2 > 36000.0 ||
-                       v2 > 56740.0))
-        beyond = 1;
+    /* O2: the 1.27 um band (Mate et al.) */
+    if ((v2 > 7536.0) && (v1 < 8500.) && xo2cn > 0.) {
+        const double a_o2 = 1. / 0.446, a_n2 = 0.3 / 0.446, a_h2o = 1.;
+        const double tau_fac =
+            xo2cn * (wk[7] / XLOSMT) * amagat * (a_o2 * x_vmr_o2 + a_n2 * x_vmr_n2 + a_h2o * x_vmr_h2o);
+        NPTC = cn_window(V1ABS, V2ABS, CN_O2I1_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_O2I1_GRID[3];
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_O2I1_X[I - 1] / VJ : 0.;
+            C[J] = tau_fac * c0;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* O2: the 1.06 um band (Mlawer et al.), two damped Lorentzians on a 2 cm-1 grid */
+    if ((v2 > 9100.0) && (v1 < 11000.) && xo2cn > 0.) {
+        const double V1_osc = 9375., HW1 = 58.96, V2_osc = 9439., HW2 = 45.04;
+        const double S1 = 1.166E-04, S2 = 3.086E-05;
+        const double V1S = 9100., v2s = 11000., DVS = 2.;
+        DVC = DVS;
+        V1C = V1ABS - DVC;
+        V2C = V2ABS + DVC;
+        if (V1C < V1S) V1C = V1S - 2. * DVS;
+        if (V2C > v2s) V2C = v2s + 2. * DVS;
+        NPTC = (int)((V2C - V1C) / DVC + 3.01);
+        V2C = V1C + DVC * (double)(NPTC - 1);
+        const double WO2 = xo2cn * (wk[7] * 1.e-20) * RHOAVE;
+        const double ADJWO2 = (wk[7] / wtot) * (1. / 0.209) * WO2;
+        for (int J = 1; J <= NPTC; J++) {
+            double c0 = 0.;
+            const double VJ = V1C + DVC * (double)(J - 1);
+            if ((VJ > V1S) && (VJ < v2s)) {
+                const double DV1 = VJ - V1_osc, DV2 = VJ - V2_osc;
+                const double DAMP1 = DV1 < 0.0 ? exp(DV1 / 176.1) : 1.0;
+                const double DAMP2 = DV2 < 0.0 ? exp(DV2 / 176.1) : 1.0;
+                const double q1 = DV1 / HW1, q2 = DV2 / HW2;
+                const double O2INF =
+                    0.31831 * (((S1 * DAMP1 / HW1) / (1. + q1 * q1)) + ((S2 * DAMP2 / HW2) / (1. + q2 * q2))) * 1.054;
+                c0 = O2INF / VJ;
+            }
+            C[J] = c0 * ADJWO2;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* O2: the A-band (Mlawer and Gombos), XINT bounded by pre_xint */
+    if ((v2 > 12961.5) && (v1 < 13221.5) && xo2cn > 0.) {
+        const double tau_fac = xo2cn * (wk[7] / XLOSMT) * amagat;
+        NPTC = cn_window(V1ABS, V2ABS, CN_O2I3_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_O2I3_GRID[3];
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_O2I3_X[I - 1] / VJ : 0.;
+            C[J] = tau_fac * c0;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        int ist, last;
+        cn_pre_xint(CN_O2I3_GRID[0], CN_O2I3_GRID[1], V1ABS, DVABS, nptabs, &ist, &last);
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, ist, last);
+    }
+    /* O2: the visible bands (Greenblatt et al.) */
+    if ((v2 > 15000.0) && (v1 < 29870.) && xo2cn > 0.) {
+        const double WO2 = wk[7] * 1.e-20 * ((pave / 1013.) * (273. / tave)) * xo2cn;
+        const double CHIO2 = wk[7] / wtot;
+        const double ADJWO2 = CHIO2 * WO2;
+        const double t = 55. * 273. / 296.;
+        const double factor = 1. / ((XLOSMT * 1.e-20 * (t * t)) * 89.5);
+        NPTC = cn_window(V1ABS, V2ABS, CN_O2V_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_O2V_GRID[3];
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? factor * CN_O2V_S[I - 1] / VJ : 0.;
+            C[J] = c0 * ADJWO2;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* O2: the Herzberg continuum from 36000 cm-1 */
+    if (v2 > 36000.0 && xo2cn > 0.) {
+        const double WO2 = wk[7] * 1.e-20 * xo2cn;
+        NPTC = cn_window_g(V1ABS, V2ABS, 36000., 10., 0, 0.01, 0, &V1C, &V2C, &DVC, &I1);
+        if (NPTC + 2 > 6000) return -1;
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            double c0 = 0.;
+            if (I >= 1) c0 = cn_herprs(cn_hertda(VJ), tave, pave) / VJ;
+            C[J] = c0 * WO2;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
+    /* O2: the far UV (Schumann-Runge continuum) from 56740 cm-1, bounded by pre_xint */
+    if (v2 > 56740.0 && xo2cn > 0.) {
+        const double WO2 = wk[7] * 1.e-20 * xo2cn;
+        NPTC = cn_window_g(V1ABS, V2ABS, CN_O2U_GRID[0], CN_O2U_GRID[2], (int)CN_O2U_GRID[3], 1.e-5, 1, &V1C, &V2C,
+                           &DVC, &I1);
+        const int NPTS = (int)CN_O2U_GRID[3];
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_O2U_S[I - 1] / VJ : 0.;
+            C[J] = c0 * WO2;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        int ist, last;
+        cn_pre_xint(CN_O2U_GRID[0], CN_O2U_GRID[1], V1ABS, DVABS, nptabs, &ist, &last);
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, ist, last);
+    }
     /* N2 collision-induced pure rotation */
     if ((v2 > -10.0) && (v1 < 350.) && xn2cn > 0.) {
         const double a_h2o = 1.;
@@ -3200,8 +3434,22 @@ int orc_contnm_ir(double pave, double tave, double wtot, const double *wk, doubl
         C[NPTC + 1] = 0.;
         orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
     }
-    /* N2 first overtone */
-    if ((v2 > 4340.0) && (v1 < 4910.) && xn2cn > 0.) beyond = 1;
+    /* N2 first overtone (Shapiro and Gush, modified by Mlawer and Gombos) */
+    if ((v2 > 4340.0) && (v1 < 4910.) && xn2cn > 0.) {
+        const double a_o2 = 1., a_h2o = 1.;
+        const double tau_fac = xn2cn * (wn2 / XLOSMT) * amagat * (x_vmr_n2 + a_o2 * x_vmr_o2 + a_h2o * x_vmr_h2o);
+        NPTC = cn_window(V1ABS, V2ABS, CN_N2O_GRID, &V1C, &V2C, &DVC, &I1);
+        const int NPTS = (int)CN_N2O_GRID[3];
+        for (int J = 1; J <= NPTC; J++) {
+            const int I = I1 + (J - 1);
+            const double VJ = V1C + DVC * (double)(J - 1);
+            const double c0 = (I >= 1 && I <= NPTS) ? CN_N2O_X[I - 1] / VJ : 0.;
+            C[J] = tau_fac * c0;
+            if (jrad == 1) C[J] = C[J] * orc_radfn(VJ, XKT);
+        }
+        C[NPTC + 1] = 0.;
+        orc_xint(V1C, V2C, DVC, C, 1.0, V1ABS, DVABS, R, 1, nptabs);
+    }
     /* Rayleigh extinction (IAERSL 0) */
     if (v2 >= 820. && xrayl > 0.) {
         const double conv_cm2mol = xrayl * 1.E-20 / (2.68675e-1 * 1.e5);
@@ -3214,7 +3462,7 @@ int orc_contnm_ir(double pave, double tave, double wtot, const double *wk, doubl
             R[i] = R[i] + ray_ext;
         }
     }
-    return beyond;
+    return 0;
 }
 
 /* ---------------------------------------------------------------------------

@@ -123,10 +123,10 @@ class Oracle:
             jr = self.jrad
             if v1 / (layer.tave / 1.4387752) < 5. and jr != -1:
                 jr = 0
-            continuum, beyond = contnm_ir(layer.pave, layer.tave, layer.wtot, layer.w, v1, v2, jr, contnm,
-                                          absrb=continuum)
-            if beyond:
-                raise RuntimeError("contnm: a band beyond the thermal infrared is reached")
+            continuum, rc = contnm_ir(layer.pave, layer.tave, layer.wtot, layer.w, v1, v2, jr, contnm,
+                                      absrb=continuum)
+            if rc:
+                raise RuntimeError("contnm: ABSRB longer than 5050 points")
         if continuum is not None:
             absrb = np.ascontiguousarray(continuum, np.float64)
             npt = absrb.size
@@ -452,10 +452,12 @@ def absrb_grid(v1, v2):
 
 
 def contnm_ir(pave, tave, wtot, wk, v1, v2, jrad, scale=(1.0, 1.0, 1.0, 1.0, 1.0), absrb=None):
-    """orc_contnm_ir: CONTNM's CO2, O2 fundamental, N2 rotation and fundamental and
-    Rayleigh continua of one layer on OPDPTH's ABSRB grid, added to absrb (zeros by
-    default).  wk: {HITRAN molecule: column}.  Returns (absrb, beyond): beyond is 1
-    when a band this restatement leaves out would add."""
+    """orc_contnm_ir: CONTNM's continua other than water vapour (CO2; O3 Chappuis,
+    Hartley-Huggins and UV; the O2 fundamental, 1.27 um, 1.06 um, A-band, visible,
+    Herzberg and far-UV continua; the N2 rotation, fundamental and overtone; Rayleigh) of
+    one layer on OPDPTH's ABSRB grid, added to absrb (zeros by default).  wk: {HITRAN
+    molecule: column}.  Returns (absrb, rc): rc is -1 when ABSRB is longer than the
+    reference's 5050 points, else 0."""
     L = lib()
     D, P = ctypes.c_double, ctypes.c_void_p
     L.orc_contnm_ir.restype = ctypes.c_int

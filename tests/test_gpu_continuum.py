@@ -114,8 +114,11 @@ def test_mtckd_needs_a_table():
         c.close()
 
 
-# ---- CONTNM's other bands of the thermal infrared (CO2, O2, N2, Rayleigh) ----------
-CN_BANDS = [(500.0, 2500.0), (20.3, 340.7), (1300.0, 1900.0), (1990.0, 2910.0), (2300.0, 2480.0), (830.0, 1000.0)]
+# ---- CONTNM's other bands (CO2, O3, O2, N2, Rayleigh) ------------------------------
+CN_BANDS = [(500.0, 2500.0), (20.3, 340.7), (1300.0, 1900.0), (1990.0, 2910.0), (2300.0, 2480.0), (830.0, 1000.0),
+            (4300.0, 4950.0), (7500.0, 8550.0), (9050.0, 11050.0), (12900.0, 13300.0), (12970.0, 13210.0),
+            (13000.0, 18000.0), (27300.0, 30000.0), (39000.0, 42000.0), (39000.0, 40801.0), (35900.0, 38000.0),
+            (56700.0, 58000.0), (57013.0, 58000.0)]
 
 
 @pytest.mark.parametrize("jrad", [1, 0, -1])
@@ -152,34 +155,35 @@ def test_contnm_scales(ctx):
 
 
 @pytest.mark.parametrize("k", [0, 30, 59])
-def test_od_with_all_continua_matches_oracle(ctx, table, k):
-    """The OD with CONTNM's water vapour and other bands against the oracle's OD over
-    the N2 fundamental and the CO2 band head."""
+@pytest.mark.parametrize("band", [(2000.0, 2600.0, 2250.0, 2450.0), (12800.0, 13400.0, 12950.0, 13150.0),
+                                  (9300.0, 9700.0, 9400.0, 9550.0)])
+def test_od_with_all_continua_matches_oracle(ctx, table, k, band):
+    """The OD with CONTNM's water vapour and other bands against the oracle's OD over the
+    N2 fundamental and the CO2 band head, the O2 A-band, and the 1.06 um O2 band with the
+    ozone Chappuis band."""
     from oracle.py import Oracle
     from test_gpu_od_fp64 import _cmp
-    lines = W.profile_lines(20000, 2000.0, 2600.0, seed=1001)
+    lo, hi, v1, v2 = band
+    lines = W.profile_lines(20000, lo, hi, seed=1001)
     lay = W.atmosphere()[k]
-    dv = W.calculate_dv(2300.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
+    dv = W.calculate_dv(0.5 * (v1 + v2), lay.pave, lay.tave, lay.w[1] / lay.wtot)
     h = ctx.upload_lines(lines)
     ctx.jrad = 1
-    od, _ = ctx.odlay(h, lay, 2250.0, 2450.0, dv, mtckd=True, other_continua=True)
+    od, _ = ctx.odlay(h, lay, v1, v2, dv, mtckd=True, other_continua=True)
     orc = Oracle(lines)
-    ref, _ = orc.odlay(lay, 2250.0, 2450.0, dv, mtckd=table, contnm=(1.0, 1.0, 1.0, 1.0, 1.0))
+    ref, _ = orc.odlay(lay, v1, v2, dv, mtckd=table, contnm=(1.0, 1.0, 1.0, 1.0, 1.0))
     err = _cmp(od, ref)
-    print("layer %d: max rel err %.3e" % (k, err))
+    print("layer %d, %g-%g cm-1: max rel err %.3e" % (k, v1, v2, err))
     ctx.free_lines(h)
 
 
-def test_contnm_bands_left_out_are_refused(ctx):
+def test_contnm_absrb_longer_than_the_reference_is_refused(ctx):
+    """CONTNM's arrays hold 5050 points (N_ABSRB): a longer ABSRB is an argument error,
+    a range just inside it is not."""
+    from clblm_amd import _lib
     from clblm_amd._lib import ClblmError
     lay = W.atmosphere()[0]
-    lines = W.profile_lines(2000, 4300.0, 4500.0, seed=1001)
-    h = ctx.upload_lines(lines)
-    with pytest.raises(ClblmError, match="N2 overtone"):
-        ctx.odlay(h, lay, 4300.0, 4400.0, 0.01, other_continua=True)
-    ctx.contnm_scales(xn2cn=0.0)
-    try:
-        ctx.odlay(h, lay, 4300.0, 4400.0, 0.01, other_continua=True)   # the band's factor is 0: no refusal
-    finally:
-        ctx.contnm_scales()
-    ctx.free_lines(h)
+    with pytest.raises(ClblmError, match="5050"):
+        ctx.contnm_absrb([lay], 1000.0, 6100.0, 1, _lib.OD_CONTNM)
+    got = ctx.contnm_absrb([lay], 1000.0, 6000.0, 1, _lib.OD_CONTNM)
+    assert got[0].numel() == 5007
