@@ -3705,3 +3705,6 @@ int orc_fltrfn(const double *x, int64_t n, double v1, double dv, double v1f, dou
     free(XF);
     return 0;
 }
+
+/* Layer OD with its directional derivatives (the adjoint leg's checker). */
+#include "clblm_oracle_tan.c"

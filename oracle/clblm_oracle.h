@@ -65,6 +65,23 @@ int orc_odlay(const orc_lines *lines, const orc_layer *layer, orc_params *par,
 
 const char *orc_error(void);
 
+/* A direction of differentiation: the layer temperature moves by dt (K), the
+ * broadening column by the factor (1 + dlnwtot), each column W(m) by (1 + dlnw[m]). */
+typedef struct {
+    double dt, dlnwtot;
+    double dlnw[48];
+} orc_dir;
+
+/* Layer OD of the smooth model (the reference's ODLAY with CNVFNQ's shape tables
+ * replaced by their closed forms) and its derivatives along ndir (<= 16) directions.
+ * out holds 1 + ndir channels of cap points each (channel 0 the OD); dabsrb, when the
+ * continuum is on, holds ndir arrays of nptabs derivatives of ABSRB.  Species
+ * broadening is not differentiated (error). */
+int orc_odlay_tan(const orc_lines *lines, const orc_layer *layer, orc_params *par, int ndir, const orc_dir *dirs,
+                  const double *dabsrb, double *out, int64_t cap, int64_t *nsamp, double *dv_used);
+double orc_dradfn_dt(double v, double t);
+double orc_tips_dlogq(int mol, int iso, double t);
+
 /* Helpers exposed for unit tests. */
 double orc_radfn(double vi, double xkt);
 double orc_radfni(double vi, double dvi, double xkt, double *vinew, double *rdel, double *rdlast);

@@ -149,6 +149,57 @@ class Oracle:
 
 
 # ---- instrument scan (clblm_FFT.f90) ------------------------------------------------
+class _Dir(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("dlnwtot", ctypes.c_double), ("dlnw", ctypes.c_double * 48)]
+
+
+def make_dirs(dirs):
+    """dirs: sequence of (dt, dlnwtot, {m: dlnw}) -> ctypes array of orc_dir."""
+    arr = (_Dir * max(1, len(dirs)))()
+    for k, (dt, dlw, dq) in enumerate(dirs):
+        arr[k].dt, arr[k].dlnwtot = float(dt), float(dlw)
+        for m, v in (dq or {}).items():
+            arr[k].dlnw[m] = float(v)
+    return arr
+
+
+def _odlay_tan(self, layer, v1, v2, dvnormal, dirs, dvnarrow=None, sample=4.0, dvout=0.0, continuum=None,
+               dcontinuum=None):
+    """The smooth-model OD and its derivatives along dirs ((dt, dlnwtot, {m: dlnw}) each).
+    Returns (array [1 + ndir, npts], dv_used); JRAD persists as in odlay.  dcontinuum:
+    [ndir, nptabs] derivatives of the continuum along each direction."""
+    lay = _Layer()
+    lay.pave, lay.tave, lay.wtot = layer.pave, layer.tave, layer.wtot
+    for m, v in layer.w.items():
+        lay.w[m] = v
+        lay.sel[m] = 1
+    absrb = None if continuum is None else np.ascontiguousarray(continuum, np.float64)
+    dab = None if dcontinuum is None else np.ascontiguousarray(dcontinuum, np.float64)
+    npt = 0 if absrb is None else absrb.size
+    par = _Params(v1, v2, dvnormal, dvnormal if dvnarrow is None else dvnarrow, sample, dvout, self.jrad,
+                  1 if absrb is not None else 0, absrb.ctypes.data if absrb is not None else None, npt, 0)
+    D = make_dirs(dirs)
+    nd = len(dirs)
+    ns, dvu = ctypes.c_int64(), ctypes.c_double()
+    L = lib()
+    args = (ctypes.byref(self._L), ctypes.byref(lay), ctypes.byref(par), nd, D,
+            ctypes.c_void_p(dab.ctypes.data) if dab is not None else None)
+    jr0 = self.jrad
+    if L.orc_odlay_tan(*args, None, ctypes.c_int64(0), ctypes.byref(ns), ctypes.byref(dvu)):
+        raise RuntimeError(L.orc_error().decode())
+    cap = ns.value
+    out = np.zeros((nd + 1, cap))
+    par.jrad = jr0
+    if L.orc_odlay_tan(*args, out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(cap), ctypes.byref(ns),
+                       ctypes.byref(dvu)):
+        raise RuntimeError(L.orc_error().decode())
+    self.jrad = par.jrad
+    return out[:, : ns.value], dvu.value
+
+
+Oracle.odlay_tan = _odlay_tan
+
+
 class ScanPlan(ctypes.Structure):
     _fields_ = [("jfn", ctypes.c_int), ("hwhm", ctypes.c_double), ("a", ctypes.c_double),
                 ("pre_interp", ctypes.c_int), ("dvn", ctypes.c_double), ("v1z", ctypes.c_double),
