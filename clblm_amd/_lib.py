@@ -12,7 +12,7 @@ MAXMOL = 47
 FP64, FP32C = 0, 1
 
 EXPORTS = ["clblm_version", "clblm_last_error", "clblm_ctx_create", "clblm_ctx_destroy",
-           "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_batch", "clblm_sync",
+           "clblm_linelist_upload", "clblm_linelist_free", "clblm_odlay", "clblm_od_tangent", "clblm_od_batch", "clblm_sync",
            "clblm_rt_up", "clblm_level2layer", "clblm_rt_jac", "clblm_level_jac", "clblm_fft_scan",
            "clblm_timing", "clblm_work", "clblm_tuning", "clblm_autotune", "clblm_scanfn",
            "clblm_linelist_broadening", "clblm_rt_down", "clblm_tape3_read", "clblm_linelist_load_tape3",
@@ -73,6 +73,11 @@ class ODParams(ctypes.Structure):
                 ("dvnarrow", ctypes.c_double), ("sample", ctypes.c_double), ("dvout", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("contnm", ctypes.c_int32),
                 ("continuum", ctypes.c_void_p), ("nptabs", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class ODDir(ctypes.Structure):
+    """clblm_od_dir"""
+    _fields_ = [("dt", ctypes.c_double), ("dlnwtot", ctypes.c_double), ("dlnw", ctypes.c_double * (MAXMOL + 1))]
 
 
 _lib = None
@@ -163,6 +168,8 @@ def load():
     lib.clblm_od_graph_launch.argtypes = [P, P, P]
     lib.clblm_od_graph_check.argtypes = [P, P]
     lib.clblm_od_graph_destroy.argtypes = [P, P]
+    lib.clblm_od_tangent.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int,
+                                     P, P, P, P, P]
     _lib = lib
     return lib
 

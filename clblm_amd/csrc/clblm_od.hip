@@ -87,6 +87,20 @@ __host__ __device__ inline double radfn(double vi, double xkt) {
     return xvi;
 }
 
+// d/dT of RADFN(v, T / RADCN2), branch by branch: -RADFN/T on the small-x branch,
+// -2 v x e / ((1 + e)^2 T) with e = exp(-x) on the middle one, 0 above x = 10.
+__host__ __device__ inline double dradfn_dt(double v, double t) {
+    const double xkt = t / RADCN2;
+    if (!(xkt > 0.)) return 0.;
+    const double x = v / xkt;
+    if (x <= 0.01) return -0.5 * x * v / t;
+    if (x <= 10.0) {
+        const double e = exp(-x);
+        return -2. * v * x * e / ((1. + e) * (1. + e) * t);
+    }
+    return 0.;
+}
+
 // RADFNI, clblm_ODLAY_CommonSub.f90:379-525 (one step of the piecewise-linear
 // radiation term; *vinew stays >= 0 on this path)
 __host__ __device__ inline double radfni(double vi, double dvi, double xkt, double *vinew,
@@ -1147,7 +1161,19 @@ constexpr int EMAX = LIMIN4 + REC;
 struct ShrinkSums {
     double v, s, al, ad;
 };
-__global__ void __launch_bounds__(64) k_emit(const Job *jobs, F4Prep f, Comp c, Shrunk sh, int *err) {
+// Derivatives of the accepted lines' LINF4Q values (v, s, al, ad, sp2 at
+// dcomp[(5 k + q) tot + position] for direction k) and of the shrunk lines (dshr, the
+// same layout over shrunk-line indices), for the tangent batch.
+struct TanF4 {
+    const double *dcomp;
+    double *dshr;
+    int nd;
+    int64_t tot;
+};
+// TAN: also the derivatives of each emitted line: of a group's strength-weighted
+// means (d(sum s v) - v d(sum s)) / sum s and so on, of a coupled line its own.
+template <bool TAN>
+__global__ void __launch_bounds__(64) k_emit(const Job *jobs, F4Prep f, Comp c, Shrunk sh, int *err, TanF4 tf) {
     __shared__ double sv[EMAX];
     __shared__ uint8_t sf[EMAX];
     __shared__ short gs[EMAX + 1];
@@ -1241,6 +1267,46 @@ __global__ void __launch_bounds__(64) k_emit(const Job *jobs, F4Prep f, Comp c, 
             if (hb) {
                 sh.vnu[o] = b.v / b.s; sh.sp[o] = b.s; sh.alfa[o] = b.al / b.s; sh.epp[o] = b.ad / b.s;
                 sh.spp[o] = 0.; sh.co2[o] = 1;
+            }
+            if (TAN) {
+                const int64_t p0g = J.f4_off + p0, T5 = tf.tot;
+                for (int k = 0; k < tf.nd; k++) {
+                    const double *dc = tf.dcomp + (int64_t)5 * k * T5;
+                    double *ds = tf.dshr + (int64_t)5 * k * T5;
+                    int64_t oo = J.f4_off + base_out + incl - cnt;
+                    double ga[5] = {0., 0., 0., 0., 0.}, gb[5] = {0., 0., 0., 0., 0.};
+                    for (int i = gs[g]; i < gs[g + 1]; i++) {
+                        const int64_t pi = p0g + i;
+                        if (sf[i] & CF_CPL) {
+#pragma unroll
+                            for (int qq = 0; qq < 5; qq++) ds[qq * T5 + oo] = dc[qq * T5 + pi];
+                            oo++;
+                            continue;
+                        }
+                        if (!(sf[i] & CF_LIVE)) continue;
+                        double *gg = (sf[i] & CF_CO2) ? gb : ga;
+                        const double dv = dc[pi], dsv = dc[T5 + pi], dal = dc[2 * T5 + pi], dad = dc[3 * T5 + pi];
+                        gg[0] += dv * ss[i] + sv[i] * dsv;
+                        gg[1] += dsv;
+                        gg[2] += dsv * aa[i] + ss[i] * dal;
+                        gg[3] += dsv * dd[i] + ss[i] * dad;
+                    }
+                    if (ha) {
+                        ds[oo] = (ga[0] - sh.vnu[oo] * ga[1]) / a.s;
+                        ds[T5 + oo] = ga[1];
+                        ds[2 * T5 + oo] = (ga[2] - sh.alfa[oo] * ga[1]) / a.s;
+                        ds[3 * T5 + oo] = (ga[3] - sh.epp[oo] * ga[1]) / a.s;
+                        ds[4 * T5 + oo] = 0.;
+                        oo++;
+                    }
+                    if (hb) {
+                        ds[oo] = (gb[0] - sh.vnu[oo] * gb[1]) / b.s;
+                        ds[T5 + oo] = gb[1];
+                        ds[2 * T5 + oo] = (gb[2] - sh.alfa[oo] * gb[1]) / b.s;
+                        ds[3 * T5 + oo] = (gb[3] - sh.epp[oo] * gb[1]) / b.s;
+                        ds[4 * T5 + oo] = 0.;
+                    }
+                }
             }
         }
         base_out += __shfl(incl, 63, 64);
@@ -2032,16 +2098,22 @@ __device__ inline double xint_point(double v1a, double v2a, double dva, const do
     return conti * 1.0;
 }
 
-template <typename TA>
+// DRAD: the temperature derivative of the radiation term instead of the term itself,
+// scaled by dscale and added to out (JRAD 0 layers only): with the panel sums of the
+// line-sum OD this is the product rule's d(RADFNI)/dT part of the OD's temperature
+// derivative.  RADFNI's nodes stay where the term puts them; each segment's ramp
+// runs from d RADFN/dT at its start node to that at its end node.
+template <typename TA, bool DRAD = false>
 __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *panels, const R4Blk *blks,
                                                const TA *A, const double *R4, const double *absrb,
-                                               double *out, double *nat) {
+                                               double *out, double *nat, double dscale = 0.) {
     __shared__ double r3[168], r3e[8], r2[640], r1[2448];
     __shared__ double rbase[RAMP_CAP], rstep[RAMP_CAP];
     __shared__ int rk[RAMP_CAP + 1], nramp;
     const int64_t q = blockIdx.x;
     const Panel &P = panels[q];
     const Job &J = jobs[P.layer];
+    if (DRAD && J.jrad != 0) return;
     const int p = (int)(q - J.panel_off);
     const bool prev = p > 0;
     const Panel &PP = panels[prev ? q - 1 : q];
@@ -2121,7 +2193,36 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
     // Radiation term (panelOut, clblm_ODLAY.f90:2447-2483): RADFNI segments whose
     // sequential ramps are turned into exact linear pieces by one lane, then applied
     // by all lanes at the store.
-    if (J.jrad == 0) {
+    __shared__ int inplace_from;
+    if (DRAD && tid == 0) {
+        const double xkt = J.tave / RADCN2, DV = J.dv;
+        double vitst = P.vft + (double)(NLO - 1) * DV - DV, rdlast = -1., rdel, drlast = 0.;
+        int n2 = NLO - 1, np = 0;
+        inplace_from = NHI + 1;
+        do {
+            int n1 = n2 + 1;
+            double vi = P.vft + (double)(n1 - 1) * DV;
+            const bool first = rdlast < 0.;
+            (void)radfni(vi, DV, xkt, &vitst, &rdel, &rdlast);
+            n2 = (int)((vitst - P.vft) / DV + 0.001);
+            if (n2 > NHI) n2 = NHI;
+            const double d0 = first ? dradfn_dt(vi, J.tave) : drlast;
+            int intvls = (int)((vitst - vi) / DV + 0.5);
+            if (intvls < 1) intvls = 1;
+            const double dn = dradfn_dt(vitst, J.tave);
+            drlast = dn;
+            const double dstep = (dn - d0) / (double)intvls;
+            if (np < RAMP_CAP && inplace_from > NHI) {
+                rk[np] = n1; rbase[np] = d0 * dscale; rstep[np] = dstep * dscale; np++;
+            } else {   // more segments than the table holds: applied in place
+                if (inplace_from > NHI) inplace_from = n1;
+                for (int i = n1; i <= n2; i++) r1[i] = r1[i] * ((d0 + (double)(i - n1) * dstep) * dscale);
+            }
+        } while (n2 < NHI);
+        rk[np] = inplace_from;
+        nramp = np;
+    }
+    if (!DRAD && J.jrad == 0) {
         if (tid == 0) {
             const double xkt = J.tave / RADCN2, DV = J.dv;
             double vitst = P.vft + (double)(NLO - 1) * DV - DV, rdlast = -1., rdel;
@@ -2157,16 +2258,18 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
         }
         __syncthreads();
     }
+    if (DRAD) __syncthreads();
     const int np = J.jrad == 0 ? nramp : 0;
     double *o = (J.dvout > 0. ? nat : out) + P.out_off - NLO;
     int pc = 0;
     for (int i = NLO + tid; i <= NHI; i += blockDim.x) {
         double v = r1[i];
-        if (np > 0) {
+        if (np > 0 && (!DRAD || i < inplace_from)) {
             while (rk[pc + 1] <= i) pc++;
             v = v * (rbase[pc] + (double)(i - rk[pc]) * rstep[pc]);
         }
-        o[i] = v;
+        if (DRAD) o[i] = o[i] + v;
+        else o[i] = v;
     }
 }
 
@@ -2944,6 +3047,7 @@ struct clblm_ctx {
     float4 *tabgs = nullptr;
     std::vector<LineListH *> lists;
     DevArena arena;
+    DevArena tan_arena;   // scratch of the tangent batch (clblm_od_tangent)
     int *d_err = nullptr;
     unsigned long long *d_work = nullptr;   // [conv evaluations, F4 evaluations]
     double *rt_state = nullptr;             // per-layer recursion states for the reverse sweep
@@ -3630,8 +3734,8 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         hipLaunchKernelGGL(brd ? k_f4prep<true> : k_f4prep<false>, dim3((unsigned)B.max_rec, nlay), dim3(256), 0, s4, d_jobs, L.dev, f, cmp, d_err);
         hipLaunchKernelGGL(k_chunkG, dim3(cdiv(B.max_rec, 64), nlay), dim3(64), 0, s4, d_jobs, cmp, f.G);
         hipLaunchKernelGGL(k_sim, dim3(nlay), dim3(256), 0, s4, d_jobs, f, cmp, (const int *)f.G);
-        hipLaunchKernelGGL(k_emit, dim3((unsigned)std::max<int64_t>(1, B.tot_rec)), dim3(64), 0, s4, d_jobs, f, cmp,
-                           sh, d_err);
+        hipLaunchKernelGGL(k_emit<false>, dim3((unsigned)std::max<int64_t>(1, B.tot_rec)), dim3(64), 0, s4, d_jobs, f,
+                           cmp, sh, d_err, TanF4{});
         double *ftot = at<double>(base, o_ftot);
         if (fcap > 0) {
             hipLaunchKernelGGL(k_shr_local, dim3((unsigned)fcap, nlay, 2), dim3(SCAN_T), 0, s4, d_jobs, f, sh, ftot,
@@ -3856,6 +3960,7 @@ int clblm_ctx_destroy(clblm_ctx *c) {
             delete l;
         }
     if (c->arena.p) (void)hipFree(c->arena.p);
+    if (c->tan_arena.p) (void)hipFree(c->tan_arena.p);
     (void)hipFree(c->mt_buf);
     (void)hipFree(c->tab);
     (void)hipFree(c->tabs);
@@ -4922,3 +5027,6 @@ int clblm_rt_up(clblm_ctx *c, int nlay, int64_t npts, double v1, double dv, cons
 }
 
 }  // extern "C"
+
+// Layer ODs with their derivatives (the tangent batch)
+#include "clblm_tan.inc"

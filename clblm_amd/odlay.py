@@ -208,6 +208,42 @@ class Context:
         self.jrad = jr.value
         return out, off
 
+    def od_tangent(self, handle, layers, params, dirs, dcontinuum=None, stream=None):
+        """Layer ODs and their derivatives (clblm_od_tangent).  dirs: sequence of
+        (dt, dlnwtot, {m: dlnw}); dcontinuum: per layer with a continuum, an array
+        [len(dirs), nptabs] of the continuum's derivatives (None: zero).  Returns
+        (tensor [1 + len(dirs), offsets[-1]] on the GPU, offsets): row 0 the OD of the
+        smooth model (closed-form line shapes), row 1 + k its derivative along
+        dirs[k].  Synchronous."""
+        import torch
+        nl = len(layers)
+        lay = (_lib.Layer * nl)(*[_layer_struct(l) for l in layers])
+        par = (ODParams * nl)(*params)
+        nd = len(dirs)
+        D = (_lib.ODDir * max(1, nd))()
+        for k, (dt, dlw, dq) in enumerate(dirs):
+            D[k].dt, D[k].dlnwtot = float(dt), float(dlw)
+            for m, v in (dq or {}).items():
+                D[k].dlnw[m] = float(v)
+        dc = None
+        if dcontinuum is not None:
+            parts = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in dcontinuum if a is not None]
+            dc = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(1))
+        off = np.zeros(nl + 1, np.int64)
+        jr = ctypes.c_int32(self.jrad)
+        args = [self.ptr, handle, nl, ctypes.cast(lay, ctypes.c_void_p), ctypes.cast(par, ctypes.c_void_p),
+                ctypes.byref(jr), nd, ctypes.cast(D, ctypes.c_void_p),
+                ctypes.c_void_p(dc.ctypes.data) if dc is not None else None, off.ctypes.data_as(ctypes.c_void_p)]
+        check(self.lib.clblm_od_tangent(*args, None, None))
+        out = torch.empty((nd + 1, int(off[-1])), dtype=torch.float64, device="cuda")
+        jr = ctypes.c_int32(self.jrad)
+        args[5] = ctypes.byref(jr)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        check(self.lib.clblm_od_tangent(*args, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)))
+        self.jrad = jr.value
+        return out, off
+
     def od_graph(self, handle, layers, params, out=None):
         """Captures the batch of ``od_batch`` in one hipGraph (nothing runs yet):
         returns an ODGraph whose ``launch()`` fills ``out`` (the graph's float64 cuda

@@ -162,6 +162,33 @@ int clblm_od_batch(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *laye
                    const clblm_od_params *pars, int32_t *jrad_state, int64_t *offsets,
                    double *d_od, void *stream);
 
+/* ---- layer ODs with their derivatives (the adjoint leg) ------------------------
+ * No single counterpart in the reference: its Jacobians take dTau/dT and dTau/dW
+ * from ODLAY re-runs at Tave + 1 K, Wtot + 1 % and each molecule alone
+ * (drv_LevelTemp, src/clblm_src/clblm_noScattJac.f90:1005-1034; drv_LevelConc
+ * :1156-1171).  This entry point returns, in one batch, the layer ODs and their
+ * exact derivatives along up to 16 directions.  The model differentiated is
+ * ODLAY's (clblm_ODLAY.f90:757-2545, clblm_ODLAY_LineF4.f90:25-1647) with CNVFNQ's
+ * shape tables replaced by the closed forms they sample (SHAPEL/SHAPEG/VERFN,
+ * clblm_ODLAY.f90:2804-3016); its OD differs from clblm_od_batch's FP64 OD by the
+ * tables' spacing (below 1e-3 of the peak).
+ * A direction moves the layer temperature by dt (K), the broadening column Wtot by
+ * the factor (1 + dlnwtot) and each column W(m) by (1 + dlnw[m]), to first order.
+ * Arguments as clblm_od_batch (FP64 only; no CLBLM_OD_MTCKD / CLBLM_OD_CONTNM flags
+ * and no species broadening: pass the continuum as pars[l].continuum and its
+ * derivatives in dcontinuum, ndir arrays of nptabs values per layer that has a
+ * continuum, in layer order; NULL = zero).  Query with d_out == NULL (offsets).
+ * d_out: (1 + ndir) channels of offsets[nlay] doubles each: channel 0 the OD, channel
+ * 1 + k the derivative along dirs[k].  Synchronous. */
+typedef struct {
+    double dt;                     /* K */
+    double dlnwtot;                /* relative change of Wtot */
+    double dlnw[CLBLM_MAXMOL + 1]; /* relative change of W(m) */
+} clblm_od_dir;
+int clblm_od_tangent(clblm_ctx *ctx, int handle, int nlay, const clblm_layer *layers,
+                     const clblm_od_params *pars, int32_t *jrad_state, int ndir, const clblm_od_dir *dirs,
+                     const double *dcontinuum, int64_t *offsets, double *d_out, void *stream);
+
 /* ---- a batch captured in a hipGraph ----------------------------------------------
  * (no counterpart in the reference, whose layer loop calls ODLAY one layer at a
  * time; for callers that compute the same layer set again and again, e.g. a
