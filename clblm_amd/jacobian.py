@@ -50,6 +50,72 @@ def od_variants(layers, mols, dt=1.0, dw_frac=0.01):
     return out
 
 
+def continuum_tangents(continuum, lay, v1, v2, dirs, steps=(0.01, 1e-4)):
+    """A caller's continuum model (callable(layer, v1, v2) -> (absrb, v1abs, npt)) on
+    the 1 cm-1 ABSRB grid and its derivatives along dirs ((dt, dlnwtot, {m: dlnw})
+    each), by central differences of the model itself with steps (K, ln W).  The line
+    sum's derivatives are analytic; the continuum is a smooth function of a few
+    hundred points per layer that the caller owns."""
+    c0 = np.asarray(continuum(lay, v1, v2)[0], np.float64)
+    ht, hw = steps
+    out = []
+    for dt_, dlw, dq in dirs:
+        h = ht if dt_ else hw
+        moved = []
+        for sgn in (1.0, -1.0):
+            w = {m: v * (1.0 + sgn * h * dq.get(m, 0.0)) for m, v in lay.w.items()}
+            lv = dataclasses.replace(lay, tave=lay.tave + sgn * h * dt_, wtot=lay.wtot * (1.0 + sgn * h * dlw), w=w)
+            moved.append(np.asarray(continuum(lv, v1, v2)[0], np.float64))
+        out.append((moved[0] - moved[1]) / (2.0 * h))
+    return c0, np.array(out)
+
+
+def layer_od(ctx, handle, layers, v1, v2, dvs, t_lay, lnw_tot, lnw, mols, dvout, continuum=None):
+    """Differentiable layer optical depths (torch.autograd.Function on ROCm tensors).
+
+    t_lay [L] (K), lnw_tot [L] (ln of the broadening column), lnw [L, M] (ln of the
+    columns of ``mols``) are float64 tensors; the other layer fields come from
+    ``layers``.  Returns OD [L, N] on v1 + i dvout (every layer re-gridded to dvout):
+    the OD of the line sum with closed-form line shapes (clblm_od_tangent channel 0).
+    The backward pass recomputes the layers' derivatives with one tangent batch
+    instead of storing them, and contracts them with the incoming gradient:
+    dL/dT_l = sum_v g[l, v] dOD_l(v)/dT_l and likewise for ln Wtot and ln W(m)."""
+    return _LayerOD.apply(t_lay, lnw_tot, lnw, (ctx, handle, list(layers), v1, v2, list(dvs), tuple(mols), dvout,
+                                                 continuum))
+
+
+def _moved_layers(cfg, t_lay, lnw_tot, lnw):
+    _, _, layers, _, _, _, mols, _, _ = cfg
+    t = t_lay.detach().cpu().numpy()
+    lwt = lnw_tot.detach().cpu().numpy()
+    lw = lnw.detach().cpu().numpy()
+    out = []
+    for l, lay in enumerate(layers):
+        w = dict(lay.w)
+        for k, m in enumerate(mols):
+            w[m] = float(np.exp(lw[l, k]))
+        out.append(dataclasses.replace(lay, tave=float(t[l]), wtot=float(np.exp(lwt[l])), w=w))
+    return out
+
+
+def _layer_od_run(cfg, lays, dirs, jrad):
+    from ._lib import FP64
+    ctx, handle, _, v1, v2, dvs, _, dvout, continuum = cfg
+    pars, dcont = [], []
+    for l, lay in enumerate(lays):
+        c = None
+        if continuum is not None:
+            c, dc = continuum_tangents(continuum, lay, v1, v2, dirs)
+            dcont.append(dc)
+        pars.append(Context.params(v1, v2, dvs[l], dvout=dvout, precision=FP64, continuum=c))
+    ctx.jrad = jrad
+    out, off = ctx.od_tangent(handle, lays, pars, dirs, dcontinuum=dcont if (dcont and dirs) else None)
+    n = int(off[1] - off[0])
+    if not np.all(np.diff(off) == n):
+        raise ValueError("layer_od needs every layer on the dvout grid")
+    return out.view(len(dirs) + 1, len(lays), n)
+
+
 class UpwellingJacobian:
     """Radiance and level Jacobians of one scene.
 
@@ -60,8 +126,13 @@ class UpwellingJacobian:
 
     def __init__(self, ctx: Context, handle: int, layers, t_lev, dvs, *, p_lev, q_lev=None, mols=None, emis=0.98,
                  tskin=295.0, lin_in_tau=1, sec_dn=1.0, precision=FP32C, dt=1.0, dw_frac=0.01, continuum=None,
-                 dvout=None, boxcar_hw=-1.0):
+                 dvout=None, boxcar_hw=-1.0, method="variants", continuum_step=(0.01, 1e-4)):
         import torch
+        if method not in ("variants", "adjoint"):
+            raise ValueError("method must be 'variants' or 'adjoint'")
+        self.method = method
+        # central-difference steps of the caller's continuum along T (K) and ln W
+        self.cont_step = continuum_step
         self.ctx, self.h, self.layers = ctx, handle, list(layers)
         L = len(self.layers)
         self.t_lev = np.ascontiguousarray(t_lev, np.float64)
@@ -110,6 +181,40 @@ class UpwellingJacobian:
             pars.append(Context.params(v1, v2, self.dvs[k % L], dvout=self.dvout, precision=self.prec, continuum=c))
         return jobs, pars
 
+    def _continuum_tangents(self, lay, v1, v2, dirs):
+        return continuum_tangents(self.continuum, lay, v1, v2, dirs, self.cont_step)
+
+    def adjoint_dirs(self):
+        """The tangent batch's directions: temperature, the broadening column, then each
+        Jacobian molecule's column (as ln W)."""
+        return [(1.0, 0.0, {}), (0.0, 1.0, {})] + [(0.0, 0.0, {m: 1.0}) for m in self.mols]
+
+    def _block_adjoint(self, v1, v2):
+        """One tangent batch gives the layer ODs and dtau/dT, dtau/dln Wtot and
+        dtau/dln W(m) exactly (clblm_od_tangent); drv_LevelTemp's combination
+        dtau/dT - dtau/dWtot Wtot/Tave (clblm_noScattJac.f90:1027-1031) and drv_LevelConc's
+        use of dtau/dln W (:1159-1171) are then assembled by clblm_level_jac with unit
+        steps and a zero base block, so that no difference is formed."""
+        import torch
+        from ._lib import FP64
+        L, M = len(self.layers), len(self.mols)
+        dirs = self.adjoint_dirs()
+        pars, dcont = [], []
+        for l, lay in enumerate(self.layers):
+            c = None
+            if self.continuum is not None:
+                c, dc = self._continuum_tangents(lay, v1, v2, dirs)
+                dcont.append(dc)
+            pars.append(Context.params(v1, v2, self.dvs[l], dvout=self.dvout, precision=FP64, continuum=c))
+        self.ctx.jrad = 1
+        out, off = self.ctx.od_tangent(self.h, self.layers, pars, dirs, dcontinuum=dcont if dcont else None)
+        n = int(off[1] - off[0])
+        assert np.all(np.diff(off) == n), "all layers of a block share the DVout grid"
+        od = out[0].view(L, n)
+        od_var = torch.cat([torch.zeros((1, L, n), dtype=torch.float64, device=out.device),
+                            out[1:].view(2 + M, L, n)]).contiguous()
+        return od, od_var, n, 1.0, 1.0
+
     def block(self, v1, v2):
         """Radiance and Jacobians on [v1, v2] (uniform grid DVout).  Returns torch tensors:
         rad [N]; jac_t [L+1, N] = dR/dT_lev; jac_lnq [M, L+1, N] = dR/dln q_lev;
@@ -118,13 +223,17 @@ class UpwellingJacobian:
         jac_lnq, dr_dtskin and dr_demis: the block's grid, or the boxcar's output grid
         when the outputs are pre-averaged."""
         import torch
-        jobs, pars = self._jobs(v1, v2)
-        self.ctx.jrad = 1
-        od, off = self.ctx.od_batch(self.h, jobs, pars)
-        n = int(off[1] - off[0])
-        assert np.all(np.diff(off) == n), "all jobs of a block share the DVout grid"
         L, M = len(self.layers), len(self.mols)
-        od = od.view(3 + M, L, n)
+        if self.method == "adjoint":
+            od0, od, n, dt, dwf = self._block_adjoint(v1, v2)
+        else:
+            jobs, pars = self._jobs(v1, v2)
+            self.ctx.jrad = 1
+            od, off = self.ctx.od_batch(self.h, jobs, pars)
+            n = int(off[1] - off[0])
+            assert np.all(np.diff(off) == n), "all jobs of a block share the DVout grid"
+            od = od.view(3 + M, L, n)
+            od0, dt, dwf = od[0], self.dt, self.dw_frac
         dev = od.device
         e = lambda *shape: torch.empty(*shape, dtype=torch.float64, device=dev)
         rad, dr_dtau, dr_dtlev, dts, de = e(n), e(L, n), e(L + 1, n), e(n), e(n)
@@ -133,12 +242,12 @@ class UpwellingJacobian:
         D = self._d
         st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         lib = self.ctx.lib
-        check(lib.clblm_rt_jac(self.ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(self.dvout), p(od[0]),
+        check(lib.clblm_rt_jac(self.ctx.ptr, L, n, ctypes.c_double(v1), ctypes.c_double(self.dvout), p(od0),
                                p(D["tlay"]), p(D["tlev"]), p(D["up"]), p(D["lo"]), ctypes.c_double(self.emis),
                                ctypes.c_double(self.tskin), self.lin, ctypes.c_double(self.sec_dn), p(rad),
                                p(dr_dtau), p(dr_dtlev), p(dts), p(de), st))
         check(lib.clblm_level_jac(self.ctx.ptr, L, M, n, p(od), p(dr_dtau), p(dr_dtlev), p(D["wtot"]), p(D["tlay"]),
-                                  ctypes.c_double(self.dt), ctypes.c_double(self.dw_frac), p(D["up"]), p(D["lo"]),
+                                  ctypes.c_double(dt), ctypes.c_double(dwf), p(D["up"]), p(D["lo"]),
                                   p(D["wu"]), p(D["wl"]), p(jac_t), p(jac_q), st))
         grid = (float(v1), float(self.dvout))
         if self.prebox:   # output_JacArray (:1341-1409), per block as the reference does
@@ -149,7 +258,7 @@ class UpwellingJacobian:
             dts, de = box(dts)[0], box(de)[0]
             grid = (g1, gd)
         return dict(rad=rad, jac_t=jac_t, jac_lnq=jac_q, dr_dtskin=dts, dr_demis=de, dr_dtau=dr_dtau,
-                    dr_dtlev=dr_dtlev, od=od[0], od_var=od, n=n, grid=grid)
+                    dr_dtlev=dr_dtlev, od=od0, od_var=od, n=n, grid=grid)
 
 
 def upwelling_radiance_grids(ctx, od, offsets, dvs, v1, t_lay, t_lev, emis=0.98, tskin=295.0, lin_in_tau=1,
@@ -359,6 +468,27 @@ try:
             g_od, g_t = fctx.saved_tensors
             return (grad[None, :] * g_od, (grad[None, :] * g_t).sum(dim=1), None, None, None, None, None, None,
                     None, None)
+    class _LayerOD(torch.autograd.Function):
+        @staticmethod
+        def forward(fctx, t_lay, lnw_tot, lnw, cfg):
+            lays = _moved_layers(cfg, t_lay, lnw_tot, lnw)
+            jr = cfg[0].jrad
+            od = _layer_od_run(cfg, lays, [], jr)[0]
+            fctx.cfg, fctx.lays, fctx.jrad = cfg, lays, jr
+            fctx.devs = (t_lay.device, lnw_tot.device, lnw.device)
+            return od
+
+        @staticmethod
+        def backward(fctx, grad):
+            mols = fctx.cfg[6]
+            dirs = [(1.0, 0.0, {}), (0.0, 1.0, {})] + [(0.0, 0.0, {m: 1.0}) for m in mols]
+            ch = _layer_od_run(fctx.cfg, fctx.lays, dirs, fctx.jrad)   # recomputed, not stored
+            g = grad.to(torch.float64)
+            gt = (g * ch[1]).sum(dim=1)
+            gwt = (g * ch[2]).sum(dim=1)
+            gw = torch.stack([(g * ch[3 + k]).sum(dim=1) for k in range(len(mols))], dim=1) if mols else None
+            dv = fctx.devs
+            return gt.to(dv[0]), gwt.to(dv[1]), (gw.to(dv[2]) if gw is not None else None), None
 except ImportError:  # pragma: no cover - torch is part of the stack
     _Radiance = None
 
