@@ -2106,8 +2106,14 @@ __device__ inline double xint_point(double v1a, double v2a, double dva, const do
 template <typename TA, bool DRAD = false>
 __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *panels, const R4Blk *blks,
                                                const TA *A, const double *R4, const double *absrb,
-                                               double *out, double *nat, double dscale = 0.) {
+                                               double *out, double *nat, double dscale = 0., int64_t s_a = 0,
+                                               int64_t s_r4 = 0, int64_t s_abs = 0, int64_t s_out = 0,
+                                               int64_t s_nat = 0) {
     __shared__ double r3[168], r3e[8], r2[640], r1[2448];
+    {   // blockIdx.y: a channel of the tangent batch, its arrays s_* further on
+        const int64_t ch = blockIdx.y;
+        A += ch * s_a; R4 += ch * s_r4; absrb += ch * s_abs; out += ch * s_out; nat += ch * s_nat;
+    }
     __shared__ double rbase[RAMP_CAP], rstep[RAMP_CAP];
     __shared__ int rk[RAMP_CAP + 1], nramp;
     const int64_t q = blockIdx.x;
@@ -2279,10 +2285,13 @@ __global__ void __launch_bounds__(256) k_pass2(const Job *jobs, const Panel *pan
 // rounding the reference's quantised coefficient index depends on; a lane takes a
 // piece of a block and replays P from the block's start.  unit_first[l]: the first
 // work unit of layer l (nondecreasing; layers without re-grid have none).
+// blockIdx.y selects a channel of the tangent batch: nat and out advance by their strides
 __global__ void k_regrid(const Job *jobs, int nlay, const int64_t *unit_first, const double *nat, double *out,
-                         int64_t nunits) {
+                         int64_t nunits, int64_t nat_stride = 0, int64_t out_stride = 0) {
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunits) return;
+    nat += (int64_t)blockIdx.y * nat_stride;
+    out += (int64_t)blockIdx.y * out_stride;
     int l = 0, hi = nlay - 1;          // the last layer whose units start at or before u
     while (l < hi) {
         const int mid = (l + hi + 1) / 2;
@@ -3824,7 +3833,7 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
         HIPCHK(hipMemcpyAsync(d_rgf, hp + h_rgf, sizeof(int64_t) * nlay, hipMemcpyHostToDevice, st));
         if (rg_units > 0)
             hipLaunchKernelGGL(k_regrid, dim3((unsigned)cdiv(rg_units, 64)), dim3(64), 0, st, d_jobs, nlay, d_rgf,
-                               d_nat, d_out, rg_units);
+                               d_nat, d_out, rg_units, (int64_t)0, (int64_t)0);
     }
     HIPCHK(stage_event(4, st));
     if (G) {

@@ -53,9 +53,12 @@ def od_variants(layers, mols, dt=1.0, dw_frac=0.01):
 def continuum_tangents(continuum, lay, v1, v2, dirs, steps=(0.01, 1e-4)):
     """A caller's continuum model (callable(layer, v1, v2) -> (absrb, v1abs, npt)) on
     the 1 cm-1 ABSRB grid and its derivatives along dirs ((dt, dlnwtot, {m: dlnw})
-    each), by central differences of the model itself with steps (K, ln W).  The line
-    sum's derivatives are analytic; the continuum is a smooth function of a few
-    hundred points per layer that the caller owns."""
+    each): the model's own `tangents(layer, v1, v2, dirs)` when it has one, else
+    central differences of the model with steps (K, ln W).  The line sum's derivatives
+    are analytic; the continuum is a smooth function of a few hundred points per layer
+    that the caller owns."""
+    if hasattr(continuum, "tangents"):   # the model's own derivatives
+        return continuum.tangents(lay, v1, v2, dirs)
     c0 = np.asarray(continuum(lay, v1, v2)[0], np.float64)
     ht, hw = steps
     out = []

@@ -270,6 +270,18 @@ def continuum_absrb(layer: Layer, v1: float, v2: float, seed=1001, nknots=50):
     return _continuum_coef(v1abs, v2abs, npt, seed, nknots) * layer.w.get(1, 0.0), v1abs, npt
 
 
+def continuum_absrb_tangents(layer: Layer, v1: float, v2: float, dirs, seed=1001, nknots=50):
+    """continuum_absrb and its derivatives along dirs ((dt, dlnwtot, {m: dlnw}) each):
+    the coefficient does not depend on temperature and the absorber is the H2O
+    column, so the derivative along a direction is dlnw[1] times the continuum."""
+    c, v1abs, npt = continuum_absrb(layer, v1, v2, seed, nknots)
+    d = np.array([c * float((dq or {}).get(1, 0.0)) for _, _, dq in dirs]).reshape(len(dirs), npt)
+    return np.asarray(c, np.float64), d
+
+
+continuum_absrb.tangents = continuum_absrb_tangents
+
+
 @functools.lru_cache(maxsize=256)
 def _continuum_coef(v1abs, v2abs, npt, seed, nknots):
     """The continuum coefficient (cm2/molec) on v1abs + i, i < npt: the seeded spline

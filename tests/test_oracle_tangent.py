@@ -193,3 +193,15 @@ def test_species_broadening_is_refused():
         if opy.lib().orc_odlay_tan(ctypes.byref(o._L), ctypes.byref(lay), ctypes.byref(par), 0, opy.make_dirs([]),
                                    None, None, ctypes.c_int64(0), ctypes.byref(ns), ctypes.byref(dvu)):
             raise RuntimeError(opy.lib().orc_error().decode())
+
+
+def test_synthetic_continuum_derivatives():
+    """The workloads' continuum states its own derivatives (linear in the H2O column,
+    independent of T and Wtot); central differences of the model agree."""
+    from clblm_amd.jacobian import continuum_tangents
+    lay = W.Layer(pave=600.0, tave=260.0, wtot=1.5e25, w={1: 3e22, 2: 5e21})
+    dirs = [(1.0, 0.0, {}), (0.0, 1.0, {}), (0.0, 0.0, {1: 1.0}), (0.0, 0.0, {2: 1.0}), (0.5, -0.2, {1: 2.0})]
+    c0, d = W.continuum_absrb_tangents(lay, 700.0, 760.0, dirs)
+    c1, dfd = continuum_tangents(lambda l, a, b: W.continuum_absrb(l, a, b), lay, 700.0, 760.0, dirs)
+    np.testing.assert_array_equal(c0, c1)
+    np.testing.assert_allclose(d, dfd, rtol=1e-9, atol=1e-12 * np.abs(c0).max())
