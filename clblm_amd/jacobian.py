@@ -225,7 +225,6 @@ class UpwellingJacobian:
         dr_dtlev [L+1, N] and the layer ODs od [L, N].  grid = (v1, dv) of rad, jac_t,
         jac_lnq, dr_dtskin and dr_demis: the block's grid, or the boxcar's output grid
         when the outputs are pre-averaged."""
-        import torch
         L, M = len(self.layers), len(self.mols)
         if self.method == "adjoint":
             od0, od, n, dt, dwf = self._block_adjoint(v1, v2)
@@ -237,6 +236,17 @@ class UpwellingJacobian:
             assert np.all(np.diff(off) == n), "all jobs of a block share the DVout grid"
             od = od.view(3 + M, L, n)
             od0, dt, dwf = od[0], self.dt, self.dw_frac
+        return self.assemble(v1, od0, od, dt, dwf)
+
+    def assemble(self, v1, od0, od, dt=None, dwf=None):
+        """The RT and level assembly of a block from its OD variants (as block computes
+        them, or from a batch that holds many scenes' variants): od0 [L, N] the layer ODs,
+        od [(3 + M), L, N] the variants (or the derivative blocks with unit steps)."""
+        import torch
+        L, M = len(self.layers), len(self.mols)
+        dt = self.dt if dt is None else dt
+        dwf = self.dw_frac if dwf is None else dwf
+        n = od0.shape[1]
         dev = od.device
         e = lambda *shape: torch.empty(*shape, dtype=torch.float64, device=dev)
         rad, dr_dtau, dr_dtlev, dts, de = e(n), e(L, n), e(L + 1, n), e(n), e(n)

@@ -93,3 +93,95 @@ def test_256_scenes_in_one_graph(ctx):
     print("FP64 vs oracle (scenes 0, 131, 255): max rel err %.3e" % ows)
     assert ows <= 1e-10
     ctx.free_lines(h)
+
+
+
+
+def test_256_scenes_jacobians_in_graphs(ctx):
+    """Radiance and level Jacobians of all 256 scenes: every scene's (3 + M) x L OD
+    variants (M = 3: H2O, CO2, O3; the reference's T + 1 K, Wtot + 1 % and
+    molecule-alone re-runs, about 92 000 layer-OD jobs) over a 2 cm-1 window, batched
+    32 scenes (about 11 500 jobs) to a hipGraph -- one graph of all 92 000 jobs asks
+    for more scratch than the GPU holds: each job's line and panel tables live in the
+    graph's arena -- then each scene's RT and level assembly (clblm_rt_jac,
+    clblm_level_jac) from its slice of the batch.  Lines only (the synthetic
+    continuum's host evaluation for 46 000 jobs would dominate the test).  FP32C
+    against FP64 (radiance 1e-5 relative, Jacobians 1e-4 of each scene's maximum of
+    the quantity), and the FP64 batch against three scenes computed on their own
+    (UpwellingJacobian.block: the same kernels, one scene per batch)."""
+    import torch
+    from clblm_amd import _lib
+    from clblm_amd.jacobian import UpwellingJacobian
+    # V1 / XKT < 5 for every surface layer (T > 201 K): JRAD drops to 0 at each scene's
+    # first job in the batch as in a scene run on its own
+    v1, v2 = 700.0, 702.0
+    scenes = W.scene_batch(256, seed=1003)
+    lines = W.profile_lines(200000, 500.0, 2500.0, seed=1001)
+    h = ctx.upload_lines(lines)
+    mols = [1, 2, 3]
+    alld = [[W.calculate_dv(0.5 * (v1 + v2), l.pave, l.tave, l.w[1] / l.wtot) for l in s.layers] for s in scenes]
+    dvout = min(min(d) for d in alld)
+
+    def jac(s, dvs, prec):
+        L = len(s.layers)
+        return UpwellingJacobian(ctx, h, s.layers, W.level_temperatures(L, 0.1), dvs, p_lev=W.level_pressures(L, 0.1),
+                                 q_lev=W.level_densities(L, 0.1, h2o_scale=s.h2o_scale), mols=mols, precision=prec,
+                                 dvout=dvout)
+
+    worst = {"rad": 0.0, "jac_t": 0.0, "jac_lnq": 0.0}
+    keep, ms, njobs = {}, {_lib.FP64: 0.0, _lib.FP32C: 0.0}, 0
+    for c0 in range(0, len(scenes), 32):
+        sel = range(c0, min(c0 + 32, len(scenes)))
+        graphs, jbs = {}, {}
+        for prec in (_lib.FP64, _lib.FP32C):
+            jbs[prec], jobs, pars, first = [], [], [], [0]
+            for si in sel:
+                jb = jac(scenes[si], alld[si], prec)
+                j, p = jb._jobs(v1, v2)
+                jbs[prec].append(jb)
+                jobs += j
+                pars += p
+                first.append(first[-1] + len(j))
+            ctx.jrad = 1
+            g = ctx.od_graph(h, jobs, pars)
+            off = g.offsets
+            n = int(off[1] - off[0])
+            assert np.all(np.diff(off) == n)
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            for rep in range(2):
+                ev[0].record()
+                g.launch(stream=torch.cuda.current_stream().cuda_stream)
+                ev[1].record()
+                torch.cuda.synchronize()
+                g.check()
+            ms[prec] += ev[0].elapsed_time(ev[1])
+            graphs[prec] = g
+        njobs += len(jobs)
+        for k, si in enumerate(sel):
+            r = {}
+            for prec, g in graphs.items():
+                L = len(scenes[si].layers)
+                odv = g.out[first[k] * n:first[k + 1] * n].view(3 + len(mols), L, n)
+                r[prec] = jbs[prec][k].assemble(v1, odv[0], odv)
+            a, b = r[_lib.FP32C], r[_lib.FP64]
+            worst["rad"] = max(worst["rad"], float(((a["rad"] - b["rad"]).abs() / b["rad"].abs()).max()))
+            for q in ("jac_t", "jac_lnq"):
+                worst[q] = max(worst[q], float((a[q] - b[q]).abs().max() / b[q].abs().max()))
+            if si in (0, 131, 255):
+                keep[si] = {q: b[q].cpu().numpy() for q in ("rad", "jac_t", "jac_lnq")}
+        for g in graphs.values():
+            g.close()
+        del graphs, jbs
+        print("scenes %d-%d: %d layer-OD jobs of %d points; worst so far radiance %.2e, dR/dT %.2e, dR/dln q %.2e" %
+              (sel[0], sel[-1], len(jobs), n, worst["rad"], worst["jac_t"], worst["jac_lnq"]), flush=True)
+    print("256 scenes, %d layer-OD jobs: graph replays %.1f ms FP64, %.1f ms FP32C in all; FP32C vs FP64: radiance "
+          "%.2e, dR/dT_lev %.2e, dR/dln q %.2e of each scene's maximum" %
+          (njobs, ms[_lib.FP64], ms[_lib.FP32C], worst["rad"], worst["jac_t"], worst["jac_lnq"]))
+    assert worst["rad"] < 1e-5 and worst["jac_t"] < 1e-4 and worst["jac_lnq"] < 1e-4
+    for si, ref in keep.items():
+        r = jac(scenes[si], alld[si], _lib.FP64).block(v1, v2)
+        for q in ("rad", "jac_t", "jac_lnq"):
+            err = float(np.abs(ref[q] - r[q].cpu().numpy()).max() / np.abs(ref[q]).max())
+            print("scene %d %s: batch vs alone %.2e" % (si, q, err))
+            assert err < 1e-12, (si, q, err)
+    ctx.free_lines(h)
