@@ -402,7 +402,8 @@ def main():
         chosen, _ = ctx.autotune(h, layers, mk(prec), keys=keys, reps=2)
         tuned = {"f4_points_per_lane": chosen[_lib.TUNE_F4_POINTS_FP64 if prec == FP64 else _lib.TUNE_F4_POINTS_FP32C],
                  "conv_unroll": chosen[_lib.TUNE_CONV_UNROLL], "f4_xcd_chunk": chosen[_lib.TUNE_F4_XCD_CHUNK],
-                 "f4_stream": "own" if chosen[_lib.TUNE_F4_STREAM] == 1 else "batch"}
+                 "f4_stream": "own" if chosen[_lib.TUNE_F4_STREAM] == 1 else "batch",
+                 "f4_far_field": {1: "half tile", 2: "off", 3: "third of a tile"}[chosen[_lib.TUNE_F4_FAR_FIELD]]}
     dt, w, tm, off, od_eager = run_od(ctx, h, layers, mk(prec), args.steps, args.warmup)
     value = E / dt
     conv_tf = w["conv_evals"] * FLOP_CONV / (w["ms_conv"] * 1e-3) / 1e12 if w["ms_conv"] > 0 else 0.0

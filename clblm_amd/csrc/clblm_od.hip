@@ -1837,7 +1837,8 @@ template <int F4K>
 __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *blks, const int2 *tiles, Shrunk sh,
                                                  const F4Rec32 *__restrict__ recs, double *R4,
                                                  unsigned long long *work, int64_t ntile, int chunk,
-                                                 const double *__restrict__ far_w, const float *__restrict__ far_tau) {
+                                                 const double *__restrict__ far_w, const float *__restrict__ far_tau,
+                                                 int dfar) {
     const int64_t ti = xcd_tile(blockIdx.x, chunk);
     if (ti >= ntile) return;
     const int2 t = tiles[ti];
@@ -1871,7 +1872,7 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
             const int nb = (int)((c1 - base) < 64 ? (c1 - base) : 64);
             F4Plan P;
             f4_plan(P, lane < nb, (int)pre[1].y, (int)pre[1].z, (int)pre[2].x, (int)pre[2].y, (int)pre[1].w, j0, j1,
-                    (int)pre[1].x, far_tau ? 32 * F4K : 0);
+                    (int)pre[1].x, far_tau ? dfar : 0);
             if (lane < nb) nev += f4_count((int)pre[1].y, (int)pre[1].z, j0, j1);
             if (P.far) {
                 anyfar = true;
@@ -1903,12 +1904,40 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
             // not far, and the mirror images (a scalar bit scan, so far records and
             // records that miss the tile cost nothing here)
             unsigned long long need = P.touch | P.mirror;
+            // whole-tile Lorentz records without the CO2 term or a mirror, taken two at a
+            // time: s1/p1 + s2/p2 = (s1 p2 + s2 p1) / (p1 p2), one reciprocal per pair
+            // (p <= 630 cm-2 inside the 25 cm-1 range, so p1 p2 stays far from overflow),
+            // their bounds b0 subtracted once per chunk
+            const unsigned long long pairable = P.all & ~P.co2 & ~P.mirror;
             while (need) {
             // FP32 partial sums over F4CHUNK records, then one FP64 accumulation
             float a32[F4K];
 #pragma unroll
             for (int k = 0; k < F4K; k++) a32[k] = 0.f;
-            for (int c = 0; c < F4CHUNK && need; c++) {
+            float bsum = 0.f;
+            int c = 0;
+            for (unsigned long long pr = need & pairable; c + 1 < F4CHUNK && __popcll(pr) >= 2; c += 2) {
+                const int q1 = __builtin_ctzll(pr);
+                pr &= pr - 1;
+                const int q2 = __builtin_ctzll(pr);
+                pr &= pr - 1;
+                need &= ~((1ull << q1) | (1ull << q2));
+                const u32x4 h1 = srec[q1 * W], h2 = srec[q2 * W];
+                const float d1 = __uint_as_float(h1.x), s1 = __uint_as_float(h1.y), a1 = __uint_as_float(h1.z);
+                const float d2 = __uint_as_float(h2.x), s2 = __uint_as_float(h2.y), a2 = __uint_as_float(h2.z);
+                bsum += __uint_as_float(h1.w) + __uint_as_float(h2.w);
+                const float fl1 = (float)(j0 + lane - (int)srec[q1 * W + 1].x);
+                const float fl2 = (float)(j0 + lane - (int)srec[q2 * W + 1].x);
+#pragma unroll
+                for (int k = 0; k < F4K; k++) {
+                    const float x1 = __fmaf_rn(fl1 + (float)(64 * k), dvr4f, d1);
+                    const float x2 = __fmaf_rn(fl2 + (float)(64 * k), dvr4f, d2);
+                    const float p1 = __fmaf_rn(x1, x1, a1), p2 = __fmaf_rn(x2, x2, a2);
+                    const float num = __fmaf_rn(s1, p2, s2 * p1);
+                    a32[k] = __fmaf_rn(num, __builtin_amdgcn_rcpf(p1 * p2), a32[k]);
+                }
+            }
+            for (; c < F4CHUNK && need; c++) {
                 const int q = __builtin_ctzll(need);
                 need &= need - 1;
                 const unsigned long long bit = 1ull << q;
@@ -1973,7 +2002,7 @@ __global__ void __launch_bounds__(64) k_f4conv32(const Job *jobs, const R4Blk *b
                 }
             }
 #pragma unroll
-            for (int k = 0; k < F4K; k++) acc[k] += (double)a32[k];
+            for (int k = 0; k < F4K; k++) acc[k] += (double)(a32[k] - bsum);
             }
         }
     }
@@ -3041,7 +3070,7 @@ struct clblm_ctx {
     // latency-bound SHRINQ walks then run beside the pass-1 convolution.
     hipStream_t f4stream = nullptr;
     int f4_chunk = 16;       // consecutive F4 tiles per XCD run (xcd_tile)
-    int f4_far = 1;          // far-field evaluation in the FP32 gather: 1 on, 2 off
+    int f4_far = 1;          // far-field evaluation in the FP32 gather: 1 on, 2 off, 3 on from a third of a tile
     int f4_stream_mode = 1;  // 1: the F4 chain on its own stream; 2: on the batch's stream
     // far-field nodes per tile width 64 F4K (F4K = F4KMIN..F4KMAX): Lagrange weights
     // [64 F4K][F4NF] and node positions (tile points from the tile's first point)
@@ -3776,14 +3805,17 @@ static int run_batch(clblm_ctx *C, const LineListH &L, Batch &B, double *d_out, 
     const unsigned f4grid = (unsigned)(8 * (int64_t)xchunk * ((ntile + 8 * xchunk - 1) / (8 * xchunk)));
     if (ntile > 0) {
         const int fk = B.f4k;
-        const float *tau = C->f4_far == 1 ? C->far_tau[fk] : nullptr;
+        const float *tau = C->f4_far != 2 ? C->far_tau[fk] : nullptr;
+        // far records lie at least half a tile (setting 1) or a third of a tile (3)
+        // outside it
+        const int dfar = C->f4_far == 3 ? (64 * fk) / 3 : 32 * fk;
 #define F4_LAUNCH(K)                                                                                              \
     if (precision == CLBLM_FP64)                                                                                  \
         hipLaunchKernelGGL(k_f4conv64<K>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,              \
                            at<F4Rec64>(base, o_rec), d_R4, C->d_work, ntile, xchunk, d_xjj);                      \
     else                                                                                                          \
         hipLaunchKernelGGL(k_f4conv32<K>, dim3(f4grid), dim3(64), 0, s4, d_jobs, d_blk, d_tiles, sh,              \
-                           at<F4Rec32>(base, o_rec), d_R4, C->d_work, ntile, xchunk, C->far_w[fk], tau);
+                           at<F4Rec32>(base, o_rec), d_R4, C->d_work, ntile, xchunk, C->far_w[fk], tau, dfar);
         switch (fk) {
         case 2: F4_LAUNCH(2) break;
         case 4: F4_LAUNCH(4) break;
@@ -4302,7 +4334,7 @@ int clblm_autotune(clblm_ctx *c, int handle, int nlay, const clblm_layer *layers
         case CLBLM_TUNE_CONV_SPLIT: return {1, 2, 4};
         case CLBLM_TUNE_F4_XCD_CHUNK: return {4, 16, 64};
         case CLBLM_TUNE_F4_STREAM: return {1, 2};
-        case CLBLM_TUNE_F4_FAR_FIELD: return {1, 2};
+        case CLBLM_TUNE_F4_FAR_FIELD: return {1, 2, 3};
         default: return {};
         }
     };
@@ -4378,7 +4410,7 @@ int clblm_tuning(clblm_ctx *c, int key, int value, int *old_value) {
     int lo = 0, hi = 0;
     switch (key) {
     case CLBLM_TUNE_F4_XCD_CHUNK: slot = &c->f4_chunk; lo = 1; hi = 4096; break;
-    case CLBLM_TUNE_F4_FAR_FIELD: slot = &c->f4_far; lo = 1; hi = 2; break;
+    case CLBLM_TUNE_F4_FAR_FIELD: slot = &c->f4_far; lo = 1; hi = 3; break;
     case CLBLM_TUNE_F4_STREAM: slot = &c->f4_stream_mode; lo = 1; hi = 2; break;
     case CLBLM_TUNE_F4_POINTS_FP32C: slot = &c->f4k32; lo = F4KMIN; hi = F4KMAX; break;
     case CLBLM_TUNE_F4_POINTS_FP64: slot = &c->f4k64; lo = F4KMIN; hi = F4KMAX; break;

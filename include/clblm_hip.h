@@ -216,9 +216,11 @@ int clblm_od_graph_destroy(clblm_ctx *ctx, clblm_od_graph *graph);
  * loops are sequential).  key selects the parameter; value 0 only queries it;
  * *old_value (if non-NULL) receives the setting before the call.
  *   CLBLM_TUNE_F4_XCD_CHUNK  consecutive F4 tiles given to one XCD (1-4096).
- *   CLBLM_TUNE_F4_FAR_FIELD  1: CLBLM_FP32C evaluates far-wing records at 16
- *                            Chebyshev nodes per F4 tile (default); 2: at every
- *                            point.  CLBLM_FP64 always sums every point.
+ *   CLBLM_TUNE_F4_FAR_FIELD  1: CLBLM_FP32C evaluates far-wing records (centre at
+ *                            least half a tile outside the tile) at 16 Chebyshev
+ *                            nodes per F4 tile (default); 2: at every point; 3: as
+ *                            1 from a third of a tile.  CLBLM_FP64 always sums
+ *                            every point.
  *   CLBLM_TUNE_F4_STREAM     1: the F4 wing chain of a batch runs on a stream of
  *                            the context beside pass 1 (default); 2: every kernel
  *                            of the batch on the batch's stream, one after another.

@@ -90,18 +90,21 @@ def test_low_wavenumber_fp32c(ctx):
 def test_far_field_nodes_match_every_point(ctx, k):
     """FP32C evaluates far-wing F4 records at 16 Chebyshev nodes per tile and
     interpolates; the same layer with every record summed at every point agrees to
-    1e-6 relative (the interpolation itself is good to ~1e-8, the rest is FP32)."""
+    1e-6 relative (the interpolation itself is good to ~1e-8 for records half a tile
+    away, ~5e-7 of the record's peak for a third of a tile; the rest is FP32)."""
     from clblm_amd._lib import FP32C, TUNE_F4_FAR_FIELD
     lines = W.profile_lines(40000, 500.0, 800.0, seed=1001)
     lay = W.atmosphere()[k]
     dv = W.calculate_dv(650.0, lay.pave, lay.tave, lay.w[1] / lay.wtot)
     h = ctx.upload_lines(lines)
     out = {}
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         ctx.tuning(TUNE_F4_FAR_FIELD, mode)
         ctx.jrad = 1
         out[mode], _ = ctx.odlay(h, lay, 600.0, 700.0, dv, precision=FP32C)
     ctx.tuning(TUNE_F4_FAR_FIELD, 1)
-    e = float((np.abs(out[1] - out[2]) / np.abs(out[2])).max())
-    print("layer %d far-field nodes vs every point: %d points, max rel diff %.3e" % (k, out[1].size, e))
-    assert e <= 1e-6
+    for mode in (1, 3):
+        e = float((np.abs(out[mode] - out[2]) / np.abs(out[2])).max())
+        print("layer %d far-field nodes (setting %d) vs every point: %d points, max rel diff %.3e" %
+              (k, mode, out[1].size, e))
+        assert e <= 1e-6

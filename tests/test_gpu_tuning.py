@@ -38,7 +38,8 @@ def test_every_launch_shape_gives_the_same_od(ctx):
                             (_lib.TUNE_CONV_UNROLL, (2, 4, 8), _lib.FP64),
                             (_lib.TUNE_F4_XCD_CHUNK, (1, 5, 64), _lib.FP64),
                             (_lib.TUNE_F4_STREAM, (1, 2), _lib.FP64),
-                            (_lib.TUNE_F4_POINTS_FP32C, (2, 3, 4), _lib.FP32C)):
+                            (_lib.TUNE_F4_POINTS_FP32C, (2, 3, 4), _lib.FP32C),
+                            (_lib.TUNE_F4_FAR_FIELD, (1, 2, 3), _lib.FP32C)):
         old = ctx.tuning(key)
         for v in vals:
             ctx.tuning(key, v)
